@@ -1,0 +1,253 @@
+/*
+ * tomos_hip.h -- C ABI of libtomos_hip.so, the MI355X (gfx950) kernels
+ * behind the TOMOS LSQR hot path.
+ *
+ * Conventions (all entry points):
+ *   - plain pointers and sizes only; every device buffer is owned by the
+ *     caller; matrices are column-major with an explicit leading dimension;
+ *   - the packed formats are PLANAR: a wh plane and a wr plane of uint16,
+ *     each lda*n long (the reference stores wr right after wh in one
+ *     allocation, src/core/matrix/fp32_16_pack.hpp:861-874; a caller with
+ *     that layout passes wr = wh + lda*n);
+ *   - functions return 0 on success or a positive TOMOS_ERR_* code, and the
+ *     message is then available from tomos_last_error(); nothing throws;
+ *   - work is enqueued on the given stream (NULL = the default stream) and
+ *     is asynchronous unless a comment says otherwise.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the reference checkout).
+ */
+#ifndef TOMOS_HIP_H
+#define TOMOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* tomos_stream_t; /* == hipStream_t */
+typedef struct ihipEvent_t* tomos_event_t;   /* == hipEvent_t */
+
+/* Status codes. */
+enum {
+    TOMOS_OK = 0,
+    TOMOS_ERR_INVALID_ARGUMENT = 1,
+    TOMOS_ERR_HIP = 2,
+    TOMOS_ERR_UNSUPPORTED = 3,
+    TOMOS_ERR_WORKSPACE = 4,
+    TOMOS_ERR_NOT_CONVERGED = 5,
+    TOMOS_ERR_SINGULAR = 6,
+    TOMOS_ERR_INTERNAL = 7
+};
+
+/* Element types; values equal the reference's NumericType
+ * (include/core/precision.hpp:43-50) with FP32_BF16 appended. */
+enum {
+    TOMOS_F64 = 0,
+    TOMOS_F32 = 1,
+    TOMOS_F16 = 2,
+    TOMOS_FP32_16 = 3,
+    TOMOS_BF16 = 4,
+    TOMOS_FP32_BF16 = 5
+};
+
+enum { TOMOS_NO_TRANSPOSE = 0, TOMOS_TRANSPOSE = 1 };
+
+/* Encoders (FP32 -> planar wh/wr). */
+enum {
+    TOMOS_PACK_FP32_16 = 0,           /* fp32_16_split          fp32_16_pack.hpp:129-190 */
+    TOMOS_PACK_FP32_16_CLAMPED = 1,   /* fp32_16_split_clamped  fp32_16_pack.hpp:200-248 */
+    TOMOS_PACK_FP32_16_TRUNCATED = 2, /* fp32_16_split_truncated fp32_16_pack.hpp:263-299 */
+    TOMOS_PACK_FP32_BF16 = 3,         /* fp32_bf16_split         fp32_bf16_pack.hpp:68-73 */
+    TOMOS_PACK_FP32_BF16_ROUNDED = 4  /* fp32_bf16_split_rounded fp32_bf16_pack.hpp:117-145 */
+};
+
+/* Decoders (planar wh[/wr] -> FP32 or 16-bit bits). */
+enum {
+    TOMOS_DECODE_FP32_16_EXACT = 0,      /* fp32_16_combine            fp32_16_pack.hpp:301-359 */
+    TOMOS_DECODE_FP32_16_HALF = 1,       /* fp32_16_half_only          fp32_16_pack.hpp:553-571 */
+    TOMOS_DECODE_FP32_16_HALF_FAST = 2,  /* fp32_16_half_only_fast     fp32_16_pack.hpp:586-595 */
+    TOMOS_DECODE_FP32_16_BF16 = 3,       /* fp32_16_bf16_only          fp32_16_pack.hpp:812-849 */
+    TOMOS_DECODE_FP32_BF16_EXACT = 4,    /* fp32_bf16_combine          fp32_bf16_pack.hpp:78-81 */
+    TOMOS_DECODE_FP32_BF16_HI = 5,       /* fp32_bf16_bf16_only        fp32_bf16_pack.hpp:91-93 */
+    TOMOS_DECODE_FP32_BF16_EXACT_ROUNDED = 6, /* fp32_bf16_combine_rounded fp32_bf16_pack.hpp:180-186 */
+    TOMOS_DECODE_FP32_16_EXACT_TRUNCATED = 7  /* fp32_16_combine_truncated fp32_16_pack.hpp:498-522 */
+};
+
+/* GEMV/GEMM compute tiers (reference dispatch::Fp32_16 slots,
+ * src/dispatch/fp32_16_registries.hpp:40-54):
+ *   DEFAULT: the per-iteration tier -- FP32_16: half_only decode of wh
+ *            ("pure_half"), FP32_BF16: wh read as bf16 ("hi-only"); both
+ *            accumulate in FP32.  F32: FP32 accumulate.  F64: FP64.
+ *   FAST:    FP32_16 half_only decode / FP32_BF16 hi-only, FP64 accumulate
+ *            ("fast", fp32_16_kernels.cpp:90-113).  F32: FP64 accumulate.
+ *   EXACT:   wh+wr exact reconstruction, FP64 accumulate ("accurate",
+ *            fp32_16_kernels.cpp:28-55).  F32/F64: FP64 accumulate. */
+enum { TOMOS_TIER_DEFAULT = 0, TOMOS_TIER_FAST = 1, TOMOS_TIER_EXACT = 2 };
+
+/* ---- runtime helpers -------------------------------------------------- */
+const char* tomos_last_error(void);
+int tomos_version(void);
+int tomos_set_device(int device);
+int tomos_get_device_info(int* cu_count, int* arch_major, int* arch_minor,
+                          size_t* total_mem);
+int tomos_malloc(void** ptr, size_t bytes);
+int tomos_free(void* ptr);
+int tomos_memcpy_htod(void* dst, const void* src, size_t bytes, tomos_stream_t stream);
+int tomos_memcpy_dtoh(void* dst, const void* src, size_t bytes, tomos_stream_t stream);
+int tomos_memcpy_dtod(void* dst, const void* src, size_t bytes, tomos_stream_t stream);
+int tomos_memset(void* ptr, int value, size_t bytes, tomos_stream_t stream);
+int tomos_stream_synchronize(tomos_stream_t stream);
+int tomos_device_synchronize(void);
+int tomos_stream_create(tomos_stream_t* stream);
+int tomos_stream_destroy(tomos_stream_t stream);
+int tomos_event_create(tomos_event_t* ev);
+int tomos_event_destroy(tomos_event_t ev);
+int tomos_event_record(tomos_event_t ev, tomos_stream_t stream);
+int tomos_event_elapsed_ms(float* ms, tomos_event_t start, tomos_event_t stop);
+
+/* Deterministic synthetic data on the device (no reference counterpart;
+ * used by benchmarks to build the 1e6..2e6-row problems in HBM).
+ * out[i + j*ld] for i<m, j<n: N(0,1) from a counter-based hash of
+ * (seed, i + j*m) and Box-Muller, times col_scale[j] (nullable) and times
+ * 2^k with k uniform in [exp_lo, exp_hi] drawn per element (exp_lo ==
+ * exp_hi == 0 disables it). */
+int tomos_fill_normal_f32(float* out, int64_t m, int64_t n, int64_t ld, uint64_t seed,
+                          const float* col_scale, int exp_lo, int exp_hi,
+                          tomos_stream_t stream);
+
+/* ---- pack / decode ---------------------------------------------------- */
+/* Replaces cuda::split_dense_fp32_16 (fp32_16_cuda_kernels.cu:271-299) and
+ * cpu::split_dense_fp32_16 (fp32_16_kernels.cpp:8-16), for every encoder
+ * variant above.  src is m x n (ld_src); wh/wr are m x n (ld_dst). */
+int tomos_pack(int variant, int64_t m, int64_t n, const float* src, int64_t ld_src,
+               uint16_t* wh, uint16_t* wr, int64_t ld_dst, tomos_stream_t stream);
+
+/* Replaces cuda::combine_dense_fp32_16 (fp32_16_cuda_kernels.cu:301-323)
+ * for every FP32-producing decoder above.  wr may be NULL for the wh-only
+ * decoders. */
+int tomos_unpack(int decoder, int64_t m, int64_t n, const uint16_t* wh, const uint16_t* wr,
+                 int64_t ld_src, float* dst, int64_t ld_dst, tomos_stream_t stream);
+
+/* Element-wise wh -> 16-bit decode, returning raw half (HALF, HALF_FAST) or
+ * bfloat16 (BF16, FP32_BF16_HI) bit patterns: the device form of
+ * fp32_16_half_only / _fast (fp32_16_pack.hpp:553-595) and
+ * fp32_16_bf16_only (:812-849), for bit-exact checks. */
+int tomos_decode_wh(int decoder, int64_t count, const uint16_t* wh, uint16_t* out,
+                    tomos_stream_t stream);
+
+/* ---- GEMV ------------------------------------------------------------- */
+/* y = alpha*op(A)*x + beta*y (y is not read when beta == 0).
+ * Replaces the mv_dense_fn slots of dispatch::Dense / dispatch::Fp32_16
+ * (src/dispatch/dense_registries.hpp:29-31, fp32_16_registries.hpp:27-29):
+ * mv_pure_half_impl (fp32_16_registries.cpp:65-263), mv_fast_impl
+ * (:265-285), mv_accurate_impl (:287-368), the FP32_BF16 kernels
+ * (fp32_16_cuda_kernels.cu:28785, :29636, :7587) and mv_fp32_tiled
+ * (mv_cuda_kernels.cu:863-990).
+ *   a_type: TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16;
+ *   a: the F32 or F64 matrix, or the wh plane for packed types; wr: the wr
+ *      plane (used by the EXACT tier only; may be NULL otherwise);
+ *   x_type, y_type: TOMOS_F32 | TOMOS_F64.
+ * scratch: device buffer of at least tomos_gemv_scratch_size() bytes, or
+ * NULL to use a library-owned buffer (grown with hipMalloc on demand). */
+size_t tomos_gemv_scratch_size(int a_type, int tier, int trans, int64_t m, int64_t n);
+int tomos_gemv(int a_type, int tier, int trans, double alpha, int64_t m, int64_t n,
+               const void* a, const uint16_t* wr, int64_t lda, int x_type, const void* x,
+               double beta, int y_type, void* y, void* scratch, size_t scratch_bytes,
+               tomos_stream_t stream);
+
+/* ---- GEMM ------------------------------------------------------------- */
+/* C = alpha*op(A)*op(B) + beta*C with packed or plain A, FP32 B and C.
+ * Replaces the mm_dense_fn slots (dense_registries.hpp:32-34) and
+ * dispatch::Fp32_16::mm_{pure_half,fast,accurate,workspace}
+ * (fp32_16_registries.cpp:370-453): tier DEFAULT/FAST/EXACT as for GEMV.
+ * M x N result, K inner dimension. */
+int tomos_gemm(int a_type, int tier, int trans_a, int trans_b, double alpha, int64_t M,
+               int64_t N, int64_t K, const void* a, const uint16_t* wr, int64_t lda,
+               const float* b, int64_t ldb, double beta, float* c, int64_t ldc,
+               tomos_stream_t stream);
+
+/* ---- CountSketch ------------------------------------------------------ */
+/* bucket[i] = splitmix64(i ^ 0x12345678abcdef00) mod s,
+ * sign[i]   = bit0(splitmix64(i ^ 0xf00dd00d42babe55)) ? +1 : -1,
+ * bit-exact with cpu::countsketch_apply_impl's hashing
+ * (src/kernels/cpu/sketch/countsketch_kernels.cpp:59-85). */
+int tomos_countsketch_indices(int64_t m, int64_t s, int32_t* bucket, int8_t* sign,
+                              tomos_stream_t stream);
+
+/* Y (s x n, FP64, ldy) = S * A, S the hashed s x m CountSketch above,
+ * A m x n in any a_type/tier (packed A is decoded per tier).  Y is
+ * overwritten.  Replaces countsketch_apply_matrix_hashed
+ * (countsketch_cuda_kernels.cu:253-285, host :435-450) and the CPU apply.
+ * scratch: tomos_countsketch_scratch_size() bytes or NULL. */
+size_t tomos_countsketch_scratch_size(int64_t m, int64_t n, int64_t s);
+int tomos_countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a,
+                            const uint16_t* wr, int64_t lda, int64_t s, double* y,
+                            int64_t ldy, void* scratch, size_t scratch_bytes,
+                            tomos_stream_t stream);
+
+/* ---- Householder QR / triangular inverse (FP64) ---------------------- */
+/* R (n x n upper, ldr) of the Householder QR of the k x n FP64 matrix A
+ * (lda; A is overwritten with intermediate data).  k >= n required.
+ * Replaces the geqrf step of generate_sketch_qr_precond
+ * (src/core/precond/sketch_qr.cpp:38-57, CUDA sketch_qr_cuda_kernels.cu:65-151).
+ * Synchronous. */
+size_t tomos_geqrf_scratch_size(int64_t k, int64_t n);
+int tomos_geqrf_r_f64(int64_t k, int64_t n, double* a, int64_t lda, double* r, int64_t ldr,
+                      void* scratch, size_t scratch_bytes, tomos_stream_t stream);
+
+/* In-place inverse of the n x n upper-triangular R (ldr); strictly lower
+ * part set to 0.  Returns TOMOS_ERR_SINGULAR for a zero diagonal.
+ * Replaces dtrtri('U','N') (sketch_qr.cpp:59-63).  Synchronous. */
+int tomos_trtri_upper_f64(int64_t n, double* r, int64_t ldr, tomos_stream_t stream);
+
+/* generate_sketch_qr_precond (src/core/precond/sketch_qr.cpp:68-122): the
+ * k x n FP64 sketch (lda) is factored and its leading n x n block is
+ * overwritten with R^-1 (upper), the rest of the block zeroed.
+ * Synchronous. */
+int tomos_sketch_qr_precond_f64(int64_t k, int64_t n, double* a, int64_t lda,
+                                tomos_stream_t stream);
+
+/* ---- LSQR ------------------------------------------------------------- */
+/* Restarted, right-preconditioned Golub-Kahan LSQR on one GPU, with the
+ * scalar recurrences (alpha, beta, rho, phi, c, s, theta) and all norms in
+ * FP64 in device memory.  Semantics follow fused_lsqr_impl
+ * (src/core/solver/lsqr.cpp:685-747): restart loop, residual recompute
+ * u = b - A*x at each restart on `restart_tier`, inner iterations on
+ * `mv_tier`, stop when |phi_bar|/|b| < tol or < max_relres, or when the
+ * true restart residual is below them.  Replaces lsqr_dense_impl
+ * (src/dispatch/lsqr_registries.cpp:88-159). */
+typedef struct tomos_lsqr_options {
+    int a_type;        /* TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16 */
+    int mv_tier;       /* per-iteration GEMV tier (TOMOS_TIER_*) */
+    int restart_tier;  /* restart residual tier (EXACT = "implicit duplicate") */
+    int value_type;    /* Krylov vectors u, v, w: TOMOS_F32 | TOMOS_F64 */
+    int refine_type;   /* b and x: TOMOS_F32 | TOMOS_F64 */
+    int precond_type;  /* R^-1 element type (TOMOS_F32 | TOMOS_F64), or -1: none */
+    double tol;
+    double max_relres;
+    int max_iters;     /* iterations per restart */
+    int restarts;
+    int check_every;   /* host polls the device stop flag every N iterations (>=1) */
+    int fused;         /* 1: one pass over A per iteration (A*p and A^T*u fused) */
+} tomos_lsqr_options;
+
+size_t tomos_lsqr_workspace_size(const tomos_lsqr_options* opt, int64_t m, int64_t n);
+
+/* x (refine_type) is overwritten.  rinv: n x n upper (ldr) or NULL.
+ * history (host, nullable) receives up to history_cap relative residuals,
+ * one per stopping check, as lsqr_solve's residual_history.
+ * Synchronous: returns after the solve completed. */
+int tomos_lsqr_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const void* a,
+                     const uint16_t* wr, int64_t lda, const void* b, void* x,
+                     const void* rinv, int64_t ldr, void* workspace, size_t workspace_bytes,
+                     int* iterations, double* history, int history_cap, int* history_len,
+                     tomos_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOS_HIP_H */

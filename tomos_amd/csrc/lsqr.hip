@@ -1,0 +1,480 @@
+// Restarted, right-preconditioned LSQR (Golub-Kahan bidiagonalization) on
+// one GPU.
+//
+// Control flow follows fused_lsqr_impl (src/core/solver/lsqr.cpp:685-747):
+//   rhsnorm = |b|; x = 0
+//   for each restart:
+//     stop check on |phi_bar|/rhsnorm (after the first iteration)
+//     initialize (:431-513): u = b - A x (restart tier), beta = |u|, u /= beta,
+//       v = P^T A^T u (or A^T u), alpha = |v|, v /= alpha, w = v,
+//       rho_bar = alpha, phi_bar = beta; stop if beta/rhsnorm is small
+//     iterations (:519-679): u = A P v - alpha u, beta = |u|, u /= beta,
+//       v = P^T A^T u - beta v, alpha = |v|, v /= alpha; Givens update;
+//       x += (phi/rho) P w; w = v - (theta/rho) w; stop check
+// with P = R^-1 (precond_apply = a dense GEMV on R^-1, blas.cpp:88-96).
+//
+// MI355X design: every scalar of the recurrence (alpha, beta, rho_bar,
+// phi_bar, rho, c, s, theta, phi) and every norm lives in FP64 in a device
+// state block and is produced by device kernels (deterministic fixed-order
+// reductions), so an iteration is a fixed sequence of kernel launches with
+// no device->host traffic.  The stop decision is a device flag that every
+// kernel of the sequence checks first (a converged solve turns the
+// remaining launches into no-ops); the host polls the flag every
+// `check_every` iterations.  Relative-residual history is recorded on the
+// device, one entry per stop check, and copied back at the end.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "gemv.hpp"
+#include "tomos_hip.h"
+
+namespace tomos_hip {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kRedBlocks = 1024;
+
+struct LsqrState {
+    double alpha, beta, rho_bar, phi_bar;
+    double rhsnorm;
+    double x_coef;   // phi / rho
+    double w_coef;   // -theta / rho
+    double pad;
+    int done;
+    int completed;
+    int hist_len;
+    int pad2;
+};
+
+__device__ __forceinline__ double ld(const void* p, bool f64, int64_t i) {
+    return f64 ? static_cast<const double*>(p)[i] : static_cast<double>(static_cast<const float*>(p)[i]);
+}
+__device__ __forceinline__ void st(void* p, bool f64, int64_t i, double v) {
+    if (f64) static_cast<double*>(p)[i] = v;
+    else static_cast<float*>(p)[i] = static_cast<float>(v);
+}
+
+__global__ void k_state_init(LsqrState* s) {
+    if (threadIdx.x == 0) {
+        s->alpha = s->beta = s->rho_bar = s->phi_bar = 0.0;
+        s->rhsnorm = 0.0;
+        s->x_coef = s->w_coef = 0.0;
+        s->done = 0;
+        s->completed = 0;
+        s->hist_len = 0;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_zero(void* x, bool f64, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+    if (i < n) st(x, f64, i, 0.0);
+}
+
+__global__ __launch_bounds__(kT) void k_copy(const LsqrState* __restrict__ s, const void* src,
+                                             bool s64, void* dst, bool d64, int64_t n) {
+    if (s->done) return;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kT)
+        st(dst, d64, i, ld(src, s64, i));
+}
+
+// sum of squares, per-block partials
+__global__ __launch_bounds__(kT) void k_sumsq(const LsqrState* __restrict__ s, const void* x,
+                                              bool f64, int64_t n, double* __restrict__ red) {
+    if (s && s->done) return;
+    double acc = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kT) {
+        const double v = ld(x, f64, i);
+        acc += v * v;
+    }
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) red[blockIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+// out = sqrt(sum of red[0..nb)) in a fixed order
+__global__ __launch_bounds__(kT) void k_norm_finish(const LsqrState* s, const double* __restrict__ red,
+                                                    int nb, double* __restrict__ out) {
+    if (s && s->done) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kT) acc += red[i];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = sqrt((w[0] + w[1]) + (w[2] + w[3]));
+}
+
+// x *= 1 / *den
+__global__ __launch_bounds__(kT) void k_scale_inv(const LsqrState* __restrict__ s, void* x,
+                                                  bool f64, int64_t n, const double* den) {
+    if (s->done) return;
+    const double inv = 1.0 / *den;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kT)
+        st(x, f64, i, ld(x, f64, i) * inv);
+}
+
+// y += mul * (*coef) * x
+__global__ __launch_bounds__(kT) void k_axpy(const LsqrState* __restrict__ s, const void* x,
+                                             bool x64, void* y, bool y64, int64_t n,
+                                             const double* coef, double mul) {
+    if (s->done) return;
+    const double a = mul * (*coef);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kT)
+        st(y, y64, i, ld(y, y64, i) + a * ld(x, x64, i));
+}
+
+// w = (*coef) * w + v   (lsqr_scal then lsqr_axpy, lsqr.cpp:670-671)
+__global__ __launch_bounds__(kT) void k_w_update(const LsqrState* __restrict__ s, void* w,
+                                                 const void* v, bool f64, int64_t n) {
+    if (s->done) return;
+    const double c = s->w_coef;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kT) {
+        const double scaled = f64 ? c * ld(w, true, i)
+                                  : static_cast<double>(static_cast<float>(c * ld(w, false, i)));
+        st(w, f64, i, scaled + ld(v, f64, i));
+    }
+}
+
+__device__ __forceinline__ void push_history(LsqrState* s, double* hist, int cap, double rr) {
+    if (s->hist_len < cap) hist[s->hist_len] = rr;
+    s->hist_len += 1;
+}
+
+// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708)
+__global__ void k_restart_check(LsqrState* s, double* hist, int cap, double tol, double maxrr) {
+    if (threadIdx.x != 0 || s->done) return;
+    if (s->completed == 0) return;
+    if (s->rhsnorm == 0.0) { s->done = 1; return; }
+    const double rr = fabs(s->phi_bar) / s->rhsnorm;
+    push_history(s, hist, cap, rr);
+    if (rr < tol || rr < maxrr) s->done = 1;
+}
+
+// end of initialize + the true-residual test (lsqr.cpp:509-512, :723-729)
+__global__ void k_init_finish(LsqrState* s, double tol, double maxrr) {
+    if (threadIdx.x != 0 || s->done) return;
+    s->rho_bar = s->alpha;
+    s->phi_bar = s->beta;
+    if (s->rhsnorm == 0.0) { s->done = 1; return; }
+    const double rr = s->beta / s->rhsnorm;
+    if (rr < tol || rr < maxrr) s->done = 1;
+}
+
+// Givens rotation of step_2_update_sol (lsqr.cpp:642-647)
+__global__ void k_givens(LsqrState* s) {
+    if (threadIdx.x != 0 || s->done) return;
+    const double rho = hypot(s->rho_bar, s->beta);
+    const double c = s->rho_bar / rho, sn = s->beta / rho;
+    const double theta = sn * s->alpha;
+    s->rho_bar = -c * s->alpha;
+    const double phi = c * s->phi_bar;
+    s->phi_bar = sn * s->phi_bar;
+    s->x_coef = phi / rho;
+    s->w_coef = -theta / rho;
+}
+
+// per-iteration stop check (lsqr.cpp:735-744)
+__global__ void k_iter_check(LsqrState* s, double* hist, int cap, double tol, double maxrr) {
+    if (threadIdx.x != 0 || s->done) return;
+    s->completed += 1;
+    if (s->rhsnorm == 0.0) { s->done = 1; return; }
+    const double rr = fabs(s->phi_bar) / s->rhsnorm;
+    push_history(s, hist, cap, rr);
+    if (rr < tol || rr < maxrr) s->done = 1;
+}
+
+struct Layout {
+    size_t u, v, w, t, pv, state, hist, red, scratch, total;
+    size_t scratch_bytes;
+    int hist_cap;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+size_t tsize(int t) { return t == TOMOS_F64 ? 8 : 4; }
+
+Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+    Layout L;
+    const size_t vs = tsize(o.value_type);
+    const size_t ps = o.precond_type >= 0 ? tsize(o.precond_type) : vs;
+    size_t off = 0;
+    L.u = off; off = align256(off + static_cast<size_t>(m) * vs);
+    L.v = off; off = align256(off + static_cast<size_t>(n) * vs);
+    L.w = off; off = align256(off + static_cast<size_t>(n) * vs);
+    L.t = off; off = align256(off + static_cast<size_t>(n) * vs);
+    L.pv = off; off = align256(off + static_cast<size_t>(n) * ps);
+    L.state = off; off = align256(off + sizeof(LsqrState));
+    const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
+    L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 26));
+    L.hist = off; off = align256(off + static_cast<size_t>(L.hist_cap) * 8);
+    L.red = off; off = align256(off + kRedBlocks * 8);
+    size_t sc = 0;
+    sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_NO_TRANSPOSE, m, n));
+    sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_TRANSPOSE, m, n));
+    sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.restart_tier, TOMOS_NO_TRANSPOSE, m, n));
+    if (o.precond_type >= 0) {
+        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, TOMOS_TIER_DEFAULT, TOMOS_NO_TRANSPOSE, n, n));
+        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, TOMOS_TIER_DEFAULT, TOMOS_TRANSPOSE, n, n));
+    }
+    L.scratch = off;
+    L.scratch_bytes = sc;
+    off = align256(off + sc);
+    L.total = off;
+    return L;
+}
+
+void validate(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+    require(o.a_type == TOMOS_F32 || o.a_type == TOMOS_F64 || o.a_type == TOMOS_FP32_16 ||
+                o.a_type == TOMOS_FP32_BF16,
+            "lsqr: unsupported a_type");
+    require(o.value_type == TOMOS_F32 || o.value_type == TOMOS_F64, "lsqr: value_type must be F32/F64");
+    require(o.refine_type == TOMOS_F32 || o.refine_type == TOMOS_F64, "lsqr: refine_type must be F32/F64");
+    require(o.precond_type == -1 || o.precond_type == TOMOS_F32 || o.precond_type == TOMOS_F64,
+            "lsqr: precond_type must be F32, F64 or -1");
+    require(o.max_iters >= 0 && o.restarts >= 0, "lsqr: negative iteration limits");
+    require(m >= 0 && n >= 0, "lsqr: negative size");
+}
+
+unsigned grid_for(int64_t n) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kT), kRedBlocks)));
+}
+
+struct Solver {
+    tomos_lsqr_options o;
+    int64_t m, n;
+    const void* a;
+    const uint16_t* wr;
+    int64_t lda;
+    const void* b;
+    void* x;
+    const void* rinv;
+    int64_t ldr;
+    char* ws;
+    Layout L;
+    hipStream_t s;
+    bool v64, r64, p64;
+
+    LsqrState* state() const { return reinterpret_cast<LsqrState*>(ws + L.state); }
+    void* U() const { return ws + L.u; }
+    void* V() const { return ws + L.v; }
+    void* W() const { return ws + L.w; }
+    void* Tm() const { return ws + L.t; }
+    void* PV() const { return ws + L.pv; }
+    double* hist() const { return reinterpret_cast<double*>(ws + L.hist); }
+    double* red() const { return reinterpret_cast<double*>(ws + L.red); }
+    void* scratch() const { return ws + L.scratch; }
+
+    void norm(const void* vec, bool f64, int64_t len, double* out, bool guarded_by_state = true) {
+        const unsigned g = grid_for(len);
+        const LsqrState* sp = guarded_by_state ? state() : nullptr;
+        k_sumsq<<<g, kT, 0, s>>>(sp, vec, f64, len, red());
+        k_norm_finish<<<1, kT, 0, s>>>(sp, red(), static_cast<int>(g), out);
+    }
+    Coef coef(double alpha, double beta) const {
+        Coef c;
+        c.alpha = alpha;
+        c.beta = beta;
+        c.skip = &state()->done;
+        return c;
+    }
+    // A-matrix GEMV at a tier
+    void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64) {
+        gemv(o.a_type, tier, trans, c, m, n, a, wr, lda, x64 ? TOMOS_F64 : TOMOS_F32, xin,
+             y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.scratch_bytes, s);
+    }
+    // P = R^-1 GEMV
+    void gemv_p(int trans, const void* xin, bool x64, void* yout, bool y64) {
+        gemv(o.precond_type, TOMOS_TIER_DEFAULT, trans, coef(1.0, 0.0), n, n, rinv, nullptr, ldr,
+             x64 ? TOMOS_F64 : TOMOS_F32, xin, y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(),
+             L.scratch_bytes, s);
+    }
+    void scale_inv(void* vec, bool f64, int64_t len, const double* den) {
+        k_scale_inv<<<grid_for(len), kT, 0, s>>>(state(), vec, f64, len, den);
+    }
+
+    void initialize() {
+        LsqrState* S = state();
+        const bool pre = o.precond_type >= 0;
+        k_copy<<<grid_for(m), kT, 0, s>>>(S, b, r64, U(), v64, m);
+        gemv_a(o.restart_tier, TOMOS_NO_TRANSPOSE, coef(-1.0, 1.0), x, r64, U(), v64);
+        norm(U(), v64, m, &S->beta);
+        scale_inv(U(), v64, m, &S->beta);
+        if (pre) {
+            gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, Tm(), v64);
+            gemv_p(TOMOS_TRANSPOSE, Tm(), v64, V(), v64);
+        } else {
+            gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, V(), v64);
+        }
+        norm(V(), v64, n, &S->alpha);
+        scale_inv(V(), v64, n, &S->alpha);
+        k_copy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, W(), v64, n);
+        k_init_finish<<<1, 64, 0, s>>>(S, o.tol, o.max_relres);
+    }
+
+    void iterate() {
+        LsqrState* S = state();
+        const bool pre = o.precond_type >= 0;
+        Coef cn = coef(1.0, 0.0);
+        cn.dbeta = &S->alpha;
+        cn.beta_mul = -1.0;
+        if (pre) {
+            gemv_p(TOMOS_NO_TRANSPOSE, V(), v64, PV(), p64);
+            gemv_a(o.mv_tier, TOMOS_NO_TRANSPOSE, cn, PV(), p64, U(), v64);
+        } else {
+            k_copy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, Tm(), v64, n);
+            gemv_a(o.mv_tier, TOMOS_NO_TRANSPOSE, cn, Tm(), v64, U(), v64);
+        }
+        norm(U(), v64, m, &S->beta);
+        scale_inv(U(), v64, m, &S->beta);
+        gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, Tm(), v64);
+        if (pre) {
+            gemv_p(TOMOS_TRANSPOSE, Tm(), v64, PV(), p64);
+            k_axpy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, PV(), p64, n, &S->beta, -1.0);
+            norm(PV(), p64, n, &S->alpha);
+            scale_inv(PV(), p64, n, &S->alpha);
+            k_copy<<<grid_for(n), kT, 0, s>>>(S, PV(), p64, V(), v64, n);
+        } else {
+            k_axpy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, Tm(), v64, n, &S->beta, -1.0);
+            norm(Tm(), v64, n, &S->alpha);
+            scale_inv(Tm(), v64, n, &S->alpha);
+            k_copy<<<grid_for(n), kT, 0, s>>>(S, Tm(), v64, V(), v64, n);
+        }
+        k_givens<<<1, 64, 0, s>>>(S);
+        if (pre) {
+            gemv_p(TOMOS_NO_TRANSPOSE, W(), v64, PV(), p64);
+            k_axpy<<<grid_for(n), kT, 0, s>>>(S, PV(), p64, x, r64, n, &S->x_coef, 1.0);
+        } else {
+            k_axpy<<<grid_for(n), kT, 0, s>>>(S, W(), v64, x, r64, n, &S->x_coef, 1.0);
+        }
+        k_w_update<<<grid_for(n), kT, 0, s>>>(S, W(), V(), v64, n);
+        k_iter_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres);
+    }
+
+    bool done_flag() {
+        int d = 0;
+        TOMOS_HIP_CHECK(hipMemcpyAsync(&d, &state()->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+        return d != 0;
+    }
+
+    void run(int* iterations, double* history, int history_cap, int* history_len) {
+        LsqrState* S = state();
+        k_state_init<<<1, 64, 0, s>>>(S);
+        k_zero<<<static_cast<unsigned>(ceil_div(std::max<int64_t>(n, 1), kT)), kT, 0, s>>>(x, r64, n);
+        norm(b, r64, m, &S->rhsnorm, false);
+        TOMOS_LAUNCH_CHECK("lsqr setup");
+        const int every = std::max(1, o.check_every);
+        bool stop = false;
+        for (int r = 0; r < o.restarts && !stop; ++r) {
+            k_restart_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres);
+            initialize();
+            TOMOS_LAUNCH_CHECK("lsqr initialize");
+            if (done_flag()) break;
+            for (int it = 0; it < o.max_iters; ++it) {
+                iterate();
+                if ((it + 1) % every == 0 || it + 1 == o.max_iters) {
+                    TOMOS_LAUNCH_CHECK("lsqr iteration");
+                    if (done_flag()) { stop = true; break; }
+                }
+            }
+        }
+        LsqrState hs;
+        TOMOS_HIP_CHECK(hipMemcpyAsync(&hs, S, sizeof(LsqrState), hipMemcpyDeviceToHost, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+        if (iterations) *iterations = hs.completed;
+        if (history_len) *history_len = hs.hist_len;
+        if (history && history_cap > 0) {
+            const int cnt = std::min(std::min(hs.hist_len, L.hist_cap), history_cap);
+            if (cnt > 0) {
+                TOMOS_HIP_CHECK(hipMemcpyAsync(history, hist(), static_cast<size_t>(cnt) * 8,
+                                               hipMemcpyDeviceToHost, s));
+                TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+            }
+        }
+    }
+};
+
+}  // namespace
+
+size_t lsqr_workspace_bytes(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+    validate(o, m, n);
+    return layout_for(o, m, n).total;
+}
+
+void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a, const uint16_t* wr,
+                int64_t lda, const void* b, void* x, const void* rinv, int64_t ldr, void* ws,
+                size_t ws_bytes, int* iterations, double* history, int history_cap,
+                int* history_len, hipStream_t s) {
+    validate(o, m, n);
+    require(lda >= std::max<int64_t>(m, 1), "lsqr: lda < m");
+    if (o.precond_type >= 0) {
+        require(rinv != nullptr, "lsqr: precond_type set but no R^-1 given");
+        require(ldr >= std::max<int64_t>(n, 1), "lsqr: ldr < n");
+    }
+    if (n == 0) {
+        if (iterations) *iterations = 0;
+        if (history_len) *history_len = 0;
+        return;
+    }
+    require(a && b && x, "lsqr: null A, b or x");
+    const bool packed = o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16;
+    if (packed && (o.mv_tier == TOMOS_TIER_EXACT || o.restart_tier == TOMOS_TIER_EXACT))
+        require(wr != nullptr, "lsqr: EXACT tier needs the wr plane");
+    Solver sv;
+    sv.o = o;
+    sv.m = m;
+    sv.n = n;
+    sv.a = a;
+    sv.wr = wr;
+    sv.lda = lda;
+    sv.b = b;
+    sv.x = x;
+    sv.rinv = rinv;
+    sv.ldr = ldr;
+    sv.L = layout_for(o, m, n);
+    if (ws == nullptr) ws = fallback_scratch(sv.L.total);
+    else if (ws_bytes < sv.L.total)
+        throw Error(kErrWorkspace, "lsqr: workspace too small (need " + std::to_string(sv.L.total) +
+                                       " bytes, have " + std::to_string(ws_bytes) + ")");
+    sv.ws = static_cast<char*>(ws);
+    sv.s = s;
+    sv.v64 = o.value_type == TOMOS_F64;
+    sv.r64 = o.refine_type == TOMOS_F64;
+    sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
+    sv.run(iterations, history, history_cap, history_len);
+}
+
+}  // namespace tomos_hip
+
+using namespace tomos_hip;
+
+extern "C" {
+
+size_t tomos_lsqr_workspace_size(const tomos_lsqr_options* o, int64_t m, int64_t n) {
+    size_t out = 0;
+    int rc = guarded([&] {
+        require(o != nullptr, "tomos_lsqr_workspace_size: null options");
+        out = lsqr_workspace_bytes(*o, m, n);
+    });
+    return rc == 0 ? out : 0;
+}
+
+int tomos_lsqr_solve(const tomos_lsqr_options* o, int64_t m, int64_t n, const void* a,
+                     const uint16_t* wr, int64_t lda, const void* b, void* x, const void* rinv,
+                     int64_t ldr, void* ws, size_t ws_bytes, int* iterations, double* history,
+                     int history_cap, int* history_len, tomos_stream_t s) {
+    return guarded([&] {
+        require(o != nullptr, "tomos_lsqr_solve: null options");
+        lsqr_solve(*o, m, n, a, wr, lda, b, x, rinv, ldr, ws, ws_bytes, iterations, history,
+                   history_cap, history_len, s);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+// Runtime helpers behind the C ABI: last-error storage, device queries,
+// memory/stream/event wrappers, the fallback scratch cache, and the
+// deterministic synthetic-data generator used by benchmarks.
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "tomos_hip.h"
+
+namespace tomos_hip {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct DeviceCache {
+    int cu = 0;
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+std::vector<DeviceCache>& caches() {
+    static std::vector<DeviceCache> c;
+    return c;
+}
+DeviceCache& cache_for_current() {
+    int dev = 0;
+    TOMOS_HIP_CHECK(hipGetDevice(&dev));
+    auto& c = caches();
+    if (static_cast<int>(c.size()) <= dev) c.resize(dev + 1);
+    return c[dev];
+}
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+int cu_count() {
+    DeviceCache& c = cache_for_current();
+    if (c.cu == 0) {
+        int dev = 0;
+        TOMOS_HIP_CHECK(hipGetDevice(&dev));
+        TOMOS_HIP_CHECK(hipDeviceGetAttribute(&c.cu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (c.cu <= 0) c.cu = 256;
+    }
+    return c.cu;
+}
+
+void* fallback_scratch(size_t bytes) {
+    DeviceCache& c = cache_for_current();
+    if (bytes == 0) bytes = 256;
+    if (c.scratch_bytes < bytes) {
+        if (c.scratch) {
+            TOMOS_HIP_CHECK(hipDeviceSynchronize());
+            TOMOS_HIP_CHECK(hipFree(c.scratch));
+            c.scratch = nullptr;
+            c.scratch_bytes = 0;
+        }
+        size_t grow = bytes + bytes / 4;
+        TOMOS_HIP_CHECK(hipMalloc(&c.scratch, grow));
+        c.scratch_bytes = grow;
+    }
+    return c.scratch;
+}
+
+// ----------------------------------------------------------------------
+// Synthetic data: counter-based hash -> two uniforms -> Box-Muller.
+// ----------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__global__ void fill_normal_kernel(float* __restrict__ out, int64_t m, int64_t n, int64_t ld,
+                                   uint64_t seed, const float* __restrict__ col_scale,
+                                   int exp_lo, int exp_hi) {
+    const int64_t total = m * n;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += stride) {
+        const int64_t j = t / m;
+        const int64_t i = t - j * m;
+        const uint64_t h = mix64(seed ^ mix64(static_cast<uint64_t>(t)));
+        // 24-bit uniforms in (0,1] and [0,1)
+        const float u1 = (static_cast<float>((h >> 40) & 0xFFFFFFu) + 1.0f) * (1.0f / 16777216.0f);
+        const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * (1.0f / 16777216.0f);
+        float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+        if (exp_hi > exp_lo || exp_lo != 0) {
+            const uint32_t span = static_cast<uint32_t>(exp_hi - exp_lo + 1);
+            const int k = exp_lo + static_cast<int>((h & 0xFFFFu) % span);
+            z = ldexpf(z, k);
+        }
+        if (col_scale) z *= col_scale[j];
+        out[i + j * ld] = z;
+    }
+}
+
+}  // namespace tomos_hip
+
+using namespace tomos_hip;
+
+extern "C" {
+
+const char* tomos_last_error(void) { return tomos_hip::last_error(); }
+
+int tomos_version(void) { return 100; }
+
+int tomos_set_device(int device) {
+    return guarded([&] { TOMOS_HIP_CHECK(hipSetDevice(device)); });
+}
+
+int tomos_get_device_info(int* cu, int* arch_major, int* arch_minor, size_t* total_mem) {
+    return guarded([&] {
+        int dev = 0;
+        TOMOS_HIP_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        TOMOS_HIP_CHECK(hipGetDeviceProperties(&p, dev));
+        if (cu) *cu = p.multiProcessorCount;
+        if (arch_major) *arch_major = p.major;
+        if (arch_minor) *arch_minor = p.minor;
+        if (total_mem) *total_mem = p.totalGlobalMem;
+    });
+}
+
+int tomos_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        require(ptr != nullptr, "tomos_malloc: null ptr");
+        TOMOS_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+int tomos_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) TOMOS_HIP_CHECK(hipFree(ptr));
+    });
+}
+int tomos_memcpy_htod(void* dst, const void* src, size_t bytes, tomos_stream_t s) {
+    return guarded([&] {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+int tomos_memcpy_dtoh(void* dst, const void* src, size_t bytes, tomos_stream_t s) {
+    return guarded([&] {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+int tomos_memcpy_dtod(void* dst, const void* src, size_t bytes, tomos_stream_t s) {
+    return guarded([&] {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+    });
+}
+int tomos_memset(void* ptr, int value, size_t bytes, tomos_stream_t s) {
+    return guarded([&] { TOMOS_HIP_CHECK(hipMemsetAsync(ptr, value, bytes, s)); });
+}
+int tomos_stream_synchronize(tomos_stream_t s) {
+    return guarded([&] { TOMOS_HIP_CHECK(hipStreamSynchronize(s)); });
+}
+int tomos_device_synchronize(void) {
+    return guarded([&] { TOMOS_HIP_CHECK(hipDeviceSynchronize()); });
+}
+int tomos_stream_create(tomos_stream_t* s) {
+    return guarded([&] {
+        require(s != nullptr, "tomos_stream_create: null");
+        hipStream_t h;
+        TOMOS_HIP_CHECK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+        *s = h;
+    });
+}
+int tomos_stream_destroy(tomos_stream_t s) {
+    return guarded([&] {
+        if (s) TOMOS_HIP_CHECK(hipStreamDestroy(s));
+    });
+}
+int tomos_event_create(tomos_event_t* e) {
+    return guarded([&] {
+        require(e != nullptr, "tomos_event_create: null");
+        hipEvent_t h;
+        TOMOS_HIP_CHECK(hipEventCreate(&h));
+        *e = h;
+    });
+}
+int tomos_event_destroy(tomos_event_t e) {
+    return guarded([&] {
+        if (e) TOMOS_HIP_CHECK(hipEventDestroy(e));
+    });
+}
+int tomos_event_record(tomos_event_t e, tomos_stream_t s) {
+    return guarded([&] { TOMOS_HIP_CHECK(hipEventRecord(e, s)); });
+}
+int tomos_event_elapsed_ms(float* ms, tomos_event_t a, tomos_event_t b) {
+    return guarded([&] {
+        TOMOS_HIP_CHECK(hipEventSynchronize(b));
+        TOMOS_HIP_CHECK(hipEventElapsedTime(ms, a, b));
+    });
+}
+
+int tomos_fill_normal_f32(float* out, int64_t m, int64_t n, int64_t ld, uint64_t seed,
+                          const float* col_scale, int exp_lo, int exp_hi, tomos_stream_t s) {
+    return guarded([&] {
+        require(m >= 0 && n >= 0 && ld >= m, "tomos_fill_normal_f32: bad shape");
+        require(exp_hi >= exp_lo, "tomos_fill_normal_f32: exp_hi < exp_lo");
+        if (m == 0 || n == 0) return;
+        require(out != nullptr, "tomos_fill_normal_f32: null out");
+        const int64_t total = m * n;
+        const int64_t blocks = std::min<int64_t>(ceil_div(total, 256), 65536);
+        fill_normal_kernel<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s>>>(
+            out, m, n, ld, seed, col_scale, exp_lo, exp_hi);
+        TOMOS_LAUNCH_CHECK("fill_normal_kernel");
+    });
+}
+
+}  // extern "C"
