@@ -1,0 +1,296 @@
+"""Benchmark: LSQR time-to-1e-6 on one MI355X, mixed precision + sketch-QR
+preconditioner vs plain FP32, plus the packed GEMV's fraction of HBM peak.
+
+Workload (one "step"): A = G diag(d), 2,000,000 x 1000 FP32, d log-spaced
+1 .. 1e-6 (kappa ~ 1e6), stored as FP32_BF16 (planar wh/wr, 8 GB); b = A
+x_true in FP64 (a consistent system, so the FP64 least-squares solution is
+x_true).  A step builds the preconditioner (hashed CountSketch with s = 4000
+rows, FP64 Householder QR, R^-1) and runs restarted LSQR whose inner
+iterations use the hi-only (bf16) GEMV and whose restart residual uses the
+exact (wh+wr) GEMV, to relative residual 1e-6.  All inputs are resident in
+HBM before timing.  Plain FP32 LSQR (FP32 A, no preconditioner, same
+stopping rule) is timed once beside it, capped at --plain-cap iterations.
+
+The roofline object reports the hi-only FP32_BF16 GEMV y = A x at the full
+2e6 x 1000 shape, timed with HIP events on the launch stream.  The CPU
+baseline is the oracle's FP32 LSQR with OpenBLAS sgemv (the reference's CPU
+path) on a row sample of the same matrix, scaled to the full row count.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--m", type=int, default=2_000_000)
+    p.add_argument("--n", type=int, default=1000)
+    p.add_argument("--s", type=int, default=4000)
+    p.add_argument("--kappa-exp", type=float, default=6.0)
+    p.add_argument("--tol", type=float, default=1e-6)
+    p.add_argument("--inner", type=int, default=30, help="LSQR iterations per restart (mixed)")
+    p.add_argument("--restarts", type=int, default=40)
+    p.add_argument("--plain-cap", type=int, default=1500)
+    p.add_argument("--gemv-reps", type=int, default=20)
+    p.add_argument("--cpu-rows", type=int, default=200_000)
+    p.add_argument("--cpu-iters", type=int, default=30)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--check-every", type=int, default=8)
+    return p.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the LSQR path here is single-GPU "
+              "(row-sharding across GPUs is a design note, not built); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from tomos_amd import _capi as C
+    from tomos_amd._capi import DeviceBuffer
+
+    C.call("tomos_set_device", 0)
+    cu, amaj, amin, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    C.call("tomos_get_device_info", ctypes.byref(cu), ctypes.byref(amaj), ctypes.byref(amin),
+           ctypes.byref(mem))
+    m, n, s = args.m, args.n, args.s
+    log("device: %d CUs, %.0f GB; workload %d x %d, s=%d" % (cu.value, mem.value / 1e9, m, n, s))
+
+    # ---- synthetic data, resident in HBM ------------------------------------
+    d = np.logspace(0, -args.kappa_exp, n).astype(np.float32)
+    dd = DeviceBuffer.from_numpy(d)
+    A = DeviceBuffer(m * n * 4)
+    C.call("tomos_fill_normal_f32", A.ptr, m, n, m, 1, dd.ptr, 0, 0, None)
+    wh, wr = DeviceBuffer(m * n * 2), DeviceBuffer(m * n * 2)
+    C.call("tomos_pack", C.PACK_FP32_BF16, m, n, A.ptr, m, wh.ptr, wr.ptr, m, None)
+    rng = np.random.default_rng(2)
+    x_true = rng.standard_normal(n)
+    xt_d = DeviceBuffer.from_numpy(x_true)
+    b = DeviceBuffer(m * 8)
+    C.call("tomos_gemv", C.F32, C.TIER_EXACT, 0, 1.0, m, n, A.ptr, None, m, C.F64, xt_d.ptr,
+           0.0, C.F64, b.ptr, None, 0, None)
+    C.call("tomos_device_synchronize")
+    b_host = b.to_numpy(np.float64, (m,))
+    bnorm = float(np.linalg.norm(b_host))
+    log("data ready")
+
+    # ---- buffers for the solve -----------------------------------------------
+    sketch = DeviceBuffer(s * n * 8)
+    cs_bytes = C.lib().tomos_countsketch_scratch_size(m, n, s)
+    cs_scratch = DeviceBuffer(cs_bytes)
+    x = DeviceBuffer(n * 8)
+    opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64,
+                        args.tol, args.tol, args.inner, args.restarts, args.check_every, 0)
+    ar_bytes = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
+    sc_bytes = C.lib().tomos_lsqr_scratch_size(ctypes.byref(opt), m, n)
+    arena, scr = DeviceBuffer(ar_bytes), DeviceBuffer(sc_bytes)
+    hist_cap = args.inner * args.restarts + args.restarts + 1
+    hist = np.zeros(hist_cap)
+    ev0, ev1, ev2 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    for e in (ev0, ev1, ev2):
+        C.call("tomos_event_create", ctypes.byref(e))
+
+    def solve_mixed():
+        it = ctypes.c_int(0)
+        hl = ctypes.c_int(0)
+        C.call("tomos_event_record", ev0, None)
+        C.call("tomos_countsketch_apply", C.FP32_BF16, C.TIER_EXACT, m, n, wh.ptr, wr.ptr, m, s,
+               sketch.ptr, s, cs_scratch.ptr, cs_bytes, None)
+        C.call("tomos_sketch_qr_precond_f64", s, n, sketch.ptr, s, None)
+        # the leading n x n block of the sketch now holds R^-1 (ld = s)
+        C.call("tomos_event_record", ev1, None)
+        C.call("tomos_lsqr_solve", ctypes.byref(opt), m, n, wh.ptr, wr.ptr, m, b.ptr, x.ptr,
+               sketch.ptr, s, arena.ptr, ar_bytes, scr.ptr, sc_bytes, ctypes.byref(it),
+               hist.ctypes.data, hist_cap,
+               ctypes.byref(hl), None)
+        C.call("tomos_event_record", ev2, None)
+        f1, f2 = ctypes.c_float(), ctypes.c_float()
+        C.call("tomos_event_elapsed_ms", ctypes.byref(f1), ev0, ev1)
+        C.call("tomos_event_elapsed_ms", ctypes.byref(f2), ev1, ev2)
+        return it.value, hl.value, f1.value, f2.value
+
+    for w in range(args.warmup):
+        it, hl, pms, sms = solve_mixed()
+        log("warmup %d: iters=%d precond=%.1f ms solve=%.1f ms" % (w, it, pms, sms))
+    C.call("tomos_device_synchronize")
+    t0 = time.perf_counter()
+    per = []
+    for k in range(args.steps):
+        per.append(solve_mixed())
+        log("step %d: iters=%d precond=%.1f ms solve=%.1f ms" % (k, per[-1][0], per[-1][2], per[-1][3]))
+    C.call("tomos_device_synchronize")
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / max(args.steps, 1)
+    iters, hist_len, precond_ms, solve_ms = per[-1]
+
+    # ---- accuracy of the last solve (FP64 residual with the exact A) --------
+    xs = x.to_numpy(np.float64, (n,))
+    r = DeviceBuffer(m * 8)
+    C.call("tomos_memcpy_dtod", r.ptr, b.ptr, m * 8, None)
+    C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, 0, -1.0, m, n, wh.ptr, wr.ptr, m, C.F64,
+           x.ptr, 1.0, C.F64, r.ptr, None, 0, None)
+    relres = float(np.linalg.norm(r.to_numpy(np.float64, (m,)))) / bnorm
+    err = float(np.linalg.norm(xs - x_true) / np.linalg.norm(x_true))
+    log("mixed+precond: relres=%.3e err_vs_fp64=%.3e iters=%d" % (relres, err, iters))
+
+    # ---- plain FP32 LSQR on the same GPU --------------------------------------
+    plain = None
+    if not args.no_plain:
+        bf = DeviceBuffer.from_numpy(b_host.astype(np.float32))
+        xf = DeviceBuffer(n * 4)
+        po = C.LsqrOptions(C.F32, C.TIER_DEFAULT, C.TIER_DEFAULT, C.F32, C.F32, -1, args.tol,
+                           args.tol, args.plain_cap, 1, 64, 0)
+        pa = C.lib().tomos_lsqr_arena_size(ctypes.byref(po), m, n)
+        pw = C.lib().tomos_lsqr_scratch_size(ctypes.byref(po), m, n)
+        parena, pws = DeviceBuffer(pa), DeviceBuffer(pw)
+        it = ctypes.c_int(0)
+        hl = ctypes.c_int(0)
+        ph = np.zeros(args.plain_cap + 2)
+        C.call("tomos_device_synchronize")
+        t0 = time.perf_counter()
+        C.call("tomos_lsqr_solve", ctypes.byref(po), m, n, A.ptr, None, m, bf.ptr, xf.ptr, None,
+               n, parena.ptr, pa, pws.ptr, pw, ctypes.byref(it), ph.ctypes.data, ph.size, ctypes.byref(hl), None)
+        C.call("tomos_device_synchronize")
+        pt = (time.perf_counter() - t0) * 1e3
+        xp = xf.to_numpy(np.float32, (n,)).astype(np.float64)
+        xpd = DeviceBuffer.from_numpy(xp)
+        C.call("tomos_memcpy_dtod", r.ptr, b.ptr, m * 8, None)
+        C.call("tomos_gemv", C.F32, C.TIER_EXACT, 0, -1.0, m, n, A.ptr, None, m, C.F64, xpd.ptr,
+               1.0, C.F64, r.ptr, None, 0, None)
+        prr = float(np.linalg.norm(r.to_numpy(np.float64, (m,)))) / bnorm
+        plain = {"ms": round(pt, 2), "iterations": it.value, "relres": prr,
+                 "converged": bool(prr < args.tol), "cap": args.plain_cap,
+                 "ms_per_iteration": round(pt / max(it.value, 1), 4)}
+        log("plain fp32: %s" % plain)
+        del bf, xf, parena, pws, xpd
+
+    # ---- roofline: hi-only packed GEMV at the full shape ---------------------
+    y = DeviceBuffer(m * 4)
+    xv = DeviceBuffer.from_numpy(rng.standard_normal(n).astype(np.float32))
+    uv = DeviceBuffer.from_numpy(rng.standard_normal(m).astype(np.float32))
+    zv = DeviceBuffer(n * 4)
+    gsc = C.lib().tomos_gemv_scratch_size(C.FP32_BF16, 0, 1, m, n)
+    gs = DeviceBuffer(max(gsc, 1))
+
+    def time_gemv(trans, a_type, tier, a_ptr, wr_ptr):
+        xin, yout = (uv, zv) if trans else (xv, y)
+        for _ in range(2):
+            C.call("tomos_gemv", a_type, tier, trans, 1.0, m, n, a_ptr, wr_ptr, m, C.F32, xin.ptr,
+                   0.0, C.F32, yout.ptr, gs.ptr, gsc, None)
+        C.call("tomos_event_record", ev0, None)
+        for _ in range(args.gemv_reps):
+            C.call("tomos_gemv", a_type, tier, trans, 1.0, m, n, a_ptr, wr_ptr, m, C.F32, xin.ptr,
+                   0.0, C.F32, yout.ptr, gs.ptr, gsc, None)
+        C.call("tomos_event_record", ev1, None)
+        f = ctypes.c_float()
+        C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+        return f.value / args.gemv_reps
+
+    gemv_bytes_hi = 2.0 * m * n + 4.0 * n + 4.0 * m  # beta = 0
+    gemv_bytes_f32 = 4.0 * m * n + 4.0 * n + 4.0 * m
+    t_n = time_gemv(0, C.FP32_BF16, 0, wh.ptr, None)
+    t_t = time_gemv(1, C.FP32_BF16, 0, wh.ptr, None)
+    t_nx = time_gemv(0, C.FP32_BF16, 2, wh.ptr, wr.ptr)
+    t_f = time_gemv(0, C.F32, 0, A.ptr, None)
+    t_ft = time_gemv(1, C.F32, 0, A.ptr, None)
+    gbs = lambda byts, ms: byts / (ms * 1e-3) / 1e9
+    achieved = gbs(gemv_bytes_hi, t_n)
+    gemv_detail = {
+        "bf16_hi_N": {"ms": round(t_n, 4), "GBps": round(achieved, 1)},
+        "bf16_hi_T": {"ms": round(t_t, 4), "GBps": round(gbs(gemv_bytes_hi, t_t), 1)},
+        "bf16_exact_N": {"ms": round(t_nx, 4), "GBps": round(gbs(gemv_bytes_f32, t_nx), 1)},
+        "fp32_N": {"ms": round(t_f, 4), "GBps": round(gbs(gemv_bytes_f32, t_f), 1)},
+        "fp32_T": {"ms": round(t_ft, 4), "GBps": round(gbs(gemv_bytes_f32, t_ft), 1)},
+    }
+    log("gemv: %s" % gemv_detail)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "gemv_bf16_hi_2e6x1000_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    # ---- CPU baseline: the oracle's OpenBLAS FP32 LSQR on a row sample -------
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        rc = O.blas_load(threads)
+        rows = min(args.cpu_rows, m)
+        Ahost = np.empty((rows, n), np.float32, order="F")
+        # first `rows` rows of every column of the device matrix
+        for j in range(n):
+            C.call("tomos_memcpy_dtoh", Ahost[:, j].ctypes.data, A.ptr + j * m * 4, rows * 4, None)
+        bh = b_host[:rows].astype(np.float32)
+        t0 = time.perf_counter()
+        _, cit, _ = O.lsqr_f32(Ahost, bh, 1e-30, args.cpu_iters, 1e-30)
+        ct = (time.perf_counter() - t0) * 1e3
+        per_it = ct / max(cit, 1) * (m / rows)
+        cpu = {"value": round(per_it, 3), "unit": "ms per plain-FP32 LSQR iteration (full 2e6 rows)",
+               "cores": threads, "kind": "port",
+               "sample": "oracle lsqr_f32 (OpenBLAS 0.3.29 sgemv from numpy, %s) on the first %d of "
+                         "%d rows of the same matrix, %d iterations, time scaled by %d/%d"
+                         % ("loaded" if rc == 0 else "NOT loaded: plain loops", rows, m, cit, m, rows),
+               "gpu_same_unit": plain["ms_per_iteration"] if plain else None}
+        log("cpu: %s" % cpu)
+
+    speedup = None
+    if plain:
+        speedup = round(plain["ms"] / ms_per_step, 2)
+    out = {
+        "metric": "LSQR time-to-1e-6 residual, mixed+precond vs plain FP32 on MI355X; GEMV %HBM",
+        "value": round(ms_per_step, 3),
+        "unit": "ms (time to relres 1e-6, preconditioner build included)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (hi-only GEMV) / fp32 vectors / fp64 scalars+QR",
+        "data": "synthetic: A = G diag(d), G ~ N(0,1) (device hash Box-Muller), d log-spaced 1..1e-%g; b = A x_true in FP64" % args.kappa_exp,
+        "config": {"workload": "C4 sketch-preconditioned mixed-precision LSQR", "m": m, "n": n,
+                   "storage": "FP32_BF16 planar", "sketch_rows": s, "tol": args.tol,
+                   "inner_iters": args.inner, "restarts_max": args.restarts},
+        "iterations": iters,
+        "precond_ms": round(precond_ms, 3),
+        "solve_ms": round(solve_ms, 3),
+        "relres": relres,
+        "err_vs_fp64": err,
+        "plain_fp32": plain,
+        "speedup_vs_plain_fp32": speedup,
+        "speedup_is_lower_bound": bool(plain and not plain["converged"]),
+        "gemv": gemv_detail,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "gemv_n_kernel<FP32_BF16 hi-only> y=A x, %dx%d, %.3f GB algorithmic per launch"
+                               % (m, n, gemv_bytes_hi / 1e9)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+// C++ API mirror of TOMOS's BLAS front-ends, sketch + preconditioner,
+// LSQR solver and dispatch registries for the LSQR hot path
+// (include/core/blas.hpp, solver.hpp, countsketch.hpp, src/core/precond/
+// sketch_qr.hpp, src/dispatch/*_registries.hpp).
+//
+// Registry contract kept alike (src/dispatch/dense_registries.hpp:29-34,
+// fp32_16_registries.hpp:22-70, lsqr_registries.hpp:21-68): singleton tables
+// of function pointers built once and immutable, keyed by
+// (Device, NumericType...) with the same mixed-radix encode; a lookup of an
+// empty slot throws std::runtime_error.  Only the Device::HIP slots are
+// populated: there is no CPU implementation in this library.
+#pragma once
+
+#include <vector>
+
+#include "core.hpp"
+
+namespace tomos {
+
+// ---------------------------------------------------------------- BLAS
+// y = alpha*op(A)*x + beta*y through dispatch::Dense::lookup_mv with the
+// default compute rule (FP32_16 -> FP16 "pure_half", FP32_BF16 -> BF16
+// hi-only, native types -> their own type).  blas.cpp:23-54
+Status mv_apply(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x, double beta,
+                Dense& y);
+// Explicit compute tier; workspace: optional device scratch (float slots)
+// sized by mv_workspace_size().  blas.hpp:157-159
+Status mv_apply(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x, double beta,
+                Dense& y, NumericType compute_type, float* workspace = nullptr);
+std::size_t mv_workspace_size(Dense& A, TransposeType trans, NumericType compute_type);
+
+// precond_apply = mv_apply on the preconditioner.  blas.cpp:88-96
+void precond_apply(Device dev, TransposeType trans, double alpha, Dense& P, Dense& x, double beta,
+                   Dense& y);
+
+// C = alpha*op(A)*op(B) + beta*C.  use_workspace selects the
+// decode-then-GEMM tier for packed A (blas.hpp:212-214).
+Status mm_apply(Device dev, TransposeType transA, TransposeType transB, double alpha, Dense& A,
+                Dense& B, double beta, Dense& C, bool use_workspace = false);
+
+template <typename value_t>
+void set_values(value_t value, Dense& target);
+
+// FP32_16 tier entry points (blas.hpp:259-284)
+Status split_fp32_16(Dense& source, Dense& target);
+Status combine_fp32_16(Dense& source, Dense& target);
+Status mv_apply_fp32_16_pure_half(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x,
+                                  double beta, Dense& y);
+Status mv_apply_fp32_16_fast(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x,
+                             double beta, Dense& y);
+Status mv_apply_fp32_16_ext(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x,
+                            double beta, Dense& y);
+Status mm_apply_fp32_16_pure_half(Device dev, TransposeType transA, TransposeType transB,
+                                  double alpha, Dense& A, Dense& B, double beta, Dense& C);
+Status mm_apply_fp32_16_fast(Device dev, TransposeType transA, TransposeType transB, double alpha,
+                             Dense& A, Dense& B, double beta, Dense& C);
+Status mm_apply_fp32_16_ext(Device dev, TransposeType transA, TransposeType transB, double alpha,
+                            Dense& A, Dense& B, double beta, Dense& C);
+
+// FP32_BF16 counterparts (the reference has these only as raw kernels,
+// fp32_bf16_pack.hpp / fp32_16_cuda_kernels.cu:28785, :29636, :7587).
+Status split_fp32_bf16(Dense& source, Dense& target, bool rounded = false);
+Status combine_fp32_bf16(Dense& source, Dense& target, bool rounded = false);
+Status mv_apply_fp32_bf16_hi(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x,
+                             double beta, Dense& y);
+Status mv_apply_fp32_bf16_ext(Device dev, TransposeType trans, double alpha, Dense& A, Dense& x,
+                              double beta, Dense& y);
+
+// ------------------------------------------------------ sketch + precond
+// The hashed CountSketch S (k x m): column i of S has +-1 in row h(i),
+// h and the sign from splitmix64 with the reference's fixed seeds
+// (countsketch_kernels.cpp:59-85).  Deterministic.
+struct CountSketch {
+    dim<2> size;  // {k, m}
+};
+CountSketch create_count_sketch(std::size_t k, std::size_t m);
+void generate_count_sketch(CountSketch& sketch);
+// target (k x n, FP64 or FP32) = S * source.  countsketch.hpp:56-60
+void apply_count_sketch(CountSketch& sketch, Dense& source, Dense& target);
+// Householder QR of the k x n FP64 sketch; the leading n x n block is
+// overwritten with R^-1.  Throws std::invalid_argument if k < n and
+// std::runtime_error if R is singular.  sketch_qr.cpp:68-122
+void generate_sketch_qr_precond(CountSketch& sketch, Dense& sketched_mtx);
+
+// ------------------------------------------------------------------ LSQR
+// include/core/solver.hpp:15-72
+Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type);
+Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type,
+                                NumericType precond_type);
+void lsqr_solve(Context context, Dense& A, Dense& b, Dense& x, Workspace& workspace, double tol,
+                int max_iters, double max_relres, int* iterations = nullptr,
+                std::vector<double>* residual_history = nullptr, int restarts = 1,
+                Dense* mtx_refine_precision = nullptr);
+
+// ------------------------------------------------------------ registries
+namespace dispatch {
+
+constexpr std::size_t num_registry_devices = static_cast<std::size_t>(Device::Count);
+constexpr std::size_t num_registry_numeric_types = static_cast<std::size_t>(NumericType::Count);
+
+using mv_dense_fn = void (*)(TransposeType trans, double alpha, tomos::Dense& A, tomos::Dense& x,
+                             double beta, tomos::Dense& y, float* workspace);
+using mm_dense_fn = void (*)(TransposeType transA, TransposeType transB, double alpha,
+                             tomos::Dense& A, tomos::Dense& B, double beta, tomos::Dense& C);
+using split_fn = void (*)(tomos::Dense& source, tomos::Dense& target);
+using combine_fn = void (*)(tomos::Dense& source, tomos::Dense& target);
+using lsqr_dense_fn = void (*)(Context context, tomos::Dense& A, tomos::Dense& b, tomos::Dense& x,
+                               Workspace& workspace, double tol, int max_iters, double max_relres,
+                               int* iterations, std::vector<double>* history, int restarts,
+                               tomos::Dense* mtx_refine);
+using apply_count_sketch_dense_fn = void (*)(CountSketch& S, tomos::Dense& source,
+                                             tomos::Dense& target);
+
+struct Dense {
+    static const Dense& get();
+    mv_dense_fn lookup_mv(Device dev, NumericType A, NumericType x, NumericType y,
+                          NumericType compute) const;
+    mm_dense_fn lookup_mm(Device dev, NumericType A, NumericType B, NumericType C,
+                          bool use_workspace) const;
+    static std::size_t encode_mv(Device dev, NumericType A, NumericType x, NumericType y,
+                                 NumericType compute);
+    static std::size_t encode_mm(Device dev, NumericType A, NumericType B, NumericType C,
+                                 bool use_workspace);
+
+private:
+    Dense();
+    static constexpr std::size_t Nmv = num_registry_devices * num_registry_numeric_types *
+                                       num_registry_numeric_types * num_registry_numeric_types *
+                                       num_registry_numeric_types;
+    static constexpr std::size_t Nmm = num_registry_devices * num_registry_numeric_types *
+                                       num_registry_numeric_types * num_registry_numeric_types * 2;
+    std::array<mv_dense_fn, Nmv> mv_{};
+    std::array<mm_dense_fn, Nmm> mm_{};
+};
+
+// Per-device slots of the packed formats.  fp32_16_registries.hpp:37-70
+struct Fp32_16 {
+    static const Fp32_16& get();
+    split_fn lookup_split(Device dev) const;
+    combine_fn lookup_combine(Device dev) const;
+    mv_dense_fn lookup_mv_pure_half(Device dev) const;
+    mv_dense_fn lookup_mv_fast(Device dev) const;
+    mv_dense_fn lookup_mv_accurate(Device dev) const;
+    mm_dense_fn lookup_mm_pure_half(Device dev) const;
+    mm_dense_fn lookup_mm_fast(Device dev) const;
+    mm_dense_fn lookup_mm_accurate(Device dev) const;
+    mm_dense_fn lookup_mm_workspace(Device dev) const;
+
+private:
+    Fp32_16();
+    std::array<split_fn, num_registry_devices> split_{};
+    std::array<combine_fn, num_registry_devices> combine_{};
+    std::array<mv_dense_fn, num_registry_devices> mv_pure_half_{}, mv_fast_{}, mv_accurate_{};
+    std::array<mm_dense_fn, num_registry_devices> mm_pure_half_{}, mm_fast_{}, mm_accurate_{},
+        mm_workspace_{};
+};
+
+struct Fp32Bf16 {
+    static const Fp32Bf16& get();
+    split_fn lookup_split(Device dev) const;
+    split_fn lookup_split_rounded(Device dev) const;
+    combine_fn lookup_combine(Device dev) const;
+    combine_fn lookup_combine_rounded(Device dev) const;
+    mv_dense_fn lookup_mv_hi(Device dev) const;
+    mv_dense_fn lookup_mv_accurate(Device dev) const;
+    mm_dense_fn lookup_mm_hi(Device dev) const;
+    mm_dense_fn lookup_mm_accurate(Device dev) const;
+
+private:
+    Fp32Bf16();
+    std::array<split_fn, num_registry_devices> split_{}, split_rounded_{};
+    std::array<combine_fn, num_registry_devices> combine_{}, combine_rounded_{};
+    std::array<mv_dense_fn, num_registry_devices> mv_hi_{}, mv_accurate_{};
+    std::array<mm_dense_fn, num_registry_devices> mm_hi_{}, mm_accurate_{};
+};
+
+struct Lsqr {
+    static const Lsqr& get();
+    lsqr_dense_fn lookup_solve_dense(Device dev, NumericType mv_type, NumericType value_type,
+                                     NumericType refine_type, NumericType precond_type) const;
+    static std::size_t encode(Device dev, NumericType mv_type, NumericType value_type,
+                              NumericType refine_type, NumericType precond_type);
+
+private:
+    Lsqr();
+    static constexpr std::size_t N = num_registry_devices * num_registry_numeric_types *
+                                     num_registry_numeric_types * num_registry_numeric_types *
+                                     num_registry_numeric_types;
+    std::array<lsqr_dense_fn, N> dense_{};
+};
+
+struct Sketch {
+    static const Sketch& get();
+    apply_count_sketch_dense_fn lookup_apply_count_sketch_dense(Device dev,
+                                                                NumericType source) const;
+
+private:
+    Sketch();
+    std::array<apply_count_sketch_dense_fn, num_registry_devices * num_registry_numeric_types>
+        apply_{};
+};
+
+}  // namespace dispatch
+}  // namespace tomos

@@ -108,6 +108,11 @@ int tomos_event_destroy(tomos_event_t ev);
 int tomos_event_record(tomos_event_t ev, tomos_stream_t stream);
 int tomos_event_elapsed_ms(float* ms, tomos_event_t start, tomos_event_t stop);
 
+/* Device-to-device copy through a plain vectorized kernel (16-byte loads
+ * and stores); exists so profilers can calibrate HBM byte counters against
+ * a kernel of known traffic.  No reference counterpart. */
+int tomos_copy_kernel(void* dst, const void* src, size_t bytes, tomos_stream_t stream);
+
 /* Deterministic synthetic data on the device (no reference counterpart;
  * used by benchmarks to build the 1e6..2e6-row problems in HBM).
  * out[i + j*ld] for i<m, j<n: N(0,1) from a counter-based hash of
@@ -234,17 +239,26 @@ typedef struct tomos_lsqr_options {
     int fused;         /* 1: one pass over A per iteration (A*p and A^T*u fused) */
 } tomos_lsqr_options;
 
-size_t tomos_lsqr_workspace_size(const tomos_lsqr_options* opt, int64_t m, int64_t n);
+/* Arena: the Krylov vectors u (m), v, w, temp1 (n, value_type) and
+ * precond_v (n, precond_type), carved in the reference's order and sized by
+ * the reference's formula (create_lsqr_workspace, lsqr.cpp:389-406).
+ * Scratch: device state, residual history and GEMV partial sums (the
+ * reference's Workspace::mv_transpose_scratch_, workspace.hpp:170-182). */
+size_t tomos_lsqr_arena_size(const tomos_lsqr_options* opt, int64_t m, int64_t n);
+size_t tomos_lsqr_scratch_size(const tomos_lsqr_options* opt, int64_t m, int64_t n);
 
 /* x (refine_type) is overwritten.  rinv: n x n upper (ldr) or NULL.
+ * arena must hold tomos_lsqr_arena_size() bytes (else
+ * TOMOS_ERR_WORKSPACE, as lsqr_registries.cpp:59-63); scratch holds
+ * tomos_lsqr_scratch_size() bytes or is NULL (library-owned buffer).
  * history (host, nullable) receives up to history_cap relative residuals,
  * one per stopping check, as lsqr_solve's residual_history.
  * Synchronous: returns after the solve completed. */
 int tomos_lsqr_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const void* a,
                      const uint16_t* wr, int64_t lda, const void* b, void* x,
-                     const void* rinv, int64_t ldr, void* workspace, size_t workspace_bytes,
-                     int* iterations, double* history, int history_cap, int* history_len,
-                     tomos_stream_t stream);
+                     const void* rinv, int64_t ldr, void* arena, size_t arena_bytes,
+                     void* scratch, size_t scratch_bytes, int* iterations, double* history,
+                     int history_cap, int* history_len, tomos_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -45,3 +45,87 @@ def gemv(a_type, tier, trans, alpha, m, n, a_buf, wr_buf, lda, x, beta=0.0, y0=N
            wr_buf.ptr if wr_buf else None, lda, x_type, xd.ptr, beta, y_type, yd.ptr, sc_ptr,
            sc_bytes, None)
     return yd.to_numpy(ydt, (rows,))
+
+
+def lsqr(A_or_planes, b, a_type=C.F32, mv_tier=C.TIER_DEFAULT, restart_tier=None,
+         value_type=C.F32, refine_type=C.F32, rinv=None, precond_type=-1, tol=1e-6,
+         max_iters=400, max_relres=None, restarts=1, check_every=1, lda=None, m=None, n=None,
+         fused=0, history_cap=100000):
+    """Runs tomos_lsqr_solve; A_or_planes is a numpy matrix (F32/F64) or a
+    (wh_buf, wr_buf) pair of packed device planes (then m, n, lda needed)."""
+    if restart_tier is None:
+        restart_tier = mv_tier
+    if max_relres is None:
+        max_relres = tol
+    if isinstance(A_or_planes, tuple):
+        a_buf, wr_buf = A_or_planes
+        assert m is not None and n is not None
+        lda = lda or m
+    else:
+        A = np.asfortranarray(A_or_planes)
+        m, n = A.shape
+        lda = m
+        a_buf, wr_buf = dev(A), None
+    rdt = np.float64 if refine_type == C.F64 else np.float32
+    bd = dev(np.asarray(b, dtype=rdt))
+    xd = DeviceBuffer(max(n, 1) * np.dtype(rdt).itemsize)
+    rd = None
+    ldr = max(n, 1)
+    if rinv is not None:
+        pdt = np.float64 if precond_type == C.F64 else np.float32
+        rd = dev(np.asarray(rinv, dtype=pdt))
+    opt = C.LsqrOptions(a_type, mv_tier, restart_tier, value_type, refine_type, precond_type,
+                        tol, max_relres, max_iters, restarts, check_every, fused)
+    ab = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
+    sb = C.lib().tomos_lsqr_scratch_size(ctypes.byref(opt), m, n)
+    assert ab > 0 and sb > 0, C.lib().tomos_last_error()
+    arena, scr = DeviceBuffer(ab), DeviceBuffer(sb)
+    it = ctypes.c_int(0)
+    hl = ctypes.c_int(0)
+    hist = np.zeros(history_cap, np.float64)
+    C.call("tomos_lsqr_solve", ctypes.byref(opt), m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
+           lda, bd.ptr, xd.ptr, rd.ptr if rd is not None else None, ldr, arena.ptr, ab, scr.ptr,
+           sb, ctypes.byref(it), hist.ctypes.data, history_cap, ctypes.byref(hl), None)
+    x = xd.to_numpy(rdt, (n,))
+    return x, it.value, hist[:min(hl.value, history_cap)].copy()
+
+
+def countsketch_indices(m, s):
+    b = DeviceBuffer(m * 4)
+    g = DeviceBuffer(m)
+    C.call("tomos_countsketch_indices", m, s, b.ptr, g.ptr, None)
+    return b.to_numpy(np.int32, (m,)), g.to_numpy(np.int8, (m,))
+
+
+def countsketch_apply(a_type, tier, m, n, a_buf, wr_buf, lda, s):
+    y = DeviceBuffer(s * n * 8)
+    C.call("tomos_countsketch_apply", a_type, tier, m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
+           lda, s, y.ptr, s, None, 0, None)
+    return y.to_numpy(np.float64, (s, n))
+
+
+def geqrf_r(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    k, n = A.shape
+    ad = dev(A)
+    r = DeviceBuffer(n * n * 8)
+    C.call("tomos_geqrf_r_f64", k, n, ad.ptr, k, r.ptr, n, None, 0, None)
+    return r.to_numpy(np.float64, (n, n))
+
+
+def trtri(R):
+    R = np.asfortranarray(R, dtype=np.float64)
+    n = R.shape[0]
+    rd = dev(R)
+    C.call("tomos_trtri_upper_f64", n, rd.ptr, n, None)
+    return rd.to_numpy(np.float64, (n, n))
+
+
+def sketch_rinv(a_type, tier, m, n, a_buf, wr_buf, lda, s):
+    """CountSketch (s x n) of A on the GPU, then R^-1 of its QR (GPU)."""
+    y = DeviceBuffer(s * n * 8)
+    C.call("tomos_countsketch_apply", a_type, tier, m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
+           lda, s, y.ptr, s, None, 0, None)
+    C.call("tomos_sketch_qr_precond_f64", s, n, y.ptr, s, None)
+    full = y.to_numpy(np.float64, (s, n))
+    return np.triu(full[:n, :n]), y

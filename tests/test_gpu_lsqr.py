@@ -1,0 +1,147 @@
+"""GPU LSQR against the reference's known answers and the CPU oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import hipcall as H
+from tomos_amd import _capi as C
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lsqr_known_answers.json")))
+
+
+def c1_problem(m=20000, n=200, kappa_exp=3.0, noise=1e-6, seed=0):
+    """C1-shaped: A = G diag(d), d log-spaced 1 .. 10^-kappa_exp, b = A x + noise."""
+    rng = np.random.default_rng(seed)
+    d = np.logspace(0, -kappa_exp, n)
+    A = np.asfortranarray((rng.standard_normal((m, n)) * d).astype(np.float32))
+    xt = rng.standard_normal(n)
+    b = A.astype(np.float64) @ xt + noise * rng.standard_normal(m)
+    return A, xt, b
+
+
+@pytest.mark.parametrize("sysd", GOLD["systems"], ids=[s["name"] for s in GOLD["systems"]])
+def test_known_answer_systems(capi, sysd):
+    m, n = sysd["m"], sysd["n"]
+    A = np.array(sysd["a_rows"], np.float64).reshape(m, n)
+    b = np.array(sysd["b"])
+    f32 = sysd["name"].startswith("fp32")
+    t = C.F32 if f32 else C.F64
+    x, it, hist = H.lsqr(A.astype(np.float32 if f32 else np.float64), b, a_type=t, value_type=t,
+                         refine_type=t, tol=sysd["tol"], max_iters=sysd["max_iters"])
+    assert np.allclose(x, sysd["x_expect"], atol=sysd["x_tol"], rtol=0), (x, it)
+    r = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    assert r < sysd["relres_max"] or sysd["relres_max"] >= 1.0
+
+
+def test_matches_oracle_iteration_by_iteration_f64(capi, orc):
+    A, xt, b = c1_problem(m=3000, n=60, kappa_exp=2.0, noise=1e-3, seed=1)
+    A64 = A.astype(np.float64)
+    x, it, hist = H.lsqr(A64, b, a_type=C.F64, value_type=C.F64, refine_type=C.F64, tol=1e-9,
+                         max_iters=300)
+    xo, ito, histo = orc.lsqr_f64(A64, b, 1e-9, 300, 1e-9)
+    assert it == ito
+    # same iterates until rounding differences (reduction order) are amplified
+    # by the loss of orthogonality of plain LSQR
+    assert np.allclose(hist[:15], histo[:15], rtol=1e-9, atol=0)
+    assert np.allclose(hist, histo, rtol=1e-3, atol=0)
+    r = lambda z: np.linalg.norm(A64 @ z - b) / np.linalg.norm(b)
+    assert abs(r(x) - r(xo)) <= 1e-3 * r(xo)
+
+
+def test_restarts_and_history_semantics(capi, orc):
+    A, xt, b = c1_problem(m=2000, n=40, kappa_exp=2.0, noise=0.0, seed=2)
+    A64 = A.astype(np.float64)
+    for check_every in (1, 7):
+        x, it, hist = H.lsqr(A64, b, a_type=C.F64, value_type=C.F64, refine_type=C.F64, tol=1e-12,
+                             max_iters=5, restarts=6, check_every=check_every)
+        xo, ito, histo = orc.lsqr_f64(A64, b, 1e-12, 5, 1e-12, restarts=6)
+        assert it == ito and len(hist) == len(histo)
+        assert np.allclose(hist, histo, rtol=1e-6)
+        assert np.allclose(x, xo, rtol=1e-7, atol=1e-12)
+
+
+def test_c1_plain_fp32(capi, orc):
+    """C1: 20000 x 200, kappa 1e3, tol 1e-6, <= 400 iterations, FP32 on the
+    GPU vs the FP64 oracle.  400 plain iterations do not reach 1e-6 on this
+    spectrum (the oracle and scipy agree), so both run the full budget and
+    the FP32 residual must track the FP64 one."""
+    A, xt, b = c1_problem()
+    x, it, hist = H.lsqr(A, b.astype(np.float32), tol=1e-6, max_iters=400)
+    xo, ito, histo = orc.lsqr_f64(A.astype(np.float64), b, 1e-6, 400, 1e-6)
+    A64 = A.astype(np.float64)
+    rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
+    rro = np.linalg.norm(A64 @ xo - b) / np.linalg.norm(b)
+    assert it == ito == 400
+    assert rr < 1.5 * rro, (rr, rro)
+    assert np.allclose(hist[:20], histo[:20], rtol=1e-4)
+
+
+def test_sketch_precond_known_answer(capi):
+    sp = GOLD["sketch_precond"]
+    m, n, k = sp["m"], sp["n"], sp["k"]
+    A = np.array(sp["a_colmajor"]).reshape(n, m).T.copy(order="F")
+    xt = np.array(sp["x_true"])
+    b = A @ xt
+    Ad = H.dev(A)
+    rinv, _ = H.sketch_rinv(C.F64, C.TIER_DEFAULT, m, n, Ad, None, m, k)
+    x, it, _ = H.lsqr(A, b, a_type=C.F64, value_type=C.F64, refine_type=C.F64, rinv=rinv,
+                      precond_type=C.F64, tol=sp["tol"], max_iters=sp["max_iters"])
+    assert np.allclose(x, xt, atol=sp["x_tol"], rtol=0), (x, xt, it)
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < sp["relres_max"]
+
+
+def test_precond_matches_oracle_with_same_rinv(capi, orc):
+    A, xt, b = c1_problem(m=8000, n=100, kappa_exp=4.0, noise=0.0, seed=3)
+    A64 = A.astype(np.float64)
+    Ad = H.dev(A64)
+    rinv, _ = H.sketch_rinv(C.F64, C.TIER_DEFAULT, 8000, 100, Ad, None, 8000, 400)
+    x, it, hist = H.lsqr(A64, b, a_type=C.F64, value_type=C.F64, refine_type=C.F64, rinv=rinv,
+                         precond_type=C.F64, tol=1e-10, max_iters=200)
+    xo, ito, histo = orc.lsqr_f64(A64, b, 1e-10, 200, 1e-10, Rinv=rinv)
+    assert it == ito and it < 60, (it, ito)
+    assert np.allclose(hist, histo, rtol=1e-6)
+    assert np.allclose(x, xo, rtol=1e-6, atol=1e-9 * np.abs(xo).max())
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-6
+
+
+@pytest.mark.parametrize("a_type,variant", [(C.FP32_BF16, C.PACK_FP32_BF16), (C.FP32_16, C.PACK_FP32_16)])
+def test_packed_mixed_precision_restarts_reach_1e6(capi, a_type, variant):
+    """Hi-only matvecs per iteration + exact (wh+wr) residual at restarts."""
+    m, n = 20000, 100
+    A, xt, b = c1_problem(m=m, n=n, kappa_exp=3.0, noise=1e-7, seed=4)
+    wh, wr = H.pack(variant, A)
+    Ad = H.dev(A)
+    rinv, _ = H.sketch_rinv(a_type, C.TIER_EXACT, m, n, wh, wr, m, 4 * n)
+    x, it, hist = H.lsqr((wh, wr), b, a_type=a_type, mv_tier=C.TIER_DEFAULT,
+                         restart_tier=C.TIER_EXACT, value_type=C.F32, refine_type=C.F64,
+                         rinv=rinv, precond_type=C.F64, tol=1e-6, max_iters=12, restarts=20,
+                         m=m, n=n)
+    A64 = A.astype(np.float64)
+    rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
+    assert rr < 1.5e-6, (rr, it)
+    xls = np.linalg.lstsq(A64, b, rcond=None)[0]
+    assert np.linalg.norm(x - xls) / np.linalg.norm(xls) < 1e-3
+
+
+def test_errors(capi):
+    import ctypes
+    A = np.eye(3, dtype=np.float32)
+    Ad, bd, xd = H.dev(A), H.dev(np.ones(3, np.float32)), H.DeviceBuffer(12)
+    bad = C.LsqrOptions(99, 0, 0, C.F32, C.F32, -1, 1e-6, 1e-6, 10, 1, 1, 0)
+    assert C.lib().tomos_lsqr_arena_size(ctypes.byref(bad), 3, 3) == 0
+    rc = C.lib().tomos_lsqr_solve(ctypes.byref(bad), 3, 3, Ad.ptr, None, 3, bd.ptr, xd.ptr, None,
+                                  3, None, 0, None, 0, None, None, 0, None, None)
+    assert rc == 1 and b"a_type" in C.lib().tomos_last_error()
+    opt = C.LsqrOptions(C.F32, 0, 0, C.F32, C.F32, C.F64, 1e-6, 1e-6, 10, 1, 1, 0)
+    rc = C.lib().tomos_lsqr_solve(ctypes.byref(opt), 3, 3, Ad.ptr, None, 3, bd.ptr, xd.ptr, None,
+                                  3, None, 0, None, 0, None, None, 0, None, None)
+    assert rc == 1  # precond_type set without R^-1
+    opt.precond_type = -1
+    ws = H.DeviceBuffer(16)
+    rc = C.lib().tomos_lsqr_solve(ctypes.byref(opt), 3, 3, Ad.ptr, None, 3, bd.ptr, xd.ptr, None,
+                                  3, ws.ptr, 16, None, 0, None, None, 0, None, None)
+    assert rc == 4  # arena too small (lsqr_registries.cpp:59-63)
+    assert C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), 3, 3) == 3 * 4 + 3 * 3 * 4 + 3 + 3 * 4

@@ -1,0 +1,89 @@
+"""GPU CountSketch (bit-exact indices), Householder QR and triangular inverse."""
+import numpy as np
+import pytest
+
+from tests import hipcall as H
+from tomos_amd import _capi as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("m,s", [(1000000, 4000), (40, 30), (12345, 7)])
+def test_indices_bit_exact(capi, orc, m, s):
+    b, g = H.countsketch_indices(m, s)
+    eb, eg = orc.countsketch_indices(m, s)
+    assert np.array_equal(b, eb) and np.array_equal(g, eg)
+
+
+@pytest.mark.parametrize("a_type,tier", [(C.F64, 0), (C.F32, 0), (C.FP32_BF16, 2), (C.FP32_BF16, 0),
+                                         (C.FP32_16, 2), (C.FP32_16, 0)])
+def test_apply_vs_oracle(capi, orc, a_type, tier):
+    m, n, s = 20011, 37, 300
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    if a_type in (C.FP32_BF16, C.FP32_16):
+        variant = C.PACK_FP32_BF16 if a_type == C.FP32_BF16 else C.PACK_FP32_16
+        wh, wr = H.pack(variant, A)
+        dec = {(C.FP32_BF16, 2): 4, (C.FP32_BF16, 0): 5, (C.FP32_16, 2): 0, (C.FP32_16, 0): 1}[(a_type, tier)]
+        eh, er = orc.pack(0 if a_type == C.FP32_16 else 3, A)
+        Aref = orc.unpack(dec, eh, er).astype(np.float64)
+        Y = H.countsketch_apply(a_type, tier, m, n, wh, wr, m, s)
+    else:
+        Ad = H.dev(A.astype(np.float64) if a_type == C.F64 else A)
+        Aref = A.astype(np.float64)
+        Y = H.countsketch_apply(a_type, tier, m, n, Ad, None, m, s)
+    ref = orc.countsketch_apply(Aref, s)
+    err = np.abs(Y - ref).max() / np.abs(ref).max()
+    assert err < 2e-6, err
+
+
+def test_norm_preservation(capi):
+    """E |Sx|^2 / |x|^2 ~ 1 (test_countsketch_cpu_kernels.cpp:93-179, 200 trials)."""
+    m, s, trials = 5000, 500, 200
+    rng = np.random.default_rng(42)
+    X = rng.standard_normal((m, trials))
+    Y = H.countsketch_apply(C.F64, 0, m, trials, H.dev(X), None, m, s)
+    ratio = (np.linalg.norm(Y, axis=0) ** 2 / np.linalg.norm(X, axis=0) ** 2).mean()
+    assert abs(ratio - 1.0) < 0.05, ratio
+    Y2 = H.countsketch_apply(C.F64, 0, m, trials, H.dev(2 * X + 1), None, m, s)
+    Y1 = H.countsketch_apply(C.F64, 0, m, trials, H.dev(np.ones_like(X)), None, m, s)
+    # values are FP32 sums within a row split, FP64 across splits
+    assert np.allclose(Y2, 2 * Y + Y1, rtol=0, atol=2e-5 * np.abs(Y2).max())
+
+
+@pytest.mark.parametrize("k,n", [(4000, 1000), (30, 5), (257, 256), (64, 64)])
+def test_qr_r_matches_numpy(capi, k, n):
+    rng = np.random.default_rng(k + n)
+    A = rng.standard_normal((k, n)) * np.logspace(0, -6, n)
+    R = H.geqrf_r(A)
+    Rn = np.linalg.qr(A, mode="r")
+    sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
+    assert np.allclose(np.tril(R, -1), 0)
+    err = np.abs(R - sgn[:, None] * Rn).max() / np.abs(Rn).max()
+    assert err < 1e-12, err
+
+
+def test_trtri_and_precond(capi):
+    rng = np.random.default_rng(3)
+    n = 1000
+    R = np.triu(rng.standard_normal((n, n))) + np.diag(np.full(n, 30.0))
+    Ri = H.trtri(R)
+    assert np.allclose(np.tril(Ri, -1), 0)
+    assert np.abs(R @ Ri - np.eye(n)).max() < 1e-10
+    A = rng.standard_normal((2000, 300))
+    rinv, _ = H.sketch_rinv(C.F64, 0, 2000, 300, H.dev(A), None, 2000, 900)
+    Rs = np.linalg.inv(rinv)
+    # R^-1 preconditions A: the condition number of A R^-1 is small
+    s = np.linalg.svd(A @ rinv, compute_uv=False)
+    assert s[0] / s[-1] < 10, s[0] / s[-1]
+    assert np.allclose(np.tril(Rs, -1), 0, atol=1e-9 * np.abs(Rs).max())
+
+
+def test_errors(capi):
+    import ctypes
+    A = H.dev(np.ones((3, 5)))
+    rc = C.lib().tomos_sketch_qr_precond_f64(3, 5, A.ptr, 3, None)
+    assert rc == 1  # k < n (sketch_qr.cpp:41-44)
+    Z = H.dev(np.zeros((4, 4)))
+    rc = C.lib().tomos_trtri_upper_f64(4, Z.ptr, 4, None)
+    assert rc == 6  # singular R (sketch_qr.cpp:59-63)

@@ -73,6 +73,7 @@ _SIGS = {
     "tomos_event_destroy": (cint, [vp]),
     "tomos_event_record": (cint, [vp, vp]),
     "tomos_event_elapsed_ms": (cint, [ctypes.POINTER(ctypes.c_float), vp, vp]),
+    "tomos_copy_kernel": (cint, [vp, vp, sz, vp]),
     "tomos_fill_normal_f32": (cint, [vp, i64, i64, i64, ctypes.c_uint64, vp, cint, cint, vp]),
     "tomos_pack": (cint, [cint, i64, i64, vp, i64, vp, vp, i64, vp]),
     "tomos_unpack": (cint, [cint, i64, i64, vp, vp, i64, vp, i64, vp]),
@@ -90,9 +91,10 @@ _SIGS = {
     "tomos_geqrf_r_f64": (cint, [i64, i64, vp, i64, vp, i64, vp, sz, vp]),
     "tomos_trtri_upper_f64": (cint, [i64, vp, i64, vp]),
     "tomos_sketch_qr_precond_f64": (cint, [i64, i64, vp, i64, vp]),
-    "tomos_lsqr_workspace_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
+    "tomos_lsqr_arena_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
+    "tomos_lsqr_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
     "tomos_lsqr_solve": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, vp, vp, vp,
-                                i64, vp, sz, ctypes.POINTER(cint), vp, cint,
+                                i64, vp, sz, vp, sz, ctypes.POINTER(cint), vp, cint,
                                 ctypes.POINTER(cint), vp]),
 }
 

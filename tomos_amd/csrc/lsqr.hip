@@ -191,25 +191,47 @@ __global__ void k_iter_check(LsqrState* s, double* hist, int cap, double tol, do
     if (rr < tol || rr < maxrr) s->done = 1;
 }
 
+// Two buffers, as the reference's Workspace: the arena holds the Krylov
+// vectors carved in the reference's order (u, v, w, temp1 in value_type,
+// then precond_v in precond_type, each offset rounded up to its element
+// size; lsqr_registries.cpp:23-77, sized as create_lsqr_workspace,
+// lsqr.cpp:389-406), the scratch holds this implementation's device state,
+// history, reduction partials and GEMV partial sums.
 struct Layout {
-    size_t u, v, w, t, pv, state, hist, red, scratch, total;
-    size_t scratch_bytes;
+    size_t u, v, w, t, pv, arena_total;
+    size_t state, hist, red, scratch, scratch_total;
+    size_t gemv_bytes;
     int hist_cap;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 size_t tsize(int t) { return t == TOMOS_F64 ? 8 : 4; }
+size_t carve(size_t& off, size_t elem, size_t count) {
+    off = (off + elem - 1) / elem * elem;
+    const size_t at = off;
+    off += count * elem;
+    return at;
+}
+
+size_t arena_formula(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+    const size_t vs = tsize(o.value_type);
+    const size_t ps = o.precond_type >= 0 ? tsize(o.precond_type) : vs;
+    return static_cast<size_t>(m) * vs + 3 * static_cast<size_t>(n) * vs + (ps - 1) +
+           static_cast<size_t>(n) * ps;
+}
 
 Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     Layout L;
     const size_t vs = tsize(o.value_type);
     const size_t ps = o.precond_type >= 0 ? tsize(o.precond_type) : vs;
     size_t off = 0;
-    L.u = off; off = align256(off + static_cast<size_t>(m) * vs);
-    L.v = off; off = align256(off + static_cast<size_t>(n) * vs);
-    L.w = off; off = align256(off + static_cast<size_t>(n) * vs);
-    L.t = off; off = align256(off + static_cast<size_t>(n) * vs);
-    L.pv = off; off = align256(off + static_cast<size_t>(n) * ps);
+    L.u = carve(off, vs, static_cast<size_t>(m));
+    L.v = carve(off, vs, static_cast<size_t>(n));
+    L.w = carve(off, vs, static_cast<size_t>(n));
+    L.t = carve(off, vs, static_cast<size_t>(n));
+    L.pv = carve(off, ps, static_cast<size_t>(n));
+    L.arena_total = off;
+    off = 0;
     L.state = off; off = align256(off + sizeof(LsqrState));
     const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
     L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 26));
@@ -224,9 +246,9 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
         sc = std::max(sc, gemv_scratch_bytes(o.precond_type, TOMOS_TIER_DEFAULT, TOMOS_TRANSPOSE, n, n));
     }
     L.scratch = off;
-    L.scratch_bytes = sc;
+    L.gemv_bytes = sc;
     off = align256(off + sc);
-    L.total = off;
+    L.scratch_total = off;
     return L;
 }
 
@@ -256,20 +278,21 @@ struct Solver {
     void* x;
     const void* rinv;
     int64_t ldr;
-    char* ws;
+    char* arena;
+    char* sc;
     Layout L;
     hipStream_t s;
     bool v64, r64, p64;
 
-    LsqrState* state() const { return reinterpret_cast<LsqrState*>(ws + L.state); }
-    void* U() const { return ws + L.u; }
-    void* V() const { return ws + L.v; }
-    void* W() const { return ws + L.w; }
-    void* Tm() const { return ws + L.t; }
-    void* PV() const { return ws + L.pv; }
-    double* hist() const { return reinterpret_cast<double*>(ws + L.hist); }
-    double* red() const { return reinterpret_cast<double*>(ws + L.red); }
-    void* scratch() const { return ws + L.scratch; }
+    LsqrState* state() const { return reinterpret_cast<LsqrState*>(sc + L.state); }
+    void* U() const { return arena + L.u; }
+    void* V() const { return arena + L.v; }
+    void* W() const { return arena + L.w; }
+    void* Tm() const { return arena + L.t; }
+    void* PV() const { return arena + L.pv; }
+    double* hist() const { return reinterpret_cast<double*>(sc + L.hist); }
+    double* red() const { return reinterpret_cast<double*>(sc + L.red); }
+    void* scratch() const { return sc + L.scratch; }
 
     void norm(const void* vec, bool f64, int64_t len, double* out, bool guarded_by_state = true) {
         const unsigned g = grid_for(len);
@@ -287,13 +310,13 @@ struct Solver {
     // A-matrix GEMV at a tier
     void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64) {
         gemv(o.a_type, tier, trans, c, m, n, a, wr, lda, x64 ? TOMOS_F64 : TOMOS_F32, xin,
-             y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.scratch_bytes, s);
+             y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.gemv_bytes, s);
     }
     // P = R^-1 GEMV
     void gemv_p(int trans, const void* xin, bool x64, void* yout, bool y64) {
         gemv(o.precond_type, TOMOS_TIER_DEFAULT, trans, coef(1.0, 0.0), n, n, rinv, nullptr, ldr,
              x64 ? TOMOS_F64 : TOMOS_F32, xin, y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(),
-             L.scratch_bytes, s);
+             L.gemv_bytes, s);
     }
     void scale_inv(void* vec, bool f64, int64_t len, const double* den) {
         k_scale_inv<<<grid_for(len), kT, 0, s>>>(state(), vec, f64, len, den);
@@ -403,15 +426,20 @@ struct Solver {
 
 }  // namespace
 
-size_t lsqr_workspace_bytes(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+size_t lsqr_arena_bytes(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     validate(o, m, n);
-    return layout_for(o, m, n).total;
+    return std::max(arena_formula(o, m, n), layout_for(o, m, n).arena_total);
+}
+
+size_t lsqr_scratch_bytes(const tomos_lsqr_options& o, int64_t m, int64_t n) {
+    validate(o, m, n);
+    return layout_for(o, m, n).scratch_total;
 }
 
 void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a, const uint16_t* wr,
-                int64_t lda, const void* b, void* x, const void* rinv, int64_t ldr, void* ws,
-                size_t ws_bytes, int* iterations, double* history, int history_cap,
-                int* history_len, hipStream_t s) {
+                int64_t lda, const void* b, void* x, const void* rinv, int64_t ldr, void* arena,
+                size_t arena_bytes, void* scratch, size_t scratch_bytes, int* iterations,
+                double* history, int history_cap, int* history_len, hipStream_t s) {
     validate(o, m, n);
     require(lda >= std::max<int64_t>(m, 1), "lsqr: lda < m");
     if (o.precond_type >= 0) {
@@ -439,11 +467,17 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.rinv = rinv;
     sv.ldr = ldr;
     sv.L = layout_for(o, m, n);
-    if (ws == nullptr) ws = fallback_scratch(sv.L.total);
-    else if (ws_bytes < sv.L.total)
-        throw Error(kErrWorkspace, "lsqr: workspace too small (need " + std::to_string(sv.L.total) +
-                                       " bytes, have " + std::to_string(ws_bytes) + ")");
-    sv.ws = static_cast<char*>(ws);
+    // lsqr_registries.cpp:59-63: the arena must hold the carved vectors
+    if (arena == nullptr || arena_bytes < sv.L.arena_total)
+        throw Error(kErrWorkspace, "lsqr: workspace is too small for LSQR's working vectors (need " +
+                                       std::to_string(sv.L.arena_total) + " bytes, have " +
+                                       std::to_string(arena ? arena_bytes : 0) + ")");
+    if (scratch == nullptr) scratch = fallback_scratch(sv.L.scratch_total);
+    else if (scratch_bytes < sv.L.scratch_total)
+        throw Error(kErrWorkspace, "lsqr: scratch too small (need " + std::to_string(sv.L.scratch_total) +
+                                       " bytes, have " + std::to_string(scratch_bytes) + ")");
+    sv.arena = static_cast<char*>(arena);
+    sv.sc = static_cast<char*>(scratch);
     sv.s = s;
     sv.v64 = o.value_type == TOMOS_F64;
     sv.r64 = o.refine_type == TOMOS_F64;
@@ -457,23 +491,33 @@ using namespace tomos_hip;
 
 extern "C" {
 
-size_t tomos_lsqr_workspace_size(const tomos_lsqr_options* o, int64_t m, int64_t n) {
+size_t tomos_lsqr_arena_size(const tomos_lsqr_options* o, int64_t m, int64_t n) {
     size_t out = 0;
     int rc = guarded([&] {
-        require(o != nullptr, "tomos_lsqr_workspace_size: null options");
-        out = lsqr_workspace_bytes(*o, m, n);
+        require(o != nullptr, "tomos_lsqr_arena_size: null options");
+        out = lsqr_arena_bytes(*o, m, n);
+    });
+    return rc == 0 ? out : 0;
+}
+
+size_t tomos_lsqr_scratch_size(const tomos_lsqr_options* o, int64_t m, int64_t n) {
+    size_t out = 0;
+    int rc = guarded([&] {
+        require(o != nullptr, "tomos_lsqr_scratch_size: null options");
+        out = lsqr_scratch_bytes(*o, m, n);
     });
     return rc == 0 ? out : 0;
 }
 
 int tomos_lsqr_solve(const tomos_lsqr_options* o, int64_t m, int64_t n, const void* a,
                      const uint16_t* wr, int64_t lda, const void* b, void* x, const void* rinv,
-                     int64_t ldr, void* ws, size_t ws_bytes, int* iterations, double* history,
-                     int history_cap, int* history_len, tomos_stream_t s) {
+                     int64_t ldr, void* arena, size_t arena_bytes, void* scratch,
+                     size_t scratch_bytes, int* iterations, double* history, int history_cap,
+                     int* history_len, tomos_stream_t s) {
     return guarded([&] {
         require(o != nullptr, "tomos_lsqr_solve: null options");
-        lsqr_solve(*o, m, n, a, wr, lda, b, x, rinv, ldr, ws, ws_bytes, iterations, history,
-                   history_cap, history_len, s);
+        lsqr_solve(*o, m, n, a, wr, lda, b, x, rinv, ldr, arena, arena_bytes, scratch,
+                   scratch_bytes, iterations, history, history_cap, history_len, s);
     });
 }
 

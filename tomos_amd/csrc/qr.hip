@@ -72,9 +72,11 @@ __global__ __launch_bounds__(kT) void qr_step_kernel(int64_t k, int64_t n, doubl
     if (!R.active) return;
     for (int64_t j = c + 1 + blockIdx.x; j < n; j += gridDim.x) {
         double* cj = a + j * lda;
+        // read A(c,j) before the reduction's barriers: thread 0 rewrites it below
+        const double xc = cj[c];
         double part = 0.0;
         for (int64_t i = c + 1 + threadIdx.x; i < k; i += kT) part += (col[i] * R.inv_v0) * cj[i];
-        const double w = R.tau * (cj[c] + block_sum(part, red));
+        const double w = R.tau * (xc + block_sum(part, red));
         for (int64_t i = c + 1 + threadIdx.x; i < k; i += kT) cj[i] -= w * (col[i] * R.inv_v0);
         if (threadIdx.x == 0) cj[c] -= w;
         __syncthreads();

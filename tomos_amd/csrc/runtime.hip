@@ -1,6 +1,7 @@
 // Runtime helpers behind the C ABI: last-error storage, device queries,
 // memory/stream/event wrappers, the fallback scratch cache, and the
 // deterministic synthetic-data generator used by benchmarks.
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -94,6 +95,18 @@ __global__ void fill_normal_kernel(float* __restrict__ out, int64_t m, int64_t n
         if (col_scale) z *= col_scale[j];
         out[i + j * ld] = z;
     }
+}
+
+__global__ __launch_bounds__(256) void copy16_kernel(uint4* __restrict__ dst,
+                                                    const uint4* __restrict__ src, int64_t n16) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+__global__ void copy_tail_kernel(char* __restrict__ dst, const char* __restrict__ src, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 }  // namespace tomos_hip
@@ -193,6 +206,28 @@ int tomos_event_elapsed_ms(float* ms, tomos_event_t a, tomos_event_t b) {
     return guarded([&] {
         TOMOS_HIP_CHECK(hipEventSynchronize(b));
         TOMOS_HIP_CHECK(hipEventElapsedTime(ms, a, b));
+    });
+}
+
+int tomos_copy_kernel(void* dst, const void* src, size_t bytes, tomos_stream_t s) {
+    return guarded([&] {
+        if (bytes == 0) return;
+        require(dst && src, "tomos_copy_kernel: null pointer");
+        const bool aligned = reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                             reinterpret_cast<uintptr_t>(src) % 16 == 0;
+        int64_t n16 = aligned ? static_cast<int64_t>(bytes / 16) : 0;
+        if (n16 > 0) {
+            const int64_t blocks = std::min<int64_t>(ceil_div(n16, 256), int64_t(cu_count()) * 16);
+            copy16_kernel<<<dim3(static_cast<unsigned>(blocks)), 256, 0, s>>>(
+                static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
+            TOMOS_LAUNCH_CHECK("copy16_kernel");
+        }
+        const int64_t done = n16 * 16, rest = static_cast<int64_t>(bytes) - done;
+        if (rest > 0) {
+            copy_tail_kernel<<<dim3(static_cast<unsigned>(ceil_div(rest, 256))), 256, 0, s>>>(
+                static_cast<char*>(dst) + done, static_cast<const char*>(src) + done, rest);
+            TOMOS_LAUNCH_CHECK("copy_tail_kernel");
+        }
     });
 }
 

@@ -1,0 +1,608 @@
+// BLAS front-ends, sketch/preconditioner, LSQR and the dispatch registries,
+// calling the gfx950 kernels through the libtomos_hip C ABI.
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tomos/api.hpp"
+#include "tomos_hip.h"
+
+namespace tomos {
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc == TOMOS_OK) return;
+    const std::string msg = std::string(what) + ": " + tomos_last_error();
+    if (rc == TOMOS_ERR_INVALID_ARGUMENT || rc == TOMOS_ERR_WORKSPACE) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+int abi_type(NumericType t) { return static_cast<int>(t); }
+
+bool is_packed(NumericType t) { return t == NumericType::FP32_16 || t == NumericType::FP32_BF16; }
+
+// Compute tier selected by a compute NumericType for storage type A
+// (dense_registries.cpp FP32_16 delegation, blas.cpp:33-54).
+int tier_for(NumericType a, NumericType compute) {
+    if (is_packed(a)) {
+        if (compute == NumericType::FP32) return TOMOS_TIER_EXACT;
+        return TOMOS_TIER_DEFAULT;  // FP16 (FP32_16) / BF16 (FP32_BF16)
+    }
+    if (a == NumericType::FP32 && compute == NumericType::FP64) return TOMOS_TIER_EXACT;
+    return TOMOS_TIER_DEFAULT;
+}
+
+NumericType default_compute(NumericType a) {
+    if (a == NumericType::FP32_16) return NumericType::FP16;
+    if (a == NumericType::FP32_BF16) return NumericType::BF16;
+    return a;
+}
+
+std::size_t vec_len(tomos::Dense& v) { return v.get_size()[0] * v.get_size()[1]; }
+
+void gemv_call(int tier, TransposeType trans, double alpha, tomos::Dense& A, tomos::Dense& x,
+               double beta, tomos::Dense& y, float* workspace) {
+    const auto sz = A.get_size();
+    const int64_t m = static_cast<int64_t>(sz[0]), n = static_cast<int64_t>(sz[1]);
+    const bool t = trans == TransposeType::TRANSPOSE;
+    if (vec_len(x) != static_cast<std::size_t>(t ? m : n) || vec_len(y) != static_cast<std::size_t>(t ? n : m))
+        throw std::invalid_argument("mv_apply: operand sizes do not match A");
+    const void* a = is_packed(A.get_value_type()) ? static_cast<const void*>(A.wh()) : A.data();
+    const uint16_t* wr = is_packed(A.get_value_type()) ? A.wr() : nullptr;
+    const size_t need = tomos_gemv_scratch_size(abi_type(A.get_value_type()), tier, t ? 1 : 0, m, n);
+    check(tomos_gemv(abi_type(A.get_value_type()), tier, t ? 1 : 0, alpha, m, n, a, wr,
+                     static_cast<int64_t>(A.get_lead_dim()), abi_type(x.get_value_type()), x.data(),
+                     beta, abi_type(y.get_value_type()), y.data(), workspace,
+                     workspace ? need : 0, A.stream()),
+          "mv_apply");
+}
+
+template <int TIER>
+void mv_impl(TransposeType trans, double alpha, tomos::Dense& A, tomos::Dense& x, double beta,
+             tomos::Dense& y, float* workspace) {
+    gemv_call(TIER, trans, alpha, A, x, beta, y, workspace);
+}
+
+void gemm_call(int tier, TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A,
+               tomos::Dense& B, double beta, tomos::Dense& C) {
+    if (B.get_value_type() != NumericType::FP32 || C.get_value_type() != NumericType::FP32)
+        throw std::invalid_argument("mm_apply: B and C must be FP32");
+    const bool tA = ta == TransposeType::TRANSPOSE, tB = tb == TransposeType::TRANSPOSE;
+    const auto sa = A.get_size(), sb = B.get_size(), sc = C.get_size();
+    const std::size_t M = tA ? sa[1] : sa[0], K = tA ? sa[0] : sa[1];
+    const std::size_t N = tB ? sb[0] : sb[1];
+    if ((tB ? sb[1] : sb[0]) != K || sc[0] != M || sc[1] != N)
+        throw std::invalid_argument("mm_apply: operand sizes do not match");
+    const void* a = is_packed(A.get_value_type()) ? static_cast<const void*>(A.wh()) : A.data();
+    const uint16_t* wr = is_packed(A.get_value_type()) ? A.wr() : nullptr;
+    check(tomos_gemm(abi_type(A.get_value_type()), tier, tA ? 1 : 0, tB ? 1 : 0, alpha,
+                     static_cast<int64_t>(M), static_cast<int64_t>(N), static_cast<int64_t>(K), a, wr,
+                     static_cast<int64_t>(A.get_lead_dim()), static_cast<const float*>(B.data()),
+                     static_cast<int64_t>(B.get_lead_dim()), beta, static_cast<float*>(C.data()),
+                     static_cast<int64_t>(C.get_lead_dim()), A.stream()),
+          "mm_apply");
+}
+
+template <int TIER>
+void mm_impl(TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A, tomos::Dense& B,
+             double beta, tomos::Dense& C) {
+    gemm_call(TIER, ta, tb, alpha, A, B, beta, C);
+}
+
+// "workspace" tier: decode A exactly into an FP32 scratch matrix, then the
+// FP32 GEMM (fp32_16_registries.cpp:443-453 materialize tier).
+void mm_materialize(TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A,
+                    tomos::Dense& B, double beta, tomos::Dense& C) {
+    Context ctx(A.get_device(), A.stream());
+    const auto sa = A.get_size();
+    tomos::Dense F = tomos::Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, sa);
+    const int dec = A.get_value_type() == NumericType::FP32_16 ? TOMOS_DECODE_FP32_16_EXACT
+                                                               : TOMOS_DECODE_FP32_BF16_EXACT;
+    check(tomos_unpack(dec, static_cast<int64_t>(sa[0]), static_cast<int64_t>(sa[1]), A.wh(), A.wr(),
+                       static_cast<int64_t>(A.get_lead_dim()), static_cast<float*>(F.data()),
+                       static_cast<int64_t>(sa[0]), A.stream()),
+          "mm_apply (materialize)");
+    gemm_call(TOMOS_TIER_DEFAULT, ta, tb, alpha, F, B, beta, C);
+    check(tomos_stream_synchronize(A.stream()), "mm_apply (materialize)");
+}
+
+template <int VARIANT>
+void split_impl(tomos::Dense& source, tomos::Dense& target) {
+    if (source.get_value_type() != NumericType::FP32) throw std::invalid_argument("split: source must be FP32");
+    const auto s = source.get_size();
+    if (target.get_size() != s) throw std::invalid_argument("split: size mismatch");
+    check(tomos_pack(VARIANT, static_cast<int64_t>(s[0]), static_cast<int64_t>(s[1]),
+                     static_cast<const float*>(source.data()), static_cast<int64_t>(source.get_lead_dim()),
+                     target.wh(), target.wr(), static_cast<int64_t>(target.get_lead_dim()),
+                     target.stream()),
+          "split");
+}
+
+template <int DEC>
+void combine_impl(tomos::Dense& source, tomos::Dense& target) {
+    if (target.get_value_type() != NumericType::FP32) throw std::invalid_argument("combine: target must be FP32");
+    const auto s = target.get_size();
+    if (source.get_size() != s) throw std::invalid_argument("combine: size mismatch");
+    check(tomos_unpack(DEC, static_cast<int64_t>(s[0]), static_cast<int64_t>(s[1]), source.wh(),
+                       source.wr(), static_cast<int64_t>(source.get_lead_dim()),
+                       static_cast<float*>(target.data()), static_cast<int64_t>(target.get_lead_dim()),
+                       source.stream()),
+          "combine");
+}
+
+void device_guard(Device dev, const char* what) {
+    if (dev == Device::OMP) throw std::invalid_argument(std::string(what) + ": Device::OMP is not supported.");
+}
+
+}  // namespace
+
+// ============================================================ registries
+namespace dispatch {
+
+std::size_t Dense::encode_mv(Device dev, NumericType A, NumericType x, NumericType y, NumericType c) {
+    const std::size_t T = num_registry_numeric_types;
+    return static_cast<std::size_t>(dev) * T * T * T * T + static_cast<std::size_t>(A) * T * T * T +
+           static_cast<std::size_t>(x) * T * T + static_cast<std::size_t>(y) * T +
+           static_cast<std::size_t>(c);
+}
+
+std::size_t Dense::encode_mm(Device dev, NumericType A, NumericType B, NumericType C, bool ws) {
+    const std::size_t T = num_registry_numeric_types;
+    return static_cast<std::size_t>(dev) * T * T * T * 2 + static_cast<std::size_t>(A) * T * T * 2 +
+           static_cast<std::size_t>(B) * T * 2 + static_cast<std::size_t>(C) * 2 + (ws ? 1 : 0);
+}
+
+Dense::Dense() {
+    mv_.fill(nullptr);
+    mm_.fill(nullptr);
+    const Device d = Device::HIP;
+    const NumericType vt[] = {NumericType::FP32, NumericType::FP64};
+    for (NumericType x : vt)
+        for (NumericType y : vt) {
+            mv_[encode_mv(d, NumericType::FP32, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::FP32, x, y, NumericType::FP64)] = &mv_impl<TOMOS_TIER_EXACT>;
+            mv_[encode_mv(d, NumericType::FP64, x, y, NumericType::FP64)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::FP32_16, x, y, NumericType::FP16)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::FP32_16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_EXACT>;
+            mv_[encode_mv(d, NumericType::FP32_BF16, x, y, NumericType::BF16)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::FP32_BF16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_EXACT>;
+        }
+    const NumericType f = NumericType::FP32;
+    mm_[encode_mm(d, NumericType::FP32, f, f, false)] = &mm_impl<TOMOS_TIER_DEFAULT>;
+    mm_[encode_mm(d, NumericType::FP64, f, f, false)] = &mm_impl<TOMOS_TIER_DEFAULT>;
+    mm_[encode_mm(d, NumericType::FP32_16, f, f, false)] = &mm_impl<TOMOS_TIER_EXACT>;
+    mm_[encode_mm(d, NumericType::FP32_16, f, f, true)] = &mm_materialize;
+    mm_[encode_mm(d, NumericType::FP32_BF16, f, f, false)] = &mm_impl<TOMOS_TIER_EXACT>;
+    mm_[encode_mm(d, NumericType::FP32_BF16, f, f, true)] = &mm_materialize;
+}
+
+const Dense& Dense::get() {
+    static const Dense instance;
+    return instance;
+}
+
+mv_dense_fn Dense::lookup_mv(Device dev, NumericType A, NumericType x, NumericType y,
+                             NumericType c) const {
+    device_guard(dev, "lookup_mv");
+    auto fn = mv_[encode_mv(dev, A, x, y, c)];
+    if (!fn)
+        throw std::runtime_error(std::string("No dense mv kernel registered for device/A/x/y/compute = ") +
+                                 std::to_string(static_cast<int>(dev)) + "/" + numeric_type_name(A) + "/" +
+                                 numeric_type_name(x) + "/" + numeric_type_name(y) + "/" +
+                                 numeric_type_name(c));
+    return fn;
+}
+
+mm_dense_fn Dense::lookup_mm(Device dev, NumericType A, NumericType B, NumericType C, bool ws) const {
+    device_guard(dev, "lookup_mm");
+    auto fn = mm_[encode_mm(dev, A, B, C, ws)];
+    if (!fn) throw std::runtime_error("No dense mm kernel registered for the requested combination.");
+    return fn;
+}
+
+#define TOMOS_SLOT(arr, name)                                                                  \
+    if (dev == Device::OMP) throw std::invalid_argument(#name ": Device::OMP is not supported."); \
+    auto fn = arr[static_cast<std::size_t>(dev)];                                              \
+    if (!fn) throw std::runtime_error(#name ": nothing registered for this device.");          \
+    return fn;
+
+Fp32_16::Fp32_16() {
+    const std::size_t h = static_cast<std::size_t>(Device::HIP);
+    split_[h] = &split_impl<TOMOS_PACK_FP32_16>;
+    combine_[h] = &combine_impl<TOMOS_DECODE_FP32_16_EXACT>;
+    mv_pure_half_[h] = &mv_impl<TOMOS_TIER_DEFAULT>;
+    mv_fast_[h] = &mv_impl<TOMOS_TIER_FAST>;
+    mv_accurate_[h] = &mv_impl<TOMOS_TIER_EXACT>;
+    mm_pure_half_[h] = &mm_impl<TOMOS_TIER_DEFAULT>;
+    mm_fast_[h] = &mm_impl<TOMOS_TIER_FAST>;
+    mm_accurate_[h] = &mm_impl<TOMOS_TIER_EXACT>;
+    mm_workspace_[h] = &mm_materialize;
+}
+const Fp32_16& Fp32_16::get() {
+    static const Fp32_16 instance;
+    return instance;
+}
+split_fn Fp32_16::lookup_split(Device dev) const { TOMOS_SLOT(split_, lookup_split) }
+combine_fn Fp32_16::lookup_combine(Device dev) const { TOMOS_SLOT(combine_, lookup_combine) }
+mv_dense_fn Fp32_16::lookup_mv_pure_half(Device dev) const { TOMOS_SLOT(mv_pure_half_, lookup_mv_pure_half) }
+mv_dense_fn Fp32_16::lookup_mv_fast(Device dev) const { TOMOS_SLOT(mv_fast_, lookup_mv_fast) }
+mv_dense_fn Fp32_16::lookup_mv_accurate(Device dev) const { TOMOS_SLOT(mv_accurate_, lookup_mv_accurate) }
+mm_dense_fn Fp32_16::lookup_mm_pure_half(Device dev) const { TOMOS_SLOT(mm_pure_half_, lookup_mm_pure_half) }
+mm_dense_fn Fp32_16::lookup_mm_fast(Device dev) const { TOMOS_SLOT(mm_fast_, lookup_mm_fast) }
+mm_dense_fn Fp32_16::lookup_mm_accurate(Device dev) const { TOMOS_SLOT(mm_accurate_, lookup_mm_accurate) }
+mm_dense_fn Fp32_16::lookup_mm_workspace(Device dev) const { TOMOS_SLOT(mm_workspace_, lookup_mm_workspace) }
+
+Fp32Bf16::Fp32Bf16() {
+    const std::size_t h = static_cast<std::size_t>(Device::HIP);
+    split_[h] = &split_impl<TOMOS_PACK_FP32_BF16>;
+    split_rounded_[h] = &split_impl<TOMOS_PACK_FP32_BF16_ROUNDED>;
+    combine_[h] = &combine_impl<TOMOS_DECODE_FP32_BF16_EXACT>;
+    combine_rounded_[h] = &combine_impl<TOMOS_DECODE_FP32_BF16_EXACT_ROUNDED>;
+    mv_hi_[h] = &mv_impl<TOMOS_TIER_DEFAULT>;
+    mv_accurate_[h] = &mv_impl<TOMOS_TIER_EXACT>;
+    mm_hi_[h] = &mm_impl<TOMOS_TIER_DEFAULT>;
+    mm_accurate_[h] = &mm_impl<TOMOS_TIER_EXACT>;
+}
+const Fp32Bf16& Fp32Bf16::get() {
+    static const Fp32Bf16 instance;
+    return instance;
+}
+split_fn Fp32Bf16::lookup_split(Device dev) const { TOMOS_SLOT(split_, lookup_split) }
+split_fn Fp32Bf16::lookup_split_rounded(Device dev) const { TOMOS_SLOT(split_rounded_, lookup_split_rounded) }
+combine_fn Fp32Bf16::lookup_combine(Device dev) const { TOMOS_SLOT(combine_, lookup_combine) }
+combine_fn Fp32Bf16::lookup_combine_rounded(Device dev) const { TOMOS_SLOT(combine_rounded_, lookup_combine_rounded) }
+mv_dense_fn Fp32Bf16::lookup_mv_hi(Device dev) const { TOMOS_SLOT(mv_hi_, lookup_mv_hi) }
+mv_dense_fn Fp32Bf16::lookup_mv_accurate(Device dev) const { TOMOS_SLOT(mv_accurate_, lookup_mv_accurate) }
+mm_dense_fn Fp32Bf16::lookup_mm_hi(Device dev) const { TOMOS_SLOT(mm_hi_, lookup_mm_hi) }
+mm_dense_fn Fp32Bf16::lookup_mm_accurate(Device dev) const { TOMOS_SLOT(mm_accurate_, lookup_mm_accurate) }
+
+#undef TOMOS_SLOT
+
+}  // namespace dispatch
+
+// ======================================================================
+// LSQR: lsqr_dense_impl (src/dispatch/lsqr_registries.cpp:88-159)
+// ======================================================================
+namespace {
+
+template <NumericType MV, NumericType VAL, NumericType REF, NumericType PRE>
+void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::Dense& x,
+                     Workspace& ws, double tol, int max_iters, double max_relres, int* iterations,
+                     std::vector<double>* history, int restarts, tomos::Dense* mtx_refine) {
+    if (ws.device_ != context.get_device())
+        throw std::invalid_argument("lsqr_solve: workspace and context must be on the same device.");
+    if (mtx_refine)
+        throw std::invalid_argument(
+            "lsqr_solve: an explicit refine-precision duplicate of A is not supported; store A "
+            "packed and set Workspace::use_implicit_duplicate_ instead.");
+    const auto sz = A.get_size();
+    const int64_t m = static_cast<int64_t>(sz[0]), n = static_cast<int64_t>(sz[1]);
+    if (vec_len(b) != sz[0] || vec_len(x) != sz[1])
+        throw std::invalid_argument("lsqr_solve: b/x sizes do not match A");
+    if (ws.use_implicit_duplicate_ && !is_packed(MV))
+        throw std::invalid_argument(
+            "lsqr_solve: Workspace::use_implicit_duplicate_ requires A to be stored packed "
+            "(FP32_16 or FP32_BF16).");
+    const NumericType compute = ws.mv_compute_type_set_ ? ws.mv_compute_type_ : default_compute(MV);
+    // the compute type must be one the dense registry knows for A
+    (void)dispatch::Dense::get().lookup_mv(context.get_device(), MV, VAL, VAL, compute);
+    const bool pre = ws.precond_kind_ == PrecondKind::Dense;
+    tomos::Dense* P = pre ? static_cast<tomos::Dense*>(ws.precond_) : nullptr;
+
+    tomos_lsqr_options o;
+    o.a_type = abi_type(MV);
+    o.mv_tier = tier_for(MV, compute);
+    o.restart_tier = ws.use_implicit_duplicate_ ? TOMOS_TIER_EXACT : o.mv_tier;
+    o.value_type = abi_type(VAL);
+    o.refine_type = abi_type(REF);
+    o.precond_type = pre ? abi_type(PRE) : -1;
+    o.tol = tol;
+    o.max_relres = max_relres;
+    o.max_iters = max_iters;
+    o.restarts = restarts;
+    o.check_every = std::max(1, ws.check_every_);
+    o.fused = 0;
+    const size_t arena = tomos_lsqr_arena_size(&o, m, n);
+    const size_t scratch = tomos_lsqr_scratch_size(&o, m, n);
+    if (arena == 0 || scratch == 0) check(TOMOS_ERR_INVALID_ARGUMENT, "lsqr_solve");
+    if (arena > ws.capacity_)
+        throw std::invalid_argument("lsqr_solve: workspace is too small for LSQR's working vectors (need " +
+                                    std::to_string(arena) + " bytes, have " +
+                                    std::to_string(ws.capacity_) + ").");
+    // the only allocation on the solve path, once per shape (lsqr_registries.cpp:132-145)
+    ws.reserve_scratch(scratch);
+    const int cap = std::max(0, restarts) * (std::max(0, max_iters) + 1) + 1;
+    std::vector<double> hist(history ? static_cast<std::size_t>(cap) : 0);
+    int it = 0, hl = 0;
+    const void* a = is_packed(MV) ? static_cast<const void*>(A.wh()) : A.data();
+    const uint16_t* wr = is_packed(MV) ? A.wr() : nullptr;
+    check(tomos_lsqr_solve(&o, m, n, a, wr, static_cast<int64_t>(A.get_lead_dim()), b.data(), x.data(),
+                           P ? P->data() : nullptr, P ? static_cast<int64_t>(P->get_lead_dim()) : 1,
+                           ws.data_, ws.capacity_, ws.mv_transpose_scratch_,
+                           ws.mv_transpose_scratch_capacity_, &it, history ? hist.data() : nullptr,
+                           history ? cap : 0, &hl, context.get_stream()),
+          "lsqr_solve");
+    if (iterations) *iterations = it;
+    if (history) history->insert(history->end(), hist.begin(), hist.begin() + std::min(hl, cap));
+}
+
+}  // namespace
+
+namespace dispatch {
+
+std::size_t Lsqr::encode(Device dev, NumericType mv, NumericType val, NumericType ref, NumericType pre) {
+    const std::size_t T = num_registry_numeric_types;
+    return static_cast<std::size_t>(dev) * T * T * T * T + static_cast<std::size_t>(mv) * T * T * T +
+           static_cast<std::size_t>(val) * T * T + static_cast<std::size_t>(ref) * T +
+           static_cast<std::size_t>(pre);
+}
+
+namespace {
+template <NumericType MV, NumericType VAL, NumericType REF>
+void register_pre(std::array<lsqr_dense_fn, num_registry_devices * num_registry_numeric_types *
+                                                num_registry_numeric_types * num_registry_numeric_types *
+                                                num_registry_numeric_types>& t) {
+    t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP32)] =
+        &lsqr_dense_impl<MV, VAL, REF, NumericType::FP32>;
+    t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP64)] =
+        &lsqr_dense_impl<MV, VAL, REF, NumericType::FP64>;
+}
+template <NumericType MV, NumericType VAL, class Arr>
+void register_ref(Arr& t) {
+    register_pre<MV, VAL, NumericType::FP32>(t);
+    register_pre<MV, VAL, NumericType::FP64>(t);
+}
+template <NumericType MV, class Arr>
+void register_val(Arr& t) {
+    register_ref<MV, NumericType::FP32>(t);
+    register_ref<MV, NumericType::FP64>(t);
+}
+}  // namespace
+
+Lsqr::Lsqr() {
+    dense_.fill(nullptr);
+    register_val<NumericType::FP32>(dense_);
+    register_val<NumericType::FP64>(dense_);
+    register_val<NumericType::FP32_16>(dense_);
+    register_val<NumericType::FP32_BF16>(dense_);
+}
+
+const Lsqr& Lsqr::get() {
+    static const Lsqr instance;
+    return instance;
+}
+
+lsqr_dense_fn Lsqr::lookup_solve_dense(Device dev, NumericType mv, NumericType val, NumericType ref,
+                                       NumericType pre) const {
+    if (dev == Device::OMP) throw std::invalid_argument("lookup_solve_dense: Device::OMP is not supported.");
+    auto fn = dense_[encode(dev, mv, val, ref, pre)];
+    if (!fn)
+        throw std::runtime_error(
+            "No LSQR dense kernel registered for the requested "
+            "device/mv_type/value_type/refine_type/precond_type combination.");
+    return fn;
+}
+
+}  // namespace dispatch
+
+// ======================================================================
+// Sketch + preconditioner
+// ======================================================================
+namespace {
+void cs_apply_impl(CountSketch& S, tomos::Dense& source, tomos::Dense& target) {
+    if (target.get_value_type() != NumericType::FP64)
+        throw std::invalid_argument("apply_count_sketch: target must be FP64");
+    const auto ss = source.get_size(), ts = target.get_size();
+    if (S.size[1] != ss[0] || ts[0] != S.size[0] || ts[1] != ss[1])
+        throw std::invalid_argument("apply_count_sketch: size mismatch");
+    const NumericType t = source.get_value_type();
+    const void* a = is_packed(t) ? static_cast<const void*>(source.wh()) : source.data();
+    const uint16_t* wr = is_packed(t) ? source.wr() : nullptr;
+    check(tomos_countsketch_apply(abi_type(t), is_packed(t) ? TOMOS_TIER_EXACT : TOMOS_TIER_DEFAULT,
+                                  static_cast<int64_t>(ss[0]), static_cast<int64_t>(ss[1]), a, wr,
+                                  static_cast<int64_t>(source.get_lead_dim()), static_cast<int64_t>(S.size[0]),
+                                  static_cast<double*>(target.data()),
+                                  static_cast<int64_t>(target.get_lead_dim()), nullptr, 0, source.stream()),
+          "apply_count_sketch");
+}
+}  // namespace
+
+namespace dispatch {
+Sketch::Sketch() {
+    apply_.fill(nullptr);
+    const std::size_t h = static_cast<std::size_t>(Device::HIP) * num_registry_numeric_types;
+    for (NumericType t : {NumericType::FP32, NumericType::FP64, NumericType::FP32_16, NumericType::FP32_BF16})
+        apply_[h + static_cast<std::size_t>(t)] = &cs_apply_impl;
+}
+const Sketch& Sketch::get() {
+    static const Sketch instance;
+    return instance;
+}
+apply_count_sketch_dense_fn Sketch::lookup_apply_count_sketch_dense(Device dev, NumericType src) const {
+    if (dev == Device::OMP) throw std::invalid_argument("apply_count_sketch: Device::OMP is not supported.");
+    auto fn = apply_[static_cast<std::size_t>(dev) * num_registry_numeric_types + static_cast<std::size_t>(src)];
+    if (!fn) throw std::runtime_error("apply_count_sketch: nothing registered for this device/type.");
+    return fn;
+}
+}  // namespace dispatch
+
+CountSketch create_count_sketch(std::size_t k, std::size_t m) { return CountSketch{{k, m}}; }
+
+void generate_count_sketch(CountSketch& sketch) {
+    // The hashed sketch is fully defined by its size and the fixed seeds.
+    if (sketch.size[0] == 0) throw std::invalid_argument("generate_count_sketch: zero rows");
+}
+
+void apply_count_sketch(CountSketch& sketch, tomos::Dense& source, tomos::Dense& target) {
+    dispatch::Sketch::get().lookup_apply_count_sketch_dense(source.get_device(), source.get_value_type())(
+        sketch, source, target);
+}
+
+void generate_sketch_qr_precond(CountSketch& sketch, tomos::Dense& sketched) {
+    (void)sketch;
+    device_guard(sketched.get_device(), "generate_sketch_qr_precond");
+    if (sketched.get_device() != Device::HIP)
+        throw std::runtime_error("generate_sketch_qr_precond: unsupported device.");
+    if (sketched.get_value_type() != NumericType::FP64)
+        throw std::runtime_error("generate_sketch_qr_precond: unsupported NumericType (FP64 only).");
+    const auto s = sketched.get_size();
+    if (s[0] < s[1])
+        throw std::invalid_argument(
+            "generate_sketch_qr_precond: sketched_mtx must have at least as many rows as columns (k >= n).");
+    const int rc = tomos_sketch_qr_precond_f64(static_cast<int64_t>(s[0]), static_cast<int64_t>(s[1]),
+                                              static_cast<double*>(sketched.data()),
+                                              static_cast<int64_t>(sketched.get_lead_dim()), sketched.stream());
+    if (rc == TOMOS_ERR_SINGULAR)
+        throw std::runtime_error(std::string("generate_sketch_qr_precond: trtri failed -- R is singular (") +
+                                 tomos_last_error() + ")");
+    check(rc, "generate_sketch_qr_precond");
+}
+
+// ======================================================================
+// Front-ends (src/core/blas/blas.cpp, src/core/solver/lsqr.cpp:389-411)
+// ======================================================================
+Status mv_apply(Device dev, TransposeType trans, double alpha, tomos::Dense& A, tomos::Dense& x, double beta,
+                tomos::Dense& y, NumericType compute, float* workspace) {
+    auto fn = dispatch::Dense::get().lookup_mv(dev, A.get_value_type(), x.get_value_type(),
+                                               y.get_value_type(), compute);
+    fn(trans, alpha, A, x, beta, y, workspace);
+    return SUCCESS;
+}
+
+Status mv_apply(Device dev, TransposeType trans, double alpha, tomos::Dense& A, tomos::Dense& x, double beta,
+                tomos::Dense& y) {
+    return mv_apply(dev, trans, alpha, A, x, beta, y, default_compute(A.get_value_type()));
+}
+
+std::size_t mv_workspace_size(tomos::Dense& A, TransposeType trans, NumericType compute) {
+    const auto s = A.get_size();
+    return tomos_gemv_scratch_size(abi_type(A.get_value_type()), tier_for(A.get_value_type(), compute),
+                                   trans == TransposeType::TRANSPOSE ? 1 : 0, static_cast<int64_t>(s[0]),
+                                   static_cast<int64_t>(s[1]));
+}
+
+void precond_apply(Device dev, TransposeType trans, double alpha, tomos::Dense& P, tomos::Dense& x,
+                   double beta, tomos::Dense& y) {
+    mv_apply(dev, trans, alpha, P, x, beta, y);
+}
+
+Status mm_apply(Device dev, TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A, tomos::Dense& B,
+                double beta, tomos::Dense& C, bool use_workspace) {
+    auto fn = dispatch::Dense::get().lookup_mm(dev, A.get_value_type(), B.get_value_type(),
+                                               C.get_value_type(), use_workspace);
+    fn(ta, tb, alpha, A, B, beta, C);
+    return SUCCESS;
+}
+
+template <typename value_t>
+void set_values(value_t value, tomos::Dense& target) {
+    const double v = static_cast<double>(value);
+    if (v == 0.0) {
+        check(tomos_memset(target.data(), 0, target.storage_bytes(), target.stream()), "set_values");
+        return;
+    }
+    const NumericType t = target.get_value_type();
+    if (t != NumericType::FP32 && t != NumericType::FP64)
+        throw std::invalid_argument("set_values: non-zero fill needs an FP32/FP64 target");
+    const std::size_t cnt = target.get_lead_dim() * target.get_size()[1];
+    if (t == NumericType::FP64) {
+        std::vector<double> h(cnt, v);
+        target.copy_from_host(h.data(), cnt * 8);
+    } else {
+        std::vector<float> h(cnt, static_cast<float>(v));
+        target.copy_from_host(h.data(), cnt * 4);
+    }
+}
+template void set_values<double>(double, tomos::Dense&);
+template void set_values<float>(float, tomos::Dense&);
+
+Status split_fp32_16(tomos::Dense& source, tomos::Dense& target) {
+    dispatch::Fp32_16::get().lookup_split(target.get_device())(source, target);
+    return SUCCESS;
+}
+Status combine_fp32_16(tomos::Dense& source, tomos::Dense& target) {
+    dispatch::Fp32_16::get().lookup_combine(source.get_device())(source, target);
+    return SUCCESS;
+}
+Status mv_apply_fp32_16_pure_half(Device dev, TransposeType t, double a, tomos::Dense& A, tomos::Dense& x,
+                                  double b, tomos::Dense& y) {
+    dispatch::Fp32_16::get().lookup_mv_pure_half(dev)(t, a, A, x, b, y, nullptr);
+    return SUCCESS;
+}
+Status mv_apply_fp32_16_fast(Device dev, TransposeType t, double a, tomos::Dense& A, tomos::Dense& x, double b,
+                             tomos::Dense& y) {
+    dispatch::Fp32_16::get().lookup_mv_fast(dev)(t, a, A, x, b, y, nullptr);
+    return SUCCESS;
+}
+Status mv_apply_fp32_16_ext(Device dev, TransposeType t, double a, tomos::Dense& A, tomos::Dense& x, double b,
+                            tomos::Dense& y) {
+    dispatch::Fp32_16::get().lookup_mv_accurate(dev)(t, a, A, x, b, y, nullptr);
+    return SUCCESS;
+}
+Status mm_apply_fp32_16_pure_half(Device dev, TransposeType ta, TransposeType tb, double a, tomos::Dense& A,
+                                  tomos::Dense& B, double b, tomos::Dense& C) {
+    dispatch::Fp32_16::get().lookup_mm_pure_half(dev)(ta, tb, a, A, B, b, C);
+    return SUCCESS;
+}
+Status mm_apply_fp32_16_fast(Device dev, TransposeType ta, TransposeType tb, double a, tomos::Dense& A,
+                             tomos::Dense& B, double b, tomos::Dense& C) {
+    dispatch::Fp32_16::get().lookup_mm_fast(dev)(ta, tb, a, A, B, b, C);
+    return SUCCESS;
+}
+Status mm_apply_fp32_16_ext(Device dev, TransposeType ta, TransposeType tb, double a, tomos::Dense& A,
+                            tomos::Dense& B, double b, tomos::Dense& C) {
+    dispatch::Fp32_16::get().lookup_mm_accurate(dev)(ta, tb, a, A, B, b, C);
+    return SUCCESS;
+}
+Status split_fp32_bf16(tomos::Dense& source, tomos::Dense& target, bool rounded) {
+    auto& r = dispatch::Fp32Bf16::get();
+    (rounded ? r.lookup_split_rounded(target.get_device()) : r.lookup_split(target.get_device()))(source, target);
+    return SUCCESS;
+}
+Status combine_fp32_bf16(tomos::Dense& source, tomos::Dense& target, bool rounded) {
+    auto& r = dispatch::Fp32Bf16::get();
+    (rounded ? r.lookup_combine_rounded(source.get_device()) : r.lookup_combine(source.get_device()))(source,
+                                                                                                       target);
+    return SUCCESS;
+}
+Status mv_apply_fp32_bf16_hi(Device dev, TransposeType t, double a, tomos::Dense& A, tomos::Dense& x, double b,
+                             tomos::Dense& y) {
+    dispatch::Fp32Bf16::get().lookup_mv_hi(dev)(t, a, A, x, b, y, nullptr);
+    return SUCCESS;
+}
+Status mv_apply_fp32_bf16_ext(Device dev, TransposeType t, double a, tomos::Dense& A, tomos::Dense& x, double b,
+                              tomos::Dense& y) {
+    dispatch::Fp32Bf16::get().lookup_mv_accurate(dev)(t, a, A, x, b, y, nullptr);
+    return SUCCESS;
+}
+
+Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type, NumericType precond_type) {
+    // nbytes = m*sz(v) + 3n*sz(v) + (sz(p)-1) + n*sz(p)  (lsqr.cpp:389-406)
+    const std::size_t m = size[0], n = size[1];
+    const std::size_t vs = numeric_type_size(value_type), ps = numeric_type_size(precond_type);
+    Workspace ws(ctx.get_device(), m * vs + 3 * n * vs + (ps - 1) + n * ps);
+    ws.value_type_ = value_type;
+    ws.precond_type_ = precond_type;
+    return ws;
+}
+
+Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type) {
+    return create_lsqr_workspace(ctx, size, value_type, value_type);
+}
+
+void lsqr_solve(Context context, tomos::Dense& A, tomos::Dense& b, tomos::Dense& x, Workspace& workspace,
+                double tol, int max_iters, double max_relres, int* iterations,
+                std::vector<double>* residual_history, int restarts, tomos::Dense* mtx_refine) {
+    if (mtx_refine && mtx_refine->get_value_type() != b.get_value_type())
+        throw std::invalid_argument(
+            "lsqr_solve: mtx_refine_precision's NumericType must match b/x's (refine_type).");
+    auto fn = dispatch::Lsqr::get().lookup_solve_dense(context.get_device(), A.get_value_type(),
+                                                       workspace.value_type_, b.get_value_type(),
+                                                       workspace.precond_type_);
+    fn(context, A, b, x, workspace, tol, max_iters, max_relres, iterations, residual_history, restarts,
+       mtx_refine);
+}
+
+}  // namespace tomos
