@@ -188,7 +188,8 @@ def main():
     xv = DeviceBuffer.from_numpy(rng.standard_normal(n).astype(np.float32))
     uv = DeviceBuffer.from_numpy(rng.standard_normal(m).astype(np.float32))
     zv = DeviceBuffer(n * 4)
-    gsc = C.lib().tomos_gemv_scratch_size(C.FP32_BF16, 0, 1, m, n)
+    gsc = max(C.lib().tomos_gemv_scratch_size(t, tier, tr, m, n)
+              for t, tier in ((C.FP32_BF16, 0), (C.FP32_BF16, 2), (C.F32, 0)) for tr in (0, 1))
     gs = DeviceBuffer(max(gsc, 1))
 
     def time_gemv(trans, a_type, tier, a_ptr, wr_ptr):

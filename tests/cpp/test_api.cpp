@@ -1,0 +1,308 @@
+// C++ API tests: the drop-in surface (Dense / Workspace / registries /
+// lsqr_solve) used the way the reference's own tests use it
+// (test/core/solver/test_lsqr.cpp, test/dispatch/test_*_registries.cpp,
+// test/core/workspace/test_workspace.cpp, test/core/matrix/test_fp32_16.cpp).
+// Runs on the GPU; prints one line per case and exits non-zero on failure.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tomos/api.hpp"
+
+using namespace tomos;
+
+static int g_fail = 0;
+
+#define EXPECT(cond)                                                                    \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            std::printf("    expectation failed: %s (%s:%d)\n", #cond, __FILE__, __LINE__); \
+            throw std::runtime_error("expectation failed");                             \
+        }                                                                               \
+    } while (0)
+
+template <class E, class F>
+bool throws(F&& f) {
+    try {
+        f();
+    } catch (const E&) {
+        return true;
+    } catch (...) {
+        return false;
+    }
+    return false;
+}
+
+static void run(const char* name, const std::function<void()>& f) {
+    try {
+        f();
+        std::printf("PASS %s\n", name);
+    } catch (const std::exception& e) {
+        std::printf("FAIL %s: %s\n", name, e.what());
+        ++g_fail;
+    }
+    std::fflush(stdout);
+}
+
+static Context ctx{Device::HIP};
+
+template <class T>
+static Dense make(NumericType t, std::size_t m, std::size_t n, const std::vector<T>& colmajor) {
+    Dense d = Dense::create_strided(ctx, Layout::COL_MAJOR, t, {m, n});
+    d.copy_from_host(colmajor.data(), colmajor.size() * sizeof(T));
+    return d;
+}
+
+template <class T>
+static std::vector<T> fetch(const Dense& d, std::size_t count) {
+    std::vector<T> h(count);
+    d.copy_to_host(h.data(), count * sizeof(T));
+    return h;
+}
+
+static std::vector<double> rows_to_cols(const std::vector<double>& r, std::size_t m, std::size_t n) {
+    std::vector<double> c(m * n);
+    for (std::size_t i = 0; i < m; ++i)
+        for (std::size_t j = 0; j < n; ++j) c[i + j * m] = r[i * n + j];
+    return c;
+}
+
+struct Sys {
+    const char* name;
+    std::size_t m, n;
+    std::vector<double> a_rows, b, x;
+    double tol, xtol;
+};
+
+int main() {
+    run("registries_are_singletons", [] {
+        EXPECT(&dispatch::Lsqr::get() == &dispatch::Lsqr::get());
+        EXPECT(&dispatch::Dense::get() == &dispatch::Dense::get());
+        EXPECT(&dispatch::Fp32_16::get() == &dispatch::Fp32_16::get());
+    });
+    run("omp_device_throws", [] {
+        EXPECT(throws<std::invalid_argument>([] { dispatch::Fp32_16::get().lookup_split(Device::OMP); }));
+        EXPECT(throws<std::invalid_argument>([] {
+            dispatch::Lsqr::get().lookup_solve_dense(Device::OMP, NumericType::FP32, NumericType::FP32,
+                                                    NumericType::FP32, NumericType::FP32);
+        }));
+    });
+    run("cpu_slots_are_empty", [] {
+        // no CPU implementation is shipped: lookups throw (test_fp32_16_registries.cpp:31-34 style)
+        EXPECT(throws<std::runtime_error>([] { dispatch::Fp32_16::get().lookup_mm_workspace(Device::CPU); }));
+        EXPECT(throws<std::runtime_error>([] {
+            dispatch::Dense::get().lookup_mv(Device::CPU, NumericType::FP32, NumericType::FP32,
+                                             NumericType::FP32, NumericType::FP32);
+        }));
+    });
+    run("unregistered_combination_throws", [] {
+        EXPECT(throws<std::runtime_error>([] {
+            dispatch::Dense::get().lookup_mv(Device::HIP, NumericType::FP32_16, NumericType::FP32,
+                                             NumericType::FP32, NumericType::FP32_16);
+        }));
+        EXPECT(throws<std::runtime_error>([] {
+            dispatch::Lsqr::get().lookup_solve_dense(Device::HIP, NumericType::FP16, NumericType::FP32,
+                                                    NumericType::FP32, NumericType::FP32);
+        }));
+    });
+    run("workspace_arena_formula_and_precond_check", [] {
+        Workspace ws = create_lsqr_workspace(ctx, {100, 7}, NumericType::FP32, NumericType::FP64);
+        EXPECT(ws.capacity_ == 100 * 4 + 3 * 7 * 4 + 7 + 7 * 8);
+        Dense p32 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {7, 7});
+        EXPECT(throws<std::invalid_argument>([&] { add_precond(p32, ws); }));
+        Dense p64 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {7, 7});
+        add_precond(p64, ws);
+        EXPECT(ws.precond_kind_ == PrecondKind::Dense && ws.precond_ == &p64);
+    });
+
+    std::vector<Sys> systems = {
+        {"identity3", 3, 3, {1, 0, 0, 0, 1, 0, 0, 0, 1}, {1, 2, 3}, {1, 2, 3}, 1e-10, 1e-8},
+        {"diag124", 3, 3, {1, 0, 0, 0, 2, 0, 0, 0, 4}, {1, 4, 8}, {1, 2, 2}, 1e-10, 1e-8},
+        {"overdetermined_consistent", 4, 2, {1, 0, 0, 1, 1, 1, 1, -1}, {1, 1, 2, 0}, {1, 1}, 1e-10, 1e-7},
+        {"overdetermined_inconsistent", 4, 2, {1, 1, 1, 2, 1, 3, 1, 4}, {6, 5, 7, 10}, {3.5, 1.4}, 1e-10, 1e-6},
+    };
+    for (const Sys& s : systems) {
+        run((std::string("lsqr_solve_fp64_") + s.name).c_str(), [&] {
+            Dense A = make(NumericType::FP64, s.m, s.n, rows_to_cols(s.a_rows, s.m, s.n));
+            Dense b = make(NumericType::FP64, s.m, 1, s.b);
+            Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {s.n, 1});
+            Workspace ws = create_lsqr_workspace(ctx, {s.m, s.n}, NumericType::FP64);
+            int iters = 0;
+            std::vector<double> hist;
+            lsqr_solve(ctx, A, b, x, ws, s.tol, 200, s.tol, &iters, &hist);
+            auto sol = fetch<double>(x, s.n);
+            for (std::size_t i = 0; i < s.n; ++i) EXPECT(std::fabs(sol[i] - s.x[i]) < s.xtol);
+            EXPECT(iters > 0 && hist.size() == static_cast<std::size_t>(iters));
+        });
+    }
+    run("lsqr_solve_fp32_system", [] {
+        std::vector<float> a = {1, 0, 0, 0, 2, 0, 0, 0, 3}, bb = {1, 4, 9};
+        Dense A = make(NumericType::FP32, 3, 3, a);
+        Dense b = make(NumericType::FP32, 3, 1, bb);
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {3, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {3, 3}, NumericType::FP32);
+        lsqr_solve(ctx, A, b, x, ws, 1e-5, 50, 1e-5);
+        auto sol = fetch<float>(x, 3);
+        EXPECT(std::fabs(sol[0] - 1) < 1e-5 && std::fabs(sol[1] - 2) < 1e-5 && std::fabs(sol[2] - 3) < 1e-5);
+    });
+    run("lsqr_workspace_too_small_throws", [] {
+        std::vector<float> a(12, 1.0f), bb(4, 1.0f);
+        Dense A = make(NumericType::FP32, 4, 3, a);
+        Dense b = make(NumericType::FP32, 4, 1, bb);
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {3, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {2, 3}, NumericType::FP32);
+        EXPECT(throws<std::invalid_argument>([&] { lsqr_solve(ctx, A, b, x, ws, 1e-6, 10, 1e-6); }));
+    });
+    run("lsqr_refine_type_mismatch_throws", [] {
+        std::vector<float> a(12, 1.0f), bb(4, 1.0f);
+        Dense A = make(NumericType::FP32, 4, 3, a);
+        Dense b = make(NumericType::FP32, 4, 1, bb);
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {3, 1});
+        Dense Ar = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {4, 3});
+        Workspace ws = create_lsqr_workspace(ctx, {4, 3}, NumericType::FP32);
+        EXPECT(throws<std::invalid_argument>(
+            [&] { lsqr_solve(ctx, A, b, x, ws, 1e-6, 10, 1e-6, nullptr, nullptr, 1, &Ar); }));
+        EXPECT(throws<std::invalid_argument>([&] {
+            ws.use_implicit_duplicate_ = true;  // needs packed A (lsqr_registries.cpp:106-112)
+            lsqr_solve(ctx, A, b, x, ws, 1e-6, 10, 1e-6);
+        }));
+    });
+    run("sketch_qr_rejects_k_lt_n", [] {
+        CountSketch S = create_count_sketch(3, 10);
+        Dense Y = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {3, 5});
+        EXPECT(throws<std::invalid_argument>([&] { generate_sketch_qr_precond(S, Y); }));
+    });
+    run("sketch_preconditioned_lsqr_40x5", [] {
+        // LsqrSketchPrecondTest (test_lsqr.cpp:461-523): mt19937(42), U(-1,1)
+        const std::size_t m = 40, n = 5, k = 30;
+        std::mt19937 gen(42);
+        std::uniform_real_distribution<double> dist(-1.0, 1.0);
+        std::vector<double> a(m * n), xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) a[i + j * m] = dist(gen);
+        for (auto& v : xt) v = dist(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += a[i + j * m] * xt[j];
+        Dense A = make(NumericType::FP64, m, n, a);
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        CountSketch S = create_count_sketch(k, m);
+        generate_count_sketch(S);
+        Dense sk = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(S, A, sk);
+        generate_sketch_qr_precond(S, sk);
+        Dense P = Dense::wrap_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n}, k, sk.data());
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP64);
+        add_precond(P, ws);
+        lsqr_solve(ctx, A, b, x, ws, 1e-10, 100, 1e-10);
+        auto sol = fetch<double>(x, n);
+        for (std::size_t i = 0; i < n; ++i) EXPECT(std::fabs(sol[i] - xt[i]) < 1e-6);
+    });
+    run("fp32_16_and_fp32_bf16_round_trip_and_tiers", [] {
+        const std::size_t m = 20, n = 15;
+        std::mt19937 gen(1);
+        std::uniform_real_distribution<float> dist(-8.0f, 8.0f);
+        std::vector<float> a(m * n), xv(n);
+        for (auto& v : a) v = dist(gen);
+        for (auto& v : xv) v = dist(gen);
+        Dense src = make(NumericType::FP32, m, n, a);
+        Dense x = make(NumericType::FP32, n, 1, xv);
+        for (NumericType pt : {NumericType::FP32_16, NumericType::FP32_BF16}) {
+            Dense P = Dense::create_strided(ctx, Layout::COL_MAJOR, pt, {m, n});
+            Dense back = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {m, n});
+            if (pt == NumericType::FP32_16) {
+                split_fp32_16(src, P);
+                combine_fp32_16(P, back);
+            } else {
+                split_fp32_bf16(src, P);
+                combine_fp32_bf16(P, back);
+            }
+            auto r = fetch<float>(back, m * n);
+            for (std::size_t i = 0; i < m * n; ++i) EXPECT(r[i] == a[i]);
+            std::vector<double> ref(m, 0.0);
+            for (std::size_t j = 0; j < n; ++j)
+                for (std::size_t i = 0; i < m; ++i) ref[i] += double(a[i + j * m]) * xv[j];
+            Dense y = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {m, 1});
+            mv_apply(Device::HIP, TransposeType::NO_TRANSPOSE, 1.0, P, x, 0.0, y);  // 16-bit tier
+            auto lo = fetch<float>(y, m);
+            mv_apply(Device::HIP, TransposeType::NO_TRANSPOSE, 1.0, P, x, 0.0, y, NumericType::FP32);
+            auto hi = fetch<float>(y, m);
+            for (std::size_t i = 0; i < m; ++i) {
+                EXPECT(std::fabs(lo[i] - ref[i]) <= 0.15 * std::fabs(ref[i]) + 0.15);
+                EXPECT(std::fabs(hi[i] - ref[i]) <= 1e-3 * std::fabs(ref[i]) + 1e-4);
+            }
+        }
+    });
+    run("mm_apply_packed_tiers", [] {
+        const std::size_t M = 33, N = 17, K = 29;
+        std::mt19937 gen(3);
+        std::uniform_real_distribution<float> dist(-8.0f, 8.0f);
+        std::vector<float> a(M * K), bm(K * N);
+        for (auto& v : a) v = dist(gen);
+        for (auto& v : bm) v = dist(gen);
+        Dense src = make(NumericType::FP32, M, K, a);
+        Dense B = make(NumericType::FP32, K, N, bm);
+        Dense P = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32_16, {M, K});
+        split_fp32_16(src, P);
+        std::vector<double> ref(M * N, 0.0);
+        for (std::size_t j = 0; j < N; ++j)
+            for (std::size_t l = 0; l < K; ++l)
+                for (std::size_t i = 0; i < M; ++i) ref[i + j * M] += double(a[i + l * M]) * bm[l + j * K];
+        for (bool ws : {false, true}) {
+            Dense C = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {M, N});
+            mm_apply(Device::HIP, TransposeType::NO_TRANSPOSE, TransposeType::NO_TRANSPOSE, 1.0, P, B, 0.0, C, ws);
+            auto c = fetch<float>(C, M * N);
+            for (std::size_t i = 0; i < M * N; ++i) EXPECT(std::fabs(c[i] - ref[i]) <= 1e-4 * std::fabs(ref[i]) + 1e-3);
+        }
+    });
+    run("mixed_precision_lsqr_implicit_duplicates", [] {
+        // FP32_BF16 A, hi-only per-iteration matvecs, exact restart residual,
+        // sketch-QR preconditioner: converges to 1e-6 (runner flags
+        // --implicit-duplicates --precond, lsqr_precision_runner.cu:638-656)
+        const std::size_t m = 6000, n = 60, k = 240;
+        std::mt19937_64 gen(5);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::vector<float> a(m * n);
+        std::vector<double> xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j) {
+            const double d = std::pow(10.0, -4.0 * double(j) / double(n - 1));
+            for (std::size_t i = 0; i < m; ++i) a[i + j * m] = static_cast<float>(nd(gen) * d);
+        }
+        for (auto& v : xt) v = nd(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += double(a[i + j * m]) * xt[j];
+        Dense src = make(NumericType::FP32, m, n, a);
+        Dense A = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32_BF16, {m, n});
+        split_fp32_bf16(src, A);
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        CountSketch S = create_count_sketch(k, m);
+        Dense sk = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(S, A, sk);
+        generate_sketch_qr_precond(S, sk);
+        Dense P = Dense::wrap_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n}, k, sk.data());
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP32, NumericType::FP64);
+        add_precond(P, ws);
+        ws.use_implicit_duplicate_ = true;
+        ws.mv_compute_type_ = NumericType::BF16;
+        ws.mv_compute_type_set_ = true;
+        int iters = 0;
+        lsqr_solve(ctx, A, b, x, ws, 1e-6, 10, 1e-6, &iters, nullptr, 30);
+        auto sol = fetch<double>(x, n);
+        double rn = 0, bn = 0;
+        for (std::size_t i = 0; i < m; ++i) {
+            double r = -bb[i];
+            for (std::size_t j = 0; j < n; ++j) r += double(a[i + j * m]) * sol[j];
+            rn += r * r;
+            bn += bb[i] * bb[i];
+        }
+        std::printf("    iterations=%d relres=%.3e\n", iters, std::sqrt(rn / bn));
+        EXPECT(std::sqrt(rn / bn) < 1.5e-6);
+    });
+    std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
+    return g_fail ? 1 : 0;
+}

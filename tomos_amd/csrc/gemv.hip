@@ -140,19 +140,33 @@ __device__ __forceinline__ ACC load_x(const void* __restrict__ x, bool x64, int6
                : static_cast<ACC>(static_cast<const float*>(x)[j]);
 }
 
-__device__ __forceinline__ void store_y(void* __restrict__ y, bool y64, int64_t i, double alpha,
-                                        double acc, double beta) {
+// Stores alpha*acc + beta*y (y not read when beta == 0); returns the
+// stored value.
+__device__ __forceinline__ double store_y(void* __restrict__ y, bool y64, int64_t i, double alpha,
+                                          double acc, double beta) {
     if (y64) {
         double* p = static_cast<double*>(y) + i;
         double r = alpha * acc;
         if (beta != 0.0) r += beta * (*p);
         *p = r;
+        return r;
     } else {
         float* p = static_cast<float*>(y) + i;
         double r = alpha * acc;
         if (beta != 0.0) r += beta * static_cast<double>(*p);
-        *p = static_cast<float>(r);
+        const float f = static_cast<float>(r);
+        *p = f;
+        return static_cast<double>(f);
     }
+}
+
+// Block-wide FP64 sum, fixed order; every thread of the block must call it.
+__device__ __forceinline__ void block_sumsq_out(double v, double* out) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ double part[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 // ------------------------------------------------------------ NO_TRANSPOSE
@@ -194,8 +208,8 @@ __global__ __launch_bounds__(kThreads) void gemv_n_kernel(
             for (int v = 0; v < VEC; ++v) acc[v] = fma(av[v], xv, acc[v]);
         }
     }
-    if (i0 >= m) return;
     if (partial) {
+        if (i0 >= m) return;
 #pragma unroll
         for (int v = 0; v < VEC; ++v)
             if (i0 + v < m) partial[static_cast<int64_t>(blockIdx.y) * m + i0 + v] = acc[v];
@@ -203,9 +217,16 @@ __global__ __launch_bounds__(kThreads) void gemv_n_kernel(
     }
     const double al = read_coef(coef.alpha, coef.dalpha, coef.alpha_mul);
     const double be = read_coef(coef.beta, coef.dbeta, coef.beta_mul);
+    double sq = 0.0;
+    if (i0 < m) {
 #pragma unroll
-    for (int v = 0; v < VEC; ++v)
-        if (i0 + v < m) store_y(y, y64, i0 + v, al, static_cast<double>(acc[v]), be);
+        for (int v = 0; v < VEC; ++v)
+            if (i0 + v < m) {
+                const double r = store_y(y, y64, i0 + v, al, static_cast<double>(acc[v]), be);
+                sq += r * r;
+            }
+    }
+    if (coef.sumsq) block_sumsq_out(sq, coef.sumsq + blockIdx.x);
 }
 
 template <class ACC>
@@ -215,11 +236,16 @@ __global__ __launch_bounds__(kThreads) void gemv_n_finish(int64_t m, int splits,
                                                           Coef coef) {
     if (coef.skip && *coef.skip) return;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= m) return;
-    double s = 0.0;
-    for (int c = 0; c < splits; ++c) s += static_cast<double>(partial[static_cast<int64_t>(c) * m + i]);
-    store_y(y, y64, i, read_coef(coef.alpha, coef.dalpha, coef.alpha_mul), s,
-            read_coef(coef.beta, coef.dbeta, coef.beta_mul));
+    double sq = 0.0;
+    if (i < m) {
+        double s = 0.0;
+        for (int c = 0; c < splits; ++c)
+            s += static_cast<double>(partial[static_cast<int64_t>(c) * m + i]);
+        const double r = store_y(y, y64, i, read_coef(coef.alpha, coef.dalpha, coef.alpha_mul), s,
+                                 read_coef(coef.beta, coef.dbeta, coef.beta_mul));
+        sq = r * r;
+    }
+    if (coef.sumsq) block_sumsq_out(sq, coef.sumsq + blockIdx.x);
 }
 
 // --------------------------------------------------------------- TRANSPOSE
@@ -399,9 +425,9 @@ size_t scratch_for(const Kind& k, int trans, int64_t m, int64_t n) {
 }
 
 template <int FMT, int DEC, class ACC>
-void run_n(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, const void* x,
-           bool x64, void* y, bool y64, const Coef& coef, void* scratch, bool vec,
-           const PlanN& p, hipStream_t s) {
+int run_n(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, const void* x,
+          bool x64, void* y, bool y64, const Coef& coef, void* scratch, bool vec,
+          const PlanN& p, hipStream_t s) {
     constexpr int VF = vec_full<FMT>();
     ACC* part = p.splits > 1 ? static_cast<ACC*>(scratch) : nullptr;
     const int width = vec ? VF : 1;
@@ -418,13 +444,15 @@ void run_n(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
         gemv_n_finish<ACC><<<dim3(static_cast<unsigned>(ceil_div(m, kThreads))), kThreads, 0, s>>>(
             m, p.splits, part, y, y64, coef);
         TOMOS_LAUNCH_CHECK("gemv_n_finish");
+        return static_cast<int>(ceil_div(m, kThreads));
     }
+    return static_cast<int>(grid.x);
 }
 
 template <int FMT, int DEC, class ACC>
 void run_t(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, const void* x,
            bool x64, void* y, bool y64, const Coef& coef, void* scratch, bool vec,
-           const PlanT& p, hipStream_t s) {
+           const PlanT& p, hipStream_t s, bool finish = true) {
     constexpr int VF = vec_full<FMT>();
     constexpr int CB = sizeof(ACC) == 8 ? 32 : 64;
     ACC* part = static_cast<ACC*>(scratch);
@@ -439,6 +467,7 @@ void run_t(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
                 m, n, a, wr, lda, x, x64, p.rows_per_chunk, p.ncg, part, coef.skip);
         TOMOS_LAUNCH_CHECK("gemv_t_kernel");
     }
+    if (!finish) return;
     gemv_t_finish<ACC><<<dim3(static_cast<unsigned>(ceil_div(n, kThreads))), kThreads, 0, s>>>(
         n, m > 0 ? p.chunks : 0, part, y, y64, coef);
     TOMOS_LAUNCH_CHECK("gemv_t_finish");
@@ -450,9 +479,11 @@ size_t gemv_scratch_bytes(int a_type, int tier, int trans, int64_t m, int64_t n)
     return scratch_for(classify(a_type, tier), trans, m, n);
 }
 
+int64_t gemv_sumsq_blocks_max(int64_t m) { return ceil_div(std::max<int64_t>(m, 1), kThreads); }
+
 void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t n, const void* a,
           const uint16_t* wr, int64_t lda, int x_type, const void* x, int y_type, void* y,
-          void* scratch, size_t scratch_bytes, hipStream_t s) {
+          void* scratch, size_t scratch_bytes, hipStream_t s, int* sumsq_blocks) {
     const Kind k = classify(a_type, tier);
     require(trans == TOMOS_NO_TRANSPOSE || trans == TOMOS_TRANSPOSE, "gemv: bad trans");
     require(m >= 0 && n >= 0, "gemv: negative size");
@@ -460,6 +491,8 @@ void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t 
     require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "gemv: x must be F32 or F64");
     require(y_type == TOMOS_F32 || y_type == TOMOS_F64, "gemv: y must be F32 or F64");
     const int64_t rows_out = trans == TOMOS_NO_TRANSPOSE ? m : n;
+    require(coef.sumsq == nullptr || trans == TOMOS_NO_TRANSPOSE, "gemv: sumsq needs NO_TRANSPOSE");
+    if (sumsq_blocks) *sumsq_blocks = 0;
     if (rows_out == 0) return;
     require(y != nullptr, "gemv: null y");
     const int64_t inner = trans == TOMOS_NO_TRANSPOSE ? n : m;
@@ -483,7 +516,8 @@ void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t 
     if (trans == TOMOS_NO_TRANSPOSE) {
         // n == 0 falls through with an empty column range: y = beta*y.
         const PlanN p = plan_n(k, m, n);
-#define RUN_N(F, D, A) run_n<F, D, A>(m, n, a, wr, lda, x, x64, y, y64, coef, scratch, vec, p, s)
+int nb = 0;
+#define RUN_N(F, D, A) nb = run_n<F, D, A>(m, n, a, wr, lda, x, x64, y, y64, coef, scratch, vec, p, s)
         if (k.fmt == FMT_F32) { if (k.acc64) RUN_N(FMT_F32, DEC_NATIVE, double); else RUN_N(FMT_F32, DEC_NATIVE, float); }
         else if (k.fmt == FMT_F64) RUN_N(FMT_F64, DEC_NATIVE, double);
         else if (k.fmt == FMT_P16) {
@@ -496,6 +530,7 @@ void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t 
             else RUN_N(FMT_PBF, DEC_HI, float);
         }
 #undef RUN_N
+        if (sumsq_blocks) *sumsq_blocks = nb;
     } else {
         const PlanT p = plan_t(k, m, n);
 #define RUN_T(F, D, A) run_t<F, D, A>(m, n, a, wr, lda, x, x64, y, y64, coef, scratch, vec, p, s)
@@ -512,6 +547,30 @@ void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t 
         }
 #undef RUN_T
     }
+}
+
+int gemv_t_partials(int a_type, int tier, int64_t m, int64_t n, const void* a, const uint16_t* wr,
+                    int64_t lda, int x_type, const void* x, void* scratch, size_t scratch_bytes,
+                    const int* skip, hipStream_t s) {
+    const Kind k = classify(a_type, tier);
+    require(k.acc64, "gemv_t_partials: needs an FP64-accumulating tier");
+    require(m >= 1 && n >= 1 && lda >= m, "gemv_t_partials: bad sizes");
+    const size_t need = scratch_for(k, TOMOS_TRANSPOSE, m, n);
+    require(scratch != nullptr && scratch_bytes >= need, "gemv_t_partials: scratch too small");
+    const PlanT p = plan_t(k, m, n);
+    const bool x64 = x_type == TOMOS_F64;
+    const int vf = vec_full_of(k.fmt);
+    const bool vec = (lda % vf == 0) && (reinterpret_cast<uintptr_t>(a) % 16 == 0) &&
+                     (k.dec != DEC_EXACT || reinterpret_cast<uintptr_t>(wr) % 16 == 0);
+    Coef c;
+    c.skip = skip;
+#define RUN_TP(F, D) run_t<F, D, double>(m, n, a, wr, lda, x, x64, nullptr, false, c, scratch, vec, p, s, false)
+    if (k.fmt == FMT_F32) RUN_TP(FMT_F32, DEC_NATIVE);
+    else if (k.fmt == FMT_F64) RUN_TP(FMT_F64, DEC_NATIVE);
+    else if (k.fmt == FMT_P16) { if (k.dec == DEC_EXACT) RUN_TP(FMT_P16, DEC_EXACT); else RUN_TP(FMT_P16, DEC_HI); }
+    else { if (k.dec == DEC_EXACT) RUN_TP(FMT_PBF, DEC_EXACT); else RUN_TP(FMT_PBF, DEC_HI); }
+#undef RUN_TP
+    return p.chunks;
 }
 
 }  // namespace tomos_hip

@@ -27,6 +27,7 @@
 
 #include "common.hpp"
 #include "gemv.hpp"
+#include "lsqr_fused.hpp"
 #include "tomos_hip.h"
 
 namespace tomos_hip {
@@ -38,13 +39,14 @@ constexpr int kRedBlocks = 1024;
 struct LsqrState {
     double alpha, beta, rho_bar, phi_bar;
     double rhsnorm;
-    double x_coef;   // phi / rho
-    double w_coef;   // -theta / rho
-    double pad;
+    double x_coef;    // phi / rho
+    double w_coef;    // -theta / rho
+    double inv_beta;  // 1 / beta: u = u_raw * inv_beta
+    double u_coef;    // -alpha / beta: next u_raw = A p + u_coef * u_raw
     int done;
     int completed;
     int hist_len;
-    int pad2;
+    int z_fresh;      // 1 right after initialize: z = P w restarts from z = P v
 };
 
 __device__ __forceinline__ double ld(const void* p, bool f64, int64_t i) {
@@ -60,6 +62,8 @@ __global__ void k_state_init(LsqrState* s) {
         s->alpha = s->beta = s->rho_bar = s->phi_bar = 0.0;
         s->rhsnorm = 0.0;
         s->x_coef = s->w_coef = 0.0;
+        s->inv_beta = s->u_coef = 0.0;
+        s->z_fresh = 1;
         s->done = 0;
         s->completed = 0;
         s->hist_len = 0;
@@ -109,40 +113,6 @@ __global__ __launch_bounds__(kT) void k_norm_finish(const LsqrState* s, const do
     if (threadIdx.x == 0) *out = sqrt((w[0] + w[1]) + (w[2] + w[3]));
 }
 
-// x *= 1 / *den
-__global__ __launch_bounds__(kT) void k_scale_inv(const LsqrState* __restrict__ s, void* x,
-                                                  bool f64, int64_t n, const double* den) {
-    if (s->done) return;
-    const double inv = 1.0 / *den;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kT)
-        st(x, f64, i, ld(x, f64, i) * inv);
-}
-
-// y += mul * (*coef) * x
-__global__ __launch_bounds__(kT) void k_axpy(const LsqrState* __restrict__ s, const void* x,
-                                             bool x64, void* y, bool y64, int64_t n,
-                                             const double* coef, double mul) {
-    if (s->done) return;
-    const double a = mul * (*coef);
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kT)
-        st(y, y64, i, ld(y, y64, i) + a * ld(x, x64, i));
-}
-
-// w = (*coef) * w + v   (lsqr_scal then lsqr_axpy, lsqr.cpp:670-671)
-__global__ __launch_bounds__(kT) void k_w_update(const LsqrState* __restrict__ s, void* w,
-                                                 const void* v, bool f64, int64_t n) {
-    if (s->done) return;
-    const double c = s->w_coef;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kT) {
-        const double scaled = f64 ? c * ld(w, true, i)
-                                  : static_cast<double>(static_cast<float>(c * ld(w, false, i)));
-        st(w, f64, i, scaled + ld(v, f64, i));
-    }
-}
-
 __device__ __forceinline__ void push_history(LsqrState* s, double* hist, int cap, double rr) {
     if (s->hist_len < cap) hist[s->hist_len] = rr;
     s->hist_len += 1;
@@ -158,37 +128,141 @@ __global__ void k_restart_check(LsqrState* s, double* hist, int cap, double tol,
     if (rr < tol || rr < maxrr) s->done = 1;
 }
 
-// end of initialize + the true-residual test (lsqr.cpp:509-512, :723-729)
-__global__ void k_init_finish(LsqrState* s, double tol, double maxrr) {
-    if (threadIdx.x != 0 || s->done) return;
-    s->rho_bar = s->alpha;
-    s->phi_bar = s->beta;
-    if (s->rhsnorm == 0.0) { s->done = 1; return; }
-    const double rr = s->beta / s->rhsnorm;
-    if (rr < tol || rr < maxrr) s->done = 1;
+// beta = sqrt(sum red[0..nb)) in a fixed order; inv_beta = 1/beta
+__global__ __launch_bounds__(kT) void k_beta_finish(LsqrState* s, const double* __restrict__ red, int nb) {
+    if (s->done) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kT) acc += red[i];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = sqrt((w[0] + w[1]) + (w[2] + w[3]));
+        s->beta = b;
+        s->inv_beta = 1.0 / b;
+    }
 }
 
-// Givens rotation of step_2_update_sol (lsqr.cpp:642-647)
-__global__ void k_givens(LsqrState* s) {
-    if (threadIdx.x != 0 || s->done) return;
-    const double rho = hypot(s->rho_bar, s->beta);
-    const double c = s->rho_bar / rho, sn = s->beta / rho;
-    const double theta = sn * s->alpha;
-    s->rho_bar = -c * s->alpha;
-    const double phi = c * s->phi_bar;
-    s->phi_bar = sn * s->phi_bar;
-    s->x_coef = phi / rho;
-    s->w_coef = -theta / rho;
+// temp[j] = inv_beta * sum_s tsum[s][j] (fixed order): A^T u from the
+// single-pass kernel's level-1 sums
+__global__ __launch_bounds__(kT) void k_temp_finish(const LsqrState* __restrict__ S, int64_t n,
+                                                    int splits, const double* __restrict__ tsum,
+                                                    void* temp, bool t64) {
+    if (S->done) return;
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+    if (j >= n) return;
+    double a = 0.0;
+    for (int sp = 0; sp < splits; ++sp) a += tsum[static_cast<int64_t>(sp) * n + j];
+    st(temp, t64, j, a * S->inv_beta);
 }
 
-// per-iteration stop check (lsqr.cpp:735-744)
-__global__ void k_iter_check(LsqrState* s, double* hist, int cap, double tol, double maxrr) {
-    if (threadIdx.x != 0 || s->done) return;
-    s->completed += 1;
-    if (s->rhsnorm == 0.0) { s->done = 1; return; }
-    const double rr = fabs(s->phi_bar) / s->rhsnorm;
-    push_history(s, hist, cap, rr);
-    if (rr < tol || rr < maxrr) s->done = 1;
+constexpr int kTail = 1024;
+
+__device__ __forceinline__ double tail_block_sum(double v, double* sh) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kTail / 64; ++i) t += sh[i];  // same order in every thread
+    return t;
+}
+
+// The n-sized part of an LSQR step in one block (lsqr.cpp:557-590 after
+// the A^T u matvec, and :632-679):
+//   src = P^T (A^T u)  (pre: sum of the R^-T chunk partials; else temp)
+//   q = src - beta*v (iteration) or src (initialize)
+//   alpha = |q|, v = q/alpha
+//   iteration: Givens update; x += (phi/rho) * (pre ? z : w) with
+//     z = P w carried by z <- P v_new + (-theta/rho) z (no extra GEMV);
+//     w <- v_new + (-theta/rho) w
+//   initialize: w = v (z restarts), rho_bar = alpha, phi_bar = beta,
+//     true-residual stop test
+//   next u_raw coefficient: -alpha/beta; stop check + history.
+__global__ __launch_bounds__(kTail) void k_tail(LsqrState* S, int init, int pre,
+                                                const double* __restrict__ rpart, int rchunks,
+                                                const void* temp, void* v, void* w, void* z,
+                                                const void* p, bool v64, bool p64, void* x,
+                                                bool x64, int64_t n, double* hist, int cap,
+                                                double tol, double maxrr) {
+    if (S->done) return;
+    __shared__ double sh[kTail / 64];
+    __shared__ double bc[4];
+    const double beta = S->beta;
+    auto src = [&](int64_t j) -> double {
+        if (pre) {
+            double a = 0.0;
+            for (int c = 0; c < rchunks; ++c) a += rpart[static_cast<int64_t>(c) * n + j];
+            return a;
+        }
+        return ld(temp, v64, j);
+    };
+    double ss = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        const double q = init ? src(j) : src(j) - beta * ld(v, v64, j);
+        ss += q * q;
+    }
+    const double alpha = sqrt(tail_block_sum(ss, sh));
+    if (threadIdx.x == 0) {
+        if (init) {
+            bc[0] = bc[1] = bc[2] = 0.0;
+        } else {
+            const double rho = hypot(S->rho_bar, beta);
+            const double c = S->rho_bar / rho, sn = beta / rho;
+            const double theta = sn * alpha;
+            const double phi = c * S->phi_bar;
+            S->rho_bar = -c * alpha;
+            S->phi_bar = sn * S->phi_bar;
+            bc[0] = phi / rho;       // x coefficient
+            bc[1] = -theta / rho;    // new w coefficient
+            bc[2] = S->w_coef;       // previous w coefficient (z recurrence)
+        }
+    }
+    __syncthreads();
+    const double xc = bc[0], wc = bc[1], wc_prev = bc[2];
+    const bool fresh = S->z_fresh != 0;
+    const double inv_alpha = 1.0 / alpha;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        const double q = init ? src(j) : src(j) - beta * ld(v, v64, j);
+        const double vn = q * inv_alpha;
+        if (init) {
+            if (!pre) st(w, v64, j, vn);
+        } else if (pre) {
+            const double pj = ld(p, p64, j);
+            const double zj = fresh ? pj : pj + wc_prev * ld(z, p64, j);
+            st(z, p64, j, zj);
+            st(x, x64, j, ld(x, x64, j) + xc * zj);
+        } else {
+            const double wj = ld(w, v64, j);
+            st(x, x64, j, ld(x, x64, j) + xc * wj);
+            st(w, v64, j, vn + wc * wj);
+        }
+        st(v, v64, j, vn);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    S->alpha = alpha;
+    S->u_coef = -alpha / beta;
+    if (init) {
+        S->rho_bar = alpha;
+        S->phi_bar = beta;
+        S->z_fresh = 1;
+        S->w_coef = 0.0;
+        if (S->rhsnorm == 0.0) { S->done = 1; return; }
+        const double rr = beta / S->rhsnorm;
+        if (rr < tol || rr < maxrr) S->done = 1;
+        return;
+    }
+    S->z_fresh = 0;
+    S->w_coef = wc;
+    S->x_coef = xc;
+    S->completed += 1;
+    if (S->rhsnorm == 0.0) { S->done = 1; return; }
+    const double rr = fabs(S->phi_bar) / S->rhsnorm;
+    push_history(S, hist, cap, rr);
+    if (rr < tol || rr < maxrr) S->done = 1;
 }
 
 // Two buffers, as the reference's Workspace: the arena holds the Krylov
@@ -199,7 +273,9 @@ __global__ void k_iter_check(LsqrState* s, double* hist, int cap, double tol, do
 // history, reduction partials and GEMV partial sums.
 struct Layout {
     size_t u, v, w, t, pv, arena_total;
-    size_t state, hist, red, scratch, scratch_total;
+    size_t state, hist, red, z, tpart, tsum, scratch, scratch_total;
+    bool fused_possible;
+    FusedPlan fplan;
     size_t gemv_bytes;
     int hist_cap;
 };
@@ -236,14 +312,24 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
     L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 26));
     L.hist = off; off = align256(off + static_cast<size_t>(L.hist_cap) * 8);
-    L.red = off; off = align256(off + kRedBlocks * 8);
+    L.red = off; off = align256(off + static_cast<size_t>(std::max<int64_t>(gemv_sumsq_blocks_max(m), kRedBlocks)) * 8);
+    L.z = off; off = align256(off + static_cast<size_t>(n) * 8);  // z = P w (precond type)
+    L.fused_possible = o.fused != 0 && o.mv_tier == TOMOS_TIER_DEFAULT && n >= 1 && n <= 1536 &&
+                       (o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16);
+    L.tpart = L.tsum = off;
+    if (L.fused_possible) {
+        L.fplan = lsqr_fused_plan(o.a_type, m, n);
+        L.tpart = off; off = align256(off + L.fplan.tpart_bytes);
+        L.tsum = off; off = align256(off + L.fplan.tsum_bytes);
+    }
     size_t sc = 0;
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_NO_TRANSPOSE, m, n));
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_TRANSPOSE, m, n));
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.restart_tier, TOMOS_NO_TRANSPOSE, m, n));
     if (o.precond_type >= 0) {
-        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, TOMOS_TIER_DEFAULT, TOMOS_NO_TRANSPOSE, n, n));
-        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, TOMOS_TIER_DEFAULT, TOMOS_TRANSPOSE, n, n));
+        const int pt = o.precond_type == TOMOS_F64 ? TOMOS_TIER_DEFAULT : TOMOS_TIER_EXACT;
+        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, pt, TOMOS_NO_TRANSPOSE, n, n));
+        sc = std::max(sc, gemv_scratch_bytes(o.precond_type, pt, TOMOS_TRANSPOSE, n, n));
     }
     L.scratch = off;
     L.gemv_bytes = sc;
@@ -292,6 +378,8 @@ struct Solver {
     void* PV() const { return arena + L.pv; }
     double* hist() const { return reinterpret_cast<double*>(sc + L.hist); }
     double* red() const { return reinterpret_cast<double*>(sc + L.red); }
+    void* Z() const { return sc + L.z; }
+    bool use_fused = false;
     void* scratch() const { return sc + L.scratch; }
 
     void norm(const void* vec, bool f64, int64_t len, double* out, bool guarded_by_state = true) {
@@ -308,76 +396,77 @@ struct Solver {
         return c;
     }
     // A-matrix GEMV at a tier
-    void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64) {
+    void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64,
+                int* nb = nullptr) {
         gemv(o.a_type, tier, trans, c, m, n, a, wr, lda, x64 ? TOMOS_F64 : TOMOS_F32, xin,
-             y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.gemv_bytes, s);
+             y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.gemv_bytes, s, nb);
     }
+    // P GEMVs accumulate in FP64 whatever the storage type of R^-1
+    int ptier() const { return o.precond_type == TOMOS_F64 ? TOMOS_TIER_DEFAULT : TOMOS_TIER_EXACT; }
     // P = R^-1 GEMV
     void gemv_p(int trans, const void* xin, bool x64, void* yout, bool y64) {
-        gemv(o.precond_type, TOMOS_TIER_DEFAULT, trans, coef(1.0, 0.0), n, n, rinv, nullptr, ldr,
+        gemv(o.precond_type, ptier(), trans, coef(1.0, 0.0), n, n, rinv, nullptr, ldr,
              x64 ? TOMOS_F64 : TOMOS_F32, xin, y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(),
              L.gemv_bytes, s);
-    }
-    void scale_inv(void* vec, bool f64, int64_t len, const double* den) {
-        k_scale_inv<<<grid_for(len), kT, 0, s>>>(state(), vec, f64, len, den);
     }
 
     void initialize() {
         LsqrState* S = state();
-        const bool pre = o.precond_type >= 0;
+        // u_raw = b - A x at the restart tier; beta = |u_raw| (fused partials)
         k_copy<<<grid_for(m), kT, 0, s>>>(S, b, r64, U(), v64, m);
-        gemv_a(o.restart_tier, TOMOS_NO_TRANSPOSE, coef(-1.0, 1.0), x, r64, U(), v64);
-        norm(U(), v64, m, &S->beta);
-        scale_inv(U(), v64, m, &S->beta);
-        if (pre) {
-            gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, Tm(), v64);
-            gemv_p(TOMOS_TRANSPOSE, Tm(), v64, V(), v64);
-        } else {
-            gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, V(), v64);
+        Coef cr = coef(-1.0, 1.0);
+        cr.sumsq = red();
+        int nb = 0;
+        gemv_a(o.restart_tier, TOMOS_NO_TRANSPOSE, cr, x, r64, U(), v64, &nb);
+        k_beta_finish<<<1, kT, 0, s>>>(S, red(), nb);
+        tail(true);
+    }
+
+    // temp = A^T u = (A^T u_raw) / beta, then (pre) the R^-T chunk partials,
+    // then the one-block tail.
+    void tail(bool init, bool temp_ready = false) {
+        LsqrState* S = state();
+        const bool pre = o.precond_type >= 0;
+        if (!temp_ready) {
+            Coef ct = coef(0.0, 0.0);
+            ct.dalpha = &S->inv_beta;
+            gemv_a(o.mv_tier, TOMOS_TRANSPOSE, ct, U(), v64, Tm(), v64);
         }
-        norm(V(), v64, n, &S->alpha);
-        scale_inv(V(), v64, n, &S->alpha);
-        k_copy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, W(), v64, n);
-        k_init_finish<<<1, 64, 0, s>>>(S, o.tol, o.max_relres);
+        int rch = 0;
+        if (pre)
+            rch = gemv_t_partials(o.precond_type, ptier(), n, n, rinv, nullptr, ldr,
+                                  v64 ? TOMOS_F64 : TOMOS_F32, Tm(), scratch(), L.gemv_bytes,
+                                  &S->done, s);
+        k_tail<<<1, kTail, 0, s>>>(S, init ? 1 : 0, pre ? 1 : 0, static_cast<const double*>(scratch()),
+                                   rch, Tm(), V(), W(), Z(), PV(), v64, p64, x, r64, n, hist(),
+                                   L.hist_cap, o.tol, o.max_relres);
     }
 
     void iterate() {
         LsqrState* S = state();
         const bool pre = o.precond_type >= 0;
+        // u_raw = A (P v) + (-alpha/beta) u_raw, |u_raw|^2 fused
         Coef cn = coef(1.0, 0.0);
-        cn.dbeta = &S->alpha;
-        cn.beta_mul = -1.0;
-        if (pre) {
-            gemv_p(TOMOS_NO_TRANSPOSE, V(), v64, PV(), p64);
-            gemv_a(o.mv_tier, TOMOS_NO_TRANSPOSE, cn, PV(), p64, U(), v64);
-        } else {
-            k_copy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, Tm(), v64, n);
-            gemv_a(o.mv_tier, TOMOS_NO_TRANSPOSE, cn, Tm(), v64, U(), v64);
+        cn.dbeta = &S->u_coef;
+        cn.sumsq = red();
+        int nb = 0;
+        if (pre) gemv_p(TOMOS_NO_TRANSPOSE, V(), v64, PV(), p64);
+        const void* xin = pre ? PV() : V();
+        const bool x64 = pre ? p64 : v64;
+        if (use_fused) {
+            // one pass over A: u_raw and the A^T u_raw partials together
+            lsqr_fused_launch(o.a_type, m, n, a, lda, xin, x64, U(), v64, &S->u_coef, red(),
+                              reinterpret_cast<float*>(sc + L.tpart),
+                              reinterpret_cast<double*>(sc + L.tsum), L.fplan, &S->done, s);
+            k_beta_finish<<<1, kT, 0, s>>>(S, red(), L.fplan.nblocks);
+            k_temp_finish<<<grid_for(n), kT, 0, s>>>(S, n, L.fplan.splits,
+                                                    reinterpret_cast<const double*>(sc + L.tsum), Tm(), v64);
+            tail(false, true);
+            return;
         }
-        norm(U(), v64, m, &S->beta);
-        scale_inv(U(), v64, m, &S->beta);
-        gemv_a(o.mv_tier, TOMOS_TRANSPOSE, coef(1.0, 0.0), U(), v64, Tm(), v64);
-        if (pre) {
-            gemv_p(TOMOS_TRANSPOSE, Tm(), v64, PV(), p64);
-            k_axpy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, PV(), p64, n, &S->beta, -1.0);
-            norm(PV(), p64, n, &S->alpha);
-            scale_inv(PV(), p64, n, &S->alpha);
-            k_copy<<<grid_for(n), kT, 0, s>>>(S, PV(), p64, V(), v64, n);
-        } else {
-            k_axpy<<<grid_for(n), kT, 0, s>>>(S, V(), v64, Tm(), v64, n, &S->beta, -1.0);
-            norm(Tm(), v64, n, &S->alpha);
-            scale_inv(Tm(), v64, n, &S->alpha);
-            k_copy<<<grid_for(n), kT, 0, s>>>(S, Tm(), v64, V(), v64, n);
-        }
-        k_givens<<<1, 64, 0, s>>>(S);
-        if (pre) {
-            gemv_p(TOMOS_NO_TRANSPOSE, W(), v64, PV(), p64);
-            k_axpy<<<grid_for(n), kT, 0, s>>>(S, PV(), p64, x, r64, n, &S->x_coef, 1.0);
-        } else {
-            k_axpy<<<grid_for(n), kT, 0, s>>>(S, W(), v64, x, r64, n, &S->x_coef, 1.0);
-        }
-        k_w_update<<<grid_for(n), kT, 0, s>>>(S, W(), V(), v64, n);
-        k_iter_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres);
+        gemv_a(o.mv_tier, TOMOS_NO_TRANSPOSE, cn, xin, x64, U(), v64, &nb);
+        k_beta_finish<<<1, kT, 0, s>>>(S, red(), nb);
+        tail(false);
     }
 
     bool done_flag() {
@@ -482,6 +571,7 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.v64 = o.value_type == TOMOS_F64;
     sv.r64 = o.refine_type == TOMOS_F64;
     sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
+    sv.use_fused = sv.L.fused_possible && lsqr_fused_supported(o.a_type, o.mv_tier, m, n, lda, a);
     sv.run(iterations, history, history_cap, history_len);
 }
 
