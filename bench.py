@@ -50,11 +50,47 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--check-every", type=int, default=8)
+    p.add_argument("--fused", type=int, default=1,
+                   help="single-pass LSQR matvec kernel (A read once per iteration)")
+    p.add_argument("--gemm", type=int, default=8192, help="split GEMM size (0 = skip)")
+    p.add_argument("--kernels-only", action="store_true",
+                   help="only the hi-only GEMV-N at the full shape and a copy of the same bytes "
+                        "(for hardware-counter runs); prints their timings")
     return p.parse_args()
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kernels_only(args, C, DeviceBuffer, m, n, wh):
+    """GEMV-N (hi-only FP32_BF16, y = A x) and tomos_copy_kernel over the
+    same 2*m*n bytes, each launched --gemv-reps times: the copy's counter
+    readings calibrate FETCH_SIZE / WRITE_SIZE into bytes."""
+    rng = np.random.default_rng(3)
+    xv = DeviceBuffer.from_numpy(rng.standard_normal(n).astype(np.float32))
+    y = DeviceBuffer(m * 4)
+    nbytes = 2 * m * n
+    dst = DeviceBuffer(nbytes)
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    for e in (ev0, ev1):
+        C.call("tomos_event_create", ctypes.byref(e))
+    f = ctypes.c_float()
+    out = {}
+    for name, fn in (
+        ("gemv_n_bf16_hi", lambda: C.call("tomos_gemv", C.FP32_BF16, C.TIER_DEFAULT, 0, 1.0, m, n,
+                                          wh.ptr, None, m, C.F32, xv.ptr, 0.0, C.F32, y.ptr, None, 0, None)),
+        ("copy", lambda: C.call("tomos_copy_kernel", dst.ptr, wh.ptr, nbytes, None)),
+    ):
+        fn()
+        C.call("tomos_event_record", ev0, None)
+        for _ in range(args.gemv_reps):
+            fn()
+        C.call("tomos_event_record", ev1, None)
+        C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+        out[name] = {"ms": round(f.value / args.gemv_reps, 4), "bytes": nbytes,
+                     "launches": args.gemv_reps + 1}
+    print(json.dumps({"kernels_only": out, "m": m, "n": n}), flush=True)
 
 
 def main():
@@ -92,6 +128,8 @@ def main():
     b_host = b.to_numpy(np.float64, (m,))
     bnorm = float(np.linalg.norm(b_host))
     log("data ready")
+    if args.kernels_only:
+        return kernels_only(args, C, DeviceBuffer, m, n, wh)
 
     # ---- buffers for the solve -----------------------------------------------
     sketch = DeviceBuffer(s * n * 8)
@@ -99,7 +137,7 @@ def main():
     cs_scratch = DeviceBuffer(cs_bytes)
     x = DeviceBuffer(n * 8)
     opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64,
-                        args.tol, args.tol, args.inner, args.restarts, args.check_every, 0)
+                        args.tol, args.tol, args.inner, args.restarts, args.check_every, args.fused)
     ar_bytes = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
     sc_bytes = C.lib().tomos_lsqr_scratch_size(ctypes.byref(opt), m, n)
     arena, scr = DeviceBuffer(ar_bytes), DeviceBuffer(sc_bytes)
@@ -158,7 +196,7 @@ def main():
         bf = DeviceBuffer.from_numpy(b_host.astype(np.float32))
         xf = DeviceBuffer(n * 4)
         po = C.LsqrOptions(C.F32, C.TIER_DEFAULT, C.TIER_DEFAULT, C.F32, C.F32, -1, args.tol,
-                           args.tol, args.plain_cap, 1, 64, 0)
+                           args.tol, args.plain_cap, 1, 64, args.fused)
         pa = C.lib().tomos_lsqr_arena_size(ctypes.byref(po), m, n)
         pw = C.lib().tomos_lsqr_scratch_size(ctypes.byref(po), m, n)
         parena, pws = DeviceBuffer(pa), DeviceBuffer(pw)
@@ -231,6 +269,57 @@ def main():
         except Exception:
             traffic = None
 
+    # ---- split-precision MFMA GEMM (C3 square shape) -------------------------
+    gemm = None
+    if args.gemm > 0:
+        del y, uv, gs
+        G = args.gemm
+        gemm = {"M": G, "N": G, "K": G}
+        GA, GB = DeviceBuffer(G * G * 4), DeviceBuffer(G * G * 4)
+        C.call("tomos_fill_normal_f32", GA.ptr, G, G, G, 11, None, 0, 0, None)
+        C.call("tomos_fill_normal_f32", GB.ptr, G, G, G, 12, None, 0, 0, None)
+        planes = [DeviceBuffer(G * G * 2) for _ in range(4)]
+        Cg = DeviceBuffer(G * G * 4)
+        gsz = C.lib().tomos_gemm_split_scratch_size(G, G, G)
+        gscr = DeviceBuffer(gsz)
+        cols = [0, 1, G // 2, G - 1]
+        for name, at, var in (("fp32_bf16", C.FP32_BF16, C.PACK_FP32_BF16),
+                              ("fp32_16", C.FP32_16, C.PACK_FP32_16)):
+            C.call("tomos_pack", var, G, G, GA.ptr, G, planes[0].ptr, planes[1].ptr, G, None)
+            C.call("tomos_pack", var, G, G, GB.ptr, G, planes[2].ptr, planes[3].ptr, G, None)
+            args_g = (at, G, G, G, 1.0, planes[0].ptr, planes[1].ptr, G, planes[2].ptr,
+                      planes[3].ptr, G, 0.0, Cg.ptr, G, gscr.ptr, gsz, None)
+            C.call("tomos_gemm_split", *args_g)
+            C.call("tomos_event_record", ev0, None)
+            reps = 3
+            for _ in range(reps):
+                C.call("tomos_gemm_split", *args_g)
+            C.call("tomos_event_record", ev1, None)
+            f = ctypes.c_float()
+            C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+            ms = f.value / reps
+            useful = 2.0 * G * G * G / (ms * 1e-3) / 1e12
+            # FP64-accumulated reference columns: C[:, j] = A_exact @ B_exact[:, j]
+            dec = C.DECODE_FP32_BF16_EXACT if at == C.FP32_BF16 else C.DECODE_FP32_16_EXACT
+            errs = []
+            for j in cols:
+                bcol = DeviceBuffer(G * 4)
+                C.call("tomos_unpack", dec, G, 1, planes[2].ptr + j * G * 2, planes[3].ptr + j * G * 2,
+                       G, bcol.ptr, G, None)
+                refc = DeviceBuffer(G * 8)
+                C.call("tomos_gemv", at, C.TIER_EXACT, 0, 1.0, G, G, planes[0].ptr, planes[1].ptr, G,
+                       C.F32, bcol.ptr, 0.0, C.F64, refc.ptr, None, 0, None)
+                rc_ = refc.to_numpy(np.float64, (G,))
+                gc_ = np.empty(G, np.float32)
+                C.call("tomos_memcpy_dtoh", gc_.ctypes.data, Cg.ptr + j * G * 4, G * 4, None)
+                errs.append(np.linalg.norm(gc_ - rc_) / np.linalg.norm(rc_))
+            gemm[name] = {"ms": round(ms, 3), "useful_TFLOPs": round(useful, 1),
+                          "mfma_TFLOPs": round(3 * useful, 1),
+                          "mfma_frac_of_2500": round(3 * useful / 2500.0, 4),
+                          "l2_rel_err_sampled_cols": float(max(errs))}
+        log("split gemm: %s" % gemm)
+        del GA, GB, planes, Cg, gscr
+
     # ---- CPU baseline: the oracle's OpenBLAS FP32 LSQR on a row sample -------
     cpu = None
     if not args.no_cpu:
@@ -283,6 +372,7 @@ def main():
         "speedup_vs_plain_fp32": speedup,
         "speedup_is_lower_bound": bool(plain and not plain["converged"]),
         "gemv": gemv_detail,
+        "gemm_split": gemm,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -174,6 +174,23 @@ int tomos_gemm(int a_type, int tier, int trans_a, int trans_b, double alpha, int
                const float* b, int64_t ldb, double beta, float* c, int64_t ldc,
                tomos_stream_t stream);
 
+/* Split-precision packed GEMM on the matrix cores:
+ *   C = alpha * A B + beta * C,  A (M x K, lda) and B (K x N, ldb) both
+ *   packed as a_type (FP32_16 or FP32_BF16 planes, column-major), C FP32.
+ * Each operand is decoded exactly and split into hi = rn16(a) and
+ * lo = rn16(a - hi) 16-bit planes (bf16 for FP32_BF16; fp16 with a
+ * power-of-two operand scale for FP32_16); the product is
+ * A_hi B_hi + A_hi B_lo + A_lo B_hi with FP32 accumulation (bf16/fp16
+ * MFMA 32x32x16).  No reference counterpart (the reference's 16-bit packed
+ * GEMMs use the hi word only, mm_cuda_kernels.cu:35372, :37826).
+ * scratch: tomos_gemm_split_scratch_size() bytes or NULL. */
+size_t tomos_gemm_split_scratch_size(int64_t M, int64_t N, int64_t K);
+int tomos_gemm_split(int a_type, int64_t M, int64_t N, int64_t K, double alpha,
+                     const uint16_t* a_wh, const uint16_t* a_wr, int64_t lda,
+                     const uint16_t* b_wh, const uint16_t* b_wr, int64_t ldb, double beta,
+                     float* c, int64_t ldc, void* scratch, size_t scratch_bytes,
+                     tomos_stream_t stream);
+
 /* ---- CountSketch ------------------------------------------------------ */
 /* bucket[i] = splitmix64(i ^ 0x12345678abcdef00) mod s,
  * sign[i]   = bit0(splitmix64(i ^ 0xf00dd00d42babe55)) ? +1 : -1,

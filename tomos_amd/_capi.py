@@ -83,6 +83,9 @@ _SIGS = {
                           vp, vp, sz, vp]),
     "tomos_gemm": (cint, [cint, cint, cint, cint, dbl, i64, i64, i64, vp, vp, i64, vp, i64,
                           dbl, vp, i64, vp]),
+    "tomos_gemm_split_scratch_size": (sz, [i64, i64, i64]),
+    "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
+                                sz, vp]),
     "tomos_countsketch_indices": (cint, [i64, i64, vp, vp, vp]),
     "tomos_countsketch_scratch_size": (sz, [i64, i64, i64]),
     "tomos_countsketch_apply": (cint, [cint, cint, i64, i64, vp, vp, i64, i64, vp, i64, vp, sz,

@@ -59,50 +59,83 @@ __global__ __launch_bounds__(256) void absmax_kernel(int fmt, int64_t rows, int6
     if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(mx));
 }
 
-// Column-major packed X (rows x cols, ld) -> hi/lo planes laid out as
-// out[col][row] (row contiguous) if transpose_rows == false, i.e. the
-// operand's K index must be the contiguous one:
-//   A (M x K): K is the column index -> out[m][k] needs a transpose (kmajor=true)
-//   B (K x N): K is the row index    -> out[n][k] is the natural layout
-// hi/lo are bf16 (fmt FP32_BF16) or fp16 of x * 2^scale (fmt FP32_16).
+__device__ __forceinline__ int fp16_scale_exp(int fmt, const unsigned* absmax_bits) {
+    // power-of-two scale bringing max |x| to about 2^13 (fp16 max is 65504)
+    if (fmt != TOMOS_FP32_16) return 0;
+    const float mx = __uint_as_float(*absmax_bits);
+    int e = 0;
+    frexpf(mx > 0.0f ? mx : 1.0f, &e);
+    return 13 - e;
+}
+
+__device__ __forceinline__ void split16(int fmt, float v, int sc, uint16_t& h16, uint16_t& l16) {
+    if (fmt == TOMOS_FP32_16) {
+        const float x = ldexpf(v, sc);
+        const _Float16 h = static_cast<_Float16>(x);
+        const _Float16 l = static_cast<_Float16>(x - static_cast<float>(h));
+        h16 = __builtin_bit_cast(uint16_t, h);
+        l16 = __builtin_bit_cast(uint16_t, l);
+    } else {
+        h16 = bf16_rn(v);
+        l16 = bf16_rn(v - bf16_val(h16));
+    }
+}
+
+// B operand (K x N column-major, K contiguous already): out[n][k], n < Np,
+// k < Kp, zero outside the operand.  hi/lo are bf16 (FP32_BF16) or fp16 of
+// x * 2^scale (FP32_16).
 __global__ __launch_bounds__(256) void split_planes_kernel(
     int fmt, int64_t rows, int64_t cols, const uint16_t* __restrict__ wh,
-    const uint16_t* __restrict__ wr, int64_t ld, bool k_is_col, int64_t out_ld,
-    uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits,
-    int64_t pad_rows, int64_t pad_k) {
-    // scale exponent for fp16: bring max |x| to about 2^13
-    int sc = 0;
-    if (fmt == TOMOS_FP32_16) {
-        const float mx = __uint_as_float(*absmax_bits);
-        int e = 0;
-        frexpf(mx > 0.0f ? mx : 1.0f, &e);
-        sc = 13 - e;
-    }
-    // out index space: R (non-K extent, padded) x KK (K extent, padded)
-    const int64_t R = k_is_col ? pad_rows : pad_rows;  // output "row" count
-    const int64_t KK = pad_k;
-    const int64_t total = R * KK;
+    const uint16_t* __restrict__ wr, int64_t ld, int64_t out_ld, uint16_t* __restrict__ hi,
+    uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits, int64_t pad_cols,
+    int64_t pad_k) {
+    const int sc = fp16_scale_exp(fmt, absmax_bits);
+    const int64_t total = pad_cols * pad_k;
     for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < total;
          t += static_cast<int64_t>(gridDim.x) * 256) {
-        const int64_t r = t / KK, k = t - r * KK;  // out[r][k]
-        // source element: A(m=r, k) at wh[r + k*ld]; B(k, n=r) at wh[k + r*ld]
-        const int64_t si = k_is_col ? r : k;
-        const int64_t sj = k_is_col ? k : r;
+        const int64_t r = t / pad_k, k = t - r * pad_k;  // out[r][k] = X(k, r)
         float v = 0.0f;
-        if (si < rows && sj < cols) v = decode_exact(fmt, wh[si + sj * ld], wr[si + sj * ld]);
+        if (k < rows && r < cols) v = decode_exact(fmt, wh[k + r * ld], wr[k + r * ld]);
         uint16_t h16, l16;
-        if (fmt == TOMOS_FP32_16) {
-            const float x = ldexpf(v, sc);
-            const _Float16 h = static_cast<_Float16>(x);
-            const _Float16 l = static_cast<_Float16>(x - static_cast<float>(h));
-            h16 = __builtin_bit_cast(uint16_t, h);
-            l16 = __builtin_bit_cast(uint16_t, l);
-        } else {
-            h16 = bf16_rn(v);
-            l16 = bf16_rn(v - bf16_val(h16));
-        }
+        split16(fmt, v, sc, h16, l16);
         hi[r * out_ld + k] = h16;
         lo[r * out_ld + k] = l16;
+    }
+}
+
+// A operand (M x K column-major, M contiguous): out[m][k] needs a transpose,
+// done per 64 x 64 tile through LDS so both the reads (m fastest) and the
+// writes (k fastest) are contiguous.  grid = (ceil(pad_k/64), ceil(pad_m/64)).
+__global__ __launch_bounds__(256) void split_planes_t_kernel(
+    int fmt, int64_t rows, int64_t cols, const uint16_t* __restrict__ wh,
+    const uint16_t* __restrict__ wr, int64_t ld, int64_t out_ld, uint16_t* __restrict__ hi,
+    uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits, int64_t pad_m,
+    int64_t pad_k) {
+    __shared__ uint16_t th[64][65];
+    __shared__ uint16_t tl[64][65];
+    const int sc = fp16_scale_exp(fmt, absmax_bits);
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64, m0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int tid = threadIdx.x;
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int idx = tid + it * 256, i = idx & 63, j = idx >> 6;  // i: m, j: k
+        const int64_t m = m0 + i, k = k0 + j;
+        float v = 0.0f;
+        if (m < rows && k < cols) v = decode_exact(fmt, wh[m + k * ld], wr[m + k * ld]);
+        uint16_t h16, l16;
+        split16(fmt, v, sc, h16, l16);
+        th[j][i] = h16;
+        tl[j][i] = l16;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int idx = tid + it * 256, j = idx & 63, i = idx >> 6;  // j: k, i: m
+        const int64_t m = m0 + i, k = k0 + j;
+        if (m < pad_m && k < pad_k) {
+            hi[m * out_ld + k] = th[j][i];
+            lo[m * out_ld + k] = tl[j][i];
+        }
     }
 }
 
@@ -220,7 +253,8 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(
 }  // namespace
 
 size_t gemm_split_scratch_bytes(int64_t M, int64_t N, int64_t K) {
-    const int64_t Mp = ceil_div(M, BM) * BM, Np = ceil_div(N, BN) * BN, Kp = ceil_div(K, BK) * BK;
+    const int64_t Mp = ceil_div(M, BM) * BM, Np = ceil_div(N, BN) * BN;
+    const int64_t Kp = std::max<int64_t>(ceil_div(K, BK) * BK, BK);
     return static_cast<size_t>(2 * (Mp + Np) * Kp) * 2 + 256;
 }
 
@@ -250,10 +284,11 @@ void gemm_split(int a_type, int64_t M, int64_t N, int64_t K, double alpha, const
         absmax_kernel<<<1024, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, absmax + 1);
         TOMOS_LAUNCH_CHECK("absmax_kernel");
     }
-    const unsigned blocksA = static_cast<unsigned>(std::min<int64_t>(ceil_div(Mp * Kp, 256), 65536));
+    require(ceil_div(Mp, 64) < 65536, "gemm_split: M too large");
+    const dim3 gridA(static_cast<unsigned>(ceil_div(Kp, 64)), static_cast<unsigned>(ceil_div(Mp, 64)));
     const unsigned blocksB = static_cast<unsigned>(std::min<int64_t>(ceil_div(Np * Kp, 256), 65536));
-    split_planes_kernel<<<blocksA, 256, 0, s>>>(a_type, M, K, awh, awr, lda, true, Kp, Ah, Al, absmax, Mp, Kp);
-    split_planes_kernel<<<blocksB, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, false, Kp, Bh, Bl, absmax + 1, Np, Kp);
+    split_planes_t_kernel<<<gridA, 256, 0, s>>>(a_type, M, K, awh, awr, lda, Kp, Ah, Al, absmax, Mp, Kp);
+    split_planes_kernel<<<blocksB, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, Kp, Bh, Bl, absmax + 1, Np, Kp);
     TOMOS_LAUNCH_CHECK("split_planes_kernel");
     const int64_t nblocks = (Mp / BM) * (Np / BN);
     require(nblocks < (int64_t(1) << 31), "gemm_split: grid too large");
