@@ -262,12 +262,11 @@ def main():
     }
     log("gemv: %s" % gemv_detail)
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "gemv_bf16_hi_2e6x1000_traffic.json")
-    if os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    tf = os.path.join(ROOT, "profiles", "gemv_bf16_hi_2e6x1000_traffic.md")
+    if os.path.exists(tf) and (m, n) == (2_000_000, 1000):
+        for line in open(tf):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
 
     # ---- split-precision MFMA GEMM (C3 square shape) -------------------------
     gemm = None
