@@ -11,8 +11,9 @@ exact (wh+wr) GEMV, to relative residual 1e-6.  All inputs are resident in
 HBM before timing.  Plain FP32 LSQR (FP32 A, no preconditioner, same
 stopping rule) is timed once beside it, capped at --plain-cap iterations.
 
-The roofline object reports the hi-only FP32_BF16 GEMV y = A x at the full
-2e6 x 1000 shape, timed with HIP events on the launch stream.  The CPU
+The roofline object reports the LSQR iteration's dominant kernel, the
+single-pass A p / A^T u kernel on the hi plane (tomos_matvec_pair) at the
+full 2e6 x 1000 shape, timed with HIP events on the launch stream.  The CPU
 baseline is the oracle's FP32 LSQR with OpenBLAS sgemv (the reference's CPU
 path) on a row sample of the same matrix, scaled to the full row count.
 """
@@ -63,6 +64,28 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1):
+    """Average ms of tomos_matvec_pair (u <- A p - 0.5 u, t <- A^T u) on the
+    hi plane, over --gemv-reps launches on the launch stream."""
+    rng = np.random.default_rng(4)
+    pd = DeviceBuffer.from_numpy(rng.standard_normal(n))
+    ud = DeviceBuffer.from_numpy(rng.standard_normal(m).astype(np.float32))
+    td, nd = DeviceBuffer(n * 8), DeviceBuffer(8)
+    sz = C.lib().tomos_matvec_pair_scratch_size(C.FP32_BF16, m, n)
+    sc = DeviceBuffer(sz)
+    call = lambda: C.call("tomos_matvec_pair", C.FP32_BF16, m, n, wh.ptr, m, pd.ptr, -0.5, ud.ptr,
+                          td.ptr, nd.ptr, sc.ptr, sz, None)
+    for _ in range(2):
+        call()
+    C.call("tomos_event_record", ev0, None)
+    for _ in range(args.gemv_reps):
+        call()
+    C.call("tomos_event_record", ev1, None)
+    f = ctypes.c_float()
+    C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+    return f.value / args.gemv_reps
+
+
 def kernels_only(args, C, DeviceBuffer, m, n, wh):
     """GEMV-N (hi-only FP32_BF16, y = A x) and tomos_copy_kernel over the
     same 2*m*n bytes, each launched --gemv-reps times: the copy's counter
@@ -90,6 +113,8 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
         C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
         out[name] = {"ms": round(f.value / args.gemv_reps, 4), "bytes": nbytes,
                      "launches": args.gemv_reps + 1}
+    out["matvec_pair_bf16_hi"] = {"ms": round(time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1), 4),
+                                  "bytes": 2 * m * n + 8 * m + 16 * n, "launches": args.gemv_reps + 2}
     print(json.dumps({"kernels_only": out, "m": m, "n": n}), flush=True)
 
 
@@ -252,9 +277,15 @@ def main():
     t_f = time_gemv(0, C.F32, 0, A.ptr, None)
     t_ft = time_gemv(1, C.F32, 0, A.ptr, None)
     gbs = lambda byts, ms: byts / (ms * 1e-3) / 1e9
-    achieved = gbs(gemv_bytes_hi, t_n)
+
+    # the LSQR iteration's dominant kernel: A p and A^T u from one read of the
+    # hi plane (tomos_matvec_pair = lsqr_fused_kernel + its partial sums)
+    t_pair = time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1)
+    pair_bytes = 2.0 * m * n + 8.0 * m + 16.0 * n  # hi plane, u read + written, p and t in FP64
+    achieved = gbs(pair_bytes, t_pair)
     gemv_detail = {
-        "bf16_hi_N": {"ms": round(t_n, 4), "GBps": round(achieved, 1)},
+        "pair_bf16_hi_Ap_ATu": {"ms": round(t_pair, 4), "GBps": round(achieved, 1)},
+        "bf16_hi_N": {"ms": round(t_n, 4), "GBps": round(gbs(gemv_bytes_hi, t_n), 1)},
         "bf16_hi_T": {"ms": round(t_t, 4), "GBps": round(gbs(gemv_bytes_hi, t_t), 1)},
         "bf16_exact_N": {"ms": round(t_nx, 4), "GBps": round(gbs(gemv_bytes_f32, t_nx), 1)},
         "fp32_N": {"ms": round(t_f, 4), "GBps": round(gbs(gemv_bytes_f32, t_f), 1)},
@@ -262,7 +293,7 @@ def main():
     }
     log("gemv: %s" % gemv_detail)
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "gemv_bf16_hi_2e6x1000_traffic.md")
+    tf = os.path.join(ROOT, "profiles", "lsqr_fused_bf16_2e6x1000_traffic.md")
     if os.path.exists(tf) and (m, n) == (2_000_000, 1000):
         for line in open(tf):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -375,8 +406,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "gemv_n_kernel<FP32_BF16 hi-only> y=A x, %dx%d, %.3f GB algorithmic per launch"
-                               % (m, n, gemv_bytes_hi / 1e9)},
+                     "kernel": "lsqr_fused_kernel<FP32_BF16 hi-only> (tomos_matvec_pair: u = A p + c u, "
+                               "t = A^T u from one read of A), %dx%d, %.3f GB algorithmic per launch"
+                               % (m, n, pair_bytes / 1e9)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -191,6 +191,30 @@ int tomos_gemm_split(int a_type, int64_t M, int64_t N, int64_t K, double alpha,
                      float* c, int64_t ldc, void* scratch, size_t scratch_bytes,
                      tomos_stream_t stream);
 
+/* ---- LSQR's matvec pair from one read of A ---------------------------- */
+/* u <- A p + u_coef * u  (all m rows), then t <- A^T u and *unorm2 <- |u|^2,
+ * with A read from HBM once: the two mv_apply calls of one LSQR step
+ * (src/core/solver/lsqr.cpp:560-577, A*(P v) - alpha*u followed by A^T u)
+ * with the 1/beta scaling left to the caller (A^T (u/beta) = (A^T u)/beta).
+ * A: F32, FP32_16 or FP32_BF16 (hi plane only: the per-iteration tier),
+ * column-major, 16-byte aligned, lda a multiple of 4 (F32) / 8 (packed),
+ * n <= 2048; otherwise TOMOS_ERR_UNSUPPORTED.  p, t: FP64 (n); u: FP32 (m);
+ * unorm2: one FP64 in device memory, or NULL.  Deterministic (fixed
+ * reduction order).  scratch: tomos_matvec_pair_scratch_size() bytes or NULL. */
+size_t tomos_matvec_pair_scratch_size(int a_type, int64_t m, int64_t n);
+int tomos_matvec_pair(int a_type, int64_t m, int64_t n, const void* a, int64_t lda,
+                      const double* p, double u_coef, float* u, double* t, double* unorm2,
+                      void* scratch, size_t scratch_bytes, tomos_stream_t stream);
+
+/* Tuned launch shape of the single-pass kernel behind tomos_matvec_pair and
+ * the fused LSQR iteration, per (a_type, n): row_groups (2, 4 or 8 lanes
+ * share one column of a row tile; 0 = default) and blocks_per_cu (grid
+ * size; 0 = as many as are resident at once).  Process-wide; consulted at
+ * every launch.  Scratch sizes do not depend on it.  Set by the autotuner
+ * (tomos_amd/autotune.py); no reference counterpart. */
+int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu);
+int tomos_fused_config_clear(void);
+
 /* ---- CountSketch ------------------------------------------------------ */
 /* bucket[i] = splitmix64(i ^ 0x12345678abcdef00) mod s,
  * sign[i]   = bit0(splitmix64(i ^ 0xf00dd00d42babe55)) ? +1 : -1,

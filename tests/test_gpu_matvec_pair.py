@@ -1,0 +1,88 @@
+"""tomos_matvec_pair: u <- A p + c u, t <- A^T u, |u|^2 from one read of A,
+vs FP64 products of the decoded (hi-only) matrix, over every launch shape
+the tuner can pick (row groups 2/4/8, blocks per CU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import hipcall as H
+from tomos_amd import _capi as C
+from tomos_amd._capi import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+HI_DECODER = {C.FP32_BF16: C.DECODE_FP32_BF16_HI, C.FP32_16: C.DECODE_FP32_16_HALF}
+PACKER = {C.FP32_BF16: C.PACK_FP32_BF16, C.FP32_16: C.PACK_FP32_16}
+MAX_N = {8: 512, 4: 1024, 2: 2048, 0: 2048}
+
+
+def run_pair(a_type, A, lda, p, u0, coef):
+    m, n = A.shape
+    if a_type == C.F32:
+        Ap = np.zeros((lda, n), np.float32, order="F")
+        Ap[:m] = A
+        abuf, keep = H.dev(Ap), None
+    else:
+        abuf, keep = H.pack(PACKER[a_type], A, ld=lda)
+    pd, ud = H.dev(p), H.dev(u0)
+    td, nd = DeviceBuffer(n * 8), DeviceBuffer(8)
+    C.call("tomos_matvec_pair", a_type, m, n, abuf.ptr, lda, pd.ptr, coef, ud.ptr, td.ptr, nd.ptr,
+           None, 0, None)
+    return ud.to_numpy(np.float32, (m,)), td.to_numpy(np.float64, (n,)), nd.to_numpy(np.float64, (1,))[0]
+
+
+def used_matrix(orc, a_type, A):
+    if a_type == C.F32:
+        return A.astype(np.float64)
+    eh, er = orc.pack(PACKER[a_type], A)
+    return orc.unpack(HI_DECODER[a_type], eh, er).astype(np.float64)
+
+
+@pytest.mark.parametrize("a_type", [C.F32, C.FP32_BF16, C.FP32_16])
+@pytest.mark.parametrize("m,lda,n", [(1000, 1000, 37), (4099, 4104, 1000), (777, 784, 64),
+                                      (20000, 20000, 1500), (3001, 3008, 2048), (64, 64, 512)])
+@pytest.mark.parametrize("groups,bpc", [(0, 0), (2, 0), (4, 1), (8, 0), (4, 3)])
+def test_pair_vs_fp64(capi, orc, a_type, m, lda, n, groups, bpc):
+    if n > MAX_N[groups]:
+        pytest.skip("no kernel for %d row groups at n = %d" % (groups, n))
+    rng = np.random.default_rng(m + n + groups)
+    A = (rng.standard_normal((m, n)) * 2.0 ** rng.integers(-8, 9, (m, n))).astype(np.float32, order="F")
+    p = rng.standard_normal(n)
+    u0 = rng.standard_normal(m).astype(np.float32)
+    coef = -0.37
+    C.call("tomos_fused_config_set", a_type, n, groups, bpc)
+    try:
+        u, t, nrm2 = run_pair(a_type, A, lda, p, u0, coef)
+    finally:
+        C.call("tomos_fused_config_clear")
+    Au = used_matrix(orc, a_type, A)
+    u_ref = Au @ p.astype(np.float32).astype(np.float64) + coef * u0.astype(np.float64)
+    # row sums accumulate n products in FP32: bound relative to sum |a_ij p_j|
+    scale = np.abs(Au) @ np.abs(p) + np.abs(coef * u0)
+    assert np.max(np.abs(u - u_ref) / scale) < 4e-6
+    ud = u.astype(np.float64)
+    t_ref = Au.T @ ud
+    tscale = np.abs(Au.T) @ np.abs(ud)
+    assert np.max(np.abs(t - t_ref) / tscale) < 4e-6
+    assert abs(nrm2 - ud @ ud) <= 1e-12 * (ud @ ud)
+
+
+def test_pair_deterministic_and_errors(capi):
+    rng = np.random.default_rng(1)
+    m, n = 50000, 700
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    p = rng.standard_normal(n)
+    u0 = rng.standard_normal(m).astype(np.float32)
+    r1 = run_pair(C.FP32_BF16, A, m, p, u0, 0.5)
+    r2 = run_pair(C.FP32_BF16, A, m, p, u0, 0.5)
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1]) and r1[2] == r2[2]
+    # unsupported: lda not a multiple of the 16-byte vector, n too large, bad config
+    wh, _ = H.pack(C.PACK_FP32_BF16, A[:, :10], ld=m + 1)
+    pd, ud, td = H.dev(p[:10]), H.dev(u0), DeviceBuffer(80)
+    rc = C.lib().tomos_matvec_pair(C.FP32_BF16, m, 10, ctypes.c_void_p(wh.ptr), m + 1, ctypes.c_void_p(pd.ptr),
+                                   0.0, ctypes.c_void_p(ud.ptr), ctypes.c_void_p(td.ptr), None, None, 0, None)
+    assert rc == 3  # TOMOS_ERR_UNSUPPORTED
+    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 1000, 8, 0) == 1  # G=8 needs n <= 512
+    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 3, 0) == 1
+    assert C.lib().tomos_fused_config_set(C.F64, 100, 4, 0) == 1

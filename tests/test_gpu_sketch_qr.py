@@ -17,21 +17,28 @@ def test_indices_bit_exact(capi, orc, m, s):
 
 @pytest.mark.parametrize("a_type,tier", [(C.F64, 0), (C.F32, 0), (C.FP32_BF16, 2), (C.FP32_BF16, 0),
                                          (C.FP32_16, 2), (C.FP32_16, 0)])
-def test_apply_vs_oracle(capi, orc, a_type, tier):
-    m, n, s = 20011, 37, 300
+@pytest.mark.parametrize("m,lda,n,s", [
+    (20011, 20011, 37, 300),     # odd lda: element-wise kernel
+    (40000, 40000, 37, 300),     # 16-byte row vectors, full column groups + a partial one
+    (40003, 40008, 13, 4000),    # ragged row tail inside the last split, padded lda
+    (100000, 100000, 9, 20000),  # large s: one column per group
+])
+def test_apply_vs_oracle(capi, orc, a_type, tier, m, lda, n, s):
     rng = np.random.default_rng(9)
     A = rng.standard_normal((m, n)).astype(np.float32, order="F")
     if a_type in (C.FP32_BF16, C.FP32_16):
         variant = C.PACK_FP32_BF16 if a_type == C.FP32_BF16 else C.PACK_FP32_16
-        wh, wr = H.pack(variant, A)
+        wh, wr = H.pack(variant, A, ld=lda)
         dec = {(C.FP32_BF16, 2): 4, (C.FP32_BF16, 0): 5, (C.FP32_16, 2): 0, (C.FP32_16, 0): 1}[(a_type, tier)]
         eh, er = orc.pack(0 if a_type == C.FP32_16 else 3, A)
         Aref = orc.unpack(dec, eh, er).astype(np.float64)
-        Y = H.countsketch_apply(a_type, tier, m, n, wh, wr, m, s)
+        Y = H.countsketch_apply(a_type, tier, m, n, wh, wr, lda, s)
     else:
-        Ad = H.dev(A.astype(np.float64) if a_type == C.F64 else A)
+        dt = np.float64 if a_type == C.F64 else np.float32
+        Ap = np.zeros((lda, n), dt, order="F")
+        Ap[:m] = A
         Aref = A.astype(np.float64)
-        Y = H.countsketch_apply(a_type, tier, m, n, Ad, None, m, s)
+        Y = H.countsketch_apply(a_type, tier, m, n, H.dev(Ap), None, lda, s)
     ref = orc.countsketch_apply(Aref, s)
     err = np.abs(Y - ref).max() / np.abs(ref).max()
     assert err < 2e-6, err
@@ -51,7 +58,7 @@ def test_norm_preservation(capi):
     assert np.allclose(Y2, 2 * Y + Y1, rtol=0, atol=2e-5 * np.abs(Y2).max())
 
 
-@pytest.mark.parametrize("k,n", [(4000, 1000), (30, 5), (257, 256), (64, 64)])
+@pytest.mark.parametrize("k,n", [(4000, 1000), (30, 5), (257, 256), (64, 64), (500, 77), (33, 33), (1000, 320)])
 def test_qr_r_matches_numpy(capi, k, n):
     rng = np.random.default_rng(k + n)
     A = rng.standard_normal((k, n)) * np.logspace(0, -6, n)
@@ -61,6 +68,20 @@ def test_qr_r_matches_numpy(capi, k, n):
     assert np.allclose(np.tril(R, -1), 0)
     err = np.abs(R - sgn[:, None] * Rn).max() / np.abs(Rn).max()
     assert err < 1e-12, err
+
+
+def test_qr_zero_column(capi):
+    """A zero column gives an inactive reflector (tau = 0, R(c,c) = 0), as dlarfg."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((100, 40))
+    A[:, 5] = 0.0
+    A[:, 33] = 0.0
+    R = H.geqrf_r(A)
+    Rn = np.linalg.qr(A, mode="r")
+    sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
+    sgn[sgn == 0] = 1.0
+    assert R[5, 5] == 0.0 and R[33, 33] == 0.0
+    assert np.abs(R - sgn[:, None] * Rn).max() < 1e-12 * np.abs(Rn).max()
 
 
 def test_trtri_and_precond(capi):

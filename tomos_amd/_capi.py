@@ -84,6 +84,10 @@ _SIGS = {
     "tomos_gemm": (cint, [cint, cint, cint, cint, dbl, i64, i64, i64, vp, vp, i64, vp, i64,
                           dbl, vp, i64, vp]),
     "tomos_gemm_split_scratch_size": (sz, [i64, i64, i64]),
+    "tomos_matvec_pair_scratch_size": (sz, [cint, i64, i64]),
+    "tomos_fused_config_set": (cint, [cint, i64, cint, cint]),
+    "tomos_fused_config_clear": (cint, []),
+    "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
                                 sz, vp]),
     "tomos_countsketch_indices": (cint, [i64, i64, vp, vp, vp]),

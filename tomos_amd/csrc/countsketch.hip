@@ -5,15 +5,19 @@
 // (src/kernels/cpu/sketch/countsketch_kernels.cpp:59-85).
 //
 // Apply: A (m x n, column-major, any storage format/tier) is streamed once
-// in its natural order.  A block owns (row split r, column group of CG
-// columns); its lanes read 16-byte row vectors of each column (coalesced),
-// look up the packed bucket|sign word of each row (shared by every column
-// group of the same row split, so it is L2-resident), and add
-// sign*A[i,j] into an LDS copy of its s x CG slice of Y.  The LDS slice is
-// written to a per-split partial, and a second pass sums the splits in a
-// fixed order in FP64.  Bucket/sign indices are bit-exact; the values are
-// FP32 sums within a split (order depends on LDS atomic order), FP64 across
-// splits.
+// in its natural order.  A block (1024 threads) owns (row split r, column
+// group of CG columns) and keeps its CG x s slice of Y in LDS (column-major,
+// so the random buckets of a wave spread over the LDS banks).  Each lane
+// takes one 16-byte vector of consecutive rows (8 packed / 4 FP32 / 2 FP64
+// rows), loads their packed bucket|sign words once, then for each of the CG
+// columns loads the same rows (hi plane, plus lo plane for the exact tier)
+// and adds sign*A[i,j] into LDS: CG (or 2 CG) independent 16-byte loads per
+// lane are in flight, and the index words are read once per column group.
+// The LDS slice is written to a per-split partial, and a second pass sums
+// the splits in a fixed order in FP64.  Bucket/sign indices are bit-exact;
+// the values are FP32 sums within a split (order depends on LDS atomic
+// order), FP64 across splits.  A, lda or the split start not 16-byte
+// aligned takes the element-wise kernel.
 #include <algorithm>
 
 #include "codec.hpp"
@@ -107,6 +111,125 @@ __global__ __launch_bounds__(256) void cs_apply_kernel(int64_t m, int64_t n,
     }
 }
 
+template <int FMT>
+constexpr int vec_rows() {
+    return FMT == FMT_F64 ? 2 : (FMT == FMT_F32 ? 4 : 8);
+}
+
+template <int FMT, bool EXACT>
+__device__ __forceinline__ void decode_rows(const uint4& h, const uint4& l,
+                                            float (&v)[vec_rows<FMT>()]) {
+    if constexpr (FMT == FMT_F64) {
+        v[0] = static_cast<float>(__builtin_bit_cast(double, (static_cast<uint64_t>(h.y) << 32) | h.x));
+        v[1] = static_cast<float>(__builtin_bit_cast(double, (static_cast<uint64_t>(h.w) << 32) | h.z));
+    } else if constexpr (FMT == FMT_F32) {
+        v[0] = __uint_as_float(h.x);
+        v[1] = __uint_as_float(h.y);
+        v[2] = __uint_as_float(h.z);
+        v[3] = __uint_as_float(h.w);
+    } else {
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
+        const uint32_t lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const uint32_t hh = half ? (hw[k] >> 16) : (hw[k] & 0xFFFFu);
+                const uint32_t ll = half ? (lw[k] >> 16) : (lw[k] & 0xFFFFu);
+                float x;
+                if constexpr (FMT == FMT_P16) {
+                    if constexpr (EXACT) x = codec::fp32_16_decode_exact(hh, ll);
+                    else x = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(codec::fp32_16_half_bits(hh))));
+                } else {
+                    x = EXACT ? __uint_as_float((hh << 16) | ll) : __uint_as_float(hh << 16);
+                }
+                v[2 * k + half] = x;
+            }
+        }
+    }
+}
+
+constexpr int kVecThreads = 1024;
+
+// Vector path: rows_per_split, lda and the base pointers are multiples of
+// the 16-byte row vector (checked by the host).
+template <int FMT, bool EXACT, int CG>
+__global__ __launch_bounds__(kVecThreads) void cs_apply_vec_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
+    int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
+    float* __restrict__ partial) {
+    constexpr int V = vec_rows<FMT>();
+    constexpr int EB = 16 / V;  // bytes per element of the a plane
+    extern __shared__ float ylds[];  // [CG][s]
+    const int split = blockIdx.x / ncg;
+    const int cg = blockIdx.x - split * ncg;
+    const int64_t c0 = static_cast<int64_t>(cg) * CG;
+    const int ncols = static_cast<int>(min<int64_t>(CG, n - c0));
+    for (int64_t t = threadIdx.x; t < s * CG; t += kVecThreads) ylds[t] = 0.0f;
+    __syncthreads();
+
+    const int64_t r0 = static_cast<int64_t>(split) * rows_per_split;
+    const int64_t r1 = min(m, r0 + rows_per_split);
+    const char* abase = static_cast<const char*>(a);
+    for (int64_t i0 = r0 + static_cast<int64_t>(threadIdx.x) * V; i0 < r1;
+         i0 += static_cast<int64_t>(kVecThreads) * V) {
+        if (i0 + V <= r1) {
+            uint32_t w[V];
+            if constexpr (V == 8) {
+                const uint4 p0 = *reinterpret_cast<const uint4*>(packed + i0);
+                const uint4 p1 = *reinterpret_cast<const uint4*>(packed + i0 + 4);
+                w[0] = p0.x; w[1] = p0.y; w[2] = p0.z; w[3] = p0.w;
+                w[4] = p1.x; w[5] = p1.y; w[6] = p1.z; w[7] = p1.w;
+            } else if constexpr (V == 4) {
+                const uint4 p0 = *reinterpret_cast<const uint4*>(packed + i0);
+                w[0] = p0.x; w[1] = p0.y; w[2] = p0.z; w[3] = p0.w;
+            } else {
+                const uint2 p0 = *reinterpret_cast<const uint2*>(packed + i0);
+                w[0] = p0.x; w[1] = p0.y;
+            }
+            uint4 h[CG], l[CG];
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                if (c < ncols) {
+                    const int64_t e = i0 + (c0 + c) * lda;
+                    h[c] = *reinterpret_cast<const uint4*>(abase + e * EB);
+                    if constexpr (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF))
+                        l[c] = *reinterpret_cast<const uint4*>(wr + e);
+                    else
+                        l[c] = make_uint4(0, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                if (c < ncols) {
+                    float v[V];
+                    decode_rows<FMT, EXACT>(h[c], l[c], v);
+#pragma unroll
+                    for (int r = 0; r < V; ++r) {
+                        const int b = static_cast<int>(w[r] & 0x7FFFFFFFu);
+                        atomicAdd(&ylds[c * s + b], (w[r] & 0x80000000u) ? -v[r] : v[r]);
+                    }
+                }
+            }
+        } else {
+            for (int64_t i = i0; i < r1; ++i) {
+                const uint32_t w = packed[i];
+                const int b = static_cast<int>(w & 0x7FFFFFFFu);
+                for (int c = 0; c < ncols; ++c) {
+                    const float v = load_elem<FMT, EXACT>(a, wr, i + (c0 + c) * lda);
+                    atomicAdd(&ylds[c * s + b], (w & 0x80000000u) ? -v : v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float* out = partial + static_cast<int64_t>(split) * s * n;
+    for (int64_t t = threadIdx.x; t < s * ncols; t += kVecThreads) {
+        const int64_t c = t / s;
+        out[(c0 + c) * s + (t - c * s)] = ylds[t];
+    }
+}
+
 __global__ __launch_bounds__(256) void cs_finish_kernel(int64_t s, int64_t n, int splits,
                                                         const float* __restrict__ partial,
                                                         double* __restrict__ y, int64_t ldy) {
@@ -122,7 +245,7 @@ __global__ __launch_bounds__(256) void cs_finish_kernel(int64_t s, int64_t n, in
 constexpr int64_t kLdsBytes = 128 * 1024;
 
 int cg_for(int64_t s) {
-    int cg = 8;
+    int cg = 8;  // 16-byte loads in flight per lane per plane
     while (cg > 1 && s * cg * 4 > kLdsBytes) cg >>= 1;
     return cg;
 }
@@ -136,10 +259,12 @@ CsPlan cs_plan(int64_t m, int64_t n, int64_t s) {
     CsPlan p;
     p.cg = cg_for(s);
     p.ncg = static_cast<int>(ceil_div(std::max<int64_t>(n, 1), p.cg));
-    const int64_t want = std::max<int64_t>(1, ceil_div(1024, p.ncg));
-    const int64_t max_splits = std::max<int64_t>(1, ceil_div(m, 4096));
+    // about 4 blocks per CU over the grid, each split at least 8192 rows;
+    // split starts stay multiples of 8 rows (the widest row vector)
+    const int64_t want = std::max<int64_t>(1, ceil_div(4 * 256, p.ncg));
+    const int64_t max_splits = std::max<int64_t>(1, ceil_div(m, 8192));
     p.splits = static_cast<int>(std::min(want, max_splits));
-    p.rows_per_split = ceil_div(std::max<int64_t>(m, 1), p.splits);
+    p.rows_per_split = ceil_div(ceil_div(std::max<int64_t>(m, 1), p.splits), 8) * 8;
     p.splits = static_cast<int>(ceil_div(std::max<int64_t>(m, 1), p.rows_per_split));
     return p;
 }
@@ -150,16 +275,35 @@ size_t cs_scratch(int64_t m, int64_t n, int64_t s) {
     return idx + static_cast<size_t>(p.splits) * s * n * 4;
 }
 
+template <int FMT>
+bool vec_ok(const void* a, const uint16_t* wr, int64_t lda, bool exact) {
+    constexpr int V = vec_rows<FMT>();
+    const bool lo = !(exact && (FMT == FMT_P16 || FMT == FMT_PBF)) ||
+                    (reinterpret_cast<uintptr_t>(wr) % 16 == 0);
+    return reinterpret_cast<uintptr_t>(a) % 16 == 0 && lda % V == 0 && lo;
+}
+
 template <int FMT, bool EXACT, int CG>
 void launch_apply(const CsPlan& p, int64_t m, int64_t n, const void* a, const uint16_t* wr,
                   int64_t lda, int64_t s, const uint32_t* packed, float* partial,
                   hipStream_t st) {
     const size_t lds = static_cast<size_t>(s) * CG * 4;
+    const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
+    require(blocks < (int64_t(1) << 31), "countsketch_apply: grid too large");
+    if (vec_ok<FMT>(a, wr, lda, EXACT)) {
+        auto* k = cs_apply_vec_kernel<FMT, EXACT, CG>;
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+        k<<<dim3(static_cast<unsigned>(blocks)), kVecThreads, lds, st>>>(
+            m, n, a, wr, lda, s, packed, p.rows_per_split, p.ncg, partial);
+        TOMOS_LAUNCH_CHECK("cs_apply_vec_kernel");
+        return;
+    }
     auto* k = cs_apply_kernel<FMT, EXACT, CG>;
     TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         static_cast<int>(lds)));
-    const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
     k<<<dim3(static_cast<unsigned>(blocks)), 256, lds, st>>>(m, n, a, wr, lda, s, packed,
                                                             p.rows_per_split, p.ncg, partial);
     TOMOS_LAUNCH_CHECK("cs_apply_kernel");

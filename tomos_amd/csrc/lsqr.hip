@@ -312,15 +312,18 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
     L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 26));
     L.hist = off; off = align256(off + static_cast<size_t>(L.hist_cap) * 8);
-    L.red = off; off = align256(off + static_cast<size_t>(std::max<int64_t>(gemv_sumsq_blocks_max(m), kRedBlocks)) * 8);
-    L.z = off; off = align256(off + static_cast<size_t>(n) * 8);  // z = P w (precond type)
-    L.fused_possible = o.fused != 0 && o.mv_tier == TOMOS_TIER_DEFAULT && n >= 1 && n <= 1536 &&
+    L.fused_possible = o.fused != 0 && o.mv_tier == TOMOS_TIER_DEFAULT && n >= 1 && n <= 2048 &&
                        (o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16);
+    if (L.fused_possible) L.fplan = lsqr_fused_plan(o.a_type, m, n);
+    // sized for every fused plan of (m, n), so tuning after sizing is safe
+    const int64_t red_n = std::max<int64_t>(std::max<int64_t>(gemv_sumsq_blocks_max(m), kRedBlocks),
+                                            L.fused_possible ? lsqr_fused_max_blocks(m) : 0);
+    L.red = off; off = align256(off + static_cast<size_t>(red_n) * 8);
+    L.z = off; off = align256(off + static_cast<size_t>(n) * 8);  // z = P w (precond type)
     L.tpart = L.tsum = off;
     if (L.fused_possible) {
-        L.fplan = lsqr_fused_plan(o.a_type, m, n);
-        L.tpart = off; off = align256(off + L.fplan.tpart_bytes);
-        L.tsum = off; off = align256(off + L.fplan.tsum_bytes);
+        L.tpart = off; off = align256(off + lsqr_fused_max_tpart_bytes(m, n));
+        L.tsum = off; off = align256(off + lsqr_fused_max_tsum_bytes(n));
     }
     size_t sc = 0;
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_NO_TRANSPOSE, m, n));

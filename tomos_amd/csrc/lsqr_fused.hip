@@ -23,6 +23,8 @@
 //            block and writes one partial per block at the end.
 // Deterministic: fixed reduction order, no atomics.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "codec.hpp"
 #include "common.hpp"
@@ -33,6 +35,8 @@ namespace tomos_hip {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int64_t kFusedMaxN = 2048;
+constexpr int kMaxSplits = 16;
 enum { FMT_F32 = 0, FMT_PBF = 1, FMT_P16 = 2 };
 
 template <int FMT>
@@ -93,38 +97,48 @@ struct LaneTR<L, 0, LO> {
     static __device__ __forceinline__ void run(float*, int) {}
 };
 
-template <int FMT, int ROUNDS>
+template <int G>
+constexpr int log2_small() {
+    return G >= 16 ? 4 : (G >= 8 ? 3 : (G >= 4 ? 2 : (G >= 2 ? 1 : 0)));
+}
+
+template <int FMT, int G, int ROUNDS>
 __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
     bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
     double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
     if (skip && *skip) return;
     constexpr int RPL = rows_per_lane<FMT>();
-    constexpr int TR = 8 * RPL;
+    constexpr int LOG_RPL = log2_small<RPL>();
+    constexpr int LOG_G = log2_small<G>();
+    constexpr int TR = G * RPL;          // rows per tile
+    constexpr int CS = kThreads / G;     // column slots per round
     constexpr int ESZ = FMT == FMT_F32 ? 4 : 2;
+    static_assert(ROUNDS % G == 0, "ROUNDS must be a multiple of G");
+    static_assert(LOG_G + LOG_RPL <= 6, "row groups and row bits must fit a wave");
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int g = tid & 7;    // row group inside the tile
-    const int c = tid >> 3;   // column slot 0..31
+    const int g = tid & (G - 1);  // row group inside the tile
+    const int c = tid >> LOG_G;   // column slot
     const char* abase = static_cast<const char*>(a);
 
     // p for this lane's columns, kept for the whole block
     float pv[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
-        const int64_t j = static_cast<int64_t>(r) * 32 + c;
+        const int64_t j = static_cast<int64_t>(r) * CS + c;
         pv[r] = j < n ? (p64 ? static_cast<float>(static_cast<const double*>(p)[j])
                              : static_cast<const float*>(p)[j])
                       : 0.0f;
     }
     const double cu = *u_coef;
-    constexpr int TOUT = ROUNDS / 8;  // column partials owned by this lane
+    constexpr int TOUT = ROUNDS / G;  // column partials owned by this lane
     float tacc[TOUT];
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) tacc[i] = 0.0f;
 
-    __shared__ float red[4][64];
+    __shared__ float red[4][TR];
     __shared__ float ush[TR];
     double sq = 0.0;
 
@@ -139,7 +153,7 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
         uint4 t[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r) {
-            const int64_t j = static_cast<int64_t>(r) * 32 + c;
+            const int64_t j = static_cast<int64_t>(r) * CS + c;
             if (in && j < n)
                 t[r] = *reinterpret_cast<const uint4*>(abase + (my_row + j * lda) * ESZ);
             else
@@ -156,14 +170,14 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
 #pragma unroll
             for (int k = 0; k < RPL; ++k) acc[k] = fmaf(av[k], pv[r], acc[k]);
         }
-        // reduce over the 8 column slots of this wave (lane bits 3..5)
-        LaneTR<RPL, 32, 8>::run(acc, lane);
-        // lane holds row k of group g: k = lane bits (5,4,3) for RPL 8,
-        // (5,4) for RPL 4
+        // reduce over the column slots of this wave (lane bits LOG_G..5):
+        // halving stages on bits 5.. leave row k = lane bits (5..6-LOG_RPL),
+        // plain stages on the bits below make the lanes of a row agree
+        LaneTR<RPL, 32, G>::run(acc, lane);
         {
-            const int k = RPL == 8 ? ((lane >> 3) & 7) : ((lane >> 4) & 3);
-            const bool writer = RPL == 8 ? true : ((lane & 8) == 0);
-            if (writer) red[wave][g * RPL + k] = acc[0];
+            const int k = (lane >> (6 - LOG_RPL)) & (RPL - 1);
+            constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
+            if ((lane & kDupMask) == 0) red[wave][g * RPL + k] = acc[0];
         }
         __syncthreads();
         if (tid < TR) {
@@ -186,7 +200,13 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
             ush[tid] = un;
         }
         __syncthreads();
-        // phase 2: column sums over this lane's rows with u_raw
+        // phase 2: column sums over this lane's rows with u_raw.  The empty
+        // asm marks the packed tile as redefined, so the compiler decodes it
+        // again instead of keeping phase 1's decoded floats (twice the VGPRs)
+        // live across the barriers.
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+            asm volatile("" : "+v"(t[r].x), "+v"(t[r].y), "+v"(t[r].z), "+v"(t[r].w));
         float ur[RPL];
 #pragma unroll
         for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
@@ -200,17 +220,20 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
             for (int k = 0; k < RPL; ++k) s = fmaf(k < nvalid ? av[k] : 0.0f, ur[k], s);
             cs[r] = s;
         }
-        // reduce over the 8 row groups (lane bits 0..2); lane keeps TOUT columns
-        LaneTR<ROUNDS, 4, 1>::run(cs, lane);
+        // reduce over the G row groups (lane bits 0..LOG_G-1); lane keeps TOUT columns
+        LaneTR<ROUNDS, G / 2, 1>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
         __syncthreads();  // red/ush reuse in the next tile
     }
-    // column ids of tacc: r = base + i, base = b2*(R/2) + b1*(R/4) + b0*(R/8)
-    const int base = ((g >> 2) & 1) * (ROUNDS / 2) + ((g >> 1) & 1) * (ROUNDS / 4) + (g & 1) * (ROUNDS / 8);
+    // column ids of tacc: r = base + i, the halving stage on lane bit b kept
+    // the upper half for lanes with bit b set
+    int base = 0;
+#pragma unroll
+    for (int b = 0; b < LOG_G; ++b) base += ((g >> b) & 1) * (ROUNDS >> (LOG_G - b));
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) {
-        const int64_t j = static_cast<int64_t>(base + i) * 32 + c;
+        const int64_t j = static_cast<int64_t>(base + i) * CS + c;
         if (j < n) tpart[static_cast<int64_t>(blockIdx.x) * n + j] = tacc[i];
     }
     // sum of squares of this block's u_raw rows
@@ -238,67 +261,123 @@ __global__ __launch_bounds__(kThreads) void fused_tsum_kernel(int64_t n, int nbl
     out[static_cast<int64_t>(sp) * n + j] = acc;
 }
 
-template <int FMT>
-int rounds_for(int64_t n) {
-    (void)FMT;
-    if (n <= 256) return 8;
-    if (n <= 512) return 16;
-    if (n <= 1024) return 32;
-    if (n <= 1536) return 48;
+// Column rounds for G row groups (CS = 256/G columns per round): the
+// instantiated (G, ROUNDS) pairs.
+int rounds_for(int g, int64_t n) {
+    const int64_t cs = kThreads / g;
+    const int64_t need = ceil_div(std::max<int64_t>(n, 1), cs);
+    static const int r8[] = {8, 16};
+    static const int r4[] = {4, 8, 12, 16};
+    static const int r2[] = {2, 4, 6, 8, 12, 16};
+    const int* tab = g == 8 ? r8 : (g == 4 ? r4 : r2);
+    const int len = g == 8 ? 2 : (g == 4 ? 4 : 6);
+    for (int i = 0; i < len; ++i)
+        if (tab[i] >= need) return tab[i];
     return 0;
 }
 
+// The kernel instance for (format, G, ROUNDS), or nullptr.
+template <int FMT>
+const void* fused_kernel_ptr(int g, int rounds) {
+#define FUSED_CASE(G, R) \
+    if (g == G && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R>);
+    FUSED_CASE(8, 8) FUSED_CASE(8, 16)
+    FUSED_CASE(4, 4) FUSED_CASE(4, 8) FUSED_CASE(4, 12) FUSED_CASE(4, 16)
+    FUSED_CASE(2, 2) FUSED_CASE(2, 4) FUSED_CASE(2, 6) FUSED_CASE(2, 8) FUSED_CASE(2, 12) FUSED_CASE(2, 16)
+#undef FUSED_CASE
+    return nullptr;
+}
+
+const void* fused_kernel_for(int a_type, int g, int rounds) {
+    if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF>(g, rounds);
+    if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16>(g, rounds);
+    return fused_kernel_ptr<FMT_F32>(g, rounds);
+}
+
 }  // namespace
+
+int lsqr_fused_default_g(int64_t n) { return n <= 1024 ? 4 : 2; }
 
 bool lsqr_fused_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a) {
     if (tier != TOMOS_TIER_DEFAULT) return false;
     if (a_type != TOMOS_F32 && a_type != TOMOS_FP32_BF16 && a_type != TOMOS_FP32_16) return false;
     const int vec = a_type == TOMOS_F32 ? 4 : 8;
-    if (n < 1 || n > 1536 || m < 1) return false;
+    if (n < 1 || n > kFusedMaxN || m < 1) return false;
     return lda % vec == 0 && reinterpret_cast<uintptr_t>(a) % 16 == 0;
 }
 
-FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n) {
+FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g, int blocks_per_cu) {
     FusedPlan p;
-    const int64_t tr = a_type == TOMOS_F32 ? 32 : 64;
+    if (g <= 0 && blocks_per_cu <= 0) lsqr_fused_config_get(a_type, n, &g, &blocks_per_cu);
+    if (g <= 0 || rounds_for(g, n) == 0) g = lsqr_fused_default_g(n);
+    p.g = g;
+    p.rounds = rounds_for(g, n);
+    if (blocks_per_cu <= 0) {
+        // as many blocks as fit on the device at once: one wave of blocks
+        int occ = 0;
+        const void* k = p.rounds > 0 ? fused_kernel_for(a_type, g, p.rounds) : nullptr;
+        if (k && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kThreads, 0) != hipSuccess) occ = 0;
+        blocks_per_cu = occ > 0 ? occ : 2;
+    }
+    blocks_per_cu = std::min(blocks_per_cu, 8);
+    const int64_t tr = static_cast<int64_t>(g) * (a_type == TOMOS_F32 ? 4 : 8);
     const int64_t ntiles = ceil_div(std::max<int64_t>(m, 1), tr);
-    const int64_t want = 512;  // two blocks per CU on a 256-CU MI355X
+    const int64_t want = static_cast<int64_t>(blocks_per_cu) * cu_count();
     p.tiles_per_block = std::max<int64_t>(1, ceil_div(ntiles, want));
     p.nblocks = static_cast<int>(ceil_div(ntiles, p.tiles_per_block));
-    p.splits = std::min(16, std::max(1, p.nblocks / 16));
+    p.splits = std::min(kMaxSplits, std::max(1, p.nblocks / 16));
     p.tpart_bytes = static_cast<size_t>(p.nblocks) * n * 4;
     p.tsum_bytes = static_cast<size_t>(p.splits) * n * 8;
-    (void)n;
     return p;
+}
+
+int64_t lsqr_fused_max_blocks(int64_t m) {
+    // at most 8 blocks per CU, and never more blocks than 8-row tiles
+    return std::max<int64_t>(1, std::min<int64_t>(8LL * cu_count(), ceil_div(std::max<int64_t>(m, 1), 8)));
+}
+
+size_t lsqr_fused_max_tpart_bytes(int64_t m, int64_t n) {
+    return static_cast<size_t>(lsqr_fused_max_blocks(m)) * n * 4;
+}
+
+size_t lsqr_fused_max_tsum_bytes(int64_t n) { return static_cast<size_t>(kMaxSplits) * n * 8; }
+
+namespace {
+std::mutex g_cfg_mu;
+std::map<std::pair<int, int64_t>, std::pair<int, int>> g_cfg;  // (a_type, n) -> (g, blocks/CU)
+}  // namespace
+
+void lsqr_fused_config_set(int a_type, int64_t n, int g, int blocks_per_cu) {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    if (g <= 0 && blocks_per_cu <= 0) g_cfg.erase({a_type, n});
+    else g_cfg[{a_type, n}] = {g, blocks_per_cu};
+}
+
+void lsqr_fused_config_clear() {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    g_cfg.clear();
+}
+
+bool lsqr_fused_config_get(int a_type, int64_t n, int* g, int* blocks_per_cu) {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    auto it = g_cfg.find({a_type, n});
+    if (it == g_cfg.end()) return false;
+    *g = it->second.first;
+    *blocks_per_cu = it->second.second;
+    return true;
 }
 
 void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t lda, const void* p,
                        bool p64, void* u, bool u64, const double* u_coef, double* sumsq,
                        float* tpart, double* tsum, const FusedPlan& plan, const int* skip,
                        hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>(plan.nblocks));
-#define FUSED_CASE(F, R)                                                                        \
-    lsqr_fused_kernel<F, R><<<grid, kThreads, 0, s>>>(m, n, a, lda, p, p64, u, u64, u_coef,      \
-                                                      plan.tiles_per_block, sumsq, tpart, skip)
-    const int rounds = rounds_for<0>(n);
-    require(rounds > 0, "lsqr_fused: n too large");
-    if (a_type == TOMOS_FP32_BF16) {
-        if (rounds == 8) FUSED_CASE(FMT_PBF, 8);
-        else if (rounds == 16) FUSED_CASE(FMT_PBF, 16);
-        else if (rounds == 32) FUSED_CASE(FMT_PBF, 32);
-        else FUSED_CASE(FMT_PBF, 48);
-    } else if (a_type == TOMOS_FP32_16) {
-        if (rounds == 8) FUSED_CASE(FMT_P16, 8);
-        else if (rounds == 16) FUSED_CASE(FMT_P16, 16);
-        else if (rounds == 32) FUSED_CASE(FMT_P16, 32);
-        else FUSED_CASE(FMT_P16, 48);
-    } else {
-        if (rounds == 8) FUSED_CASE(FMT_F32, 8);
-        else if (rounds == 16) FUSED_CASE(FMT_F32, 16);
-        else if (rounds == 32) FUSED_CASE(FMT_F32, 32);
-        else FUSED_CASE(FMT_F32, 48);
-    }
-#undef FUSED_CASE
+    require(plan.rounds > 0 && rounds_for(plan.g, n) == plan.rounds, "lsqr_fused: plan does not fit n");
+    const void* k = fused_kernel_for(a_type, plan.g, plan.rounds);
+    if (!k) throw Error(kErrInternal, "lsqr_fused: no kernel for this plan");
+    int64_t tpb = plan.tiles_per_block;
+    void* args[] = {&m, &n, const_cast<void**>(&a), &lda, const_cast<void**>(&p), &p64, &u, &u64,
+                    const_cast<double**>(&u_coef), &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};
+    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(kThreads), args, 0, s));
     TOMOS_LAUNCH_CHECK("lsqr_fused_kernel");
     fused_tsum_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, kThreads)), static_cast<unsigned>(plan.splits)),
                         kThreads, 0, s>>>(n, plan.nblocks, plan.splits, tpart, tsum, skip);
@@ -306,3 +385,119 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
 }
 
 }  // namespace tomos_hip
+
+namespace tomos_hip {
+namespace {
+
+__global__ void pair_setup_kernel(double* __restrict__ coef, double value) { *coef = value; }
+
+// t[j] = sum over splits of tsum[s][j]; block 0 also sums the block sumsq
+// partials into *unorm2 (fixed order).
+__global__ __launch_bounds__(kThreads) void pair_finish_kernel(int64_t n, int splits,
+                                                               const double* __restrict__ tsum,
+                                                               double* __restrict__ t, int nb,
+                                                               const double* __restrict__ sumsq,
+                                                               double* __restrict__ unorm2) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (j < n) {
+        double acc = 0.0;
+        for (int sp = 0; sp < splits; ++sp) acc += tsum[static_cast<int64_t>(sp) * n + j];
+        t[j] = acc;
+    }
+    if (blockIdx.x == 0 && unorm2) {
+        __shared__ double red[kThreads];
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kThreads) acc += sumsq[b];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = kThreads / 2; w >= 1; w >>= 1) {
+            if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *unorm2 = red[0];
+    }
+}
+
+struct PairLayout {
+    size_t coef, sumsq, tpart, tsum, total;
+};
+
+// Sized for every plan of (m, n), so a tuned plan never outgrows scratch
+// that was sized before tuning.
+PairLayout pair_layout(int64_t m, int64_t n) {
+    PairLayout L;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) / 256 * 256;
+        return at;
+    };
+    L.coef = take(8);
+    L.sumsq = take(static_cast<size_t>(lsqr_fused_max_blocks(m)) * 8);
+    L.tpart = take(lsqr_fused_max_tpart_bytes(m, n));
+    L.tsum = take(lsqr_fused_max_tsum_bytes(n));
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+}  // namespace tomos_hip
+
+using namespace tomos_hip;
+
+extern "C" {
+
+int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu) {
+    return guarded([&] {
+        require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16,
+                "tomos_fused_config_set: bad a_type");
+        require(n >= 1 && n <= kFusedMaxN, "tomos_fused_config_set: bad n");
+        require(row_groups == 0 || row_groups == 2 || row_groups == 4 || row_groups == 8,
+                "tomos_fused_config_set: row_groups must be 0, 2, 4 or 8");
+        require(row_groups == 0 || rounds_for(row_groups, n) > 0,
+                "tomos_fused_config_set: no kernel for these row_groups at this n");
+        require(blocks_per_cu >= 0 && blocks_per_cu <= 8, "tomos_fused_config_set: blocks_per_cu in 0..8");
+        lsqr_fused_config_set(a_type, n, row_groups, blocks_per_cu);
+    });
+}
+
+int tomos_fused_config_clear(void) {
+    return guarded([&] { lsqr_fused_config_clear(); });
+}
+
+size_t tomos_matvec_pair_scratch_size(int a_type, int64_t m, int64_t n) {
+    if (m < 1 || n < 1 || n > kFusedMaxN) return 0;
+    (void)a_type;
+    return pair_layout(m, n).total;
+}
+
+int tomos_matvec_pair(int a_type, int64_t m, int64_t n, const void* a, int64_t lda,
+                      const double* p, double u_coef, float* u, double* t, double* unorm2,
+                      void* scratch, size_t scratch_bytes, tomos_stream_t s) {
+    return guarded([&] {
+        require(m >= 1 && n >= 1 && lda >= m, "tomos_matvec_pair: bad sizes");
+        require(a && p && u && t, "tomos_matvec_pair: null pointer");
+        if (!lsqr_fused_supported(a_type, TOMOS_TIER_DEFAULT, m, n, lda, a))
+            throw Error(kErrUnsupported,
+                        "tomos_matvec_pair: needs F32/FP32_16/FP32_BF16 A, n <= 2048, 16-byte aligned "
+                        "A and lda a multiple of the 16-byte row vector");
+        const FusedPlan plan = lsqr_fused_plan(a_type, m, n);
+        const PairLayout L = pair_layout(m, n);
+        if (!scratch) scratch = fallback_scratch(L.total);
+        else require(scratch_bytes >= L.total, "tomos_matvec_pair: scratch too small");
+        char* base = static_cast<char*>(scratch);
+        double* coef = reinterpret_cast<double*>(base + L.coef);
+        double* sumsq = reinterpret_cast<double*>(base + L.sumsq);
+        float* tpart = reinterpret_cast<float*>(base + L.tpart);
+        double* tsum = reinterpret_cast<double*>(base + L.tsum);
+        pair_setup_kernel<<<1, 1, 0, s>>>(coef, u_coef);
+        TOMOS_LAUNCH_CHECK("pair_setup_kernel");
+        lsqr_fused_launch(a_type, m, n, a, lda, p, true, u, false, coef, sumsq, tpart, tsum, plan,
+                          nullptr, s);
+        pair_finish_kernel<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, s>>>(
+            n, plan.splits, tsum, t, plan.nblocks, sumsq, unorm2);
+        TOMOS_LAUNCH_CHECK("pair_finish_kernel");
+    });
+}
+
+}  // extern "C"
