@@ -301,6 +301,39 @@ int tomos_lsqr_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const 
                      void* scratch, size_t scratch_bytes, int* iterations, double* history,
                      int history_cap, int* history_len, tomos_stream_t stream);
 
+/* ---- Adjoint of the least-squares solve (implicit differentiation) ----- */
+/* x = argmin |A x - b| with upstream gradient g = dL/dx:
+ *   w = (A^T A)^-1 g,  grad_b = A w,  grad_A = r w^T - (A w) x^T,  r = b - A x.
+ * No reference counterpart (the reference has no adjoint).
+ *
+ * tomos_lsq_adjoint_solve: w by restarted, R-preconditioned CG on the normal
+ * equations with the LSQR solver's kernels and precision split: each
+ * restart forms res = g - A^T A w at opt->restart_tier (FP64 accumulation)
+ * and stops when |res|/|g| < opt->tol; inner iterations (opt->max_iters per
+ * restart, opt->restarts restarts) form A d and A^T (A d) at opt->mv_tier
+ * from one read of A (tomos_matvec_pair's kernel, when opt->fused and the
+ * shape allow).  rinv: R^-1 (n x n, ldr, opt->precond_type F64/F32) or NULL
+ * with precond_type -1.  g, w: FP64 (n).  value_type/refine_type are not
+ * used.  history: relative residual per step (restart checks included).
+ * scratch: tomos_lsq_adjoint_scratch_size() bytes or NULL. */
+size_t tomos_lsq_adjoint_scratch_size(const tomos_lsqr_options* opt, int64_t m, int64_t n);
+int tomos_lsq_adjoint_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const void* a,
+                            const uint16_t* wr, int64_t lda, const double* g, const void* rinv,
+                            int64_t ldr, double* w, void* scratch, size_t scratch_bytes,
+                            int* iterations, double* history, int history_cap, int* history_len,
+                            tomos_stream_t stream);
+
+/* grad_b = A w (FP64, m) at `tier`; when grad_a is non-NULL also
+ * r = b - A x (same tier) and grad_a = r w^T - grad_b x^T (FP32, m x n,
+ * column-major, ldg >= m) from one rank-2 kernel that reads only r, grad_b,
+ * w and x.  b: FP64 (m); x, w: FP64 (n).
+ * scratch: tomos_lsq_grad_scratch_size() bytes or NULL. */
+size_t tomos_lsq_grad_scratch_size(int a_type, int tier, int64_t m, int64_t n);
+int tomos_lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, const uint16_t* wr,
+                   int64_t lda, const double* b, const double* x, const double* w, double* grad_b,
+                   float* grad_a, int64_t ldg, void* scratch, size_t scratch_bytes,
+                   tomos_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,116 @@
+"""Adjoint of the least-squares solve on the GPU (C ABI and the torch
+autograd function) vs the dense FP64 QR gradient and central finite
+differences."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import hipcall as H
+from tests.adjoint_ref import dense_grads
+from tomos_amd import _capi as C
+from tomos_amd._capi import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(m, n, kappa_exp, seed, noise=1e-2):
+    rng = np.random.default_rng(seed)
+    A = (rng.standard_normal((m, n)) * np.logspace(0, -kappa_exp, n)).astype(np.float32, order="F")
+    b = A.astype(np.float64) @ rng.standard_normal(n) + noise * rng.standard_normal(m)
+    g = rng.standard_normal(n)
+    return A, b, g
+
+
+def adjoint_w(a_type, planes, m, n, lda, g, rinv_buf=None, ldr=0, tol=1e-10, fused=1,
+              inner=30, restarts=20):
+    opt = C.LsqrOptions(a_type, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64,
+                        C.F64 if rinv_buf is not None else -1, tol, tol, inner, restarts, 4, fused)
+    sb = C.lib().tomos_lsq_adjoint_scratch_size(ctypes.byref(opt), m, n)
+    assert sb > 0, C.lib().tomos_last_error()
+    scr, gd, wd = DeviceBuffer(sb), H.dev(g), DeviceBuffer(n * 8)
+    it, hl = ctypes.c_int(0), ctypes.c_int(0)
+    hist = np.zeros(4096)
+    C.call("tomos_lsq_adjoint_solve", ctypes.byref(opt), m, n, planes[0].ptr,
+           planes[1].ptr if planes[1] is not None else None, lda, gd.ptr,
+           rinv_buf.ptr if rinv_buf is not None else None, ldr, wd.ptr, scr.ptr, sb,
+           ctypes.byref(it), hist.ctypes.data, hist.size, ctypes.byref(hl), None)
+    return wd.to_numpy(np.float64, (n,)), it.value, hist[:hl.value]
+
+
+@pytest.mark.parametrize("m,n,kexp,fused", [(20000, 64, 2, 1), (5003, 37, 2, 1), (20000, 64, 2, 0)])
+def test_adjoint_w_preconditioned_packed(capi, m, n, kexp, fused):
+    A, b, g = problem(m, n, kexp, m + n)
+    lda = (m + 7) // 8 * 8
+    wh, wr = H.pack(C.PACK_FP32_BF16, A, ld=lda)
+    _, sk = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, lda, 4 * n)
+    w, iters, hist = adjoint_w(C.FP32_BF16, (wh, wr), m, n, lda, g, sk, 4 * n, fused=fused)
+    w_ref = dense_grads(A, b, g)[0]
+    assert hist[-1] < 1e-10, hist[-5:]
+    # |w - w*| <= kappa(A^T A) |res| / |g| |w*|-ish; kappa(A) = 1e2 here
+    assert np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref) < 1e-5
+    assert iters < 200
+
+
+def test_adjoint_w_unpreconditioned_f32(capi):
+    m, n = 3000, 20
+    A, b, g = problem(m, n, 0.5, 5)
+    w, iters, hist = adjoint_w(C.F32, (H.dev(A), None), m, n, m, g, None, 0, tol=1e-9, fused=1,
+                               inner=60, restarts=10)
+    w_ref = dense_grads(A, b, g)[0]
+    assert np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref) < 1e-7
+
+
+@pytest.mark.parametrize("ldg_pad", [0, 1])
+def test_grad_kernel(capi, ldg_pad):
+    m, n = 10007, 33
+    A, b, g = problem(m, n, 1, 9)
+    wh, wr = H.pack(C.PACK_FP32_BF16, A)
+    rng = np.random.default_rng(3)
+    x, w = rng.standard_normal(n), rng.standard_normal(n)
+    ldg = m + ldg_pad  # odd ldg takes the element-wise store path
+    gbd, gad = DeviceBuffer(m * 8), DeviceBuffer(ldg * n * 4)
+    bd, xd, wd = H.dev(b), H.dev(x), H.dev(w)
+    C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, m, n, wh.ptr, wr.ptr, m, bd.ptr, xd.ptr, wd.ptr,
+           gbd.ptr, gad.ptr, ldg, None, 0, None)
+    A64 = A.astype(np.float64)
+    gb_ref = A64 @ w
+    gA_ref = np.outer(b - A64 @ x, w) - np.outer(gb_ref, x)
+    gb = gbd.to_numpy(np.float64, (m,))
+    gA = gad.to_numpy(np.float32, (ldg, n))[:m]
+    assert np.abs(gb - gb_ref).max() < 1e-12 * np.abs(A64).max() * np.abs(w).sum()
+    assert np.abs(gA - gA_ref).max() < 2e-7 * np.abs(gA_ref).max()
+
+
+def test_torch_autograd_vs_dense_and_fd(capi):
+    torch = pytest.importorskip("torch")
+    from tomos_amd.autograd import lstsq, SolveOptions
+    m, n = 20000, 64
+    A_np, b_np, g_np = problem(m, n, 3, 21)
+    dev = torch.device("cuda:0")
+    A = torch.tensor(A_np, device=dev, requires_grad=True)
+    b = torch.tensor(b_np, device=dev, requires_grad=True)
+    g = torch.tensor(g_np, device=dev)
+    opts = SolveOptions(tol=1e-10, adjoint_tol=1e-10)
+    x = lstsq(A, b, opts)
+    (x @ g).backward()
+    w_ref, gb_ref, gA_ref, x_ref, _ = dense_grads(A_np, b_np, g_np)
+    xe = np.linalg.norm(x.detach().cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    assert xe < 1e-6, xe  # kappa = 1e3, FP64-refined solve
+    gb = b.grad.cpu().numpy()
+    gA = A.grad.cpu().numpy()
+    assert np.linalg.norm(gb - gb_ref) / np.linalg.norm(gb_ref) < 1e-5
+    assert np.linalg.norm(gA - gA_ref) / np.linalg.norm(gA_ref) < 1e-5
+    # central differences (h = 1e-3) of the GPU forward along random directions
+    rng = np.random.default_rng(2)
+    h = 1e-3
+    with torch.no_grad():
+        for _ in range(2):
+            db = torch.tensor(rng.standard_normal(m), device=dev)
+            fd = ((lstsq(A, b + h * db, opts) - lstsq(A, b - h * db, opts)) @ g).item() / (2 * h)
+            an = float(gb_ref @ db.cpu().numpy())
+            assert abs(fd - an) < 1e-4 * (abs(an) + np.abs(gb_ref).sum() * 1e-3), (fd, an)
+        dA = torch.tensor(rng.standard_normal((m, n)).astype(np.float32), device=dev) * A.detach().abs()
+        fd = ((lstsq(A.detach() + h * dA, b, opts) - lstsq(A.detach() - h * dA, b, opts)) @ g).item() / (2 * h)
+        an = float((gA_ref * dA.cpu().numpy().astype(np.float64)).sum())
+        assert abs(fd - an) < 1e-2 * abs(an) + 1e-6, (fd, an)

@@ -87,6 +87,12 @@ _SIGS = {
     "tomos_matvec_pair_scratch_size": (sz, [cint, i64, i64]),
     "tomos_fused_config_set": (cint, [cint, i64, cint, cint]),
     "tomos_fused_config_clear": (cint, []),
+    "tomos_lsq_adjoint_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
+    "tomos_lsq_adjoint_solve": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, vp, vp, i64,
+                                       vp, vp, sz, ctypes.POINTER(cint), vp, cint, ctypes.POINTER(cint),
+                                       vp]),
+    "tomos_lsq_grad_scratch_size": (sz, [cint, cint, i64, i64]),
+    "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
                                 sz, vp]),

@@ -185,8 +185,10 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
             float un = 0.0f;
             if (row < m) {
                 const float s = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-                double uo = u64 ? static_cast<double*>(u)[row] : static_cast<double>(static_cast<float*>(u)[row]);
-                const double r = static_cast<double>(s) + cu * uo;
+                double r = static_cast<double>(s);
+                if (cu != 0.0)  // u is not read for a zero coefficient
+                    r += cu * (u64 ? static_cast<double*>(u)[row]
+                                   : static_cast<double>(static_cast<float*>(u)[row]));
                 if (u64) {
                     static_cast<double*>(u)[row] = r;
                     un = static_cast<float>(r);
