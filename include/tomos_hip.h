@@ -256,6 +256,20 @@ int tomos_trtri_upper_f64(int64_t n, double* r, int64_t ldr, tomos_stream_t stre
 int tomos_sketch_qr_precond_f64(int64_t k, int64_t n, double* a, int64_t lda,
                                 tomos_stream_t stream);
 
+/* Many sketches at once (uneven k, n): the same factorization with every
+ * launch covering all problems (the launches of the largest one), R^-1 of
+ * each written over its leading n x n block.  scratch:
+ * tomos_sketch_qr_precond_batched_scratch_size() bytes, or NULL (then the
+ * call allocates and frees it).  Synchronous.  The batched form of
+ * generate_sketch_qr_precond (src/core/precond/sketch_qr.cpp:68-122). */
+typedef struct {
+    int64_t k, n, lda;
+    double* a;
+} tomos_qr_problem;
+size_t tomos_sketch_qr_precond_batched_scratch_size(int nprob, const tomos_qr_problem* probs);
+int tomos_sketch_qr_precond_batched_f64(int nprob, const tomos_qr_problem* probs, void* scratch,
+                                        size_t scratch_bytes, tomos_stream_t stream);
+
 /* ---- LSQR ------------------------------------------------------------- */
 /* Restarted, right-preconditioned Golub-Kahan LSQR on one GPU, with the
  * scalar recurrences (alpha, beta, rho, phi, c, s, theta) and all norms in

@@ -45,6 +45,10 @@ class TomosError(RuntimeError):
         self.code = code
 
 
+class QrProblem(ctypes.Structure):
+    _fields_ = [("k", i64), ("n", i64), ("lda", i64), ("a", vp)]
+
+
 class LsqrOptions(ctypes.Structure):
     _fields_ = [("a_type", cint), ("mv_tier", cint), ("restart_tier", cint),
                 ("value_type", cint), ("refine_type", cint), ("precond_type", cint),
@@ -92,6 +96,8 @@ _SIGS = {
                                        vp, vp, sz, ctypes.POINTER(cint), vp, cint, ctypes.POINTER(cint),
                                        vp]),
     "tomos_lsq_grad_scratch_size": (sz, [cint, cint, i64, i64]),
+    "tomos_sketch_qr_precond_batched_scratch_size": (sz, [cint, vp]),
+    "tomos_sketch_qr_precond_batched_f64": (cint, [cint, vp, vp, sz, vp]),
     "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,

@@ -7,7 +7,12 @@ forward
     preconditioner (hashed CountSketch with s = 4n rows, FP64 Householder
     QR, R^-1) and runs the restarted mixed-precision LSQR: hi-only GEMVs
     inside a restart, exact (wh+wr) FP64-accumulated residual at each
-    restart.  Returns x in FP64.
+    restart.  Restarting on b - A x alone converges to A_hi^T r = 0, not
+    A^T r = 0: for an inconsistent system (r != 0) that leaves a bias of
+    the order of the hi-plane rounding times |r|.  So the forward ends
+    with one normal-equation correction: s = A^T (b - A x) exactly (FP64),
+    delta = (A^T A)^-1 s by the adjoint's preconditioned CG (whose restarts
+    also use exact products), x += delta.  Returns x in FP64.
 backward
     implicit differentiation of the normal equations A^T A x = A^T b:
     w = (A^T A)^-1 g by R-preconditioned CG with the same packed kernels
@@ -46,6 +51,7 @@ class SolveOptions:
     sketch_factor: int = 4       # sketch rows s = sketch_factor * n
     adjoint_tol: float = 1e-10   # |g - A^T A w| / |g| for the adjoint solve
     check_every: int = 8
+    normal_correction: bool = True  # forward: x += (A^T A)^-1 A^T (b - A x), exact products
 
 
 class PackedProblem:
@@ -90,7 +96,21 @@ class PackedProblem:
                _p(self.sketch), self.s_rows, _p(arena), ab, _p(scr), sb, ctypes.byref(it), None, 0, None,
                _stream())
         self.last_iterations = it.value
+        if self.opts.normal_correction:
+            x = x + self.adjoint(self.normal_residual(b64, x))
         return x
+
+    def normal_residual(self, b64, x):
+        """s = A^T (b - A x) with the exact (wh+wr) tier, FP64 throughout."""
+        m, n, dev = self.m, self.n, x.device
+        r = b64.clone()
+        s_ = torch.empty(n, dtype=torch.float64, device=dev)
+        st = _stream()
+        C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, C.NO_TRANSPOSE, -1.0, m, n, _p(self.wh), _p(self.wr),
+               self.ld, C.F64, _p(x), 1.0, C.F64, _p(r), None, 0, st)
+        C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, C.TRANSPOSE, 1.0, m, n, _p(self.wh), _p(self.wr),
+               self.ld, C.F64, _p(r), 0.0, C.F64, _p(s_), None, 0, st)
+        return s_
 
     def adjoint(self, g64):
         """w = (A^T A)^-1 g."""

@@ -102,12 +102,15 @@ constexpr int log2_small() {
     return G >= 16 ? 4 : (G >= 8 ? 3 : (G >= 4 ? 2 : (G >= 2 ? 1 : 0)));
 }
 
+// One block's share of the single-pass product: tiles [tile0, tile1) of an
+// m x n matrix; writes the block's sum of u_raw^2 to *sumsq_out and its
+// A^T u_raw partial (n floats) to tpart_out.
 template <int FMT, int G, int ROUNDS>
-__global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
-    int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
-    bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
-    double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
-    if (skip && *skip) return;
+__device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __restrict__ a,
+                                           int64_t lda, const void* __restrict__ p, bool p64,
+                                           void* __restrict__ u, bool u64, double cu, int64_t tile0,
+                                           int64_t tile1, double* __restrict__ sumsq_out,
+                                           float* __restrict__ tpart_out) {
     constexpr int RPL = rows_per_lane<FMT>();
     constexpr int LOG_RPL = log2_small<RPL>();
     constexpr int LOG_G = log2_small<G>();
@@ -132,7 +135,6 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
                              : static_cast<const float*>(p)[j])
                       : 0.0f;
     }
-    const double cu = *u_coef;
     constexpr int TOUT = ROUNDS / G;  // column partials owned by this lane
     float tacc[TOUT];
 #pragma unroll
@@ -142,9 +144,6 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
     __shared__ float ush[TR];
     double sq = 0.0;
 
-    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
-    const int64_t ntiles = (m + TR - 1) / TR;
-    const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
     for (int64_t tile = tile0; tile < tile1; ++tile) {
         const int64_t row0 = tile * TR;
         const int64_t my_row = row0 + g * RPL;
@@ -236,14 +235,40 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) {
         const int64_t j = static_cast<int64_t>(base + i) * CS + c;
-        if (j < n) tpart[static_cast<int64_t>(blockIdx.x) * n + j] = tacc[i];
+        if (j < n) tpart_out[j] = tacc[i];
     }
     // sum of squares of this block's u_raw rows
     for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
     __shared__ double sqw[4];
     if (lane == 0) sqw[wave] = sq;
     __syncthreads();
-    if (tid == 0) sumsq[blockIdx.x] = (sqw[0] + sqw[1]) + (sqw[2] + sqw[3]);
+    if (tid == 0) *sumsq_out = (sqw[0] + sqw[1]) + (sqw[2] + sqw[3]);
+}
+
+template <int FMT, int G, int ROUNDS>
+__global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
+    bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
+    double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int TR = G * rows_per_lane<FMT>();
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
+    const int64_t ntiles = (m + TR - 1) / TR;
+    const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
+    fused_body<FMT, G, ROUNDS>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+                               tpart + static_cast<int64_t>(blockIdx.x) * n);
+}
+
+// Many problems in one launch: block b runs work item items[b] (a tile
+// range of one problem).  A problem whose skip flag is set is a no-op.
+template <int FMT, int G, int ROUNDS>
+__global__ __launch_bounds__(kThreads) void grouped_fused_kernel(const GroupProb* __restrict__ probs,
+                                                                 const GroupItem* __restrict__ items) {
+    const GroupItem it = items[blockIdx.x];
+    const GroupProb& P = probs[it.prob];
+    if (P.skip && *P.skip) return;
+    fused_body<FMT, G, ROUNDS>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
+                               it.tile1, P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
 }
 
 // level-1 reduction of the block partials: out[s][j] = sum over the blocks
@@ -290,6 +315,23 @@ const void* fused_kernel_ptr(int g, int rounds) {
     return nullptr;
 }
 
+template <int FMT>
+const void* grouped_kernel_ptr(int g, int rounds) {
+#define FUSED_CASE(G, R) \
+    if (g == G && rounds == R) return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R>);
+    FUSED_CASE(8, 8) FUSED_CASE(8, 16)
+    FUSED_CASE(4, 4) FUSED_CASE(4, 8) FUSED_CASE(4, 12) FUSED_CASE(4, 16)
+    FUSED_CASE(2, 2) FUSED_CASE(2, 4) FUSED_CASE(2, 6) FUSED_CASE(2, 8) FUSED_CASE(2, 12) FUSED_CASE(2, 16)
+#undef FUSED_CASE
+    return nullptr;
+}
+
+const void* grouped_kernel_for(int a_type, int g, int rounds) {
+    if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF>(g, rounds);
+    if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16>(g, rounds);
+    return grouped_kernel_ptr<FMT_F32>(g, rounds);
+}
+
 const void* fused_kernel_for(int a_type, int g, int rounds) {
     if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF>(g, rounds);
     if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16>(g, rounds);
@@ -299,6 +341,28 @@ const void* fused_kernel_for(int a_type, int g, int rounds) {
 }  // namespace
 
 int lsqr_fused_default_g(int64_t n) { return n <= 1024 ? 4 : 2; }
+
+int64_t fused_tile_rows(int a_type, int g) { return static_cast<int64_t>(g) * (a_type == TOMOS_F32 ? 4 : 8); }
+
+int fused_rounds(int g, int64_t n) { return (g == 2 || g == 4 || g == 8) ? rounds_for(g, n) : 0; }
+
+void grouped_fused_launch(int a_type, int g, int rounds, const GroupProb* probs,
+                          const GroupItem* items, int nitems, hipStream_t s) {
+    require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_BF16 || a_type == TOMOS_FP32_16,
+            "grouped_fused: unsupported a_type");
+    const void* k = grouped_kernel_for(a_type, g, rounds);
+    if (!k) throw Error(kErrInternal, "grouped_fused: no kernel for this shape");
+    if (nitems <= 0) return;
+    void* args[] = {const_cast<GroupProb**>(&probs), const_cast<GroupItem**>(&items)};
+    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(nitems)), dim3(kThreads), args, 0, s));
+}
+
+int grouped_fused_occupancy(int a_type, int g, int rounds) {
+    const void* k = grouped_kernel_for(a_type, g, rounds);
+    int occ = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kThreads, 0) != hipSuccess) occ = 0;
+    return std::max(occ, 1);
+}
 
 bool lsqr_fused_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a) {
     if (tier != TOMOS_TIER_DEFAULT) return false;

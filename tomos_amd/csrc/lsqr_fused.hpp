@@ -44,4 +44,35 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
                        float* tpart, double* tsum, const FusedPlan& plan, const int* skip,
                        hipStream_t s);
 
+// ---- many problems in one launch ----------------------------------------
+// Device table entry of one problem: A (hi plane / F32), p (FP64, n),
+// u_raw (FP32, m) updated in place with *u_coef, per-item partial slots
+// (sumsq[slot], tpart[slot][n]) and a skip flag.
+struct GroupProb {
+    int64_t m, n, lda;
+    const void* a;
+    const double* p;
+    float* u;
+    const double* u_coef;
+    const int* skip;
+    double* sumsq;
+    float* tpart;
+};
+
+// One block's work: tiles [tile0, tile1) of problem `prob`, partial slot `slot`.
+struct GroupItem {
+    int prob, slot;
+    int64_t tile0, tile1;
+};
+
+// Rows per tile for (a_type, g): g * (8 packed / 4 FP32 rows per 16-byte vector).
+int64_t fused_tile_rows(int a_type, int g);
+// Column rounds of the kernel for (g, n); 0 when no kernel covers n.
+int fused_rounds(int g, int64_t n);
+// grid = nitems blocks; rounds must cover the largest n of the problems.
+void grouped_fused_launch(int a_type, int g, int rounds, const GroupProb* probs,
+                          const GroupItem* items, int nitems, hipStream_t s);
+// Blocks of the grouped kernel resident per CU (occupancy), >= 1.
+int grouped_fused_occupancy(int a_type, int g, int rounds);
+
 }  // namespace tomos_hip
