@@ -315,6 +315,44 @@ int tomos_lsqr_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const 
                      void* scratch, size_t scratch_bytes, int* iterations, double* history,
                      int history_cap, int* history_len, tomos_stream_t stream);
 
+/* ---- Many uneven LSQR problems in one set of launches ----------------- */
+/* One problem: A (m x n, lda; for packed storage the hi plane, `wr` the lo
+ * plane), b and x FP64, R^-1 (FP64, n x n, ldr) of its sketch-QR
+ * preconditioner. */
+typedef struct {
+    int64_t m, n, lda;
+    const void* a;
+    const uint16_t* wr;
+    const double* b;
+    double* x;
+    const double* rinv;
+    int64_t ldr;
+} tomos_lsq_problem;
+
+/* Restarted, R-preconditioned LSQR on every problem of the batch, each with
+ * the single-problem solver's control flow, scalars and stop test
+ * (fused_lsqr_impl, src/core/solver/lsqr.cpp:685-747, per problem), but one
+ * step of all problems is three kernels whatever their number and shapes:
+ * a grouped exact-residual kernel (restarts), the grouped single-pass
+ * A p / A^T u kernel and a one-block-per-problem tail.  With use_graph the
+ * launches of one restart (1 + opt->max_iters steps) are captured once as a
+ * hipGraph and replayed; a NULL stream then runs on a private stream.
+ * Options: a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, value_type F32,
+ * refine_type F64, precond_type F64 (R^-1 required), n <= 2048, 16-byte
+ * aligned planes with lda a multiple of 4 (F32) / 8 (packed).
+ * row_groups / tiles_per_block: launch shape of the grouped single-pass
+ * kernel (0 = default; chosen by the autotuner).  iterations and
+ * final_relres (|phi_bar|/|b|, the solver's estimate): one per problem, or
+ * NULL.  scratch: tomos_lsqr_batched_scratch_size() bytes or NULL.
+ * No reference counterpart (the reference solves one problem per call). */
+size_t tomos_lsqr_batched_scratch_size(const tomos_lsqr_options* opt, int nprob,
+                                       const tomos_lsq_problem* probs, int row_groups,
+                                       int tiles_per_block);
+int tomos_lsqr_batched_solve(const tomos_lsqr_options* opt, int nprob, const tomos_lsq_problem* probs,
+                             int row_groups, int tiles_per_block, int use_graph, void* scratch,
+                             size_t scratch_bytes, int* iterations, double* final_relres,
+                             tomos_stream_t stream);
+
 /* ---- Adjoint of the least-squares solve (implicit differentiation) ----- */
 /* x = argmin |A x - b| with upstream gradient g = dL/dx:
  *   w = (A^T A)^-1 g,  grad_b = A w,  grad_A = r w^T - (A w) x^T,  r = b - A x.

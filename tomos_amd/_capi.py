@@ -49,6 +49,11 @@ class QrProblem(ctypes.Structure):
     _fields_ = [("k", i64), ("n", i64), ("lda", i64), ("a", vp)]
 
 
+class LsqProblem(ctypes.Structure):
+    _fields_ = [("m", i64), ("n", i64), ("lda", i64), ("a", vp), ("wr", vp), ("b", vp), ("x", vp),
+                ("rinv", vp), ("ldr", i64)]
+
+
 class LsqrOptions(ctypes.Structure):
     _fields_ = [("a_type", cint), ("mv_tier", cint), ("restart_tier", cint),
                 ("value_type", cint), ("refine_type", cint), ("precond_type", cint),
@@ -97,6 +102,9 @@ _SIGS = {
                                        vp]),
     "tomos_lsq_grad_scratch_size": (sz, [cint, cint, i64, i64]),
     "tomos_sketch_qr_precond_batched_scratch_size": (sz, [cint, vp]),
+    "tomos_lsqr_batched_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint]),
+    "tomos_lsqr_batched_solve": (cint, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint, cint, vp, sz,
+                                        vp, vp, vp]),
     "tomos_sketch_qr_precond_batched_f64": (cint, [cint, vp, vp, sz, vp]),
     "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),

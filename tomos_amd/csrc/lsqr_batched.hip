@@ -1,0 +1,596 @@
+// Many independent, uneven LSQR problems solved together on one GPU.
+//
+// Each problem i has its own A_i (m_i x n_i, packed planes or FP32), b_i,
+// the sketch-QR preconditioner R_i^-1 and output x_i; every problem runs
+// the restarted, right-preconditioned LSQR of the single-problem solver
+// (the control flow of fused_lsqr_impl, src/core/solver/lsqr.cpp:685-747,
+// per problem: restart check, exact-residual initialize, inner
+// iterations, per-iteration stop test) with its own FP64 scalars and stop
+// flag.  What is batched is the launch: one step of ALL problems is three
+// kernels whatever their number and shapes,
+//   restart:   grouped exact residual  u_i = b_i - A_i x_i (wh+wr, FP64
+//              accumulation), then the grouped single-pass kernel with
+//              p = 0, coefficient 1 (A_i^T u_i and |u_i|^2), then the tail;
+//   iteration: the grouped single-pass kernel (u_i <- A_i p_i - (alpha/beta)
+//              u_i and A_i^T u_i from one read of A_i) and the tail,
+// where the tail is one 1024-thread block per problem doing the n-sized
+// work: beta, A^T u / beta from the partial sums, R^-T (A^T u), alpha and
+// v, the Givens rotation, z = P w and x updates, the stop test and the
+// next p = R^-1 v.  A converged problem's blocks exit at once.
+//
+// The whole restart (1 + max_iters steps, all launches) is captured once
+// in a hipGraph and replayed `restarts` times; the host reads the done
+// flags between replays.  The grouped kernels' work lists (tile ranges per
+// block) are built on the host once per batch; their shape (row groups,
+// tiles per block) is a tunable.
+#include <algorithm>
+#include <vector>
+
+#include "codec.hpp"
+#include "common.hpp"
+#include "lsqr_fused.hpp"
+#include "tomos_hip.h"
+
+namespace tomos_hip {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kTail = 1024;
+
+struct BState {
+    double alpha, beta, rho_bar, phi_bar, rhsnorm;
+    double w_coef;    // -theta / rho of the previous step (z recurrence)
+    double u_coef;    // -alpha / beta: next u_raw = A p + u_coef u_raw
+    double one;       // 1.0: u coefficient of the restart's single-pass call
+    int done, completed, hist_len, z_fresh;
+    int pad[4];
+};
+
+// Per-problem device descriptor (the tail and residual kernels).
+struct BProb {
+    int64_t m, n, lda, ldr;
+    const void* a;
+    const uint16_t* wr;
+    const double* b;
+    double* x;
+    const double* rinv;
+    float* u;
+    double* v;
+    double* z;
+    double* pv;
+    BState* st;
+    double* hist;
+    int hist_cap;
+    int nitems;        // single-pass partial slots of this problem
+    double* sumsq;     // [nitems]
+    float* tpart;      // [nitems][n]
+};
+
+struct RowItem {
+    int prob;
+    int64_t row0, row1;
+};
+
+__global__ void k_bstate_init(BProb* P, int nprob) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nprob) return;
+    BState* s = P[i].st;
+    s->alpha = s->beta = s->rho_bar = s->phi_bar = s->rhsnorm = 0.0;
+    s->w_coef = s->u_coef = 0.0;
+    s->one = 1.0;
+    s->done = s->completed = s->hist_len = 0;
+    s->z_fresh = 1;
+}
+
+// |b_i| (FP64, fixed order), x_i = 0
+__global__ __launch_bounds__(kTail) void k_batch_setup(BProb* P) {
+    const BProb& p = P[blockIdx.x];
+    __shared__ double sh[kTail / 64];
+    double a = 0.0;
+    for (int64_t i = threadIdx.x; i < p.m; i += kTail) a += p.b[i] * p.b[i];
+    for (int64_t j = threadIdx.x; j < p.n; j += kTail) p.x[j] = 0.0;
+    for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kTail / 64; ++w) t += sh[w];
+        p.st->rhsnorm = sqrt(t);
+        if (p.st->rhsnorm == 0.0 || p.n == 0) p.st->done = 1;
+    }
+}
+
+// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708)
+__global__ void k_batch_restart_check(BProb* P, int nprob, double tol, double maxrr) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nprob) return;
+    BState* s = P[i].st;
+    if (s->done || s->completed == 0) return;
+    const double rr = fabs(s->phi_bar) / s->rhsnorm;
+    if (s->hist_len < P[i].hist_cap) P[i].hist[s->hist_len] = rr;
+    s->hist_len += 1;
+    if (rr < tol || rr < maxrr) s->done = 1;
+}
+
+// u_i = b_i - A_i x_i for a range of rows: exact decode (wh+wr for packed),
+// FP64 accumulation, FP32 store.  One lane per 16-byte row vector.
+template <int FMT>
+__global__ __launch_bounds__(kT) void k_batch_residual(const BProb* __restrict__ P,
+                                                       const RowItem* __restrict__ items) {
+    constexpr int RPL = FMT == 0 ? 4 : 8;  // FP32 / packed rows per 16-byte vector
+    const RowItem it = items[blockIdx.x];
+    const BProb& p = P[it.prob];
+    if (p.st->done) return;
+    const int64_t r = it.row0 + static_cast<int64_t>(threadIdx.x) * RPL;
+    if (r >= it.row1) return;
+    const int nv = static_cast<int>(min<int64_t>(RPL, p.m - r));
+    double acc[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+    for (int64_t j = 0; j < p.n; ++j) {
+        const double xj = p.x[j];
+        float av[RPL];
+        if constexpr (FMT == 0) {
+            const uint4 h = *reinterpret_cast<const uint4*>(static_cast<const float*>(p.a) + r + j * p.lda);
+            av[0] = __uint_as_float(h.x);
+            av[1] = __uint_as_float(h.y);
+            av[2] = __uint_as_float(h.z);
+            av[3] = __uint_as_float(h.w);
+        } else {
+            const uint16_t* wh = static_cast<const uint16_t*>(p.a) + r + j * p.lda;
+            const uint4 h = *reinterpret_cast<const uint4*>(wh);
+            const uint4 l = *reinterpret_cast<const uint4*>(p.wr + r + j * p.lda);
+            const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    const uint32_t hh = half ? hw[k] >> 16 : hw[k] & 0xFFFFu;
+                    const uint32_t ll = half ? lw[k] >> 16 : lw[k] & 0xFFFFu;
+                    av[2 * k + half] = FMT == 1 ? __uint_as_float((hh << 16) | ll)
+                                                : codec::fp32_16_decode_exact(hh, ll);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xj, acc[k]);
+    }
+    for (int k = 0; k < nv; ++k) p.u[r + k] = static_cast<float>(p.b[r + k] - acc[k]);
+}
+
+__device__ __forceinline__ double tail_sum(double v, double* sh) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kTail / 64; ++i) t += sh[i];  // same order in every thread
+    return t;
+}
+
+// The n-sized part of one LSQR step of problem blockIdx.x (lsqr.cpp:557-590
+// after A^T u, and :632-679), with P = R^-1:
+//   beta = |u_raw|, temp = A^T u_raw / beta, src = R^-T temp,
+//   q = src - beta v (iteration) or src (initialize), alpha = |q|, v = q/alpha;
+//   iteration: Givens update, z = P v_old + w_coef_prev z (z = P w), x += (phi/rho) z;
+//   initialize: rho_bar = alpha, phi_bar = beta, true-residual stop test;
+//   u_coef = -alpha/beta, stop test + history, next p = R^-1 v.
+__global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ P, int init, double tol,
+                                                      double maxrr) {
+    const BProb& p = P[blockIdx.x];
+    BState* S = p.st;
+    if (S->done) return;
+    extern __shared__ double lds[];
+    double* temp = lds;          // n
+    double* src = lds + p.n;     // n
+    __shared__ double sh[kTail / 64];
+    __shared__ double bc[4];
+    const int64_t n = p.n;
+    // state read before any barrier (thread 0 rewrites it at the end)
+    const double rho_bar = S->rho_bar, phi_bar = S->phi_bar, wc_prev = S->w_coef;
+    const bool fresh = S->z_fresh != 0;
+    double ss = 0.0;
+    for (int k = threadIdx.x; k < p.nitems; k += kTail) ss += p.sumsq[k];
+    const double beta = sqrt(tail_sum(ss, sh));
+    const double inv_beta = 1.0 / beta;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        double a = 0.0;
+        for (int k = 0; k < p.nitems; ++k) a += static_cast<double>(p.tpart[static_cast<int64_t>(k) * n + j]);
+        temp[j] = a * inv_beta;
+    }
+    __syncthreads();
+    // src = R^-T temp: column j of R^-1 (upper: rows 0..j) dotted with temp,
+    // one wave per column, lanes along the column
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t j = wave; j < n; j += kTail / 64) {
+        const double* col = p.rinv + j * p.ldr;
+        double a = 0.0;
+        for (int64_t i = lane; i <= j; i += 64) a += col[i] * temp[i];
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) src[j] = a;
+    }
+    __syncthreads();
+    double qq = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        const double q = init ? src[j] : src[j] - beta * p.v[j];
+        qq += q * q;
+    }
+    const double alpha = sqrt(tail_sum(qq, sh));
+    if (threadIdx.x == 0) {
+        if (init) {
+            bc[0] = 0.0;
+            bc[1] = 0.0;
+        } else {
+            const double rho = hypot(rho_bar, beta);
+            const double c = rho_bar / rho, sn = beta / rho;
+            const double theta = sn * alpha;
+            const double phi = c * phi_bar;
+            S->rho_bar = -c * alpha;
+            S->phi_bar = sn * phi_bar;
+            bc[0] = phi / rho;     // x coefficient
+            bc[1] = -theta / rho;  // w coefficient
+        }
+    }
+    __syncthreads();
+    const double xc = bc[0], wc = bc[1];
+    const double inv_alpha = 1.0 / alpha;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        const double q = init ? src[j] : src[j] - beta * p.v[j];
+        if (!init) {
+            const double pj = p.pv[j];
+            const double zj = fresh ? pj : pj + wc_prev * p.z[j];
+            p.z[j] = zj;
+            p.x[j] += xc * zj;
+        }
+        p.v[j] = q * inv_alpha;
+    }
+    __syncthreads();
+    // next p = R^-1 v (upper: row i uses columns i..n-1)
+    for (int64_t i = threadIdx.x; i < n; i += kTail) {
+        double a = 0.0;
+        for (int64_t j = i; j < n; ++j) a += p.rinv[i + j * p.ldr] * p.v[j];
+        p.pv[i] = a;
+    }
+    if (threadIdx.x != 0) return;
+    S->alpha = alpha;
+    S->beta = beta;
+    S->u_coef = -alpha / beta;
+    if (init) {
+        S->rho_bar = alpha;
+        S->phi_bar = beta;
+        S->z_fresh = 1;
+        S->w_coef = 0.0;
+        const double rr = beta / S->rhsnorm;
+        if (rr < tol || rr < maxrr) S->done = 1;
+        return;
+    }
+    S->z_fresh = 0;
+    S->w_coef = wc;
+    S->completed += 1;
+    const double rr = fabs(S->phi_bar) / S->rhsnorm;
+    if (S->hist_len < p.hist_cap) p.hist[S->hist_len] = rr;
+    S->hist_len += 1;
+    if (rr < tol || rr < maxrr) S->done = 1;
+}
+
+size_t a256(size_t v) { return (v + 255) / 256 * 256; }
+
+struct Plan {
+    int g = 0, rounds = 0;
+    int64_t tiles_per_item = 1;
+    int nitems = 0, nrow_items = 0;
+    int64_t nmax = 0;
+    std::vector<GroupItem> items;
+    std::vector<RowItem> row_items;
+    std::vector<int> prob_items;  // items per problem
+    // scratch layout
+    size_t probs, gprobs_iter, gprobs_init, items_off, row_items_off, zeros, states, hist, sumsq,
+        tpart, vecs, total;
+    std::vector<size_t> sumsq_at, tpart_at, vec_at, hist_at;
+    int hist_cap = 0;
+};
+
+int rpl_for(int a_type) { return a_type == TOMOS_F32 ? 4 : 8; }
+
+Plan make_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr, int g_req,
+               int tiles_req) {
+    Plan L;
+    for (int i = 0; i < nprob; ++i) L.nmax = std::max(L.nmax, pr[i].n);
+    int g = g_req;
+    if (g <= 0 || fused_rounds(g, L.nmax) == 0) g = lsqr_fused_default_g(L.nmax);
+    L.g = g;
+    L.rounds = fused_rounds(g, L.nmax);
+    require(L.rounds > 0, "lsqr_batched: n too large for the single-pass kernel (max 2048)");
+    const int64_t tr = fused_tile_rows(o.a_type, g);
+    int64_t tiles_total = 0;
+    for (int i = 0; i < nprob; ++i) tiles_total += ceil_div(std::max<int64_t>(pr[i].m, 1), tr);
+    if (tiles_req > 0) {
+        L.tiles_per_item = tiles_req;
+    } else {
+        // about four waves of resident blocks over the whole batch
+        const int64_t resident = static_cast<int64_t>(grouped_fused_occupancy(o.a_type, g, L.rounds)) * cu_count();
+        L.tiles_per_item = std::max<int64_t>(1, ceil_div(tiles_total, 4 * resident));
+    }
+    L.prob_items.resize(nprob);
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t nt = pr[i].n > 0 ? ceil_div(std::max<int64_t>(pr[i].m, 1), tr) : 0;
+        int k = 0;
+        for (int64_t t = 0; t < nt; t += L.tiles_per_item, ++k)
+            L.items.push_back(GroupItem{i, k, t, std::min(nt, t + L.tiles_per_item)});
+        L.prob_items[i] = k;
+        const int64_t rows_per = static_cast<int64_t>(kT) * rpl_for(o.a_type);
+        for (int64_t r = 0; pr[i].n > 0 && r < pr[i].m; r += rows_per)
+            L.row_items.push_back(RowItem{i, r, std::min(pr[i].m, r + rows_per)});
+    }
+    L.nitems = static_cast<int>(L.items.size());
+    L.nrow_items = static_cast<int>(L.row_items.size());
+    const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
+    L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 20));
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t at = off;
+        off = a256(off + b);
+        return at;
+    };
+    L.probs = take(sizeof(BProb) * nprob);
+    L.gprobs_iter = take(sizeof(GroupProb) * nprob);
+    L.gprobs_init = take(sizeof(GroupProb) * nprob);
+    L.items_off = take(sizeof(GroupItem) * std::max(L.nitems, 1));
+    L.row_items_off = take(sizeof(RowItem) * std::max(L.nrow_items, 1));
+    L.zeros = take(static_cast<size_t>(std::max<int64_t>(L.nmax, 1)) * 8);
+    L.states = take(sizeof(BState) * nprob);
+    for (int i = 0; i < nprob; ++i) {
+        L.hist_at.push_back(take(static_cast<size_t>(L.hist_cap) * 8));
+        L.sumsq_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[i], 1)) * 8));
+        L.tpart_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[i], 1)) *
+                                  std::max<int64_t>(pr[i].n, 1) * 4));
+        // u (FP32, m) then v, z, pv (FP64, n)
+        L.vec_at.push_back(take(a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 4) +
+                                3 * a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8)));
+    }
+    L.total = off;
+    return L;
+}
+
+void validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr) {
+    require(nprob >= 0 && (nprob == 0 || pr), "lsqr_batched: bad problem list");
+    require(o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_BF16 || o.a_type == TOMOS_FP32_16,
+            "lsqr_batched: a_type must be F32, FP32_16 or FP32_BF16");
+    require(o.mv_tier == TOMOS_TIER_DEFAULT, "lsqr_batched: iterations run at the DEFAULT tier");
+    require(o.value_type == TOMOS_F32 && o.refine_type == TOMOS_F64 && o.precond_type == TOMOS_F64,
+            "lsqr_batched: value_type F32, refine_type F64 and precond_type F64 (R^-1) are required");
+    require(o.max_iters >= 0 && o.restarts >= 0, "lsqr_batched: negative iteration limits");
+    const int vec = o.a_type == TOMOS_F32 ? 4 : 8;
+    const bool packed = o.a_type != TOMOS_F32;
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_lsq_problem& p = pr[i];
+        require(p.m >= 0 && p.n >= 0 && p.n <= 2048, "lsqr_batched: sizes (n <= 2048)");
+        if (p.n == 0) continue;
+        require(p.a && p.b && p.x && p.rinv, "lsqr_batched: null pointer in a problem");
+        require(!packed || p.wr, "lsqr_batched: packed A needs the wr plane (exact restart residual)");
+        require(p.lda >= p.m && p.lda % vec == 0, "lsqr_batched: lda must be >= m and a multiple of the 16-byte row vector");
+        require(reinterpret_cast<uintptr_t>(p.a) % 16 == 0 && (!packed || reinterpret_cast<uintptr_t>(p.wr) % 16 == 0),
+                "lsqr_batched: A planes must be 16-byte aligned");
+        require(p.ldr >= p.n, "lsqr_batched: ldr < n");
+    }
+}
+
+struct Batched {
+    tomos_lsqr_options o;
+    int nprob;
+    const tomos_lsq_problem* pr;
+    Plan L;
+    char* sc;
+    hipStream_t s;
+    BProb* dprobs() const { return reinterpret_cast<BProb*>(sc + L.probs); }
+    GroupProb* giter() const { return reinterpret_cast<GroupProb*>(sc + L.gprobs_iter); }
+    GroupProb* ginit() const { return reinterpret_cast<GroupProb*>(sc + L.gprobs_init); }
+    GroupItem* ditems() const { return reinterpret_cast<GroupItem*>(sc + L.items_off); }
+    RowItem* drows() const { return reinterpret_cast<RowItem*>(sc + L.row_items_off); }
+    BState* states() const { return reinterpret_cast<BState*>(sc + L.states); }
+
+    void upload() {
+        std::vector<BProb> bp(nprob);
+        std::vector<GroupProb> gi(nprob), gn(nprob);
+        for (int i = 0; i < nprob; ++i) {
+            const tomos_lsq_problem& p = pr[i];
+            BProb& q = bp[i];
+            q.m = p.m;
+            q.n = p.n;
+            q.lda = p.lda;
+            q.ldr = p.ldr;
+            q.a = p.a;
+            q.wr = p.wr;
+            q.b = p.b;
+            q.x = p.x;
+            q.rinv = p.rinv;
+            char* vb = sc + L.vec_at[i];
+            q.u = reinterpret_cast<float*>(vb);
+            const size_t ub = a256(static_cast<size_t>(std::max<int64_t>(p.m, 1)) * 4);
+            const size_t nb = a256(static_cast<size_t>(std::max<int64_t>(p.n, 1)) * 8);
+            q.v = reinterpret_cast<double*>(vb + ub);
+            q.z = reinterpret_cast<double*>(vb + ub + nb);
+            q.pv = reinterpret_cast<double*>(vb + ub + 2 * nb);
+            q.st = states() + i;
+            q.hist = reinterpret_cast<double*>(sc + L.hist_at[i]);
+            q.hist_cap = L.hist_cap;
+            q.nitems = L.prob_items[i];
+            q.sumsq = reinterpret_cast<double*>(sc + L.sumsq_at[i]);
+            q.tpart = reinterpret_cast<float*>(sc + L.tpart_at[i]);
+            GroupProb g;
+            g.m = p.m;
+            g.n = p.n;
+            g.lda = p.lda;
+            g.a = p.a;
+            g.p = q.pv;
+            g.u = q.u;
+            g.u_coef = &q.st->u_coef;
+            g.skip = &q.st->done;
+            g.sumsq = q.sumsq;
+            g.tpart = q.tpart;
+            gi[i] = g;
+            g.p = reinterpret_cast<const double*>(sc + L.zeros);
+            g.u_coef = &q.st->one;
+            gn[i] = g;
+        }
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dprobs(), bp.data(), sizeof(BProb) * nprob, hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipMemcpyAsync(giter(), gi.data(), sizeof(GroupProb) * nprob, hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipMemcpyAsync(ginit(), gn.data(), sizeof(GroupProb) * nprob, hipMemcpyHostToDevice, s));
+        if (L.nitems)
+            TOMOS_HIP_CHECK(hipMemcpyAsync(ditems(), L.items.data(), sizeof(GroupItem) * L.nitems,
+                                           hipMemcpyHostToDevice, s));
+        if (L.nrow_items)
+            TOMOS_HIP_CHECK(hipMemcpyAsync(drows(), L.row_items.data(), sizeof(RowItem) * L.nrow_items,
+                                           hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipMemsetAsync(sc + L.zeros, 0, static_cast<size_t>(std::max<int64_t>(L.nmax, 1)) * 8, s));
+        // the host vectors must outlive the copies
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    size_t tail_lds() const { return static_cast<size_t>(std::max<int64_t>(L.nmax, 1)) * 16; }
+
+    void tail(bool init) {
+        k_batch_tail<<<static_cast<unsigned>(nprob), kTail, tail_lds(), s>>>(dprobs(), init ? 1 : 0, o.tol,
+                                                                             o.max_relres);
+    }
+    void residual() {
+        if (!L.nrow_items) return;
+        if (o.a_type == TOMOS_F32)
+            k_batch_residual<0><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
+        else if (o.a_type == TOMOS_FP32_BF16)
+            k_batch_residual<1><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
+        else
+            k_batch_residual<2><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
+    }
+    void fused(bool init) {
+        grouped_fused_launch(o.a_type, L.g, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s);
+    }
+    // One restart: check, exact residual, A^T u, tail(init), then max_iters steps.
+    void restart_sequence() {
+        k_batch_restart_check<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob, o.tol,
+                                                                                      o.max_relres);
+        residual();
+        fused(true);
+        tail(true);
+        for (int it = 0; it < o.max_iters; ++it) {
+            fused(false);
+            tail(false);
+        }
+    }
+
+    bool all_done(std::vector<BState>& h) {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), states(), sizeof(BState) * nprob, hipMemcpyDeviceToHost, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+        for (const BState& b : h)
+            if (!b.done) return false;
+        return true;
+    }
+
+    void run(bool use_graph, int* iterations, double* final_relres) {
+        upload();
+        k_bstate_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
+        k_batch_setup<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
+        TOMOS_LAUNCH_CHECK("lsqr_batched setup");
+        const size_t lds = tail_lds();
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_tail),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+        std::vector<BState> h(nprob);
+        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph = nullptr;
+        if (use_graph && o.restarts > 0) {
+            TOMOS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            try {
+                restart_sequence();
+            } catch (...) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(s, &g2);
+                if (g2) (void)hipGraphDestroy(g2);
+                throw;
+            }
+            TOMOS_HIP_CHECK(hipStreamEndCapture(s, &graph));
+            TOMOS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        try {
+            for (int r = 0; r < o.restarts; ++r) {
+                if (all_done(h)) break;
+                if (exec) TOMOS_HIP_CHECK(hipGraphLaunch(exec, s));
+                else restart_sequence();
+                TOMOS_LAUNCH_CHECK("lsqr_batched restart");
+            }
+            all_done(h);
+        } catch (...) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        if (exec) TOMOS_HIP_CHECK(hipGraphExecDestroy(exec));
+        if (graph) TOMOS_HIP_CHECK(hipGraphDestroy(graph));
+        for (int i = 0; i < nprob; ++i) {
+            if (iterations) iterations[i] = h[i].completed;
+            if (final_relres) final_relres[i] = h[i].rhsnorm > 0 ? std::fabs(h[i].phi_bar) / h[i].rhsnorm : 0.0;
+        }
+    }
+};
+
+}  // namespace
+
+}  // namespace tomos_hip
+
+using namespace tomos_hip;
+
+extern "C" {
+
+size_t tomos_lsqr_batched_scratch_size(const tomos_lsqr_options* o, int nprob,
+                                       const tomos_lsq_problem* probs, int row_groups,
+                                       int tiles_per_block) {
+    size_t out = 0;
+    const int rc = guarded([&] {
+        require(o != nullptr, "tomos_lsqr_batched_scratch_size: null options");
+        validate(*o, nprob, probs);
+        out = make_plan(*o, nprob, probs, row_groups, tiles_per_block).total;
+    });
+    return rc == 0 ? out : 0;
+}
+
+int tomos_lsqr_batched_solve(const tomos_lsqr_options* o, int nprob, const tomos_lsq_problem* probs,
+                             int row_groups, int tiles_per_block, int use_graph, void* scratch,
+                             size_t scratch_bytes, int* iterations, double* final_relres,
+                             tomos_stream_t s) {
+    return guarded([&] {
+        require(o != nullptr, "tomos_lsqr_batched_solve: null options");
+        validate(*o, nprob, probs);
+        if (nprob == 0) return;
+        Batched B;
+        B.o = *o;
+        B.nprob = nprob;
+        B.pr = probs;
+        B.L = make_plan(*o, nprob, probs, row_groups, tiles_per_block);
+        if (!scratch) scratch = fallback_scratch(B.L.total);
+        else if (scratch_bytes < B.L.total)
+            throw Error(kErrWorkspace, "lsqr_batched: scratch too small (need " + std::to_string(B.L.total) +
+                                           " bytes, have " + std::to_string(scratch_bytes) + ")");
+        B.sc = static_cast<char*>(scratch);
+        // a graph cannot be captured from the legacy NULL stream: run on a
+        // private stream (default priority), ordered after earlier NULL-stream work
+        hipStream_t own = nullptr;
+        if (use_graph && s == nullptr) TOMOS_HIP_CHECK(hipStreamCreate(&own));
+        B.s = own ? own : s;
+        try {
+            B.run(use_graph != 0, iterations, final_relres);
+        } catch (...) {
+            if (own) {
+                (void)hipStreamSynchronize(own);
+                (void)hipStreamDestroy(own);
+            }
+            throw;
+        }
+        if (own) {
+            TOMOS_HIP_CHECK(hipStreamSynchronize(own));
+            TOMOS_HIP_CHECK(hipStreamDestroy(own));
+        }
+    });
+}
+
+}  // extern "C"
