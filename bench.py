@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--fused", type=int, default=1,
                    help="single-pass LSQR matvec kernel (A read once per iteration)")
     p.add_argument("--gemm", type=int, default=8192, help="split GEMM size (0 = skip)")
+    p.add_argument("--batched", type=int, default=64,
+                   help="batched differentiable least squares: number of uneven problems (0 = skip)")
     p.add_argument("--kernels-only", action="store_true",
                    help="only the hi-only GEMV-N at the full shape and a copy of the same bytes "
                         "(for hardware-counter runs); prints their timings")
@@ -84,6 +86,123 @@ def time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1):
     f = ctypes.c_float()
     C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
     return f.value / args.gemv_reps
+
+
+def batched_leg(args, C, DeviceBuffer, ev0, ev1):
+    """64 uneven problems: m ~ U{20000..200000}, n ~ U{64..512}, kappa
+    log-uniform in [1e2, 1e5] (seed 7), A = G diag(d) stored FP32_BF16,
+    b = A x_true.  Forward: CountSketch (s = 4n) + batched sketch-QR +
+    batched LSQR to 1e-6 (one hipGraph per restart).  Backward: for
+    g ~ N(0,1), w = (A^T A)^-1 g per problem (adjoint CG) and grad_b, grad_A
+    (rank-2 kernel, m x n FP32 each).  Also the same forward one problem at
+    a time with the single-problem solver."""
+    import ctypes
+    rng = np.random.default_rng(7)
+    P = args.batched
+    ms = rng.integers(20000, 200001, P)
+    ns = rng.integers(64, 513, P)
+    kap = 10.0 ** rng.uniform(2, 5, P)
+    probs, keep = [], []
+    for i in range(P):
+        m, n = int(ms[i]), int(ns[i])
+        lda = (m + 7) // 8 * 8
+        d = DeviceBuffer.from_numpy(np.logspace(0, -np.log10(kap[i]), n).astype(np.float32))
+        A = DeviceBuffer(lda * n * 4)
+        C.call("tomos_fill_normal_f32", A.ptr, m, n, lda, 100 + i, d.ptr, 0, 0, None)
+        wh, wr = DeviceBuffer(lda * n * 2), DeviceBuffer(lda * n * 2)
+        C.call("tomos_pack", C.PACK_FP32_BF16, m, n, A.ptr, lda, wh.ptr, wr.ptr, lda, None)
+        del A
+        xt = DeviceBuffer.from_numpy(rng.standard_normal(n))
+        bd = DeviceBuffer(m * 8)
+        C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, 0, 1.0, m, n, wh.ptr, wr.ptr, lda, C.F64, xt.ptr, 0.0,
+               C.F64, bd.ptr, None, 0, None)
+        s = 4 * n
+        probs.append(dict(m=m, n=n, lda=lda, wh=wh, wr=wr, b=bd, xt=xt, s=s, sk=DeviceBuffer(s * n * 8),
+                          x=DeviceBuffer(n * 8)))
+    cs_bytes = max(C.lib().tomos_countsketch_scratch_size(p["m"], p["n"], p["s"]) for p in probs)
+    cs = DeviceBuffer(cs_bytes)
+    qp = (C.QrProblem * P)(*[C.QrProblem(p["s"], p["n"], p["s"], p["sk"].ptr) for p in probs])
+    qsb = C.lib().tomos_sketch_qr_precond_batched_scratch_size(P, qp)
+    qs = DeviceBuffer(qsb)
+    o = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, 30, 40, 8, 1)
+    lp = (C.LsqProblem * P)(*[C.LsqProblem(p["m"], p["n"], p["lda"], p["wh"].ptr, p["wr"].ptr, p["b"].ptr,
+                                           p["x"].ptr, p["sk"].ptr, p["s"]) for p in probs])
+    sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), P, lp, 0, 0)
+    scr = DeviceBuffer(sb)
+    its = (ctypes.c_int * P)()
+    rr = (ctypes.c_double * P)()
+    C.call("tomos_device_synchronize")
+    log("batched: %d problems, %.2f G elements; data ready" % (P, float((ms * ns).sum()) / 1e9))
+
+    def forward():
+        t0 = time.perf_counter()
+        for p in probs:
+            C.call("tomos_countsketch_apply", C.FP32_BF16, C.TIER_EXACT, p["m"], p["n"], p["wh"].ptr, p["wr"].ptr,
+                   p["lda"], p["s"], p["sk"].ptr, p["s"], cs.ptr, cs_bytes, None)
+        C.call("tomos_sketch_qr_precond_batched_f64", P, qp, qs.ptr, qsb, None)
+        t1 = time.perf_counter()
+        C.call("tomos_lsqr_batched_solve", ctypes.byref(o), P, lp, 0, 0, 1, scr.ptr, sb, its, rr, None)
+        C.call("tomos_device_synchronize")
+        t2 = time.perf_counter()
+        return (t1 - t0) * 1e3, (t2 - t1) * 1e3
+
+    forward()  # warm-up (graph instantiation, code objects)
+    pre_ms, solve_ms = forward()
+    # true relative residuals (exact A, FP64)
+    worst = 0.0
+    for p in probs:
+        r = DeviceBuffer(p["m"] * 8)
+        C.call("tomos_memcpy_dtod", r.ptr, p["b"].ptr, p["m"] * 8, None)
+        C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, 0, -1.0, p["m"], p["n"], p["wh"].ptr, p["wr"].ptr,
+               p["lda"], C.F64, p["x"].ptr, 1.0, C.F64, r.ptr, None, 0, None)
+        rn = np.linalg.norm(r.to_numpy(np.float64, (p["m"],)))
+        bn = np.linalg.norm(p["b"].to_numpy(np.float64, (p["m"],)))
+        worst = max(worst, rn / bn)
+    # the same forward solves one problem at a time (single-problem solver)
+    so = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, 30, 40, 8, 1)
+    mmax = int(ms.max())
+    nmax = int(ns.max())
+    sarena = DeviceBuffer(C.lib().tomos_lsqr_arena_size(ctypes.byref(so), mmax, nmax))
+    sscr = DeviceBuffer(max(C.lib().tomos_lsqr_scratch_size(ctypes.byref(so), p["m"], p["n"]) for p in probs))
+    xs = DeviceBuffer(nmax * 8)
+    C.call("tomos_device_synchronize")
+    t0 = time.perf_counter()
+    for p in probs:
+        it = ctypes.c_int(0)
+        C.call("tomos_lsqr_solve", ctypes.byref(so), p["m"], p["n"], p["wh"].ptr, p["wr"].ptr, p["lda"], p["b"].ptr,
+               xs.ptr, p["sk"].ptr, p["s"], sarena.ptr, sarena.nbytes, sscr.ptr, sscr.nbytes, ctypes.byref(it),
+               None, 0, None, None)
+    C.call("tomos_device_synchronize")
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    # backward: adjoint solve + gradients for every problem
+    ao = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, 30, 40, 8, 1)
+    asb = max(C.lib().tomos_lsq_adjoint_scratch_size(ctypes.byref(ao), p["m"], p["n"]) for p in probs)
+    ascr = DeviceBuffer(asb)
+    gsb = max(C.lib().tomos_lsq_grad_scratch_size(C.FP32_BF16, C.TIER_EXACT, p["m"], p["n"]) for p in probs)
+    gscr = DeviceBuffer(gsb)
+    gA = DeviceBuffer(max(p["m"] * p["n"] for p in probs) * 4)
+    gb = DeviceBuffer(mmax * 8)
+    wd = DeviceBuffer(nmax * 8)
+    gs = [DeviceBuffer.from_numpy(rng.standard_normal(p["n"])) for p in probs]
+    adj_its = []
+    C.call("tomos_device_synchronize")
+    t0 = time.perf_counter()
+    for p, g in zip(probs, gs):
+        it = ctypes.c_int(0)
+        C.call("tomos_lsq_adjoint_solve", ctypes.byref(ao), p["m"], p["n"], p["wh"].ptr, p["wr"].ptr, p["lda"],
+               g.ptr, p["sk"].ptr, p["s"], wd.ptr, ascr.ptr, asb, ctypes.byref(it), None, 0, None, None)
+        C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, p["m"], p["n"], p["wh"].ptr, p["wr"].ptr, p["lda"],
+               p["b"].ptr, p["x"].ptr, wd.ptr, gb.ptr, gA.ptr, p["m"], gscr.ptr, gsb, None)
+        adj_its.append(it.value)
+    C.call("tomos_device_synchronize")
+    bwd_ms = (time.perf_counter() - t0) * 1e3
+    its_l = list(its)
+    return {"problems": P, "elements": int((ms * ns).sum()), "storage": "FP32_BF16 planar",
+            "precond_ms": round(pre_ms, 2), "solve_ms": round(solve_ms, 2), "forward_ms": round(pre_ms + solve_ms, 2),
+            "solve_one_at_a_time_ms": round(seq_ms, 2), "batching_speedup": round(seq_ms / solve_ms, 2),
+            "iterations_min": min(its_l), "iterations_max": max(its_l), "worst_relres": worst,
+            "backward_ms": round(bwd_ms, 2), "adjoint_iterations_max": max(adj_its),
+            "backward": "per problem: adjoint CG (tol 1e-6) + grad_b + grad_A (m x n FP32)"}
 
 
 def kernels_only(args, C, DeviceBuffer, m, n, wh):
@@ -350,6 +469,12 @@ def main():
         log("split gemm: %s" % gemm)
         del GA, GB, planes, Cg, gscr
 
+    # ---- batched differentiable least squares (64 uneven problems) ------------
+    batched = None
+    if args.batched > 0:
+        batched = batched_leg(args, C, DeviceBuffer, ev0, ev1)
+        log("batched: %s" % batched)
+
     # ---- CPU baseline: the oracle's OpenBLAS FP32 LSQR on a row sample -------
     cpu = None
     if not args.no_cpu:
@@ -403,6 +528,7 @@ def main():
         "speedup_is_lower_bound": bool(plain and not plain["converged"]),
         "gemv": gemv_detail,
         "gemm_split": gemm,
+        "batched_lstsq": batched,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
