@@ -208,11 +208,12 @@ int tomos_matvec_pair(int a_type, int64_t m, int64_t n, const void* a, int64_t l
 
 /* Tuned launch shape of the single-pass kernel behind tomos_matvec_pair and
  * the fused LSQR iteration, per (a_type, n): row_groups (2, 4 or 8 lanes
- * share one column of a row tile; 0 = default) and blocks_per_cu (grid
- * size; 0 = as many as are resident at once).  Process-wide; consulted at
+ * share one column of a row tile; 0 = default), threads per block (256 or
+ * 512; 0 = 256 with an explicit row_groups) and blocks_per_cu (grid size;
+ * 0 = as many as are resident at once).  Process-wide; consulted at
  * every launch.  Scratch sizes do not depend on it.  Set by the autotuner
  * (tomos_amd/autotune.py); no reference counterpart. */
-int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu);
+int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu, int threads);
 int tomos_fused_config_clear(void);
 
 /* ---- CountSketch ------------------------------------------------------ */
@@ -340,16 +341,16 @@ typedef struct {
  * Options: a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, value_type F32,
  * refine_type F64, precond_type F64 (R^-1 required), n <= 2048, 16-byte
  * aligned planes with lda a multiple of 4 (F32) / 8 (packed).
- * row_groups / tiles_per_block: launch shape of the grouped single-pass
- * kernel (0 = default; chosen by the autotuner).  iterations and
+ * row_groups / threads / tiles_per_block: launch shape of the grouped
+ * single-pass kernel (0 = the tuned shape for the largest n, else default).  iterations and
  * final_relres (|phi_bar|/|b|, the solver's estimate): one per problem, or
  * NULL.  scratch: tomos_lsqr_batched_scratch_size() bytes or NULL.
  * No reference counterpart (the reference solves one problem per call). */
 size_t tomos_lsqr_batched_scratch_size(const tomos_lsqr_options* opt, int nprob,
-                                       const tomos_lsq_problem* probs, int row_groups,
+                                       const tomos_lsq_problem* probs, int row_groups, int threads,
                                        int tiles_per_block);
 int tomos_lsqr_batched_solve(const tomos_lsqr_options* opt, int nprob, const tomos_lsq_problem* probs,
-                             int row_groups, int tiles_per_block, int use_graph, void* scratch,
+                             int row_groups, int threads, int tiles_per_block, int use_graph, void* scratch,
                              size_t scratch_bytes, int* iterations, double* final_relres,
                              tomos_stream_t stream);
 

@@ -42,17 +42,17 @@ def opts(a_type, tol=1e-6, inner=30, restarts=40):
                          C.F32, C.F64, C.F64, tol, tol, inner, restarts, 8, 1)
 
 
-def solve_batched(batch, o, use_graph=1, groups=0, tiles=0):
+def solve_batched(batch, o, use_graph=1, groups=0, tiles=0, threads=0):
     P = (C.LsqProblem * len(batch))(*[
         C.LsqProblem(p["m"], p["n"], p["lda"], p["wh"].ptr, p["wr"].ptr if p["wr"] else None, p["bd"].ptr,
                      p["xd"].ptr, p["sk"].ptr, p["s"]) for p in batch])
-    sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), len(batch), P, groups, tiles)
+    sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), len(batch), P, groups, threads, tiles)
     assert sb > 0, C.lib().tomos_last_error()
     scr = DeviceBuffer(sb)
     its = (ctypes.c_int * len(batch))()
     rr = (ctypes.c_double * len(batch))()
-    C.call("tomos_lsqr_batched_solve", ctypes.byref(o), len(batch), P, groups, tiles, use_graph, scr.ptr, sb,
-           its, rr, None)
+    C.call("tomos_lsqr_batched_solve", ctypes.byref(o), len(batch), P, groups, threads, tiles, use_graph, scr.ptr,
+           sb, its, rr, None)
     xs = [p["xd"].to_numpy(np.float64, (p["n"],)) for p in batch]
     return xs, list(its), list(rr)
 
@@ -90,13 +90,11 @@ def test_batched_graph_equals_plain_and_single(capi):
         assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(ref) + 1e-12
 
 
-@pytest.mark.parametrize("groups,tiles", [(2, 0), (8, 3), (4, 1)])
-def test_batched_launch_shapes(capi, groups, tiles):
+@pytest.mark.parametrize("groups,threads,tiles", [(2, 256, 0), (8, 256, 3), (4, 256, 1), (8, 512, 0), (4, 512, 2)])
+def test_batched_launch_shapes(capi, groups, threads, tiles):
     shapes = [(5000, 40, 1), (7001, 100, 2), (3000, 300, 1)]
     batch = make_batch(shapes, 5)
-    if groups == 8 and max(n for _, n, _ in shapes) > 512:
-        pytest.skip("no kernel")
-    xs, its, _ = solve_batched(batch, opts(C.FP32_BF16), groups=groups, tiles=tiles)
+    xs, its, _ = solve_batched(batch, opts(C.FP32_BF16), groups=groups, tiles=tiles, threads=threads)
     for p, x in zip(batch, xs):
         relres = np.linalg.norm(p["b"] - p["A"].astype(np.float64) @ x) / np.linalg.norm(p["b"])
         assert relres < 2e-6, relres

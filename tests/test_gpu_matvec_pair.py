@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 HI_DECODER = {C.FP32_BF16: C.DECODE_FP32_BF16_HI, C.FP32_16: C.DECODE_FP32_16_HALF}
 PACKER = {C.FP32_BF16: C.PACK_FP32_BF16, C.FP32_16: C.PACK_FP32_16}
-MAX_N = {8: 512, 4: 1024, 2: 2048, 0: 2048}
+MAX_N = {(8, 256): 1024, (4, 256): 1024, (2, 256): 2048, (0, 0): 2048, (8, 512): 2048, (4, 512): 2048}
 
 
 def run_pair(a_type, A, lda, p, u0, coef):
@@ -42,16 +42,17 @@ def used_matrix(orc, a_type, A):
 @pytest.mark.parametrize("a_type", [C.F32, C.FP32_BF16, C.FP32_16])
 @pytest.mark.parametrize("m,lda,n", [(1000, 1000, 37), (4099, 4104, 1000), (777, 784, 64),
                                       (20000, 20000, 1500), (3001, 3008, 2048), (64, 64, 512)])
-@pytest.mark.parametrize("groups,bpc", [(0, 0), (2, 0), (4, 1), (8, 0), (4, 3)])
-def test_pair_vs_fp64(capi, orc, a_type, m, lda, n, groups, bpc):
-    if n > MAX_N[groups]:
-        pytest.skip("no kernel for %d row groups at n = %d" % (groups, n))
+@pytest.mark.parametrize("groups,threads,bpc", [(0, 0, 0), (2, 256, 0), (4, 256, 1), (8, 256, 0), (4, 256, 3),
+                                               (8, 512, 0), (4, 512, 2)])
+def test_pair_vs_fp64(capi, orc, a_type, m, lda, n, groups, threads, bpc):
+    if n > MAX_N[(groups, threads)]:
+        pytest.skip("no kernel for %d row groups x %d threads at n = %d" % (groups, threads, n))
     rng = np.random.default_rng(m + n + groups)
     A = (rng.standard_normal((m, n)) * 2.0 ** rng.integers(-8, 9, (m, n))).astype(np.float32, order="F")
     p = rng.standard_normal(n)
     u0 = rng.standard_normal(m).astype(np.float32)
     coef = -0.37
-    C.call("tomos_fused_config_set", a_type, n, groups, bpc)
+    C.call("tomos_fused_config_set", a_type, n, groups, bpc, threads)
     try:
         u, t, nrm2 = run_pair(a_type, A, lda, p, u0, coef)
     finally:
@@ -83,6 +84,7 @@ def test_pair_deterministic_and_errors(capi):
     rc = C.lib().tomos_matvec_pair(C.FP32_BF16, m, 10, ctypes.c_void_p(wh.ptr), m + 1, ctypes.c_void_p(pd.ptr),
                                    0.0, ctypes.c_void_p(ud.ptr), ctypes.c_void_p(td.ptr), None, None, 0, None)
     assert rc == 3  # TOMOS_ERR_UNSUPPORTED
-    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 1000, 8, 0) == 1  # G=8 needs n <= 512
-    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 3, 0) == 1
-    assert C.lib().tomos_fused_config_set(C.F64, 100, 4, 0) == 1
+    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 2000, 8, 0, 256) == 1  # G=8 x 256 covers n <= 1024
+    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 3, 0, 256) == 1
+    assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 4, 0, 384) == 1
+    assert C.lib().tomos_fused_config_set(C.F64, 100, 4, 0, 256) == 1

@@ -94,7 +94,7 @@ _SIGS = {
                           dbl, vp, i64, vp]),
     "tomos_gemm_split_scratch_size": (sz, [i64, i64, i64]),
     "tomos_matvec_pair_scratch_size": (sz, [cint, i64, i64]),
-    "tomos_fused_config_set": (cint, [cint, i64, cint, cint]),
+    "tomos_fused_config_set": (cint, [cint, i64, cint, cint, cint]),
     "tomos_fused_config_clear": (cint, []),
     "tomos_lsq_adjoint_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
     "tomos_lsq_adjoint_solve": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, vp, vp, i64,
@@ -102,9 +102,9 @@ _SIGS = {
                                        vp]),
     "tomos_lsq_grad_scratch_size": (sz, [cint, cint, i64, i64]),
     "tomos_sketch_qr_precond_batched_scratch_size": (sz, [cint, vp]),
-    "tomos_lsqr_batched_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint]),
-    "tomos_lsqr_batched_solve": (cint, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint, cint, vp, sz,
-                                        vp, vp, vp]),
+    "tomos_lsqr_batched_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint, cint]),
+    "tomos_lsqr_batched_solve": (cint, [ctypes.POINTER(LsqrOptions), cint, vp, cint, cint, cint, cint, vp,
+                                        sz, vp, vp, vp]),
     "tomos_sketch_qr_precond_batched_f64": (cint, [cint, vp, vp, sz, vp]),
     "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),

@@ -5,8 +5,9 @@ tile of rows x all n columns in registers.  Two shape parameters trade
 registers (occupancy) against tile height and load width:
 
 * ``row_groups`` G in {2, 4, 8}: lanes that share one column of a tile; a
-  tile is G x 16 bytes tall per column, every lane holds ceil(n G / 256)
+  tile is G x 16 bytes tall per column, every lane holds ceil(n G / T)
   16-byte vectors;
+* ``threads`` T in {256, 512} per block;
 * ``blocks_per_cu``: grid size (0 = as many blocks as are resident at once).
 
 ``tune_fused`` times every candidate with HIP events on the real matrix,
@@ -25,17 +26,17 @@ import numpy as np
 from . import _capi as C
 from ._capi import DeviceBuffer
 
-# largest n each row-group count has a kernel for
-MAX_N = {8: 512, 4: 1024, 2: 2048}
-BLOCKS_PER_CU = (0, 1, 2, 3, 4, 6, 8)
+# largest n each (row groups, threads) shape has a kernel for
+MAX_N = {(2, 256): 2048, (4, 256): 1024, (8, 256): 1024, (4, 512): 2048, (8, 512): 2048}
+BLOCKS_PER_CU = (0, 1, 2, 3, 4)
 
 
 def candidates(n):
-    """(row_groups, blocks_per_cu) pairs with a kernel for this n."""
+    """(row_groups, threads, blocks_per_cu) shapes with a kernel for this n."""
     out = []
-    for g in (2, 4, 8):
-        if n <= MAX_N[g]:
-            out.extend((g, b) for b in BLOCKS_PER_CU)
+    for (g, t), mx in MAX_N.items():
+        if n <= mx:
+            out.extend((g, t, b) for b in BLOCKS_PER_CU)
     return out
 
 
@@ -64,7 +65,7 @@ def tune_fused(a_type, m, n, a_ptr, lda, reps=5, cands=None, install=True):
     """Time every candidate shape of the single-pass kernel on A (device
     pointer a_ptr: the hi plane for packed storage, lda) and install the
     fastest.  Returns (best, results); each entry is a dict with
-    row_groups, blocks_per_cu, ms and GBps (hi plane + vectors)."""
+    row_groups, threads, blocks_per_cu, ms and GBps (hi plane + vectors)."""
     rng = np.random.default_rng(0)
     pd = DeviceBuffer.from_numpy(rng.standard_normal(n))
     ud = DeviceBuffer.from_numpy(rng.standard_normal(m).astype(np.float32))
@@ -78,10 +79,10 @@ def tune_fused(a_type, m, n, a_ptr, lda, reps=5, cands=None, install=True):
     byts = esz * m * n + 8.0 * m + 16.0 * n
     results = []
     try:
-        for g, b in (cands or candidates(n)):
-            C.call("tomos_fused_config_set", a_type, n, g, b)
+        for g, t, b in (cands or candidates(n)):
+            C.call("tomos_fused_config_set", a_type, n, g, b, t)
             ms = _time_pair(a_type, m, n, a_ptr, lda, reps, (pd, ud, td, sc, scb), ev)
-            results.append({"row_groups": g, "blocks_per_cu": b, "ms": ms,
+            results.append({"row_groups": g, "threads": t, "blocks_per_cu": b, "ms": ms,
                             "GBps": byts / (ms * 1e-3) / 1e9})
     finally:
         C.call("tomos_fused_config_clear")
@@ -89,7 +90,7 @@ def tune_fused(a_type, m, n, a_ptr, lda, reps=5, cands=None, install=True):
             C.lib().tomos_event_destroy(e)
     best = min(results, key=lambda r: r["ms"])
     if install:
-        C.call("tomos_fused_config_set", a_type, n, best["row_groups"], best["blocks_per_cu"])
+        C.call("tomos_fused_config_set", a_type, n, best["row_groups"], best["blocks_per_cu"], best["threads"])
     return best, results
 
 
@@ -114,7 +115,8 @@ class TuneCache:
         k = 0
         for key, e in self.data.get(device_key(), {}).items():
             _, a_type, n = key.split("/")
-            C.call("tomos_fused_config_set", int(a_type), int(n), e["row_groups"], e["blocks_per_cu"])
+            C.call("tomos_fused_config_set", int(a_type), int(n), e["row_groups"], e["blocks_per_cu"],
+                   e.get("threads", 256))
             k += 1
         return k
 

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
 
 struct Plan {
-    int g = 0, rounds = 0;
+    int g = 0, threads = 256, rounds = 0;
     int64_t tiles_per_item = 1;
     int nitems = 0, nrow_items = 0;
     int64_t nmax = 0;
@@ -294,13 +294,20 @@ struct Plan {
 int rpl_for(int a_type) { return a_type == TOMOS_F32 ? 4 : 8; }
 
 Plan make_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr, int g_req,
-               int tiles_req) {
+               int nt_req, int tiles_req) {
     Plan L;
     for (int i = 0; i < nprob; ++i) L.nmax = std::max(L.nmax, pr[i].n);
-    int g = g_req;
-    if (g <= 0 || fused_rounds(g, L.nmax) == 0) g = lsqr_fused_default_g(L.nmax);
+    int g = g_req, nt = nt_req, bpc = 0;
+    if (g <= 0 && nt <= 0) lsqr_fused_config_get(o.a_type, L.nmax, &g, &bpc, &nt);
+    if (nt <= 0) nt = 256;
+    if (g <= 0 || fused_rounds(g, nt, L.nmax) == 0) {
+        const FusedShape d = lsqr_fused_default_shape(L.nmax);
+        g = d.g;
+        nt = d.threads;
+    }
     L.g = g;
-    L.rounds = fused_rounds(g, L.nmax);
+    L.threads = nt;
+    L.rounds = fused_rounds(g, nt, L.nmax);
     require(L.rounds > 0, "lsqr_batched: n too large for the single-pass kernel (max 2048)");
     const int64_t tr = fused_tile_rows(o.a_type, g);
     int64_t tiles_total = 0;
@@ -309,7 +316,7 @@ Plan make_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* 
         L.tiles_per_item = tiles_req;
     } else {
         // about four waves of resident blocks over the whole batch
-        const int64_t resident = static_cast<int64_t>(grouped_fused_occupancy(o.a_type, g, L.rounds)) * cu_count();
+        const int64_t resident = static_cast<int64_t>(grouped_fused_occupancy(o.a_type, g, nt, L.rounds)) * cu_count();
         L.tiles_per_item = std::max<int64_t>(1, ceil_div(tiles_total, 4 * resident));
     }
     L.prob_items.resize(nprob);
@@ -464,7 +471,7 @@ struct Batched {
             k_batch_residual<2><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
     }
     void fused(bool init) {
-        grouped_fused_launch(o.a_type, L.g, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s);
+        grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s);
     }
     // One restart: check, exact residual, A^T u, tail(init), then max_iters steps.
     void restart_sequence() {
@@ -543,19 +550,19 @@ using namespace tomos_hip;
 extern "C" {
 
 size_t tomos_lsqr_batched_scratch_size(const tomos_lsqr_options* o, int nprob,
-                                       const tomos_lsq_problem* probs, int row_groups,
+                                       const tomos_lsq_problem* probs, int row_groups, int threads,
                                        int tiles_per_block) {
     size_t out = 0;
     const int rc = guarded([&] {
         require(o != nullptr, "tomos_lsqr_batched_scratch_size: null options");
         validate(*o, nprob, probs);
-        out = make_plan(*o, nprob, probs, row_groups, tiles_per_block).total;
+        out = make_plan(*o, nprob, probs, row_groups, threads, tiles_per_block).total;
     });
     return rc == 0 ? out : 0;
 }
 
 int tomos_lsqr_batched_solve(const tomos_lsqr_options* o, int nprob, const tomos_lsq_problem* probs,
-                             int row_groups, int tiles_per_block, int use_graph, void* scratch,
+                             int row_groups, int threads, int tiles_per_block, int use_graph, void* scratch,
                              size_t scratch_bytes, int* iterations, double* final_relres,
                              tomos_stream_t s) {
     return guarded([&] {
@@ -566,7 +573,7 @@ int tomos_lsqr_batched_solve(const tomos_lsqr_options* o, int nprob, const tomos
         B.o = *o;
         B.nprob = nprob;
         B.pr = probs;
-        B.L = make_plan(*o, nprob, probs, row_groups, tiles_per_block);
+        B.L = make_plan(*o, nprob, probs, row_groups, threads, tiles_per_block);
         if (!scratch) scratch = fallback_scratch(B.L.total);
         else if (scratch_bytes < B.L.total)
             throw Error(kErrWorkspace, "lsqr_batched: scratch too small (need " + std::to_string(B.L.total) +

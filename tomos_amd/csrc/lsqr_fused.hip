@@ -105,7 +105,7 @@ constexpr int log2_small() {
 // One block's share of the single-pass product: tiles [tile0, tile1) of an
 // m x n matrix; writes the block's sum of u_raw^2 to *sumsq_out and its
 // A^T u_raw partial (n floats) to tpart_out.
-template <int FMT, int G, int ROUNDS>
+template <int FMT, int G, int ROUNDS, int NT>
 __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __restrict__ a,
                                            int64_t lda, const void* __restrict__ p, bool p64,
                                            void* __restrict__ u, bool u64, double cu, int64_t tile0,
@@ -115,7 +115,8 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
     constexpr int LOG_RPL = log2_small<RPL>();
     constexpr int LOG_G = log2_small<G>();
     constexpr int TR = G * RPL;          // rows per tile
-    constexpr int CS = kThreads / G;     // column slots per round
+    constexpr int CS = NT / G;           // column slots per round
+    constexpr int NW = NT / 64;          // waves per block
     constexpr int ESZ = FMT == FMT_F32 ? 4 : 2;
     static_assert(ROUNDS % G == 0, "ROUNDS must be a multiple of G");
     static_assert(LOG_G + LOG_RPL <= 6, "row groups and row bits must fit a wave");
@@ -140,7 +141,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) tacc[i] = 0.0f;
 
-    __shared__ float red[4][TR];
+    __shared__ float red[NW][TR];
     __shared__ float ush[TR];
     double sq = 0.0;
 
@@ -183,7 +184,9 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
             const int64_t row = row0 + tid;
             float un = 0.0f;
             if (row < m) {
-                const float s = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+                float s = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) s += red[w][tid];
                 double r = static_cast<double>(s);
                 if (cu != 0.0)  // u is not read for a zero coefficient
                     r += cu * (u64 ? static_cast<double*>(u)[row]
@@ -239,14 +242,19 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
     }
     // sum of squares of this block's u_raw rows
     for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
-    __shared__ double sqw[4];
+    __shared__ double sqw[NW];
     if (lane == 0) sqw[wave] = sq;
     __syncthreads();
-    if (tid == 0) *sumsq_out = (sqw[0] + sqw[1]) + (sqw[2] + sqw[3]);
+    if (tid == 0) {
+        double t = sqw[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) t += sqw[w];
+        *sumsq_out = t;
+    }
 }
 
-template <int FMT, int G, int ROUNDS>
-__global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
+template <int FMT, int G, int ROUNDS, int NT>
+__global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
     bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
     double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
@@ -255,19 +263,19 @@ __global__ __launch_bounds__(kThreads) void lsqr_fused_kernel(
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
     const int64_t ntiles = (m + TR - 1) / TR;
     const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
-    fused_body<FMT, G, ROUNDS>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+    fused_body<FMT, G, ROUNDS, NT>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
                                tpart + static_cast<int64_t>(blockIdx.x) * n);
 }
 
 // Many problems in one launch: block b runs work item items[b] (a tile
 // range of one problem).  A problem whose skip flag is set is a no-op.
-template <int FMT, int G, int ROUNDS>
-__global__ __launch_bounds__(kThreads) void grouped_fused_kernel(const GroupProb* __restrict__ probs,
+template <int FMT, int G, int ROUNDS, int NT>
+__global__ __launch_bounds__(NT) void grouped_fused_kernel(const GroupProb* __restrict__ probs,
                                                                  const GroupItem* __restrict__ items) {
     const GroupItem it = items[blockIdx.x];
     const GroupProb& P = probs[it.prob];
     if (P.skip && *P.skip) return;
-    fused_body<FMT, G, ROUNDS>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
+    fused_body<FMT, G, ROUNDS, NT>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
                                it.tile1, P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
 }
 
@@ -288,79 +296,95 @@ __global__ __launch_bounds__(kThreads) void fused_tsum_kernel(int64_t n, int nbl
     out[static_cast<int64_t>(sp) * n + j] = acc;
 }
 
-// Column rounds for G row groups (CS = 256/G columns per round): the
-// instantiated (G, ROUNDS) pairs.
-int rounds_for(int g, int64_t n) {
-    const int64_t cs = kThreads / g;
-    const int64_t need = ceil_div(std::max<int64_t>(n, 1), cs);
-    static const int r8[] = {8, 16};
-    static const int r4[] = {4, 8, 12, 16};
-    static const int r2[] = {2, 4, 6, 8, 12, 16};
-    const int* tab = g == 8 ? r8 : (g == 4 ? r4 : r2);
-    const int len = g == 8 ? 2 : (g == 4 ? 4 : 6);
-    for (int i = 0; i < len; ++i)
-        if (tab[i] >= need) return tab[i];
-    return 0;
+// Instantiated launch shapes: (row groups G, threads NT, column rounds R);
+// a lane covers R columns, a round CS = NT / G columns.
+struct Shape {
+    int g, nt, r;
+};
+constexpr Shape kShapes[] = {
+    {2, 256, 2}, {2, 256, 4}, {2, 256, 6}, {2, 256, 8}, {2, 256, 12}, {2, 256, 16},
+    {4, 256, 4}, {4, 256, 8}, {4, 256, 12}, {4, 256, 16},
+    {8, 256, 8}, {8, 256, 16}, {8, 256, 24}, {8, 256, 32},
+    {4, 512, 8}, {4, 512, 16},
+    {8, 512, 8}, {8, 512, 16}, {8, 512, 24}, {8, 512, 32},
+};
+
+// Smallest instantiated R covering n for (g, nt); 0 when none.
+int rounds_for(int g, int nt, int64_t n) {
+    int best = 0;
+    for (const Shape& sh : kShapes) {
+        if (sh.g != g || sh.nt != nt) continue;
+        if (static_cast<int64_t>(sh.r) * (nt / g) >= n && (best == 0 || sh.r < best)) best = sh.r;
+    }
+    return best;
 }
 
-// The kernel instance for (format, G, ROUNDS), or nullptr.
+#define TOMOS_FUSED_SHAPES(X)                                                                         \
+    X(2, 256, 2) X(2, 256, 4) X(2, 256, 6) X(2, 256, 8) X(2, 256, 12) X(2, 256, 16)                   \
+    X(4, 256, 4) X(4, 256, 8) X(4, 256, 12) X(4, 256, 16)                                               \
+    X(8, 256, 8) X(8, 256, 16) X(8, 256, 24) X(8, 256, 32)                                              \
+    X(4, 512, 8) X(4, 512, 16)                                                                          \
+    X(8, 512, 8) X(8, 512, 16) X(8, 512, 24) X(8, 512, 32)
+
+// The kernel instance for (format, shape), or nullptr.
 template <int FMT>
-const void* fused_kernel_ptr(int g, int rounds) {
-#define FUSED_CASE(G, R) \
-    if (g == G && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R>);
-    FUSED_CASE(8, 8) FUSED_CASE(8, 16)
-    FUSED_CASE(4, 4) FUSED_CASE(4, 8) FUSED_CASE(4, 12) FUSED_CASE(4, 16)
-    FUSED_CASE(2, 2) FUSED_CASE(2, 4) FUSED_CASE(2, 6) FUSED_CASE(2, 8) FUSED_CASE(2, 12) FUSED_CASE(2, 16)
+const void* fused_kernel_ptr(int g, int nt, int rounds) {
+#define FUSED_CASE(G, NT, R) \
+    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT>);
+    TOMOS_FUSED_SHAPES(FUSED_CASE)
 #undef FUSED_CASE
     return nullptr;
 }
 
 template <int FMT>
-const void* grouped_kernel_ptr(int g, int rounds) {
-#define FUSED_CASE(G, R) \
-    if (g == G && rounds == R) return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R>);
-    FUSED_CASE(8, 8) FUSED_CASE(8, 16)
-    FUSED_CASE(4, 4) FUSED_CASE(4, 8) FUSED_CASE(4, 12) FUSED_CASE(4, 16)
-    FUSED_CASE(2, 2) FUSED_CASE(2, 4) FUSED_CASE(2, 6) FUSED_CASE(2, 8) FUSED_CASE(2, 12) FUSED_CASE(2, 16)
+const void* grouped_kernel_ptr(int g, int nt, int rounds) {
+#define FUSED_CASE(G, NT, R) \
+    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R, NT>);
+    TOMOS_FUSED_SHAPES(FUSED_CASE)
 #undef FUSED_CASE
     return nullptr;
 }
 
-const void* grouped_kernel_for(int a_type, int g, int rounds) {
-    if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF>(g, rounds);
-    if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16>(g, rounds);
-    return grouped_kernel_ptr<FMT_F32>(g, rounds);
+const void* grouped_kernel_for(int a_type, int g, int nt, int rounds) {
+    if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF>(g, nt, rounds);
+    if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16>(g, nt, rounds);
+    return grouped_kernel_ptr<FMT_F32>(g, nt, rounds);
 }
 
-const void* fused_kernel_for(int a_type, int g, int rounds) {
-    if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF>(g, rounds);
-    if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16>(g, rounds);
-    return fused_kernel_ptr<FMT_F32>(g, rounds);
+const void* fused_kernel_for(int a_type, int g, int nt, int rounds) {
+    if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF>(g, nt, rounds);
+    if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16>(g, nt, rounds);
+    return fused_kernel_ptr<FMT_F32>(g, nt, rounds);
 }
 
 }  // namespace
 
-int lsqr_fused_default_g(int64_t n) { return n <= 1024 ? 4 : 2; }
+FusedShape lsqr_fused_default_shape(int64_t n) {
+    // 128 contiguous bytes per column per tile (G = 8) where that kernel
+    // fits its registers; 64 bytes (G = 4) up to n = 2048
+    if (n <= 1536) return FusedShape{8, 512};
+    return FusedShape{4, 512};
+}
 
 int64_t fused_tile_rows(int a_type, int g) { return static_cast<int64_t>(g) * (a_type == TOMOS_F32 ? 4 : 8); }
 
-int fused_rounds(int g, int64_t n) { return (g == 2 || g == 4 || g == 8) ? rounds_for(g, n) : 0; }
+int fused_rounds(int g, int threads, int64_t n) { return rounds_for(g, threads, n); }
 
-void grouped_fused_launch(int a_type, int g, int rounds, const GroupProb* probs,
+void grouped_fused_launch(int a_type, int g, int threads, int rounds, const GroupProb* probs,
                           const GroupItem* items, int nitems, hipStream_t s) {
     require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_BF16 || a_type == TOMOS_FP32_16,
             "grouped_fused: unsupported a_type");
-    const void* k = grouped_kernel_for(a_type, g, rounds);
+    const void* k = grouped_kernel_for(a_type, g, threads, rounds);
     if (!k) throw Error(kErrInternal, "grouped_fused: no kernel for this shape");
     if (nitems <= 0) return;
     void* args[] = {const_cast<GroupProb**>(&probs), const_cast<GroupItem**>(&items)};
-    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(nitems)), dim3(kThreads), args, 0, s));
+    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(nitems)), dim3(threads), args, 0, s));
 }
 
-int grouped_fused_occupancy(int a_type, int g, int rounds) {
-    const void* k = grouped_kernel_for(a_type, g, rounds);
+int grouped_fused_occupancy(int a_type, int g, int threads, int rounds) {
+    const void* k = grouped_kernel_for(a_type, g, threads, rounds);
     int occ = 0;
-    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kThreads, 0) != hipSuccess) occ = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0) != hipSuccess) occ = 0;
     return std::max(occ, 1);
 }
 
@@ -372,17 +396,23 @@ bool lsqr_fused_supported(int a_type, int tier, int64_t m, int64_t n, int64_t ld
     return lda % vec == 0 && reinterpret_cast<uintptr_t>(a) % 16 == 0;
 }
 
-FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g, int blocks_per_cu) {
+FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g, int blocks_per_cu, int threads) {
     FusedPlan p;
-    if (g <= 0 && blocks_per_cu <= 0) lsqr_fused_config_get(a_type, n, &g, &blocks_per_cu);
-    if (g <= 0 || rounds_for(g, n) == 0) g = lsqr_fused_default_g(n);
+    if (g <= 0 && blocks_per_cu <= 0 && threads <= 0) lsqr_fused_config_get(a_type, n, &g, &blocks_per_cu, &threads);
+    if (threads <= 0) threads = 256;
+    if (g <= 0 || rounds_for(g, threads, n) == 0) {
+        const FusedShape d = lsqr_fused_default_shape(n);
+        g = d.g;
+        threads = d.threads;
+    }
     p.g = g;
-    p.rounds = rounds_for(g, n);
+    p.threads = threads;
+    p.rounds = rounds_for(g, threads, n);
     if (blocks_per_cu <= 0) {
         // as many blocks as fit on the device at once: one wave of blocks
         int occ = 0;
-        const void* k = p.rounds > 0 ? fused_kernel_for(a_type, g, p.rounds) : nullptr;
-        if (k && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kThreads, 0) != hipSuccess) occ = 0;
+        const void* k = p.rounds > 0 ? fused_kernel_for(a_type, g, threads, p.rounds) : nullptr;
+        if (k && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0) != hipSuccess) occ = 0;
         blocks_per_cu = occ > 0 ? occ : 2;
     }
     blocks_per_cu = std::min(blocks_per_cu, 8);
@@ -409,14 +439,17 @@ size_t lsqr_fused_max_tpart_bytes(int64_t m, int64_t n) {
 size_t lsqr_fused_max_tsum_bytes(int64_t n) { return static_cast<size_t>(kMaxSplits) * n * 8; }
 
 namespace {
+struct Cfg {
+    int g, bpc, threads;
+};
 std::mutex g_cfg_mu;
-std::map<std::pair<int, int64_t>, std::pair<int, int>> g_cfg;  // (a_type, n) -> (g, blocks/CU)
+std::map<std::pair<int, int64_t>, Cfg> g_cfg;  // (a_type, n) -> shape
 }  // namespace
 
-void lsqr_fused_config_set(int a_type, int64_t n, int g, int blocks_per_cu) {
+void lsqr_fused_config_set(int a_type, int64_t n, int g, int blocks_per_cu, int threads) {
     std::lock_guard<std::mutex> lk(g_cfg_mu);
-    if (g <= 0 && blocks_per_cu <= 0) g_cfg.erase({a_type, n});
-    else g_cfg[{a_type, n}] = {g, blocks_per_cu};
+    if (g <= 0 && blocks_per_cu <= 0 && threads <= 0) g_cfg.erase({a_type, n});
+    else g_cfg[{a_type, n}] = Cfg{g, blocks_per_cu, threads};
 }
 
 void lsqr_fused_config_clear() {
@@ -424,12 +457,13 @@ void lsqr_fused_config_clear() {
     g_cfg.clear();
 }
 
-bool lsqr_fused_config_get(int a_type, int64_t n, int* g, int* blocks_per_cu) {
+bool lsqr_fused_config_get(int a_type, int64_t n, int* g, int* blocks_per_cu, int* threads) {
     std::lock_guard<std::mutex> lk(g_cfg_mu);
     auto it = g_cfg.find({a_type, n});
     if (it == g_cfg.end()) return false;
-    *g = it->second.first;
-    *blocks_per_cu = it->second.second;
+    *g = it->second.g;
+    *blocks_per_cu = it->second.bpc;
+    *threads = it->second.threads;
     return true;
 }
 
@@ -437,13 +471,14 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
                        bool p64, void* u, bool u64, const double* u_coef, double* sumsq,
                        float* tpart, double* tsum, const FusedPlan& plan, const int* skip,
                        hipStream_t s) {
-    require(plan.rounds > 0 && rounds_for(plan.g, n) == plan.rounds, "lsqr_fused: plan does not fit n");
-    const void* k = fused_kernel_for(a_type, plan.g, plan.rounds);
+    require(plan.rounds > 0 && rounds_for(plan.g, plan.threads, n) == plan.rounds,
+            "lsqr_fused: plan does not fit n");
+    const void* k = fused_kernel_for(a_type, plan.g, plan.threads, plan.rounds);
     if (!k) throw Error(kErrInternal, "lsqr_fused: no kernel for this plan");
     int64_t tpb = plan.tiles_per_block;
     void* args[] = {&m, &n, const_cast<void**>(&a), &lda, const_cast<void**>(&p), &p64, &u, &u64,
                     const_cast<double**>(&u_coef), &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};
-    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(kThreads), args, 0, s));
+    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(plan.threads), args, 0, s));
     TOMOS_LAUNCH_CHECK("lsqr_fused_kernel");
     fused_tsum_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, kThreads)), static_cast<unsigned>(plan.splits)),
                         kThreads, 0, s>>>(n, plan.nblocks, plan.splits, tpart, tsum, skip);
@@ -513,17 +548,18 @@ using namespace tomos_hip;
 
 extern "C" {
 
-int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu) {
+int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu, int threads) {
     return guarded([&] {
         require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16,
                 "tomos_fused_config_set: bad a_type");
         require(n >= 1 && n <= kFusedMaxN, "tomos_fused_config_set: bad n");
         require(row_groups == 0 || row_groups == 2 || row_groups == 4 || row_groups == 8,
                 "tomos_fused_config_set: row_groups must be 0, 2, 4 or 8");
-        require(row_groups == 0 || rounds_for(row_groups, n) > 0,
-                "tomos_fused_config_set: no kernel for these row_groups at this n");
+        require(threads == 0 || threads == 256 || threads == 512, "tomos_fused_config_set: threads must be 0, 256 or 512");
+        require(row_groups == 0 || rounds_for(row_groups, threads ? threads : 256, n) > 0,
+                "tomos_fused_config_set: no kernel for this shape at this n");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 8, "tomos_fused_config_set: blocks_per_cu in 0..8");
-        lsqr_fused_config_set(a_type, n, row_groups, blocks_per_cu);
+        lsqr_fused_config_set(a_type, n, row_groups, blocks_per_cu, threads);
     });
 }
 

@@ -8,8 +8,13 @@
 
 namespace tomos_hip {
 
+struct FusedShape {
+    int g, threads;
+};
+
 struct FusedPlan {
-    int g = 4;       // row groups per tile (lanes sharing one column)
+    int g = 8;       // row groups per tile (lanes sharing one column)
+    int threads = 256;
     int rounds = 0;  // column rounds per lane
     int nblocks = 0;
     int64_t tiles_per_block = 0;
@@ -21,20 +26,22 @@ struct FusedPlan {
 // A in a_type with the per-iteration (DEFAULT) tier, column-major,
 // 16-byte aligned, lda a multiple of the 16-byte vector, n <= 2048.
 bool lsqr_fused_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a);
-// g: row groups per tile (2, 4 or 8; 0 = default for n); blocks_per_cu:
-// grid size in blocks per CU (0 = default).  Unsupported g falls back to
-// the default.
-FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g = 0, int blocks_per_cu = 0);
-int lsqr_fused_default_g(int64_t n);
+// g: row groups per tile (2, 4 or 8), threads per block (256 or 512), 0 =
+// the tuned or default shape for n; blocks_per_cu: grid size in blocks per
+// CU (0 = as many as are resident).  A shape without a kernel for n falls
+// back to the default.
+FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g = 0, int blocks_per_cu = 0,
+                          int threads = 0);
+FusedShape lsqr_fused_default_shape(int64_t n);
 // upper bounds over every plan for (m, n): blocks and FusedPlan::tpart_bytes
 int64_t lsqr_fused_max_blocks(int64_t m);
 size_t lsqr_fused_max_tpart_bytes(int64_t m, int64_t n);
 size_t lsqr_fused_max_tsum_bytes(int64_t n);
 // Process-wide tuned shapes, looked up by lsqr_fused_plan when it is called
 // with g = blocks_per_cu = 0.  (0, 0) removes the entry.
-void lsqr_fused_config_set(int a_type, int64_t n, int g, int blocks_per_cu);
+void lsqr_fused_config_set(int a_type, int64_t n, int g, int blocks_per_cu, int threads);
 void lsqr_fused_config_clear();
-bool lsqr_fused_config_get(int a_type, int64_t n, int* g, int* blocks_per_cu);
+bool lsqr_fused_config_get(int a_type, int64_t n, int* g, int* blocks_per_cu, int* threads);
 
 // u_raw <- A p + (*u_coef) u_raw for all m rows; sumsq[b] = sum of u_raw^2
 // over block b's rows (plan.nblocks entries); tsum[s][j] = sum over the
@@ -67,12 +74,12 @@ struct GroupItem {
 
 // Rows per tile for (a_type, g): g * (8 packed / 4 FP32 rows per 16-byte vector).
 int64_t fused_tile_rows(int a_type, int g);
-// Column rounds of the kernel for (g, n); 0 when no kernel covers n.
-int fused_rounds(int g, int64_t n);
+// Column rounds of the kernel for (g, threads, n); 0 when no kernel covers n.
+int fused_rounds(int g, int threads, int64_t n);
 // grid = nitems blocks; rounds must cover the largest n of the problems.
-void grouped_fused_launch(int a_type, int g, int rounds, const GroupProb* probs,
+void grouped_fused_launch(int a_type, int g, int threads, int rounds, const GroupProb* probs,
                           const GroupItem* items, int nitems, hipStream_t s);
 // Blocks of the grouped kernel resident per CU (occupancy), >= 1.
-int grouped_fused_occupancy(int a_type, int g, int rounds);
+int grouped_fused_occupancy(int a_type, int g, int threads, int rounds);
 
 }  // namespace tomos_hip

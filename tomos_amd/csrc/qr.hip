@@ -8,10 +8,12 @@
 //
 // QR, blocked by panels of NB = 32 columns (LAPACK dgeqrf's structure):
 //  * panel: one launch per column c.  Every block recomputes the reflector
-//    of column c (norm of k-c doubles, L2-resident) and applies it to the
-//    remaining panel column it owns: w = v^T A(:,j) as a block reduction,
-//    then A(:,j) -= tau*w*v.  Column c itself is never written during its
-//    own step (other blocks are reading it); R's diagonal, tau and the
+//    of column c (LAPACK dlarfg convention: beta = -sign(x0)|x|,
+//    v = [1, x(1:)/(x0-beta)], tau = (beta-x0)/beta; a zero column gives no
+//    reflection) and applies it to the remaining panel column it owns: one
+//    pass computes |x|^2 and x^T A(:,j) together, a second does
+//    A(:,j) -= tau*(v^T A(:,j))*v.  Column c itself is never written during
+//    its own step (other blocks are reading it); R's diagonal, tau and the
 //    reflector scale go to separate arrays.
 //  * V and T: the panel's reflectors are written as an explicit unit lower
 //    trapezoidal V, and T (upper triangular, H_1...H_nb = I - V T V^T) is
@@ -57,74 +59,64 @@ __host__ __device__ inline int64_t qr_nsplit(int64_t k) {
     return s < 1 ? 1 : (s > 8 ? 8 : s);
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    return (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-// Reflector of column c: beta = -sign(x0)*||x||, v = [1, x(1:)/(x0-beta)],
-// tau = (beta - x0)/beta (LAPACK dlarfg convention).  Returns false for a
-// zero column (no reflection).
-struct Reflector {
-    double beta, tau, inv_v0;
-    bool active;
-};
-
-__device__ Reflector make_reflector(const double* __restrict__ col, int64_t c, int64_t k,
-                                    double* red) {
-    double s = 0.0;
-    for (int64_t i = c + 1 + threadIdx.x; i < k; i += kT) s += col[i] * col[i];
-    const double sigma = block_sum(s, red);
-    const double x0 = col[c];
-    Reflector R;
-    const double nrm = sqrt(x0 * x0 + sigma);
-    if (nrm == 0.0) {
-        R.active = false;
-        R.beta = 0.0;
-        R.tau = 0.0;
-        R.inv_v0 = 0.0;
-        return R;
-    }
-    R.active = true;
-    R.beta = (x0 >= 0.0) ? -nrm : nrm;
-    R.tau = (R.beta - x0) / R.beta;
-    R.inv_v0 = 1.0 / (x0 - R.beta);
-    return R;
-}
-
 __global__ void qr_set_desc_kernel(QrDesc* d, QrDesc v) { *d = v; }
 
-// Column c of the panel starting at c0 (one block per remaining panel column).
-__global__ __launch_bounds__(kT) void qr_panel_step_kernel(const QrDesc* __restrict__ D, int64_t c0,
-                                                           int64_t c) {
+constexpr int kPT = 1024;  // panel-step block
+
+// Column c of the panel starting at c0: block b applies the reflector of
+// column c to panel column j = c + 1 + b (block 0 also records diag, tau
+// and the reflector scale).  One pass over the rows computes both
+// |x(c+1:)|^2 and x(c+1:)^T A(c+1:, j) (v^T A(:,j) = A(c,j) + inv_v0 * that
+// dot, since v = [1, x(c+1:) * inv_v0]), a second applies the update.
+__global__ __launch_bounds__(kPT) void qr_panel_step_kernel(const QrDesc* __restrict__ D, int64_t c0,
+                                                            int64_t c) {
     const QrDesc& d = D[blockIdx.z];
     if (c >= d.n) return;
     const int64_t c_end = min(c0 + kNB, d.n);
-    if (blockIdx.x > 0 && c + 1 + blockIdx.x >= c_end) return;  // no column for this block
-    __shared__ double red[4];
+    const int64_t j = c + 1 + blockIdx.x;
+    const bool has_col = j < c_end;
+    if (blockIdx.x > 0 && !has_col) return;
+    __shared__ double red[2][kPT / 64];
     const double* col = d.a + c * d.lda;
-    const Reflector R = make_reflector(col, c, d.k, red);
+    double* cj = d.a + (has_col ? j : c) * d.lda;
+    const double x0 = col[c];
+    const double xc = has_col ? cj[c] : 0.0;  // read before the barriers: thread 0 rewrites it
+    double ss = 0.0, dot = 0.0;
+    for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) {
+        const double xi = col[i];
+        ss += xi * xi;
+        if (has_col) dot += xi * cj[i];
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        ss += __shfl_xor(ss, o);
+        dot += __shfl_xor(dot, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = ss;
+        red[1][wave] = dot;
+    }
+    __syncthreads();
+    double sigma = 0.0, vdot = 0.0;
+    for (int w = 0; w < kPT / 64; ++w) {  // same order in every thread
+        sigma += red[0][w];
+        vdot += red[1][w];
+    }
+    const double nrm = sqrt(x0 * x0 + sigma);
+    const bool active = nrm != 0.0;
+    const double beta = x0 >= 0.0 ? -nrm : nrm;
+    const double tau = active ? (beta - x0) / beta : 0.0;
+    const double inv_v0 = active ? 1.0 / (x0 - beta) : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        d.diag[c] = R.active ? R.beta : col[c];
-        d.tau[c] = R.active ? R.tau : 0.0;
-        d.inv_v0[c] = R.active ? R.inv_v0 : 0.0;
+        d.diag[c] = active ? beta : x0;
+        d.tau[c] = tau;
+        d.inv_v0[c] = inv_v0;
     }
-    if (!R.active) return;
-    for (int64_t j = c + 1 + blockIdx.x; j < c_end; j += gridDim.x) {
-        double* cj = d.a + j * d.lda;
-        // read A(c,j) before the reduction's barriers: thread 0 rewrites it below
-        const double xc = cj[c];
-        double part = 0.0;
-        for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kT) part += (col[i] * R.inv_v0) * cj[i];
-        const double w = R.tau * (xc + block_sum(part, red));
-        for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kT) cj[i] -= w * (col[i] * R.inv_v0);
-        if (threadIdx.x == 0) cj[c] -= w;
-        __syncthreads();
-    }
+    if (!active || !has_col) return;
+    const double w = tau * (xc + inv_v0 * vdot);
+    const double wv = w * inv_v0;
+    for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) cj[i] -= wv * col[i];
+    if (threadIdx.x == 0) cj[c] = xc - w;
 }
 
 // V (rows c0..k-1 used, ld = k): unit lower trapezoidal reflector columns.
@@ -166,7 +158,9 @@ __global__ __launch_bounds__(kT) void qr_gram_kernel(const QrDesc* __restrict__ 
 
 // T (kNB x kNB, column-major) from the Gram partials and tau (dlarft,
 // forward columnwise): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), T(i,i) = tau_i.
-__global__ __launch_bounds__(64) void qr_form_t_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+constexpr int kTT = kNB * kNB;  // form-T block: one thread per entry
+
+__global__ __launch_bounds__(kTT) void qr_form_t_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
     if (nbp64 <= 0 || c0 + nbp64 >= d.n) return;
@@ -176,7 +170,8 @@ __global__ __launch_bounds__(64) void qr_form_t_kernel(const QrDesc* __restrict_
     __shared__ double g[kNB * kNB];
     __shared__ double t[kNB * kNB];
     __shared__ double z[kNB];
-    for (int e = threadIdx.x; e < kNB * kNB; e += 64) {
+    {
+        const int e = threadIdx.x;
         double acc = 0.0;
         for (int ch = 0; ch < nchunks; ++ch) acc += d.gp[static_cast<int64_t>(ch) * kNB * kNB + e];
         g[e] = acc;
@@ -195,7 +190,7 @@ __global__ __launch_bounds__(64) void qr_form_t_kernel(const QrDesc* __restrict_
         if (threadIdx.x == 0) t[i + i * kNB] = ti;
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < kNB * kNB; e += 64) d.t[e] = t[e];
+    d.t[threadIdx.x] = t[threadIdx.x];
 }
 
 // W partials: wp[split][jt][p] = sum over the split's rows of V(i,p) A(i, c_end+jt).
@@ -446,7 +441,7 @@ void factor(const Batch& B, hipStream_t s) {
         const int64_t c_end = std::min<int64_t>(c0 + kNB, B.nmax);
         for (int64_t c = c0; c < c_end; ++c) {
             const unsigned blocks = static_cast<unsigned>(std::max<int64_t>(1, c_end - c - 1));
-            qr_panel_step_kernel<<<dim3(blocks, 1, P), kT, 0, s>>>(B.dev, c0, c);
+            qr_panel_step_kernel<<<dim3(blocks, 1, P), kPT, 0, s>>>(B.dev, c0, c);
         }
         TOMOS_LAUNCH_CHECK("qr_panel_step_kernel");
         if (c0 + kNB >= B.nmax) break;  // no problem has columns right of this panel
@@ -454,7 +449,7 @@ void factor(const Batch& B, hipStream_t s) {
         const int64_t nt = B.nmax - c0 - kNB;
         qr_form_v_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows * kNB, kT)), 1, P), kT, 0, s>>>(B.dev, c0);
         qr_gram_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, kGR)), 1, P), kT, 0, s>>>(B.dev, c0);
-        qr_form_t_kernel<<<dim3(1, 1, P), 64, 0, s>>>(B.dev, c0);
+        qr_form_t_kernel<<<dim3(1, 1, P), kTT, 0, s>>>(B.dev, c0);
         qr_w_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, kNB)), static_cast<unsigned>(qr_nsplit(B.kmax)), P),
                       kT, 0, s>>>(B.dev, c0);
         qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
