@@ -20,7 +20,7 @@
 //    formed from the Gram matrix V^T V (dlarft's recurrence).
 //  * trailing update A <- (I - V T^T V^T) A over the columns right of the
 //    panel, as two tiled FP64 products: W = V^T A (split over rows, partials
-//    summed in a fixed order) and A -= V (T^T W).
+//    summed in a fixed order, then W2 = T^T W once) and A -= V W2.
 // All reductions run in a fixed order (deterministic).
 //
 // Triangular inverse: one block per (column j, problem) solves R x = e_j by
@@ -51,6 +51,7 @@ struct QrDesc {
     double* gp;      // Gram partials: ceil(k/kGR) x kNB^2
     double* t;       // kNB^2
     double* wp;      // W partials: nsplit x n x kNB; the inverse's copy of R (n x n)
+    double* w2;      // T^T W: n x kNB (column jt at w2 + jt*kNB)
     int* bad;        // trtri: first zero diagonal (INT_MAX if none)
 };
 
@@ -234,23 +235,20 @@ __global__ __launch_bounds__(kT) void qr_w_kernel(const QrDesc* __restrict__ D, 
     }
 }
 
-// A(c0:k, c_end+jt) -= V (T^T W) on 64 x 64 tiles; every block forms its
-// 32 x 64 slice of T^T W from the split partials (fixed order).
-__global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+// W2 = T^T W for 64 trailing columns per block: the split partials of W
+// summed in a fixed order, then the triangular product.
+__global__ __launch_bounds__(kT) void qr_w2_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
     if (nbp64 <= 0) return;
     const int nbp = static_cast<int>(nbp64);
     const int64_t j0 = c0 + nbp, nt = d.n - j0;
     const int64_t rows = d.k - c0;
-    const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * 64;
-    const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * 64;
-    if (nt <= 0 || r0 >= d.k || jt0 >= nt) return;
+    const int64_t jt0 = static_cast<int64_t>(blockIdx.x) * 64;
+    if (nt <= 0 || jt0 >= nt) return;
     const int64_t rps = (rows + qr_nsplit(d.k) - 1) / qr_nsplit(d.k);
     const int nsplit = static_cast<int>((rows + rps - 1) / rps);
     __shared__ double sw[kNB][64 + 1];
-    __shared__ double sw2[kNB][64 + 1];
-    __shared__ double sv[64][kNB + 1];
     __shared__ double st[kNB * kNB];
     for (int e = threadIdx.x; e < kNB * kNB; e += kT) st[e] = d.t[e];
     for (int e = threadIdx.x; e < kNB * 64; e += kT) {
@@ -261,18 +259,39 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
             for (int sp = 0; sp < nsplit; ++sp) acc += d.wp[(static_cast<int64_t>(sp) * nt + jt) * kNB + p];
         sw[p][jj] = acc;
     }
+    __syncthreads();
+    // W2(p, j) = sum_{q <= p} T(q, p) W(q, j)
+    for (int e = threadIdx.x; e < kNB * 64; e += kT) {
+        const int p = e % kNB, jj = e / kNB;
+        const int64_t jt = jt0 + jj;
+        if (jt >= nt) continue;
+        double acc = 0.0;
+        for (int q = 0; q <= p; ++q) acc += st[q + p * kNB] * sw[q][jj];
+        d.w2[jt * kNB + p] = p < nbp ? acc : 0.0;
+    }
+}
+
+// A(c0:k, c_end+jt) -= V W2 on 64 x 64 tiles.
+__global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
+    if (nbp64 <= 0) return;
+    const int nbp = static_cast<int>(nbp64);
+    const int64_t j0 = c0 + nbp, nt = d.n - j0;
+    const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * 64;
+    const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * 64;
+    if (nt <= 0 || r0 >= d.k || jt0 >= nt) return;
+    __shared__ double sw2[kNB][64 + 1];
+    __shared__ double sv[64][kNB + 1];
+    for (int e = threadIdx.x; e < kNB * 64; e += kT) {
+        const int p = e % kNB, jj = e / kNB;
+        const int64_t jt = jt0 + jj;
+        sw2[p][jj] = (jt < nt && p < nbp) ? d.w2[jt * kNB + p] : 0.0;
+    }
     for (int e = threadIdx.x; e < 64 * kNB; e += kT) {
         const int rr = e % 64, q = e / 64;
         const int64_t i = r0 + rr;
         sv[rr][q] = (i < d.k && q < nbp) ? d.v[i + q * d.k] : 0.0;
-    }
-    __syncthreads();
-    // W2 = T^T W: W2(p, j) = sum_{q <= p} T(q, p) W(q, j)
-    for (int e = threadIdx.x; e < kNB * 64; e += kT) {
-        const int p = e % kNB, jj = e / kNB;
-        double acc = 0.0;
-        for (int q = 0; q <= p; ++q) acc += st[q + p * kNB] * sw[q][jj];
-        sw2[p][jj] = acc;
     }
     __syncthreads();
     // 256 threads: 16 row groups x 16 column groups, 4 x 4 outputs each
@@ -360,7 +379,7 @@ size_t a256(size_t v) { return (v + 255) / 256 * 256; }
 // per-problem scratch: diag, tau, inv_v0 | V | Gram partials | T |
 // W partials (also the inverse's n x n copy of R) | bad flag
 struct Sizes {
-    size_t vec3, v, gp, t, wp, total;
+    size_t vec3, v, gp, t, wp, w2, total;
 };
 
 Sizes sizes(int64_t k, int64_t n) {
@@ -372,7 +391,8 @@ Sizes sizes(int64_t k, int64_t n) {
     z.gp = a256(static_cast<size_t>(nch * kNB * kNB) * 8);
     z.t = a256(static_cast<size_t>(kNB * kNB) * 8);
     z.wp = a256(static_cast<size_t>(std::max(qr_nsplit(kk) * nn * kNB, nn * nn)) * 8);
-    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + 256;
+    z.w2 = a256(static_cast<size_t>(nn * kNB) * 8);
+    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + 256;
     return z;
 }
 
@@ -398,6 +418,8 @@ QrDesc carve(char*& p, const QrProblem& q) {
     p += z.t;
     d.wp = reinterpret_cast<double*>(p);
     p += z.wp;
+    d.w2 = reinterpret_cast<double*>(p);
+    p += z.w2;
     d.bad = reinterpret_cast<int*>(p);
     p += 256;
     return d;
@@ -452,6 +474,7 @@ void factor(const Batch& B, hipStream_t s) {
         qr_form_t_kernel<<<dim3(1, 1, P), kTT, 0, s>>>(B.dev, c0);
         qr_w_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, kNB)), static_cast<unsigned>(qr_nsplit(B.kmax)), P),
                       kT, 0, s>>>(B.dev, c0);
+        qr_w2_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, 64)), 1, P), kT, 0, s>>>(B.dev, c0);
         qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
                            kT, 0, s>>>(B.dev, c0);
         TOMOS_LAUNCH_CHECK("qr trailing update");
