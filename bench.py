@@ -236,6 +236,10 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
         C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
         out[name] = {"ms": round(f.value / args.gemv_reps, 4), "bytes": nbytes,
                      "launches": args.gemv_reps + 1}
+    if args.tune:
+        from tomos_amd import autotune
+        best, _ = autotune.tune_fused(C.FP32_BF16, m, n, wh.ptr, m, reps=3)
+        out["tuned_shape"] = best
     out["matvec_pair_bf16_hi"] = {"ms": round(time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1), 4),
                                   "bytes": 2 * m * n + 8 * m + 16 * n, "launches": args.gemv_reps + 2}
     print(json.dumps({"kernels_only": out, "m": m, "n": n}), flush=True)

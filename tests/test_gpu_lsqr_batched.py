@@ -90,7 +90,8 @@ def test_batched_graph_equals_plain_and_single(capi):
         assert np.linalg.norm(x - xs) <= 1e-3 * np.linalg.norm(ref) + 1e-12
 
 
-@pytest.mark.parametrize("groups,threads,tiles", [(2, 256, 0), (8, 256, 3), (4, 256, 1), (8, 512, 0), (4, 512, 2)])
+@pytest.mark.parametrize("groups,threads,tiles", [(2, 256, 0), (8, 256, 3), (4, 256, 1), (8, 512, 0), (4, 512, 2),
+                                                  (32, 1024, 0), (32, 1024, 5)])
 def test_batched_launch_shapes(capi, groups, threads, tiles):
     shapes = [(5000, 40, 1), (7001, 100, 2), (3000, 300, 1)]
     batch = make_batch(shapes, 5)

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 HI_DECODER = {C.FP32_BF16: C.DECODE_FP32_BF16_HI, C.FP32_16: C.DECODE_FP32_16_HALF}
 PACKER = {C.FP32_BF16: C.PACK_FP32_BF16, C.FP32_16: C.PACK_FP32_16}
-MAX_N = {(8, 256): 1024, (4, 256): 1024, (2, 256): 2048, (0, 0): 2048, (8, 512): 2048, (4, 512): 2048}
+MAX_N = {(8, 256): 1024, (4, 256): 1024, (2, 256): 2048, (0, 0): 2048, (8, 512): 2048, (4, 512): 2048,
+         (32, 1024): 2048}
 
 
 def run_pair(a_type, A, lda, p, u0, coef):
@@ -43,7 +44,7 @@ def used_matrix(orc, a_type, A):
 @pytest.mark.parametrize("m,lda,n", [(1000, 1000, 37), (4099, 4104, 1000), (777, 784, 64),
                                       (20000, 20000, 1500), (3001, 3008, 2048), (64, 64, 512)])
 @pytest.mark.parametrize("groups,threads,bpc", [(0, 0, 0), (2, 256, 0), (4, 256, 1), (8, 256, 0), (4, 256, 3),
-                                               (8, 512, 0), (4, 512, 2)])
+                                               (8, 512, 0), (4, 512, 2), (32, 1024, 0), (32, 1024, 2)])
 def test_pair_vs_fp64(capi, orc, a_type, m, lda, n, groups, threads, bpc):
     if n > MAX_N[(groups, threads)]:
         pytest.skip("no kernel for %d row groups x %d threads at n = %d" % (groups, threads, n))

@@ -7,7 +7,9 @@ registers (occupancy) against tile height and load width:
 * ``row_groups`` G in {2, 4, 8}: lanes that share one column of a tile; a
   tile is G x 16 bytes tall per column, every lane holds ceil(n G / T)
   16-byte vectors;
-* ``threads`` T in {256, 512} per block;
+* ``threads`` T in {256, 512} per block; (G, T) = (32, 1024) is the variant
+  with 512-byte column runs that reads each tile twice (the second read
+  from cache);
 * ``blocks_per_cu``: grid size (0 = as many blocks as are resident at once).
 
 ``tune_fused`` times every candidate with HIP events on the real matrix,
@@ -27,7 +29,8 @@ from . import _capi as C
 from ._capi import DeviceBuffer
 
 # largest n each (row groups, threads) shape has a kernel for
-MAX_N = {(2, 256): 2048, (4, 256): 1024, (8, 256): 1024, (4, 512): 2048, (8, 512): 2048}
+MAX_N = {(2, 256): 2048, (4, 256): 1024, (8, 256): 1024, (4, 512): 2048, (8, 512): 2048,
+         (32, 1024): 2048}
 BLOCKS_PER_CU = (0, 1, 2, 3, 4)
 
 

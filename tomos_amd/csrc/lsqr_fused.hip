@@ -253,6 +253,171 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
     }
 }
 
+// Variant with long column runs.  A tile is RL * RPL rows (32 row lanes x
+// 8 packed rows = 256 rows: 512 contiguous bytes per column) by all n
+// columns; 32 column slots of 32 row lanes each (1024 threads); slot s
+// owns columns s, s + 32, s + 64, ...  The tile is too large for
+// registers, so phase 2 reads it a second time right after phase 1 (the
+// re-read is meant to be served by L2 / the MALL, not HBM):
+//   phase 1: each lane sums its 8 rows over its slot's columns, CPB column
+//            loads in flight per batch; the 32 slots combine in LDS -> u_raw;
+//   phase 2: per batch of CPB columns each lane forms a . u over its 8
+//            rows, a transposing reduction over the 32 row lanes leaves one
+//            column per lane, added into an LDS column accumulator that
+//            the column's lane owns (no atomics, fixed order).
+// The loops over column batches are not unrolled: registers stay bounded
+// whatever n (up to 2048).
+template <int FMT>
+__device__ __forceinline__ void fused2_body(int64_t m, int64_t n, const void* __restrict__ a,
+                                            int64_t lda, const void* __restrict__ p, bool p64,
+                                            void* __restrict__ u, bool u64, double cu, int64_t tile0,
+                                            int64_t tile1, double* __restrict__ sumsq_out,
+                                            float* __restrict__ tpart_out) {
+    constexpr int NT = 1024, RL = 32, NS = NT / RL, CPB = 8;
+    constexpr int RPL = rows_per_lane<FMT>();
+    constexpr int TR = RL * RPL;
+    constexpr int ESZ = FMT == FMT_F32 ? 4 : 2;
+    constexpr int kMaxN = 2048;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int rl = tid & (RL - 1);  // row lane
+    const int slot = tid / RL;      // column slot
+    const char* abase = static_cast<const char*>(a);
+    const int64_t step = static_cast<int64_t>(NS) * lda * ESZ;  // bytes between a slot's columns
+    const int nbatch = static_cast<int>((n + NS * CPB - 1) / (NS * CPB));
+    __shared__ float pvs[kMaxN];
+    __shared__ float colacc[kMaxN];
+    __shared__ float red[NS][TR];
+    __shared__ float ush[TR];
+    for (int64_t j = tid; j < n; j += NT) {
+        pvs[j] = p64 ? static_cast<float>(static_cast<const double*>(p)[j]) : static_cast<const float*>(p)[j];
+        colacc[j] = 0.0f;
+    }
+    double sq = 0.0;
+    // after the row-lane reduction, lane bits 4..2 select the batch column
+    // (halving stages on bits 4, 3, 2); bits 1..0 hold duplicates
+    const int sub = (((rl >> 4) & 1) << 2) | (((rl >> 3) & 1) << 1) | ((rl >> 2) & 1);
+    const bool owner = (rl & 3) == 0;
+    __syncthreads();
+    for (int64_t tile = tile0; tile < tile1; ++tile) {
+        const int64_t row0 = tile * TR;
+        const int64_t my_row = row0 + rl * RPL;
+        const bool in = my_row < m;
+        const int nvalid = in ? static_cast<int>(min<int64_t>(RPL, m - my_row)) : 0;
+        const char* colp = abase + (static_cast<int64_t>(slot) * lda + my_row) * ESZ;
+        float acc[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) acc[k] = 0.0f;
+        for (int bt = 0; bt < nbatch; ++bt) {
+            uint4 t[CPB];
+#pragma unroll
+            for (int q = 0; q < CPB; ++q) {
+                const int r = bt * CPB + q;
+                const int64_t j = slot + static_cast<int64_t>(NS) * r;
+                t[q] = (in && j < n) ? *reinterpret_cast<const uint4*>(colp + r * step)
+                                     : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int q = 0; q < CPB; ++q) {
+                const int64_t j = slot + static_cast<int64_t>(NS) * (bt * CPB + q);
+                const float pj = j < n ? pvs[j] : 0.0f;
+                float av[RPL];
+                decode_vec<FMT>(t[q], av);
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) acc[k] = fmaf(av[k], pj, acc[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) red[slot][rl * RPL + k] = acc[k];
+        __syncthreads();
+        if (tid < TR) {
+            const int64_t row = row0 + tid;
+            float un = 0.0f;
+            if (row < m) {
+                float sacc = red[0][tid];
+                for (int w = 1; w < NS; ++w) sacc += red[w][tid];
+                double r = static_cast<double>(sacc);
+                if (cu != 0.0)  // u is not read for a zero coefficient
+                    r += cu * (u64 ? static_cast<double*>(u)[row]
+                                   : static_cast<double>(static_cast<float*>(u)[row]));
+                if (u64) {
+                    static_cast<double*>(u)[row] = r;
+                    un = static_cast<float>(r);
+                    sq += r * r;
+                } else {
+                    un = static_cast<float>(r);
+                    static_cast<float*>(u)[row] = un;
+                    sq += static_cast<double>(un) * un;
+                }
+            }
+            ush[tid] = un;
+        }
+        __syncthreads();
+        float ur[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[rl * RPL + k] : 0.0f;
+        for (int bt = 0; bt < nbatch; ++bt) {
+            uint4 t[CPB];
+#pragma unroll
+            for (int q = 0; q < CPB; ++q) {
+                const int r = bt * CPB + q;
+                const int64_t j = slot + static_cast<int64_t>(NS) * r;
+                t[q] = (in && j < n) ? *reinterpret_cast<const uint4*>(colp + r * step)
+                                     : make_uint4(0u, 0u, 0u, 0u);
+            }
+            float cs[CPB];
+#pragma unroll
+            for (int q = 0; q < CPB; ++q) {
+                float av[RPL];
+                decode_vec<FMT>(t[q], av);
+                float sacc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) sacc = fmaf(av[k], ur[k], sacc);
+                cs[q] = sacc;
+            }
+            LaneTR<CPB, 16, 1>::run(cs, lane);
+            const int64_t j = slot + static_cast<int64_t>(NS) * (bt * CPB + sub);
+            if (owner && j < n) colacc[j] += cs[0];
+        }
+        // red / ush are rewritten only after the next tile's first barrier
+    }
+    __syncthreads();
+    for (int64_t j = tid; j < n; j += NT) tpart_out[j] = colacc[j];
+    for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
+    __shared__ double sqw[NT / 64];
+    if (lane == 0) sqw[wave] = sq;
+    __syncthreads();
+    if (tid == 0) {
+        double tt = sqw[0];
+        for (int w = 1; w < NT / 64; ++w) tt += sqw[w];
+        *sumsq_out = tt;
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(1024) void lsqr_fused2_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
+    bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
+    double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int TR = 32 * rows_per_lane<FMT>();
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
+    const int64_t ntiles = (m + TR - 1) / TR;
+    const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
+    fused2_body<FMT>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+                        tpart + static_cast<int64_t>(blockIdx.x) * n);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(1024) void grouped_fused2_kernel(const GroupProb* __restrict__ probs,
+                                                              const GroupItem* __restrict__ items) {
+    const GroupItem it = items[blockIdx.x];
+    const GroupProb& P = probs[it.prob];
+    if (P.skip && *P.skip) return;
+    fused2_body<FMT>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0, it.tile1,
+                        P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
+}
+
 template <int FMT, int G, int ROUNDS, int NT>
 __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
@@ -307,6 +472,7 @@ constexpr Shape kShapes[] = {
     {8, 256, 8}, {8, 256, 16}, {8, 256, 24}, {8, 256, 32},
     {4, 512, 8}, {4, 512, 16},
     {8, 512, 8}, {8, 512, 16}, {8, 512, 24}, {8, 512, 32},
+    {32, 1024, 64},  // long column runs, tile read twice (up to 64 x 32 = 2048 columns)
 };
 
 // Smallest instantiated R covering n for (g, nt); 0 when none.
@@ -329,6 +495,7 @@ int rounds_for(int g, int nt, int64_t n) {
 // The kernel instance for (format, shape), or nullptr.
 template <int FMT>
 const void* fused_kernel_ptr(int g, int nt, int rounds) {
+    if (g == 32 && nt == 1024) return rounds > 0 ? reinterpret_cast<const void*>(lsqr_fused2_kernel<FMT>) : nullptr;
 #define FUSED_CASE(G, NT, R) \
     if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT>);
     TOMOS_FUSED_SHAPES(FUSED_CASE)
@@ -338,6 +505,7 @@ const void* fused_kernel_ptr(int g, int nt, int rounds) {
 
 template <int FMT>
 const void* grouped_kernel_ptr(int g, int nt, int rounds) {
+    if (g == 32 && nt == 1024) return rounds > 0 ? reinterpret_cast<const void*>(grouped_fused2_kernel<FMT>) : nullptr;
 #define FUSED_CASE(G, NT, R) \
     if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R, NT>);
     TOMOS_FUSED_SHAPES(FUSED_CASE)
@@ -553,9 +721,10 @@ int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per
         require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16,
                 "tomos_fused_config_set: bad a_type");
         require(n >= 1 && n <= kFusedMaxN, "tomos_fused_config_set: bad n");
-        require(row_groups == 0 || row_groups == 2 || row_groups == 4 || row_groups == 8,
-                "tomos_fused_config_set: row_groups must be 0, 2, 4 or 8");
-        require(threads == 0 || threads == 256 || threads == 512, "tomos_fused_config_set: threads must be 0, 256 or 512");
+        require(row_groups == 0 || row_groups == 2 || row_groups == 4 || row_groups == 8 || row_groups == 32,
+                "tomos_fused_config_set: row_groups must be 0, 2, 4, 8 or 32");
+        require(threads == 0 || threads == 256 || threads == 512 || threads == 1024,
+                "tomos_fused_config_set: threads must be 0, 256, 512 or 1024");
         require(row_groups == 0 || rounds_for(row_groups, threads ? threads : 256, n) > 0,
                 "tomos_fused_config_set: no kernel for this shape at this n");
         require(blocks_per_cu >= 0 && blocks_per_cu <= 8, "tomos_fused_config_set: blocks_per_cu in 0..8");
