@@ -10,7 +10,7 @@ def test_candidates_cover_kernel_limits():
         assert c, n
         for g, t, b in c:
             assert n <= autotune.MAX_N[(g, t)]
-    assert all(g != 8 or t != 256 for g, t, _ in autotune.candidates(1500))  # G=8 x 256 stops at 1024
+    assert all(g != 8 for g, t, _ in autotune.candidates(1500))  # G=8 shapes stop at 1024
     assert autotune.candidates(2049) == []
 
 

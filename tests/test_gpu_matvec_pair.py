@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 HI_DECODER = {C.FP32_BF16: C.DECODE_FP32_BF16_HI, C.FP32_16: C.DECODE_FP32_16_HALF}
 PACKER = {C.FP32_BF16: C.PACK_FP32_BF16, C.FP32_16: C.PACK_FP32_16}
-MAX_N = {(8, 256): 1024, (4, 256): 1024, (2, 256): 2048, (0, 0): 2048, (8, 512): 2048, (4, 512): 2048,
+MAX_N = {(8, 256): 1024, (4, 256): 1024, (2, 256): 2048, (0, 0): 2048, (8, 512): 1024, (4, 512): 2048,
          (32, 1024): 2048}
 
 

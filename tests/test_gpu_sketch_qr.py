@@ -131,3 +131,37 @@ def test_batched_precond_matches_single(capi):
         Rn = np.linalg.qr(A, mode="r")
         prod = np.abs(Rn) @ np.abs(got)
         assert np.abs(np.abs(Rn @ got) - np.eye(n)).max() < 1e-9 * max(1.0, prod.max())
+
+
+def test_apply_deterministic_and_zero_column(capi, orc):
+    """The fixed-point accumulation is exact: repeated calls agree bit for bit."""
+    m, n, s = 100000, 20, 300
+    rng = np.random.default_rng(21)
+    A = (rng.standard_normal((m, n)) * np.logspace(0, -8, n)).astype(np.float32, order="F")
+    A[:, 7] = 0.0
+    wh, wr = H.pack(C.PACK_FP32_BF16, A)
+    Y1 = H.countsketch_apply(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, m, s)
+    Y2 = H.countsketch_apply(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, m, s)
+    assert np.array_equal(Y1, Y2)
+    assert not Y1[:, 7].any()
+    ref = orc.countsketch_apply(A.astype(np.float64), s)
+    # each column to its own scale (d spans 1..1e-8)
+    for j in range(n):
+        if j == 7:
+            continue
+        assert np.abs(Y1[:, j] - ref[:, j]).max() <= 1e-9 * np.abs(ref[:, j]).max()
+
+
+def test_apply_saturated_fp32_16_falls_back(capi, orc):
+    """A saturated FP32_16 word decodes to Inf: the float path handles that column set."""
+    m, n, s = 20000, 5, 100
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    A[123, 2] = 3.0e38  # beyond FP32_16's range: stored as +Inf
+    wh, wr = H.pack(C.PACK_FP32_16, A)
+    Y = H.countsketch_apply(C.FP32_16, C.TIER_EXACT, m, n, wh, wr, m, s)
+    eh, er = orc.pack(C.PACK_FP32_16, A)
+    ref = orc.countsketch_apply(orc.unpack(C.DECODE_FP32_16_EXACT, eh, er).astype(np.float64), s)
+    assert np.array_equal(np.isinf(Y), np.isinf(ref)) and np.isinf(Y).sum() == 1
+    fin = np.isfinite(ref)
+    assert np.abs(Y[fin] - ref[fin]).max() <= 2e-6 * np.abs(ref[fin]).max()

@@ -29,7 +29,7 @@ from . import _capi as C
 from ._capi import DeviceBuffer
 
 # largest n each (row groups, threads) shape has a kernel for
-MAX_N = {(2, 256): 2048, (4, 256): 1024, (8, 256): 1024, (4, 512): 2048, (8, 512): 2048,
+MAX_N = {(2, 256): 2048, (4, 256): 1024, (8, 256): 1024, (4, 512): 2048, (8, 512): 1024,
          (32, 1024): 2048}
 BLOCKS_PER_CU = (0, 1, 2, 3, 4)
 

@@ -242,6 +242,173 @@ __global__ __launch_bounds__(256) void cs_finish_kernel(int64_t s, int64_t n, in
     y[b + c * ldy] = acc;
 }
 
+// ---- fixed-point path ------------------------------------------------------
+// LDS float atomics are the bottleneck of the scatter (about 7x slower than
+// 64-bit integer LDS atomics on gfx950), so the vector path accumulates in
+// fixed point instead: a first pass finds each column's largest hi-plane
+// magnitude, giving an exponent e_j with |A[i,j]| < 2^e_j for every row
+// (one bit of margin covers the exact value vs its hi word); each value is
+// then added as the 64-bit integer round(v * 2^(shift - e_j)), with
+// shift = 62 - H and 2^H > rows per split, so a bucket's sum cannot
+// overflow.  Integer sums are exact and order-independent, so the sketch is
+// deterministic; the resolution is 2^-shift of the column's magnitude
+// (2^-42 at 2e6 rows and 4 splits).  Columns whose max is not finite (Inf
+// or NaN from saturated FP32_16 words) take the float path for the call.
+
+// colmax_bits[j] = max over rows of |value| as float bits (monotone for
+// non-negative floats) of the values the sketch sums, or (BOUND_HI) of the
+// hi words, which bound the exact FP32_BF16 values within a factor 2.
+// grid (n, splits); 16-byte row vectors.
+template <int FMT, bool EXACT>
+__global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* __restrict__ a,
+                                                        const uint16_t* __restrict__ wr, int64_t lda,
+                                                        int64_t rows_per_split,
+                                                        unsigned* __restrict__ colmax_bits) {
+    constexpr int V = vec_rows<FMT>();
+    constexpr int EB = 16 / V;
+    const int64_t j = blockIdx.x;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rows_per_split;
+    const int64_t r1 = min(m, r0 + rows_per_split);
+    const char* col = static_cast<const char*>(a) + j * lda * EB;
+    float mx = 0.0f;
+    for (int64_t i0 = r0 + static_cast<int64_t>(threadIdx.x) * V; i0 < r1; i0 += 256 * V) {
+        float v[V];
+        if (i0 + V <= r1) {
+            const uint4 h = *reinterpret_cast<const uint4*>(col + i0 * EB);
+            uint4 l = make_uint4(0, 0, 0, 0);
+            if constexpr (EXACT) l = *reinterpret_cast<const uint4*>(wr + i0 + j * lda);
+            decode_rows<FMT, EXACT>(h, l, v);
+        } else {
+            for (int r = 0; r < V; ++r) v[r] = (i0 + r < r1) ? load_elem<FMT, EXACT>(a, wr, i0 + r + j * lda) : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+            const float av = fabsf(v[r]);
+            mx = (av > mx || av != av) ? av : mx;  // NaN wins
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float other = __shfl_xor(mx, o);
+        mx = (other > mx || other != other) ? other : mx;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(colmax_bits + j, __float_as_uint(mx));
+}
+
+// e_j from the max bits; flag = 1 if any column is not finite
+__global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigned* __restrict__ colmax_bits,
+                                                        int* __restrict__ colexp, int* __restrict__ flag) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (j >= n) return;
+    const unsigned bits = colmax_bits[j];
+    if (bits >= 0x7F800000u) {
+        atomicOr(flag, 1);
+        colexp[j] = 0;
+        return;
+    }
+    int e = -149;
+    if (bits != 0) {
+        frexpf(__uint_as_float(bits), &e);
+        e += 1;  // margin: an exact FP32_BF16 value is below twice its hi word
+    }
+    colexp[j] = e;
+}
+
+template <int FMT, bool EXACT, int CG>
+__global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
+    int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
+    const int* __restrict__ colexp, int shift, double* __restrict__ partial) {
+    constexpr int V = vec_rows<FMT>();
+    constexpr int EB = 16 / V;
+    extern __shared__ unsigned long long yfx[];  // [CG][s]
+    const int split = blockIdx.x / ncg;
+    const int cg = blockIdx.x - split * ncg;
+    const int64_t c0 = static_cast<int64_t>(cg) * CG;
+    const int ncols = static_cast<int>(min<int64_t>(CG, n - c0));
+    for (int64_t t = threadIdx.x; t < s * CG; t += kVecThreads) yfx[t] = 0ull;
+    int sc[CG];
+#pragma unroll
+    for (int c = 0; c < CG; ++c) sc[c] = c < ncols ? shift - colexp[c0 + c] : 0;
+    __syncthreads();
+
+    const int64_t r0 = static_cast<int64_t>(split) * rows_per_split;
+    const int64_t r1 = min(m, r0 + rows_per_split);
+    const char* abase = static_cast<const char*>(a);
+    for (int64_t i0 = r0 + static_cast<int64_t>(threadIdx.x) * V; i0 < r1;
+         i0 += static_cast<int64_t>(kVecThreads) * V) {
+        if (i0 + V <= r1) {
+            uint32_t w[V];
+            if constexpr (V == 8) {
+                const uint4 p0 = *reinterpret_cast<const uint4*>(packed + i0);
+                const uint4 p1 = *reinterpret_cast<const uint4*>(packed + i0 + 4);
+                w[0] = p0.x; w[1] = p0.y; w[2] = p0.z; w[3] = p0.w;
+                w[4] = p1.x; w[5] = p1.y; w[6] = p1.z; w[7] = p1.w;
+            } else if constexpr (V == 4) {
+                const uint4 p0 = *reinterpret_cast<const uint4*>(packed + i0);
+                w[0] = p0.x; w[1] = p0.y; w[2] = p0.z; w[3] = p0.w;
+            } else {
+                const uint2 p0 = *reinterpret_cast<const uint2*>(packed + i0);
+                w[0] = p0.x; w[1] = p0.y;
+            }
+            uint4 h[CG], l[CG];
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                if (c < ncols) {
+                    const int64_t e = i0 + (c0 + c) * lda;
+                    h[c] = *reinterpret_cast<const uint4*>(abase + e * EB);
+                    if constexpr (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF))
+                        l[c] = *reinterpret_cast<const uint4*>(wr + e);
+                    else
+                        l[c] = make_uint4(0, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CG; ++c) {
+                if (c < ncols) {
+                    float v[V];
+                    decode_rows<FMT, EXACT>(h[c], l[c], v);
+#pragma unroll
+                    for (int r = 0; r < V; ++r) {
+                        const int b = static_cast<int>(w[r] & 0x7FFFFFFFu);
+                        const long long q = __float2ll_rn(ldexpf(v[r], sc[c]));
+                        atomicAdd(&yfx[c * s + b],
+                                  static_cast<unsigned long long>((w[r] & 0x80000000u) ? -q : q));
+                    }
+                }
+            }
+        } else {
+            for (int64_t i = i0; i < r1; ++i) {
+                const uint32_t w = packed[i];
+                const int b = static_cast<int>(w & 0x7FFFFFFFu);
+                for (int c = 0; c < ncols; ++c) {
+                    const float v = load_elem<FMT, EXACT>(a, wr, i + (c0 + c) * lda);
+                    const long long q = __float2ll_rn(ldexpf(v, sc[c]));
+                    atomicAdd(&yfx[c * s + b], static_cast<unsigned long long>((w & 0x80000000u) ? -q : q));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    double* out = partial + static_cast<int64_t>(split) * s * n;
+    for (int64_t t = threadIdx.x; t < s * ncols; t += kVecThreads) {
+        const int64_t c = t / s;
+        const long long q = static_cast<long long>(yfx[t]);
+        out[(c0 + c) * s + (t - c * s)] = ldexp(static_cast<double>(q), colexp[c0 + c] - shift);
+    }
+}
+
+__global__ __launch_bounds__(256) void cs_finish_fx_kernel(int64_t s, int64_t n, int splits,
+                                                           const double* __restrict__ partial,
+                                                           double* __restrict__ y, int64_t ldy) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= s * n) return;
+    const int64_t c = t / s;
+    const int64_t b = t - c * s;
+    double acc = 0.0;
+    for (int r = 0; r < splits; ++r) acc += partial[static_cast<int64_t>(r) * s * n + t];
+    y[b + c * ldy] = acc;
+}
+
 constexpr int64_t kLdsBytes = 128 * 1024;
 
 int cg_for(int64_t s) {
@@ -269,11 +436,48 @@ CsPlan cs_plan(int64_t m, int64_t n, int64_t s) {
     return p;
 }
 
-size_t cs_scratch(int64_t m, int64_t n, int64_t s) {
-    const CsPlan p = cs_plan(m, n, s);
-    const size_t idx = ((static_cast<size_t>(m) * 4 + 255) / 256) * 256;
-    return idx + static_cast<size_t>(p.splits) * s * n * 4;
+// Fixed-point path: int64 LDS slices (CG columns x s buckets x 8 bytes).
+CsPlan cs_plan_fx(int64_t m, int64_t n, int64_t s) {
+    CsPlan p;
+    p.cg = 8;
+    while (p.cg > 1 && s * p.cg * 8 > kLdsBytes) p.cg >>= 1;
+    p.ncg = static_cast<int>(ceil_div(std::max<int64_t>(n, 1), p.cg));
+    const int64_t want = std::max<int64_t>(1, ceil_div(4 * 256, p.ncg));
+    const int64_t max_splits = std::max<int64_t>(1, ceil_div(m, 8192));
+    p.splits = static_cast<int>(std::min(want, max_splits));
+    p.rows_per_split = ceil_div(ceil_div(std::max<int64_t>(m, 1), p.splits), 8) * 8;
+    p.splits = static_cast<int>(ceil_div(std::max<int64_t>(m, 1), p.rows_per_split));
+    return p;
 }
+
+// 62 - H with 2^H > rows per split (plus one bit): a bucket sum of values
+// below 2^shift each stays below 2^62
+int fx_shift(int64_t rows_per_split) {
+    int h = 1;
+    while ((int64_t(1) << h) <= rows_per_split) ++h;
+    return 62 - (h + 1);
+}
+
+size_t a256(size_t v) { return (v + 255) / 256 * 256; }
+
+struct CsLayout {
+    size_t idx, colmax, colexp, flag, partial, total;
+};
+
+CsLayout cs_layout(int64_t m, int64_t n, int64_t s) {
+    const CsPlan pf = cs_plan(m, n, s), px = cs_plan_fx(m, n, s);
+    CsLayout L;
+    L.idx = 0;
+    L.colmax = a256(static_cast<size_t>(std::max<int64_t>(m, 1)) * 4);
+    L.colexp = L.colmax + a256(static_cast<size_t>(std::max<int64_t>(n, 1)) * 4);
+    L.flag = L.colexp + a256(static_cast<size_t>(std::max<int64_t>(n, 1)) * 4);
+    L.partial = L.flag + 256;
+    const size_t pb = std::max(static_cast<size_t>(pf.splits) * s * n * 4, static_cast<size_t>(px.splits) * s * n * 8);
+    L.total = L.partial + pb;
+    return L;
+}
+
+size_t cs_scratch(int64_t m, int64_t n, int64_t s) { return cs_layout(m, n, s).total; }
 
 template <int FMT>
 bool vec_ok(const void* a, const uint16_t* wr, int64_t lda, bool exact) {
@@ -321,6 +525,58 @@ void launch_apply_cg(const CsPlan& p, int64_t m, int64_t n, const void* a, const
     }
 }
 
+// Fixed-point apply on the vector path; returns false (nothing launched
+// after the exponent pass) when a column's max is not finite.
+template <int FMT, bool EXACT>
+bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
+                  const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy,
+                  hipStream_t st) {
+    if (!vec_ok<FMT>(a, wr, lda, EXACT)) return false;
+    const CsPlan p = cs_plan_fx(m, n, s);
+    unsigned* colmax = reinterpret_cast<unsigned*>(scratch + L.colmax);
+    int* colexp = reinterpret_cast<int*>(scratch + L.colexp);
+    int* flag = reinterpret_cast<int*>(scratch + L.flag);
+    double* partial = reinterpret_cast<double*>(scratch + L.partial);
+    TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
+    TOMOS_HIP_CHECK(hipMemsetAsync(flag, 0, 4, st));
+    const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
+    const int64_t crows = ceil_div(ceil_div(m, csplits), 8) * 8;
+    require(n < (int64_t(1) << 31), "countsketch_apply: n too large");
+    // FP32_BF16's exact values are bounded by 2x their hi words: the hi plane
+    // suffices; FP32_16's exact values need their own decode
+    constexpr bool kExactMax = EXACT && FMT == FMT_P16;
+    cs_colmax_kernel<FMT, kExactMax><<<dim3(static_cast<unsigned>(n), static_cast<unsigned>(ceil_div(m, crows))),
+                                       256, 0, st>>>(m, a, wr, lda, crows, colmax);
+    cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(n, colmax, colexp, flag);
+    TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
+    int hflag = 0;
+    TOMOS_HIP_CHECK(hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, st));
+    TOMOS_HIP_CHECK(hipStreamSynchronize(st));
+    if (hflag) return false;
+    const int shift = fx_shift(p.rows_per_split);
+    const size_t lds = static_cast<size_t>(s) * p.cg * 8;
+    const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
+    require(blocks < (int64_t(1) << 31), "countsketch_apply: grid too large");
+    auto launch = [&](auto* k) {
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        k<<<dim3(static_cast<unsigned>(blocks)), kVecThreads, lds, st>>>(m, n, a, wr, lda, s, packed,
+                                                                        p.rows_per_split, p.ncg, colexp, shift,
+                                                                        partial);
+    };
+    switch (p.cg) {
+        case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8>); break;
+        case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4>); break;
+        case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2>); break;
+        default: launch(cs_apply_fx_kernel<FMT, EXACT, 1>); break;
+    }
+    TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
+    cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits, partial,
+                                                                                          y, ldy);
+    TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
+    return true;
+}
+
 }  // namespace
 
 size_t countsketch_scratch_bytes(int64_t m, int64_t n, int64_t s) { return cs_scratch(m, n, s); }
@@ -340,15 +596,33 @@ void countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a
     if (scratch == nullptr) scratch = fallback_scratch(need);
     else require(scratch_bytes >= need, "countsketch_apply: scratch too small");
     const CsPlan p = cs_plan(m, n, s);
-    uint32_t* packed = static_cast<uint32_t*>(scratch);
-    float* partial = reinterpret_cast<float*>(static_cast<char*>(scratch) +
-                                              ((static_cast<size_t>(m) * 4 + 255) / 256) * 256);
+    const CsLayout L = cs_layout(m, n, s);
+    char* sc = static_cast<char*>(scratch);
+    uint32_t* packed = reinterpret_cast<uint32_t*>(sc + L.idx);
+    float* partial = reinterpret_cast<float*>(sc + L.partial);
     if (m > 0) {
         require(a != nullptr, "countsketch_apply: null A");
         cs_index_kernel<<<dim3(static_cast<unsigned>(ceil_div(m, 256))), 256, 0, st>>>(
             m, s, nullptr, nullptr, packed);
         TOMOS_LAUNCH_CHECK("cs_index_kernel");
         const bool exact = tier == TOMOS_TIER_EXACT;
+        if (exact && (a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16))
+            require(wr != nullptr, "countsketch_apply: EXACT needs wr");
+        bool done = false;
+        switch (a_type) {
+            case TOMOS_F32: done = try_apply_fx<FMT_F32, false>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st); break;
+            case TOMOS_F64: done = try_apply_fx<FMT_F64, false>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st); break;
+            case TOMOS_FP32_16:
+                done = exact ? try_apply_fx<FMT_P16, true>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st)
+                             : try_apply_fx<FMT_P16, false>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st);
+                break;
+            case TOMOS_FP32_BF16:
+                done = exact ? try_apply_fx<FMT_PBF, true>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st)
+                             : try_apply_fx<FMT_PBF, false>(m, n, a, wr, lda, s, packed, sc, L, y, ldy, st);
+                break;
+            default: break;
+        }
+        if (done) return;
         switch (a_type) {
             case TOMOS_F32: launch_apply_cg<FMT_F32, false>(p, m, n, a, wr, lda, s, packed, partial, st); break;
             case TOMOS_F64: launch_apply_cg<FMT_F64, false>(p, m, n, a, wr, lda, s, packed, partial, st); break;

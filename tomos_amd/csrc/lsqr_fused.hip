@@ -204,13 +204,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
             ush[tid] = un;
         }
         __syncthreads();
-        // phase 2: column sums over this lane's rows with u_raw.  The empty
-        // asm marks the packed tile as redefined, so the compiler decodes it
-        // again instead of keeping phase 1's decoded floats (twice the VGPRs)
-        // live across the barriers.
-#pragma unroll
-        for (int r = 0; r < ROUNDS; ++r)
-            asm volatile("" : "+v"(t[r].x), "+v"(t[r].y), "+v"(t[r].z), "+v"(t[r].w));
+        // phase 2: column sums over this lane's rows with u_raw
         float ur[RPL];
 #pragma unroll
         for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
@@ -471,7 +465,7 @@ constexpr Shape kShapes[] = {
     {4, 256, 4}, {4, 256, 8}, {4, 256, 12}, {4, 256, 16},
     {8, 256, 8}, {8, 256, 16}, {8, 256, 24}, {8, 256, 32},
     {4, 512, 8}, {4, 512, 16},
-    {8, 512, 8}, {8, 512, 16}, {8, 512, 24}, {8, 512, 32},
+    {8, 512, 8}, {8, 512, 16},
     {32, 1024, 64},  // long column runs, tile read twice (up to 64 x 32 = 2048 columns)
 };
 
@@ -490,7 +484,7 @@ int rounds_for(int g, int nt, int64_t n) {
     X(4, 256, 4) X(4, 256, 8) X(4, 256, 12) X(4, 256, 16)                                               \
     X(8, 256, 8) X(8, 256, 16) X(8, 256, 24) X(8, 256, 32)                                              \
     X(4, 512, 8) X(4, 512, 16)                                                                          \
-    X(8, 512, 8) X(8, 512, 16) X(8, 512, 24) X(8, 512, 32)
+    X(8, 512, 8) X(8, 512, 16)
 
 // The kernel instance for (format, shape), or nullptr.
 template <int FMT>
@@ -528,9 +522,10 @@ const void* fused_kernel_for(int a_type, int g, int nt, int rounds) {
 }  // namespace
 
 FusedShape lsqr_fused_default_shape(int64_t n) {
-    // 128 contiguous bytes per column per tile (G = 8) where that kernel
-    // fits its registers; 64 bytes (G = 4) up to n = 2048
-    if (n <= 1536) return FusedShape{8, 512};
+    // 128 contiguous bytes per column per tile (G = 8): the fetch granule,
+    // so no over-fetch; up to n = 1024 the whole tile fits one 256-thread
+    // block's registers.  Beyond, 64-byte runs (G = 4, twice the fetch).
+    if (n <= 1024) return FusedShape{8, 256};
     return FusedShape{4, 512};
 }
 
