@@ -238,7 +238,7 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
                      "launches": args.gemv_reps + 1}
     if args.tune:
         from tomos_amd import autotune
-        best, _ = autotune.tune_fused(C.FP32_BF16, m, n, wh.ptr, m, reps=3)
+        best, _ = autotune.tune_fused(C.FP32_BF16, m, n, wh.ptr, m, reps=10)
         out["tuned_shape"] = best
     out["matvec_pair_bf16_hi"] = {"ms": round(time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1), 4),
                                   "bytes": 2 * m * n + 8 * m + 16 * n, "launches": args.gemv_reps + 2}
@@ -288,7 +288,7 @@ def main():
         # for both storage types at this shape (set-up, outside the timed region)
         from tomos_amd import autotune
         for name, at, ptr in (("fp32_bf16", C.FP32_BF16, wh.ptr), ("f32", C.F32, A.ptr)):
-            best, _ = autotune.tune_fused(at, m, n, ptr, m, reps=3)
+            best, _ = autotune.tune_fused(at, m, n, ptr, m, reps=10)
             tuned[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in best.items()}
         log("tuned single-pass shapes: %s" % tuned)
 

@@ -212,9 +212,12 @@ int tomos_matvec_pair(int a_type, int64_t m, int64_t n, const void* a, int64_t l
  * 512; 0 = 256 with an explicit row_groups) and blocks_per_cu (grid size;
  * 0 = as many as are resident at once).  Process-wide; consulted at
  * every launch.  Scratch sizes do not depend on it.  Set by the autotuner
- * (tomos_amd/autotune.py); no reference counterpart. */
+ * (tomos_amd/autotune.py); no reference counterpart.  Setting all three to
+ * 0 removes the (a_type, n) entry; clear removes every entry. */
 int tomos_fused_config_set(int a_type, int64_t n, int row_groups, int blocks_per_cu, int threads);
 int tomos_fused_config_clear(void);
+/* Installed shape for (a_type, n): all three outputs 0 when none is set. */
+int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_per_cu, int* threads);
 
 /* ---- CountSketch ------------------------------------------------------ */
 /* bucket[i] = splitmix64(i ^ 0x12345678abcdef00) mod s,

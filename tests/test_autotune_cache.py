@@ -28,3 +28,20 @@ def test_cache_round_trip(tmp_path, monkeypatch):
     assert c2.get(1, 1000) is None
     monkeypatch.setattr(autotune, "device_key", lambda: "gfx942-304cu")
     assert c2.get(5, 1000) is None  # entries are per device
+
+
+def test_installed_entries_are_per_format(capi):
+    """Setting or erasing one (a_type, n) entry leaves the others alone."""
+    from tomos_amd import _capi as C
+    C.call("tomos_fused_config_clear")
+    try:
+        C.call("tomos_fused_config_set", C.FP32_BF16, 1000, 8, 0, 512)
+        C.call("tomos_fused_config_set", C.F32, 1000, 4, 2, 256)
+        assert autotune.installed(C.FP32_BF16, 1000) == {"row_groups": 8, "blocks_per_cu": 0, "threads": 512}
+        C.call("tomos_fused_config_set", C.F32, 1000, 0, 0, 0)  # erase one entry
+        assert autotune.installed(C.F32, 1000) is None
+        assert autotune.installed(C.FP32_BF16, 1000)["threads"] == 512
+        assert autotune.installed(C.FP32_BF16, 999) is None
+    finally:
+        C.call("tomos_fused_config_clear")
+    assert autotune.installed(C.FP32_BF16, 1000) is None

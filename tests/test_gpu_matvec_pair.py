@@ -89,3 +89,25 @@ def test_pair_deterministic_and_errors(capi):
     assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 3, 0, 256) == 1
     assert C.lib().tomos_fused_config_set(C.FP32_BF16, 100, 4, 0, 384) == 1
     assert C.lib().tomos_fused_config_set(C.F64, 100, 4, 0, 256) == 1
+
+
+def test_tuning_one_format_keeps_the_other(capi):
+    """tune_fused installs its best shape and leaves other formats' entries."""
+    from tomos_amd import autotune
+    rng = np.random.default_rng(3)
+    m, n = 20000, 100
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    wh, _ = H.pack(C.PACK_FP32_BF16, A)
+    af = H.dev(A)
+    C.call("tomos_fused_config_clear")
+    try:
+        best_b, res_b = autotune.tune_fused(C.FP32_BF16, m, n, wh.ptr, m, reps=2,
+                                            cands=[(8, 256, 0), (4, 512, 0)])
+        best_f, _ = autotune.tune_fused(C.F32, m, n, af.ptr, m, reps=2, cands=[(2, 256, 0), (8, 256, 1)])
+        assert len(res_b) == 2
+        got = autotune.installed(C.FP32_BF16, n)
+        assert got == {k: best_b[k] for k in ("row_groups", "blocks_per_cu", "threads")}
+        got = autotune.installed(C.F32, n)
+        assert got == {k: best_f[k] for k in ("row_groups", "blocks_per_cu", "threads")}
+    finally:
+        C.call("tomos_fused_config_clear")

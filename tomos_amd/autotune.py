@@ -64,6 +64,15 @@ def _time_pair(a_type, m, n, a_ptr, lda, reps, bufs, ev):
     return f.value / reps
 
 
+def installed(a_type, n):
+    """The shape installed for (a_type, n) as a dict, or None."""
+    g, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    C.call("tomos_fused_config_get", a_type, n, ctypes.byref(g), ctypes.byref(b), ctypes.byref(t))
+    if g.value == 0 and b.value == 0 and t.value == 0:
+        return None
+    return {"row_groups": g.value, "blocks_per_cu": b.value, "threads": t.value}
+
+
 def tune_fused(a_type, m, n, a_ptr, lda, reps=5, cands=None, install=True):
     """Time every candidate shape of the single-pass kernel on A (device
     pointer a_ptr: the hi plane for packed storage, lda) and install the
@@ -88,7 +97,9 @@ def tune_fused(a_type, m, n, a_ptr, lda, reps=5, cands=None, install=True):
             results.append({"row_groups": g, "threads": t, "blocks_per_cu": b, "ms": ms,
                             "GBps": byts / (ms * 1e-3) / 1e9})
     finally:
-        C.call("tomos_fused_config_clear")
+        # drop only this (a_type, n) entry: shapes tuned for other storage
+        # formats or widths stay installed
+        C.call("tomos_fused_config_set", a_type, n, 0, 0, 0)
         for e in ev:
             C.lib().tomos_event_destroy(e)
     best = min(results, key=lambda r: r["ms"])

@@ -731,6 +731,14 @@ int tomos_fused_config_clear(void) {
     return guarded([&] { lsqr_fused_config_clear(); });
 }
 
+int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_per_cu, int* threads) {
+    return guarded([&] {
+        require(row_groups && blocks_per_cu && threads, "tomos_fused_config_get: null pointer");
+        if (!lsqr_fused_config_get(a_type, n, row_groups, blocks_per_cu, threads))
+            *row_groups = *blocks_per_cu = *threads = 0;
+    });
+}
+
 size_t tomos_matvec_pair_scratch_size(int a_type, int64_t m, int64_t n) {
     if (m < 1 || n < 1 || n > kFusedMaxN) return 0;
     (void)a_type;
