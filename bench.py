@@ -60,6 +60,9 @@ def parse():
     p.add_argument("--gemm", type=int, default=8192, help="split GEMM size (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
                    help="batched differentiable least squares: number of uneven problems (0 = skip)")
+    p.add_argument("--stream-loads", type=int, default=0, choices=(0, 1, 2),
+                   help="matrix load policy: 0 auto (non-temporal above 192 MiB per pass), "
+                        "1 always non-temporal, 2 always cacheable")
     p.add_argument("--kernels-only", action="store_true",
                    help="only the hi-only GEMV-N at the full shape and a copy of the same bytes "
                         "(for hardware-counter runs); prints their timings")
@@ -257,6 +260,7 @@ def main():
     from tomos_amd._capi import DeviceBuffer
 
     C.call("tomos_set_device", 0)
+    C.call("tomos_stream_loads_set", args.stream_loads)
     cu, amaj, amin, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
     C.call("tomos_get_device_info", ctypes.byref(cu), ctypes.byref(amaj), ctypes.byref(amin),
            ctypes.byref(mem))

@@ -108,6 +108,14 @@ int tomos_event_destroy(tomos_event_t ev);
 int tomos_event_record(tomos_event_t ev, tomos_stream_t stream);
 int tomos_event_elapsed_ms(float* ms, tomos_event_t start, tomos_event_t stop);
 
+/* Cache policy of the matrix loads in the streaming kernels (GEMV, the
+ * single-pass LSQR kernel, CountSketch, the batched residual):
+ * 0 = auto (non-temporal loads when a pass reads more than 192 MiB of
+ * matrix data, cacheable below, so matrices that fit the MALL stay there
+ * between iterations), 1 = always non-temporal, 2 = always cacheable.
+ * Process-wide; results do not depend on it.  No reference counterpart. */
+int tomos_stream_loads_set(int mode);
+
 /* Device-to-device copy through a plain vectorized kernel (16-byte loads
  * and stores); exists so profilers can calibrate HBM byte counters against
  * a kernel of known traffic.  No reference counterpart. */

@@ -111,3 +111,22 @@ def test_tuning_one_format_keeps_the_other(capi):
         assert got == {k: best_f[k] for k in ("row_groups", "blocks_per_cu", "threads")}
     finally:
         C.call("tomos_fused_config_clear")
+
+
+@pytest.mark.parametrize("a_type", [C.F32, C.FP32_BF16])
+def test_load_policy_does_not_change_results(capi, a_type):
+    """Non-temporal and cacheable loads give bit-identical products (pair and GEMV)."""
+    rng = np.random.default_rng(5)
+    m, n = 30000, 300
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    p = rng.standard_normal(n)
+    u0 = rng.standard_normal(m).astype(np.float32)
+    outs = []
+    try:
+        for mode in (1, 2):
+            C.call("tomos_stream_loads_set", mode)
+            outs.append(run_pair(a_type, A, m, p, u0, 0.25))
+    finally:
+        C.call("tomos_stream_loads_set", 0)
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert C.lib().tomos_stream_loads_set(3) == 1

@@ -96,6 +96,7 @@ _SIGS = {
     "tomos_matvec_pair_scratch_size": (sz, [cint, i64, i64]),
     "tomos_fused_config_set": (cint, [cint, i64, cint, cint, cint]),
     "tomos_fused_config_clear": (cint, []),
+    "tomos_stream_loads_set": (cint, [cint]),
     "tomos_fused_config_get": (cint, [cint, i64, ctypes.POINTER(cint), ctypes.POINTER(cint), ctypes.POINTER(cint)]),
     "tomos_lsq_adjoint_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
     "tomos_lsq_adjoint_solve": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, vp, vp, i64,

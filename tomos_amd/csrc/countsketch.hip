@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void cs_finish_kernel(int64_t s, int64_t n, in
 // non-negative floats) of the values the sketch sums, or (BOUND_HI) of the
 // hi words, which bound the exact FP32_BF16 values within a factor 2.
 // grid (n, splits); 16-byte row vectors.
-template <int FMT, bool EXACT>
+template <int FMT, bool EXACT, bool NTL>
 __global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* __restrict__ a,
                                                         const uint16_t* __restrict__ wr, int64_t lda,
                                                         int64_t rows_per_split,
@@ -274,9 +274,9 @@ __global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* _
     for (int64_t i0 = r0 + static_cast<int64_t>(threadIdx.x) * V; i0 < r1; i0 += 256 * V) {
         float v[V];
         if (i0 + V <= r1) {
-            const uint4 h = *reinterpret_cast<const uint4*>(col + i0 * EB);
+            const uint4 h = load_a16<NTL>(col + i0 * EB);
             uint4 l = make_uint4(0, 0, 0, 0);
-            if constexpr (EXACT) l = *reinterpret_cast<const uint4*>(wr + i0 + j * lda);
+            if constexpr (EXACT) l = load_a16<NTL>(wr + i0 + j * lda);
             decode_rows<FMT, EXACT>(h, l, v);
         } else {
             for (int r = 0; r < V; ++r) v[r] = (i0 + r < r1) ? load_elem<FMT, EXACT>(a, wr, i0 + r + j * lda) : 0.0f;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigne
     colexp[j] = e;
 }
 
-template <int FMT, bool EXACT, int CG>
+template <int FMT, bool EXACT, int CG, bool NTL>
 __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
     int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
@@ -355,9 +355,9 @@ __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
             for (int c = 0; c < CG; ++c) {
                 if (c < ncols) {
                     const int64_t e = i0 + (c0 + c) * lda;
-                    h[c] = *reinterpret_cast<const uint4*>(abase + e * EB);
+                    h[c] = load_a16<NTL>(abase + e * EB);
                     if constexpr (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF))
-                        l[c] = *reinterpret_cast<const uint4*>(wr + e);
+                        l[c] = load_a16<NTL>(wr + e);
                     else
                         l[c] = make_uint4(0, 0, 0, 0);
                 }
@@ -545,8 +545,13 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
     // FP32_BF16's exact values are bounded by 2x their hi words: the hi plane
     // suffices; FP32_16's exact values need their own decode
     constexpr bool kExactMax = EXACT && FMT == FMT_P16;
-    cs_colmax_kernel<FMT, kExactMax><<<dim3(static_cast<unsigned>(n), static_cast<unsigned>(ceil_div(m, crows))),
-                                       256, 0, st>>>(m, a, wr, lda, crows, colmax);
+    // the apply pass rereads A right after this one: a matrix that fits the
+    // MALL is read from HBM once
+    constexpr int64_t kElemBytes = (FMT == FMT_F64 ? 8 : FMT == FMT_F32 ? 4 : 2) * (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF) ? 2 : 1);
+    const bool ntl = stream_loads(m * n * kElemBytes);
+    const dim3 cgrid(static_cast<unsigned>(n), static_cast<unsigned>(ceil_div(m, crows)));
+    if (ntl) cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, crows, colmax);
+    else cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, crows, colmax);
     cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(n, colmax, colexp, flag);
     TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
     int hflag = 0;
@@ -564,11 +569,20 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
                                                                         p.rows_per_split, p.ncg, colexp, shift,
                                                                         partial);
     };
-    switch (p.cg) {
-        case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8>); break;
-        case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4>); break;
-        case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2>); break;
-        default: launch(cs_apply_fx_kernel<FMT, EXACT, 1>); break;
+    if (ntl) {
+        switch (p.cg) {
+            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, true>); break;
+            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, true>); break;
+            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, true>); break;
+            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, true>); break;
+        }
+    } else {
+        switch (p.cg) {
+            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, false>); break;
+            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, false>); break;
+            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, false>); break;
+            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, false>); break;
+        }
     }
     TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
     cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits, partial,

@@ -52,17 +52,17 @@ struct Raw {
     uint32_t r[4];
 };
 
-template <int FMT, int DEC, int VEC>
+template <int FMT, int DEC, int VEC, bool NTL>
 __device__ __forceinline__ void load_raw(const void* __restrict__ a,
                                          const uint16_t* __restrict__ wr, int64_t idx,
                                          Raw<FMT, DEC, VEC>& R) {
     if constexpr (is16<FMT>()) {
         const uint16_t* p = static_cast<const uint16_t*>(a) + idx;
         if constexpr (VEC == 8) {
-            const uint4 t = *reinterpret_cast<const uint4*>(p);
+            const uint4 t = load_a16<NTL>(p);
             R.h[0] = t.x; R.h[1] = t.y; R.h[2] = t.z; R.h[3] = t.w;
             if constexpr (DEC == DEC_EXACT) {
-                const uint4 q = *reinterpret_cast<const uint4*>(wr + idx);
+                const uint4 q = load_a16<NTL>(wr + idx);
                 R.r[0] = q.x; R.r[1] = q.y; R.r[2] = q.z; R.r[3] = q.w;
             }
         } else {
@@ -72,7 +72,7 @@ __device__ __forceinline__ void load_raw(const void* __restrict__ a,
     } else if constexpr (FMT == FMT_F32) {
         const float* p = static_cast<const float*>(a) + idx;
         if constexpr (VEC == 4) {
-            const uint4 t = *reinterpret_cast<const uint4*>(p);
+            const uint4 t = load_a16<NTL>(p);
             R.h[0] = t.x; R.h[1] = t.y; R.h[2] = t.z; R.h[3] = t.w;
         } else {
             R.h[0] = __float_as_uint(*p);
@@ -80,7 +80,7 @@ __device__ __forceinline__ void load_raw(const void* __restrict__ a,
     } else {
         const double* p = static_cast<const double*>(a) + idx;
         if constexpr (VEC == 2) {
-            const uint4 t = *reinterpret_cast<const uint4*>(p);
+            const uint4 t = load_a16<NTL>(p);
             R.h[0] = t.x; R.h[1] = t.y; R.h[2] = t.z; R.h[3] = t.w;
         } else {
             const uint2 t = *reinterpret_cast<const uint2*>(p);
@@ -170,7 +170,7 @@ __device__ __forceinline__ void block_sumsq_out(double v, double* out) {
 }
 
 // ------------------------------------------------------------ NO_TRANSPOSE
-template <int FMT, int DEC, int VEC, class ACC>
+template <int FMT, int DEC, int VEC, class ACC, bool NTL>
 __global__ __launch_bounds__(kThreads) void gemv_n_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr,
     int64_t lda, const void* __restrict__ x, bool x64, void* __restrict__ y, bool y64, Coef coef,
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void gemv_n_kernel(
         for (; j + U <= j1; j += U) {
             Raw<FMT, DEC, VEC> R[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) load_raw<FMT, DEC, VEC>(a, wr, i0 + (j + u) * lda, R[u]);
+            for (int u = 0; u < U; ++u) load_raw<FMT, DEC, VEC, NTL>(a, wr, i0 + (j + u) * lda, R[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 ACC av[VEC];
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kThreads) void gemv_n_kernel(
         }
         for (; j < j1; ++j) {
             Raw<FMT, DEC, VEC> R;
-            load_raw<FMT, DEC, VEC>(a, wr, i0 + j * lda, R);
+            load_raw<FMT, DEC, VEC, NTL>(a, wr, i0 + j * lda, R);
             ACC av[VEC];
             decode<FMT, DEC, VEC, ACC>(R, av);
             const ACC xv = load_x<ACC>(x, x64, j);
@@ -272,7 +272,7 @@ struct WaveTR<1, OFF, T> {
     }
 };
 
-template <int FMT, int DEC, int VEC, class ACC, int CB>
+template <int FMT, int DEC, int VEC, class ACC, int CB, bool NTL>
 __global__ __launch_bounds__(kThreads) void gemv_t_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr,
     int64_t lda, const void* __restrict__ x, bool x64, int64_t rows_per_chunk, int ncg,
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kThreads) void gemv_t_kernel(
             Raw<FMT, DEC, VEC> R[UB];
 #pragma unroll
             for (int u = 0; u < UB; ++u)
-                if (cb + u < ncols) load_raw<FMT, DEC, VEC>(a, wr, base + (cb + u) * lda, R[u]);
+                if (cb + u < ncols) load_raw<FMT, DEC, VEC, NTL>(a, wr, base + (cb + u) * lda, R[u]);
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 if (cb + u < ncols) {
@@ -424,6 +424,13 @@ size_t scratch_for(const Kind& k, int trans, int64_t m, int64_t n) {
     return static_cast<size_t>(p.chunks) * static_cast<size_t>(n) * acc;
 }
 
+// Matrix bytes one GEMV pass reads.
+template <int FMT, int DEC>
+int64_t matrix_bytes(int64_t m, int64_t n) {
+    const int64_t esz = FMT == FMT_F64 ? 8 : (FMT == FMT_F32 ? 4 : (DEC == DEC_EXACT ? 4 : 2));
+    return m * n * esz;
+}
+
 template <int FMT, int DEC, class ACC>
 int run_n(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, const void* x,
           bool x64, void* y, bool y64, const Coef& coef, void* scratch, bool vec,
@@ -433,12 +440,15 @@ int run_n(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, 
     const int width = vec ? VF : 1;
     const dim3 grid(static_cast<unsigned>(ceil_div(m, static_cast<int64_t>(kThreads) * width)),
                     static_cast<unsigned>(p.splits));
-    if (width == VF)
-        gemv_n_kernel<FMT, DEC, VF, ACC><<<grid, kThreads, 0, s>>>(m, n, a, wr, lda, x, x64, y, y64,
-                                                                   coef, p.col_chunk, part);
+    if (width == VF && stream_loads(matrix_bytes<FMT, DEC>(m, n)))
+        gemv_n_kernel<FMT, DEC, VF, ACC, true><<<grid, kThreads, 0, s>>>(m, n, a, wr, lda, x, x64, y, y64,
+                                                                         coef, p.col_chunk, part);
+    else if (width == VF)
+        gemv_n_kernel<FMT, DEC, VF, ACC, false><<<grid, kThreads, 0, s>>>(m, n, a, wr, lda, x, x64, y, y64,
+                                                                          coef, p.col_chunk, part);
     else
-        gemv_n_kernel<FMT, DEC, 1, ACC><<<grid, kThreads, 0, s>>>(m, n, a, wr, lda, x, x64, y, y64,
-                                                                  coef, p.col_chunk, part);
+        gemv_n_kernel<FMT, DEC, 1, ACC, false><<<grid, kThreads, 0, s>>>(m, n, a, wr, lda, x, x64, y, y64,
+                                                                         coef, p.col_chunk, part);
     TOMOS_LAUNCH_CHECK("gemv_n_kernel");
     if (part) {
         gemv_n_finish<ACC><<<dim3(static_cast<unsigned>(ceil_div(m, kThreads))), kThreads, 0, s>>>(
@@ -459,11 +469,15 @@ void run_t(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
     const int64_t blocks = static_cast<int64_t>(p.chunks) * p.ncg;
     require(blocks < (int64_t(1) << 31), "gemv: grid too large");
     if (m > 0) {
-        if (vec)
-            gemv_t_kernel<FMT, DEC, VF, ACC, CB><<<dim3(static_cast<unsigned>(blocks)), kThreads, 0, s>>>(
+        const dim3 grid(static_cast<unsigned>(blocks));
+        if (vec && stream_loads(matrix_bytes<FMT, DEC>(m, n)))
+            gemv_t_kernel<FMT, DEC, VF, ACC, CB, true><<<grid, kThreads, 0, s>>>(
+                m, n, a, wr, lda, x, x64, p.rows_per_chunk, p.ncg, part, coef.skip);
+        else if (vec)
+            gemv_t_kernel<FMT, DEC, VF, ACC, CB, false><<<grid, kThreads, 0, s>>>(
                 m, n, a, wr, lda, x, x64, p.rows_per_chunk, p.ncg, part, coef.skip);
         else
-            gemv_t_kernel<FMT, DEC, 1, ACC, CB><<<dim3(static_cast<unsigned>(blocks)), kThreads, 0, s>>>(
+            gemv_t_kernel<FMT, DEC, 1, ACC, CB, false><<<grid, kThreads, 0, s>>>(
                 m, n, a, wr, lda, x, x64, p.rows_per_chunk, p.ncg, part, coef.skip);
         TOMOS_LAUNCH_CHECK("gemv_t_kernel");
     }

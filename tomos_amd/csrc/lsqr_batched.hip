@@ -112,46 +112,73 @@ __global__ void k_batch_restart_check(BProb* P, int nprob, double tol, double ma
     if (rr < tol || rr < maxrr) s->done = 1;
 }
 
+// Exact value of the RPL rows in one 16-byte vector (hi h, lo l).
+template <int FMT>
+__device__ __forceinline__ void residual_decode(const uint4& h, const uint4& l, float* av) {
+    if constexpr (FMT == 0) {
+        av[0] = __uint_as_float(h.x);
+        av[1] = __uint_as_float(h.y);
+        av[2] = __uint_as_float(h.z);
+        av[3] = __uint_as_float(h.w);
+    } else {
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const uint32_t hh = half ? hw[k] >> 16 : hw[k] & 0xFFFFu;
+                const uint32_t ll = half ? lw[k] >> 16 : lw[k] & 0xFFFFu;
+                av[2 * k + half] = FMT == 1 ? __uint_as_float((hh << 16) | ll) : codec::fp32_16_decode_exact(hh, ll);
+            }
+        }
+    }
+}
+
 // u_i = b_i - A_i x_i for a range of rows: exact decode (wh+wr for packed),
-// FP64 accumulation, FP32 store.  One lane per 16-byte row vector.
+// FP64 accumulation, FP32 store.  One lane per 16-byte row vector, kU
+// columns' loads in flight per lane.  Cacheable loads: the hi plane is read
+// again by the next grouped pass.
 template <int FMT>
 __global__ __launch_bounds__(kT) void k_batch_residual(const BProb* __restrict__ P,
                                                        const RowItem* __restrict__ items) {
     constexpr int RPL = FMT == 0 ? 4 : 8;  // FP32 / packed rows per 16-byte vector
+    constexpr int ESZ = FMT == 0 ? 4 : 2;
+    constexpr int kU = 8;
     const RowItem it = items[blockIdx.x];
     const BProb& p = P[it.prob];
     if (p.st->done) return;
     const int64_t r = it.row0 + static_cast<int64_t>(threadIdx.x) * RPL;
     if (r >= it.row1) return;
     const int nv = static_cast<int>(min<int64_t>(RPL, p.m - r));
+    const char* hb = static_cast<const char*>(p.a) + r * ESZ;
+    const uint16_t* lb = FMT != 0 ? p.wr + r : nullptr;
     double acc[RPL];
 #pragma unroll
     for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
-    for (int64_t j = 0; j < p.n; ++j) {
-        const double xj = p.x[j];
-        float av[RPL];
-        if constexpr (FMT == 0) {
-            const uint4 h = *reinterpret_cast<const uint4*>(static_cast<const float*>(p.a) + r + j * p.lda);
-            av[0] = __uint_as_float(h.x);
-            av[1] = __uint_as_float(h.y);
-            av[2] = __uint_as_float(h.z);
-            av[3] = __uint_as_float(h.w);
-        } else {
-            const uint16_t* wh = static_cast<const uint16_t*>(p.a) + r + j * p.lda;
-            const uint4 h = *reinterpret_cast<const uint4*>(wh);
-            const uint4 l = *reinterpret_cast<const uint4*>(p.wr + r + j * p.lda);
-            const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+    int64_t j = 0;
+    for (; j + kU <= p.n; j += kU) {
+        uint4 h[kU], l[kU];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                for (int half = 0; half < 2; ++half) {
-                    const uint32_t hh = half ? hw[k] >> 16 : hw[k] & 0xFFFFu;
-                    const uint32_t ll = half ? lw[k] >> 16 : lw[k] & 0xFFFFu;
-                    av[2 * k + half] = FMT == 1 ? __uint_as_float((hh << 16) | ll)
-                                                : codec::fp32_16_decode_exact(hh, ll);
-                }
-            }
+        for (int u = 0; u < kU; ++u) {
+            h[u] = *reinterpret_cast<const uint4*>(hb + (j + u) * p.lda * ESZ);
+            if constexpr (FMT != 0) l[u] = *reinterpret_cast<const uint4*>(lb + (j + u) * p.lda);
+            else l[u] = h[u];
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            float av[RPL];
+            residual_decode<FMT>(h[u], l[u], av);
+            const double xj = p.x[j + u];
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xj, acc[k]);
+        }
+    }
+    for (; j < p.n; ++j) {
+        const uint4 h = *reinterpret_cast<const uint4*>(hb + j * p.lda * ESZ);
+        const uint4 l = FMT != 0 ? *reinterpret_cast<const uint4*>(lb + j * p.lda) : h;
+        float av[RPL];
+        residual_decode<FMT>(h, l, av);
+        const double xj = p.x[j];
 #pragma unroll
         for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xj, acc[k]);
     }
@@ -436,6 +463,7 @@ struct Batched {
             g.skip = &q.st->done;
             g.sumsq = q.sumsq;
             g.tpart = q.tpart;
+            g.rounds = fused_rounds(L.g, L.threads, p.n);
             gi[i] = g;
             g.p = reinterpret_cast<const double*>(sc + L.zeros);
             g.u_coef = &q.st->one;

@@ -104,8 +104,8 @@ constexpr int log2_small() {
 
 // One block's share of the single-pass product: tiles [tile0, tile1) of an
 // m x n matrix; writes the block's sum of u_raw^2 to *sumsq_out and its
-// A^T u_raw partial (n floats) to tpart_out.
-template <int FMT, int G, int ROUNDS, int NT>
+// A^T u_raw partial (n floats) to tpart_out.  NTL: non-temporal loads of A.
+template <int FMT, int G, int ROUNDS, int NT, bool NTL>
 __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __restrict__ a,
                                            int64_t lda, const void* __restrict__ p, bool p64,
                                            void* __restrict__ u, bool u64, double cu, int64_t tile0,
@@ -155,7 +155,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         for (int r = 0; r < ROUNDS; ++r) {
             const int64_t j = static_cast<int64_t>(r) * CS + c;
             if (in && j < n)
-                t[r] = *reinterpret_cast<const uint4*>(abase + (my_row + j * lda) * ESZ);
+                t[r] = load_a16<NTL>(abase + (my_row + j * lda) * ESZ);
             else
                 t[r] = make_uint4(0u, 0u, 0u, 0u);
         }
@@ -261,7 +261,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 //            the column's lane owns (no atomics, fixed order).
 // The loops over column batches are not unrolled: registers stay bounded
 // whatever n (up to 2048).
-template <int FMT>
+template <int FMT, bool NTL>
 __device__ __forceinline__ void fused2_body(int64_t m, int64_t n, const void* __restrict__ a,
                                             int64_t lda, const void* __restrict__ p, bool p64,
                                             void* __restrict__ u, bool u64, double cu, int64_t tile0,
@@ -308,7 +308,7 @@ __device__ __forceinline__ void fused2_body(int64_t m, int64_t n, const void* __
             for (int q = 0; q < CPB; ++q) {
                 const int r = bt * CPB + q;
                 const int64_t j = slot + static_cast<int64_t>(NS) * r;
-                t[q] = (in && j < n) ? *reinterpret_cast<const uint4*>(colp + r * step)
+                t[q] = (in && j < n) ? load_a16<NTL>(colp + r * step)
                                      : make_uint4(0u, 0u, 0u, 0u);
             }
 #pragma unroll
@@ -356,7 +356,7 @@ __device__ __forceinline__ void fused2_body(int64_t m, int64_t n, const void* __
             for (int q = 0; q < CPB; ++q) {
                 const int r = bt * CPB + q;
                 const int64_t j = slot + static_cast<int64_t>(NS) * r;
-                t[q] = (in && j < n) ? *reinterpret_cast<const uint4*>(colp + r * step)
+                t[q] = (in && j < n) ? load_a16<NTL>(colp + r * step)
                                      : make_uint4(0u, 0u, 0u, 0u);
             }
             float cs[CPB];
@@ -388,7 +388,7 @@ __device__ __forceinline__ void fused2_body(int64_t m, int64_t n, const void* __
     }
 }
 
-template <int FMT>
+template <int FMT, bool NTL>
 __global__ __launch_bounds__(1024) void lsqr_fused2_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
     bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(1024) void lsqr_fused2_kernel(
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
     const int64_t ntiles = (m + TR - 1) / TR;
     const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
-    fused2_body<FMT>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+    fused2_body<FMT, NTL>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
                         tpart + static_cast<int64_t>(blockIdx.x) * n);
 }
 
@@ -408,11 +408,11 @@ __global__ __launch_bounds__(1024) void grouped_fused2_kernel(const GroupProb* _
     const GroupItem it = items[blockIdx.x];
     const GroupProb& P = probs[it.prob];
     if (P.skip && *P.skip) return;
-    fused2_body<FMT>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0, it.tile1,
+    fused2_body<FMT, false>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0, it.tile1,
                         P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
 }
 
-template <int FMT, int G, int ROUNDS, int NT>
+template <int FMT, int G, int ROUNDS, int NT, bool NTL>
 __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
     bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
@@ -422,20 +422,41 @@ __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
     const int64_t ntiles = (m + TR - 1) / TR;
     const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
-    fused_body<FMT, G, ROUNDS, NT>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+    fused_body<FMT, G, ROUNDS, NT, NTL>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
                                tpart + static_cast<int64_t>(blockIdx.x) * n);
+}
+
+// Body of a grouped work item with the fewest column rounds R (G, 2G, 4G,
+// ... up to RMAX) that cover the item's problem (P.rounds): a narrow
+// problem does not pay for the widest problem's rounds.
+template <int FMT, int G, int R, int RMAX, int NT>
+__device__ __forceinline__ void grouped_item(const GroupProb& P, const GroupItem& it) {
+    if constexpr (R < RMAX) {
+        if (P.rounds <= R) {
+            fused_body<FMT, G, R, NT, false>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
+                                             it.tile1, P.sumsq + it.slot,
+                                             P.tpart + static_cast<int64_t>(it.slot) * P.n);
+            return;
+        }
+        grouped_item<FMT, G, (2 * R < RMAX ? 2 * R : RMAX), RMAX, NT>(P, it);
+    } else {
+        fused_body<FMT, G, RMAX, NT, false>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
+                                            it.tile1, P.sumsq + it.slot,
+                                            P.tpart + static_cast<int64_t>(it.slot) * P.n);
+    }
 }
 
 // Many problems in one launch: block b runs work item items[b] (a tile
 // range of one problem).  A problem whose skip flag is set is a no-op.
+// Loads are cacheable: a batch of problems small enough to batch is
+// reread from the MALL at the next iteration.
 template <int FMT, int G, int ROUNDS, int NT>
 __global__ __launch_bounds__(NT) void grouped_fused_kernel(const GroupProb* __restrict__ probs,
                                                                  const GroupItem* __restrict__ items) {
     const GroupItem it = items[blockIdx.x];
     const GroupProb& P = probs[it.prob];
     if (P.skip && *P.skip) return;
-    fused_body<FMT, G, ROUNDS, NT>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
-                               it.tile1, P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
+    grouped_item<FMT, G, G, ROUNDS, NT>(P, it);
 }
 
 // level-1 reduction of the block partials: out[s][j] = sum over the blocks
@@ -487,11 +508,12 @@ int rounds_for(int g, int nt, int64_t n) {
     X(8, 512, 8) X(8, 512, 16)
 
 // The kernel instance for (format, shape), or nullptr.
-template <int FMT>
+template <int FMT, bool NTL>
 const void* fused_kernel_ptr(int g, int nt, int rounds) {
-    if (g == 32 && nt == 1024) return rounds > 0 ? reinterpret_cast<const void*>(lsqr_fused2_kernel<FMT>) : nullptr;
+    if (g == 32 && nt == 1024)
+        return rounds > 0 ? reinterpret_cast<const void*>(lsqr_fused2_kernel<FMT, NTL>) : nullptr;
 #define FUSED_CASE(G, NT, R) \
-    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT>);
+    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT, NTL>);
     TOMOS_FUSED_SHAPES(FUSED_CASE)
 #undef FUSED_CASE
     return nullptr;
@@ -513,10 +535,15 @@ const void* grouped_kernel_for(int a_type, int g, int nt, int rounds) {
     return grouped_kernel_ptr<FMT_F32>(g, nt, rounds);
 }
 
-const void* fused_kernel_for(int a_type, int g, int nt, int rounds) {
-    if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF>(g, nt, rounds);
-    if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16>(g, nt, rounds);
-    return fused_kernel_ptr<FMT_F32>(g, nt, rounds);
+const void* fused_kernel_for(int a_type, int g, int nt, int rounds, bool ntl) {
+    if (ntl) {
+        if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF, true>(g, nt, rounds);
+        if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16, true>(g, nt, rounds);
+        return fused_kernel_ptr<FMT_F32, true>(g, nt, rounds);
+    }
+    if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF, false>(g, nt, rounds);
+    if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16, false>(g, nt, rounds);
+    return fused_kernel_ptr<FMT_F32, false>(g, nt, rounds);
 }
 
 }  // namespace
@@ -571,10 +598,11 @@ FusedPlan lsqr_fused_plan(int a_type, int64_t m, int64_t n, int g, int blocks_pe
     p.g = g;
     p.threads = threads;
     p.rounds = rounds_for(g, threads, n);
+    p.stream = stream_loads(std::max<int64_t>(m, 1) * n * (a_type == TOMOS_F32 ? 4 : 2));
     if (blocks_per_cu <= 0) {
         // as many blocks as fit on the device at once: one wave of blocks
         int occ = 0;
-        const void* k = p.rounds > 0 ? fused_kernel_for(a_type, g, threads, p.rounds) : nullptr;
+        const void* k = p.rounds > 0 ? fused_kernel_for(a_type, g, threads, p.rounds, p.stream) : nullptr;
         if (k && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0) != hipSuccess) occ = 0;
         blocks_per_cu = occ > 0 ? occ : 2;
     }
@@ -636,7 +664,7 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
                        hipStream_t s) {
     require(plan.rounds > 0 && rounds_for(plan.g, plan.threads, n) == plan.rounds,
             "lsqr_fused: plan does not fit n");
-    const void* k = fused_kernel_for(a_type, plan.g, plan.threads, plan.rounds);
+    const void* k = fused_kernel_for(a_type, plan.g, plan.threads, plan.rounds, plan.stream);
     if (!k) throw Error(kErrInternal, "lsqr_fused: no kernel for this plan");
     int64_t tpb = plan.tiles_per_block;
     void* args[] = {&m, &n, const_cast<void**>(&a), &lda, const_cast<void**>(&p), &p64, &u, &u64,

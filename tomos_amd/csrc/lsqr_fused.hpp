@@ -21,6 +21,7 @@ struct FusedPlan {
     int splits = 1;
     size_t tpart_bytes = 0;  // [nblocks][n] float block partials of A^T u_raw
     size_t tsum_bytes = 0;   // [splits][n] double level-1 sums
+    bool stream = false;     // non-temporal loads of A (stream_loads of its bytes)
 };
 
 // A in a_type with the per-iteration (DEFAULT) tier, column-major,
@@ -64,6 +65,7 @@ struct GroupProb {
     const int* skip;
     double* sumsq;
     float* tpart;
+    int rounds;  // column rounds covering n for the launch's (g, threads)
 };
 
 // One block's work: tiles [tile0, tile1) of problem `prob`, partial slot `slot`.

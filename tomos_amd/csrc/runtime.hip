@@ -2,6 +2,7 @@
 // memory/stream/event wrappers, the fallback scratch cache, and the
 // deterministic synthetic-data generator used by benchmarks.
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -62,6 +63,19 @@ void* fallback_scratch(size_t bytes) {
     }
     return c.scratch;
 }
+
+namespace {
+std::atomic<int> g_stream_load_mode{0};
+}  // namespace
+
+bool stream_loads(int64_t bytes) {
+    const int mode = g_stream_load_mode.load(std::memory_order_relaxed);
+    if (mode == 1) return true;
+    if (mode == 2) return false;
+    return bytes > kStreamLoadBytes;
+}
+
+void set_stream_load_mode(int mode) { g_stream_load_mode.store(mode, std::memory_order_relaxed); }
 
 // ----------------------------------------------------------------------
 // Synthetic data: counter-based hash -> two uniforms -> Box-Muller.
@@ -206,6 +220,13 @@ int tomos_event_elapsed_ms(float* ms, tomos_event_t a, tomos_event_t b) {
     return guarded([&] {
         TOMOS_HIP_CHECK(hipEventSynchronize(b));
         TOMOS_HIP_CHECK(hipEventElapsedTime(ms, a, b));
+    });
+}
+
+int tomos_stream_loads_set(int mode) {
+    return guarded([&] {
+        require(mode >= 0 && mode <= 2, "tomos_stream_loads_set: mode must be 0, 1 or 2");
+        set_stream_load_mode(mode);
     });
 }
 
