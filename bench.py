@@ -73,6 +73,20 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def time_call(fn, reps, ev0, ev1):
+    """Average ms of fn over reps launches on the NULL stream (after 2 warm calls)."""
+    from tomos_amd import _capi as C
+    for _ in range(2):
+        fn()
+    C.call("tomos_event_record", ev0, None)
+    for _ in range(reps):
+        fn()
+    C.call("tomos_event_record", ev1, None)
+    f = ctypes.c_float()
+    C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+    return f.value / reps
+
+
 def time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1):
     """Average ms of tomos_matvec_pair (u <- A p - 0.5 u, t <- A^T u) on the
     hi plane, over --gemv-reps launches on the launch stream."""
@@ -230,6 +244,7 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
         ("gemv_n_bf16_hi", lambda: C.call("tomos_gemv", C.FP32_BF16, C.TIER_DEFAULT, 0, 1.0, m, n,
                                           wh.ptr, None, m, C.F32, xv.ptr, 0.0, C.F32, y.ptr, None, 0, None)),
         ("copy", lambda: C.call("tomos_copy_kernel", dst.ptr, wh.ptr, nbytes, None)),
+        ("read", lambda: C.call("tomos_read_kernel", wh.ptr, nbytes, dst.ptr, None)),
     ):
         fn()
         C.call("tomos_event_record", ev0, None)
@@ -417,6 +432,11 @@ def main():
     t_f = time_gemv(0, C.F32, 0, A.ptr, None)
     t_ft = time_gemv(1, C.F32, 0, A.ptr, None)
     gbs = lambda byts, ms: byts / (ms * 1e-3) / 1e9
+    # achievable read rate: a plain read-only stream over the same hi plane
+    rflag = DeviceBuffer(4)
+    C.call("tomos_memset", rflag.ptr, 0, 4, None)
+    t_rd = time_call(lambda: C.call("tomos_read_kernel", wh.ptr, 2 * m * n, rflag.ptr, None), args.gemv_reps,
+                     ev0, ev1)
 
     # the LSQR iteration's dominant kernel: A p and A^T u from one read of the
     # hi plane (tomos_matvec_pair = lsqr_fused_kernel + its partial sums)
@@ -430,6 +450,7 @@ def main():
         "bf16_exact_N": {"ms": round(t_nx, 4), "GBps": round(gbs(gemv_bytes_f32, t_nx), 1)},
         "fp32_N": {"ms": round(t_f, 4), "GBps": round(gbs(gemv_bytes_f32, t_f), 1)},
         "fp32_T": {"ms": round(t_ft, 4), "GBps": round(gbs(gemv_bytes_f32, t_ft), 1)},
+        "read_stream_same_bytes": {"ms": round(t_rd, 4), "GBps": round(gbs(2.0 * m * n, t_rd), 1)},
     }
     log("gemv: %s" % gemv_detail)
     traffic = None

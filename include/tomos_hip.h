@@ -120,6 +120,11 @@ int tomos_stream_loads_set(int mode);
  * and stores); exists so profilers can calibrate HBM byte counters against
  * a kernel of known traffic.  No reference counterpart. */
 int tomos_copy_kernel(void* dst, const void* src, size_t bytes, tomos_stream_t stream);
+/* Read-only stream over bytes (rounded down to 16-byte vectors) of src with
+ * non-temporal 16-byte loads: the achievable HBM read rate, next to the
+ * published peak.  *flag (device) is left unchanged in practice.  No
+ * reference counterpart. */
+int tomos_read_kernel(const void* src, size_t bytes, unsigned* flag, tomos_stream_t stream);
 
 /* Deterministic synthetic data on the device (no reference counterpart;
  * used by benchmarks to build the 1e6..2e6-row problems in HBM).

@@ -83,6 +83,7 @@ _SIGS = {
     "tomos_event_record": (cint, [vp, vp]),
     "tomos_event_elapsed_ms": (cint, [ctypes.POINTER(ctypes.c_float), vp, vp]),
     "tomos_copy_kernel": (cint, [vp, vp, sz, vp]),
+    "tomos_read_kernel": (cint, [vp, sz, vp, vp]),
     "tomos_fill_normal_f32": (cint, [vp, i64, i64, i64, ctypes.c_uint64, vp, cint, cint, vp]),
     "tomos_pack": (cint, [cint, i64, i64, vp, i64, vp, vp, i64, vp]),
     "tomos_unpack": (cint, [cint, i64, i64, vp, vp, i64, vp, i64, vp]),

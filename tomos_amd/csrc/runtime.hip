@@ -118,6 +118,30 @@ __global__ __launch_bounds__(256) void copy16_kernel(uint4* __restrict__ dst,
         dst[i] = src[i];
 }
 
+// Read-only stream of n16 16-byte vectors (non-temporal loads, 8 in flight
+// per lane); *flag is written only if the xor of everything read equals a
+// magic word, so the loads cannot be optimised away and nothing else is
+// stored.  Measures the achievable HBM read rate.
+__global__ __launch_bounds__(1024) void read16_kernel(const uint4* __restrict__ src, int64_t n16,
+                                                      unsigned* __restrict__ flag) {
+    constexpr int U = 8;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * 1024;
+    int64_t i = static_cast<int64_t>(blockIdx.x) * 1024 + threadIdx.x;
+    unsigned acc = 0;
+    for (; i + (U - 1) * stride < n16; i += U * stride) {
+        uint4 t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = load_stream16(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc ^= t[u].x ^ t[u].y ^ t[u].z ^ t[u].w;
+    }
+    for (; i < n16; i += stride) {
+        const uint4 t = load_stream16(src + i);
+        acc ^= t.x ^ t.y ^ t.z ^ t.w;
+    }
+    if (acc == 0x9E3779B9u) atomicOr(flag, 1u);
+}
+
 __global__ void copy_tail_kernel(char* __restrict__ dst, const char* __restrict__ src, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (i < n) dst[i] = src[i];
@@ -249,6 +273,19 @@ int tomos_copy_kernel(void* dst, const void* src, size_t bytes, tomos_stream_t s
                 static_cast<char*>(dst) + done, static_cast<const char*>(src) + done, rest);
             TOMOS_LAUNCH_CHECK("copy_tail_kernel");
         }
+    });
+}
+
+int tomos_read_kernel(const void* src, size_t bytes, unsigned* flag, tomos_stream_t s) {
+    return guarded([&] {
+        require(src && flag, "tomos_read_kernel: null pointer");
+        require(reinterpret_cast<uintptr_t>(src) % 16 == 0, "tomos_read_kernel: src must be 16-byte aligned");
+        const int64_t n16 = static_cast<int64_t>(bytes / 16);
+        if (n16 == 0) return;
+        const int64_t blocks = std::min<int64_t>(ceil_div(n16, 1024), int64_t(cu_count()) * 2);
+        read16_kernel<<<dim3(static_cast<unsigned>(blocks)), 1024, 0, s>>>(static_cast<const uint4*>(src), n16,
+                                                                          flag);
+        TOMOS_LAUNCH_CHECK("read16_kernel");
     });
 }
 
