@@ -7,14 +7,15 @@
 // largest one.
 //
 // QR, blocked by panels of NB = 32 columns (LAPACK dgeqrf's structure):
-//  * panel: one launch per column c.  Every block recomputes the reflector
-//    of column c (LAPACK dlarfg convention: beta = -sign(x0)|x|,
-//    v = [1, x(1:)/(x0-beta)], tau = (beta-x0)/beta; a zero column gives no
-//    reflection) and applies it to the remaining panel column it owns: one
-//    pass computes |x|^2 and x^T A(:,j) together, a second does
-//    A(:,j) -= tau*(v^T A(:,j))*v.  Column c itself is never written during
-//    its own step (other blocks are reading it); R's diagonal, tau and the
-//    reflector scale go to separate arrays.
+//  * panel: one launch per panel, one block per panel column.  The block
+//    owning column j applies the reflectors c < j as each column c becomes
+//    final (a per-column flag, published with release/acquire at device
+//    scope), then forms column j's reflector (LAPACK dlarfg convention:
+//    beta = -sign(x0)|x|, v = [1, x(1:)/(x0-beta)], tau = (beta-x0)/beta; a
+//    zero column gives no reflection) and publishes it: a wavefront over
+//    the panel's columns instead of one launch per column.  Column c keeps
+//    x (the reflector is implicit); R's diagonal, tau and the reflector
+//    scale go to separate arrays.
 //  * V and T: the panel's reflectors are written as an explicit unit lower
 //    trapezoidal V, and T (upper triangular, H_1...H_nb = I - V T V^T) is
 //    formed from the Gram matrix V^T V (dlarft's recurrence).
@@ -53,71 +54,437 @@ struct QrDesc {
     double* wp;      // W partials: nsplit x n x kNB; the inverse's copy of R (n x n)
     double* w2;      // T^T W: n x kNB (column jt at w2 + jt*kNB)
     int* bad;        // trtri: first zero diagonal (INT_MAX if none)
+    int* sync;       // panel kernel: [0] ticket counter, [kSyncHead + c] column c final
+    double* tl;      // TSQR: leaf T factors, kTsqrMaxLeaves x kNB^2
+    double* vc;      // TSQR: combine reflectors, kTsqrComb x kNB (row-major)
+    double* tc;      // TSQR: combine T, kNB^2
 };
+constexpr int kSyncHead = 64;  // ints before the column flags (own cache lines)
+constexpr int kTsqrMaxLeaves = 16;           // leaves per panel (tile rows 256 or 512)
+constexpr int kTsqrComb = kTsqrMaxLeaves * kNB;  // stacked rows of the combine step
+constexpr int64_t kTsqrMaxK = 16 * 512;      // k beyond this: the wavefront panel path
 
+// Row splits of W = V^T A: 128-row splits, at most 32, so a panel's W has
+// ~1000 blocks in flight at k = 4000.
 __host__ __device__ inline int64_t qr_nsplit(int64_t k) {
-    const int64_t s = (k + 511) / 512;
-    return s < 1 ? 1 : (s > 8 ? 8 : s);
+    const int64_t s = (k + 127) / 128;
+    return s < 1 ? 1 : (s > 32 ? 32 : s);
+}
+
+// Fixed-order sum of cnt doubles at p[0], p[stride], ...: four running sums
+// so that several loads are in flight, combined as (s0 + s1) + (s2 + s3).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int cnt, int64_t stride) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = 0;
+    for (; i + 4 <= cnt; i += 4) {
+        s0 += p[(i + 0) * stride];
+        s1 += p[(i + 1) * stride];
+        s2 += p[(i + 2) * stride];
+        s3 += p[(i + 3) * stride];
+    }
+    for (; i < cnt; ++i) s0 += p[i * stride];
+    return (s0 + s1) + (s2 + s3);
 }
 
 __global__ void qr_set_desc_kernel(QrDesc* d, QrDesc v) { *d = v; }
+__global__ void qr_status_reset_kernel(int* status) { *status = 0; }
 
 constexpr int kPT = 1024;  // panel-step block
 
-// Column c of the panel starting at c0: block b applies the reflector of
-// column c to panel column j = c + 1 + b (block 0 also records diag, tau
-// and the reflector scale).  One pass over the rows computes both
-// |x(c+1:)|^2 and x(c+1:)^T A(c+1:, j) (v^T A(:,j) = A(c,j) + inv_v0 * that
-// dot, since v = [1, x(c+1:) * inv_v0]), a second applies the update.
-__global__ __launch_bounds__(kPT) void qr_panel_step_kernel(const QrDesc* __restrict__ D, int64_t c0,
-                                                            int64_t c) {
+// Block-wide FP64 sum over kPT threads, same order in every thread; red
+// must not be reused before a barrier that follows every thread's read.
+__device__ __forceinline__ double panel_sum(double v, double* red) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kPT / 64; ++w) t += red[w];
+    return t;
+}
+
+// Coherent read of data another block of this launch published (it may run
+// on another XCD, whose L2 this one does not see).
+__device__ __forceinline__ double load_published(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int kPanelSpinLimit = 1 << 18;  // ~0.3 s of polling; a legitimate wait is microseconds
+
+// Whole panel in one launch.  The block with logical id b (from the
+// problem's ticket counter: a block only ever waits on blocks that started
+// before it, so every wait ends) owns panel column j = c0 + b.  For each
+// reflector c = c0 .. j-1 it waits until column c is final, reads column c
+// and its tau / inv_v0, and applies H_c to column j (one pass for
+// v^T A(:,j), one for the update; same arithmetic as LAPACK dlarf).  Then it
+// forms column j's own reflector (dlarfg convention: beta = -sign(x0)|x|,
+// tau = (beta - x0)/beta, v = [1, x(j+1:)/(x0 - beta)]; a zero column gives
+// none), records diag/tau/inv_v0, and publishes column j.  Column j keeps x
+// (the reflector is stored implicitly).  A wait that exceeds the spin limit
+// sets *status and ends the block (the host turns it into an error).
+__global__ __launch_bounds__(kPT) void qr_panel_kernel(const QrDesc* __restrict__ D, int64_t c0, int64_t panel,
+                                                       int* __restrict__ status) {
     const QrDesc& d = D[blockIdx.z];
-    if (c >= d.n) return;
-    const int64_t c_end = min(c0 + kNB, d.n);
-    const int64_t j = c + 1 + blockIdx.x;
-    const bool has_col = j < c_end;
-    if (blockIdx.x > 0 && !has_col) return;
-    __shared__ double red[2][kPT / 64];
-    const double* col = d.a + c * d.lda;
-    double* cj = d.a + (has_col ? j : c) * d.lda;
-    const double x0 = col[c];
-    const double xc = has_col ? cj[c] : 0.0;  // read before the barriers: thread 0 rewrites it
-    double ss = 0.0, dot = 0.0;
-    for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) {
-        const double xi = col[i];
-        ss += xi * xi;
-        if (has_col) dot += xi * cj[i];
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        ss += __shfl_xor(ss, o);
-        dot += __shfl_xor(dot, o);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[0][wave] = ss;
-        red[1][wave] = dot;
+    if (c0 >= d.n) return;
+    int* ticket = d.sync;
+    int* flag = d.sync + kSyncHead;
+    __shared__ double red[kPT / 64];
+    __shared__ int sb, sfail;
+    if (threadIdx.x == 0) {
+        sb = atomicAdd(ticket, 1) - static_cast<int>(panel * kNB);
+        sfail = 0;
     }
     __syncthreads();
-    double sigma = 0.0, vdot = 0.0;
-    for (int w = 0; w < kPT / 64; ++w) {  // same order in every thread
-        sigma += red[0][w];
-        vdot += red[1][w];
+    const int64_t j = c0 + sb;
+    if (sb < 0 || j >= min(c0 + kNB, d.n)) return;
+    double* cj = d.a + j * d.lda;
+    for (int64_t c = c0; c < j; ++c) {
+        if (threadIdx.x == 0) {
+            int spins = 0;
+            while (__hip_atomic_load(flag + c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                if (++spins > kPanelSpinLimit) {
+                    sfail = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (sfail) {
+            if (threadIdx.x == 0) atomicOr(status, 1);
+            return;
+        }
+        const double* col = d.a + c * d.lda;
+        const double tau = load_published(d.tau + c), inv_v0 = load_published(d.inv_v0 + c);
+        const double xc = cj[c];  // read before the reduction's barrier: thread 0 rewrites it
+        double dot = 0.0;
+        for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) dot += load_published(col + i) * cj[i];
+        dot = panel_sum(dot, red);
+        if (tau != 0.0) {
+            const double w = tau * (xc + inv_v0 * dot);
+            const double wv = w * inv_v0;
+            for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) cj[i] -= wv * load_published(col + i);
+            if (threadIdx.x == 0) cj[c] = xc - w;
+        }
+        __syncthreads();
     }
+    const double x0 = cj[j];
+    double ss = 0.0;
+    for (int64_t i = j + 1 + threadIdx.x; i < d.k; i += kPT) ss += cj[i] * cj[i];
+    const double sigma = panel_sum(ss, red);
     const double nrm = sqrt(x0 * x0 + sigma);
     const bool active = nrm != 0.0;
     const double beta = x0 >= 0.0 ? -nrm : nrm;
-    const double tau = active ? (beta - x0) / beta : 0.0;
-    const double inv_v0 = active ? 1.0 / (x0 - beta) : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        d.diag[c] = active ? beta : x0;
-        d.tau[c] = tau;
-        d.inv_v0[c] = inv_v0;
+    if (threadIdx.x == 0) {
+        d.diag[j] = active ? beta : x0;
+        d.tau[j] = active ? (beta - x0) / beta : 0.0;
+        d.inv_v0[j] = active ? 1.0 / (x0 - beta) : 0.0;
     }
-    if (!active || !has_col) return;
-    const double w = tau * (xc + inv_v0 * vdot);
-    const double wv = w * inv_v0;
-    for (int64_t i = c + 1 + threadIdx.x; i < d.k; i += kPT) cj[i] -= wv * col[i];
-    if (threadIdx.x == 0) cj[c] = xc - w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_store(flag + j, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------- TSQR panel
+// Panel of kNB columns over rows c0..k-1 as a two-level tree (tall-skinny
+// QR): the rows are cut into leaves of LR = 256 or 512 rows; each leaf is
+// factored in registers by one block (one row per thread), the leaves'
+// R factors (stacked, up to 16 x 32 rows) by one more block, and the
+// trailing columns are updated with each level's block reflector
+// I - V T V^T.  The sequential part per column is a block-local reduction,
+// never a device-wide one: four launches per panel.  R is unique up to the
+// signs of its rows, so the result matches column-by-column Householder
+// up to those signs and rounding.
+
+// Wave transpose-reduce of 32 values per lane: afterwards v[0] of lane L
+// holds the wave sum of value (L >> 1) & 31 (lanes 2i and 2i+1 agree).
+template <int HALF>
+__device__ __forceinline__ void tr_stage(double* v, int lane) {
+    const bool up = (lane & (2 * HALF)) != 0;
+#pragma unroll
+    for (int t = 0; t < HALF; ++t) {
+        const double send = up ? v[t] : v[t + HALF];
+        const double keep = up ? v[t + HALF] : v[t];
+        v[t] = keep + __shfl_xor(send, 2 * HALF);
+    }
+}
+__device__ __forceinline__ void wave_tr32(double* v, int lane) {
+    tr_stage<16>(v, lane);
+    tr_stage<8>(v, lane);
+    tr_stage<4>(v, lane);
+    tr_stage<2>(v, lane);
+    tr_stage<1>(v, lane);
+    v[0] += __shfl_xor(v[0], 1);
+}
+
+// One column step of block_qr32 with the column index J a constant, so
+// every register index is static; recurses to the next column.
+template <int NT, int J>
+__device__ __forceinline__ void block_qr32_step(double (&a)[kNB], int nbp, double* Ts, double* sh) {
+    if constexpr (J < kNB) {
+        if (J >= nbp) return;
+        constexpr int NW = NT / 64;
+        const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+        // sh: red[2][NW][kNB] | rowj[2][kNB] | u[2][kNB] | par[2][4]; buffers
+        // alternate by step parity, so two barriers per column suffice
+        constexpr int b = J & 1;
+        double* rd = sh + b * NW * kNB;
+        double* rj = sh + 2 * NW * kNB + b * kNB;
+        double* u = sh + 2 * NW * kNB + 2 * kNB + b * kNB;
+        double* par = sh + 2 * NW * kNB + 4 * kNB + b * 4;
+        if (t == J) {
+#pragma unroll
+            for (int q = 0; q < kNB; ++q) rj[q] = a[q];
+        }
+        const double x = t > J ? a[J] : 0.0;
+        double vals[kNB];
+#pragma unroll
+        for (int q = 0; q < kNB; ++q) vals[q] = x * a[q];  // vals[J] = x^2
+        wave_tr32(vals, lane);
+        if ((lane & 1) == 0) rd[wave * kNB + ((lane >> 1) & 31)] = vals[0];
+        __syncthreads();
+        // first wave: tot_q = sum over rows below J of x * A(:, q) (fixed
+        // order), the reflector, and u_q = V(J, q) + inv * tot_q, which is
+        // s_q = v_J^T A(:, q) for q > J and z_q = v_q^T v_J for q < J
+        if (t < kNB) {
+            double tq = rd[t], tj = rd[J];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) {
+                tq += rd[w * kNB + t];
+                tj += rd[w * kNB + J];
+            }
+            const double x0 = rj[J];
+            const double nrm = sqrt(x0 * x0 + tj);
+            const bool active = nrm != 0.0;
+            const double beta = x0 >= 0.0 ? -nrm : nrm;
+            const double inv = active ? 1.0 / (x0 - beta) : 0.0;
+            u[t] = rj[t] + inv * tq;
+            if (t == 0) {
+                par[0] = active ? (beta - x0) / beta : 0.0;  // tau
+                par[1] = inv;
+                par[2] = beta;
+                par[3] = active ? 1.0 : 0.0;
+            }
+        }
+        __syncthreads();
+        const double tau = par[0], inv = par[1], beta = par[2];
+        const bool upd = par[3] != 0.0 && t >= J;
+        const double v = t > J ? x * inv : 1.0;
+        if (upd) {
+            const double tv = tau * v;
+#pragma unroll
+            for (int l = J + 1; l < kNB; ++l) a[l] -= tv * u[l];
+            a[J] = t > J ? v : beta;
+        }
+        // T(0:J, J) = -tau T(0:J, 0:J) z (Ts columns < J were written before
+        // this step's barriers)
+        if (t < J) {
+            double acc = 0.0;
+#pragma unroll
+            for (int pp = 0; pp < J; ++pp) {
+                const double term = Ts[pp * kNB + t] * u[pp];
+                acc += pp >= t ? term : 0.0;
+            }
+            Ts[J * kNB + t] = -tau * acc;
+        }
+        if (t == J) Ts[J * kNB + J] = tau;
+        block_qr32_step<NT, J + 1>(a, nbp, Ts, sh);
+    }
+}
+
+// LDS scratch of block_qr32, in doubles.
+__host__ __device__ constexpr int block_qr32_lds_doubles(int nt) { return 2 * (nt / 64) * kNB + 4 * kNB + 8; }
+
+// a[] = this thread's row (row index t of the block's rows; rows past the
+// tile are zero).  Householder QR of the NT x nbp block in place: rows
+// t <= q of column q end as R, rows t > q as the reflector entries
+// (unit diagonal implicit).  Ts (LDS, column-major 32 x 32) gets the upper
+// triangular T with H_0 ... H_{nbp-1} = I - V T V^T (dlarft forward
+// columnwise, formed column by column).  dlarfg convention per column:
+// beta = -sign(x0)|x|, tau = (beta - x0)/beta, v = [1, x(1:)/(x0 - beta)];
+// a zero column gives tau = 0.  sh: block_qr32_lds_doubles(NT) doubles.
+template <int NT>
+__device__ __forceinline__ void block_qr32(double (&a)[kNB], int nbp, double* Ts, double* sh) {
+    for (int i = threadIdx.x; i < kNB * kNB; i += NT) Ts[i] = 0.0;
+    __syncthreads();
+    block_qr32_step<NT, 0>(a, nbp, Ts, sh);
+    __syncthreads();
+}
+
+// Leaf factorization: block (leaf l, problem z) factors rows
+// r0 = c0 + l*NT .. of the panel; R and V back into A, T into d.tl[l].
+template <int NT>
+__global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * NT;
+    if (r0 >= d.k) return;
+    const int rows = static_cast<int>(min<int64_t>(NT, d.k - r0));
+    __shared__ double Ts[kNB * kNB];
+    __shared__ double sh[block_qr32_lds_doubles(NT)];
+    const int t = threadIdx.x;
+    double a[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) a[q] = (t < rows && q < nbp) ? d.a[r0 + t + (c0 + q) * d.lda] : 0.0;
+    block_qr32<NT>(a, nbp, Ts, sh);
+    if (t < rows) {
+#pragma unroll
+        for (int q = 0; q < kNB; ++q)
+            if (q < nbp) d.a[r0 + t + (c0 + q) * d.lda] = a[q];
+    }
+    double* tl = d.tl + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
+    for (int i = t; i < kNB * kNB; i += NT) tl[i] = Ts[i];
+}
+
+// Combine: the leaves' R factors stacked (leaf l's row r at stacked row
+// 32 l + r) factored by one block of kTsqrComb threads.  The final R goes
+// to the first leaf's top rows (upper triangle) and its diagonal to d.diag;
+// the stacked reflectors to d.vc (explicit, unit diagonal), T to d.tc.
+__global__ __launch_bounds__(kTsqrComb) void qr_tsqr_combine_kernel(const QrDesc* __restrict__ D, int64_t c0,
+                                                                     int lr) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t kk = d.k - c0;
+    const int nleaves = static_cast<int>((kk + lr - 1) / lr);
+    __shared__ double Ts[kNB * kNB];
+    __shared__ double sh[block_qr32_lds_doubles(kTsqrComb)];
+    const int t = threadIdx.x, l = t >> 5, r = t & 31;
+    const int64_t row = c0 + static_cast<int64_t>(l) * lr + r;
+    const bool valid = l < nleaves && r < min<int64_t>(lr, kk - static_cast<int64_t>(l) * lr);
+    double a[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) a[q] = (valid && q >= r && q < nbp) ? d.a[row + (c0 + q) * d.lda] : 0.0;
+    block_qr32<kTsqrComb>(a, nbp, Ts, sh);
+    if (t < nbp) {
+#pragma unroll
+        for (int q = 0; q < kNB; ++q)
+            if (q >= t && q < nbp) d.a[c0 + t + (c0 + q) * d.lda] = a[q];
+#pragma unroll
+        for (int q = 0; q < kNB; ++q)
+            if (q == t) d.diag[c0 + t] = a[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) d.vc[t * kNB + q] = t > q ? a[q] : (t == q ? 1.0 : 0.0);
+    for (int i = t; i < kNB * kNB; i += kTsqrComb) d.tc[i] = Ts[i];
+}
+
+// Trailing update with one level's block reflector: rows S of the trailing
+// columns j0 + jt0 .. j0 + jt0 + kCT - 1 become S - V T^T V^T S.
+// COMBINE = false: S = leaf blockIdx.x's rows, V from the panel columns
+// below the leaf's diagonal, T = d.tl[leaf].  COMBINE = true: S = the
+// stacked top kNB rows of every leaf, V = d.vc, T = d.tc.  One row per
+// thread (V and S rows kept in registers); W = V^T S accumulates over
+// 256-row chunks staged in LDS, then T^T W, all in a fixed order.
+constexpr int kCT = 16;
+constexpr int kChunk = 128;
+
+template <int NT, bool COMBINE>
+__global__ __launch_bounds__(NT) void qr_tsqr_update_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t j0 = c0 + nbp, nt = d.n - j0;
+    const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * kCT;
+    if (nt <= 0 || jt0 >= nt) return;
+    const int64_t kk = d.k - c0;
+    const int t = threadIdx.x;
+    int64_t row;
+    bool valid;
+    const double* tsrc;
+    if constexpr (COMBINE) {
+        const int l = t >> 5, r = t & 31;
+        const int nleaves = static_cast<int>((kk + lr - 1) / lr);
+        row = c0 + static_cast<int64_t>(l) * lr + r;
+        valid = l < nleaves && r < min<int64_t>(lr, kk - static_cast<int64_t>(l) * lr);
+        tsrc = d.tc;
+    } else {
+        const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * NT;
+        if (r0 >= d.k) return;
+        row = r0 + t;
+        valid = row < d.k;
+        tsrc = d.tl + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
+    }
+    __shared__ double sV[kChunk][kNB + 1];
+    __shared__ double sS[kChunk][kCT + 1];
+    __shared__ double sT[kNB * kNB];
+    __shared__ double sW[kNB][kCT + 1];
+    __shared__ double sW2[kNB][kCT + 1];
+    double v[kNB], srow[kCT];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+        double x = 0.0;
+        if (valid && q < nbp) {
+            if constexpr (COMBINE) x = d.vc[t * kNB + q];
+            else x = t > q ? d.a[row + (c0 + q) * d.lda] : (t == q ? 1.0 : 0.0);
+        }
+        v[q] = x;
+    }
+#pragma unroll
+    for (int c = 0; c < kCT; ++c) srow[c] = (valid && jt0 + c < nt) ? d.a[row + (j0 + jt0 + c) * d.lda] : 0.0;
+    for (int i = t; i < kNB * kNB; i += NT) sT[i] = tsrc[i];
+    constexpr int kPer = (kNB * kCT + NT - 1) / NT;  // W entries per thread
+    double wacc[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) wacc[u] = 0.0;
+    for (int ch = 0; ch < NT / kChunk; ++ch) {  // W = V^T S, chunk by chunk
+        if (t / kChunk == ch) {
+            const int lt = t - ch * kChunk;
+#pragma unroll
+            for (int q = 0; q < kNB; ++q) sV[lt][q] = v[q];
+#pragma unroll
+            for (int c = 0; c < kCT; ++c) sS[lt][c] = srow[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * NT;
+            if (e < kNB * kCT) {
+                const int q = e % kNB, c = e / kNB;
+                double acc = wacc[u];
+#pragma unroll 16
+                for (int i = 0; i < kChunk; ++i) acc += sV[i][q] * sS[i][c];
+                wacc[u] = acc;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * NT;
+        if (e < kNB * kCT) sW[e % kNB][e / kNB] = wacc[u];
+    }
+    __syncthreads();
+    for (int e = t; e < kNB * kCT; e += NT) {  // W2 = T^T W (T upper triangular)
+        const int q = e % kNB, c = e / kNB;
+        double acc = 0.0;
+#pragma unroll
+        for (int pp = 0; pp < kNB; ++pp) {
+            const double term = sT[q * kNB + pp] * sW[pp][c];
+            acc += pp <= q ? term : 0.0;
+        }
+        sW2[q][c] = acc;
+    }
+    __syncthreads();
+    if (!valid) return;
+#pragma unroll
+    for (int c = 0; c < kCT; ++c) {
+        if (jt0 + c >= nt) continue;
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < kNB; ++q) acc += v[q] * sW2[q][c];
+        d.a[row + (j0 + jt0 + c) * d.lda] = srow[c] - acc;
+    }
+}
+
+// Zero the panel kernel's ticket counters and column flags; status = 0.
+__global__ __launch_bounds__(kT) void qr_sync_reset_kernel(const QrDesc* __restrict__ D, int* __restrict__ status) {
+    const QrDesc& d = D[blockIdx.z];
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+    if (t < kSyncHead + d.n) d.sync[t] = 0;
+    if (blockIdx.z == 0 && t == 0) *status = 0;
 }
 
 // V (rows c0..k-1 used, ld = k): unit lower trapezoidal reflector columns.
@@ -173,9 +540,7 @@ __global__ __launch_bounds__(kTT) void qr_form_t_kernel(const QrDesc* __restrict
     __shared__ double z[kNB];
     {
         const int e = threadIdx.x;
-        double acc = 0.0;
-        for (int ch = 0; ch < nchunks; ++ch) acc += d.gp[static_cast<int64_t>(ch) * kNB * kNB + e];
-        g[e] = acc;
+        g[e] = strided_sum(d.gp + e, nchunks, kNB * kNB);
         t[e] = 0.0;
     }
     __syncthreads();
@@ -254,10 +619,7 @@ __global__ __launch_bounds__(kT) void qr_w2_kernel(const QrDesc* __restrict__ D,
     for (int e = threadIdx.x; e < kNB * 64; e += kT) {
         const int p = e % kNB, jj = e / kNB;
         const int64_t jt = jt0 + jj;
-        double acc = 0.0;
-        if (jt < nt && p < nbp)
-            for (int sp = 0; sp < nsplit; ++sp) acc += d.wp[(static_cast<int64_t>(sp) * nt + jt) * kNB + p];
-        sw[p][jj] = acc;
+        sw[p][jj] = (jt < nt && p < nbp) ? strided_sum(d.wp + jt * kNB + p, nsplit, nt * kNB) : 0.0;
     }
     __syncthreads();
     // W2(p, j) = sum_{q <= p} T(q, p) W(q, j)
@@ -379,7 +741,7 @@ size_t a256(size_t v) { return (v + 255) / 256 * 256; }
 // per-problem scratch: diag, tau, inv_v0 | V | Gram partials | T |
 // W partials (also the inverse's n x n copy of R) | bad flag
 struct Sizes {
-    size_t vec3, v, gp, t, wp, w2, total;
+    size_t vec3, v, gp, t, wp, w2, sync, tsqr, total;
 };
 
 Sizes sizes(int64_t k, int64_t n) {
@@ -392,7 +754,9 @@ Sizes sizes(int64_t k, int64_t n) {
     z.t = a256(static_cast<size_t>(kNB * kNB) * 8);
     z.wp = a256(static_cast<size_t>(std::max(qr_nsplit(kk) * nn * kNB, nn * nn)) * 8);
     z.w2 = a256(static_cast<size_t>(nn * kNB) * 8);
-    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + 256;
+    z.sync = a256(static_cast<size_t>(kSyncHead + nn) * 4);
+    z.tsqr = a256(static_cast<size_t>(kTsqrMaxLeaves * kNB * kNB + kTsqrComb * kNB + kNB * kNB) * 8);
+    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + z.sync + z.tsqr + 256;
     return z;
 }
 
@@ -420,20 +784,31 @@ QrDesc carve(char*& p, const QrProblem& q) {
     p += z.wp;
     d.w2 = reinterpret_cast<double*>(p);
     p += z.w2;
+    d.sync = reinterpret_cast<int*>(p);
+    p += z.sync;
+    d.tl = reinterpret_cast<double*>(p);
+    d.vc = d.tl + kTsqrMaxLeaves * kNB * kNB;
+    d.tc = d.vc + kTsqrComb * kNB;
+    p += z.tsqr;
     d.bad = reinterpret_cast<int*>(p);
     p += 256;
     return d;
 }
 
+size_t status_bytes(int nprob) { return a256(static_cast<size_t>(std::max(nprob, 1) + 1) * 4); }
+
 size_t batch_bytes(int nprob, const QrProblem* probs) {
-    size_t b = a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc));
+    size_t b = a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)) + status_bytes(nprob);
     for (int i = 0; i < nprob; ++i) b += sizes(probs[i].k, probs[i].n).total;
     return b;
 }
 
-// Descriptor table at the start of scratch, then the problems' regions.
+// Descriptor table at the start of scratch, then the status words
+// ([0] panel wait failure, [1 + i] problem i's first zero diagonal), then
+// the problems' regions.
 struct Batch {
     QrDesc* dev = nullptr;
+    int* status = nullptr;
     int nprob = 0;
     int64_t kmax = 0, nmax = 0;
 };
@@ -446,7 +821,8 @@ Batch setup(int nprob, const QrProblem* probs, void* scratch, size_t scratch_byt
     B.nprob = nprob;
     char* base = static_cast<char*>(scratch);
     B.dev = reinterpret_cast<QrDesc*>(base);
-    char* p = base + a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc));
+    B.status = reinterpret_cast<int*>(base + a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)));
+    char* p = base + a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)) + status_bytes(nprob);
     for (int i = 0; i < nprob; ++i) {
         const QrDesc d = carve(p, probs[i]);
         qr_set_desc_kernel<<<1, 1, 0, s>>>(B.dev + i, d);  // by value: no host buffer to keep alive
@@ -454,18 +830,38 @@ Batch setup(int nprob, const QrProblem* probs, void* scratch, size_t scratch_byt
         B.nmax = std::max(B.nmax, probs[i].n);
     }
     TOMOS_LAUNCH_CHECK("qr_set_desc_kernel");
+    qr_status_reset_kernel<<<1, 1, 0, s>>>(B.status);
+    TOMOS_LAUNCH_CHECK("qr_status_reset_kernel");
     return B;
 }
 
 void factor(const Batch& B, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
+    qr_sync_reset_kernel<<<dim3(static_cast<unsigned>(ceil_div(kSyncHead + B.nmax, kT)), 1, P), kT, 0, s>>>(
+        B.dev, B.status);
+    TOMOS_LAUNCH_CHECK("qr_sync_reset_kernel");
+    // TSQR panels when every problem has at most 16 leaves of 256 or 512 rows
+    const bool tsqr = B.kmax <= kTsqrMaxK;
+    const int lr = B.kmax <= kTsqrMaxLeaves * 256 ? 256 : 512;
     for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
-        const int64_t c_end = std::min<int64_t>(c0 + kNB, B.nmax);
-        for (int64_t c = c0; c < c_end; ++c) {
-            const unsigned blocks = static_cast<unsigned>(std::max<int64_t>(1, c_end - c - 1));
-            qr_panel_step_kernel<<<dim3(blocks, 1, P), kPT, 0, s>>>(B.dev, c0, c);
+        if (tsqr) {
+            const unsigned nleaves = static_cast<unsigned>(ceil_div(B.kmax - c0, lr));
+            if (lr == 256) qr_tsqr_leaf_kernel<256><<<dim3(nleaves, 1, P), 256, 0, s>>>(B.dev, c0);
+            else qr_tsqr_leaf_kernel<512><<<dim3(nleaves, 1, P), 512, 0, s>>>(B.dev, c0);
+            qr_tsqr_combine_kernel<<<dim3(1, 1, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
+            TOMOS_LAUNCH_CHECK("qr_tsqr panel");
+            if (c0 + kNB >= B.nmax) break;
+            const unsigned ctiles = static_cast<unsigned>(ceil_div(B.nmax - c0 - kNB, kCT));
+            if (lr == 256)
+                qr_tsqr_update_kernel<256, false><<<dim3(nleaves, ctiles, P), 256, 0, s>>>(B.dev, c0, lr);
+            else
+                qr_tsqr_update_kernel<512, false><<<dim3(nleaves, ctiles, P), 512, 0, s>>>(B.dev, c0, lr);
+            qr_tsqr_update_kernel<kTsqrComb, true><<<dim3(1, ctiles, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
+            TOMOS_LAUNCH_CHECK("qr_tsqr update");
+            continue;
         }
-        TOMOS_LAUNCH_CHECK("qr_panel_step_kernel");
+        qr_panel_kernel<<<dim3(kNB, 1, P), kPT, 0, s>>>(B.dev, c0, c0 / kNB, B.status);
+        TOMOS_LAUNCH_CHECK("qr_panel_kernel");
         if (c0 + kNB >= B.nmax) break;  // no problem has columns right of this panel
         const int64_t rows = B.kmax - c0;
         const int64_t nt = B.nmax - c0 - kNB;
@@ -483,6 +879,26 @@ void factor(const Batch& B, hipStream_t s) {
     TOMOS_LAUNCH_CHECK("extract_r_kernel");
 }
 
+// status[1 + i] = problem i's first zero diagonal (when with_bad), else
+// INT_MAX; status[0] is the panel kernel's wait-failure flag.
+__global__ void gather_status_kernel(const QrDesc* __restrict__ D, int nprob, bool with_bad, int* __restrict__ status) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < nprob) status[1 + i] = (with_bad && D[i].n > 0) ? *D[i].bad : 0x7FFFFFFF;
+}
+
+// The status words on the host (one copy, one synchronisation); throws if
+// a panel wait failed.
+std::vector<int> read_status(const Batch& B, bool with_bad, hipStream_t s) {
+    gather_status_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob, with_bad,
+                                                                                    B.status);
+    TOMOS_LAUNCH_CHECK("gather_status_kernel");
+    std::vector<int> h(static_cast<size_t>(B.nprob) + 1, 0);
+    TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), B.status, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[0] != 0) throw Error(kErrInternal, "geqrf: panel kernel wait limit reached");
+    return h;
+}
+
 // R -> R^-1 in place for every problem; throws on a zero diagonal.
 void invert(const Batch& B, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
@@ -490,14 +906,9 @@ void invert(const Batch& B, hipStream_t s) {
     reset_bad_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob);
     diag_zero_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
     TOMOS_LAUNCH_CHECK("diag_zero_kernel");
-    std::vector<QrDesc> h(B.nprob);
-    TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), B.dev, sizeof(QrDesc) * B.nprob, hipMemcpyDeviceToHost, s));
-    TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    const std::vector<int> st = read_status(B, true, s);
     for (int i = 0; i < B.nprob; ++i) {
-        if (h[i].n == 0) continue;
-        int bad = 0x7FFFFFFF;
-        TOMOS_HIP_CHECK(hipMemcpyAsync(&bad, h[i].bad, sizeof(int), hipMemcpyDeviceToHost, s));
-        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+        const int bad = st[1 + i];
         if (bad != 0x7FFFFFFF)
             throw Error(kErrSingular, "trtri: R is singular (zero diagonal at " + std::to_string(bad) +
                                           (B.nprob > 1 ? ", problem " + std::to_string(i) : std::string()) + ")");
@@ -538,6 +949,7 @@ void geqrf_r_batched(int nprob, const QrProblem* probs, bool invert_r, void* scr
     const Batch B = setup(nprob, probs, scratch, scratch_bytes, s);
     factor(B, s);
     if (invert_r) invert(B, s);
+    else read_status(B, false, s);
     TOMOS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
