@@ -102,10 +102,33 @@ constexpr int log2_small() {
     return G >= 16 ? 4 : (G >= 8 ? 3 : (G >= 4 ? 2 : (G >= 2 ? 1 : 0)));
 }
 
+// Block barrier for the LDS exchanges between the waves (red, ush) that
+// leaves the next tile's DMA in flight.  A workgroup fence would also wait
+// for the DMA (its LDS writes count as outstanding vector-memory work), so
+// instead: this wave's own DS operations are waited for (lgkmcnt(0)), the
+// compiler is kept from moving memory operations across the barrier, and
+// the hardware barrier synchronises the waves.  The DMA writes only the
+// slots of the next tile, which nothing reads before its own wait.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+template <bool STAGE>
+__device__ __forceinline__ void tile_barrier() {
+    if constexpr (STAGE) lds_barrier();
+    else __syncthreads();
+}
+
 // One block's share of the single-pass product: tiles [tile0, tile1) of an
 // m x n matrix; writes the block's sum of u_raw^2 to *sumsq_out and its
 // A^T u_raw partial (n floats) to tpart_out.  NTL: non-temporal loads of A.
-template <int FMT, int G, int ROUNDS, int NT, bool NTL>
+// STAGE: the tile goes through LDS by DMA (global_load_lds_dwordx4, each
+// lane's 16 bytes at its own slot: base + 16 * lane), and the next tile's
+// loads are issued before this tile's arithmetic and barriers, so HBM
+// requests stay in flight throughout (one tile buffer: NW x ROUNDS KB).
+template <int FMT, int G, int ROUNDS, int NT, bool NTL, bool STAGE = false>
 __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __restrict__ a,
                                            int64_t lda, const void* __restrict__ p, bool p64,
                                            void* __restrict__ u, bool u64, double cu, int64_t tile0,
@@ -143,7 +166,23 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 
     __shared__ float red[NW][TR];
     __shared__ float ush[TR];
+    __shared__ uint4 stage[STAGE ? NW : 1][STAGE ? ROUNDS : 1][64];
     double sq = 0.0;
+    // LDS-DMA of tile `tile` into this wave's slots; lanes outside A load
+    // the first vector instead (in bounds) and are masked when read back
+    auto issue = [&](int64_t tile) {
+        const int64_t row = tile * TR + g * RPL;
+        const bool ok_row = tile < tile1 && row < m;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int64_t j = static_cast<int64_t>(r) * CS + c;
+            const char* src = (ok_row && j < n) ? abase + (row + j * lda) * ESZ : abase;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(src), &stage[wave][r][0], 16, 0,
+                                             NTL ? 2 : 0);
+        }
+    };
+    if constexpr (STAGE) issue(tile0);
+    double upre = 0.0;  // STAGE: this tile's u_raw row (threads < TR)
 
     for (int64_t tile = tile0; tile < tile1; ++tile) {
         const int64_t row0 = tile * TR;
@@ -151,13 +190,37 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         const bool in = my_row < m;
         const int nvalid = in ? static_cast<int>(min<int64_t>(RPL, m - my_row)) : 0;
         uint4 t[ROUNDS];
+        if constexpr (STAGE) {
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA has landed
 #pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) {
-            const int64_t j = static_cast<int64_t>(r) * CS + c;
-            if (in && j < n)
-                t[r] = load_a16<NTL>(abase + (my_row + j * lda) * ESZ);
-            else
-                t[r] = make_uint4(0u, 0u, 0u, 0u);
+            for (int r = 0; r < ROUNDS; ++r) t[r] = stage[wave][r][lane];
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): slots read before they are refilled
+            // this tile's u_raw rows, loaded before the DMA so that waiting
+            // for them does not wait for the next tile
+            if (tid < TR && row0 + tid < m && cu != 0.0)
+                upre = u64 ? static_cast<double*>(u)[row0 + tid]
+                           : static_cast<double>(static_cast<float*>(u)[row0 + tid]);
+            issue(tile + 1);
+            // lanes outside A hold the stand-in vector: clear them (bit masks,
+            // no branches)
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * CS + c;
+                const uint32_t keep = (in && j < n) ? 0xFFFFFFFFu : 0u;
+                t[r].x &= keep;
+                t[r].y &= keep;
+                t[r].z &= keep;
+                t[r].w &= keep;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * CS + c;
+                if (in && j < n)
+                    t[r] = load_a16<NTL>(abase + (my_row + j * lda) * ESZ);
+                else
+                    t[r] = make_uint4(0u, 0u, 0u, 0u);
+            }
         }
         // phase 1: row sums over this lane's columns
         float acc[RPL];
@@ -179,7 +242,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
             constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
             if ((lane & kDupMask) == 0) red[wave][g * RPL + k] = acc[0];
         }
-        __syncthreads();
+        tile_barrier<STAGE>();
         if (tid < TR) {
             const int64_t row = row0 + tid;
             float un = 0.0f;
@@ -188,9 +251,11 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 #pragma unroll
                 for (int w = 1; w < NW; ++w) s += red[w][tid];
                 double r = static_cast<double>(s);
-                if (cu != 0.0)  // u is not read for a zero coefficient
-                    r += cu * (u64 ? static_cast<double*>(u)[row]
-                                   : static_cast<double>(static_cast<float*>(u)[row]));
+                if (cu != 0.0) {  // u is not read for a zero coefficient
+                    if constexpr (STAGE) r += cu * upre;
+                    else r += cu * (u64 ? static_cast<double*>(u)[row]
+                                        : static_cast<double>(static_cast<float*>(u)[row]));
+                }
                 if (u64) {
                     static_cast<double*>(u)[row] = r;
                     un = static_cast<float>(r);
@@ -203,7 +268,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
             }
             ush[tid] = un;
         }
-        __syncthreads();
+        tile_barrier<STAGE>();
         // phase 2: column sums over this lane's rows with u_raw
         float ur[RPL];
 #pragma unroll
@@ -222,7 +287,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         LaneTR<ROUNDS, G / 2, 1>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
-        __syncthreads();  // red/ush reuse in the next tile
+        tile_barrier<STAGE>();  // red/ush reuse in the next tile
     }
     // column ids of tacc: r = base + i, the halving stage on lane bit b kept
     // the upper half for lanes with bit b set
@@ -412,7 +477,7 @@ __global__ __launch_bounds__(1024) void grouped_fused2_kernel(const GroupProb* _
                         P.sumsq + it.slot, P.tpart + static_cast<int64_t>(it.slot) * P.n);
 }
 
-template <int FMT, int G, int ROUNDS, int NT, bool NTL>
+template <int FMT, int G, int ROUNDS, int NT, bool NTL, bool STAGE>
 __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, int64_t lda, const void* __restrict__ p,
     bool p64, void* __restrict__ u, bool u64, const double* __restrict__ u_coef, int64_t tiles_per_block,
@@ -422,7 +487,7 @@ __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
     const int64_t ntiles = (m + TR - 1) / TR;
     const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
-    fused_body<FMT, G, ROUNDS, NT, NTL>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
+    fused_body<FMT, G, ROUNDS, NT, NTL, STAGE>(m, n, a, lda, p, p64, u, u64, *u_coef, tile0, tile1, sumsq + blockIdx.x,
                                tpart + static_cast<int64_t>(blockIdx.x) * n);
 }
 
@@ -512,8 +577,10 @@ template <int FMT, bool NTL>
 const void* fused_kernel_ptr(int g, int nt, int rounds) {
     if (g == 32 && nt == 1024)
         return rounds > 0 ? reinterpret_cast<const void*>(lsqr_fused2_kernel<FMT, NTL>) : nullptr;
-#define FUSED_CASE(G, NT, R) \
-    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT, NTL>);
+// 8 row groups (128-byte column runs) stage tiles through LDS by DMA
+#define FUSED_CASE(G, NT, R)                                                                      \
+    if (g == G && nt == NT && rounds == R)                                                        \
+        return reinterpret_cast<const void*>(lsqr_fused_kernel<FMT, G, R, NT, NTL, G == 8>);
     TOMOS_FUSED_SHAPES(FUSED_CASE)
 #undef FUSED_CASE
     return nullptr;
