@@ -73,10 +73,16 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+# Untimed launches before each kernel timing: after an idle gap the device
+# clocks go through a transient of ~15 ms (a 4 GB stream measured 0.67 ms,
+# then 0.82, then 0.68 per launch); the timings are of the steady state.
+WARM_LAUNCHES = 20
+
+
 def time_call(fn, reps, ev0, ev1):
-    """Average ms of fn over reps launches on the NULL stream (after 2 warm calls)."""
+    """Average ms of fn over reps launches on the NULL stream (after warm-up launches)."""
     from tomos_amd import _capi as C
-    for _ in range(2):
+    for _ in range(WARM_LAUNCHES):
         fn()
     C.call("tomos_event_record", ev0, None)
     for _ in range(reps):
@@ -98,7 +104,7 @@ def time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1):
     sc = DeviceBuffer(sz)
     call = lambda: C.call("tomos_matvec_pair", C.FP32_BF16, m, n, wh.ptr, m, pd.ptr, -0.5, ud.ptr,
                           td.ptr, nd.ptr, sc.ptr, sz, None)
-    for _ in range(2):
+    for _ in range(WARM_LAUNCHES):
         call()
     C.call("tomos_event_record", ev0, None)
     for _ in range(args.gemv_reps):
@@ -381,7 +387,7 @@ def main():
 
     def time_gemv(trans, a_type, tier, a_ptr, wr_ptr):
         xin, yout = (uv, zv) if trans else (xv, y)
-        for _ in range(2):
+        for _ in range(WARM_LAUNCHES):
             C.call("tomos_gemv", a_type, tier, trans, 1.0, m, n, a_ptr, wr_ptr, m, C.F32, xin.ptr,
                    0.0, C.F32, yout.ptr, gs.ptr, gsc, None)
         C.call("tomos_event_record", ev0, None)

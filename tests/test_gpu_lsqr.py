@@ -112,8 +112,10 @@ def test_precond_matches_oracle_with_same_rinv(capi, orc):
 
 
 @pytest.mark.parametrize("a_type,variant", [(C.FP32_BF16, C.PACK_FP32_BF16), (C.FP32_16, C.PACK_FP32_16)])
-def test_packed_mixed_precision_restarts_reach_1e6(capi, a_type, variant):
-    """Hi-only matvecs per iteration + exact (wh+wr) residual at restarts."""
+@pytest.mark.parametrize("fused", [0, 1])
+def test_packed_mixed_precision_restarts_reach_1e6(capi, a_type, variant, fused):
+    """Hi-only matvecs per iteration + exact (wh+wr) residual at restarts
+    (fused=1 with FP32_BF16: the one-pass restart kernel, FP64 row sums)."""
     m, n = 20000, 100
     A, xt, b = c1_problem(m=m, n=n, kappa_exp=3.0, noise=1e-7, seed=4)
     wh, wr = H.pack(variant, A)
@@ -122,7 +124,7 @@ def test_packed_mixed_precision_restarts_reach_1e6(capi, a_type, variant):
     x, it, hist = H.lsqr((wh, wr), b, a_type=a_type, mv_tier=C.TIER_DEFAULT,
                          restart_tier=C.TIER_EXACT, value_type=C.F32, refine_type=C.F64,
                          rinv=rinv, precond_type=C.F64, tol=1e-6, max_iters=12, restarts=20,
-                         m=m, n=n)
+                         m=m, n=n, fused=fused)
     A64 = A.astype(np.float64)
     rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
     assert rr < 1.5e-6, (rr, it)
@@ -179,3 +181,20 @@ def test_fused_single_pass_matches_two_pass(capi, a_type, variant, n):
     r1 = np.linalg.norm(A64 @ x1 - b) / np.linalg.norm(b)
     r0 = np.linalg.norm(A64 @ x0 - b) / np.linalg.norm(b)
     assert r1 < 2 * max(r0, 1e-5), (r0, r1)
+
+
+@pytest.mark.parametrize("n", [300, 1024, 1100])
+def test_one_pass_restart_reaches_tolerance(capi, n):
+    """FP32_BF16, fused iterations, exact restarts: n <= 1024 takes the one-pass
+    restart kernel, n = 1100 the two-kernel restart; both reach relres 1e-6."""
+    m = 30001
+    A, xt, b = c1_problem(m=m, n=n, kappa_exp=3.0, noise=0.0, seed=n + 7)
+    wh, wr = H.pack(C.PACK_FP32_BF16, A, ld=30008)
+    rinv, _ = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, 30008, 4 * n)
+    x, it, hist = H.lsqr((wh, wr), b, a_type=C.FP32_BF16, mv_tier=C.TIER_DEFAULT, restart_tier=C.TIER_EXACT,
+                         value_type=C.F32, refine_type=C.F64, rinv=rinv, precond_type=C.F64, tol=1e-6,
+                         max_iters=8, restarts=30, m=m, n=n, lda=30008, fused=1)
+    A64 = A.astype(np.float64)
+    rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
+    assert rr < 1.5e-6, (rr, it)
+    assert it <= 60, it

@@ -383,6 +383,8 @@ struct Solver {
     double* red() const { return reinterpret_cast<double*>(sc + L.red); }
     void* Z() const { return sc + L.z; }
     bool use_fused = false;
+    bool use_fused_restart = false;  // initialize() in one pass (exact residual + A_hi^T u)
+    FusedPlan rplan;
     void* scratch() const { return sc + L.scratch; }
 
     void norm(const void* vec, bool f64, int64_t len, double* out, bool guarded_by_state = true) {
@@ -417,6 +419,16 @@ struct Solver {
         LsqrState* S = state();
         // u_raw = b - A x at the restart tier; beta = |u_raw| (fused partials)
         k_copy<<<grid_for(m), kT, 0, s>>>(S, b, r64, U(), v64, m);
+        if (use_fused_restart) {
+            float* tpart = reinterpret_cast<float*>(sc + L.tpart);
+            double* tsum = reinterpret_cast<double*>(sc + L.tsum);
+            lsqr_restart_launch(o.a_type, m, n, a, wr, lda, static_cast<const double*>(x), U(), v64, red(), tpart,
+                                tsum, rplan, &S->done, s);
+            k_beta_finish<<<1, kT, 0, s>>>(S, red(), rplan.nblocks);
+            k_temp_finish<<<grid_for(n), kT, 0, s>>>(S, n, rplan.splits, tsum, Tm(), v64);
+            tail(true, true);
+            return;
+        }
         Coef cr = coef(-1.0, 1.0);
         cr.sumsq = red();
         int nb = 0;
@@ -575,6 +587,9 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.r64 = o.refine_type == TOMOS_F64;
     sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
     sv.use_fused = sv.L.fused_possible && lsqr_fused_supported(o.a_type, o.mv_tier, m, n, lda, a);
+    sv.use_fused_restart = sv.use_fused && sv.r64 &&
+                           lsqr_restart_supported(o.a_type, o.restart_tier, m, n, lda, a, wr);
+    if (sv.use_fused_restart) sv.rplan = lsqr_restart_plan(o.a_type, m, n);
     sv.run(iterations, history, history_cap, history_len);
 }
 

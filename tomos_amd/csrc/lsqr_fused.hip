@@ -72,29 +72,29 @@ __device__ __forceinline__ void decode_vec(const uint4& t, float (&out)[rows_per
 // at each stage a lane keeps the half of its values selected by its bit and
 // adds the partner's copy of that half.  Stages run while L > 1; any
 // remaining offsets down to LO reduce the single value.
-template <int L, int OFF, int LO>
+template <int L, int OFF, int LO, class T = float>
 struct LaneTR {
-    static __device__ __forceinline__ void run(float* v, int lane) {
+    static __device__ __forceinline__ void run(T* v, int lane) {
         if constexpr (OFF < LO) {
             return;
         } else if constexpr (L == 1) {
             v[0] += __shfl_xor(v[0], OFF);
-            LaneTR<1, OFF / 2, LO>::run(v, lane);
+            LaneTR<1, OFF / 2, LO, T>::run(v, lane);
         } else {
             const bool upper = (lane & OFF) != 0;
 #pragma unroll
             for (int t = 0; t < L / 2; ++t) {
-                const float send = upper ? v[t] : v[t + L / 2];
-                const float keep = upper ? v[t + L / 2] : v[t];
+                const T send = upper ? v[t] : v[t + L / 2];
+                const T keep = upper ? v[t + L / 2] : v[t];
                 v[t] = keep + __shfl_xor(send, OFF);
             }
-            LaneTR<L / 2, OFF / 2, LO>::run(v, lane);
+            LaneTR<L / 2, OFF / 2, LO, T>::run(v, lane);
         }
     }
 };
-template <int L, int LO>
-struct LaneTR<L, 0, LO> {
-    static __device__ __forceinline__ void run(float*, int) {}
+template <int L, int LO, class T>
+struct LaneTR<L, 0, LO, T> {
+    static __device__ __forceinline__ void run(T*, int) {}
 };
 
 template <int G>
@@ -524,6 +524,160 @@ __global__ __launch_bounds__(NT) void grouped_fused_kernel(const GroupProb* __re
     grouped_item<FMT, G, G, ROUNDS, NT>(P, it);
 }
 
+// Exact values of the RPL rows of one 16-byte vector: F32 as stored, packed
+// formats from the hi word h and the lo word l (wh + wr, exact FP32).
+template <int FMT>
+__device__ __forceinline__ void decode_exact_vec(const uint4& h, const uint4& l,
+                                                 float (&out)[rows_per_lane<FMT>()]) {
+    if constexpr (FMT == FMT_F32) {
+        decode_vec<FMT>(h, out);
+    } else {
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (FMT == FMT_PBF) {
+                out[2 * k] = __uint_as_float((hw[k] << 16) | (lw[k] & 0xFFFFu));
+                out[2 * k + 1] = __uint_as_float((hw[k] & 0xFFFF0000u) | (lw[k] >> 16));
+            } else {
+                out[2 * k] = codec::fp32_16_decode_exact(hw[k] & 0xFFFFu, lw[k] & 0xFFFFu);
+                out[2 * k + 1] = codec::fp32_16_decode_exact(hw[k] >> 16, lw[k] >> 16);
+            }
+        }
+    }
+}
+
+// LSQR restart in one pass over A (lsqr.cpp:441-469 initialize + the first
+// A^T u): u_raw = u - A x with the exact values (wh + wr) and FP64 row sums
+// (u holds b on entry; x in FP64), |u_raw|^2 per block, and the block
+// partials of A_hi^T u_raw from the hi words of the same tile (the
+// per-iteration tier).  Tile: 8 row groups x 512 threads x 16 column rounds
+// (n <= 1024), both planes in registers, loaded directly.
+constexpr int kRsG = 8, kRsNT = 512, kRsR = 16;
+constexpr int64_t kRestartMaxN = static_cast<int64_t>(kRsNT / kRsG) * kRsR;
+
+template <int FMT, bool NTL>
+__global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
+    const double* __restrict__ x, void* __restrict__ u, bool u64, int64_t tiles_per_block,
+    double* __restrict__ sumsq, float* __restrict__ tpart, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    constexpr int G = kRsG, NT = kRsNT, R = kRsR;
+    constexpr int RPL = rows_per_lane<FMT>();
+    constexpr int LOG_RPL = log2_small<RPL>();
+    constexpr int LOG_G = log2_small<G>();
+    constexpr int TR = G * RPL, CS = NT / G, NW = NT / 64, TOUT = R / G;
+    constexpr int ESZ = FMT == FMT_F32 ? 4 : 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = tid & (G - 1), c = tid >> LOG_G;
+    const char* abase = static_cast<const char*>(a);
+    const int64_t ntiles = (m + TR - 1) / TR;
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * tiles_per_block;
+    const int64_t tile1 = min(ntiles, tile0 + tiles_per_block);
+    __shared__ double xs[kRestartMaxN];  // x in LDS (registers go to the two planes)
+    for (int j = tid; j < kRestartMaxN; j += NT) xs[j] = j < n ? x[j] : 0.0;
+    __syncthreads();
+    float tacc[TOUT];
+#pragma unroll
+    for (int i = 0; i < TOUT; ++i) tacc[i] = 0.0f;
+    __shared__ double red[NW][TR];
+    __shared__ float ush[TR];
+    double sq = 0.0;
+    for (int64_t tile = tile0; tile < tile1; ++tile) {
+        const int64_t row0 = tile * TR;
+        const int64_t my_row = row0 + g * RPL;
+        const bool in = my_row < m;
+        const int nvalid = in ? static_cast<int>(min<int64_t>(RPL, m - my_row)) : 0;
+        uint4 th[R], tl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t j = static_cast<int64_t>(r) * CS + c;
+            const bool ok = in && j < n;
+            th[r] = ok ? load_a16<NTL>(abase + (my_row + j * lda) * ESZ) : make_uint4(0u, 0u, 0u, 0u);
+            if constexpr (FMT != FMT_F32)
+                tl[r] = ok ? load_a16<NTL>(wr + my_row + j * lda) : make_uint4(0u, 0u, 0u, 0u);
+            else
+                tl[r] = th[r];
+        }
+        // (A x) rows in FP64
+        double acc[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float av[RPL];
+            decode_exact_vec<FMT>(th[r], tl[r], av);
+            const double xr = xs[r * CS + c];
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xr, acc[k]);
+        }
+        LaneTR<RPL, 32, G, double>::run(acc, lane);
+        {
+            const int k = (lane >> (6 - LOG_RPL)) & (RPL - 1);
+            constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
+            if ((lane & kDupMask) == 0) red[wave][g * RPL + k] = acc[0];
+        }
+        __syncthreads();
+        if (tid < TR) {
+            const int64_t row = row0 + tid;
+            float un = 0.0f;
+            if (row < m) {
+                double sx = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) sx += red[w][tid];
+                if (u64) {
+                    const double r = static_cast<double*>(u)[row] - sx;
+                    static_cast<double*>(u)[row] = r;
+                    un = static_cast<float>(r);
+                    sq += r * r;
+                } else {
+                    un = static_cast<float>(static_cast<double>(static_cast<float*>(u)[row]) - sx);
+                    static_cast<float*>(u)[row] = un;
+                    sq += static_cast<double>(un) * un;
+                }
+            }
+            ush[tid] = un;
+        }
+        __syncthreads();
+        // A_hi^T u_raw over this lane's rows
+        float ur[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
+        float cs[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float av[RPL];
+            decode_vec<FMT>(th[r], av);
+            float sacc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) sacc = fmaf(k < nvalid ? av[k] : 0.0f, ur[k], sacc);
+            cs[r] = sacc;
+        }
+        LaneTR<R, G / 2, 1>::run(cs, lane);
+#pragma unroll
+        for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
+        __syncthreads();
+    }
+    int base = 0;
+#pragma unroll
+    for (int bb = 0; bb < LOG_G; ++bb) base += ((g >> bb) & 1) * (R >> (LOG_G - bb));
+    float* tp = tpart + static_cast<int64_t>(blockIdx.x) * n;
+#pragma unroll
+    for (int i = 0; i < TOUT; ++i) {
+        const int64_t j = static_cast<int64_t>(base + i) * CS + c;
+        if (j < n) tp[j] = tacc[i];
+    }
+    for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o);
+    __shared__ double sqw[NW];
+    if (lane == 0) sqw[wave] = sq;
+    __syncthreads();
+    if (tid == 0) {
+        double tt = sqw[0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) tt += sqw[w];
+        sumsq[blockIdx.x] = tt;
+    }
+}
+
 // level-1 reduction of the block partials: out[s][j] = sum over the blocks
 // of split s (fixed order)
 __global__ __launch_bounds__(kThreads) void fused_tsum_kernel(int64_t n, int nblocks, int splits,
@@ -738,6 +892,64 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
                     const_cast<double**>(&u_coef), &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};
     TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(plan.threads), args, 0, s));
     TOMOS_LAUNCH_CHECK("lsqr_fused_kernel");
+    fused_tsum_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, kThreads)), static_cast<unsigned>(plan.splits)),
+                        kThreads, 0, s>>>(n, plan.nblocks, plan.splits, tpart, tsum, skip);
+    TOMOS_LAUNCH_CHECK("fused_tsum_kernel");
+}
+
+bool lsqr_restart_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a,
+                            const uint16_t* wr) {
+    if (tier != TOMOS_TIER_EXACT || m < 1 || n < 1 || n > kRestartMaxN) return false;
+    // FP32_16's exact decode does not fit the register budget next to both
+    // planes: its restarts keep the two-kernel path
+    if (a_type != TOMOS_F32 && a_type != TOMOS_FP32_BF16) return false;
+    const int vec = a_type == TOMOS_F32 ? 4 : 8;
+    if (lda % vec != 0 || reinterpret_cast<uintptr_t>(a) % 16 != 0) return false;
+    return a_type == TOMOS_F32 || (wr != nullptr && reinterpret_cast<uintptr_t>(wr) % 16 == 0);
+}
+
+namespace {
+const void* restart_kernel_for(int a_type, bool ntl) {
+    if (a_type == TOMOS_FP32_BF16)
+        return ntl ? reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, true>)
+                   : reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, false>);
+    return ntl ? reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_F32, true>)
+               : reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_F32, false>);
+}
+}  // namespace
+
+FusedPlan lsqr_restart_plan(int a_type, int64_t m, int64_t n) {
+    FusedPlan p;
+    p.g = kRsG;
+    p.threads = kRsNT;
+    p.rounds = kRsR;
+    p.stream = stream_loads(std::max<int64_t>(m, 1) * n * 4);  // hi + lo, or F32
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, restart_kernel_for(a_type, p.stream), kRsNT, 0) !=
+        hipSuccess)
+        occ = 0;
+    const int64_t tr = static_cast<int64_t>(kRsG) * (a_type == TOMOS_F32 ? 4 : 8);
+    const int64_t ntiles = ceil_div(std::max<int64_t>(m, 1), tr);
+    const int64_t want = std::min<int64_t>(static_cast<int64_t>(std::max(occ, 1)) * cu_count(),
+                                           lsqr_fused_max_blocks(m));
+    p.tiles_per_block = std::max<int64_t>(1, ceil_div(ntiles, want));
+    p.nblocks = static_cast<int>(ceil_div(ntiles, p.tiles_per_block));
+    p.splits = std::min(kMaxSplits, std::max(1, p.nblocks / 16));
+    p.tpart_bytes = static_cast<size_t>(p.nblocks) * n * 4;
+    p.tsum_bytes = static_cast<size_t>(p.splits) * n * 8;
+    return p;
+}
+
+void lsqr_restart_launch(int a_type, int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
+                         const double* x, void* u, bool u64, double* sumsq, float* tpart, double* tsum,
+                         const FusedPlan& plan, const int* skip, hipStream_t s) {
+    require(n <= kRestartMaxN && plan.nblocks <= lsqr_fused_max_blocks(m), "lsqr_restart: plan does not fit");
+    const void* k = restart_kernel_for(a_type, plan.stream);
+    int64_t tpb = plan.tiles_per_block;
+    void* args[] = {&m, &n, const_cast<void**>(&a), const_cast<uint16_t**>(&wr), &lda, const_cast<double**>(&x),
+                    &u, &u64, &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};
+    TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(kRsNT), args, 0, s));
+    TOMOS_LAUNCH_CHECK("lsqr_restart_kernel");
     fused_tsum_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, kThreads)), static_cast<unsigned>(plan.splits)),
                         kThreads, 0, s>>>(n, plan.nblocks, plan.splits, tpart, tsum, skip);
     TOMOS_LAUNCH_CHECK("fused_tsum_kernel");
