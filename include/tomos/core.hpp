@@ -171,6 +171,12 @@ struct Workspace {
     // Iterations between host polls of the device stop flag.
     int check_every_ = 8;
 
+    // Extension (no reference counterpart): run each iteration's A p and
+    // A^T u from one read of A (single-pass kernel) and restarts in one
+    // pass where supported; the recurrence is the same up to FP32
+    // summation order.  Off by default.
+    bool single_pass_ = false;
+
     Workspace(Device device, std::size_t nbytes);
     ~Workspace();
     Workspace(Workspace&& o) noexcept;

@@ -303,7 +303,7 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
     o.max_iters = max_iters;
     o.restarts = restarts;
     o.check_every = std::max(1, ws.check_every_);
-    o.fused = 0;
+    o.fused = ws.single_pass_ ? 1 : 0;
     const size_t arena = tomos_lsqr_arena_size(&o, m, n);
     const size_t scratch = tomos_lsqr_scratch_size(&o, m, n);
     if (arena == 0 || scratch == 0) check(TOMOS_ERR_INVALID_ARGUMENT, "lsqr_solve");

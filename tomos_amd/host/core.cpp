@@ -157,7 +157,8 @@ Workspace::Workspace(Workspace&& o) noexcept
       mv_compute_type_set_(o.mv_compute_type_set_),
       mv_transpose_scratch_(o.mv_transpose_scratch_),
       mv_transpose_scratch_capacity_(o.mv_transpose_scratch_capacity_),
-      check_every_(o.check_every_) {
+      check_every_(o.check_every_),
+      single_pass_(o.single_pass_) {
     o.data_ = nullptr;
     o.mv_transpose_scratch_ = nullptr;
     o.mv_transpose_scratch_capacity_ = 0;
