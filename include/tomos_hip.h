@@ -164,9 +164,12 @@ int tomos_decode_wh(int decoder, int64_t count, const uint16_t* wh, uint16_t* ou
  * (:265-285), mv_accurate_impl (:287-368), the FP32_BF16 kernels
  * (fp32_16_cuda_kernels.cu:28785, :29636, :7587) and mv_fp32_tiled
  * (mv_cuda_kernels.cu:863-990).
- *   a_type: TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16;
- *   a: the F32 or F64 matrix, or the wh plane for packed types; wr: the wr
- *      plane (used by the EXACT tier only; may be NULL otherwise);
+ *   a_type: TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16 |
+ *           TOMOS_F16 | TOMOS_BF16 (native 16-bit storage, the reference's
+ *           mv_fp16_native_mixed, mv_cuda_kernels.cu:1898-1926; FAST and
+ *           EXACT both accumulate the stored values in FP64);
+ *   a: the F32, F64 or 16-bit matrix, or the wh plane for packed types; wr:
+ *      the wr plane (used by the EXACT tier only; may be NULL otherwise);
  *   x_type, y_type: TOMOS_F32 | TOMOS_F64.
  * scratch: device buffer of at least tomos_gemv_scratch_size() bytes, or
  * NULL to use a library-owned buffer (grown with hipMalloc on demand). */
@@ -297,7 +300,7 @@ int tomos_sketch_qr_precond_batched_f64(int nprob, const tomos_qr_problem* probs
  * true restart residual is below them.  Replaces lsqr_dense_impl
  * (src/dispatch/lsqr_registries.cpp:88-159). */
 typedef struct tomos_lsqr_options {
-    int a_type;        /* TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16 */
+    int a_type;        /* TOMOS_F32 | TOMOS_F64 | TOMOS_FP32_16 | TOMOS_FP32_BF16 | TOMOS_F16 | TOMOS_BF16 */
     int mv_tier;       /* per-iteration GEMV tier (TOMOS_TIER_*) */
     int restart_tier;  /* restart residual tier (EXACT = "implicit duplicate") */
     int value_type;    /* Krylov vectors u, v, w: TOMOS_F32 | TOMOS_F64 */
@@ -331,6 +334,19 @@ int tomos_lsqr_solve(const tomos_lsqr_options* opt, int64_t m, int64_t n, const 
                      const void* rinv, int64_t ldr, void* arena, size_t arena_bytes,
                      void* scratch, size_t scratch_bytes, int* iterations, double* history,
                      int history_cap, int* history_len, tomos_stream_t stream);
+
+/* tomos_lsqr_solve with an explicit refine-precision duplicate of A: the
+ * restart residuals u = b - A x read a_ref (ref_type TOMOS_F32 or
+ * TOMOS_F64, lda_ref) with FP64 accumulation instead of A, while the inner
+ * iterations use A at mv_tier (lsqr_solve's mtx_refine_precision,
+ * lsqr.cpp:463-469, lsqr_registries.cpp:88-159).  Same arena and scratch
+ * sizes as tomos_lsqr_solve. */
+int tomos_lsqr_solve_refined(const tomos_lsqr_options* opt, int64_t m, int64_t n, const void* a,
+                             const uint16_t* wr, int64_t lda, int ref_type, const void* a_ref,
+                             int64_t lda_ref, const void* b, void* x, const void* rinv, int64_t ldr,
+                             void* arena, size_t arena_bytes, void* scratch, size_t scratch_bytes,
+                             int* iterations, double* history, int history_cap, int* history_len,
+                             tomos_stream_t stream);
 
 /* ---- Many uneven LSQR problems in one set of launches ----------------- */
 /* One problem: A (m x n, lda; for packed storage the hi plane, `wr` the lo

@@ -4,6 +4,7 @@
 // test/core/workspace/test_workspace.cpp, test/core/matrix/test_fp32_16.cpp).
 // Runs on the GPU; prints one line per case and exits non-zero on failure.
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <functional>
 #include <random>
@@ -105,7 +106,7 @@ int main() {
                                              NumericType::FP32, NumericType::FP32_16);
         }));
         EXPECT(throws<std::runtime_error>([] {
-            dispatch::Lsqr::get().lookup_solve_dense(Device::HIP, NumericType::FP16, NumericType::FP32,
+            dispatch::Lsqr::get().lookup_solve_dense(Device::HIP, NumericType::FP32, NumericType::FP16,
                                                     NumericType::FP32, NumericType::FP32);
         }));
     });
@@ -302,6 +303,71 @@ int main() {
         }
         std::printf("    iterations=%d relres=%.3e\n", iters, std::sqrt(rn / bn));
         EXPECT(std::sqrt(rn / bn) < 1.5e-6);
+    });
+    run("native_bf16_lsqr_explicit_duplicate", [] {
+        // A stored as bf16 (round to nearest even) plus a copy at the refine
+        // precision (FP64, as b and x) passed as mtx_refine (runner flag
+        // --explicit-duplicates, lsqr_precision_runner.cu:559-580, 656): the
+        // restarts read the copy, so the solve reaches 1e-6 against the
+        // unrounded matrix; without it the
+        // residual stalls near bf16's rounding of A.
+        const std::size_t m = 6000, n = 60, k = 240;
+        std::mt19937_64 gen(7);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::vector<float> a(m * n);
+        std::vector<uint16_t> a16(m * n);
+        std::vector<double> xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j) {
+            const double d = std::pow(10.0, -3.0 * double(j) / double(n - 1));
+            for (std::size_t i = 0; i < m; ++i) a[i + j * m] = static_cast<float>(nd(gen) * d);
+        }
+        for (std::size_t i = 0; i < m * n; ++i) {
+            uint32_t u;
+            std::memcpy(&u, &a[i], 4);
+            a16[i] = static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        }
+        for (auto& v : xt) v = nd(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += double(a[i + j * m]) * xt[j];
+        Dense A32 = make(NumericType::FP32, m, n, a);
+        Dense A = make(NumericType::BF16, m, n, a16);
+        Dense A64 = make(NumericType::FP64, m, n, std::vector<double>(a.begin(), a.end()));
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        CountSketch S = create_count_sketch(k, m);
+        Dense sk = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(S, A32, sk);
+        generate_sketch_qr_precond(S, sk);
+        Dense P = Dense::wrap_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n}, k, sk.data());
+        auto relres = [&](const Dense& x) {
+            auto sol = fetch<double>(x, n);
+            double rn = 0, bn = 0;
+            for (std::size_t i = 0; i < m; ++i) {
+                double r = -bb[i];
+                for (std::size_t j = 0; j < n; ++j) r += double(a[i + j * m]) * sol[j];
+                rn += r * r;
+                bn += bb[i] * bb[i];
+            }
+            return std::sqrt(rn / bn);
+        };
+        double res[2];
+        for (int dup = 0; dup < 2; ++dup) {
+            Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+            Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP32, NumericType::FP64);
+            add_precond(P, ws);
+            int iters = 0;
+            lsqr_solve(ctx, A, b, x, ws, 1e-6, 8, 1e-6, &iters, nullptr, 30, dup ? &A64 : nullptr);
+            res[dup] = relres(x);
+            std::printf("    duplicate=%d iterations=%d relres=%.3e\n", dup, iters, res[dup]);
+        }
+        EXPECT(res[1] < 1.5e-6);
+        EXPECT(res[0] > res[1] && res[0] < 4.0 * std::ldexp(1.0, -8));
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP32, NumericType::FP64);
+        Dense small = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {m - 1, n});
+        EXPECT(throws<std::invalid_argument>(
+            [&] { lsqr_solve(ctx, A, b, x, ws, 1e-6, 8, 1e-6, nullptr, nullptr, 1, &small); }));
+        EXPECT(throws<std::invalid_argument>(  // must be at b/x's precision
+            [&] { lsqr_solve(ctx, A, b, x, ws, 1e-6, 8, 1e-6, nullptr, nullptr, 1, &A32); }));
     });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;

@@ -50,7 +50,7 @@ def gemv(a_type, tier, trans, alpha, m, n, a_buf, wr_buf, lda, x, beta=0.0, y0=N
 def lsqr(A_or_planes, b, a_type=C.F32, mv_tier=C.TIER_DEFAULT, restart_tier=None,
          value_type=C.F32, refine_type=C.F32, rinv=None, precond_type=-1, tol=1e-6,
          max_iters=400, max_relres=None, restarts=1, check_every=1, lda=None, m=None, n=None,
-         fused=0, history_cap=100000):
+         fused=0, history_cap=100000, refine_dup=None):
     """Runs tomos_lsqr_solve; A_or_planes is a numpy matrix (F32/F64) or a
     (wh_buf, wr_buf) pair of packed device planes (then m, n, lda needed)."""
     if restart_tier is None:
@@ -83,9 +83,17 @@ def lsqr(A_or_planes, b, a_type=C.F32, mv_tier=C.TIER_DEFAULT, restart_tier=None
     it = ctypes.c_int(0)
     hl = ctypes.c_int(0)
     hist = np.zeros(history_cap, np.float64)
-    C.call("tomos_lsqr_solve", ctypes.byref(opt), m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
-           lda, bd.ptr, xd.ptr, rd.ptr if rd is not None else None, ldr, arena.ptr, ab, scr.ptr,
-           sb, ctypes.byref(it), hist.ctypes.data, history_cap, ctypes.byref(hl), None)
+    if refine_dup is None:
+        C.call("tomos_lsqr_solve", ctypes.byref(opt), m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
+               lda, bd.ptr, xd.ptr, rd.ptr if rd is not None else None, ldr, arena.ptr, ab, scr.ptr,
+               sb, ctypes.byref(it), hist.ctypes.data, history_cap, ctypes.byref(hl), None)
+    else:
+        # (ref_type, device buffer, lda): restarts read this duplicate
+        ref_type, ref_buf, lda_ref = refine_dup
+        C.call("tomos_lsqr_solve_refined", ctypes.byref(opt), m, n, a_buf.ptr, wr_buf.ptr if wr_buf else None,
+               lda, ref_type, ref_buf.ptr, lda_ref, bd.ptr, xd.ptr, rd.ptr if rd is not None else None, ldr,
+               arena.ptr, ab, scr.ptr, sb, ctypes.byref(it), hist.ctypes.data, history_cap, ctypes.byref(hl),
+               None)
     x = xd.to_numpy(rdt, (n,))
     return x, it.value, hist[:min(hl.value, history_cap)].copy()
 

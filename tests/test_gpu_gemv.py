@@ -134,3 +134,39 @@ def test_reference_large_shape_l2(capi):
             ex = H.gemv(C.FP32_16, C.TIER_EXACT, trans, 1.0, m, n, wh, wr, m, vec)
             l2 = lambda g: np.linalg.norm(g - ref) / np.linalg.norm(ref)
             assert l2(ph) < 5e-2 and l2(ex) < 1e-3, (m, trans, l2(ph), l2(ex))
+
+
+def bf16_bits_rne(A32):
+    """Round-to-nearest-even FP32 -> bf16 bits (finite inputs)."""
+    u = np.asarray(A32, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
+@pytest.mark.parametrize("a_type", [C.F16, C.BF16])
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("shape", [(3000, 70, 3000), (2999, 33, 2999), (1000, 257, 1008)])
+def test_native_16bit_gemv(capi, a_type, trans, shape):
+    """Native IEEE half / bf16 storage: the stored values times x, FP32
+    accumulation (DEFAULT) or FP64 (FAST, EXACT), against FP64 numpy."""
+    m, n, lda = shape
+    A = data(m, n, 17 + m)
+    if a_type == C.F16:
+        bits = A.astype(np.float16).view(np.uint16)
+        A_used = A.astype(np.float16).astype(np.float64)
+    else:
+        bits = bf16_bits_rne(A)
+        A_used = (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    store = np.zeros((lda, n), np.uint16, order="F")
+    store[:m] = bits
+    Ad = H.dev(store)
+    rng = np.random.default_rng(3)
+    rows, cols = (n, m) if trans else (m, n)
+    x = rng.standard_normal(cols)
+    y0 = rng.standard_normal(rows)
+    op = A_used.T if trans else A_used
+    ref = 1.5 * (op @ x) - 0.5 * y0
+    scale = 1.5 * (np.abs(op) @ np.abs(x)) + 0.5 * np.abs(y0)
+    for tier, rtol in [(C.TIER_DEFAULT, 3e-6), (C.TIER_FAST, 1e-12), (C.TIER_EXACT, 1e-12)]:
+        got = H.gemv(a_type, tier, trans, 1.5, m, n, Ad, None, lda, x, -0.5, y0, C.F64, C.F64)
+        check_close(got, ref, scale, rtol)

@@ -198,3 +198,32 @@ def test_one_pass_restart_reaches_tolerance(capi, n):
     rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
     assert rr < 1.5e-6, (rr, it)
     assert it <= 60, it
+
+
+def _bf16_bits(A32):
+    u = np.asarray(A32, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("a_type,fused", [(C.F16, 0), (C.BF16, 0), (C.BF16, 1)])
+def test_native_16bit_storage_with_explicit_duplicate(capi, a_type, fused):
+    """A stored as native FP16 / BF16 for the iterations and an explicit FP32
+    duplicate for the restart residuals (lsqr_solve's mtx_refine_precision):
+    reaches relres 1e-6, where restarting on the 16-bit copy alone stalls
+    at its own rounding level."""
+    m, n = 20000, 100
+    A, xt, b = c1_problem(m=m, n=n, kappa_exp=3.0, noise=0.0, seed=5)
+    bits = A.astype(np.float16).view(np.uint16) if a_type == C.F16 else _bf16_bits(A)
+    A16 = H.dev(np.asfortranarray(bits))
+    A32 = H.dev(np.asfortranarray(A))
+    rinv, _ = H.sketch_rinv(C.F32, C.TIER_DEFAULT, m, n, A32, None, m, 4 * n)
+    kw = dict(a_type=a_type, mv_tier=C.TIER_DEFAULT, restart_tier=C.TIER_FAST, value_type=C.F32,
+              refine_type=C.F64, rinv=rinv, precond_type=C.F64, tol=1e-6, max_iters=10, restarts=20,
+              m=m, n=n, fused=fused)
+    x, it, _ = H.lsqr((A16, None), b, refine_dup=(C.F32, A32, m), **kw)
+    A64 = A.astype(np.float64)
+    rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
+    assert rr < 1.5e-6, (rr, it)
+    x2, it2, _ = H.lsqr((A16, None), b, **kw)
+    rr2 = np.linalg.norm(A64 @ x2 - b) / np.linalg.norm(b)
+    assert rr2 > 3 * rr, (rr, rr2)

@@ -60,6 +60,23 @@ def test_runner_solves(problem, extra, res_max, err_max):
     assert it >= 1
 
 
+@pytest.mark.parametrize("mv", ["fp16", "bf16"])
+def test_native_16bit_storage_and_explicit_duplicate(problem, mv):
+    """A stored as IEEE half or bf16: the solve alone stops near the 16-bit
+    rounding of A; with an FP32 duplicate of A (--explicit-duplicates) the
+    restarts recompute the residual from an exact copy of A at the refine
+    precision (FP64 here) and the run reaches the requested tolerance against
+    the file's A (lsqr_precision_runner.cu:559-580)."""
+    common = ("--mv-type", mv, "--refine-type", "fp64", "--precond-type", "fp64", "--precond", "--tol", "1e-6",
+              "--max-relres", "1e-6", "--max-iters", "8", "--restarts", "30")
+    it, res, err, out = run(problem, *common, "--explicit-duplicates")
+    assert "explicit FP64-precision duplicate" in out, out
+    assert res <= 1.5e-6 and err <= 1e-2, out
+    it16, res16, err16, out16 = run(problem, *common)
+    unit = 2.0 ** -11 if mv == "fp16" else 2.0 ** -8
+    assert res16 <= 4 * unit and res16 > res, out16
+
+
 def test_residual_history_file(problem, tmp_path):
     h = tmp_path / "hist.txt"
     it, res, err, out = run(problem, "--mv-type", "fp64", "--value-type", "fp64", "--refine-type", "fp64",

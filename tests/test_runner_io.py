@@ -110,7 +110,7 @@ def test_option_errors(tmp_path):
     write_array(tmp_path / "b.mtx", np.ones(4))
     base = [RUNNER, "--matrix", str(tmp_path / "A.mtx"), "--x-true", str(tmp_path / "x.mtx"),
             "--rhs", str(tmp_path / "b.mtx"), "--check-inputs"]
-    for extra, msg in ((["--mv-type", "fp16"], "native 16-bit storage"),
+    for extra, msg in ((["--mv-type", "fp8"], "unknown --mv-type"),
                        (["--explicit-duplicates", "--implicit-duplicates"], "mutually exclusive"),
                        (["--value-type", "fp16"], "supports fp64 and fp32"),
                        (["--bogus"], "unrecognized argument")):
@@ -118,3 +118,7 @@ def test_option_errors(tmp_path):
         assert r.returncode == 1 and msg in r.stderr, (extra, r.stderr)
     r = subprocess.run(base[:-1] + ["--device", "cpu"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "runs LSQR on the GPU" in r.stderr
+    for mv, name in (("fp16", "FP16"), ("half", "FP16"), ("bf16", "BF16")):
+        r = subprocess.run(base + ["--mv-type", mv, "--explicit-duplicates"], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 0 and "mv_storage=%s mv_compute=%s" % (name, name) in r.stdout, r.stderr

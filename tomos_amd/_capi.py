@@ -126,6 +126,9 @@ _SIGS = {
     "tomos_lsqr_solve": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, vp, vp, vp,
                                 i64, vp, sz, vp, sz, ctypes.POINTER(cint), vp, cint,
                                 ctypes.POINTER(cint), vp]),
+    "tomos_lsqr_solve_refined": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, cint, vp, i64, vp,
+                                        vp, vp, i64, vp, sz, vp, sz, ctypes.POINTER(cint), vp, cint,
+                                        ctypes.POINTER(cint), vp]),
 }
 
 _lib = None

@@ -31,7 +31,9 @@ namespace tomos_hip {
 
 namespace {
 
-enum { FMT_F32 = 0, FMT_F64 = 1, FMT_P16 = 2, FMT_PBF = 3 };
+// FMT_PBF with DEC_HI also serves native BF16 storage (the hi plane is a
+// bf16 array); FMT_H16 is native IEEE binary16 storage.
+enum { FMT_F32 = 0, FMT_F64 = 1, FMT_P16 = 2, FMT_PBF = 3, FMT_H16 = 4 };
 enum { DEC_NATIVE = 0, DEC_HI = 1, DEC_EXACT = 2 };
 
 constexpr int kThreads = 256;
@@ -42,7 +44,7 @@ constexpr int vec_full() {
 }
 template <int FMT>
 constexpr bool is16() {
-    return FMT == FMT_P16 || FMT == FMT_PBF;
+    return FMT == FMT_P16 || FMT == FMT_PBF || FMT == FMT_H16;
 }
 
 // Raw (undecoded) bits of VEC consecutive elements.
@@ -95,7 +97,9 @@ __device__ __forceinline__ float half_bits_to_float(uint32_t hb) {
 
 template <int FMT, int DEC>
 __device__ __forceinline__ float decode16(uint32_t h, uint32_t r) {
-    if constexpr (FMT == FMT_P16) {
+    if constexpr (FMT == FMT_H16) {
+        return half_bits_to_float(h);
+    } else if constexpr (FMT == FMT_P16) {
         if constexpr (DEC == DEC_EXACT) return codec::fp32_16_decode_exact(h, r);
         else return half_bits_to_float(codec::fp32_16_half_bits(h));
     } else {
@@ -367,6 +371,10 @@ Kind classify(int a_type, int tier) {
             return {FMT_P16, tier == TOMOS_TIER_EXACT ? DEC_EXACT : DEC_HI, tier != TOMOS_TIER_DEFAULT};
         case TOMOS_FP32_BF16:
             return {FMT_PBF, tier == TOMOS_TIER_EXACT ? DEC_EXACT : DEC_HI, tier != TOMOS_TIER_DEFAULT};
+        // native 16-bit storage: the stored value is the exact one, so EXACT
+        // (like FAST) decodes it and accumulates in FP64
+        case TOMOS_BF16: return {FMT_PBF, DEC_HI, tier != TOMOS_TIER_DEFAULT};
+        case TOMOS_F16: return {FMT_H16, DEC_HI, tier != TOMOS_TIER_DEFAULT};
         default:
             throw Error(kErrUnsupported, "gemv: unsupported matrix type " + std::to_string(a_type));
     }
@@ -534,6 +542,7 @@ int nb = 0;
 #define RUN_N(F, D, A) nb = run_n<F, D, A>(m, n, a, wr, lda, x, x64, y, y64, coef, scratch, vec, p, s)
         if (k.fmt == FMT_F32) { if (k.acc64) RUN_N(FMT_F32, DEC_NATIVE, double); else RUN_N(FMT_F32, DEC_NATIVE, float); }
         else if (k.fmt == FMT_F64) RUN_N(FMT_F64, DEC_NATIVE, double);
+        else if (k.fmt == FMT_H16) { if (k.acc64) RUN_N(FMT_H16, DEC_HI, double); else RUN_N(FMT_H16, DEC_HI, float); }
         else if (k.fmt == FMT_P16) {
             if (k.dec == DEC_EXACT) RUN_N(FMT_P16, DEC_EXACT, double);
             else if (k.acc64) RUN_N(FMT_P16, DEC_HI, double);
@@ -550,6 +559,7 @@ int nb = 0;
 #define RUN_T(F, D, A) run_t<F, D, A>(m, n, a, wr, lda, x, x64, y, y64, coef, scratch, vec, p, s)
         if (k.fmt == FMT_F32) { if (k.acc64) RUN_T(FMT_F32, DEC_NATIVE, double); else RUN_T(FMT_F32, DEC_NATIVE, float); }
         else if (k.fmt == FMT_F64) RUN_T(FMT_F64, DEC_NATIVE, double);
+        else if (k.fmt == FMT_H16) { if (k.acc64) RUN_T(FMT_H16, DEC_HI, double); else RUN_T(FMT_H16, DEC_HI, float); }
         else if (k.fmt == FMT_P16) {
             if (k.dec == DEC_EXACT) RUN_T(FMT_P16, DEC_EXACT, double);
             else if (k.acc64) RUN_T(FMT_P16, DEC_HI, double);
@@ -581,6 +591,7 @@ int gemv_t_partials(int a_type, int tier, int64_t m, int64_t n, const void* a, c
 #define RUN_TP(F, D) run_t<F, D, double>(m, n, a, wr, lda, x, x64, nullptr, false, c, scratch, vec, p, s, false)
     if (k.fmt == FMT_F32) RUN_TP(FMT_F32, DEC_NATIVE);
     else if (k.fmt == FMT_F64) RUN_TP(FMT_F64, DEC_NATIVE);
+    else if (k.fmt == FMT_H16) RUN_TP(FMT_H16, DEC_HI);
     else if (k.fmt == FMT_P16) { if (k.dec == DEC_EXACT) RUN_TP(FMT_P16, DEC_EXACT); else RUN_TP(FMT_P16, DEC_HI); }
     else { if (k.dec == DEC_EXACT) RUN_TP(FMT_PBF, DEC_EXACT); else RUN_TP(FMT_PBF, DEC_HI); }
 #undef RUN_TP

@@ -313,7 +313,8 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 26));
     L.hist = off; off = align256(off + static_cast<size_t>(L.hist_cap) * 8);
     L.fused_possible = o.fused != 0 && o.mv_tier == TOMOS_TIER_DEFAULT && n >= 1 && n <= 2048 &&
-                       (o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16);
+                       (o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16 ||
+                        o.a_type == TOMOS_BF16);
     if (L.fused_possible) L.fplan = lsqr_fused_plan(o.a_type, m, n);
     // sized for every fused plan of (m, n), so tuning after sizing is safe
     const int64_t red_n = std::max<int64_t>(std::max<int64_t>(gemv_sumsq_blocks_max(m), kRedBlocks),
@@ -329,6 +330,9 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_NO_TRANSPOSE, m, n));
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.mv_tier, TOMOS_TRANSPOSE, m, n));
     sc = std::max(sc, gemv_scratch_bytes(o.a_type, o.restart_tier, TOMOS_NO_TRANSPOSE, m, n));
+    // an explicit F32 / F64 duplicate of A may serve the restarts
+    sc = std::max(sc, gemv_scratch_bytes(TOMOS_F32, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, m, n));
+    sc = std::max(sc, gemv_scratch_bytes(TOMOS_F64, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, m, n));
     if (o.precond_type >= 0) {
         const int pt = o.precond_type == TOMOS_F64 ? TOMOS_TIER_DEFAULT : TOMOS_TIER_EXACT;
         sc = std::max(sc, gemv_scratch_bytes(o.precond_type, pt, TOMOS_NO_TRANSPOSE, n, n));
@@ -343,7 +347,7 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
 
 void validate(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     require(o.a_type == TOMOS_F32 || o.a_type == TOMOS_F64 || o.a_type == TOMOS_FP32_16 ||
-                o.a_type == TOMOS_FP32_BF16,
+                o.a_type == TOMOS_FP32_BF16 || o.a_type == TOMOS_F16 || o.a_type == TOMOS_BF16,
             "lsqr: unsupported a_type");
     require(o.value_type == TOMOS_F32 || o.value_type == TOMOS_F64, "lsqr: value_type must be F32/F64");
     require(o.refine_type == TOMOS_F32 || o.refine_type == TOMOS_F64, "lsqr: refine_type must be F32/F64");
@@ -363,6 +367,11 @@ struct Solver {
     const void* a;
     const uint16_t* wr;
     int64_t lda;
+    // explicit refine-precision duplicate of A (F32/F64) for the restart
+    // residuals, or none (lsqr.cpp:463-469 with mtx_refine_precision)
+    const void* aref = nullptr;
+    int ref_type = -1;
+    int64_t lda_ref = 0;
     const void* b;
     void* x;
     const void* rinv;
@@ -419,6 +428,17 @@ struct Solver {
         LsqrState* S = state();
         // u_raw = b - A x at the restart tier; beta = |u_raw| (fused partials)
         k_copy<<<grid_for(m), kT, 0, s>>>(S, b, r64, U(), v64, m);
+        if (aref) {
+            // the duplicate at its own precision, FP64 accumulation
+            Coef cr = coef(-1.0, 1.0);
+            cr.sumsq = red();
+            int nb = 0;
+            gemv(ref_type, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, cr, m, n, aref, nullptr, lda_ref,
+                 r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), scratch(), L.gemv_bytes, s, &nb);
+            k_beta_finish<<<1, kT, 0, s>>>(S, red(), nb);
+            tail(true);
+            return;
+        }
         if (use_fused_restart) {
             float* tpart = reinterpret_cast<float*>(sc + L.tpart);
             double* tsum = reinterpret_cast<double*>(sc + L.tsum);
@@ -543,9 +563,14 @@ size_t lsqr_scratch_bytes(const tomos_lsqr_options& o, int64_t m, int64_t n) {
 void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a, const uint16_t* wr,
                 int64_t lda, const void* b, void* x, const void* rinv, int64_t ldr, void* arena,
                 size_t arena_bytes, void* scratch, size_t scratch_bytes, int* iterations,
-                double* history, int history_cap, int* history_len, hipStream_t s) {
+                double* history, int history_cap, int* history_len, hipStream_t s, int ref_type = -1,
+                const void* aref = nullptr, int64_t lda_ref = 0) {
     validate(o, m, n);
     require(lda >= std::max<int64_t>(m, 1), "lsqr: lda < m");
+    if (aref) {
+        require(ref_type == TOMOS_F32 || ref_type == TOMOS_F64, "lsqr: the refine duplicate must be F32 or F64");
+        require(lda_ref >= std::max<int64_t>(m, 1), "lsqr: lda of the refine duplicate < m");
+    }
     if (o.precond_type >= 0) {
         require(rinv != nullptr, "lsqr: precond_type set but no R^-1 given");
         require(ldr >= std::max<int64_t>(n, 1), "lsqr: ldr < n");
@@ -570,6 +595,9 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.x = x;
     sv.rinv = rinv;
     sv.ldr = ldr;
+    sv.aref = aref;
+    sv.ref_type = ref_type;
+    sv.lda_ref = lda_ref;
     sv.L = layout_for(o, m, n);
     // lsqr_registries.cpp:59-63: the arena must hold the carved vectors
     if (arena == nullptr || arena_bytes < sv.L.arena_total)
@@ -587,7 +615,7 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.r64 = o.refine_type == TOMOS_F64;
     sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
     sv.use_fused = sv.L.fused_possible && lsqr_fused_supported(o.a_type, o.mv_tier, m, n, lda, a);
-    sv.use_fused_restart = sv.use_fused && sv.r64 &&
+    sv.use_fused_restart = sv.use_fused && sv.r64 && aref == nullptr &&
                            lsqr_restart_supported(o.a_type, o.restart_tier, m, n, lda, a, wr);
     if (sv.use_fused_restart) sv.rplan = lsqr_restart_plan(o.a_type, m, n);
     sv.run(iterations, history, history_cap, history_len);
@@ -626,6 +654,19 @@ int tomos_lsqr_solve(const tomos_lsqr_options* o, int64_t m, int64_t n, const vo
         require(o != nullptr, "tomos_lsqr_solve: null options");
         lsqr_solve(*o, m, n, a, wr, lda, b, x, rinv, ldr, arena, arena_bytes, scratch,
                    scratch_bytes, iterations, history, history_cap, history_len, s);
+    });
+}
+
+int tomos_lsqr_solve_refined(const tomos_lsqr_options* o, int64_t m, int64_t n, const void* a,
+                             const uint16_t* wr, int64_t lda, int ref_type, const void* a_ref, int64_t lda_ref,
+                             const void* b, void* x, const void* rinv, int64_t ldr, void* arena,
+                             size_t arena_bytes, void* scratch, size_t scratch_bytes, int* iterations,
+                             double* history, int history_cap, int* history_len, tomos_stream_t s) {
+    return guarded([&] {
+        require(o != nullptr, "tomos_lsqr_solve_refined: null options");
+        require(a_ref != nullptr, "tomos_lsqr_solve_refined: null refine duplicate");
+        lsqr_solve(*o, m, n, a, wr, lda, b, x, rinv, ldr, arena, arena_bytes, scratch, scratch_bytes,
+                   iterations, history, history_cap, history_len, s, ref_type, a_ref, lda_ref);
     });
 }
 

@@ -756,7 +756,9 @@ const void* grouped_kernel_for(int a_type, int g, int nt, int rounds) {
     return grouped_kernel_ptr<FMT_F32>(g, nt, rounds);
 }
 
+// Native BF16 storage runs the FP32_BF16 kernels: the array is a hi plane.
 const void* fused_kernel_for(int a_type, int g, int nt, int rounds, bool ntl) {
+    if (a_type == TOMOS_BF16) a_type = TOMOS_FP32_BF16;
     if (ntl) {
         if (a_type == TOMOS_FP32_BF16) return fused_kernel_ptr<FMT_PBF, true>(g, nt, rounds);
         if (a_type == TOMOS_FP32_16) return fused_kernel_ptr<FMT_P16, true>(g, nt, rounds);
@@ -801,7 +803,8 @@ int grouped_fused_occupancy(int a_type, int g, int threads, int rounds) {
 
 bool lsqr_fused_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a) {
     if (tier != TOMOS_TIER_DEFAULT) return false;
-    if (a_type != TOMOS_F32 && a_type != TOMOS_FP32_BF16 && a_type != TOMOS_FP32_16) return false;
+    if (a_type != TOMOS_F32 && a_type != TOMOS_FP32_BF16 && a_type != TOMOS_FP32_16 && a_type != TOMOS_BF16)
+        return false;
     const int vec = a_type == TOMOS_F32 ? 4 : 8;
     if (n < 1 || n > kFusedMaxN || m < 1) return false;
     return lda % vec == 0 && reinterpret_cast<uintptr_t>(a) % 16 == 0;

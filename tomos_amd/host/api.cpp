@@ -168,6 +168,12 @@ Dense::Dense() {
             mv_[encode_mv(d, NumericType::FP32_16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_EXACT>;
             mv_[encode_mv(d, NumericType::FP32_BF16, x, y, NumericType::BF16)] = &mv_impl<TOMOS_TIER_DEFAULT>;
             mv_[encode_mv(d, NumericType::FP32_BF16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_EXACT>;
+            // native 16-bit storage (mv_fp16_native_mixed, dense_registries.cpp:332-333):
+            // FP32 accumulation at the storage's own compute type, FP64 when FP32/FP64 is asked
+            mv_[encode_mv(d, NumericType::FP16, x, y, NumericType::FP16)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::FP16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_FAST>;
+            mv_[encode_mv(d, NumericType::BF16, x, y, NumericType::BF16)] = &mv_impl<TOMOS_TIER_DEFAULT>;
+            mv_[encode_mv(d, NumericType::BF16, x, y, NumericType::FP32)] = &mv_impl<TOMOS_TIER_FAST>;
         }
     const NumericType f = NumericType::FP32;
     mm_[encode_mm(d, NumericType::FP32, f, f, false)] = &mm_impl<TOMOS_TIER_DEFAULT>;
@@ -273,11 +279,17 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
                      std::vector<double>* history, int restarts, tomos::Dense* mtx_refine) {
     if (ws.device_ != context.get_device())
         throw std::invalid_argument("lsqr_solve: workspace and context must be on the same device.");
-    if (mtx_refine)
-        throw std::invalid_argument(
-            "lsqr_solve: an explicit refine-precision duplicate of A is not supported; store A "
-            "packed and set Workspace::use_implicit_duplicate_ instead.");
     const auto sz = A.get_size();
+    if (mtx_refine) {
+        // explicit duplicate (lsqr.cpp:463-469): the restart residuals read it
+        if (mtx_refine->get_value_type() != NumericType::FP32 && mtx_refine->get_value_type() != NumericType::FP64)
+            throw std::invalid_argument("lsqr_solve: the refine-precision duplicate must be FP32 or FP64.");
+        if (mtx_refine->get_size() != sz || mtx_refine->get_device() != A.get_device())
+            throw std::invalid_argument("lsqr_solve: the refine-precision duplicate must match A's size and device.");
+        if (ws.use_implicit_duplicate_)
+            throw std::invalid_argument(
+                "lsqr_solve: an explicit duplicate and Workspace::use_implicit_duplicate_ are mutually exclusive.");
+    }
     const int64_t m = static_cast<int64_t>(sz[0]), n = static_cast<int64_t>(sz[1]);
     if (vec_len(b) != sz[0] || vec_len(x) != sz[1])
         throw std::invalid_argument("lsqr_solve: b/x sizes do not match A");
@@ -318,12 +330,22 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
     int it = 0, hl = 0;
     const void* a = is_packed(MV) ? static_cast<const void*>(A.wh()) : A.data();
     const uint16_t* wr = is_packed(MV) ? A.wr() : nullptr;
-    check(tomos_lsqr_solve(&o, m, n, a, wr, static_cast<int64_t>(A.get_lead_dim()), b.data(), x.data(),
-                           P ? P->data() : nullptr, P ? static_cast<int64_t>(P->get_lead_dim()) : 1,
-                           ws.data_, ws.capacity_, ws.mv_transpose_scratch_,
-                           ws.mv_transpose_scratch_capacity_, &it, history ? hist.data() : nullptr,
-                           history ? cap : 0, &hl, context.get_stream()),
-          "lsqr_solve");
+    if (mtx_refine)
+        check(tomos_lsqr_solve_refined(&o, m, n, a, wr, static_cast<int64_t>(A.get_lead_dim()),
+                                       abi_type(mtx_refine->get_value_type()), mtx_refine->data(),
+                                       static_cast<int64_t>(mtx_refine->get_lead_dim()), b.data(), x.data(),
+                                       P ? P->data() : nullptr, P ? static_cast<int64_t>(P->get_lead_dim()) : 1,
+                                       ws.data_, ws.capacity_, ws.mv_transpose_scratch_,
+                                       ws.mv_transpose_scratch_capacity_, &it, history ? hist.data() : nullptr,
+                                       history ? cap : 0, &hl, context.get_stream()),
+              "lsqr_solve");
+    else
+        check(tomos_lsqr_solve(&o, m, n, a, wr, static_cast<int64_t>(A.get_lead_dim()), b.data(), x.data(),
+                               P ? P->data() : nullptr, P ? static_cast<int64_t>(P->get_lead_dim()) : 1,
+                               ws.data_, ws.capacity_, ws.mv_transpose_scratch_,
+                               ws.mv_transpose_scratch_capacity_, &it, history ? hist.data() : nullptr,
+                               history ? cap : 0, &hl, context.get_stream()),
+              "lsqr_solve");
     if (iterations) *iterations = it;
     if (history) history->insert(history->end(), hist.begin(), hist.begin() + std::min(hl, cap));
 }
@@ -367,6 +389,8 @@ Lsqr::Lsqr() {
     register_val<NumericType::FP64>(dense_);
     register_val<NumericType::FP32_16>(dense_);
     register_val<NumericType::FP32_BF16>(dense_);
+    register_val<NumericType::FP16>(dense_);
+    register_val<NumericType::BF16>(dense_);
 }
 
 const Lsqr& Lsqr::get() {

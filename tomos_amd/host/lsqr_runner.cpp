@@ -7,15 +7,16 @@
 // :264-420).
 //
 // Differences: the device is the GPU only (--device cpu is rejected: the
-// CPU path of this build is the oracle used by the tests); --mv-type fp16
-// and bf16 (native 16-bit storage) and --explicit-duplicates are rejected
-// with a message; the preconditioner's sketch is the hashed CountSketch
-// with the reference's fixed seeds (the reference runner's sketch is
-// time-seeded); the final residual is computed on the host in FP64 from
-// the file's values.  --check-inputs reads and validates the files and
-// stops before any GPU work; --two-pass turns off the single-pass kernels.
+// CPU path of this build is the oracle used by the tests); the
+// preconditioner's sketch is the hashed CountSketch with the reference's
+// fixed seeds (the reference runner's sketch is time-seeded) and, for
+// native 16-bit storage, is taken from an FP32 copy of A; the final
+// residual is computed on the host in FP64 from the file's values.
+// --check-inputs reads and validates the files and stops before any GPU
+// work; --two-pass turns off the single-pass kernels.
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,7 +47,7 @@ struct Args {
 void usage(const char* prog) {
     std::fprintf(stderr,
                  "Usage: %s --matrix A.mtx --x-true x.mtx --rhs b.mtx\n"
-                 "          [--device gpu] [--mv-type fp64|fp32|fp32_16|fp32_bf16] [--value-type fp64|fp32]\n"
+                 "          [--device gpu] [--mv-type fp64|fp32|fp16|bf16|fp32_16|fp32_bf16] [--value-type fp64|fp32]\n"
                  "          [--refine-type fp64|fp32] [--precond-type fp64|fp32]\n"
                  "          [--tol TOL] [--max-iters N] [--max-relres R] [--restarts N]\n"
                  "          [--residual-history PATH] [--precond] [--normalize-columns]\n"
@@ -72,10 +73,43 @@ NumericType storage_type(const std::string& raw) {
     if (s == "fp32" || s == "float" || s == "single") return NumericType::FP32;
     if (s == "fp32_16") return NumericType::FP32_16;
     if (s == "fp32_bf16") return NumericType::FP32_BF16;
-    if (s == "fp16" || s == "half" || s == "bf16")
-        throw std::invalid_argument("--mv-type " + raw +
-                                    ": native 16-bit storage is not built; use fp32_16 or fp32_bf16 (packed)");
+    if (s == "fp16" || s == "half") return NumericType::FP16;
+    if (s == "bf16") return NumericType::BF16;
     throw std::invalid_argument("unknown --mv-type '" + raw + "'");
+}
+
+// FP32 -> IEEE binary16 bits, round to nearest even (subnormals, overflow
+// to infinity, NaN kept quiet).
+uint16_t to_half_bits(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) return static_cast<uint16_t>(sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u));
+    if (ax >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);  // rounds past 65504
+    if (ax <= 0x33000000u) return static_cast<uint16_t>(sign);            // <= 2^-25: to zero (even)
+    const int e = static_cast<int>(ax >> 23);
+    const uint32_t mant = (ax & 0x7FFFFFu) | 0x800000u;
+    uint32_t h, rem, tie;
+    if (e < 113) {  // half subnormal: mant * 2^(e-126) units of 2^-24
+        const int shift = 126 - e;
+        h = mant >> shift;
+        rem = mant & ((1u << shift) - 1u);
+        tie = 1u << (shift - 1);
+    } else {
+        h = (static_cast<uint32_t>(e - 112) << 10) | ((mant >> 13) & 0x3FFu);
+        rem = mant & 0x1FFFu;
+        tie = 0x1000u;
+    }
+    if (rem > tie || (rem == tie && (h & 1u))) ++h;
+    return static_cast<uint16_t>(sign | h);
+}
+
+// FP32 -> bf16 bits, round to nearest even (finite inputs; NaN kept NaN).
+uint16_t to_bf16_bits(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    if ((x & 0x7FFFFFFFu) > 0x7F800000u) return static_cast<uint16_t>((x >> 16) | 0x40u);
+    return static_cast<uint16_t>((static_cast<uint64_t>(x) + 0x7FFFu + ((x >> 16) & 1u)) >> 16);
 }
 
 Args parse(int argc, char** argv) {
@@ -184,17 +218,15 @@ int run(const Args& args) {
         store = store == NumericType::FP32_BF16 ? NumericType::FP32_BF16 : NumericType::FP32_16;
         compute = store == NumericType::FP32_16 ? NumericType::FP16 : NumericType::BF16;
     }
-    if (args.explicit_dup)
-        throw std::invalid_argument("--explicit-duplicates: a second copy of A is not supported; use "
-                                    "--implicit-duplicates (packed A, exact restarts)");
     if (args.device != "gpu" && args.device != "hip" && args.device != "cuda")
         throw std::invalid_argument("--device " + args.device +
                                     ": this build runs LSQR on the GPU (the CPU path is the test oracle)");
     std::printf("==> mv_storage=%s mv_compute=%s value_type=%s refine_type=%s precond=%s precond_type=%s "
-                "normalize_columns=%s implicit_duplicates=%s single_pass=%s\n",
+                "normalize_columns=%s explicit_duplicates=%s implicit_duplicates=%s single_pass=%s\n",
                 numeric_type_name(store), numeric_type_name(compute), numeric_type_name(value),
                 numeric_type_name(refine), args.precond ? "yes" : "no", numeric_type_name(pre_t),
-                args.normalize ? "yes" : "no", args.implicit_dup ? "yes" : "no", args.two_pass ? "no" : "yes");
+                args.normalize ? "yes" : "no", args.explicit_dup ? "yes" : "no", args.implicit_dup ? "yes" : "no",
+                args.two_pass ? "no" : "yes");
     std::printf("==> max_iters=%d (per restart) restarts=%d (ceiling: %d iterations)\n", args.max_iters,
                 args.restarts, args.max_iters * args.restarts);
     if (args.check_inputs) {
@@ -226,8 +258,19 @@ int run(const Args& args) {
     }
 
     Context ctx(Device::HIP);
+    const bool native16 = store == NumericType::FP16 || store == NumericType::BF16;
     Dense Ad = [&]() {
         if (store == NumericType::FP64 || store == NumericType::FP32) return upload_as(ctx, store, m, n, Aw);
+        if (native16) {
+            std::vector<uint16_t> h(Aw.size());
+            for (size_t k = 0; k < h.size(); ++k)
+                h[k] = store == NumericType::FP16 ? to_half_bits(static_cast<float>(Aw[k]))
+                                                  : to_bf16_bits(static_cast<float>(Aw[k]));
+            Dense d = Dense::create_strided(ctx, Layout::COL_MAJOR, store,
+                                            {static_cast<size_t>(m), static_cast<size_t>(n)});
+            d.copy_from_host(h.data(), h.size() * sizeof(uint16_t));
+            return d;
+        }
         Dense a32 = upload<float>(ctx, NumericType::FP32, m, n, Aw);
         Dense p = Dense::create_strided(ctx, Layout::COL_MAJOR, store, {static_cast<size_t>(m), static_cast<size_t>(n)});
         if (store == NumericType::FP32_16) split_fp32_16(a32, p);
@@ -240,6 +283,16 @@ int run(const Args& args) {
     Workspace ws = args.precond ? create_lsqr_workspace(ctx, {static_cast<size_t>(m), static_cast<size_t>(n)}, value, pre_t)
                                 : create_lsqr_workspace(ctx, {static_cast<size_t>(m), static_cast<size_t>(n)}, value);
     std::vector<Dense> keep;
+    keep.reserve(2);  // a_refine points into it
+    // explicit duplicate at the refine precision: the restart residuals read it
+    const bool dup = args.explicit_dup && store != refine;
+    if (args.explicit_dup && !dup) std::printf("==> --explicit-duplicates has no effect here (mv_type == refine_type)\n");
+    if (dup) {
+        std::printf("==> Building explicit %s-precision duplicate of A (%lld x %lld)...\n", numeric_type_name(refine),
+                    static_cast<long long>(m), static_cast<long long>(n));
+        keep.push_back(upload_as(ctx, refine, m, n, Aw));
+    }
+    Dense* a_refine = dup ? &keep.back() : nullptr;
     if (args.precond) {
         const size_t k = 2 * static_cast<size_t>(n);
         std::printf("==> Building count-sketch + QR preconditioner (k=%zu = 2n rows, R^-1 at %s)\n", k,
@@ -247,7 +300,12 @@ int run(const Args& args) {
         CountSketch S = create_count_sketch(k, static_cast<size_t>(m));
         generate_count_sketch(S);
         Dense sk = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, static_cast<size_t>(n)});
-        apply_count_sketch(S, Ad, sk);
+        if (native16) {  // the sketch reads an FP32 copy of A
+            Dense a32 = upload<float>(ctx, NumericType::FP32, m, n, Aw);
+            apply_count_sketch(S, a32, sk);
+        } else {
+            apply_count_sketch(S, Ad, sk);
+        }
         generate_sketch_qr_precond(S, sk);
         std::vector<double> full = download(sk, static_cast<int64_t>(k) * n);
         std::vector<double> rinv(static_cast<size_t>(n * n));
@@ -269,7 +327,7 @@ int run(const Args& args) {
     tomos_device_synchronize();
     const auto t0 = std::chrono::steady_clock::now();
     lsqr_solve(ctx, Ad, bd, xd, ws, args.tol, args.max_iters, args.max_relres, &iterations,
-               want_history ? &history : nullptr, args.restarts, nullptr);
+               want_history ? &history : nullptr, args.restarts, a_refine);
     tomos_device_synchronize();
     const auto t1 = std::chrono::steady_clock::now();
     const double solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
