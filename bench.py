@@ -428,6 +428,16 @@ def main():
         "read_stream_same_bytes": {"ms": round(t_rd, 4), "GBps": round(gbs(2.0 * m * n, t_rd), 1)},
     }
     log("gemv: %s" % gemv_detail)
+
+    # dense Gaussian sketch fill (create_gaussian_sketch): 2n x 131072 FP32, 1 GiB written
+    gk, gm = 2 * n, 131072
+    gbuf = DeviceBuffer(gk * gm * 4)
+    t_gf = time_call(lambda: C.call("tomos_gaussian_fill", C.F32, gk, gm, gbuf.ptr, gk, 5, None), 5, ev0, ev1)
+    gaussian = {"rows": gk, "cols": gm, "type": "f32", "ms": round(t_gf, 4),
+                "GBps_written": round(gbs(4.0 * gk * gm, t_gf), 1),
+                "Gsamples_per_s": round(gk * gm / (t_gf * 1e-3) / 1e9, 1)}
+    del gbuf
+    log("gaussian sketch fill: %s" % gaussian)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "lsqr_fused_bf16_2e6x1000_traffic.md")
     if os.path.exists(tf) and (m, n) == (2_000_000, 1000):
@@ -579,6 +589,7 @@ def main():
         "gemv": gemv_detail,
         "tuned_shapes": tuned,
         "gemm_split": gemm,
+        "gaussian_sketch": gaussian,
         "batched_lstsq": batched,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -11,6 +11,7 @@
 // populated: there is no CPU implementation in this library.
 #pragma once
 
+#include <cstdint>
 #include <vector>
 
 #include "core.hpp"
@@ -82,6 +83,15 @@ void apply_count_sketch(CountSketch& sketch, Dense& source, Dense& target);
 // std::runtime_error if R is singular.  sketch_qr.cpp:68-122
 void generate_sketch_qr_precond(CountSketch& sketch, Dense& sketched_mtx);
 
+// Dense Gaussian sketch: S <- i.i.d. N(0,1), S FP64, FP32, FP16 or BF16
+// (include/core/sketch.hpp:11-13).  The reference seeds each call from the
+// clock; here a call without a seed takes the next seed of a process-wide
+// sequence, so successive sketches differ and a program's sketches repeat
+// from run to run.  The seeded overload fixes the values: they depend only
+// on (seed, rows, cols), not on the lead dimension.
+Status create_gaussian_sketch(Dense& S);
+Status create_gaussian_sketch(Dense& S, std::uint64_t seed);
+
 // ------------------------------------------------------------------ LSQR
 // include/core/solver.hpp:15-72
 Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type);
@@ -110,6 +120,7 @@ using lsqr_dense_fn = void (*)(Context context, tomos::Dense& A, tomos::Dense& b
                                tomos::Dense* mtx_refine);
 using apply_count_sketch_dense_fn = void (*)(CountSketch& S, tomos::Dense& source,
                                              tomos::Dense& target);
+using create_gaussian_sketch_dense_fn = Status (*)(tomos::Dense& S, std::uint64_t seed);
 
 struct Dense {
     static const Dense& get();
@@ -198,6 +209,19 @@ private:
     Sketch();
     std::array<apply_count_sketch_dense_fn, num_registry_devices * num_registry_numeric_types>
         apply_{};
+};
+
+// gaussian_sketch_registries.hpp: one slot per (device, value type).  The
+// slot takes the seed as well (the reference's generates its own).
+struct GaussianSketch {
+    static const GaussianSketch& get();
+    create_gaussian_sketch_dense_fn lookup_create_dense(Device dev, NumericType type) const;
+    static std::size_t encode_dense(Device dev, NumericType type);
+
+private:
+    GaussianSketch();
+    std::array<create_gaussian_sketch_dense_fn, num_registry_devices * num_registry_numeric_types>
+        dense_{};
 };
 
 }  // namespace dispatch

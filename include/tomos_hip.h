@@ -243,6 +243,19 @@ int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_p
 int tomos_countsketch_indices(int64_t m, int64_t s, int32_t* bucket, int8_t* sign,
                               tomos_stream_t stream);
 
+/* Dense Gaussian sketch: out (rows x cols, ld, column-major; type
+ * TOMOS_F64, TOMOS_F32, TOMOS_F16 or TOMOS_BF16) <- i.i.d. N(0,1).
+ * Replaces cuda::generate_gaussian_values
+ * (src/kernels/cuda/sketch/gaussian_sketch_cuda_kernels.cu:36-45) and the
+ * CPU generator (src/kernels/cpu/sketch/gaussian_sketch_kernels.cpp:10-18),
+ * which seed themselves from the clock; here the seed is an argument and
+ * the values depend only on (seed, rows, cols): entries (2q, j) and
+ * (2q+1, j) are the Box-Muller pair of the p-th SplitMix64 output,
+ * p = j * ceil(rows/2) + q.  Entries between rows and ld are not written.
+ * Asynchronous on `stream`. */
+int tomos_gaussian_fill(int type, int64_t rows, int64_t cols, void* out, int64_t ld, uint64_t seed,
+                        tomos_stream_t stream);
+
 /* Y (s x n, FP64, ldy) = S * A, S the hashed s x m CountSketch above,
  * A m x n in any a_type/tier (packed A is decoded per tier).  Y is
  * overwritten.  Replaces countsketch_apply_matrix_hashed

@@ -83,6 +83,8 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int8)]
         L.orc_countsketch_apply_f64.restype = None
         L.orc_countsketch_apply_f64.argtypes = [i64, i64, f64p, i64, i64, f64p, i64]
+        L.orc_gaussian_fill_f64.restype = None
+        L.orc_gaussian_fill_f64.argtypes = [i64, i64, f64p, i64, ctypes.c_uint64]
         L.orc_householder_qr.restype = ctypes.c_int
         L.orc_householder_qr.argtypes = [i64, i64, f64p, i64]
         L.orc_trtri_upper.restype = ctypes.c_int
@@ -169,6 +171,14 @@ def countsketch_apply(X, s):
     m, n = X.shape
     Y = np.zeros((s, n), np.float64, order="F")
     lib().orc_countsketch_apply_f64(m, n, _p(X, f64p), max(m, 1), s, _p(Y, f64p), s)
+    return Y
+
+
+def gaussian_fill(rows, cols, seed):
+    """(rows x cols) FP64 N(0,1) matrix of the seeded counter layout."""
+    Y = np.zeros((rows, cols), np.float64, order="F")
+    if rows and cols:
+        lib().orc_gaussian_fill_f64(rows, cols, _p(Y, f64p), rows, seed & (2**64 - 1))
     return Y
 
 

@@ -15,6 +15,10 @@
  *   pack / decode (FP32_16, FP32_BF16) ........ pinned (exhaustive + numpy/torch)
  *   CPU GEMV tiers (accurate/fast/pure_half) .. pinned (numpy fp64 / numpy fp16)
  *   hashed CountSketch ........................ pinned (pure-Python splitmix64)
+ *   dense Gaussian sketch values .............. pinned (pure-Python SplitMix64
+ *                                                stream, KS test against the
+ *                                                normal CDF, moments, pair and
+ *                                                neighbour correlations)
  *   Householder QR / triangular inverse ....... pinned (numpy.linalg.qr, R R^-1 = I)
  *   restarted LSQR ............................ pinned (known answers, numpy lstsq,
  *                                                scipy.sparse.linalg.lsqr)
@@ -507,6 +511,33 @@ ORC_API void orc_countsketch_apply_f64(int64_t m, int64_t n, const double* X, in
         double sg = (hs & 1ull) ? 1.0 : -1.0;
         for (int64_t j = 0; j < n; ++j) Y[b + j * ldy] += sg * X[i + j * ldx];
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* dense Gaussian sketch -- create_gaussian_sketch(Dense&),                  */
+/* src/dispatch/gaussian_sketch_registries.cpp:16-30: i.i.d. N(0,1) entries. */
+/* The reference draws them from a clock-seeded generator                    */
+/* (src/kernels/cpu/sketch/gaussian_sketch_kernels.cpp:10-18), so only the   */
+/* distribution is shared; this restates the library's seeded counter       */
+/* layout in FP64: entries (2q, j), (2q+1, j) are the Box-Muller pair of    */
+/* z = the p-th SplitMix64 output from `seed`, p = j*ceil(rows/2) + q,      */
+/* u1 = ((z >> 41) + 1/2) 2^-23, u2 = ((z >> 8) & 0xFFFFFF) 2^-24.          */
+/* ------------------------------------------------------------------------ */
+ORC_API void orc_gaussian_fill_f64(int64_t rows, int64_t cols, double* out, int64_t ld, uint64_t seed) {
+    const int64_t half = (rows + 1) / 2;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t q = 0; q < half; ++q) {
+            const uint64_t p = (uint64_t)j * (uint64_t)half + (uint64_t)q;
+            /* SplitMix64: state_p = seed + (p+1) gamma; orc_splitmix64 adds
+             * one gamma itself */
+            const uint64_t z = orc_splitmix64(seed + p * 0x9e3779b97f4a7c15ull);
+            const double u1 = ((double)(z >> 41) + 0.5) * 0x1p-23;
+            const double u2 = (double)((z >> 8) & 0xFFFFFFull) * 0x1p-24;
+            const double r = sqrt(-2.0 * log(u1));
+            out[2 * q + j * ld] = r * cos(two_pi * u2);
+            if (2 * q + 1 < rows) out[2 * q + 1 + j * ld] = r * sin(two_pi * u2);
+        }
 }
 
 /* ------------------------------------------------------------------------ */

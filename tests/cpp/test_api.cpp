@@ -369,6 +369,64 @@ int main() {
         EXPECT(throws<std::invalid_argument>(  // must be at b/x's precision
             [&] { lsqr_solve(ctx, A, b, x, ws, 1e-6, 8, 1e-6, nullptr, nullptr, 1, &A32); }));
     });
+    run("gaussian_sketch_registry_and_preconditioner", [] {
+        // create_gaussian_sketch (sketch.hpp:11-13): seeded values repeat,
+        // unseeded calls differ, 16-bit types are registered, packed are not
+        EXPECT(throws<std::runtime_error>(
+            [] { dispatch::GaussianSketch::get().lookup_create_dense(Device::HIP, NumericType::FP32_16); }));
+        EXPECT(throws<std::runtime_error>(
+            [] { dispatch::GaussianSketch::get().lookup_create_dense(Device::CPU, NumericType::FP32); }));
+        Dense h = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::BF16, {5, 3});
+        create_gaussian_sketch(h);
+        const std::size_t m = 6000, n = 60, k = 240;
+        Dense S1 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {k, m});
+        Dense S2 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {k, m});
+        create_gaussian_sketch(S1, 77);
+        create_gaussian_sketch(S2, 77);
+        auto v1 = fetch<float>(S1, k * m), v2 = fetch<float>(S2, k * m);
+        EXPECT(v1 == v2);
+        create_gaussian_sketch(S2);
+        auto v3 = fetch<float>(S2, k * m);
+        create_gaussian_sketch(S2);
+        auto v4 = fetch<float>(S2, k * m);
+        EXPECT(v3 != v4 && v3 != v1);
+        double mean = 0, var = 0;
+        for (float v : v1) mean += v;
+        mean /= double(v1.size());
+        for (float v : v1) var += (v - mean) * (v - mean);
+        var /= double(v1.size());
+        EXPECT(std::fabs(mean) < 0.005 && std::fabs(var - 1.0) < 0.01);
+        // the sketch as a preconditioner: S A by mm_apply, R from its QR
+        std::mt19937_64 gen(11);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::vector<float> a(m * n);
+        std::vector<double> xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j) {
+            const double d = std::pow(10.0, -4.0 * double(j) / double(n - 1));
+            for (std::size_t i = 0; i < m; ++i) a[i + j * m] = static_cast<float>(nd(gen) * d);
+        }
+        for (auto& v : xt) v = nd(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += double(a[i + j * m]) * xt[j];
+        Dense A = make(NumericType::FP32, m, n, a);
+        Dense Y = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {k, n});
+        mm_apply(Device::HIP, TransposeType::NO_TRANSPOSE, TransposeType::NO_TRANSPOSE, 1.0, S1, A, 0.0, Y);
+        auto y = fetch<float>(Y, k * n);
+        Dense sk = make(NumericType::FP64, k, n, std::vector<double>(y.begin(), y.end()));
+        CountSketch unused = create_count_sketch(k, m);
+        generate_sketch_qr_precond(unused, sk);
+        Dense P = Dense::wrap_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n}, k, sk.data());
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        int its[2] = {0, 0};
+        for (int pre = 0; pre < 2; ++pre) {
+            Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+            Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP32, NumericType::FP64);
+            if (pre) add_precond(P, ws);
+            lsqr_solve(ctx, A, b, x, ws, 1e-5, 2000, 1e-5, &its[pre], nullptr, 1);
+        }
+        std::printf("    iterations: plain %d, Gaussian-sketch preconditioned %d\n", its[0], its[1]);
+        EXPECT(its[1] > 0 && its[1] <= 40 && its[1] * 5 < its[0]);
+    });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

@@ -113,6 +113,7 @@ _SIGS = {
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
                                 sz, vp]),
+    "tomos_gaussian_fill": (cint, [cint, i64, i64, vp, i64, ctypes.c_uint64, vp]),
     "tomos_countsketch_indices": (cint, [i64, i64, vp, vp, vp]),
     "tomos_countsketch_scratch_size": (sz, [i64, i64, i64]),
     "tomos_countsketch_apply": (cint, [cint, cint, i64, i64, vp, vp, i64, i64, vp, i64, vp, sz,

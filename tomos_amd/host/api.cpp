@@ -1,6 +1,7 @@
 // BLAS front-ends, sketch/preconditioner, LSQR and the dispatch registries,
 // calling the gfx950 kernels through the libtomos_hip C ABI.
 #include <algorithm>
+#include <atomic>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -451,6 +452,51 @@ apply_count_sketch_dense_fn Sketch::lookup_apply_count_sketch_dense(Device dev, 
     return fn;
 }
 }  // namespace dispatch
+
+namespace {
+Status gaussian_dense_impl(tomos::Dense& S, std::uint64_t seed) {
+    device_guard(S.get_device(), "create_gaussian_sketch");
+    const auto sz = S.get_size();
+    check(tomos_gaussian_fill(abi_type(S.get_value_type()), static_cast<int64_t>(sz[0]),
+                              static_cast<int64_t>(sz[1]), S.data(), static_cast<int64_t>(S.get_lead_dim()),
+                              seed, S.stream()),
+          "create_gaussian_sketch");
+    return SUCCESS;
+}
+// Seeds of the unseeded overload: the SplitMix64 increments from a fixed
+// start, one per call.
+std::atomic<std::uint64_t> g_gauss_calls{0};
+}  // namespace
+
+namespace dispatch {
+std::size_t GaussianSketch::encode_dense(Device dev, NumericType type) {
+    return static_cast<std::size_t>(dev) * num_registry_numeric_types + static_cast<std::size_t>(type);
+}
+GaussianSketch::GaussianSketch() {
+    dense_.fill(nullptr);
+    for (NumericType t : {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16})
+        dense_[encode_dense(Device::HIP, t)] = &gaussian_dense_impl;
+}
+const GaussianSketch& GaussianSketch::get() {
+    static const GaussianSketch instance;
+    return instance;
+}
+create_gaussian_sketch_dense_fn GaussianSketch::lookup_create_dense(Device dev, NumericType type) const {
+    if (dev == Device::OMP) throw std::invalid_argument("create_gaussian_sketch: Device::OMP is not supported.");
+    auto fn = dense_[encode_dense(dev, type)];
+    if (!fn) throw std::runtime_error("create_gaussian_sketch: nothing registered for this device/type.");
+    return fn;
+}
+}  // namespace dispatch
+
+Status create_gaussian_sketch(tomos::Dense& S, std::uint64_t seed) {
+    return dispatch::GaussianSketch::get().lookup_create_dense(S.get_device(), S.get_value_type())(S, seed);
+}
+
+Status create_gaussian_sketch(tomos::Dense& S) {
+    const std::uint64_t k = g_gauss_calls.fetch_add(1);
+    return create_gaussian_sketch(S, 0x6a09e667f3bcc909ull + k * 0x9e3779b97f4a7c15ull);
+}
 
 CountSketch create_count_sketch(std::size_t k, std::size_t m) { return CountSketch{{k, m}}; }
 
