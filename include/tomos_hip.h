@@ -243,6 +243,41 @@ int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_p
 int tomos_countsketch_indices(int64_t m, int64_t s, int32_t* bucket, int8_t* sign,
                               tomos_stream_t stream);
 
+/* ---- Sparse (CSR) ------------------------------------------------------ */
+/* Index types of CSR row_ptrs / col_idxs (IntType, precision.hpp:42). */
+enum { TOMOS_INDEX_INT32 = 0, TOMOS_INDEX_INT64 = 1 };
+
+/* A CSR matrix: m x n, nnz entries; vals (val_type TOMOS_F64, TOMOS_F32,
+ * TOMOS_F16 or TOMOS_BF16), row_ptrs (m+1) and col_idxs (nnz) of idx_type
+ * (CsrStorage, src/core/matrix/sparse_storage.hpp:16-47).
+ * tomos_csr_check validates row_ptrs[0] = 0, non-decreasing row_ptrs,
+ * row_ptrs[m] = nnz and 0 <= col_idxs < n on the device (synchronous;
+ * TOMOS_ERR_INVALID_ARGUMENT with the reason). */
+int tomos_csr_check(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz, const void* row_ptrs,
+                    const void* col_idxs, tomos_stream_t stream);
+
+/* y = alpha*op(A)*x + beta*y, x/y TOMOS_F32 or TOMOS_F64, FP64
+ * accumulation; beta == 0 does not read y.  Replaces the mv_csr slots
+ * (sparse_registries.cpp:225-243; CPU semantics
+ * matrix_cpu_kernels.cpp:92-131).  NO_TRANSPOSE is asynchronous and
+ * memory-safe for any index contents (out-of-range entries contribute
+ * nothing); TRANSPOSE validates A, builds A^T in scratch
+ * (tomos_csr_mv_scratch_size bytes, or NULL for a library buffer) and
+ * applies it: deterministic, no atomics. */
+size_t tomos_csr_mv_scratch_size(int val_type, int idx_type, int trans, int64_t m, int64_t n, int64_t nnz);
+int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m, int64_t n, int64_t nnz,
+                 const void* vals, const void* row_ptrs, const void* col_idxs, int x_type, const void* x,
+                 double beta, int y_type, void* y, void* scratch, size_t scratch_bytes, tomos_stream_t stream);
+
+/* CSR of A^T (n x m: vals_t nnz, row_ptrs_t n+1, col_idxs_t nnz, same
+ * types) with each row of A^T in A's row order.  Validates A first.  No
+ * reference counterpart (the reference's transposed product scatters,
+ * matrix_cpu_kernels.cpp:116-130). */
+size_t tomos_csr_transpose_scratch_size(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz);
+int tomos_csr_transpose(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz, const void* vals,
+                        const void* row_ptrs, const void* col_idxs, void* vals_t, void* row_ptrs_t,
+                        void* col_idxs_t, void* scratch, size_t scratch_bytes, tomos_stream_t stream);
+
 /* Dense Gaussian sketch: out (rows x cols, ld, column-major; type
  * TOMOS_F64, TOMOS_F32, TOMOS_F16 or TOMOS_BF16) <- i.i.d. N(0,1).
  * Replaces cuda::generate_gaussian_values
@@ -360,6 +395,27 @@ int tomos_lsqr_solve_refined(const tomos_lsqr_options* opt, int64_t m, int64_t n
                              void* arena, size_t arena_bytes, void* scratch, size_t scratch_bytes,
                              int* iterations, double* history, int history_cap, int* history_len,
                              tomos_stream_t stream);
+
+/* tomos_lsqr_solve over a CSR matrix (lsqr_solve(Context, Sparse&, ...),
+ * include/core/solver.hpp:74-78; lsqr_sparse_impl,
+ * lsqr_registries.cpp:164-196).  opt->a_type is the CSR value type
+ * (TOMOS_F64, TOMOS_F32, TOMOS_F16 or TOMOS_BF16); every product
+ * accumulates in FP64, so mv_tier/restart_tier and fused are ignored.  A is
+ * validated (tomos_csr_check) and A^T is built in the scratch once per
+ * solve, so A^T u needs no atomics and iterates repeat bit for bit.
+ * ref_vals (ref_type TOMOS_F32/F64, or NULL): the refine-precision
+ * duplicate's values over A's pattern, read only by the restart residuals
+ * (mtx_refine_precision, lsqr_registries.cpp:180-183).  The arena is
+ * tomos_lsqr_arena_size(opt, m, n) bytes, the scratch
+ * tomos_lsqr_csr_scratch_size() bytes or NULL.  Synchronous. */
+size_t tomos_lsqr_csr_scratch_size(const tomos_lsqr_options* opt, int idx_type, int64_t m, int64_t n,
+                                   int64_t nnz);
+int tomos_lsqr_solve_csr(const tomos_lsqr_options* opt, int idx_type, int64_t m, int64_t n, int64_t nnz,
+                         const void* vals, const void* row_ptrs, const void* col_idxs, int ref_type,
+                         const void* ref_vals, const void* b, void* x, const void* rinv, int64_t ldr,
+                         void* arena, size_t arena_bytes, void* scratch, size_t scratch_bytes,
+                         int* iterations, double* history, int history_cap, int* history_len,
+                         tomos_stream_t stream);
 
 /* ---- Many uneven LSQR problems in one set of launches ----------------- */
 /* One problem: A (m x n, lda; for packed storage the hi plane, `wr` the lo

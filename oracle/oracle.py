@@ -83,6 +83,10 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int8)]
         L.orc_countsketch_apply_f64.restype = None
         L.orc_countsketch_apply_f64.argtypes = [i64, i64, f64p, i64, i64, f64p, i64]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.orc_csr_mv.restype = None
+        L.orc_csr_mv.argtypes = [ctypes.c_int, ctypes.c_double, i64, i64, f64p, i64p, i64p, f64p,
+                                 ctypes.c_double, f64p]
         L.orc_gaussian_fill_f64.restype = None
         L.orc_gaussian_fill_f64.argtypes = [i64, i64, f64p, i64, ctypes.c_uint64]
         L.orc_householder_qr.restype = ctypes.c_int
@@ -172,6 +176,19 @@ def countsketch_apply(X, s):
     Y = np.zeros((s, n), np.float64, order="F")
     lib().orc_countsketch_apply_f64(m, n, _p(X, f64p), max(m, 1), s, _p(Y, f64p), s)
     return Y
+
+
+def csr_mv(trans, alpha, m, n, vals, row_ptrs, col_idxs, x, beta=0.0, y=None):
+    """y = alpha op(A) x + beta y for CSR A (FP64 arithmetic)."""
+    vals = np.ascontiguousarray(vals, np.float64)
+    rp = np.ascontiguousarray(row_ptrs, np.int64)
+    ci = np.ascontiguousarray(col_idxs, np.int64)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(n if trans else m, np.float64) if y is None else np.array(y, np.float64)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    lib().orc_csr_mv(int(trans), alpha, m, n, _p(vals, f64p), _p(rp, i64p), _p(ci, i64p), _p(x, f64p),
+                     beta, _p(out, f64p))
+    return out
 
 
 def gaussian_fill(rows, cols, seed):

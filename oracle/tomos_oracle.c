@@ -15,6 +15,7 @@
  *   pack / decode (FP32_16, FP32_BF16) ........ pinned (exhaustive + numpy/torch)
  *   CPU GEMV tiers (accurate/fast/pure_half) .. pinned (numpy fp64 / numpy fp16)
  *   hashed CountSketch ........................ pinned (pure-Python splitmix64)
+ *   CSR products (A x, A^T x) ................. pinned (scipy.sparse, dense numpy)
  *   dense Gaussian sketch values .............. pinned (pure-Python SplitMix64
  *                                                stream, KS test against the
  *                                                normal CDF, moments, pair and
@@ -510,6 +511,29 @@ ORC_API void orc_countsketch_apply_f64(int64_t m, int64_t n, const double* X, in
         int64_t b = (int64_t)(hb % (uint64_t)s);
         double sg = (hs & 1ull) ? 1.0 : -1.0;
         for (int64_t j = 0; j < n; ++j) Y[b + j * ldy] += sg * X[i + j * ldx];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR y = alpha op(A) x + beta y -- cpu::mv_csr,                            */
+/* src/kernels/cpu/matrix/matrix_cpu_kernels.cpp:92-131, computed in FP64    */
+/* (the reference's compute_t for any FP64 operand); beta == 0 leaves y     */
+/* unread.  int64 indices, FP64 values: callers convert.                    */
+/* ------------------------------------------------------------------------ */
+ORC_API void orc_csr_mv(int trans, double alpha, int64_t m, int64_t n, const double* vals,
+                        const int64_t* row_ptrs, const int64_t* col_idxs, const double* x,
+                        double beta, double* y) {
+    if (!trans) {
+        for (int64_t i = 0; i < m; ++i) {
+            double acc = 0.0;
+            for (int64_t k = row_ptrs[i]; k < row_ptrs[i + 1]; ++k) acc += vals[k] * x[col_idxs[k]];
+            y[i] = beta != 0.0 ? alpha * acc + beta * y[i] : alpha * acc;
+        }
+    } else {
+        for (int64_t j = 0; j < n; ++j) y[j] = beta != 0.0 ? beta * y[j] : 0.0;
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t k = row_ptrs[i]; k < row_ptrs[i + 1]; ++k)
+                y[col_idxs[k]] += alpha * vals[k] * x[i];
     }
 }
 

@@ -137,3 +137,91 @@ def sketch_rinv(a_type, tier, m, n, a_buf, wr_buf, lda, s):
     C.call("tomos_sketch_qr_precond_f64", s, n, y.ptr, s, None)
     full = y.to_numpy(np.float64, (s, n))
     return np.triu(full[:n, :n]), y
+
+
+# ---- CSR -----------------------------------------------------------------------
+VAL_NP = {C.F64: np.float64, C.F32: np.float32, C.F16: np.float16}
+
+
+def csr_vals_bits(vt, vals):
+    """Host values in the device encoding of vt (bf16 rounds to nearest even)."""
+    v = np.asarray(vals, np.float64)
+    if vt == C.BF16:
+        u = v.astype(np.float32).view(np.uint32).astype(np.uint64)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return v.astype(VAL_NP[vt])
+
+
+def csr_decoded(vt, vals):
+    """The stored values as FP64 (what the device products multiply)."""
+    b = csr_vals_bits(vt, vals)
+    if vt == C.BF16:
+        return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return b.astype(np.float64)
+
+
+class Csr:
+    """A CSR matrix on the device (values encoded as vt, indices as it)."""
+
+    def __init__(self, m, n, row_ptrs, col_idxs, vals, vt=C.F64, it=C.INDEX_INT32):
+        idt = np.int64 if it == C.INDEX_INT64 else np.int32
+        self.m, self.n, self.vt, self.it = m, n, vt, it
+        self.nnz = len(col_idxs)
+        self.rp = dev(np.asarray(row_ptrs, idt))
+        self.ci = dev(np.asarray(col_idxs, idt)) if self.nnz else DeviceBuffer(8)
+        self.v = dev(csr_vals_bits(vt, vals)) if self.nnz else DeviceBuffer(8)
+        self.host = (np.asarray(row_ptrs, np.int64), np.asarray(col_idxs, np.int64), csr_decoded(vt, vals))
+
+    def args(self):
+        return (self.vt, self.it, self.m, self.n, self.nnz)
+
+
+def csr_mv(A, trans, alpha, x, beta=0.0, y0=None, xt=C.F64, yt=C.F64):
+    xdt = np.float64 if xt == C.F64 else np.float32
+    ydt = np.float64 if yt == C.F64 else np.float32
+    ylen = A.n if trans else A.m
+    xd = dev(np.asarray(x, xdt)) if len(x) else DeviceBuffer(8)
+    yd = dev(np.asarray(y0, ydt)) if y0 is not None else DeviceBuffer(max(ylen, 1) * np.dtype(ydt).itemsize)
+    C.call("tomos_csr_mv", A.vt, A.it, trans, alpha, A.m, A.n, A.nnz, A.v.ptr, A.rp.ptr, A.ci.ptr, xt, xd.ptr,
+           beta, yt, yd.ptr, None, 0, None)
+    return yd.to_numpy(ydt, (ylen,))
+
+
+def csr_transpose(A):
+    idt = np.int64 if A.it == C.INDEX_INT64 else np.int32
+    vdt = np.uint16 if A.vt == C.BF16 else VAL_NP[A.vt]
+    vt, pt, it = DeviceBuffer(max(A.nnz, 1) * 8), DeviceBuffer((A.n + 1) * 8), DeviceBuffer(max(A.nnz, 1) * 8)
+    C.call("tomos_csr_transpose", *A.args(), A.v.ptr, A.rp.ptr, A.ci.ptr, vt.ptr, pt.ptr, it.ptr, None, 0, None)
+    return (pt.to_numpy(idt, (A.n + 1,)), it.to_numpy(idt, (A.nnz,)), vt.to_numpy(vdt, (A.nnz,)))
+
+
+def lsqr_csr(A, b, value_type=C.F64, refine_type=C.F64, rinv=None, precond_type=-1, tol=1e-10,
+             max_iters=400, max_relres=None, restarts=1, check_every=1, ref=None, history_cap=100000):
+    """tomos_lsqr_solve_csr; ref = (ref_type, values) for the refine duplicate."""
+    if max_relres is None:
+        max_relres = tol
+    rdt = np.float64 if refine_type == C.F64 else np.float32
+    bd = dev(np.asarray(b, dtype=rdt))
+    xd = DeviceBuffer(max(A.n, 1) * np.dtype(rdt).itemsize)
+    rd = None
+    if rinv is not None:
+        pdt = np.float64 if precond_type == C.F64 else np.float32
+        rd = dev(np.asarray(rinv, dtype=pdt))
+    opt = C.LsqrOptions(A.vt, C.TIER_DEFAULT, C.TIER_DEFAULT, value_type, refine_type, precond_type,
+                        tol, max_relres, max_iters, restarts, check_every, 0)
+    ab = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), A.m, A.n)
+    sb = C.lib().tomos_lsqr_csr_scratch_size(ctypes.byref(opt), A.it, A.m, A.n, A.nnz)
+    assert ab > 0 and sb > 0, C.lib().tomos_last_error()
+    arena, scr = DeviceBuffer(ab), DeviceBuffer(sb)
+    it = ctypes.c_int(0)
+    hl = ctypes.c_int(0)
+    hist = np.zeros(history_cap, np.float64)
+    rt, rv = -1, None
+    if ref is not None:
+        rt = ref[0]
+        rv = dev(np.asarray(ref[1], np.float64 if rt == C.F64 else np.float32))
+    C.call("tomos_lsqr_solve_csr", ctypes.byref(opt), A.it, A.m, A.n, A.nnz, A.v.ptr, A.rp.ptr, A.ci.ptr, rt,
+           rv.ptr if rv is not None else None, bd.ptr, xd.ptr, rd.ptr if rd is not None else None, max(A.n, 1),
+           arena.ptr, ab, scr.ptr, sb, ctypes.byref(it), hist.ctypes.data, history_cap, ctypes.byref(hl), None)
+    x = xd.to_numpy(rdt, (A.n,))
+    return x, it.value, hist[:min(hl.value, history_cap)].copy()

@@ -29,6 +29,7 @@ PACK_FP32_BF16, PACK_FP32_BF16_ROUNDED = 3, 4
  DECODE_FP32_BF16_EXACT, DECODE_FP32_BF16_HI, DECODE_FP32_BF16_EXACT_ROUNDED,
  DECODE_FP32_16_EXACT_TRUNCATED) = range(8)
 TIER_DEFAULT, TIER_FAST, TIER_EXACT = 0, 1, 2
+INDEX_INT32, INDEX_INT64 = 0, 1
 ERR_NAMES = {1: "INVALID_ARGUMENT", 2: "HIP", 3: "UNSUPPORTED", 4: "WORKSPACE",
              5: "NOT_CONVERGED", 6: "SINGULAR", 7: "INTERNAL"}
 
@@ -114,6 +115,12 @@ _SIGS = {
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
                                 sz, vp]),
     "tomos_gaussian_fill": (cint, [cint, i64, i64, vp, i64, ctypes.c_uint64, vp]),
+    "tomos_csr_check": (cint, [cint, cint, i64, i64, i64, vp, vp, vp]),
+    "tomos_csr_mv_scratch_size": (sz, [cint, cint, cint, i64, i64, i64]),
+    "tomos_csr_mv": (cint, [cint, cint, cint, ctypes.c_double, i64, i64, i64, vp, vp, vp, cint, vp,
+                            ctypes.c_double, cint, vp, vp, sz, vp]),
+    "tomos_csr_transpose_scratch_size": (sz, [cint, cint, i64, i64, i64]),
+    "tomos_csr_transpose": (cint, [cint, cint, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
     "tomos_countsketch_indices": (cint, [i64, i64, vp, vp, vp]),
     "tomos_countsketch_scratch_size": (sz, [i64, i64, i64]),
     "tomos_countsketch_apply": (cint, [cint, cint, i64, i64, vp, vp, i64, i64, vp, i64, vp, sz,
@@ -130,6 +137,10 @@ _SIGS = {
     "tomos_lsqr_solve_refined": (cint, [ctypes.POINTER(LsqrOptions), i64, i64, vp, vp, i64, cint, vp, i64, vp,
                                         vp, vp, i64, vp, sz, vp, sz, ctypes.POINTER(cint), vp, cint,
                                         ctypes.POINTER(cint), vp]),
+    "tomos_lsqr_csr_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), cint, i64, i64, i64]),
+    "tomos_lsqr_solve_csr": (cint, [ctypes.POINTER(LsqrOptions), cint, i64, i64, i64, vp, vp, vp, cint, vp, vp,
+                                    vp, vp, i64, vp, sz, vp, sz, ctypes.POINTER(cint), vp, cint,
+                                    ctypes.POINTER(cint), vp]),
 }
 
 _lib = None

@@ -1,0 +1,437 @@
+// CSR sparse matrix-vector products (the reference's mv_csr slots,
+// src/dispatch/sparse_registries.cpp:225-243; CPU semantics
+// src/kernels/cpu/matrix/matrix_cpu_kernels.cpp:92-131) and the CSR
+// transpose that lets LSQR form A^T u without atomics.
+//
+// A*x: L lanes of a wavefront share one row (L in {1, 4, 16, 64}, from the
+// mean row length), each lane sums a lane-strided part of the row in FP64
+// and the group folds its partial sums with a butterfly of lane shuffles.
+// The order of every sum depends only on (m, nnz), so results repeat bit
+// for bit.  Bound: HBM, nnz*(value + index bytes) + (m+1) index bytes + the
+// y traffic, with x gathered through the caches.
+//
+// A^T*x: the reference scatters into y (matrix_cpu_kernels.cpp:116-130);
+// here A^T is built once as a CSR matrix of its own (a stable radix sort of
+// the column indices keeps A's row order inside each row of A^T) and the
+// A*x kernel runs on it: no atomics, deterministic, coalesced.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "codec.hpp"
+#include "common.hpp"
+#include "csr.hpp"
+#include "tomos_hip.h"
+
+namespace tomos_hip {
+namespace {
+
+constexpr int kCsrThreads = 256;
+constexpr int64_t kCsrMaxBlocks = 1024;  // also bounds the sum-of-squares partials
+
+template <int VT>
+__device__ __forceinline__ double load_val(const void* v, int64_t k) {
+    if constexpr (VT == TOMOS_F64) return static_cast<const double*>(v)[k];
+    else if constexpr (VT == TOMOS_F32) return static_cast<const float*>(v)[k];
+    else if constexpr (VT == TOMOS_F16) return codec::half_bits_to_f32(static_cast<const uint16_t*>(v)[k]);
+    else return codec::u2f(static_cast<uint32_t>(static_cast<const uint16_t*>(v)[k]) << 16);
+}
+
+template <class I>
+__device__ __forceinline__ int64_t clamp_ptr(I p, int64_t nnz) {
+    const int64_t q = static_cast<int64_t>(p);
+    return q < 0 ? 0 : (q > nnz ? nnz : q);
+}
+
+struct MvArgs {
+    int64_t m, n, nnz;
+    const void* vals;
+    const void* rp;
+    const void* ci;
+    const void* x;
+    void* y;
+    bool x64, y64;
+    Coef c;
+};
+
+template <int VT, class I, int L>
+__global__ __launch_bounds__(kCsrThreads) void csr_mv_kernel(MvArgs a) {
+    if (a.c.skip && *a.c.skip) return;
+    const double alpha = a.c.dalpha ? *a.c.dalpha * a.c.alpha_mul : a.c.alpha;
+    const double beta = a.c.dbeta ? *a.c.dbeta * a.c.beta_mul : a.c.beta;
+    const I* rp = static_cast<const I*>(a.rp);
+    const I* ci = static_cast<const I*>(a.ci);
+    const int lane = threadIdx.x % L;
+    const int64_t groups = static_cast<int64_t>(gridDim.x) * (kCsrThreads / L);
+    double sq = 0.0;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * (kCsrThreads / L) + threadIdx.x / L; row < a.m;
+         row += groups) {
+        const int64_t k0 = clamp_ptr(rp[row], a.nnz);
+        int64_t k1 = clamp_ptr(rp[row + 1], a.nnz);
+        k1 = k1 < k0 ? k0 : k1;
+        double acc = 0.0;
+        for (int64_t k = k0 + lane; k < k1; k += L) {
+            const uint64_t col = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
+            if (col < static_cast<uint64_t>(a.n)) {
+                const double xv = a.x64 ? static_cast<const double*>(a.x)[col]
+                                        : static_cast<double>(static_cast<const float*>(a.x)[col]);
+                acc = fma(load_val<VT>(a.vals, k), xv, acc);
+            }
+        }
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+        if (lane == 0) {
+            double out = alpha * acc;
+            if (beta != 0.0)
+                out += beta * (a.y64 ? static_cast<double*>(a.y)[row]
+                                     : static_cast<double>(static_cast<float*>(a.y)[row]));
+            if (a.y64) {
+                static_cast<double*>(a.y)[row] = out;
+                sq += out * out;
+            } else {
+                const float o32 = static_cast<float>(out);
+                static_cast<float*>(a.y)[row] = o32;
+                sq += static_cast<double>(o32) * o32;
+            }
+        }
+    }
+    if (a.c.sumsq) {
+        __shared__ double part[kCsrThreads / 64];
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = sq;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kCsrThreads / 64; ++w) t += part[w];
+            a.c.sumsq[blockIdx.x] = t;
+        }
+    }
+}
+
+int lanes_for(int64_t m, int64_t nnz) {
+    const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
+    return mean <= 2.0 ? 1 : mean <= 12.0 ? 4 : mean <= 48.0 ? 16 : 64;
+}
+
+int64_t mv_blocks(int64_t m, int lanes) {
+    const int64_t b = ceil_div(std::max<int64_t>(m, 1) * lanes, kCsrThreads);
+    return std::min<int64_t>(b, kCsrMaxBlocks);
+}
+
+template <int VT, class I>
+void launch_mv_vt(const MvArgs& a, int lanes, unsigned blocks, hipStream_t s) {
+    switch (lanes) {
+        case 1: csr_mv_kernel<VT, I, 1><<<blocks, kCsrThreads, 0, s>>>(a); break;
+        case 4: csr_mv_kernel<VT, I, 4><<<blocks, kCsrThreads, 0, s>>>(a); break;
+        case 16: csr_mv_kernel<VT, I, 16><<<blocks, kCsrThreads, 0, s>>>(a); break;
+        default: csr_mv_kernel<VT, I, 64><<<blocks, kCsrThreads, 0, s>>>(a); break;
+    }
+}
+
+template <class I>
+void launch_mv(int vt, const MvArgs& a, int lanes, unsigned blocks, hipStream_t s) {
+    switch (vt) {
+        case TOMOS_F64: launch_mv_vt<TOMOS_F64, I>(a, lanes, blocks, s); break;
+        case TOMOS_F32: launch_mv_vt<TOMOS_F32, I>(a, lanes, blocks, s); break;
+        case TOMOS_F16: launch_mv_vt<TOMOS_F16, I>(a, lanes, blocks, s); break;
+        default: launch_mv_vt<TOMOS_BF16, I>(a, lanes, blocks, s); break;
+    }
+}
+
+// ---- validation -------------------------------------------------------------
+// flag bits: 1 row_ptrs[0] != 0, 2 decreasing row_ptrs, 4 row_ptrs[m] != nnz,
+// 8 column index out of range
+template <class I>
+__global__ void csr_check_kernel(int64_t m, int64_t n, int64_t nnz, const I* rp, const I* ci, unsigned* flag) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    unsigned f = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= m; i += stride) {
+        const int64_t p = static_cast<int64_t>(rp[i]);
+        if (i == 0 && p != 0) f |= 1u;
+        if (i < m && static_cast<int64_t>(rp[i + 1]) < p) f |= 2u;
+        if (i == m && p != nnz) f |= 4u;
+    }
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+        const int64_t c = static_cast<int64_t>(ci[k]);
+        if (c < 0 || c >= n) f |= 8u;
+    }
+    if (f) atomicOr(flag, f);
+}
+
+// ---- transpose ----------------------------------------------------------------
+// row index of every stored entry (rows[k] = i for rp[i] <= k < rp[i+1])
+template <class I>
+__global__ void csr_row_of_kernel(int64_t m, const I* rp, I* rows) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride)
+        for (int64_t k = static_cast<int64_t>(rp[i]); k < static_cast<int64_t>(rp[i + 1]); ++k)
+            rows[k] = static_cast<I>(i);
+}
+
+// ptr_t[c] = first sorted position with key >= c, from the boundaries of
+// the sorted keys (every c in (keys[k-1], keys[k]] gets k).
+template <class U>
+__global__ void csr_ptr_from_keys_kernel(int64_t nnz, int64_t n, const U* keys, U* ptr_t) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k <= nnz; k += stride) {
+        const int64_t prev = k == 0 ? -1 : static_cast<int64_t>(keys[k - 1]);
+        int64_t cur = k == nnz ? n : static_cast<int64_t>(keys[k]);
+        cur = cur > n ? n : cur;
+        for (int64_t c = prev + 1; c <= cur; ++c) ptr_t[c] = static_cast<U>(k);
+    }
+}
+
+template <int VT, class U>
+__global__ void csr_gather_kernel(int64_t nnz, const U* perm, const void* vals, const U* rows, void* vals_t,
+                                  U* idx_t) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+        const U p = perm[k];
+        idx_t[k] = rows[p];
+        if constexpr (VT == TOMOS_F64) static_cast<double*>(vals_t)[k] = static_cast<const double*>(vals)[p];
+        else if constexpr (VT == TOMOS_F32) static_cast<float*>(vals_t)[k] = static_cast<const float*>(vals)[p];
+        else static_cast<uint16_t*>(vals_t)[k] = static_cast<const uint16_t*>(vals)[p];
+    }
+}
+
+unsigned grid_1d(int64_t work) {
+    return static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(ceil_div(work, 256), 1), 8192));
+}
+
+unsigned key_bits(int64_t n) {
+    unsigned b = 1;
+    while (b < 63 && (int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <class U>
+size_t sort_temp_bytes(int64_t nnz, int64_t n) {
+    size_t bytes = 0;
+    rocprim::counting_iterator<U> iota(U(0));
+    TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const U*>(nullptr),
+                                              static_cast<U*>(nullptr), iota, static_cast<U*>(nullptr),
+                                              static_cast<size_t>(nnz), 0, key_bits(n)));
+    return bytes;
+}
+
+template <class U>
+size_t transpose_bytes(int64_t nnz, int64_t n) {
+    const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1)) * sizeof(U);
+    return 3 * align256(e) + align256(sort_temp_bytes<U>(nnz, n));
+}
+
+template <class U>
+void transpose_impl(const CsrMat& A, void* vals_t, void* ptr_t, void* idx_t, void* scratch, hipStream_t s) {
+    const size_t e = static_cast<size_t>(std::max<int64_t>(A.nnz, 1)) * sizeof(U);
+    char* sc = static_cast<char*>(scratch);
+    U* keys_out = reinterpret_cast<U*>(sc);
+    U* perm = reinterpret_cast<U*>(sc + align256(e));
+    U* rows = reinterpret_cast<U*>(sc + 2 * align256(e));
+    void* temp = sc + 3 * align256(e);
+    size_t temp_bytes = sort_temp_bytes<U>(A.nnz, A.n);
+    if (A.nnz > 0) {
+        csr_row_of_kernel<U><<<grid_1d(A.m), 256, 0, s>>>(A.m, static_cast<const U*>(A.row_ptrs), rows);
+        TOMOS_LAUNCH_CHECK("csr_row_of_kernel");
+        rocprim::counting_iterator<U> iota(U(0));
+        TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(temp, temp_bytes, static_cast<const U*>(A.col_idxs), keys_out,
+                                                  iota, perm, static_cast<size_t>(A.nnz), 0, key_bits(A.n), s));
+    }
+    csr_ptr_from_keys_kernel<U><<<grid_1d(A.nnz + 1), 256, 0, s>>>(A.nnz, A.n, keys_out, static_cast<U*>(ptr_t));
+    TOMOS_LAUNCH_CHECK("csr_ptr_from_keys_kernel");
+    if (A.nnz == 0) return;
+    const unsigned g = grid_1d(A.nnz);
+    U* it = static_cast<U*>(idx_t);
+    switch (A.val_type) {
+        case TOMOS_F64: csr_gather_kernel<TOMOS_F64, U><<<g, 256, 0, s>>>(A.nnz, perm, A.vals, rows, vals_t, it); break;
+        case TOMOS_F32: csr_gather_kernel<TOMOS_F32, U><<<g, 256, 0, s>>>(A.nnz, perm, A.vals, rows, vals_t, it); break;
+        default: csr_gather_kernel<TOMOS_F16, U><<<g, 256, 0, s>>>(A.nnz, perm, A.vals, rows, vals_t, it); break;
+    }
+    TOMOS_LAUNCH_CHECK("csr_gather_kernel");
+}
+
+}  // namespace
+
+size_t csr_value_bytes(int t) { return t == TOMOS_F64 ? 8 : t == TOMOS_F32 ? 4 : 2; }
+size_t csr_index_bytes(int t) { return t == TOMOS_INDEX_INT64 ? 8 : 4; }
+
+void csr_require_valid_types(const CsrMat& A, const char* who) {
+    const std::string w(who);
+    require(A.val_type == TOMOS_F64 || A.val_type == TOMOS_F32 || A.val_type == TOMOS_F16 ||
+                A.val_type == TOMOS_BF16,
+            w + ": CSR values must be F64, F32, F16 or BF16");
+    require(A.idx_type == TOMOS_INDEX_INT32 || A.idx_type == TOMOS_INDEX_INT64,
+            w + ": CSR index type must be INT32 or INT64");
+    require(A.m >= 0 && A.n >= 0 && A.nnz >= 0, w + ": negative size");
+    require(A.idx_type == TOMOS_INDEX_INT64 || (A.nnz <= 0x7FFFFFFF && A.m <= 0x7FFFFFFF && A.n <= 0x7FFFFFFF),
+            w + ": sizes exceed INT32 indices");
+    require(A.row_ptrs != nullptr || A.m == 0, w + ": null row_ptrs");
+    require((A.vals && A.col_idxs) || A.nnz == 0, w + ": null values or column indices");
+}
+
+int64_t csr_sumsq_blocks_max(int64_t m) { return std::min<int64_t>(ceil_div(std::max<int64_t>(m, 1), 4), kCsrMaxBlocks); }
+
+void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_type, void* y, hipStream_t s,
+            int* sumsq_blocks) {
+    require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "csr_mv: x must be F32 or F64");
+    require(y_type == TOMOS_F32 || y_type == TOMOS_F64, "csr_mv: y must be F32 or F64");
+    if (A.m == 0) {
+        if (sumsq_blocks) *sumsq_blocks = 0;
+        return;
+    }
+    require(y != nullptr && (x != nullptr || A.n == 0), "csr_mv: null x or y");
+    MvArgs a{A.m, A.n, A.nnz, A.vals, A.row_ptrs, A.col_idxs, x, y, x_type == TOMOS_F64, y_type == TOMOS_F64, c};
+    const int lanes = lanes_for(A.m, A.nnz);
+    const unsigned blocks = static_cast<unsigned>(mv_blocks(A.m, lanes));
+    if (A.idx_type == TOMOS_INDEX_INT64) launch_mv<int64_t>(A.val_type, a, lanes, blocks, s);
+    else launch_mv<int32_t>(A.val_type, a, lanes, blocks, s);
+    TOMOS_LAUNCH_CHECK("csr_mv_kernel");
+    if (sumsq_blocks) *sumsq_blocks = static_cast<int>(blocks);
+}
+
+size_t csr_check_scratch_bytes() { return 256; }
+
+void csr_check(const CsrMat& A, void* scratch, size_t scratch_bytes, hipStream_t s) {
+    require(scratch_bytes >= csr_check_scratch_bytes(), "csr_check: scratch too small");
+    unsigned* flag = static_cast<unsigned*>(scratch);
+    TOMOS_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(unsigned), s));
+    const unsigned g = grid_1d(std::max<int64_t>(A.m + 1, A.nnz));
+    if (A.idx_type == TOMOS_INDEX_INT64)
+        csr_check_kernel<int64_t><<<g, 256, 0, s>>>(A.m, A.n, A.nnz, static_cast<const int64_t*>(A.row_ptrs),
+                                                    static_cast<const int64_t*>(A.col_idxs), flag);
+    else
+        csr_check_kernel<int32_t><<<g, 256, 0, s>>>(A.m, A.n, A.nnz, static_cast<const int32_t*>(A.row_ptrs),
+                                                    static_cast<const int32_t*>(A.col_idxs), flag);
+    TOMOS_LAUNCH_CHECK("csr_check_kernel");
+    unsigned f = 0;
+    TOMOS_HIP_CHECK(hipMemcpyAsync(&f, flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    require(!(f & 1u), "CSR: row_ptrs[0] must be 0");
+    require(!(f & 2u), "CSR: row_ptrs must be non-decreasing");
+    require(!(f & 4u), "CSR: row_ptrs[m] must equal nnz");
+    require(!(f & 8u), "CSR: column index out of range");
+}
+
+size_t csr_transpose_scratch_bytes(const CsrMat& A) {
+    return A.idx_type == TOMOS_INDEX_INT64 ? transpose_bytes<uint64_t>(A.nnz, A.n)
+                                           : transpose_bytes<uint32_t>(A.nnz, A.n);
+}
+
+void csr_transpose(const CsrMat& A, void* vals_t, void* ptr_t, void* idx_t, void* scratch, size_t scratch_bytes,
+                   hipStream_t s) {
+    require(scratch_bytes >= csr_transpose_scratch_bytes(A), "csr_transpose: scratch too small");
+    require(ptr_t != nullptr && ((vals_t && idx_t) || A.nnz == 0), "csr_transpose: null output");
+    // the index buffers hold non-negative values: sorted as unsigned
+    if (A.idx_type == TOMOS_INDEX_INT64) transpose_impl<uint64_t>(A, vals_t, ptr_t, idx_t, scratch, s);
+    else transpose_impl<uint32_t>(A, vals_t, ptr_t, idx_t, scratch, s);
+}
+
+}  // namespace tomos_hip
+
+using namespace tomos_hip;
+
+namespace {
+CsrMat make_csr(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz, const void* vals, const void* rp,
+                const void* ci) {
+    CsrMat A;
+    A.val_type = val_type;
+    A.idx_type = idx_type;
+    A.m = m;
+    A.n = n;
+    A.nnz = nnz;
+    A.vals = vals;
+    A.row_ptrs = rp;
+    A.col_idxs = ci;
+    return A;
+}
+}  // namespace
+
+extern "C" {
+
+int tomos_csr_check(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz, const void* row_ptrs,
+                    const void* col_idxs, tomos_stream_t st) {
+    return guarded([&] {
+        const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, nullptr, row_ptrs, col_idxs);
+        CsrMat T = A;
+        T.vals = nnz ? col_idxs : nullptr;  // values are not read
+        csr_require_valid_types(T, "tomos_csr_check");
+        csr_check(A, fallback_scratch(csr_check_scratch_bytes()), csr_check_scratch_bytes(), st);
+    });
+}
+
+size_t tomos_csr_transpose_scratch_size(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz) {
+    const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, nullptr, nullptr, nullptr);
+    if (m < 0 || n < 0 || nnz < 0) return 0;
+    try {
+        return csr_transpose_scratch_bytes(A);
+    } catch (...) {
+        return 0;
+    }
+}
+
+int tomos_csr_transpose(int val_type, int idx_type, int64_t m, int64_t n, int64_t nnz, const void* vals,
+                        const void* row_ptrs, const void* col_idxs, void* vals_t, void* row_ptrs_t, void* col_idxs_t,
+                        void* scratch, size_t scratch_bytes, tomos_stream_t st) {
+    return guarded([&] {
+        const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, vals, row_ptrs, col_idxs);
+        csr_require_valid_types(A, "tomos_csr_transpose");
+        csr_check(A, fallback_scratch(csr_check_scratch_bytes()), csr_check_scratch_bytes(), st);
+        const size_t need = csr_transpose_scratch_bytes(A);
+        if (!scratch) {
+            scratch = fallback_scratch(need);
+            scratch_bytes = need;
+        }
+        csr_transpose(A, vals_t, row_ptrs_t, col_idxs_t, scratch, scratch_bytes, st);
+    });
+}
+
+size_t tomos_csr_mv_scratch_size(int val_type, int idx_type, int trans, int64_t m, int64_t n, int64_t nnz) {
+    if (m < 0 || n < 0 || nnz < 0) return 0;
+    if (!trans) return 256;
+    const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, nullptr, nullptr, nullptr);
+    try {
+        const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+        return align256(e * csr_value_bytes(val_type)) + align256(e * csr_index_bytes(idx_type)) +
+               align256(static_cast<size_t>(n + 1) * csr_index_bytes(idx_type)) + csr_transpose_scratch_bytes(A);
+    } catch (...) {
+        return 0;
+    }
+}
+
+int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m, int64_t n, int64_t nnz,
+                 const void* vals, const void* row_ptrs, const void* col_idxs, int x_type, const void* x, double beta,
+                 int y_type, void* y, void* scratch, size_t scratch_bytes, tomos_stream_t st) {
+    return guarded([&] {
+        const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, vals, row_ptrs, col_idxs);
+        csr_require_valid_types(A, "tomos_csr_mv");
+        require(trans == TOMOS_NO_TRANSPOSE || trans == TOMOS_TRANSPOSE, "tomos_csr_mv: bad trans");
+        Coef c;
+        c.alpha = alpha;
+        c.beta = beta;
+        if (trans == TOMOS_NO_TRANSPOSE) {
+            csr_mv(A, c, x_type, x, y_type, y, st);
+            return;
+        }
+        // A^T x: transpose into scratch, then the row kernel on A^T
+        const size_t need = tomos_csr_mv_scratch_size(val_type, idx_type, trans, m, n, nnz);
+        if (!scratch) {
+            scratch = fallback_scratch(need);
+            scratch_bytes = need;
+        }
+        require(scratch_bytes >= need, "tomos_csr_mv: scratch too small");
+        csr_check(A, scratch, scratch_bytes, st);
+        const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+        char* sc = static_cast<char*>(scratch);
+        void* vt = sc;
+        sc += align256(e * csr_value_bytes(val_type));
+        void* it = sc;
+        sc += align256(e * csr_index_bytes(idx_type));
+        void* pt = sc;
+        sc += align256(static_cast<size_t>(n + 1) * csr_index_bytes(idx_type));
+        csr_transpose(A, vt, pt, it, sc, csr_transpose_scratch_bytes(A), st);
+        const CsrMat T = make_csr(val_type, idx_type, n, m, nnz, vt, pt, it);
+        csr_mv(T, c, x_type, x, y_type, y, st);
+    });
+}
+
+}  // extern "C"

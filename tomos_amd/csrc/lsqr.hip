@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "csr.hpp"
 #include "gemv.hpp"
 #include "lsqr_fused.hpp"
 #include "tomos_hip.h"
@@ -278,6 +279,9 @@ struct Layout {
     FusedPlan fplan;
     size_t gemv_bytes;
     int hist_cap;
+    // CSR A: A^T as CSR (values, row pointers, column indices) and the
+    // transpose's working space
+    size_t csr_vals = 0, csr_ptr = 0, csr_idx = 0, csr_tmp = 0, csr_tmp_bytes = 0;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -345,6 +349,23 @@ Layout layout_for(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     return L;
 }
 
+// Dense layout plus A^T for a CSR matrix (the single-pass kernels are
+// dense-only).
+Layout csr_layout_for(const tomos_lsqr_options& o, const CsrMat& A) {
+    tomos_lsqr_options od = o;
+    od.fused = 0;
+    Layout L = layout_for(od, A.m, A.n);
+    const size_t e = static_cast<size_t>(std::max<int64_t>(A.nnz, 1));
+    size_t off = L.scratch_total;
+    L.csr_vals = off; off = align256(off + e * csr_value_bytes(A.val_type));
+    L.csr_ptr = off; off = align256(off + static_cast<size_t>(A.n + 1) * csr_index_bytes(A.idx_type));
+    L.csr_idx = off; off = align256(off + e * csr_index_bytes(A.idx_type));
+    L.csr_tmp_bytes = csr_transpose_scratch_bytes(A);
+    L.csr_tmp = off; off = align256(off + L.csr_tmp_bytes);
+    L.scratch_total = off;
+    return L;
+}
+
 void validate(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     require(o.a_type == TOMOS_F32 || o.a_type == TOMOS_F64 || o.a_type == TOMOS_FP32_16 ||
                 o.a_type == TOMOS_FP32_BF16 || o.a_type == TOMOS_F16 || o.a_type == TOMOS_BF16,
@@ -391,6 +412,10 @@ struct Solver {
     double* hist() const { return reinterpret_cast<double*>(sc + L.hist); }
     double* red() const { return reinterpret_cast<double*>(sc + L.red); }
     void* Z() const { return sc + L.z; }
+    // CSR A (instead of the dense a/lda): its transpose, and the refine
+    // duplicate's values over the same pattern
+    const CsrMat* csr = nullptr;
+    CsrMat csr_t, csr_ref;
     bool use_fused = false;
     bool use_fused_restart = false;  // initialize() in one pass (exact residual + A_hi^T u)
     FusedPlan rplan;
@@ -412,6 +437,11 @@ struct Solver {
     // A-matrix GEMV at a tier
     void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64,
                 int* nb = nullptr) {
+        if (csr) {  // FP64 accumulation at every tier
+            csr_mv(trans == TOMOS_TRANSPOSE ? csr_t : *csr, c, x64 ? TOMOS_F64 : TOMOS_F32, xin,
+                   y64 ? TOMOS_F64 : TOMOS_F32, yout, s, nb);
+            return;
+        }
         gemv(o.a_type, tier, trans, c, m, n, a, wr, lda, x64 ? TOMOS_F64 : TOMOS_F32, xin,
              y64 ? TOMOS_F64 : TOMOS_F32, yout, scratch(), L.gemv_bytes, s, nb);
     }
@@ -433,8 +463,12 @@ struct Solver {
             Coef cr = coef(-1.0, 1.0);
             cr.sumsq = red();
             int nb = 0;
-            gemv(ref_type, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, cr, m, n, aref, nullptr, lda_ref,
-                 r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), scratch(), L.gemv_bytes, s, &nb);
+            if (csr)
+                csr_mv(csr_ref, cr, r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), s, &nb);
+            else
+                gemv(ref_type, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, cr, m, n, aref, nullptr, lda_ref,
+                     r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), scratch(), L.gemv_bytes, s,
+                     &nb);
             k_beta_finish<<<1, kT, 0, s>>>(S, red(), nb);
             tail(true);
             return;
@@ -621,11 +655,118 @@ void lsqr_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a
     sv.run(iterations, history, history_cap, history_len);
 }
 
+// LSQR over a CSR matrix (lsqr_sparse_impl, lsqr_registries.cpp:164-196):
+// the same iteration with A p from the CSR rows and A^T u from a CSR
+// transpose built once per solve.  ref_vals (ref_type F32/F64, or null):
+// the refine duplicate's values over A's pattern, read by the restarts.
+void lsqr_solve_csr(const tomos_lsqr_options& o, const CsrMat& A, int ref_type, const void* ref_vals,
+                    const void* b, void* x, const void* rinv, int64_t ldr, void* arena, size_t arena_bytes,
+                    void* scratch, size_t scratch_bytes, int* iterations, double* history, int history_cap,
+                    int* history_len, hipStream_t s) {
+    validate(o, A.m, A.n);
+    csr_require_valid_types(A, "lsqr (CSR)");
+    require(o.a_type == A.val_type, "lsqr (CSR): options.a_type must be the CSR value type");
+    if (ref_vals) require(ref_type == TOMOS_F32 || ref_type == TOMOS_F64, "lsqr: the refine duplicate must be F32 or F64");
+    if (o.precond_type >= 0) {
+        require(rinv != nullptr, "lsqr: precond_type set but no R^-1 given");
+        require(ldr >= std::max<int64_t>(A.n, 1), "lsqr: ldr < n");
+    }
+    if (A.n == 0) {
+        if (iterations) *iterations = 0;
+        if (history_len) *history_len = 0;
+        return;
+    }
+    require(b && x, "lsqr: null b or x");
+    Solver sv;
+    sv.o = o;
+    sv.o.fused = 0;
+    sv.m = A.m;
+    sv.n = A.n;
+    sv.b = b;
+    sv.x = x;
+    sv.rinv = rinv;
+    sv.ldr = ldr;
+    sv.L = csr_layout_for(o, A);
+    if (arena == nullptr || arena_bytes < sv.L.arena_total)
+        throw Error(kErrWorkspace, "lsqr: workspace is too small for LSQR's working vectors (need " +
+                                       std::to_string(sv.L.arena_total) + " bytes, have " +
+                                       std::to_string(arena ? arena_bytes : 0) + ")");
+    if (scratch == nullptr) scratch = fallback_scratch(sv.L.scratch_total);
+    else if (scratch_bytes < sv.L.scratch_total)
+        throw Error(kErrWorkspace, "lsqr: scratch too small (need " + std::to_string(sv.L.scratch_total) +
+                                       " bytes, have " + std::to_string(scratch_bytes) + ")");
+    sv.arena = static_cast<char*>(arena);
+    sv.sc = static_cast<char*>(scratch);
+    sv.s = s;
+    sv.v64 = o.value_type == TOMOS_F64;
+    sv.r64 = o.refine_type == TOMOS_F64;
+    sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
+    // indices validated once (synchronous), then A^T built in the scratch
+    csr_check(A, sv.sc + sv.L.csr_tmp, sv.L.csr_tmp_bytes, s);
+    csr_transpose(A, sv.sc + sv.L.csr_vals, sv.sc + sv.L.csr_ptr, sv.sc + sv.L.csr_idx, sv.sc + sv.L.csr_tmp,
+                  sv.L.csr_tmp_bytes, s);
+    sv.csr = &A;
+    sv.csr_t = A;
+    sv.csr_t.m = A.n;
+    sv.csr_t.n = A.m;
+    sv.csr_t.vals = sv.sc + sv.L.csr_vals;
+    sv.csr_t.row_ptrs = sv.sc + sv.L.csr_ptr;
+    sv.csr_t.col_idxs = sv.sc + sv.L.csr_idx;
+    if (ref_vals) {
+        sv.csr_ref = A;
+        sv.csr_ref.val_type = ref_type;
+        sv.csr_ref.vals = ref_vals;
+        sv.aref = ref_vals;
+        sv.ref_type = ref_type;
+    }
+    sv.run(iterations, history, history_cap, history_len);
+}
+
 }  // namespace tomos_hip
 
 using namespace tomos_hip;
 
 extern "C" {
+
+size_t tomos_lsqr_csr_scratch_size(const tomos_lsqr_options* o, int idx_type, int64_t m, int64_t n,
+                                   int64_t nnz) {
+    size_t out = 0;
+    int rc = guarded([&] {
+        require(o != nullptr, "tomos_lsqr_csr_scratch_size: null options");
+        validate(*o, m, n);
+        CsrMat A;
+        A.val_type = o->a_type;
+        A.idx_type = idx_type;
+        A.m = m;
+        A.n = n;
+        A.nnz = nnz;
+        require(nnz >= 0 && (idx_type == TOMOS_INDEX_INT32 || idx_type == TOMOS_INDEX_INT64),
+                "tomos_lsqr_csr_scratch_size: bad nnz or index type");
+        out = csr_layout_for(*o, A).scratch_total;
+    });
+    return rc == 0 ? out : 0;
+}
+
+int tomos_lsqr_solve_csr(const tomos_lsqr_options* o, int idx_type, int64_t m, int64_t n, int64_t nnz,
+                         const void* vals, const void* row_ptrs, const void* col_idxs, int ref_type,
+                         const void* ref_vals, const void* b, void* x, const void* rinv, int64_t ldr, void* arena,
+                         size_t arena_bytes, void* scratch, size_t scratch_bytes, int* iterations, double* history,
+                         int history_cap, int* history_len, tomos_stream_t s) {
+    return guarded([&] {
+        require(o != nullptr, "tomos_lsqr_solve_csr: null options");
+        CsrMat A;
+        A.val_type = o->a_type;
+        A.idx_type = idx_type;
+        A.m = m;
+        A.n = n;
+        A.nnz = nnz;
+        A.vals = vals;
+        A.row_ptrs = row_ptrs;
+        A.col_idxs = col_idxs;
+        lsqr_solve_csr(*o, A, ref_type, ref_vals, b, x, rinv, ldr, arena, arena_bytes, scratch, scratch_bytes,
+                       iterations, history, history_cap, history_len, s);
+    });
+}
 
 size_t tomos_lsqr_arena_size(const tomos_lsqr_options* o, int64_t m, int64_t n) {
     size_t out = 0;
