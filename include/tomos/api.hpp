@@ -83,6 +83,14 @@ void apply_count_sketch(CountSketch& sketch, Dense& source, Dense& target);
 // std::runtime_error if R is singular.  sketch_qr.cpp:68-122
 void generate_sketch_qr_precond(CountSketch& sketch, Dense& sketched_mtx);
 
+// The reference's CSR sketch form (k x m, one entry per row at a hashed
+// column; countsketch.hpp:56-60): values +-1.  Deterministic per process
+// (a seed sequence, as the unseeded Gaussian sketch below); needs nnz == k.
+void generate_count_sketch(Sparse& sketch);
+// target (k x n, FP32 or FP64) = S * source (m x n, FP32 or FP64) for a CSR S
+// (the reference registers this on the CPU only, sketch_registries.cpp:25-40).
+void apply_count_sketch(Sparse& sketch, Dense& source, Dense& target);
+
 // Dense Gaussian sketch: S <- i.i.d. N(0,1), S FP64, FP32, FP16 or BF16
 // (include/core/sketch.hpp:11-13).  The reference seeds each call from the
 // clock; here a call without a seed takes the next seed of a process-wide
@@ -91,6 +99,16 @@ void generate_sketch_qr_precond(CountSketch& sketch, Dense& sketched_mtx);
 // on (seed, rows, cols), not on the lead dimension.
 Status create_gaussian_sketch(Dense& S);
 Status create_gaussian_sketch(Dense& S, std::uint64_t seed);
+// Sparse Gaussian sketch: the CSR form above with N(0,1) values
+// (sketch.hpp:15-18); needs nnz == rows.
+Status create_gaussian_sketch(Sparse& S);
+Status create_gaussian_sketch(Sparse& S, std::uint64_t seed);
+
+// ------------------------------------------------------------ sparse BLAS
+// y = alpha*op(A)*x + beta*y for CSR A; x, y FP32 or FP64, FP64
+// accumulation.  blas.hpp:127-137.  TRANSPOSE validates A and builds A^T
+// (deterministic, no atomics) for the call.
+Status mv_apply(Device dev, TransposeType trans, double alpha, Sparse& A, Dense& x, double beta, Dense& y);
 
 // ------------------------------------------------------------------ LSQR
 // include/core/solver.hpp:15-72
@@ -101,6 +119,13 @@ void lsqr_solve(Context context, Dense& A, Dense& b, Dense& x, Workspace& worksp
                 int max_iters, double max_relres, int* iterations = nullptr,
                 std::vector<double>* residual_history = nullptr, int restarts = 1,
                 Dense* mtx_refine_precision = nullptr);
+// CSR A (solver.hpp:74-78): A^T built once per call; mtx_refine_precision,
+// when given, must have A's pattern (same row pointers and column indices)
+// with values at b/x's precision.
+void lsqr_solve(Context context, Sparse& A, Dense& b, Dense& x, Workspace& workspace, double tol,
+                int max_iters, double max_relres, int* iterations = nullptr,
+                std::vector<double>* residual_history = nullptr, int restarts = 1,
+                Sparse* mtx_refine_precision = nullptr);
 
 // ------------------------------------------------------------ registries
 namespace dispatch {
@@ -121,6 +146,13 @@ using lsqr_dense_fn = void (*)(Context context, tomos::Dense& A, tomos::Dense& b
 using apply_count_sketch_dense_fn = void (*)(CountSketch& S, tomos::Dense& source,
                                              tomos::Dense& target);
 using create_gaussian_sketch_dense_fn = Status (*)(tomos::Dense& S, std::uint64_t seed);
+using create_gaussian_sketch_sparse_fn = Status (*)(tomos::Sparse& S, std::uint64_t seed);
+using mv_sparse_fn = void (*)(TransposeType trans, double alpha, tomos::Sparse& A, tomos::Dense& x, double beta,
+                              tomos::Dense& y);
+using lsqr_sparse_fn = void (*)(Context context, tomos::Sparse& A, tomos::Dense& b, tomos::Dense& x,
+                                Workspace& workspace, double tol, int max_iters, double max_relres,
+                                int* iterations, std::vector<double>* history, int restarts,
+                                tomos::Sparse* mtx_refine);
 
 struct Dense {
     static const Dense& get();
@@ -189,6 +221,8 @@ struct Lsqr {
     static const Lsqr& get();
     lsqr_dense_fn lookup_solve_dense(Device dev, NumericType mv_type, NumericType value_type,
                                      NumericType refine_type, NumericType precond_type) const;
+    lsqr_sparse_fn lookup_solve_sparse(Device dev, NumericType mv_type, NumericType value_type,
+                                       NumericType refine_type, NumericType precond_type) const;
     static std::size_t encode(Device dev, NumericType mv_type, NumericType value_type,
                               NumericType refine_type, NumericType precond_type);
 
@@ -198,6 +232,20 @@ private:
                                      num_registry_numeric_types * num_registry_numeric_types *
                                      num_registry_numeric_types;
     std::array<lsqr_dense_fn, N> dense_{};
+    std::array<lsqr_sparse_fn, N> sparse_{};
+};
+
+// CSR products per (device, A, x, y, index type).  sparse_registries.hpp
+struct Sparse {
+    static const Sparse& get();
+    mv_sparse_fn lookup_mv(Device dev, NumericType A, NumericType x, NumericType y, IntType index) const;
+    static std::size_t encode_mv(Device dev, NumericType A, NumericType x, NumericType y, IntType index);
+
+private:
+    Sparse();
+    static constexpr std::size_t N = num_registry_devices * num_registry_numeric_types *
+                                     num_registry_numeric_types * num_registry_numeric_types * 2;
+    std::array<mv_sparse_fn, N> mv_{};
 };
 
 struct Sketch {
@@ -216,12 +264,15 @@ private:
 struct GaussianSketch {
     static const GaussianSketch& get();
     create_gaussian_sketch_dense_fn lookup_create_dense(Device dev, NumericType type) const;
+    create_gaussian_sketch_sparse_fn lookup_create_sparse(Device dev, NumericType type, IntType index) const;
     static std::size_t encode_dense(Device dev, NumericType type);
 
 private:
     GaussianSketch();
     std::array<create_gaussian_sketch_dense_fn, num_registry_devices * num_registry_numeric_types>
         dense_{};
+    std::array<create_gaussian_sketch_sparse_fn, num_registry_devices * num_registry_numeric_types * 2>
+        sparse_{};
 };
 
 }  // namespace dispatch

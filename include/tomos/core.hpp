@@ -139,6 +139,69 @@ StridedView<T> as_strided_view(Dense& d) {
                           static_cast<T*>(d.data())};
 }
 
+// Index type of CSR row pointers and column indices.  precision.hpp:42
+enum class IntType : std::size_t { INT32 = 0, INT64 = 1, Count };
+
+// CSR sparse matrix (m x n, nnz stored entries; row_ptrs m+1, col_idxs
+// nnz, values nnz, all on the device).  include/core/matrix.hpp:131-180;
+// storage as CsrStorage, src/core/matrix/sparse_storage.hpp:16-47.
+// Value types FP64, FP32, FP16, BF16.
+class Sparse {
+public:
+    static Sparse create_csr(Context& ctx, NumericType value_type, IntType index_type, dim<2> size,
+                             std::size_t nnz);
+
+    Sparse(Sparse&& o) noexcept;
+    Sparse& operator=(Sparse&& o) noexcept;
+    Sparse(const Sparse&) = delete;
+    Sparse& operator=(const Sparse&) = delete;
+    ~Sparse();
+
+    NumericType get_value_type() const { return type_; }
+    IntType get_index_type() const { return index_; }
+    Device get_device() const { return device_; }
+    dim<2> get_size() const { return size_; }
+    std::size_t get_nnz() const { return nnz_; }
+    stream_t stream() const { return stream_; }
+    void* values() const { return values_; }
+    void* row_ptrs() const { return row_ptrs_; }
+    void* col_idxs() const { return col_idxs_; }
+
+    // host -> device staging: row_ptrs (m+1), col_idxs (nnz) in the index
+    // type, values (nnz) in the value type; any pointer may be null (skipped)
+    void copy_from_host(const void* row_ptrs, const void* col_idxs, const void* values);
+    void copy_to_host(void* row_ptrs, void* col_idxs, void* values) const;
+
+private:
+    Sparse() = default;
+    void release();
+    Device device_ = Device::HIP;
+    NumericType type_ = NumericType::FP64;
+    IntType index_ = IntType::INT32;
+    dim<2> size_{0, 0};
+    std::size_t nnz_ = 0;
+    void* values_ = nullptr;
+    void* row_ptrs_ = nullptr;
+    void* col_idxs_ = nullptr;
+    stream_t stream_ = nullptr;
+};
+
+// Device pointers of a CSR matrix.  matrix.hpp CsrView
+template <typename T, typename I>
+struct CsrView {
+    dim<2> size_;
+    std::size_t nnz_;
+    T* values_;
+    I* row_ptrs_;
+    I* col_idxs_;
+};
+
+template <typename T, typename I>
+CsrView<T, I> as_csr_view(Sparse& s) {
+    return CsrView<T, I>{s.get_size(), s.get_nnz(), static_cast<T*>(s.values()),
+                         static_cast<I*>(s.row_ptrs()), static_cast<I*>(s.col_idxs())};
+}
+
 // Identifies the concrete type behind Workspace::precond_.  workspace.hpp:21-35
 enum class PrecondKind : std::size_t { None = 0, Dense = 1, Sparse = 2 };
 

@@ -291,6 +291,17 @@ int tomos_csr_transpose(int val_type, int idx_type, int64_t m, int64_t n, int64_
 int tomos_gaussian_fill(int type, int64_t rows, int64_t cols, void* out, int64_t ld, uint64_t seed,
                         tomos_stream_t stream);
 
+/* The reference's CSR sketch form (rows x cols, one entry per row: row i
+ * holds column h(i); generate_countsketch_csr_idxs,
+ * countsketch_kernels.cpp:34-53, which draws h from rand() seeded by the
+ * clock): row_ptrs[i] = i, col_idxs[i] = z_i % cols with z_i the i-th
+ * SplitMix64 output from seed ^ 0xc2b2ae3d27d4eb4f; values +-1 from z_i's
+ * top bit (kind 0, the count sketch) or N(0,1) as tomos_gaussian_fill of a
+ * rows x 1 column with `seed` (kind 1, create_gaussian_sketch(Sparse&),
+ * gaussian_sketch_registries.cpp:33-57).  nnz = rows.  Asynchronous. */
+int tomos_csr_sketch(int kind, int val_type, int idx_type, int64_t rows, int64_t cols, void* vals,
+                     void* row_ptrs, void* col_idxs, uint64_t seed, tomos_stream_t stream);
+
 /* Y (s x n, FP64, ldy) = S * A, S the hashed s x m CountSketch above,
  * A m x n in any a_type/tier (packed A is decoded per tier).  Y is
  * overwritten.  Replaces countsketch_apply_matrix_hashed

@@ -427,6 +427,157 @@ int main() {
         std::printf("    iterations: plain %d, Gaussian-sketch preconditioned %d\n", its[0], its[1]);
         EXPECT(its[1] > 0 && its[1] <= 40 && its[1] * 5 < its[0]);
     });
+    // ------------------------------------------------------------ CSR
+    auto make_csr = [](NumericType vt, std::size_t m, std::size_t n, const std::vector<int>& rp,
+                       const std::vector<int>& ci, const std::vector<double>& v) {
+        Sparse A = Sparse::create_csr(ctx, vt, IntType::INT32, {m, n}, ci.size());
+        std::vector<unsigned char> vb(v.size() * numeric_type_size(vt));
+        for (std::size_t k = 0; k < v.size(); ++k) {
+            if (vt == NumericType::FP64) std::memcpy(&vb[k * 8], &v[k], 8);
+            else {
+                const float f = static_cast<float>(v[k]);
+                if (vt == NumericType::FP32) std::memcpy(&vb[k * 4], &f, 4);
+                else {
+                    uint32_t u;
+                    std::memcpy(&u, &f, 4);
+                    uint16_t h = vt == NumericType::BF16 ? static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16)
+                                                         : static_cast<uint16_t>(0);
+                    if (vt == NumericType::FP16) {  // the test values are small integers: exact in half
+                        const int e = f == 0.0f ? 0 : static_cast<int>((u >> 23) & 0xFF) - 127 + 15;
+                        h = f == 0.0f ? 0 : static_cast<uint16_t>(((u >> 16) & 0x8000u) | (e << 10) | ((u >> 13) & 0x3FFu));
+                    }
+                    std::memcpy(&vb[k * 2], &h, 2);
+                }
+            }
+        }
+        A.copy_from_host(rp.data(), ci.data(), vb.data());
+        return A;
+    };
+    run("sparse_lsqr_known_answers", [&] {
+        // SparseDiagonalSystem / SparseOverdeterminedConsistent (test_lsqr.cpp:375-440)
+        Sparse D = make_csr(NumericType::FP64, 3, 3, {0, 1, 2, 3}, {0, 1, 2}, {1, 2, 4});
+        Dense b = make(NumericType::FP64, 3, 1, std::vector<double>{1, 4, 8});
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {3, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {3, 3}, NumericType::FP64);
+        lsqr_solve(ctx, D, b, x, ws, 1e-10, 50, 1e-10);
+        auto s1 = fetch<double>(x, 3);
+        EXPECT(std::fabs(s1[0] - 1) < 1e-8 && std::fabs(s1[1] - 2) < 1e-8 && std::fabs(s1[2] - 2) < 1e-8);
+        for (NumericType mv : {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16}) {
+            Sparse A = make_csr(mv, 4, 2, {0, 1, 2, 4, 6}, {0, 1, 0, 1, 0, 1}, {1, 1, 1, 1, 1, -1});
+            Dense b2 = make(NumericType::FP64, 4, 1, std::vector<double>{1, 1, 2, 0});
+            Dense x2 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {2, 1});
+            Dense P = make(NumericType::FP64, 2, 2, std::vector<double>{0.9, 0, 0, 0.9});
+            Workspace ws2 = create_lsqr_workspace(ctx, {4, 2}, NumericType::FP32, NumericType::FP64);
+            add_precond(P, ws2);
+            lsqr_solve(ctx, A, b2, x2, ws2, 1e-8, 100, 1e-8);
+            auto s2 = fetch<double>(x2, 2);
+            EXPECT(std::fabs(s2[0] - 1) < 1e-6 && std::fabs(s2[1] - 1) < 1e-6);
+        }
+    });
+    run("sparse_mv_apply_and_registry", [&] {
+        Sparse A = make_csr(NumericType::FP32, 4, 2, {0, 1, 2, 4, 6}, {0, 1, 0, 1, 0, 1}, {1, 1, 1, 1, 1, -1});
+        Dense x = make(NumericType::FP64, 2, 1, std::vector<double>{2, 3});
+        Dense y = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {4, 1});
+        mv_apply(Device::HIP, TransposeType::NO_TRANSPOSE, 1.0, A, x, 0.0, y);
+        EXPECT((fetch<double>(y, 4) == std::vector<double>{2, 3, 5, -1}));
+        Dense u = make(NumericType::FP32, 4, 1, std::vector<float>{1, 1, 1, 1});
+        Dense t = make(NumericType::FP32, 2, 1, std::vector<float>{10, 10});
+        mv_apply(Device::HIP, TransposeType::TRANSPOSE, 2.0, A, u, 1.0, t);  // 2 A^T 1 + t = (16, 12)
+        EXPECT((fetch<float>(t, 2) == std::vector<float>{16, 12}));
+        EXPECT(throws<std::runtime_error>([] {
+            dispatch::Sparse::get().lookup_mv(Device::HIP, NumericType::FP32_16, NumericType::FP32,
+                                              NumericType::FP32, IntType::INT32);
+        }));
+        EXPECT(throws<std::runtime_error>([] {
+            dispatch::Lsqr::get().lookup_solve_sparse(Device::HIP, NumericType::FP32_BF16, NumericType::FP32,
+                                                      NumericType::FP32, NumericType::FP32);
+        }));
+        Sparse bad = make_csr(NumericType::FP64, 2, 2, {0, 1, 2}, {0, 5}, {1, 1});
+        Dense v = make(NumericType::FP64, 2, 1, std::vector<double>{1, 1});
+        Dense w = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {2, 1});
+        EXPECT(throws<std::invalid_argument>(
+            [&] { mv_apply(Device::HIP, TransposeType::TRANSPOSE, 1.0, bad, v, 0.0, w); }));
+    });
+    run("csr_sketches_and_sketch_preconditioned_lsqr", [&] {
+        // LsqrSketchPrecondTest with the CSR sketch (test_lsqr.cpp:461-523)
+        const std::size_t m = 40, n = 5, k = 30;
+        std::mt19937 gen(42);
+        std::uniform_real_distribution<double> dist(-1.0, 1.0);
+        std::vector<double> a(m * n), xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) a[i + j * m] = dist(gen);
+        for (auto& v : xt) v = dist(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += a[i + j * m] * xt[j];
+        Dense A = make(NumericType::FP64, m, n, a);
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        Sparse S = Sparse::create_csr(ctx, NumericType::FP64, IntType::INT32, {k, m}, k);
+        generate_count_sketch(S);
+        std::vector<int> rp(k + 1), ci(k);
+        std::vector<double> sv(k);
+        S.copy_to_host(rp.data(), ci.data(), sv.data());
+        bool ok = true;
+        for (std::size_t i = 0; i <= k; ++i) ok = ok && rp[i] == static_cast<int>(i);
+        for (std::size_t i = 0; i < k; ++i) ok = ok && ci[i] >= 0 && ci[i] < static_cast<int>(m) && std::fabs(sv[i]) == 1.0;
+        EXPECT(ok);
+        Dense sk = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(S, A, sk);
+        auto skh = fetch<double>(sk, k * n);
+        double maxerr = 0;
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < k; ++i)
+                maxerr = std::max(maxerr, std::fabs(skh[i + j * k] - sv[i] * a[ci[i] + j * m]));
+        EXPECT(maxerr == 0.0);
+        // the Gaussian CSR form: same shape, N(0,1) values, seeded
+        Sparse G1 = Sparse::create_csr(ctx, NumericType::FP32, IntType::INT64, {k, m}, k);
+        Sparse G2 = Sparse::create_csr(ctx, NumericType::FP32, IntType::INT64, {k, m}, k);
+        create_gaussian_sketch(G1, 5);
+        create_gaussian_sketch(G2, 5);
+        std::vector<float> g1(k), g2(k);
+        G1.copy_to_host(nullptr, nullptr, g1.data());
+        G2.copy_to_host(nullptr, nullptr, g2.data());
+        EXPECT(g1 == g2 && g1[0] != g1[1]);
+        Sparse wrong = Sparse::create_csr(ctx, NumericType::FP32, IntType::INT32, {k, m}, k + 1);
+        EXPECT(throws<std::invalid_argument>([&] { create_gaussian_sketch(wrong); }));
+        // preconditioner from a full-rank sketch: the CountSketch of the reference test (k=30 rows)
+        CountSketch H = create_count_sketch(k, m);
+        Dense hs = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(H, A, hs);
+        generate_sketch_qr_precond(H, hs);
+        Dense P = Dense::wrap_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n}, k, hs.data());
+        std::vector<int> rpa(m + 1);
+        std::vector<int> cia;
+        std::vector<double> va;
+        for (std::size_t i = 0; i < m; ++i) {
+            rpa[i] = static_cast<int>(cia.size());
+            for (std::size_t j = 0; j < n; ++j) {
+                cia.push_back(static_cast<int>(j));
+                va.push_back(a[i + j * m]);
+            }
+        }
+        rpa[m] = static_cast<int>(cia.size());
+        Sparse As = make_csr(NumericType::FP64, m, n, rpa, cia, va);
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP64);
+        add_precond(P, ws);
+        int iters = 0;
+        lsqr_solve(ctx, As, b, x, ws, 1e-10, 100, 1e-10, &iters);
+        auto sol = fetch<double>(x, n);
+        for (std::size_t i = 0; i < n; ++i) EXPECT(std::fabs(sol[i] - xt[i]) < 1e-6);
+        std::printf("    sparse sketch-preconditioned iterations=%d\n", iters);
+        // explicit duplicate must share A's pattern
+        Sparse other = make_csr(NumericType::FP64, 4, 2, {0, 1, 2, 4, 6}, {0, 1, 0, 1, 1, 0}, {1, 1, 1, 1, 1, -1});
+        Sparse A16 = make_csr(NumericType::FP16, 4, 2, {0, 1, 2, 4, 6}, {0, 1, 0, 1, 0, 1}, {1, 1, 1, 1, 1, -1});
+        Sparse A64 = make_csr(NumericType::FP64, 4, 2, {0, 1, 2, 4, 6}, {0, 1, 0, 1, 0, 1}, {1, 1, 1, 1, 1, -1});
+        Dense b4 = make(NumericType::FP64, 4, 1, std::vector<double>{1, 1, 2, 0});
+        Dense x4 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {2, 1});
+        Workspace ws4 = create_lsqr_workspace(ctx, {4, 2}, NumericType::FP64);
+        EXPECT(throws<std::invalid_argument>(
+            [&] { lsqr_solve(ctx, A16, b4, x4, ws4, 1e-10, 20, 1e-10, nullptr, nullptr, 2, &other); }));
+        lsqr_solve(ctx, A16, b4, x4, ws4, 1e-12, 20, 1e-12, nullptr, nullptr, 3, &A64);
+        auto s4 = fetch<double>(x4, 2);
+        EXPECT(std::fabs(s4[0] - 1) < 1e-10 && std::fabs(s4[1] - 1) < 1e-10);
+    });
     std::printf("%s: %d failure(s)\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

@@ -207,3 +207,26 @@ def test_explicit_duplicate_restarts(capi):
     x1, it1, _ = H.lsqr_csr(A16, b, rinv=rinv, precond_type=C.F64, tol=1e-9, max_iters=8, restarts=30,
                             ref=(C.F64, v))
     assert r(x1) < 2e-9 and r(x0) > 10 * r(x1), (r(x0), r(x1))
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("it", [C.INDEX_INT32, C.INDEX_INT64])
+def test_csr_sketch_pattern(capi, orc, kind, it):
+    """The reference's CSR sketch form (one entry per row,
+    countsketch_kernels.cpp:34-53): row i holds column z_i % cols, z_i the
+    i-th SplitMix64 output from seed ^ 0xc2b2ae3d27d4eb4f; values +-1 from
+    z_i's top bit, or the Gaussian fill of a rows x 1 column."""
+    from tests.test_oracle_gaussian import stream
+    rows, cols, seed = 1001, 70000, 1234
+    idt = np.int64 if it == C.INDEX_INT64 else np.int32
+    rp, ci, v = (H.DeviceBuffer((rows + 1) * 8), H.DeviceBuffer(rows * 8), H.DeviceBuffer(rows * 8))
+    C.call("tomos_csr_sketch", kind, C.F64, it, rows, cols, v.ptr, rp.ptr, ci.ptr, seed, None)
+    z = [stream(seed ^ 0xC2B2AE3D27D4EB4F, i) for i in range(rows)]
+    assert np.array_equal(rp.to_numpy(idt, (rows + 1,)), np.arange(rows + 1))
+    assert np.array_equal(ci.to_numpy(idt, (rows,)), [q % cols for q in z])
+    vals = v.to_numpy(np.float64, (rows,))
+    if kind == 0:
+        assert np.array_equal(vals, [1.0 if q >> 63 else -1.0 for q in z])
+    else:
+        ref = orc.gaussian_fill(rows, 1, seed)[:, 0]
+        assert np.all(np.abs(vals - ref) <= 1e-6 * (1 + np.abs(ref)))

@@ -116,6 +116,7 @@ _SIGS = {
                                 sz, vp]),
     "tomos_gaussian_fill": (cint, [cint, i64, i64, vp, i64, ctypes.c_uint64, vp]),
     "tomos_csr_check": (cint, [cint, cint, i64, i64, i64, vp, vp, vp]),
+    "tomos_csr_sketch": (cint, [cint, cint, cint, i64, i64, vp, vp, vp, ctypes.c_uint64, vp]),
     "tomos_csr_mv_scratch_size": (sz, [cint, cint, cint, i64, i64, i64]),
     "tomos_csr_mv": (cint, [cint, cint, cint, ctypes.c_double, i64, i64, i64, vp, vp, vp, cint, vp,
                             ctypes.c_double, cint, vp, vp, sz, vp]),

@@ -351,6 +351,75 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
     if (history) history->insert(history->end(), hist.begin(), hist.begin() + std::min(hl, cap));
 }
 
+int abi_index(IntType t) { return t == IntType::INT64 ? TOMOS_INDEX_INT64 : TOMOS_INDEX_INT32; }
+
+// lsqr_sparse_impl (src/dispatch/lsqr_registries.cpp:164-196)
+template <NumericType MV, NumericType VAL, NumericType REF, NumericType PRE>
+void lsqr_sparse_impl(Context context, tomos::Sparse& A, tomos::Dense& b, tomos::Dense& x, Workspace& ws,
+                      double tol, int max_iters, double max_relres, int* iterations, std::vector<double>* history,
+                      int restarts, tomos::Sparse* mtx_refine) {
+    if (ws.device_ != context.get_device())
+        throw std::invalid_argument("lsqr_solve: workspace and context must be on the same device.");
+    const auto sz = A.get_size();
+    const int64_t m = static_cast<int64_t>(sz[0]), n = static_cast<int64_t>(sz[1]);
+    if (vec_len(b) != sz[0] || vec_len(x) != sz[1])
+        throw std::invalid_argument("lsqr_solve: b/x sizes do not match A");
+    if (mtx_refine) {
+        if (mtx_refine->get_value_type() != NumericType::FP32 && mtx_refine->get_value_type() != NumericType::FP64)
+            throw std::invalid_argument("lsqr_solve: the refine-precision duplicate must be FP32 or FP64.");
+        if (mtx_refine->get_size() != sz || mtx_refine->get_nnz() != A.get_nnz() ||
+            mtx_refine->get_index_type() != A.get_index_type() || mtx_refine->get_device() != A.get_device())
+            throw std::invalid_argument("lsqr_solve: the refine-precision duplicate must match A's size and pattern.");
+        // the restarts read its values over A's indices: the patterns must agree
+        const std::size_t is = A.get_index_type() == IntType::INT64 ? 8 : 4;
+        std::vector<unsigned char> ra((sz[0] + 1) * is), rb(ra.size()), ca(A.get_nnz() * is), cb(ca.size());
+        A.copy_to_host(ra.data(), ca.data(), nullptr);
+        mtx_refine->copy_to_host(rb.data(), cb.data(), nullptr);
+        if (ra != rb || ca != cb)
+            throw std::invalid_argument("lsqr_solve: the refine-precision duplicate must have A's sparsity pattern.");
+    }
+    if (ws.use_implicit_duplicate_)
+        throw std::invalid_argument("lsqr_solve: Workspace::use_implicit_duplicate_ needs dense packed A.");
+    const bool pre = ws.precond_kind_ == PrecondKind::Dense;
+    if (ws.precond_kind_ == PrecondKind::Sparse)
+        throw std::invalid_argument("lsqr_solve: a sparse preconditioner is not supported; use a dense R^-1.");
+    tomos::Dense* P = pre ? static_cast<tomos::Dense*>(ws.precond_) : nullptr;
+    tomos_lsqr_options o;
+    o.a_type = abi_type(MV);
+    o.mv_tier = TOMOS_TIER_DEFAULT;
+    o.restart_tier = TOMOS_TIER_DEFAULT;
+    o.value_type = abi_type(VAL);
+    o.refine_type = abi_type(REF);
+    o.precond_type = pre ? abi_type(PRE) : -1;
+    o.tol = tol;
+    o.max_relres = max_relres;
+    o.max_iters = max_iters;
+    o.restarts = restarts;
+    o.check_every = std::max(1, ws.check_every_);
+    o.fused = 0;
+    const int idx = abi_index(A.get_index_type());
+    const int64_t nnz = static_cast<int64_t>(A.get_nnz());
+    const size_t arena = tomos_lsqr_arena_size(&o, m, n);
+    const size_t scratch = tomos_lsqr_csr_scratch_size(&o, idx, m, n, nnz);
+    if (arena == 0 || scratch == 0) check(TOMOS_ERR_INVALID_ARGUMENT, "lsqr_solve");
+    if (arena > ws.capacity_)
+        throw std::invalid_argument("lsqr_solve: workspace is too small for LSQR's working vectors (need " +
+                                    std::to_string(arena) + " bytes, have " + std::to_string(ws.capacity_) + ").");
+    ws.reserve_scratch(scratch);
+    const int cap = std::max(0, restarts) * (std::max(0, max_iters) + 1) + 1;
+    std::vector<double> hist(history ? static_cast<std::size_t>(cap) : 0);
+    int it = 0, hl = 0;
+    check(tomos_lsqr_solve_csr(&o, idx, m, n, nnz, A.values(), A.row_ptrs(), A.col_idxs(),
+                               mtx_refine ? abi_type(mtx_refine->get_value_type()) : -1,
+                               mtx_refine ? mtx_refine->values() : nullptr, b.data(), x.data(),
+                               P ? P->data() : nullptr, P ? static_cast<int64_t>(P->get_lead_dim()) : 1, ws.data_,
+                               ws.capacity_, ws.mv_transpose_scratch_, ws.mv_transpose_scratch_capacity_, &it,
+                               history ? hist.data() : nullptr, history ? cap : 0, &hl, context.get_stream()),
+          "lsqr_solve");
+    if (iterations) *iterations = it;
+    if (history) history->insert(history->end(), hist.begin(), hist.begin() + std::min(hl, cap));
+}
+
 }  // namespace
 
 namespace dispatch {
@@ -363,14 +432,21 @@ std::size_t Lsqr::encode(Device dev, NumericType mv, NumericType val, NumericTyp
 }
 
 namespace {
+constexpr std::size_t kLsqrSlots = num_registry_devices * num_registry_numeric_types * num_registry_numeric_types *
+                                   num_registry_numeric_types * num_registry_numeric_types;
 template <NumericType MV, NumericType VAL, NumericType REF>
-void register_pre(std::array<lsqr_dense_fn, num_registry_devices * num_registry_numeric_types *
-                                                num_registry_numeric_types * num_registry_numeric_types *
-                                                num_registry_numeric_types>& t) {
+void register_pre(std::array<lsqr_dense_fn, kLsqrSlots>& t) {
     t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP32)] =
         &lsqr_dense_impl<MV, VAL, REF, NumericType::FP32>;
     t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP64)] =
         &lsqr_dense_impl<MV, VAL, REF, NumericType::FP64>;
+}
+template <NumericType MV, NumericType VAL, NumericType REF>
+void register_pre(std::array<lsqr_sparse_fn, kLsqrSlots>& t) {
+    t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP32)] =
+        &lsqr_sparse_impl<MV, VAL, REF, NumericType::FP32>;
+    t[Lsqr::encode(Device::HIP, MV, VAL, REF, NumericType::FP64)] =
+        &lsqr_sparse_impl<MV, VAL, REF, NumericType::FP64>;
 }
 template <NumericType MV, NumericType VAL, class Arr>
 void register_ref(Arr& t) {
@@ -392,11 +468,71 @@ Lsqr::Lsqr() {
     register_val<NumericType::FP32_BF16>(dense_);
     register_val<NumericType::FP16>(dense_);
     register_val<NumericType::BF16>(dense_);
+    sparse_.fill(nullptr);
+    register_val<NumericType::FP64>(sparse_);
+    register_val<NumericType::FP32>(sparse_);
+    register_val<NumericType::FP16>(sparse_);
+    register_val<NumericType::BF16>(sparse_);
 }
 
 const Lsqr& Lsqr::get() {
     static const Lsqr instance;
     return instance;
+}
+
+lsqr_sparse_fn Lsqr::lookup_solve_sparse(Device dev, NumericType mv, NumericType val, NumericType ref,
+                                         NumericType pre) const {
+    if (dev == Device::OMP) throw std::invalid_argument("lookup_solve_sparse: Device::OMP is not supported.");
+    auto fn = sparse_[encode(dev, mv, val, ref, pre)];
+    if (!fn)
+        throw std::runtime_error(
+            "No LSQR sparse kernel registered for the requested "
+            "device/mv_type/value_type/refine_type/precond_type combination.");
+    return fn;
+}
+
+// ---- CSR products (sparse_registries.cpp:225-243) ----
+std::size_t Sparse::encode_mv(Device dev, NumericType A, NumericType x, NumericType y, IntType index) {
+    const std::size_t T = num_registry_numeric_types;
+    return (((static_cast<std::size_t>(dev) * T + static_cast<std::size_t>(A)) * T + static_cast<std::size_t>(x)) *
+                T +
+            static_cast<std::size_t>(y)) *
+               2 +
+           static_cast<std::size_t>(index);
+}
+
+namespace {
+void mv_csr_impl(TransposeType trans, double alpha, tomos::Sparse& A, tomos::Dense& x, double beta,
+                 tomos::Dense& y) {
+    const auto sz = A.get_size();
+    const bool t = trans == TransposeType::TRANSPOSE;
+    if (vec_len(x) != (t ? sz[0] : sz[1]) || vec_len(y) != (t ? sz[1] : sz[0]))
+        throw std::invalid_argument("mv_apply: operand sizes do not match A");
+    check(tomos_csr_mv(abi_type(A.get_value_type()), abi_index(A.get_index_type()), t ? 1 : 0, alpha,
+                       static_cast<int64_t>(sz[0]), static_cast<int64_t>(sz[1]), static_cast<int64_t>(A.get_nnz()),
+                       A.values(), A.row_ptrs(), A.col_idxs(), abi_type(x.get_value_type()), x.data(), beta,
+                       abi_type(y.get_value_type()), y.data(), nullptr, 0, A.stream()),
+          "mv_apply");
+}
+}  // namespace
+
+Sparse::Sparse() {
+    mv_.fill(nullptr);
+    for (NumericType a : {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16})
+        for (NumericType xv : {NumericType::FP64, NumericType::FP32})
+            for (NumericType yv : {NumericType::FP64, NumericType::FP32})
+                for (IntType it : {IntType::INT32, IntType::INT64})
+                    mv_[encode_mv(Device::HIP, a, xv, yv, it)] = &mv_csr_impl;
+}
+const Sparse& Sparse::get() {
+    static const Sparse instance;
+    return instance;
+}
+mv_sparse_fn Sparse::lookup_mv(Device dev, NumericType A, NumericType x, NumericType y, IntType index) const {
+    if (dev == Device::OMP) throw std::invalid_argument("mv_apply: Device::OMP is not supported.");
+    auto fn = mv_[encode_mv(dev, A, x, y, index)];
+    if (!fn) throw std::runtime_error("mv_apply: no CSR kernel registered for this device/type combination.");
+    return fn;
 }
 
 lsqr_dense_fn Lsqr::lookup_solve_dense(Device dev, NumericType mv, NumericType val, NumericType ref,
@@ -466,6 +602,24 @@ Status gaussian_dense_impl(tomos::Dense& S, std::uint64_t seed) {
 // Seeds of the unseeded overload: the SplitMix64 increments from a fixed
 // start, one per call.
 std::atomic<std::uint64_t> g_gauss_calls{0};
+Status csr_sketch_impl(tomos::Sparse& S, std::uint64_t seed, int kind, const char* who) {
+    device_guard(S.get_device(), who);
+    const auto sz = S.get_size();
+    if (S.get_nnz() != sz[0])
+        throw std::invalid_argument(std::string(who) + ": nnz_per_row == 1 only; the sketch's nnz must equal its rows.");
+    check(tomos_csr_sketch(kind, abi_type(S.get_value_type()), abi_index(S.get_index_type()),
+                           static_cast<int64_t>(sz[0]), static_cast<int64_t>(sz[1]), S.values(), S.row_ptrs(),
+                           S.col_idxs(), seed, S.stream()),
+          who);
+    return SUCCESS;
+}
+Status gaussian_sparse_impl(tomos::Sparse& S, std::uint64_t seed) {
+    return csr_sketch_impl(S, seed, 1, "create_gaussian_sketch");
+}
+std::uint64_t next_sketch_seed() {
+    const std::uint64_t k = g_gauss_calls.fetch_add(1);
+    return 0x6a09e667f3bcc909ull + k * 0x9e3779b97f4a7c15ull;
+}
 }  // namespace
 
 namespace dispatch {
@@ -474,8 +628,12 @@ std::size_t GaussianSketch::encode_dense(Device dev, NumericType type) {
 }
 GaussianSketch::GaussianSketch() {
     dense_.fill(nullptr);
-    for (NumericType t : {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16})
+    sparse_.fill(nullptr);
+    for (NumericType t : {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16}) {
         dense_[encode_dense(Device::HIP, t)] = &gaussian_dense_impl;
+        for (IntType it : {IntType::INT32, IntType::INT64})
+            sparse_[encode_dense(Device::HIP, t) * 2 + static_cast<std::size_t>(it)] = &gaussian_sparse_impl;
+    }
 }
 const GaussianSketch& GaussianSketch::get() {
     static const GaussianSketch instance;
@@ -489,14 +647,51 @@ create_gaussian_sketch_dense_fn GaussianSketch::lookup_create_dense(Device dev, 
 }
 }  // namespace dispatch
 
+
+dispatch::create_gaussian_sketch_sparse_fn dispatch::GaussianSketch::lookup_create_sparse(Device dev, NumericType type,
+                                                                                IntType index) const {
+    if (dev == Device::OMP) throw std::invalid_argument("create_gaussian_sketch: Device::OMP is not supported.");
+    auto fn = sparse_[encode_dense(dev, type) * 2 + static_cast<std::size_t>(index)];
+    if (!fn) throw std::runtime_error("create_gaussian_sketch: nothing registered for this device/type.");
+    return fn;
+}
+
+Status create_gaussian_sketch(tomos::Sparse& S, std::uint64_t seed) {
+    return dispatch::GaussianSketch::get().lookup_create_sparse(S.get_device(), S.get_value_type(),
+                                                                S.get_index_type())(S, seed);
+}
+
+Status create_gaussian_sketch(tomos::Sparse& S) { return create_gaussian_sketch(S, next_sketch_seed()); }
+
+void generate_count_sketch(tomos::Sparse& sketch) {
+    csr_sketch_impl(sketch, next_sketch_seed(), 0, "generate_count_sketch");
+}
+
+void apply_count_sketch(tomos::Sparse& sketch, tomos::Dense& source, tomos::Dense& target) {
+    // one CSR product per column of the source
+    const auto ss = sketch.get_size(), xs = source.get_size(), ys = target.get_size();
+    if (xs[0] != ss[1] || ys[0] != ss[0] || ys[1] != xs[1])
+        throw std::invalid_argument("apply_count_sketch: sizes do not match (S k x m, source m x n, target k x n).");
+    const NumericType xt = source.get_value_type(), yt = target.get_value_type();
+    if ((xt != NumericType::FP32 && xt != NumericType::FP64) || (yt != NumericType::FP32 && yt != NumericType::FP64))
+        throw std::invalid_argument("apply_count_sketch: source and target must be FP32 or FP64.");
+    for (std::size_t j = 0; j < xs[1]; ++j)
+        check(tomos_csr_mv(abi_type(sketch.get_value_type()), abi_index(sketch.get_index_type()), 0, 1.0,
+                           static_cast<int64_t>(ss[0]), static_cast<int64_t>(ss[1]),
+                           static_cast<int64_t>(sketch.get_nnz()), sketch.values(), sketch.row_ptrs(),
+                           sketch.col_idxs(), abi_type(xt),
+                           static_cast<char*>(source.data()) + j * source.get_lead_dim() * numeric_type_size(xt), 0.0,
+                           abi_type(yt),
+                           static_cast<char*>(target.data()) + j * target.get_lead_dim() * numeric_type_size(yt),
+                           nullptr, 0, sketch.stream()),
+              "apply_count_sketch");
+}
+
 Status create_gaussian_sketch(tomos::Dense& S, std::uint64_t seed) {
     return dispatch::GaussianSketch::get().lookup_create_dense(S.get_device(), S.get_value_type())(S, seed);
 }
 
-Status create_gaussian_sketch(tomos::Dense& S) {
-    const std::uint64_t k = g_gauss_calls.fetch_add(1);
-    return create_gaussian_sketch(S, 0x6a09e667f3bcc909ull + k * 0x9e3779b97f4a7c15ull);
-}
+Status create_gaussian_sketch(tomos::Dense& S) { return create_gaussian_sketch(S, next_sketch_seed()); }
 
 CountSketch create_count_sketch(std::size_t k, std::size_t m) { return CountSketch{{k, m}}; }
 
@@ -660,6 +855,27 @@ Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_typ
 
 Workspace create_lsqr_workspace(Context& ctx, dim<2> size, NumericType value_type) {
     return create_lsqr_workspace(ctx, size, value_type, value_type);
+}
+
+Status mv_apply(Device dev, TransposeType trans, double alpha, tomos::Sparse& A, tomos::Dense& x, double beta,
+                tomos::Dense& y) {
+    device_guard(dev, "mv_apply");
+    dispatch::Sparse::get().lookup_mv(dev, A.get_value_type(), x.get_value_type(), y.get_value_type(),
+                                      A.get_index_type())(trans, alpha, A, x, beta, y);
+    return SUCCESS;
+}
+
+void lsqr_solve(Context context, tomos::Sparse& A, tomos::Dense& b, tomos::Dense& x, Workspace& workspace,
+                double tol, int max_iters, double max_relres, int* iterations,
+                std::vector<double>* residual_history, int restarts, tomos::Sparse* mtx_refine) {
+    if (mtx_refine && mtx_refine->get_value_type() != b.get_value_type())
+        throw std::invalid_argument(
+            "lsqr_solve: mtx_refine_precision's NumericType must match b/x's (refine_type).");
+    auto fn = dispatch::Lsqr::get().lookup_solve_sparse(context.get_device(), A.get_value_type(),
+                                                        workspace.value_type_, b.get_value_type(),
+                                                        workspace.precond_type_);
+    fn(context, A, b, x, workspace, tol, max_iters, max_relres, iterations, residual_history, restarts,
+       mtx_refine);
 }
 
 void lsqr_solve(Context context, tomos::Dense& A, tomos::Dense& b, tomos::Dense& x, Workspace& workspace,

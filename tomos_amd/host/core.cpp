@@ -1,4 +1,5 @@
 // Dense / Workspace / Context implementation over the libtomos_hip C ABI.
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -135,6 +136,89 @@ void Dense::copy_from_host(const void* src, std::size_t bytes) {
 void Dense::copy_to_host(void* dst, std::size_t bytes) const {
     if (bytes > storage_bytes()) throw std::invalid_argument("Dense::copy_to_host: too many bytes");
     check(tomos_memcpy_dtoh(dst, values_, bytes, stream_), "Dense::copy_to_host");
+}
+
+namespace {
+std::size_t index_size(IntType t) { return t == IntType::INT64 ? 8 : 4; }
+}  // namespace
+
+Sparse Sparse::create_csr(Context& ctx, NumericType value_type, IntType index_type, dim<2> size,
+                          std::size_t nnz) {
+    require_device(ctx.get_device(), "Sparse::create_csr");
+    if (value_type != NumericType::FP64 && value_type != NumericType::FP32 && value_type != NumericType::FP16 &&
+        value_type != NumericType::BF16)
+        throw std::invalid_argument("Sparse::create_csr: values must be FP64, FP32, FP16 or BF16.");
+    if (index_type != IntType::INT32 && index_type != IntType::INT64)
+        throw std::invalid_argument("Sparse::create_csr: bad index type.");
+    if (index_type == IntType::INT32 && (nnz > 0x7FFFFFFFu || size[0] > 0x7FFFFFFFu || size[1] > 0x7FFFFFFFu))
+        throw std::invalid_argument("Sparse::create_csr: sizes exceed INT32 indices.");
+    Sparse sp;
+    sp.device_ = ctx.get_device();
+    sp.type_ = value_type;
+    sp.index_ = index_type;
+    sp.size_ = size;
+    sp.nnz_ = nnz;
+    sp.stream_ = ctx.get_stream();
+    const std::size_t is = index_size(index_type);
+    const std::size_t pb = (size[0] + 1) * is, ib = std::max<std::size_t>(nnz, 1) * is,
+                      vb = std::max<std::size_t>(nnz, 1) * numeric_type_size(value_type);
+    try {
+        check(tomos_malloc(&sp.row_ptrs_, pb), "Sparse::create_csr");
+        check(tomos_malloc(&sp.col_idxs_, ib), "Sparse::create_csr");
+        check(tomos_malloc(&sp.values_, vb), "Sparse::create_csr");
+        // an all-zero CSR is valid: every row empty (row_ptrs[m] = 0 = nnz only when nnz == 0)
+        check(tomos_memset(sp.row_ptrs_, 0, pb, sp.stream_), "Sparse::create_csr");
+        check(tomos_memset(sp.col_idxs_, 0, ib, sp.stream_), "Sparse::create_csr");
+        check(tomos_memset(sp.values_, 0, vb, sp.stream_), "Sparse::create_csr");
+    } catch (...) {
+        sp.release();
+        throw;
+    }
+    return sp;
+}
+
+void Sparse::release() {
+    if (values_) tomos_free(values_);
+    if (row_ptrs_) tomos_free(row_ptrs_);
+    if (col_idxs_) tomos_free(col_idxs_);
+    values_ = row_ptrs_ = col_idxs_ = nullptr;
+}
+
+Sparse::Sparse(Sparse&& o) noexcept { *this = std::move(o); }
+
+Sparse& Sparse::operator=(Sparse&& o) noexcept {
+    if (this != &o) {
+        release();
+        device_ = o.device_;
+        type_ = o.type_;
+        index_ = o.index_;
+        size_ = o.size_;
+        nnz_ = o.nnz_;
+        values_ = o.values_;
+        row_ptrs_ = o.row_ptrs_;
+        col_idxs_ = o.col_idxs_;
+        stream_ = o.stream_;
+        o.values_ = o.row_ptrs_ = o.col_idxs_ = nullptr;
+    }
+    return *this;
+}
+
+Sparse::~Sparse() { release(); }
+
+void Sparse::copy_from_host(const void* rp, const void* ci, const void* vals) {
+    const std::size_t is = index_size(index_);
+    if (rp) check(tomos_memcpy_htod(row_ptrs_, rp, (size_[0] + 1) * is, stream_), "Sparse::copy_from_host");
+    if (ci && nnz_) check(tomos_memcpy_htod(col_idxs_, ci, nnz_ * is, stream_), "Sparse::copy_from_host");
+    if (vals && nnz_)
+        check(tomos_memcpy_htod(values_, vals, nnz_ * numeric_type_size(type_), stream_), "Sparse::copy_from_host");
+}
+
+void Sparse::copy_to_host(void* rp, void* ci, void* vals) const {
+    const std::size_t is = index_size(index_);
+    if (rp) check(tomos_memcpy_dtoh(rp, row_ptrs_, (size_[0] + 1) * is, stream_), "Sparse::copy_to_host");
+    if (ci && nnz_) check(tomos_memcpy_dtoh(ci, col_idxs_, nnz_ * is, stream_), "Sparse::copy_to_host");
+    if (vals && nnz_)
+        check(tomos_memcpy_dtoh(vals, values_, nnz_ * numeric_type_size(type_), stream_), "Sparse::copy_to_host");
 }
 
 Workspace::Workspace(Device device, std::size_t nbytes) : device_(device), capacity_(nbytes) {
