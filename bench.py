@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--fused", type=int, default=1,
                    help="single-pass LSQR matvec kernel (A read once per iteration)")
     p.add_argument("--gemm", type=int, default=8192, help="split GEMM size (0 = skip)")
+    p.add_argument("--sparse-row", type=int, default=50,
+                   help="entries per row of the CSR LSQR leg (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
                    help="batched differentiable least squares: number of uneven problems (0 = skip)")
     p.add_argument("--stream-loads", type=int, default=0, choices=(0, 1, 2),
@@ -113,6 +115,81 @@ def time_pair(C, DeviceBuffer, args, m, n, wh, ev0, ev1):
     f = ctypes.c_float()
     C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
     return f.value / args.gemv_reps
+
+
+def sparse_leg(args, C, DeviceBuffer, ev0, ev1):
+    """CSR LSQR at the tall shape: m x n with args.sparse_row entries per row
+    at uniform random columns (seed 8), N(0,1) FP32 values, int32 indices;
+    b = A x_true.  Times A p (the CSR row kernel, HBM-bound), the LSQR
+    iteration (A p + A^T u from the panel-blocked transpose + vector
+    kernels: a 40-iteration run minus a 0-iteration run), the setup
+    (validation, transpose, first residual) and a full unpreconditioned
+    FP32/FP64 solve to 1e-6 (the matrix is well conditioned: m >> n, i.i.d.
+    entries)."""
+    import ctypes
+    m, n, k = args.m, args.n, args.sparse_row
+    nnz = m * k
+    rng = np.random.default_rng(8)
+    rp = DeviceBuffer.from_numpy((np.arange(m + 1, dtype=np.int64) * k).astype(np.int32))
+    ci = DeviceBuffer.from_numpy(rng.integers(0, n, nnz, dtype=np.int32))
+    vals = DeviceBuffer(nnz * 4)
+    C.call("tomos_fill_normal_f32", vals.ptr, nnz, 1, nnz, 21, None, 0, 0, None)
+    xt = rng.standard_normal(n)
+    xd = DeviceBuffer.from_numpy(xt)
+    bd = DeviceBuffer(m * 8)
+    C.call("tomos_csr_mv", C.F32, C.INDEX_INT32, 0, 1.0, m, n, nnz, vals.ptr, rp.ptr, ci.ptr, C.F64, xd.ptr, 0.0,
+           C.F64, bd.ptr, None, 0, None)
+    # A p: rows of A
+    y = DeviceBuffer(m * 8)
+    sn = C.lib().tomos_csr_mv_scratch_size(C.F32, C.INDEX_INT32, 0, m, n, nnz)
+    scn = DeviceBuffer(sn)
+    t_n = time_call(lambda: C.call("tomos_csr_mv", C.F32, C.INDEX_INT32, 0, 1.0, m, n, nnz, vals.ptr, rp.ptr,
+                                   ci.ptr, C.F64, xd.ptr, 0.0, C.F64, y.ptr, scn.ptr, sn, None), 10, ev0, ev1)
+    del y, scn
+    # the solve
+    opt = C.LsqrOptions(C.F32, C.TIER_DEFAULT, C.TIER_DEFAULT, C.F32, C.F64, -1, 1e-6, 1e-6, 200, 1, 8, 0)
+    ab = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
+    sb = C.lib().tomos_lsqr_csr_scratch_size(ctypes.byref(opt), C.INDEX_INT32, m, n, nnz)
+    arena, scr = DeviceBuffer(ab), DeviceBuffer(sb)
+    x = DeviceBuffer(n * 8)
+    its = ctypes.c_int(0)
+    solve = lambda: C.call("tomos_lsqr_solve_csr", ctypes.byref(opt), C.INDEX_INT32, m, n, nnz, vals.ptr, rp.ptr,
+                           ci.ptr, -1, None, bd.ptr, x.ptr, None, 1, arena.ptr, ab, scr.ptr, sb, ctypes.byref(its),
+                           None, 0, None, None)
+    solve()
+    C.call("tomos_device_synchronize")
+    t0 = time.perf_counter()
+    solve()
+    C.call("tomos_device_synchronize")
+    t_solve = (time.perf_counter() - t0) * 1e3
+    iters = its.value
+    err = float(np.linalg.norm(x.to_numpy(np.float64, (n,)) - xt) / np.linalg.norm(xt))
+
+    # per-iteration cost: a 40-iteration run (tolerance out of reach) minus
+    # a 0-iteration run (validation, transpose, first residual)
+    def timed(max_iters):
+        o = C.LsqrOptions(C.F32, C.TIER_DEFAULT, C.TIER_DEFAULT, C.F32, C.F64, -1, 1e-30, 1e-30, max_iters, 1,
+                          1000, 0)
+        C.call("tomos_device_synchronize")
+        t = time.perf_counter()
+        C.call("tomos_lsqr_solve_csr", ctypes.byref(o), C.INDEX_INT32, m, n, nnz, vals.ptr, rp.ptr, ci.ptr, -1,
+               None, bd.ptr, x.ptr, None, 1, arena.ptr, ab, scr.ptr, sb, ctypes.byref(its), None, 0, None, None)
+        C.call("tomos_device_synchronize")
+        return (time.perf_counter() - t) * 1e3
+    timed(0)
+    t_setup = timed(0)
+    t_iter = (timed(40) - t_setup) / 40.0
+    gbs = lambda byts, ms: byts / (ms * 1e-3) / 1e9
+    n_bytes = nnz * 8.0 + (m + 1) * 4.0 + n * 8.0 + m * 8.0
+    t_bytes = nnz * 8.0 + (n + 1) * 4.0 + m * 8.0 + n * 8.0
+    t_t = t_iter - t_n  # the iteration's A^T u (plus its few vector kernels)
+    return {"m": m, "n": n, "nnz": nnz, "values": "f32", "index": "int32",
+            "Ap_ms": round(t_n, 4), "Ap_GBps": round(gbs(n_bytes, t_n), 1),
+            "iteration_ms": round(t_iter, 4), "iteration_GBps": round(gbs(n_bytes + t_bytes, t_iter), 1),
+            "ATu_ms_iteration_minus_Ap": round(t_t, 4), "ATu_GBps": round(gbs(t_bytes, t_t), 1),
+            "setup_ms": round(t_setup, 2),
+            "solve_ms": round(t_solve, 2), "iterations": iters, "err_vs_x_true": err,
+            "bytes_per_product": {"Ap": n_bytes, "ATu": t_bytes}}
 
 
 def batched_leg(args, C, DeviceBuffer, ev0, ev1):
@@ -535,6 +612,12 @@ def main():
         batched = batched_leg(args, C, DeviceBuffer, ev0, ev1)
         log("batched: %s" % batched)
 
+    # ---- sparse (CSR) LSQR at the tall shape ----------------------------------
+    sparse = None
+    if args.sparse_row > 0:
+        sparse = sparse_leg(args, C, DeviceBuffer, ev0, ev1)
+        log("sparse: %s" % sparse)
+
     # ---- CPU baseline: the oracle's OpenBLAS FP32 LSQR on a row sample -------
     cpu = None
     if not args.no_cpu:
@@ -591,6 +674,7 @@ def main():
         "gemm_split": gemm,
         "gaussian_sketch": gaussian,
         "batched_lstsq": batched,
+        "sparse_lsqr": sparse,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -36,6 +36,8 @@ SHAPES = [
     (500, 2000, 30.0),   # 16 lanes
     (64, 5000, 300.0),   # 64 lanes
     (200000, 100, 5.0),  # more rows than one pass of the grid
+    (8, 100000, 20000.0),  # few long rows: rows split into segments
+    (300, 50000, 2000.0),  # split, fewer segments per row
     (3, 4, 0.0),         # no entries
     (0, 5, 0.0),         # no rows
 ]
@@ -178,6 +180,19 @@ def test_matches_oracle_iteration_by_iteration(capi, orc):
     assert abs(r(x) - r(xo)) <= 1e-3 * r(xo)
     x2, it2, hist2 = H.lsqr_csr(A, b, tol=1e-9, max_iters=400)
     assert it2 == it and np.array_equal(x2, x) and np.array_equal(hist2, hist)
+
+
+def test_tall_matrix_uses_split_transpose_and_matches_oracle(capi, orc):
+    """A^T of a 100000 x 50 matrix has 50 rows of ~16000 entries: the
+    segmented kernel forms A^T u; iterates still follow the oracle."""
+    rp, ci, v, D, xt, b = sparse_problem(14, m=100000, n=50, mean=8.0)
+    A = H.Csr(D.shape[0], D.shape[1], rp, ci, v, C.F64)
+    x, it, hist = H.lsqr_csr(A, b, tol=1e-9, max_iters=300)
+    xo, ito, histo = orc.lsqr_f64(D, b, 1e-9, 300, 1e-9)
+    assert abs(it - ito) <= 1
+    assert np.allclose(hist[:15], histo[:15], rtol=1e-9, atol=0)
+    x2, it2, _ = H.lsqr_csr(A, b, tol=1e-9, max_iters=300)
+    assert it2 == it and np.array_equal(x2, x)
 
 
 @pytest.mark.parametrize("it_type", [C.INDEX_INT32, C.INDEX_INT64])

@@ -14,6 +14,8 @@
 // here A^T is built once as a CSR matrix of its own (a stable radix sort of
 // the column indices keeps A's row order inside each row of A^T) and the
 // A*x kernel runs on it: no atomics, deterministic, coalesced.
+#include <type_traits>
+
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
@@ -26,7 +28,7 @@ namespace tomos_hip {
 namespace {
 
 constexpr int kCsrThreads = 256;
-constexpr int64_t kCsrMaxBlocks = 1024;  // also bounds the sum-of-squares partials
+constexpr int64_t kCsrMaxBlocks = 8192;  // also bounds the sum-of-squares partials
 
 template <int VT>
 __device__ __forceinline__ double load_val(const void* v, int64_t k) {
@@ -50,12 +52,30 @@ struct MvArgs {
     const void* x;
     void* y;
     bool x64, y64;
+    bool xlds;  // x staged in LDS as FP64 (n <= kXLdsMax)
     Coef c;
 };
+
+// x of at most this many entries is copied to LDS by every block: gathers
+// from LDS instead of cache lines
+constexpr int64_t kXLdsMax = 8192;
+
+extern __shared__ double csr_xs[];
+
+__device__ __forceinline__ double load_x(const MvArgs& a, uint64_t col) {
+    if (a.xlds) return csr_xs[col];
+    return a.x64 ? static_cast<const double*>(a.x)[col] : static_cast<double>(static_cast<const float*>(a.x)[col]);
+}
 
 template <int VT, class I, int L>
 __global__ __launch_bounds__(kCsrThreads) void csr_mv_kernel(MvArgs a) {
     if (a.c.skip && *a.c.skip) return;
+    if (a.xlds) {
+        for (int64_t j = threadIdx.x; j < a.n; j += kCsrThreads)
+            csr_xs[j] = a.x64 ? static_cast<const double*>(a.x)[j]
+                              : static_cast<double>(static_cast<const float*>(a.x)[j]);
+        __syncthreads();
+    }
     const double alpha = a.c.dalpha ? *a.c.dalpha * a.c.alpha_mul : a.c.alpha;
     const double beta = a.c.dbeta ? *a.c.dbeta * a.c.beta_mul : a.c.beta;
     const I* rp = static_cast<const I*>(a.rp);
@@ -68,15 +88,21 @@ __global__ __launch_bounds__(kCsrThreads) void csr_mv_kernel(MvArgs a) {
         const int64_t k0 = clamp_ptr(rp[row], a.nnz);
         int64_t k1 = clamp_ptr(rp[row + 1], a.nnz);
         k1 = k1 < k0 ? k0 : k1;
-        double acc = 0.0;
-        for (int64_t k = k0 + lane; k < k1; k += L) {
-            const uint64_t col = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
-            if (col < static_cast<uint64_t>(a.n)) {
-                const double xv = a.x64 ? static_cast<const double*>(a.x)[col]
-                                        : static_cast<double>(static_cast<const float*>(a.x)[col]);
-                acc = fma(load_val<VT>(a.vals, k), xv, acc);
-            }
+        // two independent chains keep two entries' loads in flight per lane
+        double acc0 = 0.0, acc1 = 0.0;
+        int64_t k = k0 + lane;
+        for (; k + L < k1; k += 2 * L) {
+            const uint64_t c0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
+            const uint64_t c1 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + L]));
+            const double v0 = load_val<VT>(a.vals, k), v1 = load_val<VT>(a.vals, k + L);
+            if (c0 < static_cast<uint64_t>(a.n)) acc0 = fma(v0, load_x(a, c0), acc0);
+            if (c1 < static_cast<uint64_t>(a.n)) acc1 = fma(v1, load_x(a, c1), acc1);
         }
+        if (k < k1) {
+            const uint64_t c0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
+            if (c0 < static_cast<uint64_t>(a.n)) acc0 = fma(load_val<VT>(a.vals, k), load_x(a, c0), acc0);
+        }
+        double acc = acc0 + acc1;
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
         if (lane == 0) {
@@ -107,9 +133,168 @@ __global__ __launch_bounds__(kCsrThreads) void csr_mv_kernel(MvArgs a) {
     }
 }
 
+// ---- long rows: each row split into S segments ------------------------------
+// A matrix of few, long rows (A^T of a tall LSQR matrix: n rows of nnz/n
+// entries each) leaves most of the chip idle with one row group per row, so
+// block (s, r) sums the s-th of S equal segments of row r into
+// part[r * S + s] and a second kernel adds each row's S partials in order.
+struct SegArgs {
+    int64_t rows, n, nnz, S;
+    const void* vals;
+    const void* rp;
+    const void* ci;
+    const void* x;
+    bool x64;
+    const int* skip;
+    double* part;
+};
+
+template <int VT, class I>
+__global__ __launch_bounds__(kCsrThreads) void csr_seg_kernel(SegArgs a) {
+    if (a.skip && *a.skip) return;
+    const int64_t r = blockIdx.y, sgi = blockIdx.x;
+    const I* rp = static_cast<const I*>(a.rp);
+    const I* ci = static_cast<const I*>(a.ci);
+    const int64_t k0 = clamp_ptr(rp[r], a.nnz);
+    int64_t k1 = clamp_ptr(rp[r + 1], a.nnz);
+    k1 = k1 < k0 ? k0 : k1;
+    const int64_t seg = (k1 - k0 + a.S - 1) / a.S;
+    const int64_t b = k0 + sgi * seg;
+    const int64_t e = b + seg < k1 ? b + seg : k1;
+    double acc = 0.0;
+    for (int64_t k = b + threadIdx.x; k < e; k += kCsrThreads) {
+        const uint64_t col = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
+        if (col < static_cast<uint64_t>(a.n)) {
+            const double xv = a.x64 ? static_cast<const double*>(a.x)[col]
+                                    : static_cast<double>(static_cast<const float*>(a.x)[col]);
+            acc = fma(load_val<VT>(a.vals, k), xv, acc);
+        }
+    }
+    __shared__ double part[kCsrThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kCsrThreads / 64; ++w) t += part[w];
+        a.part[r * a.S + sgi] = t;
+    }
+}
+
+__global__ __launch_bounds__(kCsrThreads) void csr_seg_finish_kernel(int64_t rows, int64_t S, const double* part,
+                                                                     int64_t rs, int64_t ss, void* y, bool y64,
+                                                                     Coef c) {
+    if (c.skip && *c.skip) return;
+    const double alpha = c.dalpha ? *c.dalpha * c.alpha_mul : c.alpha;
+    const double beta = c.dbeta ? *c.dbeta * c.beta_mul : c.beta;
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * kCsrThreads + threadIdx.x;
+    double sq = 0.0;
+    if (r < rows) {
+        double acc = 0.0;
+        for (int64_t s = 0; s < S; ++s) acc += part[r * rs + s * ss];
+        double out = alpha * acc;
+        if (beta != 0.0)
+            out += beta * (y64 ? static_cast<double*>(y)[r] : static_cast<double>(static_cast<float*>(y)[r]));
+        if (y64) {
+            static_cast<double*>(y)[r] = out;
+            sq = out * out;
+        } else {
+            const float o32 = static_cast<float>(out);
+            static_cast<float*>(y)[r] = o32;
+            sq = static_cast<double>(o32) * o32;
+        }
+    }
+    if (c.sumsq) {
+        __shared__ double p[kCsrThreads / 64];
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if (threadIdx.x % 64 == 0) p[threadIdx.x / 64] = sq;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kCsrThreads / 64; ++w) t += p[w];
+            c.sumsq[blockIdx.x] = t;
+        }
+    }
+}
+
+// ---- A^T x by panels, the panel's slice of x in LDS -------------------------
+// Block (q, p) loads x[p*PR, p*PR + len) into LDS and forms, for columns c
+// of chunk q, part[p*nout + c] = sum over row c of panel p of the blocked
+// transpose (entries index A's rows, all inside the panel's slice).
+constexpr int kPanelThreads = 1024;
+constexpr int64_t kPanelLdsBytes = 65536;
+
+struct PanelArgs {
+    int64_t panels, nout, nnz, panel_rows, mrows, chunk;
+    const void* vals;
+    const void* rp;
+    const void* ci;
+    const void* x;
+    bool x64;
+    const int* skip;
+    double* part;
+};
+
+template <int VT, class I, bool X64>
+__global__ __launch_bounds__(kPanelThreads) void csr_panel_tmv_kernel(PanelArgs a) {
+    if (a.skip && *a.skip) return;
+    using XT = typename std::conditional<X64, double, float>::type;
+    XT* xs = reinterpret_cast<XT*>(csr_xs);
+    const int64_t p = blockIdx.y;
+    const int64_t base = p * a.panel_rows;
+    const int64_t len = std::min<int64_t>(a.panel_rows, a.mrows - base);
+    const XT* xg = static_cast<const XT*>(a.x) + base;
+    for (int64_t i = threadIdx.x; i < len; i += kPanelThreads) xs[i] = xg[i];
+    __syncthreads();
+    const I* rp = static_cast<const I*>(a.rp);
+    const I* ci = static_cast<const I*>(a.ci);
+    const int lane = threadIdx.x % 64;
+    const int64_t c0 = static_cast<int64_t>(blockIdx.x) * a.chunk;
+    const int64_t c1 = std::min<int64_t>(c0 + a.chunk, a.nout);
+    for (int64_t c = c0 + threadIdx.x / 64; c < c1; c += kPanelThreads / 64) {
+        const int64_t r = p * a.nout + c;
+        const int64_t k0 = clamp_ptr(rp[r], a.nnz);
+        int64_t k1 = clamp_ptr(rp[r + 1], a.nnz);
+        k1 = k1 < k0 ? k0 : k1;
+        double acc0 = 0.0, acc1 = 0.0;
+        int64_t k = k0 + lane;
+        for (; k + 64 < k1; k += 128) {
+            const uint64_t o0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]) - base);
+            const uint64_t o1 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 64]) - base);
+            const double v0 = load_val<VT>(a.vals, k), v1 = load_val<VT>(a.vals, k + 64);
+            if (o0 < static_cast<uint64_t>(len)) acc0 = fma(v0, static_cast<double>(xs[o0]), acc0);
+            if (o1 < static_cast<uint64_t>(len)) acc1 = fma(v1, static_cast<double>(xs[o1]), acc1);
+        }
+        if (k < k1) {
+            const uint64_t o0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]) - base);
+            if (o0 < static_cast<uint64_t>(len)) acc0 = fma(load_val<VT>(a.vals, k), static_cast<double>(xs[o0]), acc0);
+        }
+        double acc = acc0 + acc1;
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) a.part[r] = acc;
+    }
+}
+
+template <int VT, class I>
+void launch_panel(const PanelArgs& g, dim3 grid, size_t lds, hipStream_t s) {
+    if (g.x64) csr_panel_tmv_kernel<VT, I, true><<<grid, kPanelThreads, lds, s>>>(g);
+    else csr_panel_tmv_kernel<VT, I, false><<<grid, kPanelThreads, lds, s>>>(g);
+}
+
+// segments per row for the split path, or 0 for the row-group kernel
+int64_t seg_count(int64_t m, int64_t nnz) {
+    if (m <= 0 || m > 65535) return 0;
+    const double mean = static_cast<double>(nnz) / static_cast<double>(m);
+    if (mean < 1024.0) return 0;
+    const int64_t want = ceil_div(8192, m);                                 // about 8192 blocks in all
+    const int64_t most = static_cast<int64_t>(mean / 256.0);                // segments of >= 256 entries
+    return std::max<int64_t>(1, std::min<int64_t>(std::min(want, most), 65535));
+}
+
+// about four entries per lane
 int lanes_for(int64_t m, int64_t nnz) {
     const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
-    return mean <= 2.0 ? 1 : mean <= 12.0 ? 4 : mean <= 48.0 ? 16 : 64;
+    return mean <= 4.0 ? 1 : mean <= 16.0 ? 4 : mean <= 32.0 ? 8 : mean <= 64.0 ? 16 : mean <= 128.0 ? 32 : 64;
 }
 
 int64_t mv_blocks(int64_t m, int lanes) {
@@ -119,11 +304,14 @@ int64_t mv_blocks(int64_t m, int lanes) {
 
 template <int VT, class I>
 void launch_mv_vt(const MvArgs& a, int lanes, unsigned blocks, hipStream_t s) {
+    const size_t lds = a.xlds ? static_cast<size_t>(a.n) * sizeof(double) : 0;
     switch (lanes) {
-        case 1: csr_mv_kernel<VT, I, 1><<<blocks, kCsrThreads, 0, s>>>(a); break;
-        case 4: csr_mv_kernel<VT, I, 4><<<blocks, kCsrThreads, 0, s>>>(a); break;
-        case 16: csr_mv_kernel<VT, I, 16><<<blocks, kCsrThreads, 0, s>>>(a); break;
-        default: csr_mv_kernel<VT, I, 64><<<blocks, kCsrThreads, 0, s>>>(a); break;
+        case 1: csr_mv_kernel<VT, I, 1><<<blocks, kCsrThreads, lds, s>>>(a); break;
+        case 4: csr_mv_kernel<VT, I, 4><<<blocks, kCsrThreads, lds, s>>>(a); break;
+        case 8: csr_mv_kernel<VT, I, 8><<<blocks, kCsrThreads, lds, s>>>(a); break;
+        case 16: csr_mv_kernel<VT, I, 16><<<blocks, kCsrThreads, lds, s>>>(a); break;
+        case 32: csr_mv_kernel<VT, I, 32><<<blocks, kCsrThreads, lds, s>>>(a); break;
+        default: csr_mv_kernel<VT, I, 64><<<blocks, kCsrThreads, lds, s>>>(a); break;
     }
 }
 
@@ -215,29 +403,50 @@ size_t sort_temp_bytes(int64_t nnz, int64_t n) {
     return bytes;
 }
 
+// panel keys: (row / panel_rows) * n + column
 template <class U>
-size_t transpose_bytes(int64_t nnz, int64_t n) {
-    const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1)) * sizeof(U);
-    return 3 * align256(e) + align256(sort_temp_bytes<U>(nnz, n));
+__global__ void csr_panel_key_kernel(int64_t nnz, int64_t n, int64_t panel_rows, const U* rows, const U* ci,
+                                     U* keys) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride)
+        keys[k] = static_cast<U>((static_cast<int64_t>(rows[k]) / panel_rows) * n + static_cast<int64_t>(ci[k]));
 }
 
 template <class U>
-void transpose_impl(const CsrMat& A, void* vals_t, void* ptr_t, void* idx_t, void* scratch, hipStream_t s) {
+size_t transpose_bytes(int64_t nnz, int64_t keys) {
+    const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1)) * sizeof(U);
+    return 4 * align256(e) + align256(sort_temp_bytes<U>(nnz, keys));
+}
+
+// CSR of the panel-blocked transpose: row p*n + c holds column c of A's
+// rows [p*panel_rows, (p+1)*panel_rows) in A's row order (panels = 1: A^T).
+template <class U>
+void transpose_impl(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr_t, void* idx_t, void* scratch,
+                    hipStream_t s) {
+    const int64_t panels = A.m > 0 ? ceil_div(A.m, panel_rows) : 1;
+    const int64_t nkeys = panels * A.n;
     const size_t e = static_cast<size_t>(std::max<int64_t>(A.nnz, 1)) * sizeof(U);
     char* sc = static_cast<char*>(scratch);
     U* keys_out = reinterpret_cast<U*>(sc);
     U* perm = reinterpret_cast<U*>(sc + align256(e));
     U* rows = reinterpret_cast<U*>(sc + 2 * align256(e));
-    void* temp = sc + 3 * align256(e);
-    size_t temp_bytes = sort_temp_bytes<U>(A.nnz, A.n);
+    U* keys_in = reinterpret_cast<U*>(sc + 3 * align256(e));
+    void* temp = sc + 4 * align256(e);
+    size_t temp_bytes = sort_temp_bytes<U>(A.nnz, nkeys);
     if (A.nnz > 0) {
         csr_row_of_kernel<U><<<grid_1d(A.m), 256, 0, s>>>(A.m, static_cast<const U*>(A.row_ptrs), rows);
         TOMOS_LAUNCH_CHECK("csr_row_of_kernel");
+        const U* keys = static_cast<const U*>(A.col_idxs);
+        if (panels > 1) {
+            csr_panel_key_kernel<U><<<grid_1d(A.nnz), 256, 0, s>>>(A.nnz, A.n, panel_rows, rows, keys, keys_in);
+            TOMOS_LAUNCH_CHECK("csr_panel_key_kernel");
+            keys = keys_in;
+        }
         rocprim::counting_iterator<U> iota(U(0));
-        TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(temp, temp_bytes, static_cast<const U*>(A.col_idxs), keys_out,
-                                                  iota, perm, static_cast<size_t>(A.nnz), 0, key_bits(A.n), s));
+        TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_out, iota, perm,
+                                                  static_cast<size_t>(A.nnz), 0, key_bits(nkeys), s));
     }
-    csr_ptr_from_keys_kernel<U><<<grid_1d(A.nnz + 1), 256, 0, s>>>(A.nnz, A.n, keys_out, static_cast<U*>(ptr_t));
+    csr_ptr_from_keys_kernel<U><<<grid_1d(A.nnz + 1), 256, 0, s>>>(A.nnz, nkeys, keys_out, static_cast<U*>(ptr_t));
     TOMOS_LAUNCH_CHECK("csr_ptr_from_keys_kernel");
     if (A.nnz == 0) return;
     const unsigned g = grid_1d(A.nnz);
@@ -269,10 +478,18 @@ void csr_require_valid_types(const CsrMat& A, const char* who) {
     require((A.vals && A.col_idxs) || A.nnz == 0, w + ": null values or column indices");
 }
 
-int64_t csr_sumsq_blocks_max(int64_t m) { return std::min<int64_t>(ceil_div(std::max<int64_t>(m, 1), 4), kCsrMaxBlocks); }
+int64_t csr_sumsq_blocks_max(int64_t m) {
+    return std::max<int64_t>(std::min<int64_t>(ceil_div(std::max<int64_t>(m, 1), 4), kCsrMaxBlocks),
+                             ceil_div(std::max<int64_t>(m, 1), kCsrThreads));
+}
 
-void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_type, void* y, hipStream_t s,
-            int* sumsq_blocks) {
+size_t csr_mv_scratch_bytes(const CsrMat& A) {
+    const int64_t S = seg_count(A.m, A.nnz);
+    return S ? align256(static_cast<size_t>(A.m * S) * sizeof(double)) : 0;
+}
+
+void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_type, void* y, void* scratch,
+            size_t scratch_bytes, hipStream_t s, int* sumsq_blocks) {
     require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "csr_mv: x must be F32 or F64");
     require(y_type == TOMOS_F32 || y_type == TOMOS_F64, "csr_mv: y must be F32 or F64");
     if (A.m == 0) {
@@ -280,7 +497,37 @@ void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_typ
         return;
     }
     require(y != nullptr && (x != nullptr || A.n == 0), "csr_mv: null x or y");
-    MvArgs a{A.m, A.n, A.nnz, A.vals, A.row_ptrs, A.col_idxs, x, y, x_type == TOMOS_F64, y_type == TOMOS_F64, c};
+    const int64_t S = seg_count(A.m, A.nnz);
+    if (S) {
+        require(scratch && scratch_bytes >= csr_mv_scratch_bytes(A), "csr_mv: scratch too small");
+        SegArgs g{A.m, A.n, A.nnz, S, A.vals, A.row_ptrs, A.col_idxs, x, x_type == TOMOS_F64, c.skip,
+                  static_cast<double*>(scratch)};
+        const dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>(A.m));
+        if (A.idx_type == TOMOS_INDEX_INT64) {
+            switch (A.val_type) {
+                case TOMOS_F64: csr_seg_kernel<TOMOS_F64, int64_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                case TOMOS_F32: csr_seg_kernel<TOMOS_F32, int64_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                case TOMOS_F16: csr_seg_kernel<TOMOS_F16, int64_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                default: csr_seg_kernel<TOMOS_BF16, int64_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+            }
+        } else {
+            switch (A.val_type) {
+                case TOMOS_F64: csr_seg_kernel<TOMOS_F64, int32_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                case TOMOS_F32: csr_seg_kernel<TOMOS_F32, int32_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                case TOMOS_F16: csr_seg_kernel<TOMOS_F16, int32_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+                default: csr_seg_kernel<TOMOS_BF16, int32_t><<<grid, kCsrThreads, 0, s>>>(g); break;
+            }
+        }
+        TOMOS_LAUNCH_CHECK("csr_seg_kernel");
+        const unsigned fb = static_cast<unsigned>(ceil_div(A.m, kCsrThreads));
+        csr_seg_finish_kernel<<<fb, kCsrThreads, 0, s>>>(A.m, S, static_cast<const double*>(scratch), S, 1, y,
+                                                         y_type == TOMOS_F64, c);
+        TOMOS_LAUNCH_CHECK("csr_seg_finish_kernel");
+        if (sumsq_blocks) *sumsq_blocks = static_cast<int>(fb);
+        return;
+    }
+    MvArgs a{A.m,   A.n, A.nnz, A.vals, A.row_ptrs, A.col_idxs, x, y, x_type == TOMOS_F64, y_type == TOMOS_F64,
+             A.n > 0 && A.n <= kXLdsMax && A.nnz >= 4 * A.n, c};
     const int lanes = lanes_for(A.m, A.nnz);
     const unsigned blocks = static_cast<unsigned>(mv_blocks(A.m, lanes));
     if (A.idx_type == TOMOS_INDEX_INT64) launch_mv<int64_t>(A.val_type, a, lanes, blocks, s);
@@ -312,18 +559,81 @@ void csr_check(const CsrMat& A, void* scratch, size_t scratch_bytes, hipStream_t
     require(!(f & 8u), "CSR: column index out of range");
 }
 
-size_t csr_transpose_scratch_bytes(const CsrMat& A) {
-    return A.idx_type == TOMOS_INDEX_INT64 ? transpose_bytes<uint64_t>(A.nnz, A.n)
-                                           : transpose_bytes<uint32_t>(A.nnz, A.n);
+int64_t csr_panel_rows(const CsrMat& A, int x_type) {
+    const int64_t pr = kPanelLdsBytes / (x_type == TOMOS_F64 ? 8 : 4);
+    if (A.m <= pr || A.n == 0) return std::max<int64_t>(A.m, 1);
+    int64_t panels = ceil_div(A.m, pr);
+    // keys p*n + c must fit the index type
+    const int64_t key_max = A.idx_type == TOMOS_INDEX_INT64 ? (int64_t(1) << 62) : int64_t(0x7FFFFFFF);
+    panels = std::max<int64_t>(1, std::min<int64_t>(panels, key_max / A.n));
+    const int64_t rows = ceil_div(A.m, panels);
+    require(rows <= pr, "CSR transpose: too many panels for the index type");
+    return rows;
 }
 
-void csr_transpose(const CsrMat& A, void* vals_t, void* ptr_t, void* idx_t, void* scratch, size_t scratch_bytes,
-                   hipStream_t s) {
-    require(scratch_bytes >= csr_transpose_scratch_bytes(A), "csr_transpose: scratch too small");
+int64_t csr_panels(const CsrMat& A, int64_t panel_rows) { return A.m > 0 ? ceil_div(A.m, panel_rows) : 1; }
+
+size_t csr_transpose_scratch_bytes(const CsrMat& A, int64_t panel_rows) {
+    const int64_t nkeys = csr_panels(A, panel_rows) * A.n;
+    return A.idx_type == TOMOS_INDEX_INT64 ? transpose_bytes<uint64_t>(A.nnz, nkeys)
+                                           : transpose_bytes<uint32_t>(A.nnz, nkeys);
+}
+
+void csr_transpose(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr_t, void* idx_t, void* scratch,
+                   size_t scratch_bytes, hipStream_t s) {
+    require(panel_rows >= 1, "csr_transpose: bad panel size");
+    require(scratch_bytes >= csr_transpose_scratch_bytes(A, panel_rows), "csr_transpose: scratch too small");
     require(ptr_t != nullptr && ((vals_t && idx_t) || A.nnz == 0), "csr_transpose: null output");
     // the index buffers hold non-negative values: sorted as unsigned
-    if (A.idx_type == TOMOS_INDEX_INT64) transpose_impl<uint64_t>(A, vals_t, ptr_t, idx_t, scratch, s);
-    else transpose_impl<uint32_t>(A, vals_t, ptr_t, idx_t, scratch, s);
+    if (A.idx_type == TOMOS_INDEX_INT64) transpose_impl<uint64_t>(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
+    else transpose_impl<uint32_t>(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
+}
+
+size_t csr_tmv_scratch_bytes(const CsrMat& Tb, int64_t panels, int64_t nout) {
+    (void)Tb;
+    return align256(static_cast<size_t>(std::max<int64_t>(panels * nout, 1)) * sizeof(double));
+}
+
+void csr_tmv(const CsrMat& Tb, int64_t panels, int64_t panel_rows, int64_t nout, const Coef& c, int x_type,
+             const void* x, int y_type, void* y, void* scratch, size_t scratch_bytes, hipStream_t s,
+             int* sumsq_blocks) {
+    require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "csr_tmv: x must be F32 or F64");
+    require(panel_rows * (x_type == TOMOS_F64 ? 8 : 4) <= kPanelLdsBytes, "csr_tmv: panel does not fit LDS");
+    require(scratch_bytes >= csr_tmv_scratch_bytes(Tb, panels, nout), "csr_tmv: scratch too small");
+    if (nout == 0) {
+        if (sumsq_blocks) *sumsq_blocks = 0;
+        return;
+    }
+    // per-panel partial products, then each output's panels added in order
+    double* part = static_cast<double*>(scratch);
+    // about 1024 blocks in all; a block's columns share its LDS slice
+    const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, panels), ceil_div(nout, 16)));
+    PanelArgs g{panels, nout, Tb.nnz, panel_rows, Tb.n, ceil_div(nout, chunks), Tb.vals, Tb.row_ptrs,
+                Tb.col_idxs, x, x_type == TOMOS_F64, c.skip, part};
+    const dim3 grid(static_cast<unsigned>(ceil_div(nout, g.chunk)), static_cast<unsigned>(panels));
+    const size_t lds = static_cast<size_t>(std::min<int64_t>(panel_rows, std::max<int64_t>(Tb.n, 1))) *
+                       (x_type == TOMOS_F64 ? 8 : 4);
+    if (Tb.idx_type == TOMOS_INDEX_INT64) {
+        switch (Tb.val_type) {
+            case TOMOS_F64: launch_panel<TOMOS_F64, int64_t>(g, grid, lds, s); break;
+            case TOMOS_F32: launch_panel<TOMOS_F32, int64_t>(g, grid, lds, s); break;
+            case TOMOS_F16: launch_panel<TOMOS_F16, int64_t>(g, grid, lds, s); break;
+            default: launch_panel<TOMOS_BF16, int64_t>(g, grid, lds, s); break;
+        }
+    } else {
+        switch (Tb.val_type) {
+            case TOMOS_F64: launch_panel<TOMOS_F64, int32_t>(g, grid, lds, s); break;
+            case TOMOS_F32: launch_panel<TOMOS_F32, int32_t>(g, grid, lds, s); break;
+            case TOMOS_F16: launch_panel<TOMOS_F16, int32_t>(g, grid, lds, s); break;
+            default: launch_panel<TOMOS_BF16, int32_t>(g, grid, lds, s); break;
+        }
+    }
+    TOMOS_LAUNCH_CHECK("csr_panel_tmv_kernel");
+    const unsigned fb = static_cast<unsigned>(ceil_div(std::max<int64_t>(nout, 1), kCsrThreads));
+    require(fb <= kCsrMaxBlocks || c.sumsq == nullptr, "csr_tmv: too many outputs for the partial sums");
+    csr_seg_finish_kernel<<<fb, kCsrThreads, 0, s>>>(nout, panels, part, 1, nout, y, y_type == TOMOS_F64, c);
+    TOMOS_LAUNCH_CHECK("csr_seg_finish_kernel");
+    if (sumsq_blocks) *sumsq_blocks = static_cast<int>(fb);
 }
 
 }  // namespace tomos_hip
@@ -363,7 +673,7 @@ size_t tomos_csr_transpose_scratch_size(int val_type, int idx_type, int64_t m, i
     const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, nullptr, nullptr, nullptr);
     if (m < 0 || n < 0 || nnz < 0) return 0;
     try {
-        return csr_transpose_scratch_bytes(A);
+        return csr_transpose_scratch_bytes(A, std::max<int64_t>(m, 1));
     } catch (...) {
         return 0;
     }
@@ -376,23 +686,27 @@ int tomos_csr_transpose(int val_type, int idx_type, int64_t m, int64_t n, int64_
         const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, vals, row_ptrs, col_idxs);
         csr_require_valid_types(A, "tomos_csr_transpose");
         csr_check(A, fallback_scratch(csr_check_scratch_bytes()), csr_check_scratch_bytes(), st);
-        const size_t need = csr_transpose_scratch_bytes(A);
+        const size_t need = csr_transpose_scratch_bytes(A, std::max<int64_t>(m, 1));
         if (!scratch) {
             scratch = fallback_scratch(need);
             scratch_bytes = need;
         }
-        csr_transpose(A, vals_t, row_ptrs_t, col_idxs_t, scratch, scratch_bytes, st);
+        csr_transpose(A, std::max<int64_t>(m, 1), vals_t, row_ptrs_t, col_idxs_t, scratch, scratch_bytes, st);
     });
 }
 
 size_t tomos_csr_mv_scratch_size(int val_type, int idx_type, int trans, int64_t m, int64_t n, int64_t nnz) {
     if (m < 0 || n < 0 || nnz < 0) return 0;
-    if (!trans) return 256;
     const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, nullptr, nullptr, nullptr);
+    if (!trans) return std::max<size_t>(256, csr_mv_scratch_bytes(A));
     try {
+        // sized for FP64 x: the most panels
+        const int64_t pr = csr_panel_rows(A, TOMOS_F64), panels = csr_panels(A, pr);
+        const CsrMat T = make_csr(val_type, idx_type, panels * n, m, nnz, nullptr, nullptr, nullptr);
         const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1));
         return align256(e * csr_value_bytes(val_type)) + align256(e * csr_index_bytes(idx_type)) +
-               align256(static_cast<size_t>(n + 1) * csr_index_bytes(idx_type)) + csr_transpose_scratch_bytes(A);
+               align256(static_cast<size_t>(panels * n + 1) * csr_index_bytes(idx_type)) +
+               std::max(csr_transpose_scratch_bytes(A, pr), csr_tmv_scratch_bytes(T, panels, n));
     } catch (...) {
         return 0;
     }
@@ -408,18 +722,20 @@ int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m,
         Coef c;
         c.alpha = alpha;
         c.beta = beta;
-        if (trans == TOMOS_NO_TRANSPOSE) {
-            csr_mv(A, c, x_type, x, y_type, y, st);
-            return;
-        }
-        // A^T x: transpose into scratch, then the row kernel on A^T
         const size_t need = tomos_csr_mv_scratch_size(val_type, idx_type, trans, m, n, nnz);
         if (!scratch) {
             scratch = fallback_scratch(need);
             scratch_bytes = need;
         }
         require(scratch_bytes >= need, "tomos_csr_mv: scratch too small");
+        if (trans == TOMOS_NO_TRANSPOSE) {
+            csr_mv(A, c, x_type, x, y_type, y, scratch, scratch_bytes, st);
+            return;
+        }
+        // A^T x: transpose into scratch, then the row kernels on A^T
         csr_check(A, scratch, scratch_bytes, st);
+        require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "tomos_csr_mv: x must be F32 or F64");
+        const int64_t pr = csr_panel_rows(A, x_type), panels = csr_panels(A, pr);
         const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1));
         char* sc = static_cast<char*>(scratch);
         void* vt = sc;
@@ -427,10 +743,10 @@ int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m,
         void* it = sc;
         sc += align256(e * csr_index_bytes(idx_type));
         void* pt = sc;
-        sc += align256(static_cast<size_t>(n + 1) * csr_index_bytes(idx_type));
-        csr_transpose(A, vt, pt, it, sc, csr_transpose_scratch_bytes(A), st);
-        const CsrMat T = make_csr(val_type, idx_type, n, m, nnz, vt, pt, it);
-        csr_mv(T, c, x_type, x, y_type, y, st);
+        sc += align256(static_cast<size_t>(panels * n + 1) * csr_index_bytes(idx_type));
+        csr_transpose(A, pr, vt, pt, it, sc, csr_transpose_scratch_bytes(A, pr), st);
+        const CsrMat T = make_csr(val_type, idx_type, panels * n, m, nnz, vt, pt, it);
+        csr_tmv(T, panels, pr, n, c, x_type, x, y_type, y, sc, csr_tmv_scratch_bytes(T, panels, n), st);
     });
 }
 

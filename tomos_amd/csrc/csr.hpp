@@ -28,8 +28,11 @@ void csr_require_valid_types(const CsrMat& A, const char* who);
 // coef.sumsq, their count into *sumsq_blocks).  Memory-safe for any index
 // contents: row ranges are clamped to [0, nnz] and out-of-range column
 // indices contribute nothing (csr_check reports both).
-void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_type, void* y,
-            hipStream_t s, int* sumsq_blocks = nullptr);
+// scratch: csr_mv_scratch_bytes(A) bytes (row-segment partial sums of a
+// matrix with few, long rows; 0 otherwise).
+void csr_mv(const CsrMat& A, const Coef& c, int x_type, const void* x, int y_type, void* y, void* scratch,
+            size_t scratch_bytes, hipStream_t s, int* sumsq_blocks = nullptr);
+size_t csr_mv_scratch_bytes(const CsrMat& A);
 int64_t csr_sumsq_blocks_max(int64_t m);
 
 // Device-side validation: row_ptrs[0] = 0, non-decreasing, row_ptrs[m] =
@@ -37,12 +40,26 @@ int64_t csr_sumsq_blocks_max(int64_t m);
 void csr_check(const CsrMat& A, void* scratch, size_t scratch_bytes, hipStream_t s);
 size_t csr_check_scratch_bytes();
 
-// CSR of A^T (n x m) into (vals_t, ptr_t, idx_t): within each row of A^T
-// the entries keep A's row order, so products with it are deterministic.
-// A must be valid (csr_check).  scratch: csr_transpose_scratch_bytes().
-size_t csr_transpose_scratch_bytes(const CsrMat& A);
-void csr_transpose(const CsrMat& A, void* vals_t, void* ptr_t, void* idx_t, void* scratch,
+// Panel-blocked transpose of A: A's rows are cut into panels of
+// panel_rows rows and row p*n + c of the result (a CSR matrix of
+// panels*n rows over A's m columns) holds column c of panel p, in A's row
+// order; panel_rows >= m gives plain A^T (n x m).  Products with it are
+// deterministic.  A must be valid (csr_check).  csr_panel_rows(A, x_type)
+// picks panels whose slice of the gathered vector (of x_type) fits the
+// 64 KB of LDS a product block stages it in.
+int64_t csr_panel_rows(const CsrMat& A, int x_type);
+int64_t csr_panels(const CsrMat& A, int64_t panel_rows);
+size_t csr_transpose_scratch_bytes(const CsrMat& A, int64_t panel_rows);
+void csr_transpose(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr_t, void* idx_t, void* scratch,
                    size_t scratch_bytes, hipStream_t s);
+
+// y (nout) = alpha * A^T x + beta * y from the blocked transpose Tb
+// (panels * nout rows): per-panel products, then each output's panel
+// partials summed in panel order.  Coef contract as csr_mv.
+size_t csr_tmv_scratch_bytes(const CsrMat& Tb, int64_t panels, int64_t nout);
+void csr_tmv(const CsrMat& Tb, int64_t panels, int64_t panel_rows, int64_t nout, const Coef& c, int x_type,
+             const void* x, int y_type, void* y, void* scratch, size_t scratch_bytes, hipStream_t s,
+             int* sumsq_blocks = nullptr);
 
 size_t csr_value_bytes(int val_type);
 size_t csr_index_bytes(int idx_type);

@@ -281,7 +281,9 @@ struct Layout {
     int hist_cap;
     // CSR A: A^T as CSR (values, row pointers, column indices) and the
     // transpose's working space
-    size_t csr_vals = 0, csr_ptr = 0, csr_idx = 0, csr_tmp = 0, csr_tmp_bytes = 0;
+    size_t csr_vals = 0, csr_ptr = 0, csr_idx = 0, csr_tmp = 0, csr_tmp_bytes = 0, csr_part = 0,
+           csr_part_bytes = 0;
+    int64_t csr_panel_rows = 0, csr_panels = 1;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -357,11 +359,23 @@ Layout csr_layout_for(const tomos_lsqr_options& o, const CsrMat& A) {
     Layout L = layout_for(od, A.m, A.n);
     const size_t e = static_cast<size_t>(std::max<int64_t>(A.nnz, 1));
     size_t off = L.scratch_total;
+    // the row-group kernel may use more blocks (sum-of-squares partials)
+    // than the dense GEMV
+    L.red = off; off = align256(off + static_cast<size_t>(std::max<int64_t>(csr_sumsq_blocks_max(A.m), kRedBlocks)) * 8);
+    L.csr_panel_rows = csr_panel_rows(A, o.value_type);
+    L.csr_panels = csr_panels(A, L.csr_panel_rows);
     L.csr_vals = off; off = align256(off + e * csr_value_bytes(A.val_type));
-    L.csr_ptr = off; off = align256(off + static_cast<size_t>(A.n + 1) * csr_index_bytes(A.idx_type));
+    L.csr_ptr = off;
+    off = align256(off + static_cast<size_t>(L.csr_panels * A.n + 1) * csr_index_bytes(A.idx_type));
     L.csr_idx = off; off = align256(off + e * csr_index_bytes(A.idx_type));
-    L.csr_tmp_bytes = csr_transpose_scratch_bytes(A);
+    L.csr_tmp_bytes = csr_transpose_scratch_bytes(A, L.csr_panel_rows);
     L.csr_tmp = off; off = align256(off + L.csr_tmp_bytes);
+    CsrMat T = A;
+    T.m = L.csr_panels * A.n;
+    T.n = A.m;
+    L.csr_part_bytes =
+        std::max<size_t>(std::max(csr_mv_scratch_bytes(A), csr_tmv_scratch_bytes(T, L.csr_panels, A.n)), 256);
+    L.csr_part = off; off = align256(off + L.csr_part_bytes);
     L.scratch_total = off;
     return L;
 }
@@ -438,8 +452,12 @@ struct Solver {
     void gemv_a(int tier, int trans, const Coef& c, const void* xin, bool x64, void* yout, bool y64,
                 int* nb = nullptr) {
         if (csr) {  // FP64 accumulation at every tier
-            csr_mv(trans == TOMOS_TRANSPOSE ? csr_t : *csr, c, x64 ? TOMOS_F64 : TOMOS_F32, xin,
-                   y64 ? TOMOS_F64 : TOMOS_F32, yout, s, nb);
+            if (trans == TOMOS_TRANSPOSE)
+                csr_tmv(csr_t, L.csr_panels, L.csr_panel_rows, n, c, x64 ? TOMOS_F64 : TOMOS_F32, xin,
+                        y64 ? TOMOS_F64 : TOMOS_F32, yout, sc + L.csr_part, L.csr_part_bytes, s, nb);
+            else
+                csr_mv(*csr, c, x64 ? TOMOS_F64 : TOMOS_F32, xin, y64 ? TOMOS_F64 : TOMOS_F32, yout,
+                       sc + L.csr_part, L.csr_part_bytes, s, nb);
             return;
         }
         gemv(o.a_type, tier, trans, c, m, n, a, wr, lda, x64 ? TOMOS_F64 : TOMOS_F32, xin,
@@ -464,7 +482,8 @@ struct Solver {
             cr.sumsq = red();
             int nb = 0;
             if (csr)
-                csr_mv(csr_ref, cr, r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), s, &nb);
+                csr_mv(csr_ref, cr, r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(),
+                       sc + L.csr_part, L.csr_part_bytes, s, &nb);
             else
                 gemv(ref_type, TOMOS_TIER_EXACT, TOMOS_NO_TRANSPOSE, cr, m, n, aref, nullptr, lda_ref,
                      r64 ? TOMOS_F64 : TOMOS_F32, x, v64 ? TOMOS_F64 : TOMOS_F32, U(), scratch(), L.gemv_bytes, s,
@@ -703,11 +722,11 @@ void lsqr_solve_csr(const tomos_lsqr_options& o, const CsrMat& A, int ref_type, 
     sv.p64 = o.precond_type >= 0 ? o.precond_type == TOMOS_F64 : sv.v64;
     // indices validated once (synchronous), then A^T built in the scratch
     csr_check(A, sv.sc + sv.L.csr_tmp, sv.L.csr_tmp_bytes, s);
-    csr_transpose(A, sv.sc + sv.L.csr_vals, sv.sc + sv.L.csr_ptr, sv.sc + sv.L.csr_idx, sv.sc + sv.L.csr_tmp,
-                  sv.L.csr_tmp_bytes, s);
+    csr_transpose(A, sv.L.csr_panel_rows, sv.sc + sv.L.csr_vals, sv.sc + sv.L.csr_ptr, sv.sc + sv.L.csr_idx,
+                  sv.sc + sv.L.csr_tmp, sv.L.csr_tmp_bytes, s);
     sv.csr = &A;
     sv.csr_t = A;
-    sv.csr_t.m = A.n;
+    sv.csr_t.m = sv.L.csr_panels * A.n;
     sv.csr_t.n = A.m;
     sv.csr_t.vals = sv.sc + sv.L.csr_vals;
     sv.csr_t.row_ptrs = sv.sc + sv.L.csr_ptr;
