@@ -347,12 +347,39 @@ __global__ void csr_check_kernel(int64_t m, int64_t n, int64_t nnz, const I* rp,
 
 // ---- transpose ----------------------------------------------------------------
 // row index of every stored entry (rows[k] = i for rp[i] <= k < rp[i+1])
+// 16 lanes per row: consecutive lanes write consecutive entries
 template <class I>
 __global__ void csr_row_of_kernel(int64_t m, const I* rp, I* rows) {
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride)
-        for (int64_t k = static_cast<int64_t>(rp[i]); k < static_cast<int64_t>(rp[i + 1]); ++k)
+    const int lane = threadIdx.x % 16;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * (blockDim.x / 16);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 16) + threadIdx.x / 16; i < m; i += stride)
+        for (int64_t k = static_cast<int64_t>(rp[i]) + lane; k < static_cast<int64_t>(rp[i + 1]); k += 16)
             rows[k] = static_cast<I>(i);
+}
+
+// 4-byte and 2-byte values with 32-bit indices travel through the sort
+// packed with their row index (row in the high word), so the transpose
+// needs no gather through the permutation
+template <int VT>
+__global__ void csr_pack_rowval_kernel(int64_t nnz, const uint32_t* rows, const void* vals, uint64_t* out) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+        uint32_t v;
+        if constexpr (VT == TOMOS_F32) v = static_cast<const uint32_t*>(vals)[k];
+        else v = static_cast<const uint16_t*>(vals)[k];
+        out[k] = (static_cast<uint64_t>(rows[k]) << 32) | v;
+    }
+}
+
+template <int VT>
+__global__ void csr_unpack_rowval_kernel(int64_t nnz, const uint64_t* in, void* vals_t, uint32_t* idx_t) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+        const uint64_t w = in[k];
+        idx_t[k] = static_cast<uint32_t>(w >> 32);
+        if constexpr (VT == TOMOS_F32) static_cast<uint32_t*>(vals_t)[k] = static_cast<uint32_t>(w);
+        else static_cast<uint16_t*>(vals_t)[k] = static_cast<uint16_t>(w);
+    }
 }
 
 // ptr_t[c] = first sorted position with key >= c, from the boundaries of
@@ -412,10 +439,70 @@ __global__ void csr_panel_key_kernel(int64_t nnz, int64_t n, int64_t panel_rows,
         keys[k] = static_cast<U>((static_cast<int64_t>(rows[k]) / panel_rows) * n + static_cast<int64_t>(ci[k]));
 }
 
+size_t sort_temp_bytes_packed(int64_t nnz, int64_t n) {
+    size_t bytes = 0;
+    TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const uint32_t*>(nullptr),
+                                              static_cast<uint32_t*>(nullptr), static_cast<const uint64_t*>(nullptr),
+                                              static_cast<uint64_t*>(nullptr), static_cast<size_t>(nnz), 0,
+                                              key_bits(n)));
+    return bytes;
+}
+
+// values of 4 or 2 bytes with 32-bit indices: sorted packed with the row
+bool packed_sort(const CsrMat& A) { return A.idx_type == TOMOS_INDEX_INT32 && A.val_type != TOMOS_F64; }
+
 template <class U>
 size_t transpose_bytes(int64_t nnz, int64_t keys) {
     const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1)) * sizeof(U);
     return 4 * align256(e) + align256(sort_temp_bytes<U>(nnz, keys));
+}
+
+size_t transpose_bytes_packed(int64_t nnz, int64_t keys) {
+    const size_t e = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+    // keys_out, keys_in (4 B), rows (4 B), packed in/out (8 B each)
+    return 3 * align256(4 * e) + 2 * align256(8 * e) + align256(sort_temp_bytes_packed(nnz, keys));
+}
+
+// transpose_impl for packed_sort(A): one sort carries (row, value)
+void transpose_packed(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr_t, void* idx_t, void* scratch,
+                      hipStream_t s) {
+    const int64_t panels = A.m > 0 ? ceil_div(A.m, panel_rows) : 1;
+    const int64_t nkeys = panels * A.n;
+    const size_t e = static_cast<size_t>(std::max<int64_t>(A.nnz, 1));
+    char* sc = static_cast<char*>(scratch);
+    uint32_t* keys_out = reinterpret_cast<uint32_t*>(sc);
+    uint32_t* keys_in = reinterpret_cast<uint32_t*>(sc + align256(4 * e));
+    uint32_t* rows = reinterpret_cast<uint32_t*>(sc + 2 * align256(4 * e));
+    uint64_t* pin = reinterpret_cast<uint64_t*>(sc + 3 * align256(4 * e));
+    uint64_t* pout = reinterpret_cast<uint64_t*>(sc + 3 * align256(4 * e) + align256(8 * e));
+    void* temp = sc + 3 * align256(4 * e) + 2 * align256(8 * e);
+    size_t temp_bytes = sort_temp_bytes_packed(A.nnz, nkeys);
+    if (A.nnz > 0) {
+        csr_row_of_kernel<uint32_t><<<grid_1d(A.m * 16), 256, 0, s>>>(A.m, static_cast<const uint32_t*>(A.row_ptrs),
+                                                                       rows);
+        TOMOS_LAUNCH_CHECK("csr_row_of_kernel");
+        const uint32_t* keys = static_cast<const uint32_t*>(A.col_idxs);
+        if (panels > 1) {
+            csr_panel_key_kernel<uint32_t><<<grid_1d(A.nnz), 256, 0, s>>>(A.nnz, A.n, panel_rows, rows, keys,
+                                                                          keys_in);
+            TOMOS_LAUNCH_CHECK("csr_panel_key_kernel");
+            keys = keys_in;
+        }
+        const unsigned g = grid_1d(A.nnz);
+        if (A.val_type == TOMOS_F32) csr_pack_rowval_kernel<TOMOS_F32><<<g, 256, 0, s>>>(A.nnz, rows, A.vals, pin);
+        else csr_pack_rowval_kernel<TOMOS_F16><<<g, 256, 0, s>>>(A.nnz, rows, A.vals, pin);
+        TOMOS_LAUNCH_CHECK("csr_pack_rowval_kernel");
+        TOMOS_HIP_CHECK(rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_out, pin, pout,
+                                                  static_cast<size_t>(A.nnz), 0, key_bits(nkeys), s));
+        if (A.val_type == TOMOS_F32)
+            csr_unpack_rowval_kernel<TOMOS_F32><<<g, 256, 0, s>>>(A.nnz, pout, vals_t, static_cast<uint32_t*>(idx_t));
+        else
+            csr_unpack_rowval_kernel<TOMOS_F16><<<g, 256, 0, s>>>(A.nnz, pout, vals_t, static_cast<uint32_t*>(idx_t));
+        TOMOS_LAUNCH_CHECK("csr_unpack_rowval_kernel");
+    }
+    csr_ptr_from_keys_kernel<uint32_t><<<grid_1d(A.nnz + 1), 256, 0, s>>>(A.nnz, nkeys, keys_out,
+                                                                          static_cast<uint32_t*>(ptr_t));
+    TOMOS_LAUNCH_CHECK("csr_ptr_from_keys_kernel");
 }
 
 // CSR of the panel-blocked transpose: row p*n + c holds column c of A's
@@ -434,7 +521,7 @@ void transpose_impl(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr
     void* temp = sc + 4 * align256(e);
     size_t temp_bytes = sort_temp_bytes<U>(A.nnz, nkeys);
     if (A.nnz > 0) {
-        csr_row_of_kernel<U><<<grid_1d(A.m), 256, 0, s>>>(A.m, static_cast<const U*>(A.row_ptrs), rows);
+        csr_row_of_kernel<U><<<grid_1d(A.m * 16), 256, 0, s>>>(A.m, static_cast<const U*>(A.row_ptrs), rows);
         TOMOS_LAUNCH_CHECK("csr_row_of_kernel");
         const U* keys = static_cast<const U*>(A.col_idxs);
         if (panels > 1) {
@@ -575,6 +662,7 @@ int64_t csr_panels(const CsrMat& A, int64_t panel_rows) { return A.m > 0 ? ceil_
 
 size_t csr_transpose_scratch_bytes(const CsrMat& A, int64_t panel_rows) {
     const int64_t nkeys = csr_panels(A, panel_rows) * A.n;
+    if (packed_sort(A)) return transpose_bytes_packed(A.nnz, nkeys);
     return A.idx_type == TOMOS_INDEX_INT64 ? transpose_bytes<uint64_t>(A.nnz, nkeys)
                                            : transpose_bytes<uint32_t>(A.nnz, nkeys);
 }
@@ -585,7 +673,8 @@ void csr_transpose(const CsrMat& A, int64_t panel_rows, void* vals_t, void* ptr_
     require(scratch_bytes >= csr_transpose_scratch_bytes(A, panel_rows), "csr_transpose: scratch too small");
     require(ptr_t != nullptr && ((vals_t && idx_t) || A.nnz == 0), "csr_transpose: null output");
     // the index buffers hold non-negative values: sorted as unsigned
-    if (A.idx_type == TOMOS_INDEX_INT64) transpose_impl<uint64_t>(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
+    if (packed_sort(A)) transpose_packed(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
+    else if (A.idx_type == TOMOS_INDEX_INT64) transpose_impl<uint64_t>(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
     else transpose_impl<uint32_t>(A, panel_rows, vals_t, ptr_t, idx_t, scratch, s);
 }
 
