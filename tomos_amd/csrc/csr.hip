@@ -88,21 +88,26 @@ __global__ __launch_bounds__(kCsrThreads) void csr_mv_kernel(MvArgs a) {
         const int64_t k0 = clamp_ptr(rp[row], a.nnz);
         int64_t k1 = clamp_ptr(rp[row + 1], a.nnz);
         k1 = k1 < k0 ? k0 : k1;
-        // two independent chains keep two entries' loads in flight per lane
-        double acc0 = 0.0, acc1 = 0.0;
+        // four independent chains keep four entries' loads in flight per lane
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
         int64_t k = k0 + lane;
-        for (; k + L < k1; k += 2 * L) {
+        for (; k + 3 * L < k1; k += 4 * L) {
             const uint64_t c0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
             const uint64_t c1 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + L]));
+            const uint64_t c2 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 2 * L]));
+            const uint64_t c3 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 3 * L]));
             const double v0 = load_val<VT>(a.vals, k), v1 = load_val<VT>(a.vals, k + L);
+            const double v2 = load_val<VT>(a.vals, k + 2 * L), v3 = load_val<VT>(a.vals, k + 3 * L);
             if (c0 < static_cast<uint64_t>(a.n)) acc0 = fma(v0, load_x(a, c0), acc0);
             if (c1 < static_cast<uint64_t>(a.n)) acc1 = fma(v1, load_x(a, c1), acc1);
+            if (c2 < static_cast<uint64_t>(a.n)) acc2 = fma(v2, load_x(a, c2), acc2);
+            if (c3 < static_cast<uint64_t>(a.n)) acc3 = fma(v3, load_x(a, c3), acc3);
         }
-        if (k < k1) {
+        for (; k < k1; k += L) {
             const uint64_t c0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
             if (c0 < static_cast<uint64_t>(a.n)) acc0 = fma(load_val<VT>(a.vals, k), load_x(a, c0), acc0);
         }
-        double acc = acc0 + acc1;
+        double acc = (acc0 + acc1) + (acc2 + acc3);
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
         if (lane == 0) {
@@ -256,20 +261,26 @@ __global__ __launch_bounds__(kPanelThreads) void csr_panel_tmv_kernel(PanelArgs 
         const int64_t k0 = clamp_ptr(rp[r], a.nnz);
         int64_t k1 = clamp_ptr(rp[r + 1], a.nnz);
         k1 = k1 < k0 ? k0 : k1;
-        double acc0 = 0.0, acc1 = 0.0;
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+        const uint64_t ulen = static_cast<uint64_t>(len);
         int64_t k = k0 + lane;
-        for (; k + 64 < k1; k += 128) {
+        for (; k + 192 < k1; k += 256) {
             const uint64_t o0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]) - base);
             const uint64_t o1 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 64]) - base);
+            const uint64_t o2 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 128]) - base);
+            const uint64_t o3 = static_cast<uint64_t>(static_cast<int64_t>(ci[k + 192]) - base);
             const double v0 = load_val<VT>(a.vals, k), v1 = load_val<VT>(a.vals, k + 64);
-            if (o0 < static_cast<uint64_t>(len)) acc0 = fma(v0, static_cast<double>(xs[o0]), acc0);
-            if (o1 < static_cast<uint64_t>(len)) acc1 = fma(v1, static_cast<double>(xs[o1]), acc1);
+            const double v2 = load_val<VT>(a.vals, k + 128), v3 = load_val<VT>(a.vals, k + 192);
+            if (o0 < ulen) acc0 = fma(v0, static_cast<double>(xs[o0]), acc0);
+            if (o1 < ulen) acc1 = fma(v1, static_cast<double>(xs[o1]), acc1);
+            if (o2 < ulen) acc2 = fma(v2, static_cast<double>(xs[o2]), acc2);
+            if (o3 < ulen) acc3 = fma(v3, static_cast<double>(xs[o3]), acc3);
         }
-        if (k < k1) {
+        for (; k < k1; k += 64) {
             const uint64_t o0 = static_cast<uint64_t>(static_cast<int64_t>(ci[k]) - base);
-            if (o0 < static_cast<uint64_t>(len)) acc0 = fma(load_val<VT>(a.vals, k), static_cast<double>(xs[o0]), acc0);
+            if (o0 < ulen) acc0 = fma(load_val<VT>(a.vals, k), static_cast<double>(xs[o0]), acc0);
         }
-        double acc = acc0 + acc1;
+        double acc = (acc0 + acc1) + (acc2 + acc3);
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
         if (lane == 0) a.part[r] = acc;
     }
@@ -279,6 +290,47 @@ template <int VT, class I>
 void launch_panel(const PanelArgs& g, dim3 grid, size_t lds, hipStream_t s) {
     if (g.x64) csr_panel_tmv_kernel<VT, I, true><<<grid, kPanelThreads, lds, s>>>(g);
     else csr_panel_tmv_kernel<VT, I, false><<<grid, kPanelThreads, lds, s>>>(g);
+}
+
+// y[r] = alpha * sum_p part[p*nout + r] + beta * y[r]: one wavefront per
+// output, lanes over the panels, a fixed butterfly order
+__global__ __launch_bounds__(kCsrThreads) void csr_panel_finish_kernel(int64_t nout, int64_t panels,
+                                                                       const double* part, void* y, bool y64,
+                                                                       Coef c) {
+    if (c.skip && *c.skip) return;
+    const double alpha = c.dalpha ? *c.dalpha * c.alpha_mul : c.alpha;
+    const double beta = c.dbeta ? *c.dbeta * c.beta_mul : c.beta;
+    const int lane = threadIdx.x % 64;
+    double sq = 0.0;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (kCsrThreads / 64) + threadIdx.x / 64; r < nout;
+         r += static_cast<int64_t>(gridDim.x) * (kCsrThreads / 64)) {
+        double acc = 0.0;
+        for (int64_t p = lane; p < panels; p += 64) acc += part[p * nout + r];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) {
+            double out = alpha * acc;
+            if (beta != 0.0)
+                out += beta * (y64 ? static_cast<double*>(y)[r] : static_cast<double>(static_cast<float*>(y)[r]));
+            if (y64) {
+                static_cast<double*>(y)[r] = out;
+                sq += out * out;
+            } else {
+                const float o32 = static_cast<float>(out);
+                static_cast<float*>(y)[r] = o32;
+                sq += static_cast<double>(o32) * o32;
+            }
+        }
+    }
+    if (c.sumsq) {
+        __shared__ double p4[kCsrThreads / 64];
+        if (lane == 0) p4[threadIdx.x / 64] = sq;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kCsrThreads / 64; ++w) t += p4[w];
+            c.sumsq[blockIdx.x] = t;
+        }
+    }
 }
 
 // segments per row for the split path, or 0 for the row-group kernel
@@ -291,10 +343,10 @@ int64_t seg_count(int64_t m, int64_t nnz) {
     return std::max<int64_t>(1, std::min<int64_t>(std::min(want, most), 65535));
 }
 
-// about four entries per lane
+// about eight entries per lane (two rounds of the four chains)
 int lanes_for(int64_t m, int64_t nnz) {
     const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
-    return mean <= 4.0 ? 1 : mean <= 16.0 ? 4 : mean <= 32.0 ? 8 : mean <= 64.0 ? 16 : mean <= 128.0 ? 32 : 64;
+    return mean <= 8.0 ? 1 : mean <= 32.0 ? 4 : mean <= 64.0 ? 8 : mean <= 128.0 ? 16 : mean <= 256.0 ? 32 : 64;
 }
 
 int64_t mv_blocks(int64_t m, int lanes) {
@@ -718,10 +770,9 @@ void csr_tmv(const CsrMat& Tb, int64_t panels, int64_t panel_rows, int64_t nout,
         }
     }
     TOMOS_LAUNCH_CHECK("csr_panel_tmv_kernel");
-    const unsigned fb = static_cast<unsigned>(ceil_div(std::max<int64_t>(nout, 1), kCsrThreads));
-    require(fb <= kCsrMaxBlocks || c.sumsq == nullptr, "csr_tmv: too many outputs for the partial sums");
-    csr_seg_finish_kernel<<<fb, kCsrThreads, 0, s>>>(nout, panels, part, 1, nout, y, y_type == TOMOS_F64, c);
-    TOMOS_LAUNCH_CHECK("csr_seg_finish_kernel");
+    const unsigned fb = static_cast<unsigned>(std::min<int64_t>(ceil_div(nout, kCsrThreads / 64), kCsrMaxBlocks));
+    csr_panel_finish_kernel<<<fb, kCsrThreads, 0, s>>>(nout, panels, part, y, y_type == TOMOS_F64, c);
+    TOMOS_LAUNCH_CHECK("csr_panel_finish_kernel");
     if (sumsq_blocks) *sumsq_blocks = static_cast<int>(fb);
 }
 
