@@ -342,22 +342,22 @@ __global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restri
 // 32 l + r) factored by one block of kTsqrComb threads.  The final R goes
 // to the first leaf's top rows (upper triangle) and its diagonal to d.diag;
 // the stacked reflectors to d.vc (explicit, unit diagonal), T to d.tc.
-__global__ __launch_bounds__(kTsqrComb) void qr_tsqr_combine_kernel(const QrDesc* __restrict__ D, int64_t c0,
-                                                                     int lr) {
+template <int NC>
+__global__ __launch_bounds__(NC) void qr_tsqr_combine_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
     const int64_t kk = d.k - c0;
     const int nleaves = static_cast<int>((kk + lr - 1) / lr);
     __shared__ double Ts[kNB * kNB];
-    __shared__ double sh[block_qr32_lds_doubles(kTsqrComb)];
+    __shared__ double sh[block_qr32_lds_doubles(NC)];
     const int t = threadIdx.x, l = t >> 5, r = t & 31;
     const int64_t row = c0 + static_cast<int64_t>(l) * lr + r;
     const bool valid = l < nleaves && r < min<int64_t>(lr, kk - static_cast<int64_t>(l) * lr);
     double a[kNB];
 #pragma unroll
     for (int q = 0; q < kNB; ++q) a[q] = (valid && q >= r && q < nbp) ? d.a[row + (c0 + q) * d.lda] : 0.0;
-    block_qr32<kTsqrComb>(a, nbp, Ts, sh);
+    block_qr32<NC>(a, nbp, Ts, sh);
     if (t < nbp) {
 #pragma unroll
         for (int q = 0; q < kNB; ++q)
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kTsqrComb) void qr_tsqr_combine_kernel(const QrDesc
     }
 #pragma unroll
     for (int q = 0; q < kNB; ++q) d.vc[t * kNB + q] = t > q ? a[q] : (t == q ? 1.0 : 0.0);
-    for (int i = t; i < kNB * kNB; i += kTsqrComb) d.tc[i] = Ts[i];
+    for (int i = t; i < kNB * kNB; i += NC) d.tc[i] = Ts[i];
 }
 
 // Trailing update with one level's block reflector: rows S of the trailing
@@ -840,15 +840,18 @@ void factor(const Batch& B, hipStream_t s) {
     qr_sync_reset_kernel<<<dim3(static_cast<unsigned>(ceil_div(kSyncHead + B.nmax, kT)), 1, P), kT, 0, s>>>(
         B.dev, B.status);
     TOMOS_LAUNCH_CHECK("qr_sync_reset_kernel");
-    // TSQR panels when every problem has at most 16 leaves of 256 or 512 rows
+    // TSQR panels when every problem has at most 16 leaves of 256 or 512 rows;
+    // at most 8 leaves (a 256-row combine) whenever 512-row leaves allow it
     const bool tsqr = B.kmax <= kTsqrMaxK;
-    const int lr = B.kmax <= kTsqrMaxLeaves * 256 ? 256 : 512;
+    const int lr = B.kmax <= 8 * 256 ? 256 : 512;
+    const bool comb256 = ceil_div(B.kmax, lr) <= 8;
     for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
         if (tsqr) {
             const unsigned nleaves = static_cast<unsigned>(ceil_div(B.kmax - c0, lr));
             if (lr == 256) qr_tsqr_leaf_kernel<256><<<dim3(nleaves, 1, P), 256, 0, s>>>(B.dev, c0);
             else qr_tsqr_leaf_kernel<512><<<dim3(nleaves, 1, P), 512, 0, s>>>(B.dev, c0);
-            qr_tsqr_combine_kernel<<<dim3(1, 1, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
+            if (comb256) qr_tsqr_combine_kernel<256><<<dim3(1, 1, P), 256, 0, s>>>(B.dev, c0, lr);
+            else qr_tsqr_combine_kernel<kTsqrComb><<<dim3(1, 1, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
             TOMOS_LAUNCH_CHECK("qr_tsqr panel");
             if (c0 + kNB >= B.nmax) break;
             const unsigned ctiles = static_cast<unsigned>(ceil_div(B.nmax - c0 - kNB, kCT));
@@ -856,7 +859,8 @@ void factor(const Batch& B, hipStream_t s) {
                 qr_tsqr_update_kernel<256, false><<<dim3(nleaves, ctiles, P), 256, 0, s>>>(B.dev, c0, lr);
             else
                 qr_tsqr_update_kernel<512, false><<<dim3(nleaves, ctiles, P), 512, 0, s>>>(B.dev, c0, lr);
-            qr_tsqr_update_kernel<kTsqrComb, true><<<dim3(1, ctiles, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
+            if (comb256) qr_tsqr_update_kernel<256, true><<<dim3(1, ctiles, P), 256, 0, s>>>(B.dev, c0, lr);
+            else qr_tsqr_update_kernel<kTsqrComb, true><<<dim3(1, ctiles, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
             TOMOS_LAUNCH_CHECK("qr_tsqr update");
             continue;
         }
