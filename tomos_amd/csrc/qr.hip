@@ -209,9 +209,34 @@ __device__ __forceinline__ void tr_stage(double* v, int lane) {
         v[t] = keep + __shfl_xor(send, 2 * HALF);
     }
 }
+// The two widest stages with gfx950's lane-swap instructions: for
+// HALF = 16 (partner lane ^ 32) v_permlane32_swap(v[t], v[t+16]) leaves
+// each lane its own half and its partner's matching half, for HALF = 8
+// (partner ^ 16) v_permlane16_swap does the same between 16-lane rows; the
+// sum of the two swapped registers is the stage's result, with no selects.
+__device__ __forceinline__ void swap_add(double& lo, double hi, bool rows16) {
+    const uint64_t a = __builtin_bit_cast(uint64_t, lo), b = __builtin_bit_cast(uint64_t, hi);
+    uint32_t a0, a1, b0, b1;
+    if (rows16) {
+        const auto l = __builtin_amdgcn_permlane16_swap(static_cast<uint32_t>(a), static_cast<uint32_t>(b), false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(static_cast<uint32_t>(a >> 32), static_cast<uint32_t>(b >> 32),
+                                                        false, false);
+        a0 = l[0]; b0 = l[1]; a1 = h[0]; b1 = h[1];
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap(static_cast<uint32_t>(a), static_cast<uint32_t>(b), false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(static_cast<uint32_t>(a >> 32), static_cast<uint32_t>(b >> 32),
+                                                        false, false);
+        a0 = l[0]; b0 = l[1]; a1 = h[0]; b1 = h[1];
+    }
+    lo = __builtin_bit_cast(double, (static_cast<uint64_t>(a1) << 32) | a0) +
+         __builtin_bit_cast(double, (static_cast<uint64_t>(b1) << 32) | b0);
+}
+
 __device__ __forceinline__ void wave_tr32(double* v, int lane) {
-    tr_stage<16>(v, lane);
-    tr_stage<8>(v, lane);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) swap_add(v[t], v[t + 16], false);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) swap_add(v[t], v[t + 8], true);
     tr_stage<4>(v, lane);
     tr_stage<2>(v, lane);
     tr_stage<1>(v, lane);
