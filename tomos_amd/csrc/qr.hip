@@ -199,16 +199,9 @@ __global__ __launch_bounds__(kPT) void qr_panel_kernel(const QrDesc* __restrict_
 
 // Wave transpose-reduce of 32 values per lane: afterwards v[0] of lane L
 // holds the wave sum of value (L >> 1) & 31 (lanes 2i and 2i+1 agree).
-template <int HALF>
-__device__ __forceinline__ void tr_stage(double* v, int lane) {
-    const bool up = (lane & (2 * HALF)) != 0;
-#pragma unroll
-    for (int t = 0; t < HALF; ++t) {
-        const double send = up ? v[t] : v[t + HALF];
-        const double keep = up ? v[t + HALF] : v[t];
-        v[t] = keep + __shfl_xor(send, 2 * HALF);
-    }
-}
+// Stage b (b = 5 .. 1) exchanges half of each lane's values with a partner
+// across bit b: the lane keeps the half its own bit selects and adds the
+// partner's copy of that half.
 // The two widest stages with gfx950's lane-swap instructions: for
 // HALF = 16 (partner lane ^ 32) v_permlane32_swap(v[t], v[t+16]) leaves
 // each lane its own half and its partner's matching half, for HALF = 8
@@ -232,15 +225,40 @@ __device__ __forceinline__ void swap_add(double& lo, double hi, bool rows16) {
          __builtin_bit_cast(double, (static_cast<uint64_t>(b1) << 32) | b0);
 }
 
+// The narrow stages with DPP lane reads inside 16-lane rows: the partner
+// only has to differ in the stage's bit and agree in the higher ones, so
+// row_mirror (lane ^ 15) serves for bit 3, row_half_mirror (lane ^ 7) for
+// bit 2, and quad permutations for bits 1 and 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_read(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), CTRL, 0xF, 0xF, true));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), CTRL, 0xF, 0xF, true));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+template <int HALF, int CTRL>
+__device__ __forceinline__ void tr_stage_dpp(double* v, int lane) {
+    const bool up = (lane & (2 * HALF)) != 0;
+#pragma unroll
+    for (int t = 0; t < HALF; ++t) {
+        const double send = up ? v[t] : v[t + HALF];
+        const double keep = up ? v[t + HALF] : v[t];
+        v[t] = keep + dpp_read<CTRL>(send);
+    }
+}
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppQuadXor2 = 0x4E, kDppQuadXor1 = 0xB1;
+
 __device__ __forceinline__ void wave_tr32(double* v, int lane) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) swap_add(v[t], v[t + 16], false);
 #pragma unroll
     for (int t = 0; t < 8; ++t) swap_add(v[t], v[t + 8], true);
-    tr_stage<4>(v, lane);
-    tr_stage<2>(v, lane);
-    tr_stage<1>(v, lane);
-    v[0] += __shfl_xor(v[0], 1);
+    tr_stage_dpp<4, kDppRowMirror>(v, lane);
+    tr_stage_dpp<2, kDppRowHalfMirror>(v, lane);
+    tr_stage_dpp<1, kDppQuadXor2>(v, lane);
+    v[0] += dpp_read<kDppQuadXor1>(v[0]);
 }
 
 // One column step of block_qr32 with the column index J a constant, so
