@@ -242,9 +242,17 @@ __global__ __launch_bounds__(kThreads) void gemv_n_finish(int64_t m, int splits,
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     double sq = 0.0;
     if (i < m) {
-        double s = 0.0;
-        for (int c = 0; c < splits; ++c)
-            s += static_cast<double>(partial[static_cast<int64_t>(c) * m + i]);
+        // four chains: the partial sums' loads overlap
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int c = 0;
+        for (; c + 3 < splits; c += 4) {
+            s0 += static_cast<double>(partial[static_cast<int64_t>(c) * m + i]);
+            s1 += static_cast<double>(partial[static_cast<int64_t>(c + 1) * m + i]);
+            s2 += static_cast<double>(partial[static_cast<int64_t>(c + 2) * m + i]);
+            s3 += static_cast<double>(partial[static_cast<int64_t>(c + 3) * m + i]);
+        }
+        for (; c < splits; ++c) s0 += static_cast<double>(partial[static_cast<int64_t>(c) * m + i]);
+        const double s = (s0 + s1) + (s2 + s3);
         const double r = store_y(y, y64, i, read_coef(coef.alpha, coef.dalpha, coef.alpha_mul), s,
                                  read_coef(coef.beta, coef.dbeta, coef.beta_mul));
         sq = r * r;
@@ -349,8 +357,16 @@ __global__ __launch_bounds__(kThreads) void gemv_t_finish(int64_t n, int chunks,
     if (coef.skip && *coef.skip) return;
     const int64_t j = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (j >= n) return;
-    double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += static_cast<double>(partial[static_cast<int64_t>(c) * n + j]);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = 0;
+    for (; c + 3 < chunks; c += 4) {
+        s0 += static_cast<double>(partial[static_cast<int64_t>(c) * n + j]);
+        s1 += static_cast<double>(partial[static_cast<int64_t>(c + 1) * n + j]);
+        s2 += static_cast<double>(partial[static_cast<int64_t>(c + 2) * n + j]);
+        s3 += static_cast<double>(partial[static_cast<int64_t>(c + 3) * n + j]);
+    }
+    for (; c < chunks; ++c) s0 += static_cast<double>(partial[static_cast<int64_t>(c) * n + j]);
+    const double s = (s0 + s1) + (s2 + s3);
     store_y(y, y64, j, read_coef(coef.alpha, coef.dalpha, coef.alpha_mul), s,
             read_coef(coef.beta, coef.dbeta, coef.beta_mul));
 }
@@ -396,7 +412,10 @@ PlanN plan_n(const Kind& k, int64_t m, int64_t n) {
     p.col_chunk = std::max<int64_t>(n, 1);
     if (p.bx < kTargetBlocks / 2 && n > 16) {
         int64_t want = ceil_div(kTargetBlocks, p.bx);
-        int64_t max_splits = ceil_div(n, 8);
+        // at most 32 column splits: the finishing pass adds them per row in
+        // sequence, and more blocks than that on a small matrix cost more
+        // there than they save in the pass over A
+        int64_t max_splits = std::min<int64_t>(ceil_div(n, 8), 32);
         int64_t s = std::min(want, max_splits);
         p.col_chunk = ceil_div(n, s);
         p.splits = static_cast<int>(ceil_div(n, p.col_chunk));
