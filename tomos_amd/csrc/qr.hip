@@ -261,21 +261,30 @@ __device__ __forceinline__ void wave_tr32(double* v, int lane) {
     v[0] += dpp_read<kDppQuadXor1>(v[0]);
 }
 
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), l));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // One column step of block_qr32 with the column index J a constant, so
-// every register index is static; recurses to the next column.
+// every register index is static; recurses to the next column.  One
+// barrier per column: after it every wavefront reads all waves' column sums
+// and forms the reflector and u itself (the same arithmetic in every wave,
+// so every wave gets the same bits); lane q holds u_q.
 template <int NT, int J>
 __device__ __forceinline__ void block_qr32_step(double (&a)[kNB], int nbp, double* Ts, double* sh) {
     if constexpr (J < kNB) {
         if (J >= nbp) return;
         constexpr int NW = NT / 64;
         const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-        // sh: red[2][NW][kNB] | rowj[2][kNB] | u[2][kNB] | par[2][4]; buffers
-        // alternate by step parity, so two barriers per column suffice
+        // sh: red[2][NW][kNB] | rowj[2][kNB]; the buffers alternate by step
+        // parity, so a buffer is rewritten only after every wave has passed
+        // the next step's barrier, i.e. finished reading it
         constexpr int b = J & 1;
         double* rd = sh + b * NW * kNB;
         double* rj = sh + 2 * NW * kNB + b * kNB;
-        double* u = sh + 2 * NW * kNB + 2 * kNB + b * kNB;
-        double* par = sh + 2 * NW * kNB + 4 * kNB + b * 4;
         if (t == J) {
 #pragma unroll
             for (int q = 0; q < kNB; ++q) rj[q] = a[q];
@@ -287,46 +296,38 @@ __device__ __forceinline__ void block_qr32_step(double (&a)[kNB], int nbp, doubl
         wave_tr32(vals, lane);
         if ((lane & 1) == 0) rd[wave * kNB + ((lane >> 1) & 31)] = vals[0];
         __syncthreads();
-        // first wave: tot_q = sum over rows below J of x * A(:, q) (fixed
-        // order), the reflector, and u_q = V(J, q) + inv * tot_q, which is
-        // s_q = v_J^T A(:, q) for q > J and z_q = v_q^T v_J for q < J
-        if (t < kNB) {
-            double tq = rd[t], tj = rd[J];
+        // tot_q = sum over rows below J of x * A(:, q) (fixed order), the
+        // reflector, and u_q = V(J, q) + inv * tot_q: s_q = v_J^T A(:, q) for
+        // q > J and z_q = v_q^T v_J for q < J
+        const int q = lane & 31;
+        double tq = rd[q], tj = rd[J];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) {
-                tq += rd[w * kNB + t];
-                tj += rd[w * kNB + J];
-            }
-            const double x0 = rj[J];
-            const double nrm = sqrt(x0 * x0 + tj);
-            const bool active = nrm != 0.0;
-            const double beta = x0 >= 0.0 ? -nrm : nrm;
-            const double inv = active ? 1.0 / (x0 - beta) : 0.0;
-            u[t] = rj[t] + inv * tq;
-            if (t == 0) {
-                par[0] = active ? (beta - x0) / beta : 0.0;  // tau
-                par[1] = inv;
-                par[2] = beta;
-                par[3] = active ? 1.0 : 0.0;
-            }
+        for (int w = 1; w < NW; ++w) {
+            tq += rd[w * kNB + q];
+            tj += rd[w * kNB + J];
         }
-        __syncthreads();
-        const double tau = par[0], inv = par[1], beta = par[2];
-        const bool upd = par[3] != 0.0 && t >= J;
+        const double x0 = rj[J];
+        const double nrm = sqrt(x0 * x0 + tj);
+        const bool active = nrm != 0.0;
+        const double beta = x0 >= 0.0 ? -nrm : nrm;
+        const double inv = active ? 1.0 / (x0 - beta) : 0.0;
+        const double tau = active ? (beta - x0) / beta : 0.0;
+        const double uq = rj[q] + inv * tq;
+        const bool upd = active && t >= J;
         const double v = t > J ? x * inv : 1.0;
         if (upd) {
             const double tv = tau * v;
 #pragma unroll
-            for (int l = J + 1; l < kNB; ++l) a[l] -= tv * u[l];
+            for (int l = J + 1; l < kNB; ++l) a[l] -= tv * read_lane(uq, l);
             a[J] = t > J ? v : beta;
         }
-        // T(0:J, J) = -tau T(0:J, 0:J) z (Ts columns < J were written before
-        // this step's barriers)
+        // T(0:J, J) = -tau T(0:J, 0:J) z (wave 0 holds rows t < kNB of T;
+        // its earlier columns were written by this same wave)
         if (t < J) {
             double acc = 0.0;
 #pragma unroll
             for (int pp = 0; pp < J; ++pp) {
-                const double term = Ts[pp * kNB + t] * u[pp];
+                const double term = Ts[pp * kNB + t] * read_lane(uq, pp);
                 acc += pp >= t ? term : 0.0;
             }
             Ts[J * kNB + t] = -tau * acc;
