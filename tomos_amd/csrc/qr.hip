@@ -760,24 +760,69 @@ __global__ __launch_bounds__(kT) void copy_r_kernel(const QrDesc* __restrict__ D
     d.wp[i + j * d.n] = d.r[i + j * d.ldr];
 }
 
-// Column j of R^-1 (problem blockIdx.z) from the copy in wp into d.r.
-__global__ __launch_bounds__(kT) void trtri_col_kernel(const QrDesc* __restrict__ D) {
+// Column j of R^-1 (problem blockIdx.z) from the copy in wp into d.r:
+// R x = e_j by back substitution in 32-row blocks.  For block I (bottom
+// up) the block's right-hand side e_j - R(I, >I) x(>I) is a GEMV over the
+// rows already solved (8 column strides x 32 rows, partial sums added in a
+// fixed order), then wavefront 0 solves the 32 x 32 diagonal block, one
+// lane per row, the solved entry broadcast by readlane.
+constexpr int kTri = 256;  // threads of trtri_col_kernel: 8 column slots x 32 rows
+
+__global__ __launch_bounds__(kTri) void trtri_col_kernel(const QrDesc* __restrict__ D) {
     extern __shared__ double b[];
+    __shared__ double part[kTri / 32][32];
     const QrDesc& d = D[blockIdx.z];
     const int64_t n = d.n;
     const int64_t j = blockIdx.x;
     if (j >= n) return;
     const double* rc = d.wp;
-    for (int64_t i = threadIdx.x; i <= j; i += kT) b[i] = (i == j) ? 1.0 : 0.0;
-    __syncthreads();
-    for (int64_t i = j; i >= 0; --i) {
-        const double xi = b[i] / rc[i + i * n];
+    const int t = threadIdx.x, r = t & 31, cs = t >> 5;  // row in block, column stride slot
+    for (int64_t I = j / 32; I >= 0; --I) {
+        const int64_t r0 = I * 32, r1 = min<int64_t>(r0 + 32, j + 1);
+        // partial sums of R(r0 + r, c) x(c) over solved columns c in [r1, j]
+        double acc = 0.0;
+        if (r0 + r < r1) {
+            constexpr int S = kTri / 32;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;  // four loads in flight
+            int64_t c = r1 + cs;
+            for (; c + 3 * S <= j; c += 4 * S) {
+                a0 += rc[r0 + r + c * n] * b[c];
+                a1 += rc[r0 + r + (c + S) * n] * b[c + S];
+                a2 += rc[r0 + r + (c + 2 * S) * n] * b[c + 2 * S];
+                a3 += rc[r0 + r + (c + 3 * S) * n] * b[c + 3 * S];
+            }
+            for (; c <= j; c += S) a0 += rc[r0 + r + c * n] * b[c];
+            acc = (a0 + a1) + (a2 + a3);
+        }
+        part[cs][r] = acc;
         __syncthreads();
-        for (int64_t q = threadIdx.x; q < i; q += kT) b[q] -= xi * rc[q + i * n];
-        if (threadIdx.x == 0) b[i] = xi;
+        if (t < 64) {
+            // lane t < 32 holds row r0 + t of the diagonal block
+            double rowv[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q)
+                rowv[q] = (t < 32 && r0 + t < r1 && r0 + q < r1) ? rc[r0 + t + (r0 + q) * n] : 0.0;
+            double rhs = 0.0;
+            if (r0 + t < r1) {
+                rhs = (r0 + t == j) ? 1.0 : 0.0;
+#pragma unroll
+                for (int q = 0; q < kTri / 32; ++q) rhs -= part[q][t];
+            }
+            // diagonal block: rows r0 .. r1-1, bottom up
+#pragma unroll
+            for (int lc = 31; lc >= 0; --lc) {
+                if (r0 + lc < r1) {
+                    const double mine = t == lc ? rhs / rowv[lc] : 0.0;
+                    const double xc = read_lane(mine, lc);
+                    if (t < lc) rhs -= rowv[lc] * xc;
+                    if (t == lc) rhs = xc;
+                }
+            }
+            if (r0 + t < r1 && t < 32) b[r0 + t] = rhs;
+        }
         __syncthreads();
     }
-    for (int64_t i = threadIdx.x; i < n; i += kT) d.r[i + j * d.ldr] = (i <= j) ? b[i] : 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kTri) d.r[i + j * d.ldr] = (i <= j) ? b[i] : 0.0;
 }
 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
@@ -966,7 +1011,7 @@ void invert(const Batch& B, hipStream_t s) {
     TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(trtri_col_kernel),
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         static_cast<int>(lds)));
-    trtri_col_kernel<<<dim3(static_cast<unsigned>(B.nmax), 1, P), kT, lds, s>>>(B.dev);
+    trtri_col_kernel<<<dim3(static_cast<unsigned>(B.nmax), 1, P), kTri, lds, s>>>(B.dev);
     TOMOS_LAUNCH_CHECK("trtri_col_kernel");
 }
 
