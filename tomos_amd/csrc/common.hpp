@@ -71,6 +71,15 @@ inline void require(bool cond, const std::string& msg) {
 // Number of compute units of the current device (cached per device).
 int cu_count();
 
+// A second stream of the current device (created on first use, kept until
+// the process ends) and events to fork work onto it and join it back.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[2] = {nullptr, nullptr};
+};
+const SideStream& side_stream();
+
 // Per-device grow-only scratch for callers that pass no scratch buffer.
 // Not thread-safe; kept until process exit.
 void* fallback_scratch(size_t bytes);

@@ -62,6 +62,10 @@ struct QrDesc {
 constexpr int kSyncHead = 64;  // ints before the column flags (own cache lines)
 constexpr int kTsqrMaxLeaves = 16;           // leaves per panel (tile rows 256 or 512)
 constexpr int kTsqrComb = kTsqrMaxLeaves * kNB;  // stacked rows of the combine step
+// doubles of one set of TSQR block-reflector factors (leaf T's, combine V
+// and T); two sets alternate by panel parity, so a panel's trailing update
+// can still read its set while the next panel is factored
+constexpr int64_t kTsqrSet = kTsqrMaxLeaves * kNB * kNB + kTsqrComb * kNB + kNB * kNB;
 constexpr int64_t kTsqrMaxK = 16 * 512;      // k beyond this: the wavefront panel path
 
 // Row splits of W = V^T A: 128-row splits, at most 32, so a panel's W has
@@ -359,7 +363,7 @@ __device__ __forceinline__ void block_qr32(double (&a)[kNB], int nbp, double* Ts
 // Leaf factorization: block (leaf l, problem z) factors rows
 // r0 = c0 + l*NT .. of the panel; R and V back into A, T into d.tl[l].
 template <int NT>
-__global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+__global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restrict__ D, int64_t c0, int par) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restri
         for (int q = 0; q < kNB; ++q)
             if (q < nbp) d.a[r0 + t + (c0 + q) * d.lda] = a[q];
     }
-    double* tl = d.tl + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
+    double* tl = d.tl + par * kTsqrSet + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
     for (int i = t; i < kNB * kNB; i += NT) tl[i] = Ts[i];
 }
 
@@ -387,7 +391,8 @@ __global__ __launch_bounds__(NT) void qr_tsqr_leaf_kernel(const QrDesc* __restri
 // to the first leaf's top rows (upper triangle) and its diagonal to d.diag;
 // the stacked reflectors to d.vc (explicit, unit diagonal), T to d.tc.
 template <int NC>
-__global__ __launch_bounds__(NC) void qr_tsqr_combine_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr) {
+__global__ __launch_bounds__(NC) void qr_tsqr_combine_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr,
+                                                             int par) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
@@ -410,9 +415,11 @@ __global__ __launch_bounds__(NC) void qr_tsqr_combine_kernel(const QrDesc* __res
         for (int q = 0; q < kNB; ++q)
             if (q == t) d.diag[c0 + t] = a[q];
     }
+    double* vc = d.vc + par * kTsqrSet;
+    double* tc = d.tc + par * kTsqrSet;
 #pragma unroll
-    for (int q = 0; q < kNB; ++q) d.vc[t * kNB + q] = t > q ? a[q] : (t == q ? 1.0 : 0.0);
-    for (int i = t; i < kNB * kNB; i += NC) d.tc[i] = Ts[i];
+    for (int q = 0; q < kNB; ++q) vc[t * kNB + q] = t > q ? a[q] : (t == q ? 1.0 : 0.0);
+    for (int i = t; i < kNB * kNB; i += NC) tc[i] = Ts[i];
 }
 
 // Trailing update with one level's block reflector: rows S of the trailing
@@ -426,12 +433,13 @@ constexpr int kCT = 16;
 constexpr int kChunk = 128;
 
 template <int NT, bool COMBINE>
-__global__ __launch_bounds__(NT) void qr_tsqr_update_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr) {
+__global__ __launch_bounds__(NT) void qr_tsqr_update_kernel(const QrDesc* __restrict__ D, int64_t c0, int lr,
+                                                            int par, int tile0) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
     const int64_t j0 = c0 + nbp, nt = d.n - j0;
-    const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * kCT;
+    const int64_t jt0 = static_cast<int64_t>(tile0 + static_cast<int>(blockIdx.y)) * kCT;
     if (nt <= 0 || jt0 >= nt) return;
     const int64_t kk = d.k - c0;
     const int t = threadIdx.x;
@@ -443,13 +451,13 @@ __global__ __launch_bounds__(NT) void qr_tsqr_update_kernel(const QrDesc* __rest
         const int nleaves = static_cast<int>((kk + lr - 1) / lr);
         row = c0 + static_cast<int64_t>(l) * lr + r;
         valid = l < nleaves && r < min<int64_t>(lr, kk - static_cast<int64_t>(l) * lr);
-        tsrc = d.tc;
+        tsrc = d.tc + par * kTsqrSet;
     } else {
         const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * NT;
         if (r0 >= d.k) return;
         row = r0 + t;
         valid = row < d.k;
-        tsrc = d.tl + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
+        tsrc = d.tl + par * kTsqrSet + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
     }
     __shared__ double sV[kChunk][kNB + 1];
     __shared__ double sS[kChunk][kCT + 1];
@@ -461,7 +469,7 @@ __global__ __launch_bounds__(NT) void qr_tsqr_update_kernel(const QrDesc* __rest
     for (int q = 0; q < kNB; ++q) {
         double x = 0.0;
         if (valid && q < nbp) {
-            if constexpr (COMBINE) x = d.vc[t * kNB + q];
+            if constexpr (COMBINE) x = d.vc[par * kTsqrSet + t * kNB + q];
             else x = t > q ? d.a[row + (c0 + q) * d.lda] : (t == q ? 1.0 : 0.0);
         }
         v[q] = x;
@@ -844,7 +852,7 @@ Sizes sizes(int64_t k, int64_t n) {
     z.wp = a256(static_cast<size_t>(std::max(qr_nsplit(kk) * nn * kNB, nn * nn)) * 8);
     z.w2 = a256(static_cast<size_t>(nn * kNB) * 8);
     z.sync = a256(static_cast<size_t>(kSyncHead + nn) * 4);
-    z.tsqr = a256(static_cast<size_t>(kTsqrMaxLeaves * kNB * kNB + kTsqrComb * kNB + kNB * kNB) * 8);
+    z.tsqr = a256(static_cast<size_t>(2 * kTsqrSet) * 8);
     z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + z.sync + z.tsqr + 256;
     return z;
 }
@@ -934,23 +942,48 @@ void factor(const Batch& B, hipStream_t s) {
     const bool tsqr = B.kmax <= kTsqrMaxK;
     const int lr = B.kmax <= 8 * 256 ? 256 : 512;
     const bool comb256 = ceil_div(B.kmax, lr) <= 8;
+    const bool lookahead = tsqr && P > 1;
+    const SideStream& side = lookahead ? side_stream() : SideStream{};
     for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
         if (tsqr) {
+            // Look-ahead (batches of several problems, where the updates are
+            // large enough to repay the cross-stream waits): after panel p
+            // is factored, only the next panel's columns get its update on
+            // `s`; the rest of its trailing update runs on the side stream
+            // while panel p+1 is factored.  Panel p waits for the side update
+            // of p-2 (its own columns) and the next-panel update for the side
+            // update of p-1.
+            const int p = static_cast<int>(c0 / kNB), par = p & 1;
             const unsigned nleaves = static_cast<unsigned>(ceil_div(B.kmax - c0, lr));
-            if (lr == 256) qr_tsqr_leaf_kernel<256><<<dim3(nleaves, 1, P), 256, 0, s>>>(B.dev, c0);
-            else qr_tsqr_leaf_kernel<512><<<dim3(nleaves, 1, P), 512, 0, s>>>(B.dev, c0);
-            if (comb256) qr_tsqr_combine_kernel<256><<<dim3(1, 1, P), 256, 0, s>>>(B.dev, c0, lr);
-            else qr_tsqr_combine_kernel<kTsqrComb><<<dim3(1, 1, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
+            if (lookahead && p >= 2) TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[par], 0));
+            if (lr == 256) qr_tsqr_leaf_kernel<256><<<dim3(nleaves, 1, P), 256, 0, s>>>(B.dev, c0, par);
+            else qr_tsqr_leaf_kernel<512><<<dim3(nleaves, 1, P), 512, 0, s>>>(B.dev, c0, par);
+            if (comb256) qr_tsqr_combine_kernel<256><<<dim3(1, 1, P), 256, 0, s>>>(B.dev, c0, lr, par);
+            else qr_tsqr_combine_kernel<kTsqrComb><<<dim3(1, 1, P), kTsqrComb, 0, s>>>(B.dev, c0, lr, par);
             TOMOS_LAUNCH_CHECK("qr_tsqr panel");
             if (c0 + kNB >= B.nmax) break;
-            const unsigned ctiles = static_cast<unsigned>(ceil_div(B.nmax - c0 - kNB, kCT));
-            if (lr == 256)
-                qr_tsqr_update_kernel<256, false><<<dim3(nleaves, ctiles, P), 256, 0, s>>>(B.dev, c0, lr);
-            else
-                qr_tsqr_update_kernel<512, false><<<dim3(nleaves, ctiles, P), 512, 0, s>>>(B.dev, c0, lr);
-            if (comb256) qr_tsqr_update_kernel<256, true><<<dim3(1, ctiles, P), 256, 0, s>>>(B.dev, c0, lr);
-            else qr_tsqr_update_kernel<kTsqrComb, true><<<dim3(1, ctiles, P), kTsqrComb, 0, s>>>(B.dev, c0, lr);
-            TOMOS_LAUNCH_CHECK("qr_tsqr update");
+            const int ctiles = static_cast<int>(ceil_div(B.nmax - c0 - kNB, kCT));
+            const int near = std::min(ctiles, kNB / kCT);  // the next panel's columns
+            auto update = [&](hipStream_t st, int t0, int count) {
+                const dim3 gl(nleaves, static_cast<unsigned>(count), P), gc(1, static_cast<unsigned>(count), P);
+                if (lr == 256) qr_tsqr_update_kernel<256, false><<<gl, 256, 0, st>>>(B.dev, c0, lr, par, t0);
+                else qr_tsqr_update_kernel<512, false><<<gl, 512, 0, st>>>(B.dev, c0, lr, par, t0);
+                if (comb256) qr_tsqr_update_kernel<256, true><<<gc, 256, 0, st>>>(B.dev, c0, lr, par, t0);
+                else qr_tsqr_update_kernel<kTsqrComb, true><<<gc, kTsqrComb, 0, st>>>(B.dev, c0, lr, par, t0);
+                TOMOS_LAUNCH_CHECK("qr_tsqr update");
+            };
+            if (!lookahead) {
+                update(s, 0, ctiles);
+                continue;
+            }
+            if (ctiles > near) {
+                TOMOS_HIP_CHECK(hipEventRecord(side.fork, s));
+                TOMOS_HIP_CHECK(hipStreamWaitEvent(side.stream, side.fork, 0));
+                update(side.stream, near, ctiles - near);
+                TOMOS_HIP_CHECK(hipEventRecord(side.join[par], side.stream));
+            }
+            if (p >= 1) TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[par ^ 1], 0));
+            update(s, 0, near);
             continue;
         }
         qr_panel_kernel<<<dim3(kNB, 1, P), kPT, 0, s>>>(B.dev, c0, c0 / kNB, B.status);
@@ -967,6 +1000,10 @@ void factor(const Batch& B, hipStream_t s) {
         qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
                            kT, 0, s>>>(B.dev, c0);
         TOMOS_LAUNCH_CHECK("qr trailing update");
+    }
+    if (lookahead) {  // every side-stream update is done before R is read
+        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[0], 0));
+        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[1], 0));
     }
     extract_r_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax * B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
     TOMOS_LAUNCH_CHECK("extract_r_kernel");

@@ -19,6 +19,8 @@ struct DeviceCache {
     int cu = 0;
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    SideStream side;
+    bool side_ready = false;
 };
 std::vector<DeviceCache>& caches() {
     static std::vector<DeviceCache> c;
@@ -45,6 +47,19 @@ int cu_count() {
         if (c.cu <= 0) c.cu = 256;
     }
     return c.cu;
+}
+
+const SideStream& side_stream() {
+    DeviceCache& c = cache_for_current();
+    if (!c.side_ready) {
+        // non-blocking: no implicit ordering with the legacy default stream;
+        // default priority
+        TOMOS_HIP_CHECK(hipStreamCreateWithFlags(&c.side.stream, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&c.side.fork, &c.side.join[0], &c.side.join[1]})
+            TOMOS_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        c.side_ready = true;
+    }
+    return c.side;
 }
 
 void* fallback_scratch(size_t bytes) {
