@@ -235,6 +235,19 @@ int tomos_fused_config_clear(void);
 /* Installed shape for (a_type, n): all three outputs 0 when none is set. */
 int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_per_cu, int* threads);
 
+/* Kernel timer (no reference counterpart: the reference times whole solves
+ * with std::chrono, lsqr_precision_runner.cu:653-659).  Between begin and
+ * end, every launch of the single-pass kernel (tomos_matvec_pair, the fused
+ * LSQR iteration, the adjoint's fused products) made by the calling host
+ * thread is bracketed by HIP events on its stream, up to `capacity`
+ * launches (1..65536; later launches are not timed).  end waits for the
+ * recorded launches, writes their device durations (ms, launch order) to
+ * durations_ms[0 .. min(count, capacity)), the count to *launches, and
+ * frees the events.  A launch enqueued after the solve's stop flag was set
+ * exits at once and shows as a duration of a few microseconds. */
+int tomos_kernel_timer_begin(int capacity);
+int tomos_kernel_timer_end(float* durations_ms, int capacity, int* launches);
+
 /* ---- CountSketch ------------------------------------------------------ */
 /* bucket[i] = splitmix64(i ^ 0x12345678abcdef00) mod s,
  * sign[i]   = bit0(splitmix64(i ^ 0xf00dd00d42babe55)) ? +1 : -1,

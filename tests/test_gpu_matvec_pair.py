@@ -130,3 +130,28 @@ def test_load_policy_does_not_change_results(capi, a_type):
         C.call("tomos_stream_loads_set", 0)
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     assert C.lib().tomos_stream_loads_set(3) == 1
+
+
+def test_kernel_timer_counts_single_pass_launches(capi):
+    """tomos_kernel_timer_begin/_end bracket each single-pass kernel launch
+    with HIP events: the count equals the launches made (capped at the
+    capacity), durations are positive, and the timer leaves the
+    products unchanged."""
+    m, n = 20000, 300
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    p = rng.standard_normal(n)
+    u0 = rng.standard_normal(m).astype(np.float32)
+    u_ref, t_ref, _ = run_pair(C.F32, A, m, p, u0, 0.5)
+    C.call("tomos_kernel_timer_begin", 3)
+    outs = [run_pair(C.F32, A, m, p, u0, 0.5) for _ in range(5)]
+    cnt = ctypes.c_int()
+    dur = np.zeros(8, np.float32)
+    C.call("tomos_kernel_timer_end", dur.ctypes.data, 8, ctypes.byref(cnt))
+    assert cnt.value == 3
+    assert np.all(dur[:3] > 0.0) and np.all(dur[3:] == 0.0)
+    for u, t, _ in outs:
+        assert np.array_equal(u, u_ref) and np.array_equal(t, t_ref)
+    # no timer running: end is an error, begin rejects a bad capacity
+    assert C.lib().tomos_kernel_timer_end(dur.ctypes.data, 8, ctypes.byref(cnt)) != 0
+    assert C.lib().tomos_kernel_timer_begin(0) != 0

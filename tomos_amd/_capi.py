@@ -112,6 +112,8 @@ _SIGS = {
     "tomos_sketch_qr_precond_batched_f64": (cint, [cint, vp, vp, sz, vp]),
     "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
+    "tomos_kernel_timer_begin": (cint, [cint]),
+    "tomos_kernel_timer_end": (cint, [vp, cint, vp]),
     "tomos_gemm_split": (cint, [cint, i64, i64, i64, dbl, vp, vp, i64, vp, vp, i64, dbl, vp, i64, vp,
                                 sz, vp]),
     "tomos_gaussian_fill": (cint, [cint, i64, i64, vp, i64, ctypes.c_uint64, vp]),

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "codec.hpp"
 #include "common.hpp"
@@ -769,6 +770,23 @@ const void* fused_kernel_for(int a_type, int g, int nt, int rounds, bool ntl) {
     return fused_kernel_ptr<FMT_F32, false>(g, nt, rounds);
 }
 
+// Kernel timer: HIP events around every single-pass kernel launch made by
+// this host thread between tomos_kernel_timer_begin and _end (a fixed pool
+// of event pairs, created at begin; launches beyond it are not timed).
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // 2 per launch
+    size_t used = 0;             // pairs recorded
+};
+thread_local KernelTimer g_timer;
+
+void timer_destroy() {
+    for (hipEvent_t e : g_timer.ev) (void)hipEventDestroy(e);
+    g_timer.ev.clear();
+    g_timer.used = 0;
+    g_timer.on = false;
+}
+
 }  // namespace
 
 FusedShape lsqr_fused_default_shape(int64_t n) {
@@ -893,8 +911,11 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
     int64_t tpb = plan.tiles_per_block;
     void* args[] = {&m, &n, const_cast<void**>(&a), &lda, const_cast<void**>(&p), &p64, &u, &u64,
                     const_cast<double**>(&u_coef), &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};
+    const bool timed = g_timer.on && 2 * (g_timer.used + 1) <= g_timer.ev.size();
+    if (timed) TOMOS_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.used], s));
     TOMOS_HIP_CHECK(hipLaunchKernel(k, dim3(static_cast<unsigned>(plan.nblocks)), dim3(plan.threads), args, 0, s));
     TOMOS_LAUNCH_CHECK("lsqr_fused_kernel");
+    if (timed) TOMOS_HIP_CHECK(hipEventRecord(g_timer.ev[2 * g_timer.used++ + 1], s));
     fused_tsum_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, kThreads)), static_cast<unsigned>(plan.splits)),
                         kThreads, 0, s>>>(n, plan.nblocks, plan.splits, tpart, tsum, skip);
     TOMOS_LAUNCH_CHECK("fused_tsum_kernel");
@@ -1046,6 +1067,32 @@ int tomos_fused_config_get(int a_type, int64_t n, int* row_groups, int* blocks_p
         require(row_groups && blocks_per_cu && threads, "tomos_fused_config_get: null pointer");
         if (!lsqr_fused_config_get(a_type, n, row_groups, blocks_per_cu, threads))
             *row_groups = *blocks_per_cu = *threads = 0;
+    });
+}
+
+int tomos_kernel_timer_begin(int capacity) {
+    return guarded([&] {
+        require(capacity >= 1 && capacity <= 65536, "tomos_kernel_timer_begin: capacity out of range");
+        timer_destroy();
+        g_timer.ev.resize(2 * static_cast<size_t>(capacity), nullptr);
+        for (hipEvent_t& e : g_timer.ev) TOMOS_HIP_CHECK(hipEventCreate(&e));
+        g_timer.on = true;
+    });
+}
+
+int tomos_kernel_timer_end(float* durations_ms, int capacity, int* launches) {
+    return guarded([&] {
+        require(launches && (durations_ms || capacity == 0), "tomos_kernel_timer_end: null pointer");
+        require(g_timer.on, "tomos_kernel_timer_end: no timer running on this thread");
+        const size_t k = g_timer.used;
+        for (size_t i = 0; i < k; ++i) {
+            TOMOS_HIP_CHECK(hipEventSynchronize(g_timer.ev[2 * i + 1]));
+            float ms = 0.0f;
+            TOMOS_HIP_CHECK(hipEventElapsedTime(&ms, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]));
+            if (static_cast<int64_t>(i) < capacity) durations_ms[i] = ms;
+        }
+        *launches = static_cast<int>(k);
+        timer_destroy();
     });
 }
 
