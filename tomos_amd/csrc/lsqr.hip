@@ -432,6 +432,7 @@ struct Solver {
     CsrMat csr_t, csr_ref;
     bool use_fused = false;
     bool use_fused_restart = false;  // initialize() in one pass (exact residual + A_hi^T u)
+    bool x_zero = false;             // x is still the zero vector of run()'s start
     FusedPlan rplan;
     void* scratch() const { return sc + L.scratch; }
 
@@ -495,8 +496,8 @@ struct Solver {
         if (use_fused_restart) {
             float* tpart = reinterpret_cast<float*>(sc + L.tpart);
             double* tsum = reinterpret_cast<double*>(sc + L.tsum);
-            lsqr_restart_launch(o.a_type, m, n, a, wr, lda, static_cast<const double*>(x), U(), v64, red(), tpart,
-                                tsum, rplan, &S->done, s);
+            lsqr_restart_launch(o.a_type, m, n, a, wr, lda, static_cast<const double*>(x), x_zero, U(), v64,
+                                red(), tpart, tsum, rplan, &S->done, s);
             k_beta_finish<<<1, kT, 0, s>>>(S, red(), rplan.nblocks);
             k_temp_finish<<<grid_for(n), kT, 0, s>>>(S, n, rplan.splits, tsum, Tm(), v64);
             tail(true, true);
@@ -568,6 +569,7 @@ struct Solver {
         LsqrState* S = state();
         k_state_init<<<1, 64, 0, s>>>(S);
         k_zero<<<static_cast<unsigned>(ceil_div(std::max<int64_t>(n, 1), kT)), kT, 0, s>>>(x, r64, n);
+        x_zero = true;
         norm(b, r64, m, &S->rhsnorm, false);
         TOMOS_LAUNCH_CHECK("lsqr setup");
         const int every = std::max(1, o.check_every);
@@ -575,6 +577,7 @@ struct Solver {
         for (int r = 0; r < o.restarts && !stop; ++r) {
             k_restart_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres);
             initialize();
+            x_zero = false;
             TOMOS_LAUNCH_CHECK("lsqr initialize");
             if (done_flag()) break;
             for (int it = 0; it < o.max_iters; ++it) {

@@ -552,11 +552,13 @@ __device__ __forceinline__ void decode_exact_vec(const uint4& h, const uint4& l,
 // (u holds b on entry; x in FP64), |u_raw|^2 per block, and the block
 // partials of A_hi^T u_raw from the hi words of the same tile (the
 // per-iteration tier).  Tile: 8 row groups x 512 threads x 16 column rounds
-// (n <= 1024), both planes in registers, loaded directly.
+// (n <= 1024), both planes in registers, loaded directly.  XZERO (x == 0,
+// the first restart of a solve): A x is the zero vector for finite A, so the
+// lo plane is not read and u_raw = u.
 constexpr int kRsG = 8, kRsNT = 512, kRsR = 16;
 constexpr int64_t kRestartMaxN = static_cast<int64_t>(kRsNT / kRsG) * kRsR;
 
-template <int FMT, bool NTL>
+template <int FMT, bool NTL, bool XZERO = false>
 __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
     const double* __restrict__ x, void* __restrict__ u, bool u64, int64_t tiles_per_block,
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
             const int64_t j = static_cast<int64_t>(r) * CS + c;
             const bool ok = in && j < n;
             th[r] = ok ? load_a16<NTL>(abase + (my_row + j * lda) * ESZ) : make_uint4(0u, 0u, 0u, 0u);
-            if constexpr (FMT != FMT_F32)
+            if constexpr (FMT != FMT_F32 && !XZERO)
                 tl[r] = ok ? load_a16<NTL>(wr + my_row + j * lda) : make_uint4(0u, 0u, 0u, 0u);
             else
                 tl[r] = th[r];
@@ -603,13 +605,15 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
         double acc[RPL];
 #pragma unroll
         for (int k = 0; k < RPL; ++k) acc[k] = 0.0;
+        if constexpr (!XZERO) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float av[RPL];
-            decode_exact_vec<FMT>(th[r], tl[r], av);
-            const double xr = xs[r * CS + c];
+            for (int r = 0; r < R; ++r) {
+                float av[RPL];
+                decode_exact_vec<FMT>(th[r], tl[r], av);
+                const double xr = xs[r * CS + c];
 #pragma unroll
-            for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xr, acc[k]);
+                for (int k = 0; k < RPL; ++k) acc[k] = fma(static_cast<double>(av[k]), xr, acc[k]);
+            }
         }
         LaneTR<RPL, 32, G, double>::run(acc, lane);
         {
@@ -933,10 +937,15 @@ bool lsqr_restart_supported(int a_type, int tier, int64_t m, int64_t n, int64_t 
 }
 
 namespace {
-const void* restart_kernel_for(int a_type, bool ntl) {
-    if (a_type == TOMOS_FP32_BF16)
+const void* restart_kernel_for(int a_type, bool ntl, bool xzero = false) {
+    if (a_type == TOMOS_FP32_BF16) {
+        if (xzero)
+            return ntl ? reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, true, true>)
+                       : reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, false, true>);
         return ntl ? reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, true>)
                    : reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_PBF, false>);
+    }
+    // F32: the one plane is needed for A^T u anyway
     return ntl ? reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_F32, true>)
                : reinterpret_cast<const void*>(lsqr_restart_kernel<FMT_F32, false>);
 }
@@ -965,10 +974,10 @@ FusedPlan lsqr_restart_plan(int a_type, int64_t m, int64_t n) {
 }
 
 void lsqr_restart_launch(int a_type, int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
-                         const double* x, void* u, bool u64, double* sumsq, float* tpart, double* tsum,
-                         const FusedPlan& plan, const int* skip, hipStream_t s) {
+                         const double* x, bool xzero, void* u, bool u64, double* sumsq, float* tpart,
+                         double* tsum, const FusedPlan& plan, const int* skip, hipStream_t s) {
     require(n <= kRestartMaxN && plan.nblocks <= lsqr_fused_max_blocks(m), "lsqr_restart: plan does not fit");
-    const void* k = restart_kernel_for(a_type, plan.stream);
+    const void* k = restart_kernel_for(a_type, plan.stream, xzero);
     int64_t tpb = plan.tiles_per_block;
     void* args[] = {&m, &n, const_cast<void**>(&a), const_cast<uint16_t**>(&wr), &lda, const_cast<double**>(&x),
                     &u, &u64, &tpb, &sumsq, &tpart, const_cast<int**>(&skip)};

@@ -55,14 +55,16 @@ void lsqr_fused_launch(int a_type, int64_t m, int64_t n, const void* a, int64_t 
 // ---- restart in one pass ------------------------------------------------
 // u_raw = u - A x at the exact tier (u holds b on entry, x FP64, FP64 row
 // sums), sumsq[b] per block and tsum = level-1 sums of A_hi^T u_raw, from
-// one read of both planes.  Supported for F32 and FP32_BF16 at the EXACT
-// tier, n <= 1024, aligned as the single-pass kernel.
+// one read of both planes.  xzero: x is known to be zero (the first restart
+// of a solve), so u_raw = u and only the hi plane is read.  Supported for
+// F32 and FP32_BF16 at the EXACT tier, n <= 1024, aligned as the
+// single-pass kernel.
 bool lsqr_restart_supported(int a_type, int tier, int64_t m, int64_t n, int64_t lda, const void* a,
                             const uint16_t* wr);
 FusedPlan lsqr_restart_plan(int a_type, int64_t m, int64_t n);
 void lsqr_restart_launch(int a_type, int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
-                         const double* x, void* u, bool u64, double* sumsq, float* tpart, double* tsum,
-                         const FusedPlan& plan, const int* skip, hipStream_t s);
+                         const double* x, bool xzero, void* u, bool u64, double* sumsq, float* tpart,
+                         double* tsum, const FusedPlan& plan, const int* skip, hipStream_t s);
 
 // ---- many problems in one launch ----------------------------------------
 // Device table entry of one problem: A (hi plane / F32), p (FP64, n),
