@@ -62,6 +62,8 @@ def parse():
                    help="entries per row of the CSR LSQR leg (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
                    help="batched differentiable least squares: number of uneven problems (0 = skip)")
+    p.add_argument("--adjoint-workers", type=int, default=8,
+                   help="host threads (one HIP stream each) of the batched backward")
     p.add_argument("--stream-loads", type=int, default=0, choices=(0, 1, 2),
                    help="matrix load policy: 0 auto (non-temporal above 192 MiB per pass), "
                         "1 always non-temporal, 2 always cacheable")
@@ -299,14 +301,38 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
                p["b"].ptr, p["x"].ptr, wd.ptr, gb.ptr, gA.ptr, p["m"], gscr.ptr, gsb, None)
         adj_its.append(it.value)
     C.call("tomos_device_synchronize")
+    bwd_seq_ms = (time.perf_counter() - t0) * 1e3
+    # the same backward with tomos_lsq_adjoint_batched: problems spread over
+    # host threads, one HIP stream each (every problem its own w, grad_b, grad_A)
+    P_ = len(probs)
+    wds = [DeviceBuffer(p["n"] * 8) for p in probs]
+    gbds = [DeviceBuffer(p["m"] * 8) for p in probs]
+    gads = [DeviceBuffer(p["m"] * p["n"] * 4) for p in probs]
+    ptrs = lambda bufs: (ctypes.c_void_p * P_)(*[x.ptr for x in bufs])
+    ldg = (ctypes.c_int64 * P_)(*[p["m"] for p in probs])
+    aw = args.adjoint_workers
+    absb = C.lib().tomos_lsq_adjoint_batched_scratch_size(ctypes.byref(ao), P_, lp, aw)
+    abscr = DeviceBuffer(absb)
+    aits = (ctypes.c_int * P_)()
+    bargs = (ctypes.byref(ao), P_, lp, ptrs(gs), ptrs(wds), ptrs(gbds), ptrs(gads), ldg, aw, abscr.ptr, absb,
+             aits, None)
+    C.call("tomos_lsq_adjoint_batched", *bargs)  # warm-up (thread and stream start-up)
+    C.call("tomos_device_synchronize")
+    t0 = time.perf_counter()
+    C.call("tomos_lsq_adjoint_batched", *bargs)
+    C.call("tomos_device_synchronize")
     bwd_ms = (time.perf_counter() - t0) * 1e3
+    assert list(aits) == adj_its, "batched adjoint iterations differ from the one-problem calls"
+    del wds, gbds, gads, abscr
     its_l = list(its)
     return {"problems": P, "elements": int((ms * ns).sum()), "storage": "FP32_BF16 planar",
             "precond_ms": round(pre_ms, 2), "solve_ms": round(solve_ms, 2), "forward_ms": round(pre_ms + solve_ms, 2),
             "solve_one_at_a_time_ms": round(seq_ms, 2), "batching_speedup": round(seq_ms / solve_ms, 2),
             "iterations_min": min(its_l), "iterations_max": max(its_l), "worst_relres": worst,
-            "backward_ms": round(bwd_ms, 2), "adjoint_iterations_max": max(adj_its),
-            "backward": "per problem: adjoint CG (tol 1e-6) + grad_b + grad_A (m x n FP32)"}
+            "backward_ms": round(bwd_ms, 2), "backward_one_at_a_time_ms": round(bwd_seq_ms, 2),
+            "adjoint_workers": args.adjoint_workers, "adjoint_iterations_max": max(adj_its),
+            "backward": "tomos_lsq_adjoint_batched: per problem adjoint CG (tol 1e-6) + grad_b + grad_A "
+                        "(m x n FP32), problems over %d host threads / HIP streams" % args.adjoint_workers}
 
 
 def kernels_only(args, C, DeviceBuffer, m, n, wh):

@@ -512,6 +512,26 @@ int tomos_lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, co
                    float* grad_a, int64_t ldg, void* scratch, size_t scratch_bytes,
                    tomos_stream_t stream);
 
+/* Adjoint and gradients of many problems (tomos_lsq_problem: A, wr, lda, b,
+ * x, R^-1 per problem, as for tomos_lsqr_batched_solve): for each problem p,
+ * w[p] = (A^T A)^-1 g[p] by tomos_lsq_adjoint_solve's control flow and
+ * kernels, then grad_b[p] = A w[p] and, when grad_a and grad_a[p] are
+ * non-NULL, grad_a[p] (FP32, ldg[p] >= m) as tomos_lsq_grad at
+ * opt->restart_tier.  The problems (largest first) are taken by `workers`
+ * host threads (1..16), each issuing on its own HIP stream (default
+ * priority) with its own slice of the scratch, so the small launches of
+ * different problems overlap; every result is bit-identical to the
+ * one-problem calls.  scratch (required):
+ * tomos_lsq_adjoint_batched_scratch_size() bytes.  iterations: one per
+ * problem, or NULL.  Synchronous: waits for `stream` before starting and
+ * for every worker before returning.  No reference counterpart. */
+size_t tomos_lsq_adjoint_batched_scratch_size(const tomos_lsqr_options* opt, int nprob,
+                                              const tomos_lsq_problem* probs, int workers);
+int tomos_lsq_adjoint_batched(const tomos_lsqr_options* opt, int nprob, const tomos_lsq_problem* probs,
+                              const double* const* g, double* const* w, double* const* grad_b,
+                              float* const* grad_a, const int64_t* ldg, int workers, void* scratch,
+                              size_t scratch_bytes, int* iterations, tomos_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,3 +114,52 @@ def test_torch_autograd_vs_dense_and_fd(capi):
         fd = ((lstsq(A.detach() + h * dA, b, opts) - lstsq(A.detach() - h * dA, b, opts)) @ g).item() / (2 * h)
         an = float((gA_ref * dA.cpu().numpy().astype(np.float64)).sum())
         assert abs(fd - an) < 1e-2 * abs(an) + 1e-6, (fd, an)
+
+
+@pytest.mark.parametrize("workers", [1, 3, 16])
+def test_adjoint_batched_matches_one_problem_calls(capi, workers):
+    """tomos_lsq_adjoint_batched (problems spread over host threads, one HIP
+    stream each) gives bit-identical w, grad_b and grad_A to the
+    one-problem tomos_lsq_adjoint_solve + tomos_lsq_grad calls, for uneven
+    shapes (packed FP32_BF16 with the sketch-QR preconditioner)."""
+    shapes = [(20000, 64), (5003, 37), (12000, 128), (4096, 16), (9001, 90)]
+    opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-8, 1e-8, 30, 20, 4, 1)
+    keep, probs, gs, ws, gbs, gas = [], [], [], [], [], []
+    for k, (m, n) in enumerate(shapes):
+        A, b, g = problem(m, n, 2, 100 + k)
+        lda = (m + 7) // 8 * 8
+        wh, wr = H.pack(C.PACK_FP32_BF16, A, ld=lda)
+        _, sk = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, lda, 4 * n)
+        bd, xd = H.dev(b), H.dev(np.random.default_rng(k).standard_normal(n))
+        gd, wd, gbd, gad = H.dev(g), DeviceBuffer(n * 8), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
+        keep += [wh, wr, sk, bd, xd]
+        probs.append(C.LsqProblem(m, n, lda, wh.ptr, wr.ptr, bd.ptr, xd.ptr, sk.ptr, 4 * n))
+        gs.append(gd)
+        ws.append(wd)
+        gbs.append(gbd)
+        gas.append(gad)
+    P = len(shapes)
+    lp = (C.LsqProblem * P)(*probs)
+    ptrs = lambda bufs: (ctypes.c_void_p * P)(*[x.ptr for x in bufs])
+    ldg = (ctypes.c_int64 * P)(*[m for m, _ in shapes])
+    sb = C.lib().tomos_lsq_adjoint_batched_scratch_size(ctypes.byref(opt), P, lp, workers)
+    assert sb > 0, C.lib().tomos_last_error()
+    scr = DeviceBuffer(sb)
+    its = (ctypes.c_int * P)()
+    C.call("tomos_lsq_adjoint_batched", ctypes.byref(opt), P, lp, ptrs(gs), ptrs(ws), ptrs(gbs), ptrs(gas), ldg,
+           workers, scr.ptr, sb, its, None)
+    for k, (m, n) in enumerate(shapes):
+        q = probs[k]
+        asb = C.lib().tomos_lsq_adjoint_scratch_size(ctypes.byref(opt), m, n)
+        ascr = DeviceBuffer(asb)
+        w1, it1 = DeviceBuffer(n * 8), ctypes.c_int(0)
+        C.call("tomos_lsq_adjoint_solve", ctypes.byref(opt), m, n, q.a, q.wr, q.lda, gs[k].ptr, q.rinv, q.ldr,
+               w1.ptr, ascr.ptr, asb, ctypes.byref(it1), None, 0, None, None)
+        gsb = C.lib().tomos_lsq_grad_scratch_size(C.FP32_BF16, C.TIER_EXACT, m, n)
+        gscr, gb1, ga1 = DeviceBuffer(gsb), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
+        C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, m, n, q.a, q.wr, q.lda, q.b, q.x, w1.ptr, gb1.ptr,
+               ga1.ptr, m, gscr.ptr, gsb, None)
+        assert its[k] == it1.value
+        assert np.array_equal(ws[k].to_numpy(np.float64, (n,)), w1.to_numpy(np.float64, (n,)))
+        assert np.array_equal(gbs[k].to_numpy(np.float64, (m,)), gb1.to_numpy(np.float64, (m,)))
+        assert np.array_equal(gas[k].to_numpy(np.float32, (m, n)), ga1.to_numpy(np.float32, (m, n)))

@@ -111,6 +111,9 @@ _SIGS = {
                                         sz, vp, vp, vp]),
     "tomos_sketch_qr_precond_batched_f64": (cint, [cint, vp, vp, sz, vp]),
     "tomos_lsq_grad": (cint, [cint, cint, i64, i64, vp, vp, i64, vp, vp, vp, vp, vp, i64, vp, sz, vp]),
+    "tomos_lsq_adjoint_batched_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), cint, vp, cint]),
+    "tomos_lsq_adjoint_batched": (cint, [ctypes.POINTER(LsqrOptions), cint, vp, vp, vp, vp, vp, vp, cint, vp,
+                                         sz, vp, vp]),
     "tomos_matvec_pair": (cint, [cint, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp, sz, vp]),
     "tomos_kernel_timer_begin": (cint, [cint]),
     "tomos_kernel_timer_end": (cint, [vp, cint, vp]),

@@ -27,6 +27,11 @@
 // and w, x (n each) and streams the m x n FP32 result: no copy of A is made.
 // No reference counterpart: the reference has no adjoint.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 #include "gemv.hpp"
@@ -38,6 +43,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kBlk = 1024;  // one-block n-sized kernels
+constexpr int kMaxAdjWorkers = 16;  // host threads (one HIP stream each) of the batched adjoint
 
 struct AdjState {
     double rho;       // res . z
@@ -417,6 +423,107 @@ size_t grad_scratch_bytes(int a_type, int tier, int64_t m, int64_t n) {
 }  // namespace
 }  // namespace tomos_hip
 
+namespace tomos_hip {
+namespace {
+
+void adjoint_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a, const uint16_t* wr,
+                   int64_t lda, const double* g, const void* rinv, int64_t ldr, double* w, void* scratch,
+                   size_t scratch_bytes, int* iterations, double* history, int history_cap, int* history_len,
+                   hipStream_t s) {
+    adj_validate(o, m, n);
+    require(lda >= std::max<int64_t>(m, 1), "adjoint: lda < m");
+    if (o.precond_type >= 0) {
+        require(rinv != nullptr, "adjoint: precond_type set but no R^-1 given");
+        require(ldr >= std::max<int64_t>(n, 1), "adjoint: ldr < n");
+    }
+    if (n == 0) {
+        if (iterations) *iterations = 0;
+        if (history_len) *history_len = 0;
+        return;
+    }
+    require(a && g && w, "adjoint: null A, g or w");
+    const bool packed = o.a_type == TOMOS_FP32_16 || o.a_type == TOMOS_FP32_BF16;
+    if (packed && (o.mv_tier == TOMOS_TIER_EXACT || o.restart_tier == TOMOS_TIER_EXACT))
+        require(wr != nullptr, "adjoint: EXACT tier needs the wr plane");
+    Adj J;
+    J.o = o;
+    J.m = m;
+    J.n = n;
+    J.lda = lda;
+    J.ldr = ldr;
+    J.a = a;
+    J.wr = wr;
+    J.g = g;
+    J.rinv = rinv;
+    J.w = w;
+    J.L = adj_layout(o, m, n);
+    if (!scratch) scratch = fallback_scratch(J.L.total);
+    else if (scratch_bytes < J.L.total)
+        throw Error(kErrWorkspace, "adjoint: scratch too small (need " + std::to_string(J.L.total) +
+                                       " bytes, have " + std::to_string(scratch_bytes) + ")");
+    J.sc = static_cast<char*>(scratch);
+    J.s = s;
+    J.use_fused = J.L.fused_possible && lsqr_fused_supported(o.a_type, o.mv_tier, m, n, lda, a);
+    if (J.use_fused) J.plan = lsqr_fused_plan(o.a_type, m, n);
+    J.run(iterations, history, history_cap, history_len);
+}
+
+void lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
+              const double* b, const double* x, const double* w, double* grad_b, float* grad_a, int64_t ldg,
+              void* scratch, size_t scratch_bytes, hipStream_t s) {
+    require(m >= 0 && n >= 0 && lda >= std::max<int64_t>(m, 1), "tomos_lsq_grad: bad sizes");
+    if (m == 0 || n == 0) return;
+    require(a && b && x && w && grad_b, "tomos_lsq_grad: null pointer");
+    require(!grad_a || ldg >= m, "tomos_lsq_grad: ldg < m");
+    const size_t need = grad_scratch_bytes(a_type, tier, m, n);
+    if (!scratch) scratch = fallback_scratch(need);
+    else require(scratch_bytes >= need, "tomos_lsq_grad: scratch too small");
+    char* base = static_cast<char*>(scratch);
+    double* r = reinterpret_cast<double*>(base);
+    void* gsc = base + a256(static_cast<size_t>(m) * 8);
+    const size_t gbytes = gemv_scratch_bytes(a_type, tier, TOMOS_NO_TRANSPOSE, m, n);
+    Coef c1;
+    c1.alpha = 1.0;
+    c1.beta = 0.0;
+    gemv(a_type, tier, TOMOS_NO_TRANSPOSE, c1, m, n, a, wr, lda, TOMOS_F64, w, TOMOS_F64, grad_b, gsc, gbytes, s);
+    if (!grad_a) return;
+    TOMOS_HIP_CHECK(hipMemcpyAsync(r, b, static_cast<size_t>(m) * 8, hipMemcpyDeviceToDevice, s));
+    Coef c2;
+    c2.alpha = -1.0;
+    c2.beta = 1.0;
+    gemv(a_type, tier, TOMOS_NO_TRANSPOSE, c2, m, n, a, wr, lda, TOMOS_F64, x, TOMOS_F64, r, gsc, gbytes, s);
+    const dim3 grid_cols(1, static_cast<unsigned>(ceil_div(n, kCols)));
+    require(ceil_div(n, kCols) < 65536, "tomos_lsq_grad: n too large");
+    if (reinterpret_cast<uintptr_t>(grad_a) % 16 == 0 && ldg % 4 == 0) {
+        const dim3 grid(static_cast<unsigned>(ceil_div(ceil_div(m, 4), kT)), grid_cols.y);
+        k_rank2_vec<<<grid, kT, 0, s>>>(m, n, r, grad_b, w, x, grad_a, ldg);
+    } else {
+        const dim3 grid(static_cast<unsigned>(ceil_div(m, kT)), grid_cols.y);
+        k_rank2_scalar<<<grid, kT, 0, s>>>(m, n, r, grad_b, w, x, grad_a, ldg);
+    }
+    TOMOS_LAUNCH_CHECK("k_rank2");
+}
+
+// One worker's scratch: the adjoint's and the gradient's (exact tier) for
+// the largest problem of the batch, one after the other.
+struct BatchedAdjLayout {
+    size_t adj = 0, grad = 0, per_worker = 0;
+};
+BatchedAdjLayout batched_adj_layout(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* probs) {
+    BatchedAdjLayout L;
+    for (int p = 0; p < nprob; ++p) {
+        L.adj = std::max(L.adj, adj_layout(o, probs[p].m, probs[p].n).total);
+        L.grad = std::max(L.grad, grad_scratch_bytes(o.a_type, o.restart_tier, probs[p].m, probs[p].n));
+    }
+    L.adj = a256(L.adj);
+    L.grad = a256(L.grad);
+    L.per_worker = L.adj + L.grad;
+    return L;
+}
+
+}  // namespace
+}  // namespace tomos_hip
+
 using namespace tomos_hip;
 
 extern "C" {
@@ -438,42 +545,8 @@ int tomos_lsq_adjoint_solve(const tomos_lsqr_options* o, int64_t m, int64_t n, c
                             tomos_stream_t s) {
     return guarded([&] {
         require(o != nullptr, "tomos_lsq_adjoint_solve: null options");
-        adj_validate(*o, m, n);
-        require(lda >= std::max<int64_t>(m, 1), "adjoint: lda < m");
-        if (o->precond_type >= 0) {
-            require(rinv != nullptr, "adjoint: precond_type set but no R^-1 given");
-            require(ldr >= std::max<int64_t>(n, 1), "adjoint: ldr < n");
-        }
-        if (n == 0) {
-            if (iterations) *iterations = 0;
-            if (history_len) *history_len = 0;
-            return;
-        }
-        require(a && g && w, "adjoint: null A, g or w");
-        const bool packed = o->a_type == TOMOS_FP32_16 || o->a_type == TOMOS_FP32_BF16;
-        if (packed && (o->mv_tier == TOMOS_TIER_EXACT || o->restart_tier == TOMOS_TIER_EXACT))
-            require(wr != nullptr, "adjoint: EXACT tier needs the wr plane");
-        Adj J;
-        J.o = *o;
-        J.m = m;
-        J.n = n;
-        J.lda = lda;
-        J.ldr = ldr;
-        J.a = a;
-        J.wr = wr;
-        J.g = g;
-        J.rinv = rinv;
-        J.w = w;
-        J.L = adj_layout(*o, m, n);
-        if (!scratch) scratch = fallback_scratch(J.L.total);
-        else if (scratch_bytes < J.L.total)
-            throw Error(kErrWorkspace, "adjoint: scratch too small (need " + std::to_string(J.L.total) +
-                                           " bytes, have " + std::to_string(scratch_bytes) + ")");
-        J.sc = static_cast<char*>(scratch);
-        J.s = s;
-        J.use_fused = J.L.fused_possible && lsqr_fused_supported(o->a_type, o->mv_tier, m, n, lda, a);
-        if (J.use_fused) J.plan = lsqr_fused_plan(o->a_type, m, n);
-        J.run(iterations, history, history_cap, history_len);
+        adjoint_solve(*o, m, n, a, wr, lda, g, rinv, ldr, w, scratch, scratch_bytes, iterations, history,
+                      history_cap, history_len, s);
     });
 }
 
@@ -487,39 +560,89 @@ int tomos_lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, co
                    float* grad_a, int64_t ldg, void* scratch, size_t scratch_bytes,
                    tomos_stream_t s) {
     return guarded([&] {
-        require(m >= 0 && n >= 0 && lda >= std::max<int64_t>(m, 1), "tomos_lsq_grad: bad sizes");
-        if (m == 0 || n == 0) return;
-        require(a && b && x && w && grad_b, "tomos_lsq_grad: null pointer");
-        require(!grad_a || ldg >= m, "tomos_lsq_grad: ldg < m");
-        const size_t need = grad_scratch_bytes(a_type, tier, m, n);
-        if (!scratch) scratch = fallback_scratch(need);
-        else require(scratch_bytes >= need, "tomos_lsq_grad: scratch too small");
-        char* base = static_cast<char*>(scratch);
-        double* r = reinterpret_cast<double*>(base);
-        void* gsc = base + a256(static_cast<size_t>(m) * 8);
-        const size_t gbytes = gemv_scratch_bytes(a_type, tier, TOMOS_NO_TRANSPOSE, m, n);
-        Coef c1;
-        c1.alpha = 1.0;
-        c1.beta = 0.0;
-        gemv(a_type, tier, TOMOS_NO_TRANSPOSE, c1, m, n, a, wr, lda, TOMOS_F64, w, TOMOS_F64, grad_b,
-             gsc, gbytes, s);
-        if (!grad_a) return;
-        TOMOS_HIP_CHECK(hipMemcpyAsync(r, b, static_cast<size_t>(m) * 8, hipMemcpyDeviceToDevice, s));
-        Coef c2;
-        c2.alpha = -1.0;
-        c2.beta = 1.0;
-        gemv(a_type, tier, TOMOS_NO_TRANSPOSE, c2, m, n, a, wr, lda, TOMOS_F64, x, TOMOS_F64, r, gsc,
-             gbytes, s);
-        const dim3 grid_cols(1, static_cast<unsigned>(ceil_div(n, kCols)));
-        require(ceil_div(n, kCols) < 65536, "tomos_lsq_grad: n too large");
-        if (reinterpret_cast<uintptr_t>(grad_a) % 16 == 0 && ldg % 4 == 0) {
-            const dim3 grid(static_cast<unsigned>(ceil_div(ceil_div(m, 4), kT)), grid_cols.y);
-            k_rank2_vec<<<grid, kT, 0, s>>>(m, n, r, grad_b, w, x, grad_a, ldg);
-        } else {
-            const dim3 grid(static_cast<unsigned>(ceil_div(m, kT)), grid_cols.y);
-            k_rank2_scalar<<<grid, kT, 0, s>>>(m, n, r, grad_b, w, x, grad_a, ldg);
+        lsq_grad(a_type, tier, m, n, a, wr, lda, b, x, w, grad_b, grad_a, ldg, scratch, scratch_bytes, s);
+    });
+}
+
+size_t tomos_lsq_adjoint_batched_scratch_size(const tomos_lsqr_options* o, int nprob,
+                                              const tomos_lsq_problem* probs, int workers) {
+    size_t out = 0;
+    const int rc = guarded([&] {
+        require(o != nullptr && (nprob == 0 || probs != nullptr), "tomos_lsq_adjoint_batched_scratch_size: null");
+        require(nprob >= 0 && workers >= 1 && workers <= kMaxAdjWorkers,
+                "tomos_lsq_adjoint_batched_scratch_size: bad nprob or workers");
+        for (int p = 0; p < nprob; ++p) adj_validate(*o, probs[p].m, probs[p].n);
+        out = batched_adj_layout(*o, nprob, probs).per_worker * static_cast<size_t>(std::min(workers, std::max(nprob, 1)));
+    });
+    return rc == 0 ? out : 0;
+}
+
+int tomos_lsq_adjoint_batched(const tomos_lsqr_options* o, int nprob, const tomos_lsq_problem* probs,
+                              const double* const* g, double* const* w, double* const* grad_b,
+                              float* const* grad_a, const int64_t* ldg, int workers, void* scratch,
+                              size_t scratch_bytes, int* iterations, tomos_stream_t stream) {
+    return guarded([&] {
+        require(o != nullptr, "tomos_lsq_adjoint_batched: null options");
+        require(nprob >= 0 && workers >= 1 && workers <= kMaxAdjWorkers,
+                "tomos_lsq_adjoint_batched: bad nprob or workers (1..16)");
+        if (nprob == 0) return;
+        require(probs && g && w && grad_b, "tomos_lsq_adjoint_batched: null array");
+        for (int p = 0; p < nprob; ++p) {
+            const tomos_lsq_problem& q = probs[p];
+            adj_validate(*o, q.m, q.n);
+            require(q.a && q.b && q.x && g[p] && w[p] && grad_b[p], "tomos_lsq_adjoint_batched: null pointer");
+            require(q.lda >= std::max<int64_t>(q.m, 1), "tomos_lsq_adjoint_batched: lda < m");
+            require(!grad_a || !grad_a[p] || (ldg && ldg[p] >= q.m), "tomos_lsq_adjoint_batched: ldg < m");
         }
-        TOMOS_LAUNCH_CHECK("k_rank2");
+        const int nw = std::min(workers, nprob);
+        const BatchedAdjLayout L = batched_adj_layout(*o, nprob, probs);
+        require(scratch != nullptr && scratch_bytes >= L.per_worker * static_cast<size_t>(nw),
+                "tomos_lsq_adjoint_batched: scratch missing or too small");
+        // largest problems first, taken dynamically by the workers
+        std::vector<int> order(static_cast<size_t>(nprob));
+        for (int p = 0; p < nprob; ++p) order[static_cast<size_t>(p)] = p;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return probs[x].m * probs[x].n > probs[y].m * probs[y].n;
+        });
+        TOMOS_HIP_CHECK(hipStreamSynchronize(stream));  // inputs written on the caller's stream
+        std::vector<hipStream_t> st(static_cast<size_t>(nw), nullptr);
+        for (auto& x : st) TOMOS_HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        std::atomic<int> next{0};
+        std::mutex err_mu;
+        std::string err;
+        int err_code = 0;
+        auto work = [&](int wi) {
+            char* base = static_cast<char*>(scratch) + L.per_worker * static_cast<size_t>(wi);
+            hipStream_t s = st[static_cast<size_t>(wi)];
+            try {
+                for (int k = next++; k < nprob; k = next++) {
+                    const int p = order[static_cast<size_t>(k)];
+                    const tomos_lsq_problem& q = probs[p];
+                    int its = 0;
+                    adjoint_solve(*o, q.m, q.n, q.a, q.wr, q.lda, g[p], q.rinv, q.ldr, w[p], base, L.adj, &its,
+                                  nullptr, 0, nullptr, s);
+                    lsq_grad(o->a_type, o->restart_tier, q.m, q.n, q.a, q.wr, q.lda, q.b, q.x, w[p], grad_b[p],
+                             grad_a ? grad_a[p] : nullptr, ldg ? ldg[p] : q.m, base + L.adj, L.grad, s);
+                    if (iterations) iterations[p] = its;
+                }
+                TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+            } catch (const Error& e) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (err.empty()) { err = e.what(); err_code = e.code; }
+                next = nprob;
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (err.empty()) { err = e.what(); err_code = kErrInternal; }
+                next = nprob;
+            }
+        };
+        std::vector<std::thread> th;
+        for (int wi = 1; wi < nw; ++wi) th.emplace_back(work, wi);
+        work(0);
+        for (auto& t : th) t.join();
+        for (auto& x : st) (void)hipStreamSynchronize(x);
+        for (auto& x : st) (void)hipStreamDestroy(x);
+        if (!err.empty()) throw Error(err_code, err);
     });
 }
 
