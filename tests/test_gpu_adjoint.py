@@ -163,3 +163,30 @@ def test_adjoint_batched_matches_one_problem_calls(capi, workers):
         assert np.array_equal(ws[k].to_numpy(np.float64, (n,)), w1.to_numpy(np.float64, (n,)))
         assert np.array_equal(gbs[k].to_numpy(np.float64, (m,)), gb1.to_numpy(np.float64, (m,)))
         assert np.array_equal(gas[k].to_numpy(np.float32, (m, n)), ga1.to_numpy(np.float32, (m, n)))
+
+
+def test_torch_lstsq_batched_vs_dense(capi):
+    """lstsq_batched (batched LSQR forward, batched adjoint backward) on
+    uneven problems: x, grad_b and grad_A of every problem vs the dense FP64
+    QR solution and gradients, for the loss sum_i x_i . g_i."""
+    torch = pytest.importorskip("torch")
+    from tomos_amd.autograd import lstsq_batched, SolveOptions
+    dev = torch.device("cuda:0")
+    shapes = [(20000, 64, 3), (7001, 40, 2), (12000, 100, 3), (4096, 16, 1)]
+    As, bs, gs, refs = [], [], [], []
+    for k, (m, n, kexp) in enumerate(shapes):
+        A_np, b_np, g_np = problem(m, n, kexp, 300 + k)
+        As.append(torch.tensor(A_np, device=dev, requires_grad=True))
+        bs.append(torch.tensor(b_np, device=dev, requires_grad=True))
+        gs.append(torch.tensor(g_np, device=dev))
+        refs.append(dense_grads(A_np, b_np, g_np))
+    opts = SolveOptions(tol=1e-10, adjoint_tol=1e-10, adjoint_workers=3)
+    xs = lstsq_batched(As, bs, opts)
+    sum((x * g).sum() for x, g in zip(xs, gs)).backward()
+    for k, (x, A, b) in enumerate(zip(xs, As, bs)):
+        w_ref, gb_ref, gA_ref, x_ref, _ = refs[k]
+        xe = np.linalg.norm(x.detach().cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+        assert xe < 1e-6, (k, xe)
+        gb, gA = b.grad.cpu().numpy(), A.grad.cpu().numpy()
+        assert np.linalg.norm(gb - gb_ref) / np.linalg.norm(gb_ref) < 1e-5, k
+        assert np.linalg.norm(gA - gA_ref) / np.linalg.norm(gA_ref) < 1e-5, k

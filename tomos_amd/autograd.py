@@ -19,9 +19,16 @@ backward
     (``tomos_lsq_adjoint_solve``), then ``tomos_lsq_grad``: grad_b = A w and
     grad_A = r w^T - (A w) x^T (r = b - A x) from one fused rank-2 kernel.
 
+``lstsq_batched(As, bs)`` does the same for many uneven problems at once:
+one ``tomos_lsqr_batched_solve`` (grouped kernels, a hipGraph per restart)
+for the forward, and ``tomos_lsq_adjoint_batched`` (problems over worker
+threads with one HIP stream each) for the normal-equation correction and
+the backward.
+
 The truncating FP32_BF16 split is lossless (hi and lo halves of the FP32
 bits), so the solve and the gradients refer to A itself.  Everything runs
-on torch's current stream; there is no CPU path.
+on torch's current stream (the batched adjoint waits for it and returns
+when its own streams are done); there is no CPU path.
 """
 import ctypes
 from dataclasses import dataclass
@@ -52,6 +59,7 @@ class SolveOptions:
     adjoint_tol: float = 1e-10   # |g - A^T A w| / |g| for the adjoint solve
     check_every: int = 8
     normal_correction: bool = True  # forward: x += (A^T A)^-1 A^T (b - A x), exact products
+    adjoint_workers: int = 8     # lstsq_batched: host threads (HIP streams) of the batched adjoint
 
 
 class PackedProblem:
@@ -159,3 +167,86 @@ class _Lstsq(torch.autograd.Function):
 def lstsq(A, b, opts=None):
     """x = argmin |A x - b| (FP64 result), differentiable in A and b."""
     return _Lstsq.apply(A, b, opts or SolveOptions())
+
+
+def _problem_table(probs, b64s, xs):
+    return (C.LsqProblem * len(probs))(*[
+        C.LsqProblem(p.m, p.n, p.ld, p.wh.data_ptr(), p.wr.data_ptr(), b.data_ptr(), x.data_ptr(),
+                     p.sketch.data_ptr(), p.s_rows) for p, b, x in zip(probs, b64s, xs)])
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() if t is not None else None for t in ts])
+
+
+def _adjoint_batched(probs, b64s, xs, gs, tol, want_a):
+    """w_i = (A_i^T A_i)^-1 g_i, grad_b_i = A_i w_i and (want_a) grad_A_i for
+    every problem, through tomos_lsq_adjoint_batched."""
+    opts = probs[0].opts
+    o = probs[0].options(tol, opts.inner, opts.restarts)
+    dev = gs[0].device
+    P = len(probs)
+    table = _problem_table(probs, b64s, xs)
+    ws = [torch.empty(p.n, dtype=torch.float64, device=dev) for p in probs]
+    gbs = [torch.empty(p.m, dtype=torch.float64, device=dev) for p in probs]
+    gas = [torch.empty((p.n, p.m), dtype=torch.float32, device=dev) if want_a else None for p in probs]
+    ldg = (ctypes.c_int64 * P)(*[p.m for p in probs])
+    workers = max(1, min(16, opts.adjoint_workers))
+    sb = C.lib().tomos_lsq_adjoint_batched_scratch_size(ctypes.byref(o), P, table, workers)
+    if sb == 0:
+        raise RuntimeError(C.lib().tomos_last_error().decode())
+    scr = _bytes(sb, dev)
+    its = (ctypes.c_int * P)()
+    C.call("tomos_lsq_adjoint_batched", ctypes.byref(o), P, table, _ptrs(gs), _ptrs(ws), _ptrs(gbs),
+           _ptrs(gas) if want_a else None, ldg, workers, _p(scr), sb, its, _stream())
+    return ws, gbs, [g.t() if g is not None else None for g in gas], list(its)
+
+
+class _LstsqBatched(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, opts, nprob, *tensors):
+        As, bs = tensors[:nprob], tensors[nprob:]
+        probs = [PackedProblem(A.detach(), opts) for A in As]
+        dev = As[0].device
+        b64s = [b.detach().to(torch.float64).contiguous() for b in bs]
+        xs = [torch.zeros(p.n, dtype=torch.float64, device=dev) for p in probs]
+        o = probs[0].options(opts.tol, opts.inner, opts.restarts)
+        table = _problem_table(probs, b64s, xs)
+        sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), nprob, table, 0, 0, 0)
+        if sb == 0:
+            raise RuntimeError(C.lib().tomos_last_error().decode())
+        scr = _bytes(sb, dev)
+        its = (ctypes.c_int * nprob)()
+        C.call("tomos_lsqr_batched_solve", ctypes.byref(o), nprob, table, 0, 0, 0, 1, _p(scr), sb, its, None,
+               _stream())
+        if opts.normal_correction:
+            ss = [p.normal_residual(b, x) for p, b, x in zip(probs, b64s, xs)]
+            deltas, _, _, _ = _adjoint_batched(probs, b64s, xs, ss, opts.adjoint_tol, False)
+            xs = [x + d for x, d in zip(xs, deltas)]
+        ctx.probs, ctx.b64s, ctx.nprob = probs, b64s, nprob
+        ctx.b_dtypes = [b.dtype for b in bs]
+        ctx.last_iterations = list(its)
+        ctx.save_for_backward(*xs)
+        return tuple(xs)
+
+    @staticmethod
+    def backward(ctx, *gs):
+        xs = ctx.saved_tensors
+        probs, P = ctx.probs, ctx.nprob
+        dev = xs[0].device
+        g64 = [(g if g is not None else torch.zeros(p.n, device=dev)).detach().to(torch.float64).contiguous()
+               for g, p in zip(gs, probs)]
+        want_a = any(ctx.needs_input_grad[2:2 + P])
+        _, gbs, gas, _ = _adjoint_batched(probs, ctx.b64s, list(xs), g64, probs[0].opts.adjoint_tol, want_a)
+        grad_As = [ga if ctx.needs_input_grad[2 + i] else None for i, ga in enumerate(gas)]
+        grad_bs = [gb.to(dt) if ctx.needs_input_grad[2 + P + i] else None
+                   for i, (gb, dt) in enumerate(zip(gbs, ctx.b_dtypes))]
+        return (None, None, *grad_As, *grad_bs)
+
+
+def lstsq_batched(As, bs, opts=None):
+    """[argmin |A_i x - b_i|] for lists of problems of any shapes (FP64
+    results), differentiable in every A_i and b_i."""
+    if len(As) != len(bs) or not As:
+        raise ValueError("need as many right-hand sides as matrices (at least one)")
+    return list(_LstsqBatched.apply(opts or SolveOptions(), len(As), *As, *bs))
