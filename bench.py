@@ -62,6 +62,8 @@ def parse():
                    help="entries per row of the CSR LSQR leg (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
                    help="batched differentiable least squares: number of uneven problems (0 = skip)")
+    p.add_argument("--batched-restarts", type=int, default=12,
+                   help="restart cap of the batched solve (its graph holds that many restarts)")
     p.add_argument("--adjoint-workers", type=int, default=8,
                    help="host threads (one HIP stream each) of the batched backward")
     p.add_argument("--stream-loads", type=int, default=0, choices=(0, 1, 2),
@@ -230,7 +232,10 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
     qp = (C.QrProblem * P)(*[C.QrProblem(p["s"], p["n"], p["s"], p["sk"].ptr) for p in probs])
     qsb = C.lib().tomos_sketch_qr_precond_batched_scratch_size(P, qp)
     qs = DeviceBuffer(qsb)
-    o = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, args.inner, 60, 8, 1)
+    # the whole solve is one graph launch with blind restarts: the restart cap
+    # bounds the launches enqueued (12 x 8 iterations, over twice what C5 needs)
+    o = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, args.inner,
+                      args.batched_restarts, 8, 1, 1)
     lp = (C.LsqProblem * P)(*[C.LsqProblem(p["m"], p["n"], p["lda"], p["wh"].ptr, p["wr"].ptr, p["b"].ptr,
                                            p["x"].ptr, p["sk"].ptr, p["s"]) for p in probs])
     sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), P, lp, 0, 0, 0)
@@ -265,7 +270,8 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
         bn = np.linalg.norm(p["b"].to_numpy(np.float64, (p["m"],)))
         worst = max(worst, rn / bn)
     # the same forward solves one problem at a time (single-problem solver)
-    so = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, args.inner, 60, 8, 1)
+    so = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-6, 1e-6, args.inner, 60, 8, 1,
+                       1)
     mmax = int(ms.max())
     nmax = int(ns.max())
     sarena = DeviceBuffer(C.lib().tomos_lsqr_arena_size(ctypes.byref(so), mmax, nmax))
@@ -425,8 +431,9 @@ def main():
     cs_bytes = C.lib().tomos_countsketch_scratch_size(m, n, s)
     cs_scratch = DeviceBuffer(cs_bytes)
     x = DeviceBuffer(n * 8)
+    # confirm = 1: the solve stops only when a restart's exact residual is below tol
     opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64,
-                        args.tol, args.tol, args.inner, args.restarts, args.check_every, args.fused)
+                        args.tol, args.tol, args.inner, args.restarts, args.check_every, args.fused, 1)
     ar_bytes = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
     sc_bytes = C.lib().tomos_lsqr_scratch_size(ctypes.byref(opt), m, n)
     arena, scr = DeviceBuffer(ar_bytes), DeviceBuffer(sc_bytes)

@@ -78,18 +78,29 @@ CountSketch create_count_sketch(std::size_t k, std::size_t m);
 void generate_count_sketch(CountSketch& sketch);
 // target (k x n, FP64 or FP32) = S * source.  countsketch.hpp:56-60
 void apply_count_sketch(CountSketch& sketch, Dense& source, Dense& target);
-// Householder QR of the k x n FP64 sketch; the leading n x n block is
-// overwritten with R^-1.  Throws std::invalid_argument if k < n and
-// std::runtime_error if R is singular.  sketch_qr.cpp:68-122
+// Householder QR of the k x n sketch (FP64, or FP32 factored in FP64 and
+// written back as FP32); the leading n x n block is overwritten with R^-1.
+// Throws std::invalid_argument if k < n and std::runtime_error if R is
+// singular.  sketch_qr.cpp:68-122
 void generate_sketch_qr_precond(CountSketch& sketch, Dense& sketched_mtx);
+// The reference's signature (sketch_qr.hpp:19): the sketch is consulted
+// only for its device, which must be sketched_mtx's.
+void generate_sketch_qr_precond(Sparse& sketch, Dense& sketched_mtx);
 
 // The reference's CSR sketch form (k x m, one entry per row at a hashed
-// column; countsketch.hpp:56-60): values +-1.  Deterministic per process
-// (a seed sequence, as the unseeded Gaussian sketch below); needs nnz == k.
+// column; countsketch.hpp:56-60): values +-1.  generate_count_sketch runs
+// the registry's generate_idxs slot and then its generate_values slot
+// (countsketch.cpp:15-18).  Deterministic per process (a seed sequence, as
+// the unseeded Gaussian sketch below); needs nnz == k.
 void generate_count_sketch(Sparse& sketch);
-// target (k x n, FP32 or FP64) = S * source (m x n, FP32 or FP64) for a CSR S
-// (the reference registers this on the CPU only, sketch_registries.cpp:25-40).
+// target (k x n, FP32 or FP64) = S * source (m x n, FP32 or FP64) for a CSR
+// S, through dispatch::Sketch::lookup_apply_count_sketch_dense (the
+// reference registers this on the CPU only, sketch_registries.cpp:25-40).
 void apply_count_sketch(Sparse& sketch, Dense& source, Dense& target);
+// CSR * CSR into a dense target (countsketch.hpp:62): declared for the
+// drop-in, nothing is registered on the accelerator, so the lookup throws
+// std::runtime_error.
+void apply_count_sketch(Sparse& sketch, Sparse& source, Dense& target);
 
 // Dense Gaussian sketch: S <- i.i.d. N(0,1), S FP64, FP32, FP16 or BF16
 // (include/core/sketch.hpp:11-13).  The reference seeds each call from the
@@ -143,8 +154,11 @@ using lsqr_dense_fn = void (*)(Context context, tomos::Dense& A, tomos::Dense& b
                                Workspace& workspace, double tol, int max_iters, double max_relres,
                                int* iterations, std::vector<double>* history, int restarts,
                                tomos::Dense* mtx_refine);
-using apply_count_sketch_dense_fn = void (*)(CountSketch& S, tomos::Dense& source,
-                                             tomos::Dense& target);
+// sketch_registries.hpp:14-19
+using generate_count_sketch_values_fn = Status (*)(tomos::Sparse&);
+using generate_count_sketch_idxs_fn = Status (*)(tomos::Sparse&);
+using apply_count_sketch_dense_fn = void (*)(tomos::Sparse&, tomos::Dense&, tomos::Dense&);
+using apply_count_sketch_sparse_fn = void (*)(tomos::Sparse&, tomos::Sparse&, tomos::Dense&);
 using create_gaussian_sketch_dense_fn = Status (*)(tomos::Dense& S, std::uint64_t seed);
 using create_gaussian_sketch_sparse_fn = Status (*)(tomos::Sparse& S, std::uint64_t seed);
 using mv_sparse_fn = void (*)(TransposeType trans, double alpha, tomos::Sparse& A, tomos::Dense& x, double beta,
@@ -248,15 +262,39 @@ private:
     std::array<mv_sparse_fn, N> mv_{};
 };
 
+// sketch_registries.hpp:21-66: slots keyed by (device, value type),
+// (device, index type) and (device, sketch, source, target, index types)
+// with the same mixed-radix encodes.
+constexpr std::size_t num_registry_index_types = static_cast<std::size_t>(IntType::Count);
+
 struct Sketch {
     static const Sketch& get();
-    apply_count_sketch_dense_fn lookup_apply_count_sketch_dense(Device dev,
-                                                                NumericType source) const;
+    generate_count_sketch_values_fn lookup_generate_count_sketch_values(Device dev, NumericType type) const;
+    generate_count_sketch_idxs_fn lookup_generate_count_sketch_idxs(Device dev, IntType type) const;
+    apply_count_sketch_dense_fn lookup_apply_count_sketch_dense(Device dev, NumericType sketch_type,
+                                                                NumericType source_type,
+                                                                NumericType target_type,
+                                                                IntType index_type) const;
+    apply_count_sketch_sparse_fn lookup_apply_count_sketch_sparse(Device dev, NumericType sketch_type,
+                                                                  NumericType source_type,
+                                                                  NumericType target_type,
+                                                                  IntType index_type) const;
+    static std::size_t encode_values(Device dev, NumericType type);
+    static std::size_t encode_idxs(Device dev, IntType type);
+    static std::size_t encode_apply(Device dev, NumericType sketch_type, NumericType source_type,
+                                    NumericType target_type, IntType index_type);
 
 private:
     Sketch();
-    std::array<apply_count_sketch_dense_fn, num_registry_devices * num_registry_numeric_types>
-        apply_{};
+    static constexpr std::size_t N_values = num_registry_devices * num_registry_numeric_types;
+    static constexpr std::size_t N_idxs = num_registry_devices * num_registry_index_types;
+    static constexpr std::size_t N_apply = num_registry_devices * num_registry_numeric_types *
+                                           num_registry_numeric_types * num_registry_numeric_types *
+                                           num_registry_index_types;
+    std::array<generate_count_sketch_values_fn, N_values> generate_values_{};
+    std::array<generate_count_sketch_idxs_fn, N_idxs> generate_idxs_{};
+    std::array<apply_count_sketch_dense_fn, N_apply> apply_dense_{};
+    std::array<apply_count_sketch_sparse_fn, N_apply> apply_sparse_{};
 };
 
 // gaussian_sketch_registries.hpp: one slot per (device, value type).  The

@@ -237,8 +237,16 @@ struct Workspace {
     // Extension (no reference counterpart): run each iteration's A p and
     // A^T u from one read of A (single-pass kernel) and restarts in one
     // pass where supported; the recurrence is the same up to FP32
-    // summation order.  Off by default.
-    bool single_pass_ = false;
+    // summation order.  On by default (the kernels fall back to separate
+    // products where it does not apply); false reads A twice per iteration
+    // as the reference's two GEMVs do.
+    bool single_pass_ = true;
+
+    // Extension: a stop on the |phi_bar| estimate only ends the current
+    // restart, and the solve stops when the next restart's true residual is
+    // below tol.  Off by default (the reference stops on the estimate,
+    // lsqr.cpp:417-425).
+    bool confirm_true_residual_ = false;
 
     Workspace(Device device, std::size_t nbytes);
     ~Workspace();

@@ -136,6 +136,14 @@ int tomos_fill_normal_f32(float* out, int64_t m, int64_t n, int64_t ld, uint64_t
                           const float* col_scale, int exp_lo, int exp_hi,
                           tomos_stream_t stream);
 
+/* dst (rows x cols, ldd) <- src (rows x cols, lds), column-major, types
+ * TOMOS_F32 / TOMOS_F64 in either direction (FP64 -> FP32 rounds to
+ * nearest).  The reference casts a host copy element by element where a
+ * matrix changes precision (lsqr_precision_runner.cu:620-628); here the
+ * cast runs on the device.  Asynchronous. */
+int tomos_convert(int src_type, int dst_type, int64_t rows, int64_t cols, const void* src, int64_t lds,
+                  void* dst, int64_t ldd, tomos_stream_t stream);
+
 /* ---- pack / decode ---------------------------------------------------- */
 /* Replaces cuda::split_dense_fp32_16 (fp32_16_cuda_kernels.cu:271-299) and
  * cpu::split_dense_fp32_16 (fp32_16_kernels.cpp:8-16), for every encoder
@@ -282,6 +290,17 @@ int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m,
                  const void* vals, const void* row_ptrs, const void* col_idxs, int x_type, const void* x,
                  double beta, int y_type, void* y, void* scratch, size_t scratch_bytes, tomos_stream_t stream);
 
+/* Y (m x ncols, ldy) = alpha * A X + beta * Y, A CSR (m x n), X dense
+ * (n x ncols, ldx), X / Y TOMOS_F32 or TOMOS_F64, each output one FP64 sum
+ * over its row's entries in storage order; beta == 0 does not read Y.
+ * Replaces cpu::mm_csr with both operands untransposed
+ * (src/kernels/cpu/matrix/mm_cpu_kernels.cpp:85-108), the kernel behind
+ * apply_count_sketch(Sparse&, Dense&, Dense&) (sketch_registries.cpp:27-41).
+ * One launch, memory-safe for any index contents.  Asynchronous. */
+int tomos_csr_mm(int val_type, int idx_type, double alpha, int64_t m, int64_t n, int64_t nnz, const void* vals,
+                 const void* row_ptrs, const void* col_idxs, int64_t ncols, int x_type, const void* X,
+                 int64_t ldx, double beta, int y_type, void* Y, int64_t ldy, tomos_stream_t stream);
+
 /* CSR of A^T (n x m: vals_t nnz, row_ptrs_t n+1, col_idxs_t nnz, same
  * types) with each row of A^T in A's row order.  Validates A first.  No
  * reference counterpart (the reference's transposed product scatters,
@@ -311,7 +330,12 @@ int tomos_gaussian_fill(int type, int64_t rows, int64_t cols, void* out, int64_t
  * SplitMix64 output from seed ^ 0xc2b2ae3d27d4eb4f; values +-1 from z_i's
  * top bit (kind 0, the count sketch) or N(0,1) as tomos_gaussian_fill of a
  * rows x 1 column with `seed` (kind 1, create_gaussian_sketch(Sparse&),
- * gaussian_sketch_registries.cpp:33-57).  nnz = rows.  Asynchronous. */
+ * gaussian_sketch_registries.cpp:33-57).  nnz = rows.  Kind 2 writes the
+ * pattern only (row_ptrs, col_idxs; vals untouched and may be NULL) and
+ * kind 3 the +-1 values only (row_ptrs and col_idxs untouched, may be NULL):
+ * the separate generate_count_sketch_idxs / _values registry slots
+ * (sketch_registries.hpp:14-15, called in that order by
+ * generate_count_sketch, countsketch.cpp:15-18).  Asynchronous. */
 int tomos_csr_sketch(int kind, int val_type, int idx_type, int64_t rows, int64_t cols, void* vals,
                      void* row_ptrs, void* col_idxs, uint64_t seed, tomos_stream_t stream);
 
@@ -384,6 +408,10 @@ typedef struct tomos_lsqr_options {
     int restarts;
     int check_every;   /* host polls the device stop flag every N iterations (>=1) */
     int fused;         /* 1: one pass over A per iteration (A*p and A^T*u fused) */
+    int confirm;       /* 1: a stop on the |phi_bar| estimate only ends the current restart; the
+                          solve stops when the next restart's true residual |b - A x|/|b|
+                          (restart tier) is below tol or max_relres.  0: the reference's rule.
+                          Zero-initialise the struct so older callers get 0. */
 } tomos_lsqr_options;
 
 /* Arena: the Krylov vectors u (m), v, w, temp1 (n, value_type) and

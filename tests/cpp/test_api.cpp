@@ -304,6 +304,121 @@ int main() {
         std::printf("    iterations=%d relres=%.3e\n", iters, std::sqrt(rn / bn));
         EXPECT(std::sqrt(rn / bn) < 1.5e-6);
     });
+    run("reference_runner_sketch_sequence", [] {
+        // The runner's preconditioner build and solve, call for call
+        // (lsqr_precision_runner.cu:609-656): Sparse S (k = 2n rows, nnz = k),
+        // generate_count_sketch(S), apply_count_sketch(S, A, sketched),
+        // generate_sketch_qr_precond(S, sketched), R^-1 copied into its own
+        // Dense, then lsqr_solve on packed A with implicit duplicates.
+        const std::size_t m = 6000, n = 60, k = 2 * n;
+        std::mt19937_64 gen(11);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        std::vector<float> a(m * n);
+        std::vector<double> a64(m * n), xt(n), bb(m, 0.0);
+        for (std::size_t j = 0; j < n; ++j) {
+            const double d = std::pow(10.0, -3.0 * double(j) / double(n - 1));
+            for (std::size_t i = 0; i < m; ++i) {
+                a[i + j * m] = static_cast<float>(nd(gen) * d);
+                a64[i + j * m] = a[i + j * m];
+            }
+        }
+        for (auto& v : xt) v = nd(gen);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < m; ++i) bb[i] += a64[i + j * m] * xt[j];
+        Dense A64 = make(NumericType::FP64, m, n, a64);
+        Dense A32 = make(NumericType::FP32, m, n, a);
+        // registry shape (sketch_registries.hpp:21-66)
+        const auto& reg = dispatch::Sketch::get();
+        EXPECT(&reg == &dispatch::Sketch::get());
+        EXPECT(throws<std::runtime_error>([&] {
+            reg.lookup_apply_count_sketch_sparse(Device::HIP, NumericType::FP64, NumericType::FP64,
+                                                 NumericType::FP64, IntType::INT32);
+        }));
+        EXPECT(throws<std::runtime_error>(
+            [&] { reg.lookup_generate_count_sketch_values(Device::CPU, NumericType::FP64); }));
+        EXPECT(throws<std::invalid_argument>(
+            [&] { reg.lookup_generate_count_sketch_idxs(Device::OMP, IntType::INT32); }));
+        EXPECT(throws<std::runtime_error>([&] {
+            reg.lookup_apply_count_sketch_dense(Device::HIP, NumericType::FP64, NumericType::FP32_BF16,
+                                                NumericType::FP64, IntType::INT32);
+        }));
+
+        Sparse S = Sparse::create_csr(ctx, NumericType::FP64, IntType::INT32, {k, m}, k);
+        generate_count_sketch(S);
+        Dense sketched = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {k, n});
+        apply_count_sketch(S, A64, sketched);
+        // the product is the signed row gather S A, exactly
+        std::vector<int> rp(k + 1), ci(k);
+        std::vector<double> sv(k);
+        S.copy_to_host(rp.data(), ci.data(), sv.data());
+        auto skh = fetch<double>(sketched, k * n);
+        double gerr = 0;
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < k; ++i) gerr = std::max(gerr, std::fabs(skh[i + j * k] - sv[i] * a64[ci[i] + j * m]));
+        EXPECT(gerr == 0.0);
+        // FP32 build type (the runner builds at FP32 unless --precond-type fp64):
+        // the same S on the FP32 source into an FP32 target, factored in FP64
+        Dense sk32 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, {k, n});
+        apply_count_sketch(S, A32, sk32);
+        generate_sketch_qr_precond(S, sketched);  // leading n x n block <- R^-1
+        generate_sketch_qr_precond(S, sk32);
+        auto r64 = fetch<double>(sketched, k * n);
+        auto r32 = fetch<float>(sk32, k * n);
+        double dmax = 0, rmax = 0;
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < n; ++i) {
+                dmax = std::max(dmax, std::fabs(double(r32[i + j * k]) - r64[i + j * k]));
+                rmax = std::max(rmax, std::fabs(r64[i + j * k]));
+                if (i > j) EXPECT(r64[i + j * k] == 0.0 && r32[i + j * k] == 0.0f);
+            }
+        std::printf("    R^-1 from the FP32 sketch vs the FP64 one: max |diff| / max |R^-1| = %.2e\n", dmax / rmax);
+        EXPECT(dmax <= 1e-5 * rmax);
+        Dense precond = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, n});
+        std::vector<double> ph(n * n);
+        for (std::size_t j = 0; j < n; ++j)
+            for (std::size_t i = 0; i < n; ++i) ph[i + j * n] = r64[i + j * k];
+        precond.copy_from_host(ph.data(), ph.size() * 8);
+
+        Dense A = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32_BF16, {m, n});
+        split_fp32_bf16(A32, A);
+        Dense b = make(NumericType::FP64, m, 1, bb);
+        Dense x = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        Workspace ws = create_lsqr_workspace(ctx, {m, n}, NumericType::FP32, NumericType::FP64);
+        add_precond(precond, ws);
+        ws.use_implicit_duplicate_ = true;
+        ws.mv_compute_type_ = NumericType::BF16;
+        ws.mv_compute_type_set_ = true;
+        EXPECT(ws.single_pass_);  // the single-pass kernels are the default
+        ws.confirm_true_residual_ = true;
+        int iters = 0;
+        std::vector<double> hist;
+        lsqr_solve(ctx, A, b, x, ws, 1e-6, 10, 1e-6, &iters, &hist, 40);
+        auto sol = fetch<double>(x, n);
+        double rn = 0, bn = 0;
+        for (std::size_t i = 0; i < m; ++i) {
+            double r = -bb[i];
+            for (std::size_t j = 0; j < n; ++j) r += a64[i + j * m] * sol[j];
+            rn += r * r;
+            bn += bb[i] * bb[i];
+        }
+        std::printf("    iterations=%d history=%zu true relres=%.3e\n", iters, hist.size(), std::sqrt(rn / bn));
+        EXPECT(std::sqrt(rn / bn) <= 1e-6);
+        // the two-pass path of the same solve reaches the same tolerance
+        Dense x2 = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {n, 1});
+        ws.single_pass_ = false;
+        lsqr_solve(ctx, A, b, x2, ws, 1e-6, 10, 1e-6, &iters, nullptr, 40);
+        auto sol2 = fetch<double>(x2, n);
+        double dx = 0, xn = 0;
+        for (std::size_t j = 0; j < n; ++j) {
+            dx = std::max(dx, std::fabs(sol2[j] - sol[j]));
+            xn = std::max(xn, std::fabs(sol[j]));
+        }
+        std::printf("    two-pass iterations=%d, max |x_two - x_single| / max |x| = %.2e\n", iters, dx / xn);
+        EXPECT(dx <= 1e-2 * xn);
+        Sparse S2 = Sparse::create_csr(ctx, NumericType::FP64, IntType::INT32, {3, m}, 3);
+        Dense small = Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP64, {3, n});
+        EXPECT(throws<std::invalid_argument>([&] { generate_sketch_qr_precond(S2, small); }));
+    });
     run("native_bf16_lsqr_explicit_duplicate", [] {
         // A stored as bf16 (round to nearest even) plus a copy at the refine
         // precision (FP64, as b and x) passed as mtx_refine (runner flag

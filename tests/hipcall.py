@@ -50,7 +50,7 @@ def gemv(a_type, tier, trans, alpha, m, n, a_buf, wr_buf, lda, x, beta=0.0, y0=N
 def lsqr(A_or_planes, b, a_type=C.F32, mv_tier=C.TIER_DEFAULT, restart_tier=None,
          value_type=C.F32, refine_type=C.F32, rinv=None, precond_type=-1, tol=1e-6,
          max_iters=400, max_relres=None, restarts=1, check_every=1, lda=None, m=None, n=None,
-         fused=0, history_cap=100000, refine_dup=None):
+         fused=0, history_cap=100000, refine_dup=None, confirm=0):
     """Runs tomos_lsqr_solve; A_or_planes is a numpy matrix (F32/F64) or a
     (wh_buf, wr_buf) pair of packed device planes (then m, n, lda needed)."""
     if restart_tier is None:
@@ -75,7 +75,7 @@ def lsqr(A_or_planes, b, a_type=C.F32, mv_tier=C.TIER_DEFAULT, restart_tier=None
         pdt = np.float64 if precond_type == C.F64 else np.float32
         rd = dev(np.asarray(rinv, dtype=pdt))
     opt = C.LsqrOptions(a_type, mv_tier, restart_tier, value_type, refine_type, precond_type,
-                        tol, max_relres, max_iters, restarts, check_every, fused)
+                        tol, max_relres, max_iters, restarts, check_every, fused, confirm)
     ab = C.lib().tomos_lsqr_arena_size(ctypes.byref(opt), m, n)
     sb = C.lib().tomos_lsqr_scratch_size(ctypes.byref(opt), m, n)
     assert ab > 0 and sb > 0, C.lib().tomos_last_error()

@@ -245,3 +245,49 @@ def test_csr_sketch_pattern(capi, orc, kind, it):
     else:
         ref = orc.gaussian_fill(rows, 1, seed)[:, 0]
         assert np.all(np.abs(vals - ref) <= 1e-6 * (1 + np.abs(ref)))
+
+
+@pytest.mark.parametrize("it", [C.INDEX_INT32, C.INDEX_INT64])
+@pytest.mark.parametrize("xt,yt,beta", [(C.F64, C.F64, 0.0), (C.F32, C.F64, 0.5), (C.F64, C.F32, -1.0)])
+def test_csr_mm_vs_oracle_columns(capi, orc, it, xt, yt, beta):
+    """Y = alpha A X + beta Y (tomos_csr_mm, the apply_count_sketch kernel) =
+    the oracle's CSR product column by column; ragged rows, repeated and
+    empty rows, ldx / ldy above the row counts."""
+    import ctypes  # noqa: F401
+    rng = np.random.default_rng(31 + it)
+    m, n, ncols = 700, 900, 37
+    rp, ci, v = random_csr(rng, m, n, 3.0)
+    idt = np.int64 if it == C.INDEX_INT64 else np.int32
+    xdt = np.float64 if xt == C.F64 else np.float32
+    ydt = np.float64 if yt == C.F64 else np.float32
+    ldx, ldy = n + 3, m + 5
+    X = np.zeros((ldx, ncols), xdt, order="F")
+    X[:n] = rng.standard_normal((n, ncols))
+    Y0 = np.zeros((ldy, ncols), ydt, order="F")
+    Y0[:m] = rng.standard_normal((m, ncols))
+    vals, rpd, cid = H.dev(v), H.dev(rp.astype(idt)), H.dev(ci.astype(idt))
+    Xd, Yd = H.dev(X), H.dev(Y0)
+    C.call("tomos_csr_mm", C.F64, it, 2.0, m, n, len(v), vals.ptr, rpd.ptr, cid.ptr, ncols, xt, Xd.ptr, ldx, beta,
+           yt, Yd.ptr, ldy, None)
+    Y = Yd.to_numpy(ydt, (ldy, ncols))
+    for j in range(ncols):
+        ref = orc.csr_mv(0, 2.0, m, n, rp, ci, v, X[:n, j].astype(np.float64), beta, Y0[:m, j].astype(np.float64))
+        tol = 1e-13 if yt == C.F64 else 1e-6
+        scale = np.abs(ref).max() + 1.0
+        assert np.max(np.abs(Y[:m, j] - ref)) <= tol * scale * 4, j
+    # rows between m and ldy are untouched
+    assert np.array_equal(Y[m:], Y0[m:])
+
+
+def test_convert_roundtrip(capi):
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((50, 7))
+    src = np.zeros((53, 7), np.float64, order="F")
+    src[:50] = A
+    s, d32, d64 = H.dev(src), H.DeviceBuffer(52 * 7 * 4), H.DeviceBuffer(50 * 7 * 8)
+    C.call("tomos_convert", C.F64, C.F32, 50, 7, s.ptr, 53, d32.ptr, 52, None)
+    C.call("tomos_convert", C.F32, C.F64, 50, 7, d32.ptr, 52, d64.ptr, 50, None)
+    got32 = d32.to_numpy(np.float32, (52, 7))[:50]
+    assert np.array_equal(got32, A.astype(np.float32))
+    assert np.array_equal(d64.to_numpy(np.float64, (50, 7)), A.astype(np.float32).astype(np.float64))
+    assert C.lib().tomos_convert(C.F16, C.F32, 1, 1, s.ptr, 1, d32.ptr, 1, None) == 1

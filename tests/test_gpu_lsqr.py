@@ -9,6 +9,9 @@ from tests import hipcall as H
 from tomos_amd import _capi as C
 
 pytestmark = pytest.mark.gpu
+# tol = 1e-6 on the true residual; the factor covers the FP32 rounding of the
+# stored residual vector the solver's norm is taken from (~1e-7 relative)
+TRUE_TOL = 1e-6 * (1 + 1e-6)
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lsqr_known_answers.json")))
 
 
@@ -124,10 +127,11 @@ def test_packed_mixed_precision_restarts_reach_1e6(capi, a_type, variant, fused)
     x, it, hist = H.lsqr((wh, wr), b, a_type=a_type, mv_tier=C.TIER_DEFAULT,
                          restart_tier=C.TIER_EXACT, value_type=C.F32, refine_type=C.F64,
                          rinv=rinv, precond_type=C.F64, tol=1e-6, max_iters=12, restarts=20,
-                         m=m, n=n, fused=fused)
+                         m=m, n=n, fused=fused, confirm=1)
     A64 = A.astype(np.float64)
     rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
-    assert rr < 1.5e-6, (rr, it)
+    # confirm=1: the solve stops only on the true (exact-tier) restart residual
+    assert rr <= TRUE_TOL, (rr, it)
     xls = np.linalg.lstsq(A64, b, rcond=None)[0]
     assert np.linalg.norm(x - xls) / np.linalg.norm(xls) < 1e-3
 
@@ -193,10 +197,10 @@ def test_one_pass_restart_reaches_tolerance(capi, n):
     rinv, _ = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, 30008, 4 * n)
     x, it, hist = H.lsqr((wh, wr), b, a_type=C.FP32_BF16, mv_tier=C.TIER_DEFAULT, restart_tier=C.TIER_EXACT,
                          value_type=C.F32, refine_type=C.F64, rinv=rinv, precond_type=C.F64, tol=1e-6,
-                         max_iters=8, restarts=30, m=m, n=n, lda=30008, fused=1)
+                         max_iters=8, restarts=30, m=m, n=n, lda=30008, fused=1, confirm=1)
     A64 = A.astype(np.float64)
     rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
-    assert rr < 1.5e-6, (rr, it)
+    assert rr <= TRUE_TOL, (rr, it)
     assert it <= 60, it
 
 
@@ -220,10 +224,10 @@ def test_native_16bit_storage_with_explicit_duplicate(capi, a_type, fused):
     kw = dict(a_type=a_type, mv_tier=C.TIER_DEFAULT, restart_tier=C.TIER_FAST, value_type=C.F32,
               refine_type=C.F64, rinv=rinv, precond_type=C.F64, tol=1e-6, max_iters=10, restarts=20,
               m=m, n=n, fused=fused)
-    x, it, _ = H.lsqr((A16, None), b, refine_dup=(C.F32, A32, m), **kw)
+    x, it, _ = H.lsqr((A16, None), b, refine_dup=(C.F32, A32, m), confirm=1, **kw)
     A64 = A.astype(np.float64)
     rr = np.linalg.norm(A64 @ x - b) / np.linalg.norm(b)
-    assert rr < 1.5e-6, (rr, it)
+    assert rr <= TRUE_TOL, (rr, it)
     x2, it2, _ = H.lsqr((A16, None), b, **kw)
     rr2 = np.linalg.norm(A64 @ x2 - b) / np.linalg.norm(b)
     assert rr2 > 3 * rr, (rr, rr2)

@@ -59,7 +59,7 @@ class LsqrOptions(ctypes.Structure):
     _fields_ = [("a_type", cint), ("mv_tier", cint), ("restart_tier", cint),
                 ("value_type", cint), ("refine_type", cint), ("precond_type", cint),
                 ("tol", dbl), ("max_relres", dbl), ("max_iters", cint), ("restarts", cint),
-                ("check_every", cint), ("fused", cint)]
+                ("check_every", cint), ("fused", cint), ("confirm", cint)]
 
 
 # name -> (restype, argtypes)
@@ -86,6 +86,7 @@ _SIGS = {
     "tomos_copy_kernel": (cint, [vp, vp, sz, vp]),
     "tomos_read_kernel": (cint, [vp, sz, vp, vp]),
     "tomos_fill_normal_f32": (cint, [vp, i64, i64, i64, ctypes.c_uint64, vp, cint, cint, vp]),
+    "tomos_convert": (cint, [cint, cint, i64, i64, vp, i64, vp, i64, vp]),
     "tomos_pack": (cint, [cint, i64, i64, vp, i64, vp, vp, i64, vp]),
     "tomos_unpack": (cint, [cint, i64, i64, vp, vp, i64, vp, i64, vp]),
     "tomos_decode_wh": (cint, [cint, i64, vp, vp, vp]),
@@ -125,6 +126,8 @@ _SIGS = {
     "tomos_csr_mv_scratch_size": (sz, [cint, cint, cint, i64, i64, i64]),
     "tomos_csr_mv": (cint, [cint, cint, cint, ctypes.c_double, i64, i64, i64, vp, vp, vp, cint, vp,
                             ctypes.c_double, cint, vp, vp, sz, vp]),
+    "tomos_csr_mm": (cint, [cint, cint, ctypes.c_double, i64, i64, i64, vp, vp, vp, i64, cint, vp, i64,
+                            ctypes.c_double, cint, vp, i64, vp]),
     "tomos_csr_transpose_scratch_size": (sz, [cint, cint, i64, i64, i64]),
     "tomos_csr_transpose": (cint, [cint, cint, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
     "tomos_countsketch_indices": (cint, [i64, i64, vp, vp, vp]),

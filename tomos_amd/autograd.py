@@ -60,6 +60,7 @@ class SolveOptions:
     check_every: int = 8
     normal_correction: bool = True  # forward: x += (A^T A)^-1 A^T (b - A x), exact products
     adjoint_workers: int = 8     # lstsq_batched: host threads (HIP streams) of the batched adjoint
+    confirm: bool = True         # forward: stop only when a restart's true residual is below tol
 
 
 class PackedProblem:
@@ -88,12 +89,12 @@ class PackedProblem:
         # R^-1 overwrites the leading n x n block (ld = s_rows)
         C.call("tomos_sketch_qr_precond_f64", self.s_rows, n, _p(self.sketch), self.s_rows, s)
 
-    def options(self, tol, iters, restarts):
+    def options(self, tol, iters, restarts, confirm=False):
         return C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64,
-                             tol, tol, iters, restarts, self.opts.check_every, 1)
+                             tol, tol, iters, restarts, self.opts.check_every, 1, int(confirm))
 
     def solve(self, b64):
-        o = self.options(self.opts.tol, self.opts.inner, self.opts.restarts)
+        o = self.options(self.opts.tol, self.opts.inner, self.opts.restarts, self.opts.confirm)
         m, n, dev = self.m, self.n, b64.device
         x = torch.empty(n, dtype=torch.float64, device=dev)
         ab = C.lib().tomos_lsqr_arena_size(ctypes.byref(o), m, n)
@@ -210,7 +211,7 @@ class _LstsqBatched(torch.autograd.Function):
         dev = As[0].device
         b64s = [b.detach().to(torch.float64).contiguous() for b in bs]
         xs = [torch.zeros(p.n, dtype=torch.float64, device=dev) for p in probs]
-        o = probs[0].options(opts.tol, opts.inner, opts.restarts)
+        o = probs[0].options(opts.tol, opts.inner, opts.restarts, opts.confirm)
         table = _problem_table(probs, b64s, xs)
         sb = C.lib().tomos_lsqr_batched_scratch_size(ctypes.byref(o), nprob, table, 0, 0, 0)
         if sb == 0:

@@ -287,7 +287,7 @@ void adj_validate(const tomos_lsqr_options& o, int64_t m, int64_t n) {
 }
 
 struct Adj {
-    tomos_lsqr_options o;
+    tomos_lsqr_options o{};
     int64_t m, n, lda, ldr;
     const void* a;
     const uint16_t* wr;
@@ -472,7 +472,14 @@ void lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, const u
               const double* b, const double* x, const double* w, double* grad_b, float* grad_a, int64_t ldg,
               void* scratch, size_t scratch_bytes, hipStream_t s) {
     require(m >= 0 && n >= 0 && lda >= std::max<int64_t>(m, 1), "tomos_lsq_grad: bad sizes");
-    if (m == 0 || n == 0) return;
+    if (m == 0) return;
+    if (n == 0) {
+        // x is empty: grad_b = A w = 0, grad_A is m x 0
+        require(grad_b != nullptr, "tomos_lsq_grad: null grad_b");
+        k_adj_zero<<<static_cast<unsigned>(ceil_div(m, kT)), kT, 0, s>>>(grad_b, m);
+        TOMOS_LAUNCH_CHECK("k_adj_zero");
+        return;
+    }
     require(a && b && x && w && grad_b, "tomos_lsq_grad: null pointer");
     require(!grad_a || ldg >= m, "tomos_lsq_grad: ldg < m");
     const size_t need = grad_scratch_bytes(a_type, tier, m, n);

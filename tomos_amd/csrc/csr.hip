@@ -776,6 +776,82 @@ void csr_tmv(const CsrMat& Tb, int64_t panels, int64_t panel_rows, int64_t nout,
     if (sumsq_blocks) *sumsq_blocks = static_cast<int>(fb);
 }
 
+namespace {
+
+// Y (m x ncols) = alpha * A X + beta * Y for CSR A (m x n) and dense X
+// (n x ncols, ldx): one thread per output row, columns strided over
+// blockIdx.y; each output is one FP64 sum over the row's entries in
+// storage order (deterministic).  Clamped row ranges and skipped
+// out-of-range columns keep every access in bounds whatever the indices.
+template <int VT, class I>
+__global__ __launch_bounds__(kCsrThreads) void csr_mm_kernel(int64_t m, int64_t n, int64_t nnz, int64_t ncols,
+                                                             const void* vals, const I* __restrict__ rp,
+                                                             const I* __restrict__ ci, const void* X, bool x64,
+                                                             int64_t ldx, void* Y, bool y64, int64_t ldy,
+                                                             double alpha, double beta) {
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kCsrThreads + threadIdx.x;
+    if (row >= m) return;
+    const int64_t k0 = clamp_ptr(rp[row], nnz);
+    int64_t k1 = clamp_ptr(rp[row + 1], nnz);
+    k1 = k1 < k0 ? k0 : k1;
+    for (int64_t j = blockIdx.y; j < ncols; j += gridDim.y) {
+        double acc = 0.0;
+        for (int64_t k = k0; k < k1; ++k) {
+            const uint64_t c = static_cast<uint64_t>(static_cast<int64_t>(ci[k]));
+            if (c >= static_cast<uint64_t>(n)) continue;
+            const int64_t at = static_cast<int64_t>(c) + j * ldx;
+            const double xv = x64 ? static_cast<const double*>(X)[at]
+                                  : static_cast<double>(static_cast<const float*>(X)[at]);
+            acc = fma(load_val<VT>(vals, k), xv, acc);
+        }
+        const int64_t o = row + j * ldy;
+        double out = alpha * acc;
+        if (beta != 0.0)
+            out += beta * (y64 ? static_cast<double*>(Y)[o] : static_cast<double>(static_cast<float*>(Y)[o]));
+        if (y64) static_cast<double*>(Y)[o] = out;
+        else static_cast<float*>(Y)[o] = static_cast<float>(out);
+    }
+}
+
+template <int VT, class I>
+void launch_csr_mm(const CsrMat& A, int64_t ncols, const void* X, bool x64, int64_t ldx, void* Y, bool y64,
+                   int64_t ldy, double alpha, double beta, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>(ceil_div(A.m, kCsrThreads)),
+                    static_cast<unsigned>(std::min<int64_t>(ncols, 65535)));
+    csr_mm_kernel<VT, I><<<grid, kCsrThreads, 0, s>>>(A.m, A.n, A.nnz, ncols, A.vals, static_cast<const I*>(A.row_ptrs),
+                                                      static_cast<const I*>(A.col_idxs), X, x64, ldx, Y, y64, ldy,
+                                                      alpha, beta);
+}
+
+}  // namespace
+
+void csr_mm(const CsrMat& A, int64_t ncols, int x_type, const void* X, int64_t ldx, int y_type, void* Y, int64_t ldy,
+            double alpha, double beta, hipStream_t s) {
+    csr_require_valid_types(A, "csr_mm");
+    require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "csr_mm: X must be F32 or F64");
+    require(y_type == TOMOS_F32 || y_type == TOMOS_F64, "csr_mm: Y must be F32 or F64");
+    require(ncols >= 0, "csr_mm: negative column count");
+    require(ldx >= std::max<int64_t>(A.n, 1) && ldy >= std::max<int64_t>(A.m, 1), "csr_mm: leading dimension too small");
+    require(ceil_div(A.m, kCsrThreads) <= 0x7FFFFFFF, "csr_mm: too many rows");
+    if (A.m == 0 || ncols == 0) return;
+    require(A.row_ptrs && Y && (A.n == 0 || X) && (A.nnz == 0 || (A.vals && A.col_idxs)), "csr_mm: null pointer");
+    const bool x64 = x_type == TOMOS_F64, y64 = y_type == TOMOS_F64;
+#define TOMOS_CSR_MM(I)                                                                                   \
+    switch (A.val_type) {                                                                                 \
+        case TOMOS_F64: launch_csr_mm<TOMOS_F64, I>(A, ncols, X, x64, ldx, Y, y64, ldy, alpha, beta, s); break; \
+        case TOMOS_F32: launch_csr_mm<TOMOS_F32, I>(A, ncols, X, x64, ldx, Y, y64, ldy, alpha, beta, s); break; \
+        case TOMOS_F16: launch_csr_mm<TOMOS_F16, I>(A, ncols, X, x64, ldx, Y, y64, ldy, alpha, beta, s); break; \
+        default: launch_csr_mm<TOMOS_BF16, I>(A, ncols, X, x64, ldx, Y, y64, ldy, alpha, beta, s); break;      \
+    }
+    if (A.idx_type == TOMOS_INDEX_INT64) {
+        TOMOS_CSR_MM(int64_t)
+    } else {
+        TOMOS_CSR_MM(int32_t)
+    }
+#undef TOMOS_CSR_MM
+    TOMOS_LAUNCH_CHECK("csr_mm_kernel");
+}
+
 }  // namespace tomos_hip
 
 using namespace tomos_hip;
@@ -887,6 +963,15 @@ int tomos_csr_mv(int val_type, int idx_type, int trans, double alpha, int64_t m,
         csr_transpose(A, pr, vt, pt, it, sc, csr_transpose_scratch_bytes(A, pr), st);
         const CsrMat T = make_csr(val_type, idx_type, panels * n, m, nnz, vt, pt, it);
         csr_tmv(T, panels, pr, n, c, x_type, x, y_type, y, sc, csr_tmv_scratch_bytes(T, panels, n), st);
+    });
+}
+
+int tomos_csr_mm(int val_type, int idx_type, double alpha, int64_t m, int64_t n, int64_t nnz, const void* vals,
+                 const void* row_ptrs, const void* col_idxs, int64_t ncols, int x_type, const void* X, int64_t ldx,
+                 double beta, int y_type, void* Y, int64_t ldy, tomos_stream_t st) {
+    return guarded([&] {
+        const CsrMat A = make_csr(val_type, idx_type, m, n, nnz, vals, row_ptrs, col_idxs);
+        csr_mm(A, ncols, x_type, X, ldx, y_type, Y, ldy, alpha, beta, st);
     });
 }
 

@@ -61,6 +61,12 @@ void csr_tmv(const CsrMat& Tb, int64_t panels, int64_t panel_rows, int64_t nout,
              const void* x, int y_type, void* y, void* scratch, size_t scratch_bytes, hipStream_t s,
              int* sumsq_blocks = nullptr);
 
+// Y (m x ncols, ldy) = alpha * A X + beta * Y for dense X (n x ncols, ldx);
+// X, Y F32 or F64, FP64 sums in storage order; beta == 0 does not read Y.
+// Memory-safe for any index contents (as csr_mv).
+void csr_mm(const CsrMat& A, int64_t ncols, int x_type, const void* X, int64_t ldx, int y_type, void* Y, int64_t ldy,
+            double alpha, double beta, hipStream_t s);
+
 size_t csr_value_bytes(int val_type);
 size_t csr_index_bytes(int idx_type);
 

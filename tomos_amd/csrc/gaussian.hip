@@ -122,13 +122,13 @@ constexpr uint64_t kColSalt = 0xc2b2ae3d27d4eb4full;
 
 template <class I, int VT>
 __global__ void csr_sketch_pattern_kernel(int64_t rows, int64_t cols, uint64_t seed, I* rp, I* ci, void* vals,
-                                          bool signs) {
+                                          bool signs, bool pattern) {
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= rows; i += stride) {
-        rp[i] = static_cast<I>(i);
+        if (pattern) rp[i] = static_cast<I>(i);
         if (i == rows) break;
         const uint64_t z = sm64_mix((seed ^ kColSalt) + (static_cast<uint64_t>(i) + 1) * kGolden);
-        ci[i] = static_cast<I>(z % static_cast<uint64_t>(cols));
+        if (pattern) ci[i] = static_cast<I>(z % static_cast<uint64_t>(cols));
         if (signs) {
             const float v = (z >> 63) ? 1.0f : -1.0f;
             if constexpr (VT == TOMOS_F64) static_cast<double*>(vals)[i] = v;
@@ -141,15 +141,15 @@ __global__ void csr_sketch_pattern_kernel(int64_t rows, int64_t cols, uint64_t s
 
 template <class I>
 void launch_pattern(int vt, int64_t rows, int64_t cols, uint64_t seed, void* rp, void* ci, void* vals, bool signs,
-                    hipStream_t st) {
+                    bool pattern, hipStream_t st) {
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(ceil_div(rows + 1, 256), 1), 8192));
     I* r = static_cast<I*>(rp);
     I* c = static_cast<I*>(ci);
     switch (vt) {
-        case TOMOS_F64: csr_sketch_pattern_kernel<I, TOMOS_F64><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs); break;
-        case TOMOS_F32: csr_sketch_pattern_kernel<I, TOMOS_F32><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs); break;
-        case TOMOS_F16: csr_sketch_pattern_kernel<I, TOMOS_F16><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs); break;
-        default: csr_sketch_pattern_kernel<I, TOMOS_BF16><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs); break;
+        case TOMOS_F64: csr_sketch_pattern_kernel<I, TOMOS_F64><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs, pattern); break;
+        case TOMOS_F32: csr_sketch_pattern_kernel<I, TOMOS_F32><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs, pattern); break;
+        case TOMOS_F16: csr_sketch_pattern_kernel<I, TOMOS_F16><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs, pattern); break;
+        default: csr_sketch_pattern_kernel<I, TOMOS_BF16><<<g, 256, 0, st>>>(rows, cols, seed, r, c, vals, signs, pattern); break;
     }
     TOMOS_LAUNCH_CHECK("csr_sketch_pattern_kernel");
 }
@@ -162,18 +162,21 @@ using namespace tomos_hip;
 extern "C" int tomos_csr_sketch(int kind, int val_type, int idx_type, int64_t rows, int64_t cols, void* vals,
                                 void* row_ptrs, void* col_idxs, uint64_t seed, tomos_stream_t st) {
     return guarded([&] {
-        require(kind == 0 || kind == 1, "tomos_csr_sketch: kind must be 0 (+-1) or 1 (Gaussian)");
+        require(kind >= 0 && kind <= 3,
+                "tomos_csr_sketch: kind must be 0 (+-1), 1 (Gaussian), 2 (pattern only) or 3 (+-1 values only)");
         require(val_type == TOMOS_F64 || val_type == TOMOS_F32 || val_type == TOMOS_F16 || val_type == TOMOS_BF16,
                 "tomos_csr_sketch: values must be F64, F32, F16 or BF16");
         require(idx_type == TOMOS_INDEX_INT32 || idx_type == TOMOS_INDEX_INT64, "tomos_csr_sketch: bad index type");
         require(rows >= 0 && cols >= 1, "tomos_csr_sketch: need rows >= 0 and cols >= 1");
         require(idx_type == TOMOS_INDEX_INT64 || (rows <= 0x7FFFFFFF && cols <= 0x7FFFFFFF),
                 "tomos_csr_sketch: sizes exceed INT32 indices");
-        require(row_ptrs && (rows == 0 || (vals && col_idxs)), "tomos_csr_sketch: null output");
+        const bool pattern = kind != 3, signs = kind == 0 || kind == 3;
+        require((!pattern || row_ptrs) && (rows == 0 || ((vals || kind == 2) && (col_idxs || kind == 3))),
+                "tomos_csr_sketch: null output");
         if (idx_type == TOMOS_INDEX_INT64)
-            launch_pattern<int64_t>(val_type, rows, cols, seed, row_ptrs, col_idxs, vals, kind == 0, st);
+            launch_pattern<int64_t>(val_type, rows, cols, seed, row_ptrs, col_idxs, vals, signs, pattern, st);
         else
-            launch_pattern<int32_t>(val_type, rows, cols, seed, row_ptrs, col_idxs, vals, kind == 0, st);
+            launch_pattern<int32_t>(val_type, rows, cols, seed, row_ptrs, col_idxs, vals, signs, pattern, st);
         if (kind == 1 && rows > 0) {
             switch (val_type) {
                 case TOMOS_F64: launch_fill<TOMOS_F64>(rows, 1, vals, rows, seed, st); break;

@@ -71,6 +71,12 @@ __global__ void k_state_init(LsqrState* s) {
     }
 }
 
+// A restart that follows an unconfirmed estimate stop (done == 2) runs
+// again: the stop flag is cleared before its true-residual test.
+__global__ void k_resume(LsqrState* s) {
+    if (threadIdx.x == 0 && s->done == 2) s->done = 0;
+}
+
 __global__ __launch_bounds__(kT) void k_zero(void* x, bool f64, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
     if (i < n) st(x, f64, i, 0.0);
@@ -119,14 +125,16 @@ __device__ __forceinline__ void push_history(LsqrState* s, double* hist, int cap
     s->hist_len += 1;
 }
 
-// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708)
-__global__ void k_restart_check(LsqrState* s, double* hist, int cap, double tol, double maxrr) {
+// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708).
+// confirm: the estimate alone does not stop the solve; the restart's true
+// residual decides.
+__global__ void k_restart_check(LsqrState* s, double* hist, int cap, double tol, double maxrr, int confirm) {
     if (threadIdx.x != 0 || s->done) return;
     if (s->completed == 0) return;
     if (s->rhsnorm == 0.0) { s->done = 1; return; }
     const double rr = fabs(s->phi_bar) / s->rhsnorm;
     push_history(s, hist, cap, rr);
-    if (rr < tol || rr < maxrr) s->done = 1;
+    if (!confirm && (rr < tol || rr < maxrr)) s->done = 1;
 }
 
 // beta = sqrt(sum red[0..nb)) in a fixed order; inv_beta = 1/beta
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(kTail) void k_tail(LsqrState* S, int init, int pre,
                                                 const void* temp, void* v, void* w, void* z,
                                                 const void* p, bool v64, bool p64, void* x,
                                                 bool x64, int64_t n, double* hist, int cap,
-                                                double tol, double maxrr) {
+                                                double tol, double maxrr, int confirm) {
     if (S->done) return;
     __shared__ double sh[kTail / 64];
     __shared__ double bc[4];
@@ -263,7 +271,9 @@ __global__ __launch_bounds__(kTail) void k_tail(LsqrState* S, int init, int pre,
     if (S->rhsnorm == 0.0) { S->done = 1; return; }
     const double rr = fabs(S->phi_bar) / S->rhsnorm;
     push_history(S, hist, cap, rr);
-    if (rr < tol || rr < maxrr) S->done = 1;
+    // confirm: end this restart early (2); the next restart's exact
+    // residual decides whether the solve stops
+    if (rr < tol || rr < maxrr) S->done = confirm ? 2 : 1;
 }
 
 // Two buffers, as the reference's Workspace: the arena holds the Krylov
@@ -389,6 +399,7 @@ void validate(const tomos_lsqr_options& o, int64_t m, int64_t n) {
     require(o.precond_type == -1 || o.precond_type == TOMOS_F32 || o.precond_type == TOMOS_F64,
             "lsqr: precond_type must be F32, F64 or -1");
     require(o.max_iters >= 0 && o.restarts >= 0, "lsqr: negative iteration limits");
+    require(o.confirm == 0 || o.confirm == 1, "lsqr: confirm must be 0 or 1");
     require(m >= 0 && n >= 0, "lsqr: negative size");
 }
 
@@ -528,7 +539,7 @@ struct Solver {
                                   &S->done, s);
         k_tail<<<1, kTail, 0, s>>>(S, init ? 1 : 0, pre ? 1 : 0, static_cast<const double*>(scratch()),
                                    rch, Tm(), V(), W(), Z(), PV(), v64, p64, x, r64, n, hist(),
-                                   L.hist_cap, o.tol, o.max_relres);
+                                   L.hist_cap, o.tol, o.max_relres, o.confirm);
     }
 
     void iterate() {
@@ -558,11 +569,13 @@ struct Solver {
         tail(false);
     }
 
-    bool done_flag() {
+    // 0 running, 1 stopped, 2 estimate below tol awaiting the next
+    // restart's true residual (confirm mode)
+    int done_flag() {
         int d = 0;
         TOMOS_HIP_CHECK(hipMemcpyAsync(&d, &state()->done, sizeof(int), hipMemcpyDeviceToHost, s));
         TOMOS_HIP_CHECK(hipStreamSynchronize(s));
-        return d != 0;
+        return d;
     }
 
     void run(int* iterations, double* history, int history_cap, int* history_len) {
@@ -575,16 +588,19 @@ struct Solver {
         const int every = std::max(1, o.check_every);
         bool stop = false;
         for (int r = 0; r < o.restarts && !stop; ++r) {
-            k_restart_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres);
+            if (o.confirm) k_resume<<<1, 64, 0, s>>>(S);
+            k_restart_check<<<1, 64, 0, s>>>(S, hist(), L.hist_cap, o.tol, o.max_relres, o.confirm);
             initialize();
             x_zero = false;
             TOMOS_LAUNCH_CHECK("lsqr initialize");
-            if (done_flag()) break;
+            if (done_flag() == 1) break;
             for (int it = 0; it < o.max_iters; ++it) {
                 iterate();
                 if ((it + 1) % every == 0 || it + 1 == o.max_iters) {
                     TOMOS_LAUNCH_CHECK("lsqr iteration");
-                    if (done_flag()) { stop = true; break; }
+                    const int d = done_flag();
+                    if (d == 1) stop = true;
+                    if (d != 0) break;
                 }
             }
         }

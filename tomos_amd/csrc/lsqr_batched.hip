@@ -18,12 +18,17 @@
 // v, the Givens rotation, z = P w and x updates, the stop test and the
 // next p = R^-1 v.  A converged problem's blocks exit at once.
 //
-// The whole restart (1 + max_iters steps, all launches) is captured once
-// in a hipGraph and replayed `restarts` times; the host reads the done
-// flags between replays.  The grouped kernels' work lists (tile ranges per
-// block) are built on the host once per batch; their shape (row groups,
-// tiles per block) is a tunable.
+// The whole solve (state set-up and `restarts` restarts of 1 + max_iters
+// steps, every launch) is captured as one hipGraph, instantiated once per
+// launch configuration and kept, and submitted as ONE graph launch; a
+// stopped problem's blocks exit at once in the launches that follow, and
+// the host synchronizes once at the end.  The grouped kernels' work lists
+// (tile ranges per block) are built on the host per call and read by the
+// graph from the scratch; their shape (row groups, tiles per block) is a
+// tunable.
 #include <algorithm>
+#include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "codec.hpp"
@@ -100,16 +105,20 @@ __global__ __launch_bounds__(kTail) void k_batch_setup(BProb* P) {
     }
 }
 
-// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708)
-__global__ void k_batch_restart_check(BProb* P, int nprob, double tol, double maxrr) {
+// check_stopping_criteria at the top of a restart (lsqr.cpp:417-425, :708).
+// confirm: a problem whose estimate fell below tol (done == 2) runs this
+// restart again, and only its true residual (the tail's initialize test)
+// stops it.
+__global__ void k_batch_restart_check(BProb* P, int nprob, double tol, double maxrr, int confirm) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= nprob) return;
     BState* s = P[i].st;
+    if (s->done == 2) s->done = 0;
     if (s->done || s->completed == 0) return;
     const double rr = fabs(s->phi_bar) / s->rhsnorm;
     if (s->hist_len < P[i].hist_cap) P[i].hist[s->hist_len] = rr;
     s->hist_len += 1;
-    if (rr < tol || rr < maxrr) s->done = 1;
+    if (!confirm && (rr < tol || rr < maxrr)) s->done = 1;
 }
 
 // Exact value of the RPL rows in one 16-byte vector (hi h, lo l).
@@ -204,7 +213,7 @@ __device__ __forceinline__ double tail_sum(double v, double* sh) {
 //   initialize: rho_bar = alpha, phi_bar = beta, true-residual stop test;
 //   u_coef = -alpha/beta, stop test + history, next p = R^-1 v.
 __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ P, int init, double tol,
-                                                      double maxrr) {
+                                                      double maxrr, int confirm) {
     const BProb& p = P[blockIdx.x];
     BState* S = p.st;
     if (S->done) return;
@@ -298,7 +307,7 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
     const double rr = fabs(S->phi_bar) / S->rhsnorm;
     if (S->hist_len < p.hist_cap) p.hist[S->hist_len] = rr;
     S->hist_len += 1;
-    if (rr < tol || rr < maxrr) S->done = 1;
+    if (rr < tol || rr < maxrr) S->done = confirm ? 2 : 1;
 }
 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
@@ -395,6 +404,7 @@ void validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* p
     require(o.value_type == TOMOS_F32 && o.refine_type == TOMOS_F64 && o.precond_type == TOMOS_F64,
             "lsqr_batched: value_type F32, refine_type F64 and precond_type F64 (R^-1) are required");
     require(o.max_iters >= 0 && o.restarts >= 0, "lsqr_batched: negative iteration limits");
+    require(o.confirm == 0 || o.confirm == 1, "lsqr_batched: confirm must be 0 or 1");
     const int vec = o.a_type == TOMOS_F32 ? 4 : 8;
     const bool packed = o.a_type != TOMOS_F32;
     for (int i = 0; i < nprob; ++i) {
@@ -411,7 +421,7 @@ void validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* p
 }
 
 struct Batched {
-    tomos_lsqr_options o;
+    tomos_lsqr_options o{};
     int nprob;
     const tomos_lsq_problem* pr;
     Plan L;
@@ -487,7 +497,7 @@ struct Batched {
 
     void tail(bool init) {
         k_batch_tail<<<static_cast<unsigned>(nprob), kTail, tail_lds(), s>>>(dprobs(), init ? 1 : 0, o.tol,
-                                                                             o.max_relres);
+                                                                             o.max_relres, o.confirm);
     }
     void residual() {
         if (!L.nrow_items) return;
@@ -504,7 +514,7 @@ struct Batched {
     // One restart: check, exact residual, A^T u, tail(init), then max_iters steps.
     void restart_sequence() {
         k_batch_restart_check<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob, o.tol,
-                                                                                      o.max_relres);
+                                                                                      o.max_relres, o.confirm);
         residual();
         fused(true);
         tail(true);
@@ -514,58 +524,138 @@ struct Batched {
         }
     }
 
+    // every problem stopped (done == 1; 2 waits for its next restart)
     bool all_done(std::vector<BState>& h) {
         TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), states(), sizeof(BState) * nprob, hipMemcpyDeviceToHost, s));
         TOMOS_HIP_CHECK(hipStreamSynchronize(s));
         for (const BState& b : h)
-            if (!b.done) return false;
+            if (b.done != 1) return false;
         return true;
+    }
+
+    // Everything a captured solve depends on: the launch grids, the tail's
+    // LDS, the scalar arguments and the scratch address (the kernels read
+    // their problem tables and work lists from the scratch, rewritten by
+    // upload() before every launch).
+    struct GraphKey {
+        int device, nprob, nitems, nrow_items, g, threads, rounds, a_type, max_iters, restarts, confirm;
+        size_t lds;
+        double tol, maxrr;
+        const void* scratch;
+        bool operator==(const GraphKey& k) const {
+            return device == k.device && nprob == k.nprob && nitems == k.nitems && nrow_items == k.nrow_items &&
+                   g == k.g && threads == k.threads && rounds == k.rounds && a_type == k.a_type &&
+                   max_iters == k.max_iters && restarts == k.restarts && confirm == k.confirm && lds == k.lds &&
+                   tol == k.tol && maxrr == k.maxrr && scratch == k.scratch;
+        }
+    };
+    GraphKey key() const {
+        GraphKey k;
+        TOMOS_HIP_CHECK(hipGetDevice(&k.device));
+        k.nprob = nprob;
+        k.nitems = L.nitems;
+        k.nrow_items = L.nrow_items;
+        k.g = L.g;
+        k.threads = L.threads;
+        k.rounds = L.rounds;
+        k.a_type = o.a_type;
+        k.max_iters = o.max_iters;
+        k.restarts = o.restarts;
+        k.confirm = o.confirm;
+        k.lds = tail_lds();
+        k.tol = o.tol;
+        k.maxrr = o.max_relres;
+        k.scratch = sc;
+        return k;
+    }
+
+    // The whole solve: state set-up, then `restarts` restart sequences.
+    // Problems that have stopped turn their blocks of the later launches
+    // into immediate exits, so the sequence can run to its end blindly.
+    void solve_sequence() {
+        k_bstate_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
+        k_batch_setup<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
+        for (int r = 0; r < o.restarts; ++r) restart_sequence();
     }
 
     void run(bool use_graph, int* iterations, double* final_relres) {
         upload();
-        k_bstate_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
-        k_batch_setup<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
-        TOMOS_LAUNCH_CHECK("lsqr_batched setup");
         const size_t lds = tail_lds();
         TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_tail),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
         std::vector<BState> h(nprob);
-        hipGraphExec_t exec = nullptr;
-        hipGraph_t graph = nullptr;
         if (use_graph && o.restarts > 0) {
-            TOMOS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-            try {
-                restart_sequence();
-            } catch (...) {
-                hipGraph_t g2 = nullptr;
-                (void)hipStreamEndCapture(s, &g2);
-                if (g2) (void)hipGraphDestroy(g2);
-                throw;
+            // one graph launch for the whole solve, instantiated once per key
+            // and kept (library-wide cache of the last few keys)
+            hipGraphExec_t exec = graph_cache_find(key());
+            if (!exec) {
+                hipGraph_t graph = nullptr;
+                TOMOS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+                try {
+                    solve_sequence();
+                } catch (...) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                TOMOS_HIP_CHECK(hipStreamEndCapture(s, &graph));
+                const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                TOMOS_HIP_CHECK(e);
+                graph_cache_insert(key(), exec);
             }
-            TOMOS_HIP_CHECK(hipStreamEndCapture(s, &graph));
-            TOMOS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        }
-        try {
+            TOMOS_HIP_CHECK(hipGraphLaunch(exec, s));
+            TOMOS_LAUNCH_CHECK("lsqr_batched graph");
+            all_done(h);
+        } else {
+            k_bstate_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
+            k_batch_setup<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
+            TOMOS_LAUNCH_CHECK("lsqr_batched setup");
             for (int r = 0; r < o.restarts; ++r) {
                 if (all_done(h)) break;
-                if (exec) TOMOS_HIP_CHECK(hipGraphLaunch(exec, s));
-                else restart_sequence();
+                restart_sequence();
                 TOMOS_LAUNCH_CHECK("lsqr_batched restart");
             }
             all_done(h);
-        } catch (...) {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-            throw;
         }
-        if (exec) TOMOS_HIP_CHECK(hipGraphExecDestroy(exec));
-        if (graph) TOMOS_HIP_CHECK(hipGraphDestroy(graph));
         for (int i = 0; i < nprob; ++i) {
             if (iterations) iterations[i] = h[i].completed;
             if (final_relres) final_relres[i] = h[i].rhsnorm > 0 ? std::fabs(h[i].phi_bar) / h[i].rhsnorm : 0.0;
         }
+    }
+
+    // Library-wide cache of instantiated solve graphs (a few keys, oldest
+    // evicted).  Every solve synchronizes before it returns, so an evicted
+    // graph is not running.
+    static constexpr int kGraphCache = 8;
+    struct CacheEntry {
+        GraphKey key;
+        hipGraphExec_t exec;
+    };
+    static std::vector<CacheEntry>& cache() {
+        static std::vector<CacheEntry> c;
+        return c;
+    }
+    static std::mutex& cache_mutex() {
+        static std::mutex m;
+        return m;
+    }
+    static hipGraphExec_t graph_cache_find(const GraphKey& k) {
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        for (const CacheEntry& e : cache())
+            if (e.key == k) return e.exec;
+        return nullptr;
+    }
+    static void graph_cache_insert(const GraphKey& k, hipGraphExec_t exec) {
+        std::lock_guard<std::mutex> lock(cache_mutex());
+        auto& c = cache();
+        if (static_cast<int>(c.size()) >= kGraphCache) {
+            (void)hipGraphExecDestroy(c.front().exec);
+            c.erase(c.begin());
+        }
+        c.push_back(CacheEntry{k, exec});
     }
 };
 

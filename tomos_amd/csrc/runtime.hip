@@ -162,6 +162,21 @@ __global__ void copy_tail_kernel(char* __restrict__ dst, const char* __restrict_
     if (i < n) dst[i] = src[i];
 }
 
+// dst (rows x cols, ldd) <- src (rows x cols, lds), F32 <-> F64 (either
+// direction, or a strided copy of one type); FP64 -> FP32 rounds to nearest.
+__global__ __launch_bounds__(256) void convert_kernel(int64_t rows, int64_t cols, const void* __restrict__ src,
+                                                      bool s64, int64_t lds, void* __restrict__ dst, bool d64,
+                                                      int64_t ldd) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= rows) return;
+    for (int64_t j = blockIdx.y; j < cols; j += gridDim.y) {
+        const double v = s64 ? static_cast<const double*>(src)[i + j * lds]
+                             : static_cast<double>(static_cast<const float*>(src)[i + j * lds]);
+        if (d64) static_cast<double*>(dst)[i + j * ldd] = v;
+        else static_cast<float*>(dst)[i + j * ldd] = static_cast<float>(v);
+    }
+}
+
 }  // namespace tomos_hip
 
 using namespace tomos_hip;
@@ -301,6 +316,24 @@ int tomos_read_kernel(const void* src, size_t bytes, unsigned* flag, tomos_strea
         read16_kernel<<<dim3(static_cast<unsigned>(blocks)), 1024, 0, s>>>(static_cast<const uint4*>(src), n16,
                                                                           flag);
         TOMOS_LAUNCH_CHECK("read16_kernel");
+    });
+}
+
+int tomos_convert(int src_type, int dst_type, int64_t rows, int64_t cols, const void* src, int64_t lds, void* dst,
+                  int64_t ldd, tomos_stream_t s) {
+    return guarded([&] {
+        require((src_type == TOMOS_F32 || src_type == TOMOS_F64) && (dst_type == TOMOS_F32 || dst_type == TOMOS_F64),
+                "tomos_convert: types must be F32 or F64");
+        require(rows >= 0 && cols >= 0, "tomos_convert: negative size");
+        require(lds >= std::max<int64_t>(rows, 1) && ldd >= std::max<int64_t>(rows, 1),
+                "tomos_convert: leading dimension < rows");
+        if (rows == 0 || cols == 0) return;
+        require(src && dst, "tomos_convert: null pointer");
+        require(ceil_div(rows, 256) <= 0x7FFFFFFF, "tomos_convert: too many rows");
+        const dim3 grid(static_cast<unsigned>(ceil_div(rows, 256)), static_cast<unsigned>(std::min<int64_t>(cols, 65535)));
+        convert_kernel<<<grid, 256, 0, s>>>(rows, cols, src, src_type == TOMOS_F64, lds, dst, dst_type == TOMOS_F64,
+                                           ldd);
+        TOMOS_LAUNCH_CHECK("convert_kernel");
     });
 }
 

@@ -304,7 +304,7 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
     const bool pre = ws.precond_kind_ == PrecondKind::Dense;
     tomos::Dense* P = pre ? static_cast<tomos::Dense*>(ws.precond_) : nullptr;
 
-    tomos_lsqr_options o;
+    tomos_lsqr_options o{};
     o.a_type = abi_type(MV);
     o.mv_tier = tier_for(MV, compute);
     o.restart_tier = ws.use_implicit_duplicate_ ? TOMOS_TIER_EXACT : o.mv_tier;
@@ -317,6 +317,7 @@ void lsqr_dense_impl(Context context, tomos::Dense& A, tomos::Dense& b, tomos::D
     o.restarts = restarts;
     o.check_every = std::max(1, ws.check_every_);
     o.fused = ws.single_pass_ ? 1 : 0;
+    o.confirm = ws.confirm_true_residual_ ? 1 : 0;
     const size_t arena = tomos_lsqr_arena_size(&o, m, n);
     const size_t scratch = tomos_lsqr_scratch_size(&o, m, n);
     if (arena == 0 || scratch == 0) check(TOMOS_ERR_INVALID_ARGUMENT, "lsqr_solve");
@@ -384,7 +385,7 @@ void lsqr_sparse_impl(Context context, tomos::Sparse& A, tomos::Dense& b, tomos:
     if (ws.precond_kind_ == PrecondKind::Sparse)
         throw std::invalid_argument("lsqr_solve: a sparse preconditioner is not supported; use a dense R^-1.");
     tomos::Dense* P = pre ? static_cast<tomos::Dense*>(ws.precond_) : nullptr;
-    tomos_lsqr_options o;
+    tomos_lsqr_options o{};
     o.a_type = abi_type(MV);
     o.mv_tier = TOMOS_TIER_DEFAULT;
     o.restart_tier = TOMOS_TIER_DEFAULT;
@@ -397,6 +398,7 @@ void lsqr_sparse_impl(Context context, tomos::Sparse& A, tomos::Dense& b, tomos:
     o.restarts = restarts;
     o.check_every = std::max(1, ws.check_every_);
     o.fused = 0;
+    o.confirm = ws.confirm_true_residual_ ? 1 : 0;
     const int idx = abi_index(A.get_index_type());
     const int64_t nnz = static_cast<int64_t>(A.get_nnz());
     const size_t arena = tomos_lsqr_arena_size(&o, m, n);
@@ -552,7 +554,7 @@ lsqr_dense_fn Lsqr::lookup_solve_dense(Device dev, NumericType mv, NumericType v
 // Sketch + preconditioner
 // ======================================================================
 namespace {
-void cs_apply_impl(CountSketch& S, tomos::Dense& source, tomos::Dense& target) {
+void cs_apply_hashed(CountSketch& S, tomos::Dense& source, tomos::Dense& target) {
     if (target.get_value_type() != NumericType::FP64)
         throw std::invalid_argument("apply_count_sketch: target must be FP64");
     const auto ss = source.get_size(), ts = target.get_size();
@@ -570,21 +572,74 @@ void cs_apply_impl(CountSketch& S, tomos::Dense& source, tomos::Dense& target) {
 }
 }  // namespace
 
+// The registry's accelerator slots: CSR sketch pattern / values
+// (tomos_csr_sketch kinds 2 and 3) and S * source for a CSR S (tomos_csr_mm).
+Status cs_generate_idxs_impl(tomos::Sparse& S);
+Status cs_generate_values_impl(tomos::Sparse& S);
+void cs_apply_csr_impl(tomos::Sparse& S, tomos::Dense& source, tomos::Dense& target);
+
 namespace dispatch {
+std::size_t Sketch::encode_values(Device dev, NumericType type) {
+    return static_cast<std::size_t>(dev) * num_registry_numeric_types + static_cast<std::size_t>(type);
+}
+std::size_t Sketch::encode_idxs(Device dev, IntType type) {
+    return static_cast<std::size_t>(dev) * num_registry_index_types + static_cast<std::size_t>(type);
+}
+std::size_t Sketch::encode_apply(Device dev, NumericType sk, NumericType src, NumericType tgt, IntType idx) {
+    constexpr std::size_t T = num_registry_numeric_types, I = num_registry_index_types;
+    return static_cast<std::size_t>(dev) * T * T * T * I + static_cast<std::size_t>(sk) * T * T * I +
+           static_cast<std::size_t>(src) * T * I + static_cast<std::size_t>(tgt) * I +
+           static_cast<std::size_t>(idx);
+}
 Sketch::Sketch() {
-    apply_.fill(nullptr);
-    const std::size_t h = static_cast<std::size_t>(Device::HIP) * num_registry_numeric_types;
-    for (NumericType t : {NumericType::FP32, NumericType::FP64, NumericType::FP32_16, NumericType::FP32_BF16})
-        apply_[h + static_cast<std::size_t>(t)] = &cs_apply_impl;
+    generate_values_.fill(nullptr);
+    generate_idxs_.fill(nullptr);
+    apply_dense_.fill(nullptr);
+    apply_sparse_.fill(nullptr);
+    const NumericType vals[] = {NumericType::FP64, NumericType::FP32, NumericType::FP16, NumericType::BF16};
+    const NumericType dense[] = {NumericType::FP64, NumericType::FP32};
+    const IntType idxs[] = {IntType::INT32, IntType::INT64};
+    for (NumericType v : vals) generate_values_[encode_values(Device::HIP, v)] = &cs_generate_values_impl;
+    for (IntType i : idxs) generate_idxs_[encode_idxs(Device::HIP, i)] = &cs_generate_idxs_impl;
+    for (NumericType sk : vals)
+        for (NumericType src : dense)
+            for (NumericType tgt : dense)
+                for (IntType i : idxs) apply_dense_[encode_apply(Device::HIP, sk, src, tgt, i)] = &cs_apply_csr_impl;
+    // apply_sparse_ (CSR * CSR) stays empty: SpGEMM is not part of this library
 }
 const Sketch& Sketch::get() {
     static const Sketch instance;
     return instance;
 }
-apply_count_sketch_dense_fn Sketch::lookup_apply_count_sketch_dense(Device dev, NumericType src) const {
-    if (dev == Device::OMP) throw std::invalid_argument("apply_count_sketch: Device::OMP is not supported.");
-    auto fn = apply_[static_cast<std::size_t>(dev) * num_registry_numeric_types + static_cast<std::size_t>(src)];
-    if (!fn) throw std::runtime_error("apply_count_sketch: nothing registered for this device/type.");
+namespace {
+void no_omp(Device dev, const char* who) {
+    if (dev == Device::OMP) throw std::invalid_argument(std::string(who) + ": Device::OMP is not supported.");
+}
+}  // namespace
+generate_count_sketch_values_fn Sketch::lookup_generate_count_sketch_values(Device dev, NumericType type) const {
+    no_omp(dev, "generate_count_sketch");
+    auto fn = generate_values_[encode_values(dev, type)];
+    if (!fn) throw std::runtime_error("generate_count_sketch: no values kernel registered for this device/type.");
+    return fn;
+}
+generate_count_sketch_idxs_fn Sketch::lookup_generate_count_sketch_idxs(Device dev, IntType type) const {
+    no_omp(dev, "generate_count_sketch");
+    auto fn = generate_idxs_[encode_idxs(dev, type)];
+    if (!fn) throw std::runtime_error("generate_count_sketch: no index kernel registered for this device/type.");
+    return fn;
+}
+apply_count_sketch_dense_fn Sketch::lookup_apply_count_sketch_dense(Device dev, NumericType sk, NumericType src,
+                                                                    NumericType tgt, IntType idx) const {
+    no_omp(dev, "apply_count_sketch");
+    auto fn = apply_dense_[encode_apply(dev, sk, src, tgt, idx)];
+    if (!fn) throw std::runtime_error("apply_count_sketch: nothing registered for this device/type combination (dense source).");
+    return fn;
+}
+apply_count_sketch_sparse_fn Sketch::lookup_apply_count_sketch_sparse(Device dev, NumericType sk, NumericType src,
+                                                                      NumericType tgt, IntType idx) const {
+    no_omp(dev, "apply_count_sketch");
+    auto fn = apply_sparse_[encode_apply(dev, sk, src, tgt, idx)];
+    if (!fn) throw std::runtime_error("apply_count_sketch: nothing registered for this device/type combination (sparse source).");
     return fn;
 }
 }  // namespace dispatch
@@ -663,28 +718,47 @@ Status create_gaussian_sketch(tomos::Sparse& S, std::uint64_t seed) {
 
 Status create_gaussian_sketch(tomos::Sparse& S) { return create_gaussian_sketch(S, next_sketch_seed()); }
 
-void generate_count_sketch(tomos::Sparse& sketch) {
-    csr_sketch_impl(sketch, next_sketch_seed(), 0, "generate_count_sketch");
+Status cs_generate_idxs_impl(tomos::Sparse& S) {
+    return csr_sketch_impl(S, next_sketch_seed(), 2, "generate_count_sketch");
 }
 
-void apply_count_sketch(tomos::Sparse& sketch, tomos::Dense& source, tomos::Dense& target) {
-    // one CSR product per column of the source
+Status cs_generate_values_impl(tomos::Sparse& S) {
+    return csr_sketch_impl(S, next_sketch_seed(), 3, "generate_count_sketch");
+}
+
+void cs_apply_csr_impl(tomos::Sparse& sketch, tomos::Dense& source, tomos::Dense& target) {
     const auto ss = sketch.get_size(), xs = source.get_size(), ys = target.get_size();
     if (xs[0] != ss[1] || ys[0] != ss[0] || ys[1] != xs[1])
         throw std::invalid_argument("apply_count_sketch: sizes do not match (S k x m, source m x n, target k x n).");
+    if (sketch.get_device() != source.get_device() || source.get_device() != target.get_device())
+        throw std::invalid_argument("apply_count_sketch: sketch, source and target must be on the same device.");
     const NumericType xt = source.get_value_type(), yt = target.get_value_type();
-    if ((xt != NumericType::FP32 && xt != NumericType::FP64) || (yt != NumericType::FP32 && yt != NumericType::FP64))
-        throw std::invalid_argument("apply_count_sketch: source and target must be FP32 or FP64.");
-    for (std::size_t j = 0; j < xs[1]; ++j)
-        check(tomos_csr_mv(abi_type(sketch.get_value_type()), abi_index(sketch.get_index_type()), 0, 1.0,
-                           static_cast<int64_t>(ss[0]), static_cast<int64_t>(ss[1]),
-                           static_cast<int64_t>(sketch.get_nnz()), sketch.values(), sketch.row_ptrs(),
-                           sketch.col_idxs(), abi_type(xt),
-                           static_cast<char*>(source.data()) + j * source.get_lead_dim() * numeric_type_size(xt), 0.0,
-                           abi_type(yt),
-                           static_cast<char*>(target.data()) + j * target.get_lead_dim() * numeric_type_size(yt),
-                           nullptr, 0, sketch.stream()),
-              "apply_count_sketch");
+    check(tomos_csr_mm(abi_type(sketch.get_value_type()), abi_index(sketch.get_index_type()), 1.0,
+                       static_cast<int64_t>(ss[0]), static_cast<int64_t>(ss[1]), static_cast<int64_t>(sketch.get_nnz()),
+                       sketch.values(), sketch.row_ptrs(), sketch.col_idxs(), static_cast<int64_t>(xs[1]), abi_type(xt),
+                       source.data(), static_cast<int64_t>(source.get_lead_dim()), 0.0, abi_type(yt), target.data(),
+                       static_cast<int64_t>(target.get_lead_dim()), sketch.stream()),
+          "apply_count_sketch");
+}
+
+// countsketch.cpp:15-18
+void generate_count_sketch(tomos::Sparse& sketch) {
+    const auto& reg = dispatch::Sketch::get();
+    reg.lookup_generate_count_sketch_idxs(sketch.get_device(), sketch.get_index_type())(sketch);
+    reg.lookup_generate_count_sketch_values(sketch.get_device(), sketch.get_value_type())(sketch);
+}
+
+// countsketch.cpp:20-26 (sketch_dispatch: lookup by the four types)
+void apply_count_sketch(tomos::Sparse& sketch, tomos::Dense& source, tomos::Dense& target) {
+    dispatch::Sketch::get().lookup_apply_count_sketch_dense(sketch.get_device(), sketch.get_value_type(),
+                                                            source.get_value_type(), target.get_value_type(),
+                                                            sketch.get_index_type())(sketch, source, target);
+}
+
+void apply_count_sketch(tomos::Sparse& sketch, tomos::Sparse& source, tomos::Dense& target) {
+    dispatch::Sketch::get().lookup_apply_count_sketch_sparse(sketch.get_device(), sketch.get_value_type(),
+                                                             source.get_value_type(), target.get_value_type(),
+                                                             sketch.get_index_type())(sketch, source, target);
 }
 
 Status create_gaussian_sketch(tomos::Dense& S, std::uint64_t seed) {
@@ -700,29 +774,72 @@ void generate_count_sketch(CountSketch& sketch) {
     if (sketch.size[0] == 0) throw std::invalid_argument("generate_count_sketch: zero rows");
 }
 
+// The hashed sketch is an extension outside the reference's Sparse-keyed
+// registry: it calls the CountSketch kernel directly.
 void apply_count_sketch(CountSketch& sketch, tomos::Dense& source, tomos::Dense& target) {
-    dispatch::Sketch::get().lookup_apply_count_sketch_dense(source.get_device(), source.get_value_type())(
-        sketch, source, target);
+    device_guard(source.get_device(), "apply_count_sketch");
+    const NumericType t = source.get_value_type();
+    if (t != NumericType::FP32 && t != NumericType::FP64 && !is_packed(t))
+        throw std::runtime_error("apply_count_sketch: nothing registered for this device/type.");
+    cs_apply_hashed(sketch, source, target);
 }
 
-void generate_sketch_qr_precond(CountSketch& sketch, tomos::Dense& sketched) {
-    (void)sketch;
-    device_guard(sketched.get_device(), "generate_sketch_qr_precond");
-    if (sketched.get_device() != Device::HIP)
-        throw std::runtime_error("generate_sketch_qr_precond: unsupported device.");
-    if (sketched.get_value_type() != NumericType::FP64)
-        throw std::runtime_error("generate_sketch_qr_precond: unsupported NumericType (FP64 only).");
-    const auto s = sketched.get_size();
-    if (s[0] < s[1])
-        throw std::invalid_argument(
-            "generate_sketch_qr_precond: sketched_mtx must have at least as many rows as columns (k >= n).");
-    const int rc = tomos_sketch_qr_precond_f64(static_cast<int64_t>(s[0]), static_cast<int64_t>(s[1]),
-                                              static_cast<double*>(sketched.data()),
-                                              static_cast<int64_t>(sketched.get_lead_dim()), sketched.stream());
+namespace {
+void sketch_qr_f64(double* a, int64_t k, int64_t n, int64_t ld, stream_t st) {
+    const int rc = tomos_sketch_qr_precond_f64(k, n, a, ld, st);
     if (rc == TOMOS_ERR_SINGULAR)
         throw std::runtime_error(std::string("generate_sketch_qr_precond: trtri failed -- R is singular (") +
                                  tomos_last_error() + ")");
     check(rc, "generate_sketch_qr_precond");
+}
+}  // namespace
+
+// sketch_qr.cpp:68-122: FP64 sketches are factored in place; an FP32 sketch
+// is widened to FP64 on the device, factored there (the FP64 Householder QR
+// and R^-1), and R^-1 is rounded back into its leading n x n block.
+void generate_sketch_qr_precond_impl(tomos::Dense& sketched) {
+    device_guard(sketched.get_device(), "generate_sketch_qr_precond");
+    if (sketched.get_device() != Device::HIP)
+        throw std::runtime_error("generate_sketch_qr_precond: unsupported device.");
+    const NumericType t = sketched.get_value_type();
+    if (t != NumericType::FP64 && t != NumericType::FP32)
+        throw std::runtime_error(
+            "generate_sketch_qr_precond: unsupported NumericType (only FP64/FP32 are supported).");
+    const auto s = sketched.get_size();
+    if (s[0] < s[1])
+        throw std::invalid_argument(
+            "generate_sketch_qr_precond: sketched_mtx must have at least as many rows as columns (k >= n).");
+    const int64_t k = static_cast<int64_t>(s[0]), n = static_cast<int64_t>(s[1]),
+                  ld = static_cast<int64_t>(sketched.get_lead_dim());
+    if (t == NumericType::FP64) {
+        sketch_qr_f64(static_cast<double*>(sketched.data()), k, n, ld, sketched.stream());
+        return;
+    }
+    if (n == 0) return;
+    void* wide = nullptr;
+    check(tomos_malloc(&wide, static_cast<size_t>(k) * static_cast<size_t>(n) * 8), "generate_sketch_qr_precond");
+    struct Free {
+        void* p;
+        ~Free() { tomos_free(p); }
+    } guard{wide};
+    check(tomos_convert(TOMOS_F32, TOMOS_F64, k, n, sketched.data(), ld, wide, k, sketched.stream()),
+          "generate_sketch_qr_precond");
+    sketch_qr_f64(static_cast<double*>(wide), k, n, k, sketched.stream());
+    check(tomos_convert(TOMOS_F64, TOMOS_F32, n, n, wide, k, sketched.data(), ld, sketched.stream()),
+          "generate_sketch_qr_precond");
+    check(tomos_stream_synchronize(sketched.stream()), "generate_sketch_qr_precond");
+}
+
+void generate_sketch_qr_precond(CountSketch& sketch, tomos::Dense& sketched) {
+    (void)sketch;
+    generate_sketch_qr_precond_impl(sketched);
+}
+
+void generate_sketch_qr_precond(tomos::Sparse& sketch, tomos::Dense& sketched) {
+    if (sketch.get_device() != sketched.get_device())
+        throw std::invalid_argument(
+            "generate_sketch_qr_precond: sketch and sketched_mtx must be on the same device.");
+    generate_sketch_qr_precond_impl(sketched);
 }
 
 // ======================================================================

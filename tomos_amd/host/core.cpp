@@ -242,7 +242,8 @@ Workspace::Workspace(Workspace&& o) noexcept
       mv_transpose_scratch_(o.mv_transpose_scratch_),
       mv_transpose_scratch_capacity_(o.mv_transpose_scratch_capacity_),
       check_every_(o.check_every_),
-      single_pass_(o.single_pass_) {
+      single_pass_(o.single_pass_),
+      confirm_true_residual_(o.confirm_true_residual_) {
     o.data_ = nullptr;
     o.mv_transpose_scratch_ = nullptr;
     o.mv_transpose_scratch_capacity_ = 0;
