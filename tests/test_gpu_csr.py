@@ -271,7 +271,7 @@ def test_csr_mm_vs_oracle_columns(capi, orc, it, xt, yt, beta):
            yt, Yd.ptr, ldy, None)
     Y = Yd.to_numpy(ydt, (ldy, ncols))
     for j in range(ncols):
-        ref = orc.csr_mv(0, 2.0, m, n, rp, ci, v, X[:n, j].astype(np.float64), beta, Y0[:m, j].astype(np.float64))
+        ref = orc.csr_mv(0, 2.0, m, n, v, rp, ci, X[:n, j].astype(np.float64), beta, Y0[:m, j].astype(np.float64))
         tol = 1e-13 if yt == C.F64 else 1e-6
         scale = np.abs(ref).max() + 1.0
         assert np.max(np.abs(Y[:m, j] - ref)) <= tol * scale * 4, j
