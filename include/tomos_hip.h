@@ -187,6 +187,20 @@ int tomos_gemv(int a_type, int tier, int trans, double alpha, int64_t m, int64_t
                double beta, int y_type, void* y, void* scratch, size_t scratch_bytes,
                tomos_stream_t stream);
 
+/* tomos_gemv with no scratch argument, allocating nothing.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results
+ * as tomos_gemv.  Its partial sums live in a fixed device array of
+ * TOMOS_GEMV_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * a shape whose tomos_gemv_scratch_size() exceeds it is rejected with
+ * TOMOS_ERR_WORKSPACE.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (different streams or host threads) race. */
+#define TOMOS_GEMV_NOALLOC_SCRATCH_BYTES (32u << 20)
+int tomos_gemv_noalloc(int a_type, int tier, int trans, double alpha, int64_t m, int64_t n,
+                       const void* a, const uint16_t* wr, int64_t lda, int x_type, const void* x,
+                       double beta, int y_type, void* y, tomos_stream_t stream);
+
 /* ---- GEMM ------------------------------------------------------------- */
 /* C = alpha*op(A)*op(B) + beta*C with packed or plain A, FP32 B and C.
  * Replaces the mm_dense_fn slots (dense_registries.hpp:32-34) and

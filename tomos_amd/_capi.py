@@ -93,6 +93,8 @@ _SIGS = {
     "tomos_gemv_scratch_size": (sz, [cint, cint, cint, i64, i64]),
     "tomos_gemv": (cint, [cint, cint, cint, dbl, i64, i64, vp, vp, i64, cint, vp, dbl, cint,
                           vp, vp, sz, vp]),
+    "tomos_gemv_noalloc": (cint, [cint, cint, cint, dbl, i64, i64, vp, vp, i64, cint, vp, dbl,
+                                  cint, vp, vp]),
     "tomos_gemm": (cint, [cint, cint, cint, cint, dbl, i64, i64, i64, vp, vp, i64, vp, i64,
                           dbl, vp, i64, vp]),
     "tomos_gemm_split_scratch_size": (sz, [i64, i64, i64]),
