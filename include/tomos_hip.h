@@ -86,6 +86,10 @@ enum {
  *   EXACT:   wh+wr exact reconstruction, FP64 accumulate ("accurate",
  *            fp32_16_kernels.cpp:28-55).  F32/F64: FP64 accumulate. */
 enum { TOMOS_TIER_DEFAULT = 0, TOMOS_TIER_FAST = 1, TOMOS_TIER_EXACT = 2 };
+/* GEMM only: the reference's materialize slot (exact decode, then an FP32
+ * GEMM: dispatch::Fp32_16::mm_workspace, fp32_16_registries.cpp:432-449),
+ * computed as a 3-term split on 16-bit matrix cores. */
+enum { TOMOS_TIER_MATERIALIZE = 3 };
 
 /* ---- runtime helpers -------------------------------------------------- */
 const char* tomos_last_error(void);
@@ -205,8 +209,19 @@ int tomos_gemv_noalloc(int a_type, int tier, int trans, double alpha, int64_t m,
 /* C = alpha*op(A)*op(B) + beta*C with packed or plain A, FP32 B and C.
  * Replaces the mm_dense_fn slots (dense_registries.hpp:32-34) and
  * dispatch::Fp32_16::mm_{pure_half,fast,accurate,workspace}
- * (fp32_16_registries.cpp:370-453): tier DEFAULT/FAST/EXACT as for GEMV.
- * M x N result, K inner dimension. */
+ * (fp32_16_registries.cpp:370-453).  M x N result, K inner dimension.
+ * Packed A (FP32_16, FP32_BF16):
+ *   DEFAULT      16-bit matrix cores (the reference's tensor-core GEMMs on
+ *                wh, mm_cuda_kernels.cu:35372, :37826): A's hi word as the
+ *                16-bit tiers decode it (FP32_BF16: bf16; FP32_16:
+ *                fp32_16_half_only, fp16), B rounded to nearest-even
+ *                bf16 / fp16, FP32 accumulate;
+ *   FAST, EXACT  as for GEMV (FP64 accumulate; FAST: hi word, EXACT: wh+wr);
+ *   MATERIALIZE  A_hi B_hi + A_hi B_lo + A_lo B_hi of the exact values on
+ *                16-bit matrix cores, FP32 accumulate.
+ * DEFAULT and MATERIALIZE stage both operands as K-contiguous 16-bit planes
+ * in the library-owned scratch buffer (grown with hipMalloc on demand; not
+ * thread-safe).  F32 / F64 A: DEFAULT accumulates in FP32, FAST/EXACT FP64. */
 int tomos_gemm(int a_type, int tier, int trans_a, int trans_b, double alpha, int64_t M,
                int64_t N, int64_t K, const void* a, const uint16_t* wr, int64_t lda,
                const float* b, int64_t ldb, double beta, float* c, int64_t ldc,

@@ -61,6 +61,17 @@ def test_adjoint_w_unpreconditioned_f32(capi):
     assert np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref) < 1e-7
 
 
+def test_grad_kernel_no_columns(capi):
+    """m > 0, n == 0: x and w are empty, so grad_b = A w = 0 is written over
+    whatever grad_b held, and grad_A (m x 0) is untouched."""
+    m = 1000
+    gbd = H.dev(np.full(m, 7.0))
+    a = DeviceBuffer(m * 2)
+    C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, m, 0, a.ptr, a.ptr, m, gbd.ptr, None, None, gbd.ptr,
+           None, m, None, 0, None)
+    assert np.array_equal(gbd.to_numpy(np.float64, (m,)), np.zeros(m))
+
+
 @pytest.mark.parametrize("ldg_pad", [0, 1])
 def test_grad_kernel(capi, ldg_pad):
     m, n = 10007, 33

@@ -29,9 +29,11 @@ def test_fp32_16_gemm_vs_oracle(capi, orc, ta, tb, tier, dec):
     got = run_gemm(C.FP32_16, tier, ta, tb, 1.5, M, N, K, wh, wr, Ash[0], B, 0.7, C0)
     eh, er = orc.pack(0, A)
     exp = C0.copy(order="F")
+    # DEFAULT runs on fp16 matrix cores: B is rounded to fp16 first
+    Bo = np.asfortranarray(B.astype(np.float16).astype(np.float32)) if tier == 0 else B
     orc.lib().orc_mm_packed(dec, ta, tb, 1.5, M, N, K,
                             eh.ctypes.data_as(orc.u16p), er.ctypes.data_as(orc.u16p), Ash[0],
-                            B.ctypes.data_as(orc.f32p), B.shape[0], 0.7,
+                            Bo.ctypes.data_as(orc.f32p), B.shape[0], 0.7,
                             exp.ctypes.data_as(orc.f32p), M)
     Ad = orc.unpack(dec, eh, er).astype(np.float64)
     Aop = Ad.T if ta else Ad

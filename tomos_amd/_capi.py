@@ -29,6 +29,7 @@ PACK_FP32_BF16, PACK_FP32_BF16_ROUNDED = 3, 4
  DECODE_FP32_BF16_EXACT, DECODE_FP32_BF16_HI, DECODE_FP32_BF16_EXACT_ROUNDED,
  DECODE_FP32_16_EXACT_TRUNCATED) = range(8)
 TIER_DEFAULT, TIER_FAST, TIER_EXACT = 0, 1, 2
+TIER_MATERIALIZE = 3  # GEMM only: the materialize slot as a 3-term split on matrix cores
 INDEX_INT32, INDEX_INT64 = 0, 1
 ERR_NAMES = {1: "INVALID_ARGUMENT", 2: "HIP", 3: "UNSUPPORTED", 4: "WORKSPACE",
              5: "NOT_CONVERGED", 6: "SINGULAR", 7: "INTERNAL"}

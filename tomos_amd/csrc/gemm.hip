@@ -2,13 +2,15 @@
 // packed (decoded while staging the A tile into LDS).
 //
 // Tiers follow dispatch::Fp32_16's mm slots (fp32_16_registries.cpp:370-453):
-// DEFAULT decodes wh only and accumulates in FP32 (pure_half / hi-only),
-// FAST decodes wh only and accumulates in FP64, EXACT rebuilds FP32 from
-// wh+wr and accumulates in FP64 ("accurate").
+// DEFAULT decodes wh only and accumulates in FP32 (pure_half / hi-only; on
+// matrix cores for packed A), FAST decodes wh only and accumulates in FP64,
+// EXACT rebuilds FP32 from wh+wr and accumulates in FP64 ("accurate"),
+// MATERIALIZE (packed A only) is the 3-term split on matrix cores.
 //
-// This is the SIMT tile kernel: 64x64 C tile per 256-thread block, 4x4
-// outputs per thread, K staged through LDS in slices of 16.  The
-// split-precision MFMA kernel lives in gemm_split.hip.
+// This file holds the SIMT tile kernel (64x64 C tile per 256-thread block,
+// 4x4 outputs per thread, K staged through LDS in slices of 16) that runs
+// the FP64-accumulating tiers and plain F32/F64 A.  Packed A's DEFAULT and
+// MATERIALIZE tiers run on matrix cores (gemm_split.hip).
 #include <algorithm>
 
 #include "codec.hpp"
@@ -118,12 +120,20 @@ void launch(int64_t M, int64_t N, int64_t K, const void* a, const uint16_t* wr, 
 
 }  // namespace
 
+// packed-A DEFAULT and MATERIALIZE tiers on matrix cores (gemm_split.hip)
+void gemm_packed_mfma(int a_type, int tier, bool ta, bool tb, double alpha, int64_t M, int64_t N, int64_t K,
+                      const uint16_t* wh, const uint16_t* wr, int64_t lda, const float* b, int64_t ldb,
+                      double beta, float* c, int64_t ldc, hipStream_t s);
+
 void gemm(int a_type, int tier, int trans_a, int trans_b, double alpha, int64_t M, int64_t N,
           int64_t K, const void* a, const uint16_t* wr, int64_t lda, const float* b, int64_t ldb,
           double beta, float* c, int64_t ldc, hipStream_t s) {
     require(M >= 0 && N >= 0 && K >= 0, "gemm: negative size");
-    require(tier == TOMOS_TIER_DEFAULT || tier == TOMOS_TIER_FAST || tier == TOMOS_TIER_EXACT,
+    require(tier == TOMOS_TIER_DEFAULT || tier == TOMOS_TIER_FAST || tier == TOMOS_TIER_EXACT ||
+                tier == TOMOS_TIER_MATERIALIZE,
             "gemm: bad tier");
+    const bool packed = a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16;
+    require(tier != TOMOS_TIER_MATERIALIZE || packed, "gemm: MATERIALIZE needs a packed A");
     const bool ta = trans_a == TOMOS_TRANSPOSE, tb = trans_b == TOMOS_TRANSPOSE;
     require(lda >= std::max<int64_t>(ta ? K : M, 1), "gemm: lda too small");
     require(ldb >= std::max<int64_t>(tb ? N : K, 1), "gemm: ldb too small");
@@ -132,6 +142,12 @@ void gemm(int a_type, int tier, int trans_a, int trans_b, double alpha, int64_t 
     if (M == 0 || N == 0) return;
     require(c != nullptr, "gemm: null C");
     if (K > 0) require(a && b, "gemm: null A or B");
+    if (packed && (tier == TOMOS_TIER_DEFAULT || tier == TOMOS_TIER_MATERIALIZE)) {
+        require(tier == TOMOS_TIER_DEFAULT || wr || K == 0, "gemm: MATERIALIZE needs wr");
+        gemm_packed_mfma(a_type, tier, ta, tb, alpha, M, N, K, static_cast<const uint16_t*>(a), wr, lda, b, ldb,
+                         beta, c, ldc, s);
+        return;
+    }
     const bool exact = tier == TOMOS_TIER_EXACT;
     const bool acc64 = tier != TOMOS_TIER_DEFAULT || a_type == TOMOS_F64;
     switch (a_type) {

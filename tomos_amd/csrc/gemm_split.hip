@@ -1,21 +1,27 @@
-// Split-precision packed GEMM on CDNA4 matrix cores (gfx950):
-//   C = A B  ~=  A_hi B_hi + A_hi B_lo + A_lo B_hi,   FP32 accumulate.
-// (No reference counterpart: the reference's 16-bit packed GEMMs use the hi
-// word only, mm_cuda_kernels.cu:35372 / :37826.)
+// Packed GEMMs on CDNA4 matrix cores (gfx950), FP32 accumulate:
+//   hi-only   C = A_hi B_hi                      (tomos_gemm, tier DEFAULT)
+//   split     C ~= A_hi B_hi + A_hi B_lo + A_lo B_hi
+//             (tomos_gemm_split with both operands packed; tomos_gemm, tier
+//             MATERIALIZE, with FP32 B).
+// The hi-only form is the reference's 16-bit tensor-core GEMM on wh
+// (mm_fp32_bf16_pure_bf16_gemm_direct_fastk_precomp2_serp,
+// mm_cuda_kernels.cu:35372; mm_fp32_16_pure_half_wide_bigreg_sq_*, :37826);
+// the split has no reference counterpart.
 //
 // Two steps:
-//  1. split: each packed operand (FP32_16 or FP32_BF16 planes, column-major)
-//     is decoded exactly to FP32 and written as hi/lo 16-bit planes in a
-//     K-contiguous layout (A as [M][K], B as [N][K]) so every MFMA operand is
-//     one 16-byte load:  hi = rn16(a), lo = rn16(a - hi).  FP32_BF16 uses
-//     bf16 planes; FP32_16 uses fp16 planes with a power-of-two scale 2^s
-//     per operand, chosen from the operand's largest magnitude, so the hi and
-//     lo terms stay clear of fp16 overflow and underflow (C is rescaled by
-//     2^-(sA+sB) in the epilogue).
+//  1. stage: each operand is written as 16-bit planes in a K-contiguous
+//     layout (A as [M][K], B as [N][K]) so every MFMA operand is one 16-byte
+//     load.  Hi-only: A's hi word as the 16-bit tiers decode it (FP32_BF16:
+//     bf16 as stored; FP32_16: fp32_16_half_only, fp16) and FP32 B rounded
+//     to nearest-even bf16 / fp16.  Split: the exact value a, hi = rn16(a),
+//     lo = rn16(a - hi); FP32_16 uses fp16 planes with a power-of-two scale
+//     2^s per operand, chosen from the operand's largest magnitude, so the hi
+//     and lo terms stay clear of fp16 overflow and underflow (C is rescaled
+//     by 2^-(sA+sB) in the epilogue); FP32_BF16 uses bf16 planes.
 //  2. gemm: 512 threads, 256 x 256 C tile, K slice of 32 staged through LDS
-//     (hi and lo of both operands, rows padded against bank conflicts);
-//     each wave owns 128 x 64 = 4 x 2 MFMA 32x32x16 tiles and issues 3
-//     MFMAs per tile per 16-wide k step into one FP32 accumulator.
+//     (rows padded against bank conflicts); each wave owns 128 x 64 = 4 x 2
+//     MFMA 32x32x16 tiles and issues 1 (hi-only) or 3 (split) MFMAs per
+//     tile per 16-wide k step into one FP32 accumulator.
 #include <algorithm>
 
 #include "codec.hpp"
@@ -34,46 +40,72 @@ constexpr int BM = 256, BN = 256, BK = 32;
 constexpr int LDK = BK + 8;  // padded K stride (elements) of an LDS row
 constexpr int kGemmThreads = 512;                     // 8 waves: 2 (M) x 4 (N)
 constexpr int kPlaneElems = BM * LDK;                 // one plane's K slice in LDS (BM == BN)
-constexpr int kGemmLds = 4 * kPlaneElems * 2;          // bytes: hi/lo of A and B, 80 KiB
 
-// ------------------------------------------------------------------ split
+// ---------------------------------------------------------------- operands
+// A GEMM operand as the prep kernels read it: X column-major with leading
+// dimension ld, either packed planes (wh/wr of FP32_16 or FP32_BF16) or a
+// plain FP32 array (f != nullptr).
+struct Src {
+    int fmt;                 // TOMOS_FP32_16 | TOMOS_FP32_BF16 (packed); ignored for FP32
+    const uint16_t* wh;
+    const uint16_t* wr;
+    const float* f;
+    int64_t ld;
+};
+
 __device__ __forceinline__ float decode_exact(int fmt, uint32_t h, uint32_t r) {
     return fmt == TOMOS_FP32_16 ? codec::fp32_16_decode_exact(h, r) : __uint_as_float((h << 16) | r);
+}
+
+// exact value of element idx (split mode)
+__device__ __forceinline__ float src_value(const Src& s, int64_t idx) {
+    return s.f ? s.f[idx] : decode_exact(s.fmt, s.wh[idx], s.wr[idx]);
 }
 
 __device__ __forceinline__ uint16_t bf16_rn(float v) {
     return static_cast<uint16_t>(codec::f32_to_bf16_bits(v));
 }
+__device__ __forceinline__ uint16_t f16_rn(float v) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+}
 __device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
 
-// max |a| over a packed column-major operand (exact decode), for the fp16 scale
-__global__ __launch_bounds__(256) void absmax_kernel(int fmt, int64_t rows, int64_t cols,
-                                                     const uint16_t* __restrict__ wh,
-                                                     const uint16_t* __restrict__ wr, int64_t ld,
+// 16-bit MFMA operand bits of element idx (hi-only mode): the packed hi word
+// decoded as the reference's 16-bit tiers read it (FP32_BF16: wh is the
+// bf16 value, fp32_bf16_bf16_only; FP32_16: fp32_16_half_only), or an FP32
+// value rounded to nearest-even bf16 / fp16.
+__device__ __forceinline__ uint16_t src_hi_bits(const Src& s, int64_t idx, bool f16) {
+    if (s.f) return f16 ? f16_rn(s.f[idx]) : bf16_rn(s.f[idx]);
+    const uint32_t h = s.wh[idx];
+    return static_cast<uint16_t>(s.fmt == TOMOS_FP32_16 ? codec::fp32_16_half_bits(h) : h);
+}
+
+// max |x| over an operand (exact values), for the fp16 split scale
+__global__ __launch_bounds__(256) void absmax_kernel(Src s, int64_t rows, int64_t cols,
                                                      unsigned* __restrict__ out_bits) {
     float mx = 0.0f;
     const int64_t total = rows * cols;
     for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < total;
          t += static_cast<int64_t>(gridDim.x) * 256) {
         const int64_t j = t / rows, i = t - j * rows;
-        const float v = fabsf(decode_exact(fmt, wh[i + j * ld], wr[i + j * ld]));
+        const float v = fabsf(src_value(s, i + j * s.ld));
         if (v <= 3.0e38f) mx = fmaxf(mx, v);  // ignore Inf/NaN
     }
     for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(mx));
 }
 
-__device__ __forceinline__ int fp16_scale_exp(int fmt, const unsigned* absmax_bits) {
+__device__ __forceinline__ int fp16_scale_exp(bool f16, const unsigned* absmax_bits) {
     // power-of-two scale bringing max |x| to about 2^13 (fp16 max is 65504)
-    if (fmt != TOMOS_FP32_16) return 0;
+    if (!f16) return 0;
     const float mx = __uint_as_float(*absmax_bits);
     int e = 0;
     frexpf(mx > 0.0f ? mx : 1.0f, &e);
     return 13 - e;
 }
 
-__device__ __forceinline__ void split16(int fmt, float v, int sc, uint16_t& h16, uint16_t& l16) {
-    if (fmt == TOMOS_FP32_16) {
+__device__ __forceinline__ void split16(bool f16, float v, int sc, uint16_t& h16, uint16_t& l16) {
+    if (f16) {
         const float x = ldexpf(v, sc);
         const _Float16 h = static_cast<_Float16>(x);
         const _Float16 l = static_cast<_Float16>(x - static_cast<float>(h));
@@ -85,122 +117,138 @@ __device__ __forceinline__ void split16(int fmt, float v, int sc, uint16_t& h16,
     }
 }
 
-// B operand (K x N column-major, K contiguous already): out[n][k], n < Np,
-// k < Kp, zero outside the operand.  hi/lo are bf16 (FP32_BF16) or fp16 of
-// x * 2^scale (FP32_16).
-__global__ __launch_bounds__(256) void split_planes_kernel(
-    int fmt, int64_t rows, int64_t cols, const uint16_t* __restrict__ wh,
-    const uint16_t* __restrict__ wr, int64_t ld, int64_t out_ld, uint16_t* __restrict__ hi,
-    uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits, int64_t pad_cols,
-    int64_t pad_k) {
-    const int sc = fp16_scale_exp(fmt, absmax_bits);
-    const int64_t total = pad_cols * pad_k;
+enum { PREP_HI = 1, PREP_SPLIT = 3 };
+
+// hi (and, for PREP_SPLIT, lo) 16-bit bits of element idx
+template <int MODE>
+__device__ __forceinline__ void prep_elem(const Src& s, int64_t idx, bool valid, bool f16, int sc,
+                                          uint16_t& h16, uint16_t& l16) {
+    if constexpr (MODE == PREP_HI) {
+        h16 = valid ? src_hi_bits(s, idx, f16) : uint16_t(0);
+        l16 = 0;
+    } else {
+        split16(f16, valid ? src_value(s, idx) : 0.0f, sc, h16, l16);
+    }
+}
+
+// K-contiguous operand (X is K x R column-major): out[r][k] = X(k, r) for
+// r < pad_r, k < pad_k, zero outside X.  hi (and lo) planes have row
+// stride out_ld.
+template <int MODE>
+__global__ __launch_bounds__(256) void prep_k_kernel(Src s, int64_t rows, int64_t cols, int64_t out_ld,
+                                                     uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                                     const unsigned* __restrict__ absmax_bits, bool f16,
+                                                     int64_t pad_r, int64_t pad_k) {
+    const int sc = MODE == PREP_SPLIT ? fp16_scale_exp(f16, absmax_bits) : 0;
+    const int64_t total = pad_r * pad_k;
     for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < total;
          t += static_cast<int64_t>(gridDim.x) * 256) {
-        const int64_t r = t / pad_k, k = t - r * pad_k;  // out[r][k] = X(k, r)
-        float v = 0.0f;
-        if (k < rows && r < cols) v = decode_exact(fmt, wh[k + r * ld], wr[k + r * ld]);
+        const int64_t r = t / pad_k, k = t - r * pad_k;
         uint16_t h16, l16;
-        split16(fmt, v, sc, h16, l16);
+        prep_elem<MODE>(s, k + r * s.ld, k < rows && r < cols, f16, sc, h16, l16);
         hi[r * out_ld + k] = h16;
-        lo[r * out_ld + k] = l16;
+        if constexpr (MODE == PREP_SPLIT) lo[r * out_ld + k] = l16;
     }
 }
 
-// Vector forms of absmax_kernel and split_planes_kernel for operands whose
-// columns are whole 16-byte runs (rows % 8 == 0, ld % 8 == 0, 16-byte
-// aligned planes): 8 elements per thread, 16-byte loads and stores.
-__device__ __forceinline__ void decode8(int fmt, const u32x4& h, const u32x4& r, float* v) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        v[2 * q] = decode_exact(fmt, h[q] & 0xFFFFu, r[q] & 0xFFFFu);
-        v[2 * q + 1] = decode_exact(fmt, h[q] >> 16, r[q] >> 16);
-    }
-}
-
-__global__ __launch_bounds__(256) void absmax_vec_kernel(int fmt, int64_t rows, int64_t cols,
-                                                         const uint16_t* __restrict__ wh,
-                                                         const uint16_t* __restrict__ wr, int64_t ld,
-                                                         unsigned* __restrict__ out_bits) {
-    float mx = 0.0f;
-    const int64_t cpc = rows / 8, total = cpc * cols;
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < total;
-         t += static_cast<int64_t>(gridDim.x) * 256) {
-        const int64_t j = t / cpc, i = (t - j * cpc) * 8;
-        const u32x4 h = *reinterpret_cast<const u32x4*>(wh + i + j * ld);
-        const u32x4 r = *reinterpret_cast<const u32x4*>(wr + i + j * ld);
-        float v[8];
-        decode8(fmt, h, r, v);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float a = fabsf(v[e]);
-            if (a <= 3.0e38f) mx = fmaxf(mx, a);  // ignore Inf/NaN
-        }
-    }
-    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(mx));
-}
-
-__global__ __launch_bounds__(256) void split_planes_vec_kernel(
-    int fmt, int64_t rows, int64_t cols, const uint16_t* __restrict__ wh,
-    const uint16_t* __restrict__ wr, int64_t ld, int64_t out_ld, uint16_t* __restrict__ hi,
-    uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits, int64_t pad_cols,
-    int64_t pad_k) {
-    const int sc = fp16_scale_exp(fmt, absmax_bits);
-    const int64_t cpr = pad_k / 8, total = pad_cols * cpr;
+// Vector form of prep_k_kernel for operands whose columns are whole 16-byte
+// runs of 16-bit words (rows % 8 == 0, ld % 8 == 0, 16-byte aligned planes)
+// or 32-byte runs of FP32: 8 elements per thread, 16-byte stores.
+template <int MODE>
+__global__ __launch_bounds__(256) void prep_k_vec_kernel(Src s, int64_t rows, int64_t cols, int64_t out_ld,
+                                                         uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                                         const unsigned* __restrict__ absmax_bits, bool f16,
+                                                         int64_t pad_r, int64_t pad_k) {
+    const int sc = MODE == PREP_SPLIT ? fp16_scale_exp(f16, absmax_bits) : 0;
+    const int64_t cpr = pad_k / 8, total = pad_r * cpr;
     for (int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; t < total;
          t += static_cast<int64_t>(gridDim.x) * 256) {
         const int64_t r = t / cpr, k = (t - r * cpr) * 8;  // out[r][k..k+7] = X(k..k+7, r)
-        float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (k < rows && r < cols)
-            decode8(fmt, *reinterpret_cast<const u32x4*>(wh + k + r * ld),
-                    *reinterpret_cast<const u32x4*>(wr + k + r * ld), v);
+        const bool valid = k < rows && r < cols;
+        const int64_t base = k + r * s.ld;
+        uint16_t h[8], l[8];
+        if (s.f) {
+            float v[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (valid) {
+                const float4 v0 = *reinterpret_cast<const float4*>(s.f + base);
+                const float4 v1 = *reinterpret_cast<const float4*>(s.f + base + 4);
+                v[0] = v0.x; v[1] = v0.y; v[2] = v0.z; v[3] = v0.w;
+                v[4] = v1.x; v[5] = v1.y; v[6] = v1.z; v[7] = v1.w;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if constexpr (MODE == PREP_HI) {
+                    h[e] = f16 ? f16_rn(v[e]) : bf16_rn(v[e]);
+                    l[e] = 0;
+                } else {
+                    split16(f16, v[e], sc, h[e], l[e]);
+                }
+            }
+        } else {
+            u32x4 wh8 = {0u, 0u, 0u, 0u}, wr8 = {0u, 0u, 0u, 0u};
+            if (valid) {
+                wh8 = *reinterpret_cast<const u32x4*>(s.wh + base);
+                if constexpr (MODE == PREP_SPLIT) wr8 = *reinterpret_cast<const u32x4*>(s.wr + base);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const uint32_t hw = b ? (wh8[q] >> 16) : (wh8[q] & 0xFFFFu);
+                    const int e = 2 * q + b;
+                    if constexpr (MODE == PREP_HI) {
+                        h[e] = valid ? static_cast<uint16_t>(s.fmt == TOMOS_FP32_16 ? codec::fp32_16_half_bits(hw)
+                                                                                    : hw)
+                                     : uint16_t(0);
+                        l[e] = 0;
+                    } else {
+                        const uint32_t rw = b ? (wr8[q] >> 16) : (wr8[q] & 0xFFFFu);
+                        split16(f16, valid ? decode_exact(s.fmt, hw, rw) : 0.0f, sc, h[e], l[e]);
+                    }
+                }
+        }
         u32x4 h8, l8;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            uint16_t h0, l0, h1, l1;
-            split16(fmt, v[2 * q], sc, h0, l0);
-            split16(fmt, v[2 * q + 1], sc, h1, l1);
-            h8[q] = static_cast<uint32_t>(h0) | (static_cast<uint32_t>(h1) << 16);
-            l8[q] = static_cast<uint32_t>(l0) | (static_cast<uint32_t>(l1) << 16);
+            h8[q] = static_cast<uint32_t>(h[2 * q]) | (static_cast<uint32_t>(h[2 * q + 1]) << 16);
+            l8[q] = static_cast<uint32_t>(l[2 * q]) | (static_cast<uint32_t>(l[2 * q + 1]) << 16);
         }
         *reinterpret_cast<u32x4*>(hi + r * out_ld + k) = h8;
-        *reinterpret_cast<u32x4*>(lo + r * out_ld + k) = l8;
+        if constexpr (MODE == PREP_SPLIT) *reinterpret_cast<u32x4*>(lo + r * out_ld + k) = l8;
     }
 }
 
-// A operand (M x K column-major, M contiguous): out[m][k] needs a transpose,
-// done per 64 x 64 tile through LDS so both the reads (m fastest) and the
-// writes (k fastest) are contiguous.  grid = (ceil(pad_k/64), ceil(pad_m/64)).
-__global__ __launch_bounds__(256) void split_planes_t_kernel(
-    int fmt, int64_t rows, int64_t cols, const uint16_t* __restrict__ wh,
-    const uint16_t* __restrict__ wr, int64_t ld, int64_t out_ld, uint16_t* __restrict__ hi,
-    uint16_t* __restrict__ lo, const unsigned* __restrict__ absmax_bits, int64_t pad_m,
-    int64_t pad_k) {
+// R-contiguous operand (X is R x K column-major): out[r][k] = X(r, k) needs
+// a transpose, done per 64 x 64 tile through LDS so both the reads (r
+// fastest) and the writes (k fastest) are contiguous.
+// grid = (ceil(pad_k/64), ceil(pad_r/64)).
+template <int MODE>
+__global__ __launch_bounds__(256) void prep_t_kernel(Src s, int64_t rows, int64_t cols, int64_t out_ld,
+                                                     uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                                     const unsigned* __restrict__ absmax_bits, bool f16,
+                                                     int64_t pad_r, int64_t pad_k) {
     __shared__ uint16_t th[64][65];
     __shared__ uint16_t tl[64][65];
-    const int sc = fp16_scale_exp(fmt, absmax_bits);
-    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64, m0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int sc = MODE == PREP_SPLIT ? fp16_scale_exp(f16, absmax_bits) : 0;
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * 64, r0 = static_cast<int64_t>(blockIdx.y) * 64;
     const int tid = threadIdx.x;
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
-        const int idx = tid + it * 256, i = idx & 63, j = idx >> 6;  // i: m, j: k
-        const int64_t m = m0 + i, k = k0 + j;
-        float v = 0.0f;
-        if (m < rows && k < cols) v = decode_exact(fmt, wh[m + k * ld], wr[m + k * ld]);
+        const int idx = tid + it * 256, i = idx & 63, j = idx >> 6;  // i: r, j: k
+        const int64_t r = r0 + i, k = k0 + j;
         uint16_t h16, l16;
-        split16(fmt, v, sc, h16, l16);
+        prep_elem<MODE>(s, r + k * s.ld, r < rows && k < cols, f16, sc, h16, l16);
         th[j][i] = h16;
-        tl[j][i] = l16;
+        if constexpr (MODE == PREP_SPLIT) tl[j][i] = l16;
     }
     __syncthreads();
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
-        const int idx = tid + it * 256, j = idx & 63, i = idx >> 6;  // j: k, i: m
-        const int64_t m = m0 + i, k = k0 + j;
-        if (m < pad_m && k < pad_k) {
-            hi[m * out_ld + k] = th[j][i];
-            lo[m * out_ld + k] = tl[j][i];
+        const int idx = tid + it * 256, j = idx & 63, i = idx >> 6;  // j: k, i: r
+        const int64_t r = r0 + i, k = k0 + j;
+        if (r < pad_r && k < pad_k) {
+            hi[r * out_ld + k] = th[j][i];
+            if constexpr (MODE == PREP_SPLIT) lo[r * out_ld + k] = tl[j][i];
         }
     }
 }
@@ -242,8 +290,8 @@ __device__ __forceinline__ void tile_of(int64_t tiles_m, int64_t tiles_n, int64_
 // planes is staged through LDS (rows padded to 80 bytes: conflict-free
 // 16-byte fragment reads); the next slice is loaded into registers while
 // the current one is multiplied.
-template <bool F16, int NT>
-__global__ __launch_bounds__(NT) void gemm_split_kernel(
+template <bool F16, int NT, int TERMS>
+__global__ __launch_bounds__(NT) void gemm_mfma_kernel(
     int64_t M, int64_t N, int64_t Kp, const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
     const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl, int64_t ldk, double alpha,
     double beta, float* __restrict__ C, int64_t ldc, const unsigned* __restrict__ absA,
@@ -251,11 +299,12 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
     constexpr int WN = NT / 128;      // waves along N
     constexpr int TJ = 8 / WN;        // 32-wide n tiles per wave
     constexpr int CH = 1024 / NT;     // 16-byte chunks per plane slice per thread
+    static_assert(TERMS == 1 || TERMS == 3, "hi-only or 3-term split");
     extern __shared__ __attribute__((aligned(16))) uint16_t gemm_lds[];
     uint16_t* sAh = gemm_lds;
-    uint16_t* sAl = sAh + kPlaneElems;
-    uint16_t* sBh = sAl + kPlaneElems;
-    uint16_t* sBl = sBh + kPlaneElems;
+    uint16_t* sBh = sAh + kPlaneElems;
+    uint16_t* sAl = sBh + kPlaneElems;  // TERMS == 3 only
+    uint16_t* sBl = sAl + kPlaneElems;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -286,17 +335,21 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
         pah[q] = *reinterpret_cast<const u32x4*>(Ah + oa[q]);
-        pal[q] = *reinterpret_cast<const u32x4*>(Al + oa[q]);
         pbh[q] = *reinterpret_cast<const u32x4*>(Bh + ob[q]);
-        pbl[q] = *reinterpret_cast<const u32x4*>(Bl + ob[q]);
+        if constexpr (TERMS == 3) {
+            pal[q] = *reinterpret_cast<const u32x4*>(Al + oa[q]);
+            pbl[q] = *reinterpret_cast<const u32x4*>(Bl + ob[q]);
+        }
     }
     for (int64_t k0 = 0; k0 < Kp; k0 += BK) {
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
             *reinterpret_cast<u32x4*>(&sAh[so[q]]) = pah[q];
-            *reinterpret_cast<u32x4*>(&sAl[so[q]]) = pal[q];
             *reinterpret_cast<u32x4*>(&sBh[so[q]]) = pbh[q];
-            *reinterpret_cast<u32x4*>(&sBl[so[q]]) = pbl[q];
+            if constexpr (TERMS == 3) {
+                *reinterpret_cast<u32x4*>(&sAl[so[q]]) = pal[q];
+                *reinterpret_cast<u32x4*>(&sBl[so[q]]) = pbl[q];
+            }
         }
         __syncthreads();
         if (k0 + BK < Kp) {
@@ -304,9 +357,11 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
                 pah[q] = *reinterpret_cast<const u32x4*>(Ah + oa[q] + k1);
-                pal[q] = *reinterpret_cast<const u32x4*>(Al + oa[q] + k1);
                 pbh[q] = *reinterpret_cast<const u32x4*>(Bh + ob[q] + k1);
-                pbl[q] = *reinterpret_cast<const u32x4*>(Bl + ob[q] + k1);
+                if constexpr (TERMS == 3) {
+                    pal[q] = *reinterpret_cast<const u32x4*>(Al + oa[q] + k1);
+                    pbl[q] = *reinterpret_cast<const u32x4*>(Bl + ob[q] + k1);
+                }
             }
         }
 #pragma unroll
@@ -317,20 +372,22 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
             for (int i = 0; i < 4; ++i) {
                 const int r = (wm * 128 + i * 32 + (lane & 31)) * LDK + kk;
                 ah[i] = *reinterpret_cast<const u32x4*>(&sAh[r]);
-                al[i] = *reinterpret_cast<const u32x4*>(&sAl[r]);
+                if constexpr (TERMS == 3) al[i] = *reinterpret_cast<const u32x4*>(&sAl[r]);
             }
 #pragma unroll
             for (int j = 0; j < TJ; ++j) {
                 const int r = (wn * (32 * TJ) + j * 32 + (lane & 31)) * LDK + kk;
                 bh[j] = *reinterpret_cast<const u32x4*>(&sBh[r]);
-                bl[j] = *reinterpret_cast<const u32x4*>(&sBl[r]);
+                if constexpr (TERMS == 3) bl[j] = *reinterpret_cast<const u32x4*>(&sBl[r]);
             }
 #pragma unroll
             for (int j = 0; j < TJ; ++j)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    mfma32<F16>(bh[j], al[i], acc[j][i]);
-                    mfma32<F16>(bl[j], ah[i], acc[j][i]);
+                    if constexpr (TERMS == 3) {
+                        mfma32<F16>(bh[j], al[i], acc[j][i]);
+                        mfma32<F16>(bl[j], ah[i], acc[j][i]);
+                    }
                     mfma32<F16>(bh[j], ah[i], acc[j][i]);
                 }
         }
@@ -339,7 +396,7 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
     // epilogue: acc[j][i][v] is C(row, col) with row = m tile i, lane % 32 and
     // col = n tile j, 8 (v / 4) + 4 (lane / 32) + v % 4
     double scale = alpha;
-    if (F16) {
+    if (F16 && absA) {  // fp16 split planes carry the scale 2^(13 - e) per operand
         int ea = 0, eb = 0;
         frexpf(fmaxf(__uint_as_float(*absA), 1e-38f), &ea);
         frexpf(fmaxf(__uint_as_float(*absB), 1e-38f), &eb);
@@ -364,12 +421,119 @@ __global__ __launch_bounds__(NT) void gemm_split_kernel(
     }
 }
 
+// Staged operand planes: absmax words, then A hi [Mp][Kp], B hi [Np][Kp]
+// and, for the split, A lo and B lo.
+struct Planes {
+    int64_t Mp, Np, Kp;
+    unsigned* absmax;
+    uint16_t *Ah, *Bh, *Al, *Bl;
+};
+
+Planes planes_of(int terms, int64_t M, int64_t N, int64_t K, void* base_ptr) {
+    Planes p;
+    p.Mp = ceil_div(M, BM) * BM;
+    p.Np = ceil_div(N, BN) * BN;
+    p.Kp = std::max<int64_t>(ceil_div(K, BK) * BK, BK);
+    char* base = static_cast<char*>(base_ptr);
+    p.absmax = reinterpret_cast<unsigned*>(base);
+    p.Ah = reinterpret_cast<uint16_t*>(base + 256);
+    p.Bh = p.Ah + p.Mp * p.Kp;
+    p.Al = terms == 3 ? p.Bh + p.Np * p.Kp : nullptr;
+    p.Bl = terms == 3 ? p.Al + p.Mp * p.Kp : nullptr;
+    return p;
+}
+
+// Stage one operand: X is rows x cols column-major; kcontig means the GEMM's
+// K runs down X's columns (X is K x R), otherwise along its rows (X is R x K).
+void stage(int mode, bool f16, const Src& x, bool kcontig, int64_t rows, int64_t cols, int64_t pad_r,
+           int64_t Kp, uint16_t* hi, uint16_t* lo, const unsigned* absmax, hipStream_t s) {
+    if (kcontig) {
+        const bool vec = rows % 8 == 0 && x.ld % 8 == 0 &&
+                         (x.f ? reinterpret_cast<uintptr_t>(x.f) % 16 == 0
+                              : reinterpret_cast<uintptr_t>(x.wh) % 16 == 0 &&
+                                    (mode == PREP_HI || reinterpret_cast<uintptr_t>(x.wr) % 16 == 0));
+        if (vec && rows > 0) {
+            const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(pad_r * Kp / 8, 256), 65536));
+            if (mode == PREP_HI)
+                prep_k_vec_kernel<PREP_HI><<<blocks, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r, Kp);
+            else
+                prep_k_vec_kernel<PREP_SPLIT><<<blocks, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r,
+                                                                     Kp);
+        } else {
+            const unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(pad_r * Kp, 256), 65536));
+            if (mode == PREP_HI)
+                prep_k_kernel<PREP_HI><<<blocks, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r, Kp);
+            else
+                prep_k_kernel<PREP_SPLIT><<<blocks, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r, Kp);
+        }
+    } else {
+        require(ceil_div(pad_r, 64) < 65536, "gemm: operand too large");
+        const dim3 grid(static_cast<unsigned>(ceil_div(Kp, 64)), static_cast<unsigned>(ceil_div(pad_r, 64)));
+        if (mode == PREP_HI)
+            prep_t_kernel<PREP_HI><<<grid, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r, Kp);
+        else
+            prep_t_kernel<PREP_SPLIT><<<grid, 256, 0, s>>>(x, rows, cols, Kp, hi, lo, absmax, f16, pad_r, Kp);
+    }
+    TOMOS_LAUNCH_CHECK("gemm operand staging");
+}
+
 }  // namespace
 
-size_t gemm_split_scratch_bytes(int64_t M, int64_t N, int64_t K) {
+size_t gemm_mfma_scratch_bytes(int terms, int64_t M, int64_t N, int64_t K) {
     const int64_t Mp = ceil_div(M, BM) * BM, Np = ceil_div(N, BN) * BN;
     const int64_t Kp = std::max<int64_t>(ceil_div(K, BK) * BK, BK);
-    return static_cast<size_t>(2 * (Mp + Np) * Kp) * 2 + 256;
+    return static_cast<size_t>((terms == 3 ? 2 : 1) * (Mp + Np) * Kp) * 2 + 256;
+}
+
+size_t gemm_split_scratch_bytes(int64_t M, int64_t N, int64_t K) { return gemm_mfma_scratch_bytes(3, M, N, K); }
+
+// C = alpha op(A) op(B) + beta C on 16-bit MFMA with FP32 accumulation:
+// terms == 1 multiplies the hi words (A's packed hi word as the 16-bit
+// tiers decode it, B rounded to nearest-even bf16/fp16), terms == 3 the
+// split A_hi B_hi + A_hi B_lo + A_lo B_hi of the exact values.  f16 picks
+// fp16 planes (FP32_16) over bf16 ones (FP32_BF16).  A is M x K (a_kcontig:
+// stored K x M, i.e. transposed), B is K x N (b_kcontig: stored K x N, the
+// plain layout; otherwise stored N x K).
+void gemm_mfma(int terms, bool f16, const Src& a, bool a_kcontig, const Src& b, bool b_kcontig, int64_t M,
+               int64_t N, int64_t K, double alpha, double beta, float* c, int64_t ldc, void* scratch,
+               size_t scratch_bytes, hipStream_t s) {
+    require(terms == 1 || terms == 3, "gemm_mfma: terms must be 1 or 3");
+    if (M == 0 || N == 0) return;
+    const size_t need = gemm_mfma_scratch_bytes(terms, M, N, K);
+    if (!scratch) scratch = fallback_scratch(need);
+    else require(scratch_bytes >= need, "gemm: scratch too small (need " + std::to_string(need) + " bytes)");
+    const Planes p = planes_of(terms, M, N, K, scratch);
+    const int mode = terms == 3 ? PREP_SPLIT : PREP_HI;
+    const bool scaled = terms == 3 && f16;
+    if (scaled) {
+        TOMOS_HIP_CHECK(hipMemsetAsync(p.absmax, 0, 8, s));
+        if (K > 0) {
+            absmax_kernel<<<2048, 256, 0, s>>>(a, a_kcontig ? K : M, a_kcontig ? M : K, p.absmax);
+            absmax_kernel<<<2048, 256, 0, s>>>(b, b_kcontig ? K : N, b_kcontig ? N : K, p.absmax + 1);
+            TOMOS_LAUNCH_CHECK("absmax_kernel");
+        }
+    }
+    stage(mode, f16, a, a_kcontig, a_kcontig ? K : M, a_kcontig ? M : K, p.Mp, p.Kp, p.Ah, p.Al, p.absmax, s);
+    stage(mode, f16, b, b_kcontig, b_kcontig ? K : N, b_kcontig ? N : K, p.Np, p.Kp, p.Bh, p.Bl, p.absmax + 1, s);
+    const int64_t nblocks = (p.Mp / BM) * (p.Np / BN);
+    require(nblocks < (int64_t(1) << 31), "gemm: grid too large");
+    const unsigned* absA = scaled ? p.absmax : nullptr;
+    const unsigned* absB = scaled ? p.absmax + 1 : nullptr;
+    const int lds = (terms == 3 ? 4 : 2) * kPlaneElems * 2;
+    auto launch = [&](auto* k) {
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        k<<<static_cast<unsigned>(nblocks), kGemmThreads, lds, s>>>(M, N, p.Kp, p.Ah, p.Al, p.Bh, p.Bl, p.Kp, alpha,
+                                                                    beta, c, ldc, absA, absB);
+    };
+    if (terms == 3) {
+        if (f16) launch(gemm_mfma_kernel<true, kGemmThreads, 3>);
+        else launch(gemm_mfma_kernel<false, kGemmThreads, 3>);
+    } else {
+        if (f16) launch(gemm_mfma_kernel<true, kGemmThreads, 1>);
+        else launch(gemm_mfma_kernel<false, kGemmThreads, 1>);
+    }
+    TOMOS_LAUNCH_CHECK("gemm_mfma_kernel");
 }
 
 void gemm_split(int a_type, int64_t M, int64_t N, int64_t K, double alpha, const uint16_t* awh,
@@ -381,52 +545,19 @@ void gemm_split(int a_type, int64_t M, int64_t N, int64_t K, double alpha, const
             "gemm_split: bad leading dimension");
     if (M == 0 || N == 0) return;
     require(c != nullptr && (K == 0 || (awh && awr && bwh && bwr)), "gemm_split: null pointer");
-    const int64_t Mp = ceil_div(M, BM) * BM, Np = ceil_div(N, BN) * BN, Kp = std::max<int64_t>(ceil_div(K, BK) * BK, BK);
-    const size_t need = gemm_split_scratch_bytes(M, N, K);
-    if (!scratch) scratch = fallback_scratch(need);
-    else require(scratch_bytes >= need, "gemm_split: scratch too small");
-    char* base = static_cast<char*>(scratch);
-    unsigned* absmax = reinterpret_cast<unsigned*>(base);
-    uint16_t* Ah = reinterpret_cast<uint16_t*>(base + 256);
-    uint16_t* Al = Ah + Mp * Kp;
-    uint16_t* Bh = Al + Mp * Kp;
-    uint16_t* Bl = Bh + Np * Kp;
-    TOMOS_HIP_CHECK(hipMemsetAsync(absmax, 0, 8, s));
-    const bool f16 = a_type == TOMOS_FP32_16;
-    // whole 16-byte column runs: the vector kernels apply
-    auto runs16 = [](int64_t rows, int64_t ld, const uint16_t* h, const uint16_t* r) {
-        return rows % 8 == 0 && ld % 8 == 0 && reinterpret_cast<uintptr_t>(h) % 16 == 0 &&
-               reinterpret_cast<uintptr_t>(r) % 16 == 0;
-    };
-    const bool vecA = runs16(M, lda, awh, awr), vecB = runs16(K, ldb, bwh, bwr);
-    if (f16 && K > 0) {
-        if (vecA) absmax_vec_kernel<<<2048, 256, 0, s>>>(a_type, M, K, awh, awr, lda, absmax);
-        else absmax_kernel<<<1024, 256, 0, s>>>(a_type, M, K, awh, awr, lda, absmax);
-        if (vecB) absmax_vec_kernel<<<2048, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, absmax + 1);
-        else absmax_kernel<<<1024, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, absmax + 1);
-        TOMOS_LAUNCH_CHECK("absmax_kernel");
-    }
-    require(ceil_div(Mp, 64) < 65536, "gemm_split: M too large");
-    const dim3 gridA(static_cast<unsigned>(ceil_div(Kp, 64)), static_cast<unsigned>(ceil_div(Mp, 64)));
-    const unsigned blocksB = static_cast<unsigned>(std::min<int64_t>(ceil_div(Np * Kp, 256), 65536));
-    split_planes_t_kernel<<<gridA, 256, 0, s>>>(a_type, M, K, awh, awr, lda, Kp, Ah, Al, absmax, Mp, Kp);
-    if (vecB && K > 0)
-        split_planes_vec_kernel<<<static_cast<unsigned>(std::min<int64_t>(ceil_div(Np * Kp / 8, 256), 65536)), 256,
-                                  0, s>>>(a_type, K, N, bwh, bwr, ldb, Kp, Bh, Bl, absmax + 1, Np, Kp);
-    else
-        split_planes_kernel<<<blocksB, 256, 0, s>>>(a_type, K, N, bwh, bwr, ldb, Kp, Bh, Bl, absmax + 1, Np, Kp);
-    TOMOS_LAUNCH_CHECK("split_planes_kernel");
-    const int64_t nblocks = (Mp / BM) * (Np / BN);
-    require(nblocks < (int64_t(1) << 31), "gemm_split: grid too large");
-    auto launch = [&](auto* k, int nt) {
-        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, kGemmLds));
-        k<<<static_cast<unsigned>(nblocks), nt, kGemmLds, s>>>(M, N, Kp, Ah, Al, Bh, Bl, Kp, alpha, beta, c,
-                                                             ldc, absmax, absmax + 1);
-    };
-    if (f16) launch(gemm_split_kernel<true, kGemmThreads>, kGemmThreads);
-    else launch(gemm_split_kernel<false, kGemmThreads>, kGemmThreads);
-    TOMOS_LAUNCH_CHECK("gemm_split_kernel");
+    const Src a{a_type, awh, awr, nullptr, lda}, b{a_type, bwh, bwr, nullptr, ldb};
+    gemm_mfma(3, a_type == TOMOS_FP32_16, a, false, b, true, M, N, K, alpha, beta, c, ldc, scratch, scratch_bytes,
+              s);
+}
+
+// tomos_gemm's packed-A tiers on matrix cores: DEFAULT (hi words) and
+// MATERIALIZE (3-term split of the exact values), FP32 B.
+void gemm_packed_mfma(int a_type, int tier, bool ta, bool tb, double alpha, int64_t M, int64_t N, int64_t K,
+                      const uint16_t* wh, const uint16_t* wr, int64_t lda, const float* b, int64_t ldb,
+                      double beta, float* c, int64_t ldc, hipStream_t s) {
+    const int terms = tier == TOMOS_TIER_MATERIALIZE ? 3 : 1;
+    const Src a{a_type, wh, wr, nullptr, lda}, bs{a_type, nullptr, nullptr, b, ldb};
+    gemm_mfma(terms, a_type == TOMOS_FP32_16, a, ta, bs, !tb, M, N, K, alpha, beta, c, ldc, nullptr, 0, s);
 }
 
 }  // namespace tomos_hip

@@ -92,20 +92,12 @@ void mm_impl(TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A, 
     gemm_call(TIER, ta, tb, alpha, A, B, beta, C);
 }
 
-// "workspace" tier: decode A exactly into an FP32 scratch matrix, then the
-// FP32 GEMM (fp32_16_registries.cpp:443-453 materialize tier).
+// "workspace" tier (fp32_16_registries.cpp:432-449 materialize tier: the
+// exact values of A through an FP32 GEMM): here the 3-term split of the
+// exact values on 16-bit matrix cores, FP32 accumulate.
 void mm_materialize(TransposeType ta, TransposeType tb, double alpha, tomos::Dense& A,
                     tomos::Dense& B, double beta, tomos::Dense& C) {
-    Context ctx(A.get_device(), A.stream());
-    const auto sa = A.get_size();
-    tomos::Dense F = tomos::Dense::create_strided(ctx, Layout::COL_MAJOR, NumericType::FP32, sa);
-    const int dec = A.get_value_type() == NumericType::FP32_16 ? TOMOS_DECODE_FP32_16_EXACT
-                                                               : TOMOS_DECODE_FP32_BF16_EXACT;
-    check(tomos_unpack(dec, static_cast<int64_t>(sa[0]), static_cast<int64_t>(sa[1]), A.wh(), A.wr(),
-                       static_cast<int64_t>(A.get_lead_dim()), static_cast<float*>(F.data()),
-                       static_cast<int64_t>(sa[0]), A.stream()),
-          "mm_apply (materialize)");
-    gemm_call(TOMOS_TIER_DEFAULT, ta, tb, alpha, F, B, beta, C);
+    gemm_call(TOMOS_TIER_MATERIALIZE, ta, tb, alpha, A, B, beta, C);
     check(tomos_stream_synchronize(A.stream()), "mm_apply (materialize)");
 }
 
