@@ -99,12 +99,20 @@ def lib():
                                    ctypes.POINTER(ctypes.c_int), f64p, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]
         L.orc_lsqr_f32.restype = ctypes.c_int
-        L.orc_lsqr_f32.argtypes = [i64, i64, f32p, i64, f32p, f32p, ctypes.c_double,
+        L.orc_lsqr_f32.argtypes = [i64, i64, f32p, i64, f32p, f32p, f32p, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int), f64p, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]
         L.orc_blas_load.restype = ctypes.c_int
         L.orc_blas_load.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.orc_blas_loaded.restype = ctypes.c_int
+        L.orc_blas_loaded.argtypes = []
+        L.orc_countsketch_apply_f32_omp.restype = None
+        L.orc_countsketch_apply_f32_omp.argtypes = [i64, i64, f32p, i64, i64, f64p, i64]
+        L.orc_sgemv.restype = None
+        L.orc_sgemv.argtypes = [ctypes.c_int, i64, i64, ctypes.c_float, f32p, i64, f32p, ctypes.c_float, f32p]
+        L.orc_stream_copy.restype = None
+        L.orc_stream_copy.argtypes = [i64, f64p, f64p]
         L.orc_exhaustive_split_combine.restype = ctypes.c_int64
         L.orc_exhaustive_split_combine.argtypes = [ctypes.c_int, ctypes.c_uint64,
                                                    ctypes.c_uint64,
@@ -252,7 +260,9 @@ def blas_load(threads):
     return lib().orc_blas_load(p.encode(), int(threads))
 
 
-def lsqr_f32(A, b, tol, max_iters, max_relres, restarts=1, history_cap=100000):
+def lsqr_f32(A, b, tol, max_iters, max_relres, restarts=1, history_cap=100000, Rinv=None):
+    """FP32 LSQR (OpenBLAS sgemv / BLAS-1 once blas_load() has bound them),
+    optionally right-preconditioned by the n x n FP32 matrix Rinv."""
     A = np.asfortranarray(A, dtype=np.float32)
     m, n = A.shape
     b = np.ascontiguousarray(b, np.float32)
@@ -260,7 +270,53 @@ def lsqr_f32(A, b, tol, max_iters, max_relres, restarts=1, history_cap=100000):
     hist = np.zeros(history_cap, np.float64)
     it = ctypes.c_int(0)
     hl = ctypes.c_int(0)
-    lib().orc_lsqr_f32(m, n, _p(A, f32p), max(m, 1), _p(b, f32p), _p(x, f32p), tol, max_iters,
+    Rp = None
+    if Rinv is not None:
+        Rinv = np.asfortranarray(Rinv, dtype=np.float32)
+        assert Rinv.shape == (n, n)
+        Rp = _p(Rinv, f32p)
+    lib().orc_lsqr_f32(m, n, _p(A, f32p), max(m, 1), _p(b, f32p), _p(x, f32p), Rp, tol, max_iters,
                        max_relres, restarts, ctypes.byref(it), _p(hist, f64p), history_cap,
                        ctypes.byref(hl))
     return x, it.value, hist[:min(hl.value, history_cap)].copy()
+
+
+def countsketch_apply_f32(A, s):
+    """Hashed CountSketch S A of an FP32 matrix, FP64 result (s x n), over
+    the host cores (OpenMP across columns)."""
+    A = np.asfortranarray(A, dtype=np.float32)
+    m, n = A.shape
+    Y = np.zeros((s, n), np.float64, order="F")
+    lib().orc_countsketch_apply_f32_omp(m, n, _p(A, f32p), max(m, 1), s, _p(Y, f64p), s)
+    return Y
+
+
+def sgemv_ms(trans, A, reps=3):
+    """Best time (ms) of y = A x (trans 0) or y = A^T x (trans 1) through
+    the bound cblas_sgemv, on an FP32 Fortran-ordered A."""
+    import time
+    A = np.asfortranarray(A, dtype=np.float32)
+    m, n = A.shape
+    x = np.ones(m if trans else n, np.float32)
+    y = np.zeros(n if trans else m, np.float32)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lib().orc_sgemv(int(trans), m, n, 1.0, _p(A, f32p), max(m, 1), _p(x, f32p), 0.0, _p(y, f32p))
+        best = min(best, time.perf_counter() - t0)
+    return best * 1e3
+
+
+def stream_copy_gbs(nbytes=1 << 31, reps=5):
+    """Host copy bandwidth (STREAM copy, OpenMP threads), GB/s counting the
+    read and the write; best of reps."""
+    import time
+    n = nbytes // 8
+    a = np.ones(n, np.float64)
+    d = np.empty(n, np.float64)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lib().orc_stream_copy(n, _p(a, f64p), _p(d, f64p))
+        best = min(best, time.perf_counter() - t0)
+    return 2.0 * 8 * n / best / 1e9

@@ -751,20 +751,28 @@ done:
 }
 
 /* ------------------------------------------------------------------------ */
-/* Plain-FP32 LSQR with OpenBLAS (the CPU baseline).                        */
-/* Follows the same lsqr.cpp control flow with value_t = float: vectors in   */
-/* float, BLAS-1 through cblas_s* (lsqr.cpp:194-319), matvecs through       */
-/* cblas_sgemv (src/kernels/cpu/matrix/matrix_cpu_kernels.cpp:44-54),       */
-/* scalars carried in double between steps.  The OpenBLAS shared object is  */
-/* the one numpy bundles (ILP64 symbols with the scipy_ prefix); its path   */
-/* is passed in.  Falls back to plain loops when no library is given.       */
+/* FP32 LSQR with OpenBLAS (the CPU baseline).                              */
+/* Follows the lsqr.cpp control flow with value_t = float: vectors in float, */
+/* BLAS-1 through cblas_snrm2/sscal/saxpy/scopy (lsqr.cpp:194-319), matvecs  */
+/* through cblas_sgemv (src/kernels/cpu/matrix/matrix_cpu_kernels.cpp:      */
+/* 44-54), scalars carried in double between steps; with Rinv (n x n, FP32) */
+/* the right preconditioner is applied by sgemv on Rinv (precond_apply,     */
+/* blas.cpp:88-96), as orc_lsqr_f64 does.  The OpenBLAS shared object is the */
+/* one numpy bundles (ILP64 symbols with the scipy_ prefix); its path is     */
+/* passed in.  Falls back to plain loops when no library is given.          */
 /* ------------------------------------------------------------------------ */
 typedef void (*sgemv_fn)(int, int, int64_t, int64_t, float, const float*, int64_t,
                          const float*, int64_t, float, float*, int64_t);
 typedef float (*snrm2_fn)(int64_t, const float*, int64_t);
+typedef void (*sscal_fn)(int64_t, float, float*, int64_t);
+typedef void (*saxpy_fn)(int64_t, float, const float*, int64_t, float*, int64_t);
+typedef void (*scopy_fn)(int64_t, const float*, int64_t, float*, int64_t);
 typedef void (*set_threads_fn)(int);
 static sgemv_fn p_sgemv = 0;
 static snrm2_fn p_snrm2 = 0;
+static sscal_fn p_sscal = 0;
+static saxpy_fn p_saxpy = 0;
+static scopy_fn p_scopy = 0;
 static set_threads_fn p_set_threads = 0;
 
 ORC_API int orc_blas_load(const char* path, int threads) {
@@ -772,11 +780,20 @@ ORC_API int orc_blas_load(const char* path, int threads) {
     if (!h) return -1;
     p_sgemv = (sgemv_fn)dlsym(h, "scipy_cblas_sgemv64_");
     p_snrm2 = (snrm2_fn)dlsym(h, "scipy_cblas_snrm264_");
+    p_sscal = (sscal_fn)dlsym(h, "scipy_cblas_sscal64_");
+    p_saxpy = (saxpy_fn)dlsym(h, "scipy_cblas_saxpy64_");
+    p_scopy = (scopy_fn)dlsym(h, "scipy_cblas_scopy64_");
     p_set_threads = (set_threads_fn)dlsym(h, "scipy_openblas_set_num_threads64_");
-    if (!p_sgemv || !p_snrm2) { p_sgemv = 0; p_snrm2 = 0; return -2; }
+    if (!p_sgemv || !p_snrm2 || !p_sscal || !p_saxpy || !p_scopy) {
+        p_sgemv = 0; p_snrm2 = 0; p_sscal = 0; p_saxpy = 0; p_scopy = 0;
+        return -2;
+    }
     if (p_set_threads && threads > 0) p_set_threads(threads);
     return 0;
 }
+
+/* 1 when the OpenBLAS routines are bound (orc_blas_load succeeded) */
+ORC_API int orc_blas_loaded(void) { return p_sgemv != 0; }
 
 static void sgemv(int trans, int64_t m, int64_t n, float alpha, const float* A, int64_t lda,
                   const float* x, float beta, float* y) {
@@ -801,15 +818,35 @@ static double snrm2(int64_t n, const float* x) {
     for (int64_t i = 0; i < n; ++i) s += (double)x[i] * (double)x[i];
     return sqrt(s);
 }
+static void sscal(int64_t n, float a, float* x) {
+    if (p_sscal) { p_sscal(n, a, x, 1); return; }
+    for (int64_t i = 0; i < n; ++i) x[i] *= a;
+}
+static void saxpy(int64_t n, float a, const float* x, float* y) {
+    if (p_saxpy) { p_saxpy(n, a, x, 1, y, 1); return; }
+    for (int64_t i = 0; i < n; ++i) y[i] += a * x[i];
+}
+static void scopy(int64_t n, const float* x, float* y) {
+    if (p_scopy) { p_scopy(n, x, 1, y, 1); return; }
+    memcpy(y, x, (size_t)n * sizeof(float));
+}
+
+/* y = alpha op(A) x + beta y through the bound cblas_sgemv (the CPU      */
+/* baseline's GEMV timing).                                                */
+ORC_API void orc_sgemv(int trans, int64_t m, int64_t n, float alpha, const float* A, int64_t lda,
+                       const float* x, float beta, float* y) {
+    sgemv(trans, m, n, alpha, A, lda, x, beta, y);
+}
 
 ORC_API int orc_lsqr_f32(int64_t m, int64_t n, const float* A, int64_t lda,
-                         const float* b, float* x, double tol, int max_iters,
+                         const float* b, float* x, const float* Rinv, double tol, int max_iters,
                          double max_relres, int restarts, int* iterations,
                          double* history, int history_cap, int* history_len) {
     float* u = (float*)calloc((size_t)m, sizeof(float));
     float* v = (float*)calloc((size_t)n, sizeof(float));
     float* w = (float*)calloc((size_t)n, sizeof(float));
     float* t = (float*)calloc((size_t)n, sizeof(float));
+    float* pv = (float*)calloc((size_t)n, sizeof(float));
     int completed = 0, hl = 0;
     double alpha = 0, beta = 0, rho_bar = 0, phi_bar = 0;
     for (int64_t j = 0; j < n; ++j) x[j] = 0.0f;
@@ -822,37 +859,54 @@ ORC_API int orc_lsqr_f32(int64_t m, int64_t n, const float* A, int64_t lda,
             hl++;
             if (rr < tol || rr < max_relres) goto done;
         }
-        memcpy(u, b, (size_t)m * sizeof(float));
+        scopy(m, b, u);
         sgemv(0, m, n, -1.0f, A, lda, x, 1.0f, u);
         beta = snrm2(m, u);
-        { float sc = (float)(1.0 / beta); for (int64_t i = 0; i < m; ++i) u[i] *= sc; }
-        sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, v);
+        sscal(m, (float)(1.0 / beta), u);
+        if (Rinv) {
+            sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, t);
+            sgemv(1, n, n, 1.0f, Rinv, n, t, 0.0f, v);
+        } else {
+            sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, v);
+        }
         alpha = snrm2(n, v);
-        { float sc = (float)(1.0 / alpha); for (int64_t j = 0; j < n; ++j) v[j] *= sc; }
-        memcpy(w, v, (size_t)n * sizeof(float));
+        sscal(n, (float)(1.0 / alpha), v);
+        scopy(n, v, w);
         rho_bar = alpha;
         phi_bar = beta;
         if (rhsnorm == 0.0) goto done;
         if (beta / rhsnorm < tol || beta / rhsnorm < max_relres) goto done;
         for (int it = 0; it < max_iters; ++it) {
-            memcpy(t, v, (size_t)n * sizeof(float));
+            if (Rinv) sgemv(0, n, n, 1.0f, Rinv, n, v, 0.0f, t);
+            else scopy(n, v, t);
             sgemv(0, m, n, 1.0f, A, lda, t, (float)(-alpha), u);
             beta = snrm2(m, u);
-            { float sc = (float)(1.0 / beta); for (int64_t i = 0; i < m; ++i) u[i] *= sc; }
-            sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, t);
-            { float sb = (float)(-beta); for (int64_t j = 0; j < n; ++j) t[j] += sb * v[j]; }
+            sscal(m, (float)(1.0 / beta), u);
+            if (Rinv) {
+                sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, pv);
+                sgemv(1, n, n, 1.0f, Rinv, n, pv, 0.0f, t);
+            } else {
+                sgemv(1, m, n, 1.0f, A, lda, u, 0.0f, t);
+            }
+            saxpy(n, (float)(-beta), v, t);
             alpha = snrm2(n, t);
-            { float sc = (float)(1.0 / alpha); for (int64_t j = 0; j < n; ++j) t[j] *= sc; }
-            memcpy(v, t, (size_t)n * sizeof(float));
+            sscal(n, (float)(1.0 / alpha), t);
+            scopy(n, t, v);
             double rho = hypot(rho_bar, beta);
             double c = rho_bar / rho, s = beta / rho;
             double theta = s * alpha;
             rho_bar = -c * alpha;
             double phi = c * phi_bar;
             phi_bar = s * phi_bar;
-            { float f = (float)(phi / rho); for (int64_t j = 0; j < n; ++j) x[j] += f * w[j]; }
-            { float f = (float)(-theta / rho);
-              for (int64_t j = 0; j < n; ++j) w[j] = v[j] + f * w[j]; }
+            if (Rinv) {
+                sgemv(0, n, n, 1.0f, Rinv, n, w, 0.0f, pv);
+                saxpy(n, (float)(phi / rho), pv, x);
+            } else {
+                saxpy(n, (float)(phi / rho), w, x);
+            }
+            /* w = v - (theta / rho) w */
+            sscal(n, (float)(-theta / rho), w);
+            saxpy(n, 1.0f, v, w);
             completed++;
             if (rhsnorm == 0.0) goto done;
             double rr = fabs(phi_bar) / rhsnorm;
@@ -864,8 +918,34 @@ ORC_API int orc_lsqr_f32(int64_t m, int64_t n, const float* A, int64_t lda,
 done:
     if (iterations) *iterations = completed;
     if (history_len) *history_len = hl;
-    free(u); free(v); free(w); free(t);
+    free(u); free(v); free(w); free(t); free(pv);
     return 0;
+}
+
+/* Hashed CountSketch of an FP32 matrix on the host cores (the CPU baseline's */
+/* preconditioner build): Y[h(i), j] += s(i) * X[i, j] in FP64, the same     */
+/* hash and sign as orc_countsketch_apply_f64 (countsketch_kernels.cpp:      */
+/* 59-85); columns are independent, so they are split over OpenMP threads.  */
+ORC_API void orc_countsketch_apply_f32_omp(int64_t m, int64_t n, const float* X, int64_t ldx,
+                                           int64_t s, double* Y, int64_t ldy) {
+    int32_t* bucket = (int32_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int32_t));
+    int8_t* sign = (int8_t*)malloc((size_t)(m > 0 ? m : 1));
+    orc_countsketch_indices(m, s, bucket, sign);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t j = 0; j < n; ++j) {
+        const float* xc = X + j * ldx;
+        double* yc = Y + j * ldy;
+        for (int64_t i = 0; i < m; ++i) yc[bucket[i]] += (double)sign[i] * (double)xc[i];
+    }
+    free(bucket);
+    free(sign);
+}
+
+/* Host memory bandwidth probe (STREAM "copy"): dst[i] = src[i] over OpenMP */
+/* threads; the caller times it and counts 2 * 8 * n bytes.                 */
+ORC_API void orc_stream_copy(int64_t n, const double* src, double* dst) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) dst[i] = src[i];
 }
 
 /* ------------------------------------------------------------------------ */

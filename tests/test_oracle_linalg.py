@@ -149,3 +149,49 @@ def test_lsqr_f32_with_openblas(orc):
     x, it, _ = orc.lsqr_f32(A, b, 1e-5, 400, 1e-5)
     assert rc in (0, -1)
     assert np.linalg.norm(A.astype(np.float64) @ x - b) / np.linalg.norm(b) < 2e-5
+
+
+def test_lsqr_f32_preconditioned_with_openblas(orc):
+    """Right-preconditioned FP32 LSQR (the CPU baseline's solver): with Rinv
+    from the QR of a CountSketch of A it converges in a few iterations at
+    kappa 1e4, where the plain FP32 run needs far more, and its solution
+    matches numpy's FP64 least squares and the FP64 oracle's."""
+    assert orc.blas_load(2) == 0 and orc.lib().orc_blas_loaded() == 1  # sgemv + BLAS-1 bound
+    rng = np.random.default_rng(9)
+    m, n = 4000, 40
+    A = (rng.standard_normal((m, n)) * np.logspace(0, -4, n)).astype(np.float32, order="F")
+    xt = rng.standard_normal(n)
+    b = A.astype(np.float64) @ xt
+    sk = orc.countsketch_apply_f32(A, 4 * n)
+    R = np.linalg.qr(sk, mode="r")
+    Rinv = np.linalg.inv(R)
+    x, it, hist = orc.lsqr_f32(A, b.astype(np.float32), 1e-6, 8, 1e-6, restarts=20, Rinv=Rinv)
+    rr = np.linalg.norm(A.astype(np.float64) @ x - b) / np.linalg.norm(b)
+    assert rr < 2e-6, rr
+    assert it < 40, it
+    x64, it64, _ = orc.lsqr_f64(A.astype(np.float64), b, 1e-13, 8, 1e-13, restarts=40, Rinv=Rinv)
+    xls = np.linalg.lstsq(A.astype(np.float64), b, rcond=None)[0]
+    # kappa 1e4 amplifies FP32 rounding in the solution, not in the residual
+    assert np.linalg.norm(x - xls) / np.linalg.norm(xls) < 1e-2
+    assert np.linalg.norm(x64 - xls) / np.linalg.norm(xls) < 1e-8
+    _, it_plain, _ = orc.lsqr_f32(A, b.astype(np.float32), 1e-6, 8, 1e-6, restarts=20)
+    assert it_plain > 2 * it
+
+
+def test_countsketch_f32_omp_matches_fp64(orc):
+    """The OpenMP FP32-input CountSketch sums each bucket in the same row
+    order as the FP64 oracle, so the two agree bit for bit; both match the
+    explicit sketch matrix product."""
+    rng = np.random.default_rng(10)
+    m, n, s = 5003, 37, 101
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    Y = orc.countsketch_apply_f32(A, s)
+    assert np.array_equal(Y, orc.countsketch_apply(A.astype(np.float64), s))
+    bk, sg = orc.countsketch_indices(m, s)
+    S = np.zeros((s, m))
+    S[bk, np.arange(m)] = sg
+    assert np.allclose(Y, S @ A.astype(np.float64), rtol=1e-12, atol=1e-12)
+
+
+def test_stream_copy_probe(orc):
+    assert orc.stream_copy_gbs(1 << 24, 2) > 0.1
