@@ -50,8 +50,8 @@ def parse():
     p.add_argument("--restarts", type=int, default=40)
     p.add_argument("--plain-cap", type=int, default=1500)
     p.add_argument("--gemv-reps", type=int, default=20)
-    p.add_argument("--cpu-rows", type=int, default=200_000)
-    p.add_argument("--cpu-iters", type=int, default=30)
+    p.add_argument("--cpu-rows", type=int, default=0,
+                   help="rows of the workload the CPU baseline uses (0 = all)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--check-every", type=int, default=8)
@@ -378,6 +378,112 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
     print(json.dumps({"kernels_only": out, "m": m, "n": n}), flush=True)
 
 
+def c1_problem(m=20000, n=200):
+    """C1: A = G diag(d), G ~ N(0,1) (numpy seed 1), d log-spaced 1..1e-3,
+    x_true ~ N(0,1) (seed 2), b = A x_true + 1e-6 N(0,1) (seed 3) in FP64."""
+    G = np.random.default_rng(1).standard_normal((m, n))
+    A = (G * np.logspace(0, -3, n)).astype(np.float32, order="F")
+    xt = np.random.default_rng(2).standard_normal(n)
+    b = A.astype(np.float64) @ xt + 1e-6 * np.random.default_rng(3).standard_normal(m)
+    return A, b
+
+
+def c1_leg(C, DeviceBuffer, O):
+    """C1 (plain FP32 LSQR, 20000 x 200, kappa 1e3, tol 1e-6, <= 400
+    iterations, one restart) on the host cores (median of 3) and on the GPU."""
+    A, b = c1_problem()
+    m, n = A.shape
+    b32 = b.astype(np.float32)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        xc, itc, _ = O.lsqr_f32(A, b32, 1e-6, 400, 1e-6)
+        runs.append((time.perf_counter() - t0) * 1e3)
+    rr_cpu = float(np.linalg.norm(A.astype(np.float64) @ xc - b) / np.linalg.norm(b))
+    Ad, bd, xd = DeviceBuffer.from_numpy(A), DeviceBuffer.from_numpy(b32), DeviceBuffer(n * 4)
+    po = C.LsqrOptions(C.F32, C.TIER_DEFAULT, C.TIER_DEFAULT, C.F32, C.F32, -1, 1e-6, 1e-6, 400, 1, 64, 1)
+    pa = C.lib().tomos_lsqr_arena_size(ctypes.byref(po), m, n)
+    pw = C.lib().tomos_lsqr_scratch_size(ctypes.byref(po), m, n)
+    parena, pws = DeviceBuffer(pa), DeviceBuffer(pw)
+    it, hl = ctypes.c_int(0), ctypes.c_int(0)
+    gpu_runs = []
+    for _ in range(4):
+        C.call("tomos_device_synchronize")
+        t0 = time.perf_counter()
+        C.call("tomos_lsqr_solve", ctypes.byref(po), m, n, Ad.ptr, None, m, bd.ptr, xd.ptr, None, n, parena.ptr, pa,
+               pws.ptr, pw, ctypes.byref(it), None, 0, ctypes.byref(hl), None)
+        C.call("tomos_device_synchronize")
+        gpu_runs.append((time.perf_counter() - t0) * 1e3)
+    xg = xd.to_numpy(np.float32, (n,)).astype(np.float64)
+    rr_gpu = float(np.linalg.norm(A.astype(np.float64) @ xg - b) / np.linalg.norm(b))
+    return {"shape": [m, n], "cpu_ms_median_of_3": round(float(np.median(runs)), 3),
+            "cpu_ms_runs": [round(r, 3) for r in runs], "cpu_iterations": itc, "cpu_relres": rr_cpu,
+            "gpu_ms_median_of_3_after_warmup": round(float(np.median(gpu_runs[1:])), 3),
+            "gpu_iterations": it.value, "gpu_relres": rr_gpu}
+
+
+def cpu_leg(args, C, DeviceBuffer, A, b, m, n, s, b_host, bnorm, x_true):
+    """The reference's CPU path on the headline workload, timed on the host
+    cores: CountSketch of the FP32 matrix (s rows), FP64 Householder QR of
+    the sketch and R^-1 (LAPACK dgeqrf / dtrtri in numpy's OpenBLAS 0.3.29),
+    then FP32 LSQR right-preconditioned by R^-1 (cblas_sgemv and BLAS-1) with
+    the GPU solve's restart cycle, to the same tolerance.  Also: C1 on the
+    host (median of 3) and on the GPU, host sgemv rates at the workload
+    shape, and a STREAM-style host copy rate."""
+    from oracle import oracle as O
+    import scipy.linalg.lapack as lapack
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rc = O.blas_load(threads)
+    detail = {"threads": threads, "openblas": "loaded" if rc == 0 else "NOT loaded (plain loops)"}
+    detail["stream_copy_GBps"] = round(O.stream_copy_gbs(1 << 31, 3), 1)
+    detail["c1"] = c1_leg(C, DeviceBuffer, O)
+    log("cpu c1: %s" % detail["c1"])
+    rows = m if args.cpu_rows <= 0 else min(args.cpu_rows, m)
+    Ahost = np.empty((rows, n), np.float32, order="F")
+    for j in range(n):  # the first `rows` rows of every column of the device matrix
+        C.call("tomos_memcpy_dtoh", Ahost[:, j].ctypes.data, A.ptr + j * m * 4, rows * 4, None)
+    gb = 4.0 * rows * n / 1e9
+    tn, tt = O.sgemv_ms(0, Ahost, 2), O.sgemv_ms(1, Ahost, 2)
+    detail["sgemv"] = {"rows": rows, "N_ms": round(tn, 2), "N_GBps": round(gb / (tn * 1e-3), 1),
+                       "T_ms": round(tt, 2), "T_GBps": round(gb / (tt * 1e-3), 1)}
+    bh = b_host[:rows].astype(np.float32)
+    t0 = time.perf_counter()
+    sk = O.countsketch_apply_f32(Ahost, s)
+    t1 = time.perf_counter()
+    R = np.linalg.qr(sk, mode="r")
+    Rinv, info = lapack.dtrtri(R, lower=0)
+    t2 = time.perf_counter()
+    xc, itc, hist = O.lsqr_f32(Ahost, bh, args.tol, args.inner, args.tol, restarts=args.restarts,
+                               Rinv=Rinv.astype(np.float32))
+    t3 = time.perf_counter()
+    # true relative residual in FP64 on the GPU (exact FP32 A)
+    rr = None
+    if rows == m:
+        xd = DeviceBuffer.from_numpy(xc.astype(np.float64))
+        r = DeviceBuffer(m * 8)
+        C.call("tomos_memcpy_dtod", r.ptr, b.ptr, m * 8, None)
+        C.call("tomos_gemv", C.F32, C.TIER_EXACT, 0, -1.0, m, n, A.ptr, None, m, C.F64, xd.ptr, 1.0, C.F64,
+               r.ptr, None, 0, None)
+        rr = float(np.linalg.norm(r.to_numpy(np.float64, (m,)))) / bnorm
+    total = (t3 - t0) * 1e3
+    detail["precond_lsqr"] = {"sketch_ms": round((t1 - t0) * 1e3, 1), "qr_trtri_ms": round((t2 - t1) * 1e3, 1),
+                              "lsqr_ms": round((t3 - t2) * 1e3, 1), "iterations": itc,
+                              "ms_per_iteration": round((t3 - t2) * 1e3 / max(itc, 1), 2), "relres": rr,
+                              "err_vs_x_true": float(np.linalg.norm(xc - x_true) / np.linalg.norm(x_true)),
+                              "lapack_info": int(info)}
+    del Ahost
+    converged = rr is not None and rr <= args.tol * (1 + 1e-6)
+    cpu = {"value": round(total, 1),
+           "unit": "ms (time to relres %g on the host cores, preconditioner build included)" % args.tol,
+           "cores": threads, "kind": "port",
+           "converged": converged,
+           "sample": "the whole workload (%d x %d FP32, %s rows used): oracle CountSketch (s=%d, OpenMP), numpy "
+                     "LAPACK dgeqrf + scipy dtrtri (OpenBLAS 0.3.29), oracle FP32 LSQR with the FP32 R^-1 right "
+                     "preconditioner (cblas_sgemv + BLAS-1, OpenBLAS %s), %d iterations per restart, "
+                     "one run" % (m, n, rows, s, detail["openblas"], args.inner)}
+    return cpu, detail
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -658,6 +764,34 @@ def main():
                           "mfma_TFLOPs": round(3 * useful, 1),
                           "mfma_frac_of_2500": round(3 * useful / 2500.0, 4),
                           "l2_rel_err_sampled_cols": float(max(errs))}
+            # mm_apply on packed A with FP32 B (tomos_gemm): the hi-only tier
+            # (16-bit matrix cores on the hi word) and the materialize tier
+            # (3-term split); whole calls, operand staging included
+            for tname, tier, terms in (("mm_hi_only", C.TIER_DEFAULT, 1), ("mm_materialize", C.TIER_MATERIALIZE, 3)):
+                args_m = (at, tier, 0, 0, 1.0, G, G, G, planes[0].ptr, planes[1].ptr, G, GB.ptr, G, 0.0,
+                          Cg.ptr, G, None)
+                C.call("tomos_gemm", *args_m)
+                C.call("tomos_event_record", ev0, None)
+                for _ in range(reps):
+                    C.call("tomos_gemm", *args_m)
+                C.call("tomos_event_record", ev1, None)
+                C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+                mms = f.value / reps
+                useful_m = 2.0 * G * G * G / (mms * 1e-3) / 1e12
+                errs_m = []
+                for j in cols:
+                    refc = DeviceBuffer(G * 8)
+                    C.call("tomos_gemv", at, C.TIER_EXACT, 0, 1.0, G, G, planes[0].ptr, planes[1].ptr, G,
+                           C.F32, GB.ptr + j * G * 4, 0.0, C.F64, refc.ptr, None, 0, None)
+                    rc_ = refc.to_numpy(np.float64, (G,))
+                    gc_ = np.empty(G, np.float32)
+                    C.call("tomos_memcpy_dtoh", gc_.ctypes.data, Cg.ptr + j * G * 4, G * 4, None)
+                    errs_m.append(np.linalg.norm(gc_ - rc_) / np.linalg.norm(rc_))
+                gemm[name + "_" + tname] = {
+                    "ms": round(mms, 3), "useful_TFLOPs": round(useful_m, 1),
+                    "mfma_TFLOPs": round(terms * useful_m, 1),
+                    "mfma_frac_of_2500": round(terms * useful_m / 2500.0, 4),
+                    "l2_rel_err_vs_exact_A_sampled_cols": float(max(errs_m))}
         log("split gemm: %s" % gemm)
         del GA, GB, planes, Cg, gscr
 
@@ -673,29 +807,11 @@ def main():
         sparse = sparse_leg(args, C, DeviceBuffer, ev0, ev1)
         log("sparse: %s" % sparse)
 
-    # ---- CPU baseline: the oracle's OpenBLAS FP32 LSQR on a row sample -------
-    cpu = None
+    # ---- CPU baselines on the host cores (OpenBLAS 0.3.29 from numpy) ---------
+    cpu, cpu_detail = None, None
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        rc = O.blas_load(threads)
-        rows = min(args.cpu_rows, m)
-        Ahost = np.empty((rows, n), np.float32, order="F")
-        # first `rows` rows of every column of the device matrix
-        for j in range(n):
-            C.call("tomos_memcpy_dtoh", Ahost[:, j].ctypes.data, A.ptr + j * m * 4, rows * 4, None)
-        bh = b_host[:rows].astype(np.float32)
-        t0 = time.perf_counter()
-        _, cit, _ = O.lsqr_f32(Ahost, bh, 1e-30, args.cpu_iters, 1e-30)
-        ct = (time.perf_counter() - t0) * 1e3
-        per_it = ct / max(cit, 1) * (m / rows)
-        cpu = {"value": round(per_it, 3), "unit": "ms per plain-FP32 LSQR iteration (full 2e6 rows)",
-               "cores": threads, "kind": "port",
-               "sample": "oracle lsqr_f32 (OpenBLAS 0.3.29 sgemv from numpy, %s) on the first %d of "
-                         "%d rows of the same matrix, %d iterations, time scaled by %d/%d"
-                         % ("loaded" if rc == 0 else "NOT loaded: plain loops", rows, m, cit, m, rows),
-               "gpu_same_unit": plain["ms_per_iteration"] if plain else None}
-        log("cpu: %s" % cpu)
+        cpu, cpu_detail = cpu_leg(args, C, DeviceBuffer, A, b, m, n, s, b_host, bnorm, x_true)
+        log("cpu: %s %s" % (cpu, cpu_detail))
 
     speedup = None
     if plain:
@@ -738,6 +854,7 @@ def main():
                                "launches that streamed A inside the timed solves (HIP events on the "
                                "solve stream)" % (m, n, pair_bytes / 1e9, k_full.size)},
         "cpu_baseline": cpu,
+        "cpu_detail": cpu_detail,
     }
     print(json.dumps(out), flush=True)
 
