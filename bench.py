@@ -58,6 +58,7 @@ def parse():
     p.add_argument("--fused", type=int, default=1,
                    help="single-pass LSQR matvec kernel (A read once per iteration)")
     p.add_argument("--gemm", type=int, default=8192, help="split GEMM size (0 = skip)")
+    p.add_argument("--c2", type=int, default=1, help="FP32_16 GEMV / GEMV^T rows at 1e6 x 1000 (0 = skip)")
     p.add_argument("--sparse-row", type=int, default=50,
                    help="entries per row of the CSR LSQR leg (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
@@ -378,6 +379,46 @@ def kernels_only(args, C, DeviceBuffer, m, n, wh):
     print(json.dumps({"kernels_only": out, "m": m, "n": n}), flush=True)
 
 
+def c2_leg(C, DeviceBuffer, ev0, ev1, reps):
+    """C2: packed GEMV and GEMV^T at 1,000,000 x 1000 with A_ij = z 2^k
+    (z ~ N(0,1), k uniform in [-8, 8], device generator seed 4) stored as
+    FP32_16; x, y ~ N(0,1) (numpy seeds 5, 6).  The FP32_16 pure-half tier
+    (DEFAULT: half_only decode of wh, the reference's registered 16-bit
+    tier) and the exact tier (wh+wr), each against the FP64 product of the
+    exact values (EXACT tier with FP64 output)."""
+    m, n = 1_000_000, 1000
+    A = DeviceBuffer(m * n * 4)
+    C.call("tomos_fill_normal_f32", A.ptr, m, n, m, 4, None, -8, 8, None)
+    wh, wr = DeviceBuffer(m * n * 2), DeviceBuffer(m * n * 2)
+    C.call("tomos_pack", C.PACK_FP32_16, m, n, A.ptr, m, wh.ptr, wr.ptr, m, None)
+    del A
+    x = np.random.default_rng(5).standard_normal(n).astype(np.float32)
+    y = np.random.default_rng(6).standard_normal(m).astype(np.float32)
+    out = {"m": m, "n": n, "storage": "FP32_16 planar", "data": "z 2^k, k in [-8, 8]"}
+    for trans, vin, rows in ((0, x, m), (1, y, n)):
+        vd = DeviceBuffer.from_numpy(vin)
+        ref = DeviceBuffer(rows * 8)
+        C.call("tomos_gemv", C.FP32_16, C.TIER_EXACT, trans, 1.0, m, n, wh.ptr, wr.ptr, m, C.F32, vd.ptr, 0.0, C.F64,
+               ref.ptr, None, 0, None)
+        r64 = ref.to_numpy(np.float64, (rows,))
+        yd = DeviceBuffer(rows * 4)
+        for name, tier, esz in (("pure_half", C.TIER_DEFAULT, 2), ("exact", C.TIER_EXACT, 4)):
+            sb = C.lib().tomos_gemv_scratch_size(C.FP32_16, tier, trans, m, n)
+            sc = DeviceBuffer(max(sb, 1))
+            call = lambda: C.call("tomos_gemv", C.FP32_16, tier, trans, 1.0, m, n, wh.ptr, wr.ptr, m, C.F32, vd.ptr,
+                                  0.0, C.F32, yd.ptr, sc.ptr, sb, None)
+            call()
+            t = time_call(call, reps, ev0, ev1)
+            got = yd.to_numpy(np.float32, (rows,)).astype(np.float64)
+            byts = esz * m * n + 4.0 * (m + n)
+            key = "%s_%s" % (name, "T" if trans else "N")
+            out[key] = {"ms": round(t, 4), "GBps": round(byts / (t * 1e-3) / 1e9, 1),
+                        "GFlops": round(2.0 * m * n / (t * 1e-3) / 1e9, 1),
+                        "l2_rel_err_vs_fp64": float(np.linalg.norm(got - r64) / np.linalg.norm(r64)),
+                        "max_rel_err_vs_fp64": float(np.max(np.abs(got - r64)) / np.max(np.abs(r64)))}
+    return out
+
+
 def c1_problem(m=20000, n=200):
     """C1: A = G diag(d), G ~ N(0,1) (numpy seed 1), d log-spaced 1..1e-3,
     x_true ~ N(0,1) (seed 2), b = A x_true + 1e-6 N(0,1) (seed 3) in FP64."""
@@ -666,6 +707,10 @@ def main():
         "read_stream_same_bytes": {"ms": round(t_rd, 4), "GBps": round(gbs(2.0 * m * n, t_rd), 1)},
     }
     log("gemv: %s" % gemv_detail)
+    c2 = None
+    if args.c2:
+        c2 = c2_leg(C, DeviceBuffer, ev0, ev1, args.gemv_reps)
+        log("c2 fp32_16 gemv: %s" % c2)
 
     # dense Gaussian sketch fill (create_gaussian_sketch): 2n x 131072 FP32, 1 GiB written
     gk, gm = 2 * n, 131072
@@ -841,6 +886,7 @@ def main():
         "speedup_vs_plain_fp32": speedup,
         "speedup_is_lower_bound": bool(plain and not plain["converged"]),
         "gemv": gemv_detail,
+        "c2_fp32_16_gemv": c2,
         "tuned_shapes": tuned,
         "gemm_split": gemm,
         "gaussian_sketch": gaussian,

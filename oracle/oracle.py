@@ -109,6 +109,9 @@ def lib():
         L.orc_blas_loaded.argtypes = []
         L.orc_countsketch_apply_f32_omp.restype = None
         L.orc_countsketch_apply_f32_omp.argtypes = [i64, i64, f32p, i64, i64, f64p, i64]
+        L.orc_pack_compare_range.restype = ctypes.c_int64
+        L.orc_pack_compare_range.argtypes = [ctypes.c_int, ctypes.c_uint64, i64, u16p, u16p,
+                                             ctypes.POINTER(ctypes.c_int64)]
         L.orc_sgemv.restype = None
         L.orc_sgemv.argtypes = [ctypes.c_int, i64, i64, ctypes.c_float, f32p, i64, f32p, ctypes.c_float, f32p]
         L.orc_stream_copy.restype = None

@@ -395,6 +395,27 @@ ORC_API void orc_pack(int variant, int64_t m, int64_t n, const float* src, int64
         }
 }
 
+/* Exhaustive encoder check: wh/wr hold the encoding (variant as orc_pack)  */
+/* of the fp32 bit patterns start .. start + count - 1, in order.  Returns   */
+/* the number of elements that differ from the oracle's encoding and, in    */
+/* *first_bad, the smallest such bit pattern (-1 if none).  OpenMP.          */
+ORC_API int64_t orc_pack_compare_range(int variant, uint64_t start, int64_t count, const uint16_t* wh,
+                                       const uint16_t* wr, int64_t* first_bad) {
+    int64_t bad = 0, fb = INT64_MAX;
+#pragma omp parallel for reduction(+ : bad) reduction(min : fb) schedule(static)
+    for (int64_t i = 0; i < count; ++i) {
+        const uint32_t bits = (uint32_t)(start + (uint64_t)i);
+        uint16_t h, r;
+        orc_pack(variant, 1, 1, (const float*)&bits, 1, &h, &r, 1);
+        if (h != wh[i] || r != wr[i]) {
+            bad++;
+            if ((int64_t)bits < fb) fb = (int64_t)bits;
+        }
+    }
+    if (first_bad) *first_bad = bad ? fb : -1;
+    return bad;
+}
+
 /* decoders selectable by id, for array decode / GEMV operand decode */
 enum { ORC_DEC16_EXACT = 0, ORC_DEC16_HALF = 1, ORC_DEC16_HALF_FAST = 2,
        ORC_DEC16_BF16 = 3, ORC_DECBF_EXACT = 4, ORC_DECBF_HI = 5,

@@ -165,3 +165,42 @@ def test_apply_saturated_fp32_16_falls_back(capi, orc):
     assert np.array_equal(np.isinf(Y), np.isinf(ref)) and np.isinf(Y).sum() == 1
     fin = np.isfinite(ref)
     assert np.abs(Y[fin] - ref[fin]).max() <= 2e-6 * np.abs(ref[fin]).max()
+
+
+@pytest.mark.parametrize("k,n", [(12000, 300), (9000, 70)])
+def test_qr_tall_wavefront_path(capi, k, n):
+    """k above 8192 takes the panel wavefront factorization (a block per
+    panel column, per-column flags between blocks) instead of TSQR: R
+    against numpy's QR up to row signs, and R^-1 from the sketch-QR entry
+    point inverts it."""
+    print("wavefront qr k=%d n=%d" % (k, n), flush=True)
+    rng = np.random.default_rng(k + n)
+    A = rng.standard_normal((k, n)) * np.logspace(0, -5, n)
+    R = H.geqrf_r(A)
+    Rn = np.linalg.qr(A, mode="r")
+    sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
+    assert np.allclose(np.tril(R, -1), 0)
+    err = np.abs(R - sgn[:, None] * Rn).max() / np.abs(Rn).max()
+    assert err < 1e-12, err
+    ad = H.dev(np.asfortranarray(A))
+    C.call("tomos_sketch_qr_precond_f64", k, n, ad.ptr, k, None)
+    Ri = ad.to_numpy(np.float64, (k, n))[:n]
+    assert np.abs(R @ Ri - np.eye(n)).max() < 1e-8
+
+
+def test_qr_tall_wavefront_batched_matches_single(capi):
+    """Two problems above the TSQR row limit in one batched call give the
+    single calls' R^-1 bit for bit."""
+    rng = np.random.default_rng(77)
+    shapes = [(12000, 96), (8500, 40)]
+    mats = [rng.standard_normal(s) * np.logspace(0, -4, s[1]) for s in shapes]
+    singles = []
+    for A in mats:
+        ad = H.dev(np.asfortranarray(A))
+        C.call("tomos_sketch_qr_precond_f64", A.shape[0], A.shape[1], ad.ptr, A.shape[0], None)
+        singles.append(ad.to_numpy(np.float64, A.shape)[:A.shape[1]])
+    bufs = [H.dev(np.asfortranarray(A)) for A in mats]
+    qp = (C.QrProblem * 2)(*[C.QrProblem(A.shape[0], A.shape[1], A.shape[0], b.ptr) for A, b in zip(mats, bufs)])
+    C.call("tomos_sketch_qr_precond_batched_f64", 2, qp, None, 0, None)
+    for A, b, s in zip(mats, bufs, singles):
+        assert np.array_equal(b.to_numpy(np.float64, A.shape)[:A.shape[1]], s)

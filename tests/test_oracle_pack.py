@@ -205,3 +205,73 @@ def test_fp32_16_bf16_only_vs_torch(orc):
     ref = torch.from_numpy(val.astype(np.float32)).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
     keep = fin & ~np.isnan(val)
     assert np.array_equal(got[keep], ref[keep])
+
+
+def _wh_value(w):
+    """Value of FP32_16 hi words by the header's closed-form layout
+    (fp32_16_pack.hpp:11-67): control bit 0 -> the word is fp16; control 1 ->
+    sign | dir | shift_by | mant9 with exponent 143 + shift_by (dir 1) or
+    112 - shift_by (dir 0), shift_by 14 saturating."""
+    w = w.astype(np.int64)
+    ctl = (w & 1) == 1
+    sb = (w >> 10) & 0xF
+    direc = (w >> 14) & 1
+    sign = np.where(w & 0x8000, -1.0, 1.0)
+    mant = ((w >> 1) & 0x1FF).astype(np.float64)
+    e = np.where(direc == 1, 143 + sb, 112 - sb).astype(np.float64) - 127.0
+    val = np.where(ctl, sign * (1.0 + mant / 512.0) * 2.0 ** e, w.astype(np.uint16).view(np.float16).astype(np.float64))
+    sat = ctl & (sb == 14)
+    return np.where(sat, np.where(direc == 1, sign * np.inf, sign * 0.0), val), ctl
+
+
+@pytest.mark.parametrize("band", ["overflow", "underflow"])
+def test_fp32_16_split_extended_bands_vs_rne9(orc, band):
+    """The split encoder's hi word in the extended bands (fp32 exponents
+    143..156 above the fp16 range, 99..112 below it): its value by the
+    closed-form layout equals v rounded to nearest-even at 9 fraction bits
+    (numpy, FP64), and fp32_16_bf16_only of it equals torch's bf16 rounding
+    of that value.  Independent of the oracle's own combine."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21 if band == "overflow" else 22)
+    lo, hi = (143, 156) if band == "overflow" else (99, 112)
+    e32 = rng.integers(lo, hi + 1, 200000).astype(np.uint32)
+    mant = rng.integers(0, 1 << 23, 200000).astype(np.uint32)
+    sgn = rng.integers(0, 2, 200000).astype(np.uint32)
+    bits = (sgn << 31) | (e32 << 23) | mant
+    v = bits.view(np.float32)
+    wh, _ = orc.pack(0, v.reshape(-1, 1))
+    w = wh[:, 0]
+    got, ctl = _wh_value(w)
+    ex = np.floor(np.log2(np.abs(v.astype(np.float64))))
+    q = 2.0 ** (ex - 9)
+    v9 = np.round(v.astype(np.float64) / q) * q  # half to even
+    # a carry out of the 9 bits can leave the band (it saturates there)
+    e9 = np.floor(np.log2(np.abs(v9)))
+    inband = (e9 + 127 >= lo) & (e9 + 127 <= hi)
+    assert np.all(ctl[inband]), "extended-band words carry the control bit"
+    assert np.array_equal(got[inband], v9[inband])
+    L = orc.lib()
+    b16 = np.array([L.orc_fp32_16_bf16_only(int(x)) for x in w[inband][:20000]], np.uint16)
+    ref = torch.from_numpy(v9[inband][:20000].astype(np.float32)).to(torch.bfloat16).view(torch.int16).numpy()
+    assert np.array_equal(b16, ref.view(np.uint16))
+
+
+def test_pack_compare_range_detects_mismatch(orc):
+    """orc_pack_compare_range (the exhaustive GPU sweep's checker) counts
+    every element that differs from the oracle's encoding: zero on the
+    oracle's own output, and a single flipped bit is found with its pattern."""
+    import ctypes
+    start, count = 0x3F7FFF00, 4096
+    bits = (np.arange(count, dtype=np.uint64) + start).astype(np.uint32)
+    for variant in range(5):
+        wh, wr = orc.pack(variant, bits.view(np.float32).reshape(-1, 1))
+        wh, wr = np.ascontiguousarray(wh[:, 0]), np.ascontiguousarray(wr[:, 0])
+        fb = ctypes.c_int64(0)
+        L = orc.lib()
+        assert L.orc_pack_compare_range(variant, start, count, wh.ctypes.data_as(orc.u16p),
+                                        wr.ctypes.data_as(orc.u16p), ctypes.byref(fb)) == 0
+        assert fb.value == -1
+        wr[1234] ^= 4
+        assert L.orc_pack_compare_range(variant, start, count, wh.ctypes.data_as(orc.u16p),
+                                        wr.ctypes.data_as(orc.u16p), ctypes.byref(fb)) == 1
+        assert fb.value == start + 1234

@@ -441,6 +441,16 @@ PlanT plan_t(const Kind& k, int64_t m, int64_t n) {
     return p;
 }
 
+// The kind a GEMV runs with: F32 GEMV^T accumulates in FP64 in every tier
+// (the FP64-sum kernel streams A faster on gfx950 than the FP32-sum one,
+// and it is the more accurate; the reference's tiled FP32 GEMV accumulates
+// in FP64 too, mv_cuda_kernels.cu:863-990).
+Kind kind_for(int a_type, int tier, int trans) {
+    Kind k = classify(a_type, tier);
+    if (trans == TOMOS_TRANSPOSE && k.fmt == FMT_F32) k.acc64 = true;
+    return k;
+}
+
 size_t scratch_for(const Kind& k, int trans, int64_t m, int64_t n) {
     const size_t acc = k.acc64 ? 8 : 4;
     if (trans == TOMOS_NO_TRANSPOSE) {
@@ -497,7 +507,9 @@ void run_t(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
     require(blocks < (int64_t(1) << 31), "gemv: grid too large");
     if (m > 0) {
         const dim3 grid(static_cast<unsigned>(blocks));
-        if (vec && stream_loads(matrix_bytes<FMT, DEC>(m, n)))
+        // the EXACT tier reads A with cacheable loads: its non-temporal form
+        // needs 256 VGPRs (one wave per SIMD), the cacheable one ~140
+        if (vec && DEC != DEC_EXACT && stream_loads(matrix_bytes<FMT, DEC>(m, n)))
             gemv_t_kernel<FMT, DEC, VF, ACC, CB, true><<<grid, kThreads, 0, s>>>(
                 m, n, a, wr, lda, x, x64, p.rows_per_chunk, p.ncg, part, coef.skip);
         else if (vec)
@@ -517,7 +529,7 @@ void run_t(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
 }  // namespace
 
 size_t gemv_scratch_bytes(int a_type, int tier, int trans, int64_t m, int64_t n) {
-    return scratch_for(classify(a_type, tier), trans, m, n);
+    return scratch_for(kind_for(a_type, tier, trans), trans, m, n);
 }
 
 int64_t gemv_sumsq_blocks_max(int64_t m) { return ceil_div(std::max<int64_t>(m, 1), kThreads); }
@@ -525,8 +537,8 @@ int64_t gemv_sumsq_blocks_max(int64_t m) { return ceil_div(std::max<int64_t>(m, 
 void gemv(int a_type, int tier, int trans, const Coef& coef, int64_t m, int64_t n, const void* a,
           const uint16_t* wr, int64_t lda, int x_type, const void* x, int y_type, void* y,
           void* scratch, size_t scratch_bytes, hipStream_t s, int* sumsq_blocks) {
-    const Kind k = classify(a_type, tier);
     require(trans == TOMOS_NO_TRANSPOSE || trans == TOMOS_TRANSPOSE, "gemv: bad trans");
+    const Kind k = kind_for(a_type, tier, trans);
     require(m >= 0 && n >= 0, "gemv: negative size");
     require(lda >= std::max<int64_t>(m, 1), "gemv: lda < m");
     require(x_type == TOMOS_F32 || x_type == TOMOS_F64, "gemv: x must be F32 or F64");

@@ -98,6 +98,61 @@ struct LaneTR<L, 0, LO, T> {
     static __device__ __forceinline__ void run(T*, int) {}
 };
 
+// LaneTR for float values without the LDS crossbar (gfx950): the stage on
+// lane bit 5 / 4 with v_permlane32_swap / v_permlane16_swap (after the swap
+// each lane holds its kept half and the partner's copy of it, and their sum
+// is the stage's result), the stage on bit 3 with DPP row_ror:8 (lane ^ 8
+// exactly), bits 1 and 0 with quad permutations (lane ^ 2, lane ^ 1).  Bit 2
+// has no exact single-instruction DPP partner: with MIRROR (every lower lane
+// bit is reduced by the later stages, so the partner only has to differ in
+// bit 2 and agree above it) row_half_mirror (lane ^ 7) serves, otherwise the
+// crossbar shuffle does.  Same kept halves and per-stage sums as LaneTR.
+template <bool ROWS16>
+__device__ __forceinline__ float swap_add_f(float lo, float hi) {
+    if constexpr (ROWS16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+}
+template <int OFF, bool MIRROR>
+__device__ __forceinline__ float partner_f(float x) {
+    constexpr int kCtrl = OFF == 8 ? 0x128 : (OFF == 4 ? 0x141 : (OFF == 2 ? 0x4E : 0xB1));
+    if constexpr (OFF == 4 && !MIRROR) {
+        return __shfl_xor(x, 4);
+    } else {
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xF, 0xF, true));
+    }
+}
+template <int L, int OFF, int LO, bool MIRROR>
+struct LaneTRf {
+    static __device__ __forceinline__ void run(float* v, int lane) {
+        if constexpr (OFF < LO || OFF == 0) {
+            return;
+        } else if constexpr (L == 1) {
+            if constexpr (OFF >= 16) v[0] += __shfl_xor(v[0], OFF);
+            else v[0] += partner_f<OFF, MIRROR>(v[0]);
+            LaneTRf<1, OFF / 2, LO, MIRROR>::run(v, lane);
+        } else {
+            if constexpr (OFF >= 16) {
+#pragma unroll
+                for (int t = 0; t < L / 2; ++t) v[t] = swap_add_f<OFF == 16>(v[t], v[t + L / 2]);
+            } else {
+                const bool upper = (lane & OFF) != 0;
+#pragma unroll
+                for (int t = 0; t < L / 2; ++t) {
+                    const float send = upper ? v[t] : v[t + L / 2];
+                    const float keep = upper ? v[t + L / 2] : v[t];
+                    v[t] = keep + partner_f<OFF, MIRROR>(send);
+                }
+            }
+            LaneTRf<L / 2, OFF / 2, LO, MIRROR>::run(v, lane);
+        }
+    }
+};
+
 template <int G>
 constexpr int log2_small() {
     return G >= 16 ? 4 : (G >= 8 ? 3 : (G >= 4 ? 2 : (G >= 2 ? 1 : 0)));
@@ -190,6 +245,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         const int64_t my_row = row0 + g * RPL;
         const bool in = my_row < m;
         const int nvalid = in ? static_cast<int>(min<int64_t>(RPL, m - my_row)) : 0;
+        const bool tile_full = row0 + TR <= m;  // every lane has RPL rows
         uint4 t[ROUNDS];
         if constexpr (STAGE) {
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA has landed
@@ -202,16 +258,19 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
                 upre = u64 ? static_cast<double*>(u)[row0 + tid]
                            : static_cast<double>(static_cast<float*>(u)[row0 + tid]);
             issue(tile + 1);
-            // lanes outside A hold the stand-in vector: clear them (bit masks,
-            // no branches)
+            // lanes outside A hold the stand-in vector: clear them (bit masks;
+            // only the last tile and the rounds that reach past column n have
+            // such lanes, a block-uniform test)
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
-                const int64_t j = static_cast<int64_t>(r) * CS + c;
-                const uint32_t keep = (in && j < n) ? 0xFFFFFFFFu : 0u;
-                t[r].x &= keep;
-                t[r].y &= keep;
-                t[r].z &= keep;
-                t[r].w &= keep;
+                if (!tile_full || static_cast<int64_t>(r + 1) * CS > n) {
+                    const int64_t j = static_cast<int64_t>(r) * CS + c;
+                    const uint32_t keep = (in && j < n) ? 0xFFFFFFFFu : 0u;
+                    t[r].x &= keep;
+                    t[r].y &= keep;
+                    t[r].z &= keep;
+                    t[r].w &= keep;
+                }
             }
         } else {
 #pragma unroll
@@ -237,7 +296,7 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         // reduce over the column slots of this wave (lane bits LOG_G..5):
         // halving stages on bits 5.. leave row k = lane bits (5..6-LOG_RPL),
         // plain stages on the bits below make the lanes of a row agree
-        LaneTR<RPL, 32, G>::run(acc, lane);
+        LaneTRf<RPL, 32, G, false>::run(acc, lane);
         {
             const int k = (lane >> (6 - LOG_RPL)) & (RPL - 1);
             constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
@@ -275,17 +334,29 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 #pragma unroll
         for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
         float cs[ROUNDS];
+        if (tile_full) {
 #pragma unroll
-        for (int r = 0; r < ROUNDS; ++r) {
-            float av[RPL];
-            decode_vec<FMT>(t[r], av);
-            float s = 0.0f;
+            for (int r = 0; r < ROUNDS; ++r) {
+                float av[RPL];
+                decode_vec<FMT>(t[r], av);
+                float s = 0.0f;
 #pragma unroll
-            for (int k = 0; k < RPL; ++k) s = fmaf(k < nvalid ? av[k] : 0.0f, ur[k], s);
-            cs[r] = s;
+                for (int k = 0; k < RPL; ++k) s = fmaf(av[k], ur[k], s);
+                cs[r] = s;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < ROUNDS; ++r) {
+                float av[RPL];
+                decode_vec<FMT>(t[r], av);
+                float s = 0.0f;
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) s = fmaf(k < nvalid ? av[k] : 0.0f, ur[k], s);
+                cs[r] = s;
+            }
         }
         // reduce over the G row groups (lane bits 0..LOG_G-1); lane keeps TOUT columns
-        LaneTR<ROUNDS, G / 2, 1>::run(cs, lane);
+        LaneTRf<ROUNDS, G / 2, 1, true>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
         tile_barrier<STAGE>();  // red/ush reuse in the next tile
@@ -657,7 +728,7 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
             for (int k = 0; k < RPL; ++k) sacc = fmaf(k < nvalid ? av[k] : 0.0f, ur[k], sacc);
             cs[r] = sacc;
         }
-        LaneTR<R, G / 2, 1>::run(cs, lane);
+        LaneTRf<R, G / 2, 1, true>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
         __syncthreads();
