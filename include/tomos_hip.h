@@ -574,14 +574,25 @@ int tomos_lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, co
  * w[p] = (A^T A)^-1 g[p] by tomos_lsq_adjoint_solve's control flow and
  * kernels, then grad_b[p] = A w[p] and, when grad_a and grad_a[p] are
  * non-NULL, grad_a[p] (FP32, ldg[p] >= m) as tomos_lsq_grad at
- * opt->restart_tier.  The problems (largest first) are taken by `workers`
- * host threads (1..16), each issuing on its own HIP stream (default
- * priority) with its own slice of the scratch, so the small launches of
- * different problems overlap; every result is bit-identical to the
- * one-problem calls.  scratch (required):
- * tomos_lsq_adjoint_batched_scratch_size() bytes.  iterations: one per
- * problem, or NULL.  Synchronous: waits for `stream` before starting and
- * for every worker before returning.  No reference counterpart. */
+ * opt->restart_tier.
+ *   workers == 0: all problems on the device in the same launches: one
+ *     captured hipGraph of 16 CG iterations of every problem (the batched
+ *     LSQR solver's grouped single-pass kernel for A d and A^T A d, one
+ *     block per problem for the n-sized work) replayed until every problem
+ *     has stopped, with a captured restart sequence (per problem, the
+ *     restart-tier GEMV pair) between replays when some problem restarts.
+ *     Same control flow and stop tests per problem as
+ *     tomos_lsq_adjoint_solve; results agree with it to rounding.  Needs
+ *     a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, precond_type F64 (R^-1),
+ *     n <= 2048, 16-byte aligned A with lda a multiple of 4 (F32) / 8.
+ *   workers 1..16: the problems (largest first) are taken by that many host
+ *     threads, each issuing the one-problem calls on its own HIP stream
+ *     (default priority) with its own slice of the scratch; every result is
+ *     bit-identical to the one-problem calls.
+ * scratch (required): tomos_lsq_adjoint_batched_scratch_size() bytes for the
+ * same workers.  iterations: one per problem, or NULL.  Synchronous: waits
+ * for `stream` (workers >= 1) or runs on it (workers == 0) and returns when
+ * every result is written.  No reference counterpart. */
 size_t tomos_lsq_adjoint_batched_scratch_size(const tomos_lsqr_options* opt, int nprob,
                                               const tomos_lsq_problem* probs, int workers);
 int tomos_lsq_adjoint_batched(const tomos_lsqr_options* opt, int nprob, const tomos_lsq_problem* probs,

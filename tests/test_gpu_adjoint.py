@@ -176,7 +176,71 @@ def test_adjoint_batched_matches_one_problem_calls(capi, workers):
         assert np.array_equal(gas[k].to_numpy(np.float32, (m, n)), ga1.to_numpy(np.float32, (m, n)))
 
 
-def test_torch_lstsq_batched_vs_dense(capi):
+def test_adjoint_grouped_matches_one_problem_calls(capi):
+    """tomos_lsq_adjoint_batched with workers = 0 (every problem's CG in the
+    same grouped launches, replayed graph segments) against the one-problem
+    calls: w, grad_b and grad_A agree to the CG tolerance (the partial sums
+    of A^T A d are taken in another order, so not bit for bit), for uneven
+    shapes, a problem with g = 0 and restarts forced by a short inner cap;
+    and w against the dense FP64 QR adjoint."""
+    shapes = [(20000, 64), (5003, 37), (12000, 128), (4096, 16), (9001, 90), (3000, 5)]
+    opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-10, 1e-10, 12, 20, 4, 1)
+    keep, probs, gs, ws, gbs, gas, dense = [], [], [], [], [], [], []
+    for k, (m, n) in enumerate(shapes):
+        A, b, g = problem(m, n, 2, 400 + k)
+        if k == 5:
+            g = np.zeros(n)  # w = 0, grad_b = 0
+        lda = (m + 7) // 8 * 8
+        wh, wr = H.pack(C.PACK_FP32_BF16, A, ld=lda)
+        _, sk = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, lda, 4 * n)
+        xv = np.random.default_rng(k).standard_normal(n)
+        bd, xd = H.dev(b), H.dev(xv)
+        gd, wd, gbd, gad = H.dev(g), DeviceBuffer(n * 8), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
+        keep += [wh, wr, sk, bd, xd]
+        probs.append(C.LsqProblem(m, n, lda, wh.ptr, wr.ptr, bd.ptr, xd.ptr, sk.ptr, 4 * n))
+        gs.append(gd)
+        ws.append(wd)
+        gbs.append(gbd)
+        gas.append(gad)
+        dense.append(dense_grads(A, b, g)[0] if k != 5 else np.zeros(n))
+    P = len(shapes)
+    lp = (C.LsqProblem * P)(*probs)
+    ptrs = lambda bufs: (ctypes.c_void_p * P)(*[x.ptr for x in bufs])
+    ldg = (ctypes.c_int64 * P)(*[m for m, _ in shapes])
+    sb = C.lib().tomos_lsq_adjoint_batched_scratch_size(ctypes.byref(opt), P, lp, 0)
+    assert sb > 0, C.lib().tomos_last_error()
+    scr = DeviceBuffer(sb)
+    its = (ctypes.c_int * P)()
+    for rep in range(2):  # the second call replays the cached segment graphs
+        C.call("tomos_lsq_adjoint_batched", ctypes.byref(opt), P, lp, ptrs(gs), ptrs(ws), ptrs(gbs), ptrs(gas),
+               ldg, 0, scr.ptr, sb, its, None)
+        for k, (m, n) in enumerate(shapes):
+            q = probs[k]
+            asb = C.lib().tomos_lsq_adjoint_scratch_size(ctypes.byref(opt), m, n)
+            ascr = DeviceBuffer(asb)
+            w1, it1 = DeviceBuffer(n * 8), ctypes.c_int(0)
+            C.call("tomos_lsq_adjoint_solve", ctypes.byref(opt), m, n, q.a, q.wr, q.lda, gs[k].ptr, q.rinv, q.ldr,
+                   w1.ptr, ascr.ptr, asb, ctypes.byref(it1), None, 0, None, None)
+            gsb = C.lib().tomos_lsq_grad_scratch_size(C.FP32_BF16, C.TIER_EXACT, m, n)
+            gscr, gb1, ga1 = DeviceBuffer(gsb), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
+            C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, m, n, q.a, q.wr, q.lda, q.b, q.x, w1.ptr, gb1.ptr,
+                   ga1.ptr, m, gscr.ptr, gsb, None)
+            w0, wref = ws[k].to_numpy(np.float64, (n,)), w1.to_numpy(np.float64, (n,))
+            gb0, gbr = gbs[k].to_numpy(np.float64, (m,)), gb1.to_numpy(np.float64, (m,))
+            ga0, gar = gas[k].to_numpy(np.float32, (m, n)), ga1.to_numpy(np.float32, (m, n))
+            if k == 5:
+                assert np.array_equal(w0, np.zeros(n)) and np.array_equal(gb0, np.zeros(m)) and its[k] == 0
+                continue
+            rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+            # a 1e-10 normal-equation residual bounds w to ~kappa(A^T A) 1e-10
+            assert rel(w0, wref) < 1e-6, (k, rel(w0, wref))
+            assert rel(w0, dense[k]) < 1e-6, (k, rel(w0, dense[k]))
+            assert rel(gb0, gbr) < 1e-6 and rel(ga0, gar) < 1e-5, k
+            assert abs(its[k] - it1.value) <= max(3, it1.value // 5), (k, its[k], it1.value)
+
+
+@pytest.mark.parametrize("workers", [0, 3])
+def test_torch_lstsq_batched_vs_dense(capi, workers):
     """lstsq_batched (batched LSQR forward, batched adjoint backward) on
     uneven problems: x, grad_b and grad_A of every problem vs the dense FP64
     QR solution and gradients, for the loss sum_i x_i . g_i."""
@@ -191,7 +255,7 @@ def test_torch_lstsq_batched_vs_dense(capi):
         bs.append(torch.tensor(b_np, device=dev, requires_grad=True))
         gs.append(torch.tensor(g_np, device=dev))
         refs.append(dense_grads(A_np, b_np, g_np))
-    opts = SolveOptions(tol=1e-10, adjoint_tol=1e-10, adjoint_workers=3)
+    opts = SolveOptions(tol=1e-10, adjoint_tol=1e-10, adjoint_workers=workers)
     xs = lstsq_batched(As, bs, opts)
     sum((x * g).sum() for x, g in zip(xs, gs)).backward()
     for k, (x, A, b) in enumerate(zip(xs, As, bs)):

@@ -59,7 +59,8 @@ class SolveOptions:
     adjoint_tol: float = 1e-10   # |g - A^T A w| / |g| for the adjoint solve
     check_every: int = 8
     normal_correction: bool = True  # forward: x += (A^T A)^-1 A^T (b - A x), exact products
-    adjoint_workers: int = 8     # lstsq_batched: host threads (HIP streams) of the batched adjoint
+    adjoint_workers: int = 0     # lstsq_batched's adjoint: 0 = all problems in grouped device launches,
+                                 # 1..16 = host threads (one HIP stream each) running one-problem solves
     confirm: bool = True         # forward: stop only when a restart's true residual is below tol
 
 
@@ -192,7 +193,7 @@ def _adjoint_batched(probs, b64s, xs, gs, tol, want_a):
     gbs = [torch.empty(p.m, dtype=torch.float64, device=dev) for p in probs]
     gas = [torch.empty((p.n, p.m), dtype=torch.float32, device=dev) if want_a else None for p in probs]
     ldg = (ctypes.c_int64 * P)(*[p.m for p in probs])
-    workers = max(1, min(16, opts.adjoint_workers))
+    workers = max(0, min(16, opts.adjoint_workers))
     sb = C.lib().tomos_lsq_adjoint_batched_scratch_size(ctypes.byref(o), P, table, workers)
     if sb == 0:
         raise RuntimeError(C.lib().tomos_last_error().decode())

@@ -33,6 +33,7 @@
 #include <thread>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "common.hpp"
 #include "gemv.hpp"
 #include "lsqr_fused.hpp"
@@ -415,15 +416,13 @@ struct Adj {
     }
 };
 
+}  // namespace
+
 size_t grad_scratch_bytes(int a_type, int tier, int64_t m, int64_t n) {
     return a256(static_cast<size_t>(std::max<int64_t>(m, 1)) * 8) +
            gemv_scratch_bytes(a_type, tier, TOMOS_NO_TRANSPOSE, m, n);
 }
 
-}  // namespace
-}  // namespace tomos_hip
-
-namespace tomos_hip {
 namespace {
 
 void adjoint_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void* a, const uint16_t* wr,
@@ -468,6 +467,8 @@ void adjoint_solve(const tomos_lsqr_options& o, int64_t m, int64_t n, const void
     J.run(iterations, history, history_cap, history_len);
 }
 
+}  // namespace
+
 void lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda,
               const double* b, const double* x, const double* w, double* grad_b, float* grad_a, int64_t ldg,
               void* scratch, size_t scratch_bytes, hipStream_t s) {
@@ -510,6 +511,8 @@ void lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, const u
     }
     TOMOS_LAUNCH_CHECK("k_rank2");
 }
+
+namespace {
 
 // One worker's scratch: the adjoint's and the gradient's (exact tier) for
 // the largest problem of the batch, one after the other.
@@ -576,10 +579,11 @@ size_t tomos_lsq_adjoint_batched_scratch_size(const tomos_lsqr_options* o, int n
     size_t out = 0;
     const int rc = guarded([&] {
         require(o != nullptr && (nprob == 0 || probs != nullptr), "tomos_lsq_adjoint_batched_scratch_size: null");
-        require(nprob >= 0 && workers >= 1 && workers <= kMaxAdjWorkers,
+        require(nprob >= 0 && workers >= 0 && workers <= kMaxAdjWorkers,
                 "tomos_lsq_adjoint_batched_scratch_size: bad nprob or workers");
         for (int p = 0; p < nprob; ++p) adj_validate(*o, probs[p].m, probs[p].n);
-        out = batched_adj_layout(*o, nprob, probs).per_worker * static_cast<size_t>(std::min(workers, std::max(nprob, 1)));
+        if (workers == 0) out = adjoint_grouped_scratch_bytes(*o, nprob, probs);
+        else out = batched_adj_layout(*o, nprob, probs).per_worker * static_cast<size_t>(std::min(workers, std::max(nprob, 1)));
     });
     return rc == 0 ? out : 0;
 }
@@ -590,8 +594,8 @@ int tomos_lsq_adjoint_batched(const tomos_lsqr_options* o, int nprob, const tomo
                               size_t scratch_bytes, int* iterations, tomos_stream_t stream) {
     return guarded([&] {
         require(o != nullptr, "tomos_lsq_adjoint_batched: null options");
-        require(nprob >= 0 && workers >= 1 && workers <= kMaxAdjWorkers,
-                "tomos_lsq_adjoint_batched: bad nprob or workers (1..16)");
+        require(nprob >= 0 && workers >= 0 && workers <= kMaxAdjWorkers,
+                "tomos_lsq_adjoint_batched: bad nprob or workers (0..16)");
         if (nprob == 0) return;
         require(probs && g && w && grad_b, "tomos_lsq_adjoint_batched: null array");
         for (int p = 0; p < nprob; ++p) {
@@ -600,6 +604,11 @@ int tomos_lsq_adjoint_batched(const tomos_lsqr_options* o, int nprob, const tomo
             require(q.a && q.b && q.x && g[p] && w[p] && grad_b[p], "tomos_lsq_adjoint_batched: null pointer");
             require(q.lda >= std::max<int64_t>(q.m, 1), "tomos_lsq_adjoint_batched: lda < m");
             require(!grad_a || !grad_a[p] || (ldg && ldg[p] >= q.m), "tomos_lsq_adjoint_batched: ldg < m");
+        }
+        if (workers == 0) {
+            // every problem on the device in the same launches
+            adjoint_grouped(*o, nprob, probs, g, w, grad_b, grad_a, ldg, scratch, scratch_bytes, iterations, stream);
+            return;
         }
         const int nw = std::min(workers, nprob);
         const BatchedAdjLayout L = batched_adj_layout(*o, nprob, probs);

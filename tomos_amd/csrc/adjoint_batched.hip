@@ -1,0 +1,574 @@
+// The adjoint of many least-squares problems on the device, all problems in
+// the same launches (tomos_lsq_adjoint_batched with workers == 0).
+//
+// Every problem runs the single-problem adjoint's restarted,
+// R-preconditioned CG on the normal equations (adjoint.hip): restarts form
+// the exact residual res = g - A^T (A w) at the restart tier (FP64
+// accumulation) and stop when |res|/|g| < tol; inner iterations form
+// q = A d and A^T q from one read of A at the per-iteration tier.  What is
+// grouped is the launch:
+//   iteration: the grouped single-pass kernel (the batched LSQR solver's,
+//              u = A d with a zero u coefficient, A^T u per tile range, |u|^2
+//              per tile range) for all iterating problems, then one
+//              1024-thread block per problem for the n-sized work: alpha =
+//              rho/|q|^2, w += alpha d, res -= alpha A^T q, the stop test,
+//              z = R^-1 R^-T res, rho' = res.z, d = z + (rho'/rho) d;
+//   restart:   per problem, the exact-tier GEMV pair (a no-op launch for a
+//              problem that is not restarting), then one block per problem:
+//              res, |res|/|g|, the stop test, z, rho and d.
+// Each problem carries a small state machine on the device (restart
+// pending -> iterating -> restart pending ... -> done), so the host never
+// looks at a problem: it replays a captured segment of `kSegIters`
+// iterations (one graph launch) and, when some problem is waiting for a
+// restart, the captured restart sequence, checking after each launch
+// whether every problem is done.  The segment graphs are instantiated once
+// per launch configuration and kept.
+//
+// The results agree with the one-problem adjoint to rounding (the partial
+// sums of A^T q and |q|^2 are taken per tile range rather than per block),
+// not bit for bit.  No reference counterpart: the reference has no adjoint.
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "adjoint.hpp"
+#include "common.hpp"
+#include "gemv.hpp"
+#include "lsqr_fused.hpp"
+#include "tomos_hip.h"
+
+namespace tomos_hip {
+namespace {
+
+constexpr int kTail = 1024;
+constexpr int kSegIters = 16;  // inner iterations per captured segment
+
+enum : int { kPhaseRestart = 0, kPhaseIterate = 1, kPhaseDone = 2 };
+
+struct GState {
+    double rho, gnorm, resnorm;
+    double zero;        // 0.0: u coefficient of the single-pass kernel (u = A d)
+    int phase;          // kPhase*
+    int restarts_done;  // exact residuals formed so far
+    int iters;          // inner iterations of the current restart
+    int completed;      // inner iterations in total
+    int skip_restart;   // restart GEMVs are a no-op (not restarting, or w == 0)
+    int skip_iter;      // grouped single-pass kernel is a no-op (not iterating)
+    int pad[2];
+};
+
+struct GProb {
+    int64_t m, n, ldr;
+    const double* g;
+    double* w;
+    const double* rinv;
+    double *res, *z, *d, *rfull;  // n each
+    GState* st;
+    int nitems;
+    double* sumsq;  // [nitems]
+    float* tpart;   // [nitems][n]
+};
+
+__global__ void k_gadj_init(const GProb* __restrict__ P, int nprob) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nprob) return;
+    GState* s = P[i].st;
+    s->rho = s->gnorm = s->resnorm = s->zero = 0.0;
+    s->phase = P[i].n > 0 ? kPhaseRestart : kPhaseDone;
+    s->restarts_done = s->iters = s->completed = 0;
+    s->skip_restart = 1;  // the first restart has w = 0: res = g, no products
+    s->skip_iter = 1;
+}
+
+__global__ __launch_bounds__(kTail) void k_gadj_zero_w(const GProb* __restrict__ P) {
+    const GProb& p = P[blockIdx.x];
+    for (int64_t j = threadIdx.x; j < p.n; j += kTail) p.w[j] = 0.0;
+}
+
+__device__ __forceinline__ double gsum(double v, double* sh) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kTail / 64; ++i) t += sh[i];  // same order in every thread
+    return t;
+}
+
+// z = R^-1 (R^-T res) with y = R^-T res in LDS (n doubles): column j of R^-1
+// (rows 0..j) dotted with res by one wave, then row i of R^-1 (columns
+// i..n-1) dotted with y by one thread.  Returns rho' = res . z.
+__device__ double gadj_precond(const GProb& p, double* y, double* sh) {
+    const int64_t n = p.n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t j = wave; j < n; j += kTail / 64) {
+        const double* col = p.rinv + j * p.ldr;
+        double a = 0.0;
+        for (int64_t i = lane; i <= j; i += 64) a += col[i] * p.res[i];
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) y[j] = a;
+    }
+    __syncthreads();
+    double rz = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kTail) {
+        double a = 0.0;
+        for (int64_t j = i; j < n; ++j) a += p.rinv[i + j * p.ldr] * y[j];
+        p.z[i] = a;
+        rz += p.res[i] * a;
+    }
+    return gsum(rz, sh);
+}
+
+// Restart of every problem whose phase is kPhaseRestart: res = g - rfull
+// (rfull = A^T A w, or nothing on the first restart), |g| on the first, the
+// exact-residual stop test (the restart after `max_restarts` earlier ones is
+// final whatever its residual), then z = M res, rho = res.z, d = z.
+__global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict__ P, double tol, int max_restarts) {
+    const GProb& p = P[blockIdx.x];
+    GState* S = p.st;
+    if (S->phase != kPhaseRestart) return;
+    extern __shared__ double y[];
+    __shared__ double sh[kTail / 64];
+    __shared__ int go;
+    const int k = S->restarts_done;  // read before the barriers: thread 0 rewrites it
+    const bool first = k == 0;
+    double gg = 0.0, rr = 0.0;
+    for (int64_t j = threadIdx.x; j < p.n; j += kTail) {
+        const double rj = p.g[j] - (first ? 0.0 : p.rfull[j]);
+        p.res[j] = rj;
+        gg += p.g[j] * p.g[j];
+        rr += rj * rj;
+    }
+    gg = gsum(gg, sh);
+    rr = gsum(rr, sh);
+    if (threadIdx.x == 0) {
+        if (first) S->gnorm = sqrt(gg);
+        S->resnorm = sqrt(rr);
+        S->restarts_done = k + 1;
+        S->skip_restart = 1;
+        const double rel = S->gnorm > 0.0 ? S->resnorm / S->gnorm : 0.0;
+        const bool stop = S->gnorm == 0.0 || rel < tol || k >= max_restarts;
+        go = stop ? 0 : 1;
+        S->phase = stop ? kPhaseDone : kPhaseIterate;
+        S->skip_iter = stop ? 1 : 0;
+        S->iters = 0;
+    }
+    __syncthreads();
+    if (!go) return;
+    const double rz = gadj_precond(p, y, sh);
+    for (int64_t j = threadIdx.x; j < p.n; j += kTail) p.d[j] = p.z[j];
+    if (threadIdx.x == 0) S->rho = rz;
+}
+
+// The n-sized part of one CG step of every iterating problem, after the
+// grouped single-pass kernel left |q|^2 and A^T q as per-item partials.
+// The restart's inner loop ends when |res|/|g| < tol, |q| = 0 or after
+// max_iters steps; the problem then waits for its next restart.
+__global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P, double tol, int max_iters) {
+    const GProb& p = P[blockIdx.x];
+    GState* S = p.st;
+    if (S->phase != kPhaseIterate) return;
+    extern __shared__ double y[];
+    __shared__ double sh[kTail / 64];
+    __shared__ int go;
+    const double rho = S->rho;
+    const int iters = S->iters;
+    const int64_t n = p.n;
+    double q2 = 0.0;
+    for (int k = threadIdx.x; k < p.nitems; k += kTail) q2 += p.sumsq[k];
+    q2 = gsum(q2, sh);
+    const double alpha = q2 > 0.0 ? rho / q2 : 0.0;
+    double rr = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) {
+        double t = 0.0;
+        for (int k = 0; k < p.nitems; ++k) t += static_cast<double>(p.tpart[static_cast<int64_t>(k) * n + j]);
+        p.w[j] += alpha * p.d[j];
+        const double rj = p.res[j] - alpha * t;
+        p.res[j] = rj;
+        rr += rj * rj;
+    }
+    rr = gsum(rr, sh);
+    if (threadIdx.x == 0) {
+        S->completed += 1;
+        S->iters = iters + 1;
+        S->resnorm = sqrt(rr);
+        const double rel = S->resnorm / S->gnorm;
+        const bool end = rel < tol || q2 <= 0.0 || iters + 1 >= max_iters;
+        go = end ? 0 : 1;
+        if (end) {
+            S->phase = kPhaseRestart;
+            S->skip_iter = 1;
+            S->skip_restart = 0;
+        }
+    }
+    __syncthreads();
+    if (!go) return;
+    const double rz = gadj_precond(p, y, sh);
+    const double beta = rho != 0.0 ? rz / rho : 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += kTail) p.d[j] = p.z[j] + beta * p.d[j];
+    if (threadIdx.x == 0) S->rho = rz;
+}
+
+size_t a256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr int kNVecs = 5;  // n-vectors per problem in the scratch
+
+struct GPlan {
+    int g = 0, threads = 256, rounds = 0;
+    int64_t nmax = 0, tiles_per_item = 1;
+    int nitems = 0;
+    std::vector<GroupItem> items;
+    std::vector<int> prob_items;
+    size_t probs = 0, gprobs = 0, items_off = 0, states = 0, gemv_sc = 0, gemv_bytes = 0, grad_sc = 0,
+           grad_bytes = 0, total = 0;
+    std::vector<size_t> vec_at, sumsq_at, tpart_at;
+};
+
+void grouped_validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr) {
+    require(o.a_type == TOMOS_F32 || o.a_type == TOMOS_FP32_BF16 || o.a_type == TOMOS_FP32_16,
+            "adjoint (grouped): a_type must be F32, FP32_16 or FP32_BF16");
+    require(o.mv_tier == TOMOS_TIER_DEFAULT, "adjoint (grouped): iterations run at the DEFAULT tier");
+    require(o.precond_type == TOMOS_F64, "adjoint (grouped): precond_type must be F64 (R^-1 required)");
+    require(o.max_iters >= 1 && o.restarts >= 1, "adjoint (grouped): need max_iters >= 1 and restarts >= 1");
+    const int vec = o.a_type == TOMOS_F32 ? 4 : 8;
+    const bool packed = o.a_type != TOMOS_F32;
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_lsq_problem& q = pr[i];
+        require(q.m >= q.n && q.n >= 0 && q.n <= 2048, "adjoint (grouped): need m >= n and n <= 2048");
+        if (q.n == 0) continue;
+        require(q.rinv != nullptr && q.ldr >= q.n, "adjoint (grouped): R^-1 missing or ldr < n");
+        require(q.lda >= q.m && q.lda % vec == 0,
+                "adjoint (grouped): lda must be >= m and a multiple of the 16-byte row vector");
+        require(reinterpret_cast<uintptr_t>(q.a) % 16 == 0, "adjoint (grouped): A must be 16-byte aligned");
+        if (packed && o.restart_tier == TOMOS_TIER_EXACT) require(q.wr != nullptr, "adjoint (grouped): EXACT needs wr");
+    }
+}
+
+GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr) {
+    GPlan L;
+    for (int i = 0; i < nprob; ++i) L.nmax = std::max(L.nmax, pr[i].n);
+    int g = 0, nt = 0, bpc = 0;
+    lsqr_fused_config_get(o.a_type, L.nmax, &g, &bpc, &nt);
+    if (nt <= 0) nt = 256;
+    if (g <= 0 || fused_rounds(g, nt, std::max<int64_t>(L.nmax, 1)) == 0) {
+        const FusedShape d = lsqr_fused_default_shape(std::max<int64_t>(L.nmax, 1));
+        g = d.g;
+        nt = d.threads;
+    }
+    L.g = g;
+    L.threads = nt;
+    L.rounds = fused_rounds(g, nt, std::max<int64_t>(L.nmax, 1));
+    require(L.rounds > 0, "adjoint (grouped): n too large for the single-pass kernel (max 2048)");
+    const int64_t tr = fused_tile_rows(o.a_type, g);
+    int64_t tiles_total = 0;
+    for (int i = 0; i < nprob; ++i) tiles_total += pr[i].n > 0 ? ceil_div(std::max<int64_t>(pr[i].m, 1), tr) : 0;
+    const int64_t resident = static_cast<int64_t>(grouped_fused_occupancy(o.a_type, g, nt, L.rounds)) * cu_count();
+    L.tiles_per_item = std::max<int64_t>(1, ceil_div(std::max<int64_t>(tiles_total, 1), 4 * resident));
+    L.prob_items.resize(static_cast<size_t>(nprob));
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t tiles = pr[i].n > 0 ? ceil_div(std::max<int64_t>(pr[i].m, 1), tr) : 0;
+        int k = 0;
+        for (int64_t t = 0; t < tiles; t += L.tiles_per_item, ++k)
+            L.items.push_back(GroupItem{i, k, t, std::min(tiles, t + L.tiles_per_item)});
+        L.prob_items[static_cast<size_t>(i)] = k;
+    }
+    L.nitems = static_cast<int>(L.items.size());
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        const size_t at = off;
+        off = a256(off + b);
+        return at;
+    };
+    L.probs = take(sizeof(GProb) * static_cast<size_t>(std::max(nprob, 1)));
+    L.gprobs = take(sizeof(GroupProb) * static_cast<size_t>(std::max(nprob, 1)));
+    L.items_off = take(sizeof(GroupItem) * static_cast<size_t>(std::max(L.nitems, 1)));
+    L.states = take(sizeof(GState) * static_cast<size_t>(std::max(nprob, 1)));
+    for (int i = 0; i < nprob; ++i) {
+        const size_t nb = a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8);
+        // res, z, d, rfull, w (FP64, n), q (FP32, m), q64 (FP64, m)
+        L.vec_at.push_back(take(kNVecs * nb + a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 4) +
+                                a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 8)));
+        L.sumsq_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[static_cast<size_t>(i)], 1)) * 8));
+        L.tpart_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[static_cast<size_t>(i)], 1)) *
+                                  static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 4));
+        for (int tr2 = 0; tr2 < 2; ++tr2)
+            L.gemv_bytes = std::max(L.gemv_bytes, gemv_scratch_bytes(o.a_type, o.restart_tier, tr2, pr[i].m, pr[i].n));
+        L.grad_bytes = std::max(L.grad_bytes, grad_scratch_bytes(o.a_type, o.restart_tier, pr[i].m, pr[i].n));
+    }
+    L.gemv_sc = take(std::max<size_t>(L.gemv_bytes, 1));
+    L.grad_sc = take(std::max<size_t>(L.grad_bytes, 1));
+    L.total = off;
+    return L;
+}
+
+// Everything a captured segment depends on (grids, LDS, scalars, scratch).
+struct SegKey {
+    int device, kind, nprob, nitems, g, threads, rounds, a_type, restart_tier, max_iters, restarts;
+    size_t lds;
+    double tol;
+    const void* scratch;
+    bool operator==(const SegKey& k) const {
+        return device == k.device && kind == k.kind && nprob == k.nprob && nitems == k.nitems && g == k.g &&
+               threads == k.threads && rounds == k.rounds && a_type == k.a_type && restart_tier == k.restart_tier &&
+               max_iters == k.max_iters && restarts == k.restarts && lds == k.lds && tol == k.tol &&
+               scratch == k.scratch;
+    }
+};
+
+struct SegCache {
+    struct Entry {
+        SegKey key;
+        hipGraphExec_t exec;
+    };
+    std::mutex mu;
+    std::vector<Entry> v;
+    static constexpr int kMax = 8;
+    hipGraphExec_t find(const SegKey& k) {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const Entry& e : v)
+            if (e.key == k) return e.exec;
+        return nullptr;
+    }
+    // Every call synchronizes before it returns, so an evicted graph is idle.
+    void insert(const SegKey& k, hipGraphExec_t exec) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (static_cast<int>(v.size()) >= kMax) {
+            (void)hipGraphExecDestroy(v.front().exec);
+            v.erase(v.begin());
+        }
+        v.push_back(Entry{k, exec});
+    }
+};
+SegCache& seg_cache() {
+    static SegCache c;
+    return c;
+}
+
+struct Grouped {
+    tomos_lsqr_options o{};
+    int nprob = 0;
+    const tomos_lsq_problem* pr = nullptr;
+    GPlan L;
+    char* sc = nullptr;
+    hipStream_t s = nullptr;
+    std::vector<double*> q64;    // per problem: A w of the restart (FP64, m)
+    std::vector<double*> wbuf;   // per problem: w, accumulated in the scratch (FP64, n)
+    std::vector<double*> rfull;  // per problem: A^T A w of the restart (FP64, n)
+
+    GProb* dprobs() const { return reinterpret_cast<GProb*>(sc + L.probs); }
+    GroupProb* dgprobs() const { return reinterpret_cast<GroupProb*>(sc + L.gprobs); }
+    GroupItem* ditems() const { return reinterpret_cast<GroupItem*>(sc + L.items_off); }
+    GState* states() const { return reinterpret_cast<GState*>(sc + L.states); }
+    size_t lds() const { return static_cast<size_t>(std::max<int64_t>(L.nmax, 1)) * 8; }
+
+    void upload(const double* const* g) {
+        std::vector<GProb> bp(static_cast<size_t>(nprob));
+        std::vector<GroupProb> gp(static_cast<size_t>(nprob));
+        q64.assign(static_cast<size_t>(nprob), nullptr);
+        wbuf.assign(static_cast<size_t>(nprob), nullptr);
+        rfull.assign(static_cast<size_t>(nprob), nullptr);
+        for (int i = 0; i < nprob; ++i) {
+            const tomos_lsq_problem& p = pr[i];
+            GProb& q = bp[static_cast<size_t>(i)];
+            const size_t nb = a256(static_cast<size_t>(std::max<int64_t>(p.n, 1)) * 8);
+            char* vb = sc + L.vec_at[static_cast<size_t>(i)];
+            q.m = p.m;
+            q.n = p.n;
+            q.ldr = p.ldr;
+            q.g = g[i];
+            q.rinv = p.rinv;
+            q.res = reinterpret_cast<double*>(vb);
+            q.z = reinterpret_cast<double*>(vb + nb);
+            q.d = reinterpret_cast<double*>(vb + 2 * nb);
+            q.rfull = reinterpret_cast<double*>(vb + 3 * nb);
+            q.w = reinterpret_cast<double*>(vb + 4 * nb);
+            float* qf = reinterpret_cast<float*>(vb + kNVecs * nb);
+            q64[static_cast<size_t>(i)] = reinterpret_cast<double*>(
+                vb + kNVecs * nb + a256(static_cast<size_t>(std::max<int64_t>(p.m, 1)) * 4));
+            wbuf[static_cast<size_t>(i)] = q.w;
+            rfull[static_cast<size_t>(i)] = q.rfull;
+            q.st = states() + i;
+            q.nitems = L.prob_items[static_cast<size_t>(i)];
+            q.sumsq = reinterpret_cast<double*>(sc + L.sumsq_at[static_cast<size_t>(i)]);
+            q.tpart = reinterpret_cast<float*>(sc + L.tpart_at[static_cast<size_t>(i)]);
+            GroupProb& r = gp[static_cast<size_t>(i)];
+            r.m = p.m;
+            r.n = p.n;
+            r.lda = p.lda;
+            r.a = p.a;
+            r.p = q.d;
+            r.u = qf;
+            r.u_coef = &q.st->zero;
+            r.skip = &q.st->skip_iter;
+            r.sumsq = q.sumsq;
+            r.tpart = q.tpart;
+            r.rounds = fused_rounds(L.g, L.threads, std::max<int64_t>(p.n, 1));
+        }
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dprobs(), bp.data(), sizeof(GProb) * bp.size(), hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipMemcpyAsync(dgprobs(), gp.data(), sizeof(GroupProb) * gp.size(), hipMemcpyHostToDevice, s));
+        if (L.nitems)
+            TOMOS_HIP_CHECK(hipMemcpyAsync(ditems(), L.items.data(), sizeof(GroupItem) * L.items.size(),
+                                           hipMemcpyHostToDevice, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors must outlive the copies
+    }
+
+    // Restart sequence: per problem the exact-tier pair q64 = A w,
+    // rfull = A^T q64 (no-ops unless that problem restarts with w != 0),
+    // then the restart block of every problem.
+    void restart_seq() {
+        void* gsc = sc + L.gemv_sc;
+        for (int i = 0; i < nprob; ++i) {
+            const tomos_lsq_problem& p = pr[i];
+            if (p.n == 0) continue;
+            GState* st = states() + i;
+            Coef c;
+            c.alpha = 1.0;
+            c.beta = 0.0;
+            c.skip = &st->skip_restart;
+            const size_t k = static_cast<size_t>(i);
+            gemv(o.a_type, o.restart_tier, TOMOS_NO_TRANSPOSE, c, p.m, p.n, p.a, p.wr, p.lda, TOMOS_F64, wbuf[k],
+                 TOMOS_F64, q64[k], gsc, L.gemv_bytes, s);
+            gemv(o.a_type, o.restart_tier, TOMOS_TRANSPOSE, c, p.m, p.n, p.a, p.wr, p.lda, TOMOS_F64, q64[k],
+                 TOMOS_F64, rfull[k], gsc, L.gemv_bytes, s);
+        }
+        k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.restarts);
+        TOMOS_LAUNCH_CHECK("k_gadj_restart");
+    }
+    void iter_seq() {
+        for (int it = 0; it < kSegIters; ++it) {
+            if (L.nitems) grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s);
+            k_gadj_step<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.max_iters);
+        }
+        TOMOS_LAUNCH_CHECK("k_gadj_step");
+    }
+
+    SegKey key(int kind) const {
+        SegKey k;
+        TOMOS_HIP_CHECK(hipGetDevice(&k.device));
+        k.kind = kind;
+        k.nprob = nprob;
+        k.nitems = L.nitems;
+        k.g = L.g;
+        k.threads = L.threads;
+        k.rounds = L.rounds;
+        k.a_type = o.a_type;
+        k.restart_tier = o.restart_tier;
+        k.max_iters = o.max_iters;
+        k.restarts = o.restarts;
+        k.lds = lds();
+        k.tol = o.tol;
+        k.scratch = sc;
+        return k;
+    }
+    hipGraphExec_t graph(int kind) {
+        const SegKey k = key(kind);
+        hipGraphExec_t exec = seg_cache().find(k);
+        if (exec) return exec;
+        hipGraph_t gr = nullptr;
+        TOMOS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        try {
+            if (kind == 0) restart_seq();
+            else iter_seq();
+        } catch (...) {
+            hipGraph_t g2 = nullptr;
+            (void)hipStreamEndCapture(s, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+        }
+        TOMOS_HIP_CHECK(hipStreamEndCapture(s, &gr));
+        const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        TOMOS_HIP_CHECK(e);
+        seg_cache().insert(k, exec);
+        return exec;
+    }
+
+    void states_to_host(std::vector<GState>& h) {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), states(), sizeof(GState) * h.size(), hipMemcpyDeviceToHost, s));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    void run(const double* const* g, double* const* w, int* iterations) {
+        upload(g);
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gadj_restart),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds())));
+        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gadj_step),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds())));
+        k_gadj_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
+        k_gadj_zero_w<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
+        TOMOS_LAUNCH_CHECK("adjoint (grouped) set-up");
+        hipGraphExec_t rexec = graph(0);
+        hipGraphExec_t iexec = graph(1);
+        std::vector<GState> h(static_cast<size_t>(nprob));
+        // a restart cycle takes one restart launch and at most
+        // ceil(max_iters / kSegIters) + 1 rounds: a bound on the rounds
+        const int64_t max_rounds =
+            static_cast<int64_t>(o.restarts + 1) * (ceil_div(static_cast<int64_t>(o.max_iters), kSegIters) + 1) + 2;
+        bool restart_pending = true;  // every problem starts with its first restart
+        bool all_done = false;
+        for (int64_t round = 0; round < max_rounds && !all_done; ++round) {
+            if (restart_pending) TOMOS_HIP_CHECK(hipGraphLaunch(rexec, s));
+            TOMOS_HIP_CHECK(hipGraphLaunch(iexec, s));
+            states_to_host(h);
+            all_done = true;
+            restart_pending = false;
+            for (const GState& b : h) {
+                all_done = all_done && b.phase == kPhaseDone;
+                restart_pending = restart_pending || b.phase == kPhaseRestart;
+            }
+        }
+        if (!all_done) throw Error(kErrInternal, "adjoint (grouped): a problem did not finish within the round bound");
+        for (int i = 0; i < nprob; ++i) {
+            if (pr[i].n > 0)
+                TOMOS_HIP_CHECK(hipMemcpyAsync(w[i], wbuf[static_cast<size_t>(i)], static_cast<size_t>(pr[i].n) * 8,
+                                               hipMemcpyDeviceToDevice, s));
+            if (iterations) iterations[i] = h[static_cast<size_t>(i)].completed;
+        }
+    }
+};
+
+}  // namespace
+
+size_t adjoint_grouped_scratch_bytes(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* probs) {
+    grouped_validate(o, nprob, probs);
+    return grouped_plan(o, nprob, probs).total;
+}
+
+void adjoint_grouped(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* probs, const double* const* g,
+                     double* const* w, double* const* grad_b, float* const* grad_a, const int64_t* ldg,
+                     void* scratch, size_t scratch_bytes, int* iterations, hipStream_t stream) {
+    grouped_validate(o, nprob, probs);
+    if (nprob == 0) return;
+    Grouped G;
+    G.o = o;
+    G.nprob = nprob;
+    G.pr = probs;
+    G.L = grouped_plan(o, nprob, probs);
+    require(scratch != nullptr && scratch_bytes >= G.L.total, "tomos_lsq_adjoint_batched: scratch missing or too small");
+    G.sc = static_cast<char*>(scratch);
+    // a graph cannot be captured from the legacy NULL stream: a private
+    // stream (default priority, ordered after earlier NULL-stream work)
+    hipStream_t own = nullptr;
+    if (stream == nullptr) TOMOS_HIP_CHECK(hipStreamCreate(&own));
+    G.s = own ? own : stream;
+    try {
+        G.run(g, w, iterations);
+        for (int i = 0; i < nprob; ++i) {
+            const tomos_lsq_problem& q = probs[i];
+            lsq_grad(o.a_type, o.restart_tier, q.m, q.n, q.a, q.wr, q.lda, q.b, q.x, w[i], grad_b[i],
+                     grad_a ? grad_a[i] : nullptr, ldg ? ldg[i] : q.m, G.sc + G.L.grad_sc, G.L.grad_bytes, G.s);
+        }
+        TOMOS_HIP_CHECK(hipStreamSynchronize(G.s));
+    } catch (...) {
+        if (own) {
+            (void)hipStreamSynchronize(own);
+            (void)hipStreamDestroy(own);
+        }
+        throw;
+    }
+    if (own) TOMOS_HIP_CHECK(hipStreamDestroy(own));
+}
+
+}  // namespace tomos_hip
