@@ -184,7 +184,10 @@ __device__ __forceinline__ void tile_barrier() {
 // lane's 16 bytes at its own slot: base + 16 * lane), and the next tile's
 // loads are issued before this tile's arithmetic and barriers, so HBM
 // requests stay in flight throughout (one tile buffer: NW x ROUNDS KB).
-template <int FMT, int G, int ROUNDS, int NT, bool NTL, bool STAGE = false>
+// GROUPED only tells the grouped kernel's instances apart from the
+// one-problem kernel's (the HIP host pass rejects one staged instance
+// reached from both kinds of kernel); it changes nothing in the code.
+template <int FMT, int G, int ROUNDS, int NT, bool NTL, bool STAGE = false, bool GROUPED = false>
 __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __restrict__ a,
                                            int64_t lda, const void* __restrict__ p, bool p64,
                                            void* __restrict__ u, bool u64, double cu, int64_t tile0,
@@ -565,7 +568,10 @@ __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
 
 // Body of a grouped work item with the fewest column rounds R (G, 2G, 4G,
 // ... up to RMAX) that cover the item's problem (P.rounds): a narrow
-// problem does not pay for the widest problem's rounds.
+// problem does not pay for the widest problem's rounds.  The widest form
+// stages its tiles through LDS by DMA when G == 8 (as the one-problem
+// kernel does); only it does, because each form's stage buffer is LDS of
+// its own.
 template <int FMT, int G, int R, int RMAX, int NT>
 __device__ __forceinline__ void grouped_item(const GroupProb& P, const GroupItem& it) {
     if constexpr (R < RMAX) {
@@ -577,9 +583,9 @@ __device__ __forceinline__ void grouped_item(const GroupProb& P, const GroupItem
         }
         grouped_item<FMT, G, (2 * R < RMAX ? 2 * R : RMAX), RMAX, NT>(P, it);
     } else {
-        fused_body<FMT, G, RMAX, NT, false>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef, it.tile0,
-                                            it.tile1, P.sumsq + it.slot,
-                                            P.tpart + static_cast<int64_t>(it.slot) * P.n);
+        fused_body<FMT, G, RMAX, NT, false, G == 8, true>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef,
+                                                    it.tile0, it.tile1, P.sumsq + it.slot,
+                                                    P.tpart + static_cast<int64_t>(it.slot) * P.n);
     }
 }
 
