@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libtomos_oracle.so")
+# TOMOS_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of tests/run_sanitizers.sh)
+_LIB_PATH = os.environ.get("TOMOS_ORACLE_LIB") or os.path.join(_HERE, "libtomos_oracle.so")
 _lib = None
 
 i64 = ctypes.c_int64

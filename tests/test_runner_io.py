@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-RUNNER = os.path.join(ROOT, "tomos_amd", "tomos_lsqr_runner")
+# TOMOS_RUNNER selects another build (the sanitizer build of tests/run_sanitizers.sh)
+RUNNER = os.environ.get("TOMOS_RUNNER") or os.path.join(ROOT, "tomos_amd", "tomos_lsqr_runner")
 
 pytestmark = pytest.mark.skipif(not os.path.exists(RUNNER), reason="runner not built (__graft_entry__.build())")
 
