@@ -70,6 +70,9 @@ def parse():
     p.add_argument("--stream-loads", type=int, default=0, choices=(0, 1, 2),
                    help="matrix load policy: 0 auto (non-temporal above 192 MiB per pass), "
                         "1 always non-temporal, 2 always cacheable")
+    p.add_argument("--qr-gemm-only", action="store_true",
+                   help="only the sketch-QR preconditioner at 4000 x 1000 and the packed GEMMs at --gemm^3 "
+                        "(split and hi-only, FP32_BF16), two calls each (for counter runs); prints their timings")
     p.add_argument("--kernels-only", action="store_true",
                    help="only the hi-only GEMV-N at the full shape and a copy of the same bytes "
                         "(for hardware-counter runs); prints their timings")
@@ -375,6 +378,53 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
                         "backward_threads_ms: the same over %d host threads / HIP streams" % (adj_tol, aw)}
 
 
+def qr_gemm_only(args, C, DeviceBuffer):
+    """The sketch-QR preconditioner (FP64 Householder TSQR + R^-1) of a
+    4000 x 1000 sketch with column scales 1 .. 1e-6, and the FP32_BF16
+    GEMMs at G^3 (split with packed B, hi-only with FP32 B), two calls each."""
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    for e in (ev0, ev1):
+        C.call("tomos_event_create", ctypes.byref(e))
+    f = ctypes.c_float()
+    out = {}
+    k, n = 4000, 1000
+    A0 = np.asfortranarray(np.random.default_rng(1).standard_normal((k, n)) * np.logspace(0, -6, n))
+    src = DeviceBuffer.from_numpy(A0)
+    A = DeviceBuffer(k * n * 8)
+    ts = []
+    for _ in range(2):
+        C.call("tomos_memcpy_dtod", A.ptr, src.ptr, k * n * 8, None)
+        C.call("tomos_event_record", ev0, None)
+        C.call("tomos_sketch_qr_precond_f64", k, n, A.ptr, k, None)
+        C.call("tomos_event_record", ev1, None)
+        C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+        ts.append(round(f.value, 3))
+    out["sketch_qr_precond_4000x1000_ms"] = ts
+    G = args.gemm
+    if G > 0:
+        GA, GB = DeviceBuffer(G * G * 4), DeviceBuffer(G * G * 4)
+        C.call("tomos_fill_normal_f32", GA.ptr, G, G, G, 11, None, 0, 0, None)
+        C.call("tomos_fill_normal_f32", GB.ptr, G, G, G, 12, None, 0, 0, None)
+        P = [DeviceBuffer(G * G * 2) for _ in range(4)]
+        Cg = DeviceBuffer(G * G * 4)
+        C.call("tomos_pack", C.PACK_FP32_BF16, G, G, GA.ptr, G, P[0].ptr, P[1].ptr, G, None)
+        C.call("tomos_pack", C.PACK_FP32_BF16, G, G, GB.ptr, G, P[2].ptr, P[3].ptr, G, None)
+        for name, fn in (
+            ("gemm_split_ms", lambda: C.call("tomos_gemm_split", C.FP32_BF16, G, G, G, 1.0, P[0].ptr, P[1].ptr, G,
+                                             P[2].ptr, P[3].ptr, G, 0.0, Cg.ptr, G, None, 0, None)),
+            ("gemm_hi_only_ms", lambda: C.call("tomos_gemm", C.FP32_BF16, C.TIER_DEFAULT, 0, 0, 1.0, G, G, G,
+                                               P[0].ptr, P[1].ptr, G, GB.ptr, G, 0.0, Cg.ptr, G, None))):
+            ts = []
+            for _ in range(2):
+                C.call("tomos_event_record", ev0, None)
+                fn()
+                C.call("tomos_event_record", ev1, None)
+                C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+                ts.append(round(f.value, 3))
+            out[name] = ts
+    print(json.dumps({"qr_gemm_only": out}), flush=True)
+
+
 def kernels_only(args, C, DeviceBuffer, m, n, wh):
     """GEMV-N (hi-only FP32_BF16, y = A x) and tomos_copy_kernel over the
     same 2*m*n bytes, each launched --gemv-reps times: the copy's counter
@@ -576,6 +626,8 @@ def main():
            ctypes.byref(mem))
     m, n, s = args.m, args.n, args.s
     log("device: %d CUs, %.0f GB; workload %d x %d, s=%d" % (cu.value, mem.value / 1e9, m, n, s))
+    if args.qr_gemm_only:
+        return qr_gemm_only(args, C, DeviceBuffer)
 
     # ---- synthetic data, resident in HBM ------------------------------------
     d = np.logspace(0, -args.kappa_exp, n).astype(np.float32)
