@@ -379,8 +379,9 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
 
 
 def qr_gemm_only(args, C, DeviceBuffer):
-    """The sketch-QR preconditioner (FP64 Householder TSQR + R^-1) of a
-    4000 x 1000 sketch with column scales 1 .. 1e-6, and the FP32_BF16
+    """The sketch-QR preconditioner (FP64 QR + R^-1; CholeskyQR2 panels, then
+    the column-by-column Householder panels) of a 4000 x 1000 sketch with
+    column scales 1 .. 1e-6, three calls each, and the FP32_BF16
     GEMMs at G^3 (split with packed B, hi-only with FP32 B), two calls each."""
     ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
     for e in (ev0, ev1):
@@ -391,15 +392,18 @@ def qr_gemm_only(args, C, DeviceBuffer):
     A0 = np.asfortranarray(np.random.default_rng(1).standard_normal((k, n)) * np.logspace(0, -6, n))
     src = DeviceBuffer.from_numpy(A0)
     A = DeviceBuffer(k * n * 8)
-    ts = []
-    for _ in range(2):
-        C.call("tomos_memcpy_dtod", A.ptr, src.ptr, k * n * 8, None)
-        C.call("tomos_event_record", ev0, None)
-        C.call("tomos_sketch_qr_precond_f64", k, n, A.ptr, k, None)
-        C.call("tomos_event_record", ev1, None)
-        C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
-        ts.append(round(f.value, 3))
-    out["sketch_qr_precond_4000x1000_ms"] = ts
+    for mode, key in ((0, "sketch_qr_precond_4000x1000_ms"), (1, "sketch_qr_precond_householder_panels_4000x1000_ms")):
+        C.call("tomos_qr_panels_set", mode)
+        ts = []
+        for _ in range(3):
+            C.call("tomos_memcpy_dtod", A.ptr, src.ptr, k * n * 8, None)
+            C.call("tomos_event_record", ev0, None)
+            C.call("tomos_sketch_qr_precond_f64", k, n, A.ptr, k, None)
+            C.call("tomos_event_record", ev1, None)
+            C.call("tomos_event_elapsed_ms", ctypes.byref(f), ev0, ev1)
+            ts.append(round(f.value, 3))
+        out[key] = ts
+    C.call("tomos_qr_panels_set", 0)
     G = args.gemm
     if G > 0:
         GA, GB = DeviceBuffer(G * G * 4), DeviceBuffer(G * G * 4)

@@ -394,6 +394,15 @@ int tomos_geqrf_r_f64(int64_t k, int64_t n, double* a, int64_t lda, double* r, i
  * Replaces dtrtri('U','N') (sketch_qr.cpp:59-63).  Synchronous. */
 int tomos_trtri_upper_f64(int64_t n, double* r, int64_t ldr, tomos_stream_t stream);
 
+/* Panel factorization of the QR entry points above: 0 (default) = each
+ * 32-column panel by CholeskyQR2 with its Householder vectors
+ * reconstructed (no column-by-column pass over the rows), and the whole
+ * factorization redone with column-by-column Householder panels for a
+ * sketch where a panel is too ill-conditioned for it; 1 = the
+ * column-by-column Householder panels only.  Process-wide; returns the
+ * status.  No reference counterpart (the reference calls LAPACK dgeqrf). */
+int tomos_qr_panels_set(int mode);
+
 /* generate_sketch_qr_precond (src/core/precond/sketch_qr.cpp:68-122): the
  * k x n FP64 sketch (lda) is factored and its leading n x n block is
  * overwritten with R^-1 (upper), the rest of the block zeroed.

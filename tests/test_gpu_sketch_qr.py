@@ -1,4 +1,12 @@
-"""GPU CountSketch (bit-exact indices), Householder QR and triangular inverse."""
+"""GPU CountSketch (bit-exact indices), Householder QR and triangular inverse.
+
+The QR tests run under both panel factorizations (tomos_qr_panels_set):
+0 = CholeskyQR2 panels with reconstructed Householder vectors (falling
+back to the column-by-column panels for an ill-conditioned panel), 1 = the
+column-by-column Householder panels (TSQR for k <= 8192, the wavefront
+kernel above).  R is checked against numpy's (LAPACK) QR up to row signs."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -7,6 +15,17 @@ from tomos_amd import _capi as C
 from tomos_amd._capi import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
+
+PANELS = [0, 1]
+
+
+@contextlib.contextmanager
+def panels(mode):
+    C.call("tomos_qr_panels_set", mode)
+    try:
+        yield
+    finally:
+        C.call("tomos_qr_panels_set", 0)
 
 
 @pytest.mark.parametrize("m,s", [(1000000, 4000), (40, 30), (12345, 7)])
@@ -60,10 +79,12 @@ def test_norm_preservation(capi):
 
 
 @pytest.mark.parametrize("k,n", [(4000, 1000), (30, 5), (257, 256), (64, 64), (500, 77), (33, 33), (1000, 320)])
-def test_qr_r_matches_numpy(capi, k, n):
+@pytest.mark.parametrize("mode", PANELS)
+def test_qr_r_matches_numpy(capi, k, n, mode):
     rng = np.random.default_rng(k + n)
     A = rng.standard_normal((k, n)) * np.logspace(0, -6, n)
-    R = H.geqrf_r(A)
+    with panels(mode):
+        R = H.geqrf_r(A)
     Rn = np.linalg.qr(A, mode="r")
     sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
     assert np.allclose(np.tril(R, -1), 0)
@@ -71,13 +92,19 @@ def test_qr_r_matches_numpy(capi, k, n):
     assert err < 1e-12, err
 
 
-def test_qr_zero_column(capi):
-    """A zero column gives an inactive reflector (tau = 0, R(c,c) = 0), as dlarfg."""
+@pytest.mark.parametrize("mode", PANELS)
+def test_qr_zero_column(capi, mode):
+    """A zero column gives an inactive reflector (tau = 0, R(c,c) = 0), as
+    dlarfg; under the CholeskyQR2 panels its Gram matrix is singular and the
+    factorization is redone with the Householder panels (same bits)."""
     rng = np.random.default_rng(5)
     A = rng.standard_normal((100, 40))
     A[:, 5] = 0.0
     A[:, 33] = 0.0
-    R = H.geqrf_r(A)
+    with panels(mode):
+        R = H.geqrf_r(A)
+    with panels(1):
+        assert np.array_equal(R, H.geqrf_r(A))
     Rn = np.linalg.qr(A, mode="r")
     sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
     sgn[sgn == 0] = 1.0
@@ -111,9 +138,17 @@ def test_errors(capi):
     assert rc == 6  # singular R (sketch_qr.cpp:59-63)
 
 
-def test_batched_precond_matches_single(capi):
+@pytest.mark.parametrize("mode", PANELS)
+def test_batched_precond_matches_single(capi, mode):
     """Uneven sketches factored together give bit-identical R^-1 to one-at-a-time calls."""
-    import ctypes
+    C.call("tomos_qr_panels_set", mode)
+    try:
+        _batched_precond_matches_single()
+    finally:
+        C.call("tomos_qr_panels_set", 0)
+
+
+def _batched_precond_matches_single():
     rng = np.random.default_rng(17)
     shapes = [(256, 64), (900, 200), (40, 33), (2048, 512), (130, 1)]
     mats = [rng.standard_normal((k, n)) * np.logspace(0, -4, n) for k, n in shapes]
@@ -167,13 +202,23 @@ def test_apply_saturated_fp32_16_falls_back(capi, orc):
     assert np.abs(Y[fin] - ref[fin]).max() <= 2e-6 * np.abs(ref[fin]).max()
 
 
+@pytest.mark.parametrize("mode", PANELS)
 @pytest.mark.parametrize("k,n", [(12000, 300), (9000, 70)])
-def test_qr_tall_wavefront_path(capi, k, n):
-    """k above 8192 takes the panel wavefront factorization (a block per
-    panel column, per-column flags between blocks) instead of TSQR: R
-    against numpy's QR up to row signs, and R^-1 from the sketch-QR entry
-    point inverts it."""
-    print("wavefront qr k=%d n=%d" % (k, n), flush=True)
+def test_qr_tall_wavefront_path(capi, k, n, mode):
+    """k above 8192: with the Householder panels the panel wavefront
+    factorization (a block per panel column, per-column flags between
+    blocks) instead of TSQR; the CholeskyQR2 panels take any k.  R against
+    numpy's QR up to row signs, and R^-1 from the sketch-QR entry point
+    inverts it."""
+    print("tall qr k=%d n=%d panels=%d" % (k, n, mode), flush=True)
+    C.call("tomos_qr_panels_set", mode)
+    try:
+        _tall_qr(k, n)
+    finally:
+        C.call("tomos_qr_panels_set", 0)
+
+
+def _tall_qr(k, n):
     rng = np.random.default_rng(k + n)
     A = rng.standard_normal((k, n)) * np.logspace(0, -5, n)
     R = H.geqrf_r(A)
@@ -188,9 +233,18 @@ def test_qr_tall_wavefront_path(capi, k, n):
     assert np.abs(R @ Ri - np.eye(n)).max() < 1e-8
 
 
-def test_qr_tall_wavefront_batched_matches_single(capi):
+@pytest.mark.parametrize("mode", PANELS)
+def test_qr_tall_wavefront_batched_matches_single(capi, mode):
     """Two problems above the TSQR row limit in one batched call give the
     single calls' R^-1 bit for bit."""
+    C.call("tomos_qr_panels_set", mode)
+    try:
+        _tall_batched()
+    finally:
+        C.call("tomos_qr_panels_set", 0)
+
+
+def _tall_batched():
     rng = np.random.default_rng(77)
     shapes = [(12000, 96), (8500, 40)]
     mats = [rng.standard_normal(s) * np.logspace(0, -4, s[1]) for s in shapes]
@@ -204,3 +258,48 @@ def test_qr_tall_wavefront_batched_matches_single(capi):
     C.call("tomos_sketch_qr_precond_batched_f64", 2, qp, None, 0, None)
     for A, b, s in zip(mats, bufs, singles):
         assert np.array_equal(b.to_numpy(np.float64, A.shape)[:A.shape[1]], s)
+
+
+def test_cholqr_panels_taken_and_fallback(capi):
+    """Which panels ran: a well-conditioned sketch is factored by the
+    CholeskyQR2 panels (its R differs in the last bits from the Householder
+    panels' R, both within 1e-13 of LAPACK's); a sketch with two columns of
+    one panel equal to 1e-9 (cond ~1e9, beyond two Gram passes) is redone
+    with the Householder panels, bit for bit their result."""
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((3000, 200)) * np.logspace(0, -3, 200)
+    Rn = np.linalg.qr(A, mode="r")
+    R0 = H.geqrf_r(A)
+    with panels(1):
+        R1 = H.geqrf_r(A)
+    assert not np.array_equal(R0, R1)
+    for R in (R0, R1):
+        sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
+        assert np.abs(R - sgn[:, None] * Rn).max() < 1e-13 * np.abs(Rn).max()
+    B = A.copy()
+    B[:, 40] = B[:, 41] + 1e-9 * rng.standard_normal(3000) * np.abs(B[:, 41]).max()
+    Rb = H.geqrf_r(B)
+    with panels(1):
+        assert np.array_equal(Rb, H.geqrf_r(B))
+    # cond(B) ~ 1e9 determines R only to ~1e-7 (LAPACK's differs by that
+    # much); what a backward-stable QR guarantees is R^T R = B^T B to rounding
+    gram = B.T @ B
+    assert np.abs(Rb.T @ Rb - gram).max() < 1e-13 * np.abs(gram).max()
+
+
+@pytest.mark.parametrize("k,n,cond", [(4000, 1000, 1e6), (2000, 512, 1e10), (700, 64, 1e14)])
+def test_cholqr_graded_columns(capi, k, n, cond):
+    """Column scales spread over `cond` (the C4 / C5 sketches are of this
+    kind): Cholesky is invariant to column scaling, so each panel stays
+    well-conditioned and the CholeskyQR2 panels keep R within 1e-12 of
+    LAPACK's, relative to each row's diagonal."""
+    rng = np.random.default_rng(int(np.log10(cond)) + n)
+    A = rng.standard_normal((k, n)) * np.logspace(0, -np.log10(cond), n)
+    R = H.geqrf_r(A)
+    with panels(1):
+        R1 = H.geqrf_r(A)
+    assert not np.array_equal(R, R1)  # the CholeskyQR2 panels ran
+    Rn = np.linalg.qr(A, mode="r")
+    sgn = np.sign(np.diag(R)) * np.sign(np.diag(Rn))
+    rowerr = np.abs(R - sgn[:, None] * Rn).max(axis=1) / np.abs(np.diag(Rn))
+    assert rowerr.max() < 1e-12, rowerr.max()

@@ -140,6 +140,7 @@ _SIGS = {
     "tomos_geqrf_scratch_size": (sz, [i64, i64]),
     "tomos_geqrf_r_f64": (cint, [i64, i64, vp, i64, vp, i64, vp, sz, vp]),
     "tomos_trtri_upper_f64": (cint, [i64, vp, i64, vp]),
+    "tomos_qr_panels_set": (cint, [cint]),
     "tomos_sketch_qr_precond_f64": (cint, [i64, i64, vp, i64, vp]),
     "tomos_lsqr_arena_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),
     "tomos_lsqr_scratch_size": (sz, [ctypes.POINTER(LsqrOptions), i64, i64]),

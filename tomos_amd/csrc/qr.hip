@@ -27,6 +27,7 @@
 // Triangular inverse: one block per (column j, problem) solves R x = e_j by
 // column-oriented back substitution with x kept in LDS.
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -58,6 +59,8 @@ struct QrDesc {
     double* tl;      // TSQR: leaf T factors, kTsqrMaxLeaves x kNB^2
     double* vc;      // TSQR: combine reflectors, kTsqrComb x kNB (row-major)
     double* tc;      // TSQR: combine T, kNB^2
+    double* cq;      // CholeskyQR2 panels: R1^-1, R1, N (kNB^2 each, row-major), then cq_ints
+    double* bk;      // copy of A (k x n, ld k) taken before the CholeskyQR2 panels
 };
 constexpr int kSyncHead = 64;  // ints before the column flags (own cache lines)
 constexpr int kTsqrMaxLeaves = 16;           // leaves per panel (tile rows 256 or 512)
@@ -67,6 +70,15 @@ constexpr int kTsqrComb = kTsqrMaxLeaves * kNB;  // stacked rows of the combine 
 // can still read its set while the next panel is factored
 constexpr int64_t kTsqrSet = kTsqrMaxLeaves * kNB * kNB + kTsqrComb * kNB + kNB * kNB;
 constexpr int64_t kTsqrMaxK = 16 * 512;      // k beyond this: the wavefront panel path
+
+// CholeskyQR2 panel state per problem: three kNB x kNB matrices, then ints
+// [0], [1] the ticket counters of the two Gram passes and [2] the failure
+// flag (nonzero: a panel could not be factored this way).
+constexpr int kCqMat = kNB * kNB;
+constexpr int kCqDoubles = 3 * kCqMat;
+static_assert(kCqMat == 4 * 256, "the CholeskyQR2 kernels give each of 256 threads four kNB x kNB entries");
+__device__ __forceinline__ int* cq_ints(const QrDesc& d) { return reinterpret_cast<int*>(d.cq + kCqDoubles); }
+__device__ __forceinline__ bool cq_failed(const QrDesc& d) { return cq_ints(d)[2] != 0; }
 
 // Row splits of W = V^T A: 128-row splits, at most 32, so a panel's W has
 // ~1000 blocks in flight at k = 4000.
@@ -615,6 +627,7 @@ __global__ __launch_bounds__(kTT) void qr_form_t_kernel(const QrDesc* __restrict
 // grid (ceil(nt/32), nsplit, problems); 256 threads = 32 p x 8 groups of 4 columns.
 __global__ __launch_bounds__(kT) void qr_w_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
+    if (cq_failed(d)) return;  // a CholeskyQR2 panel failed: the host redoes this problem
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
     if (nbp64 <= 0) return;
     const int nbp = static_cast<int>(nbp64);
@@ -656,6 +669,7 @@ __global__ __launch_bounds__(kT) void qr_w_kernel(const QrDesc* __restrict__ D, 
 // summed in a fixed order, then the triangular product.
 __global__ __launch_bounds__(kT) void qr_w2_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
+    if (cq_failed(d)) return;  // a CholeskyQR2 panel failed: the host redoes this problem
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
     if (nbp64 <= 0) return;
     const int nbp = static_cast<int>(nbp64);
@@ -688,6 +702,7 @@ __global__ __launch_bounds__(kT) void qr_w2_kernel(const QrDesc* __restrict__ D,
 // A(c0:k, c_end+jt) -= V W2 on 64 x 64 tiles.
 __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
+    if (cq_failed(d)) return;  // a CholeskyQR2 panel failed: the host redoes this problem
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
     if (nbp64 <= 0) return;
     const int nbp = static_cast<int>(nbp64);
@@ -695,6 +710,19 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
     const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * 64;
     const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * 64;
     if (nt <= 0 || r0 >= d.k || jt0 >= nt) return;
+    // 256 threads: 16 row groups x 16 column groups, 4 x 4 outputs each; this
+    // thread's A entries are loaded first, overlapping the staging of V and W2
+    const int rg = threadIdx.x % 16, cgp = threadIdx.x / 16;
+    double aold[4][4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        const int64_t jt = jt0 + cgp + 16 * y;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            const int64_t i = r0 + rg + 16 * x;
+            aold[x][y] = (jt < nt && i < d.k) ? d.a[i + (j0 + jt) * d.lda] : 0.0;
+        }
+    }
     __shared__ double sw2[kNB][64 + 1];
     __shared__ double sv[64][kNB + 1];
     for (int e = threadIdx.x; e < kNB * 64; e += kT) {
@@ -708,8 +736,6 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
         sv[rr][q] = (i < d.k && q < nbp) ? d.v[i + q * d.k] : 0.0;
     }
     __syncthreads();
-    // 256 threads: 16 row groups x 16 column groups, 4 x 4 outputs each
-    const int rg = threadIdx.x % 16, cgp = threadIdx.x / 16;
     double acc[4][4];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -734,7 +760,615 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
             const int64_t i = r0 + rg + 16 * x;
-            if (i < d.k) colp[i] -= acc[x][y];
+            if (i < d.k) colp[i] = aold[x][y] - acc[x][y];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CholeskyQR2 panels
+// A panel of nbp <= kNB columns over rows c0..k-1 (P) factored without a
+// column-by-column pass over its rows:
+//   G1 = P^T P, R1 = chol(G1);  G2 = (P R1^-1)^T (P R1^-1), R2 = chol(G2);
+//   Rc = R2 R1, Q = P Rc^-1 (CholeskyQR2: Q has orthonormal columns to
+//   working precision while cond(P) stays below ~1e7).
+// The Householder form of Q is then reconstructed (Ballard, Demmel,
+// Grigori, Jacquelin, Knight, Nguyen, "Reconstructing Householder vectors
+// from tall-skinny QR", 2015): with S = diag(-sign(Q_ii)) and Q S - I =
+// Y U (LU without pivoting, Y unit lower trapezoidal), T = -U Y1^-T makes
+// H = I - Y T Y^T orthogonal with Q S as its first nbp columns, so the
+// panel's R is S Rc and the trailing columns get H^T A exactly as after a
+// Householder panel (qr_cq_w / qr_update kernels).  Y's rows below the
+// panel's top block are P2 N with N = Rc^-1 S U^-1.  Every row-sized step
+// is one launch over all row chunks; the kNB x kNB algebra after each Gram
+// pass is one block per problem (qr_cq_small_kernel), its factorizations
+// on one wavefront with a matrix column per lane.
+// A Gram matrix that is not numerically positive definite (a pivot below
+// kCqPivot times its diagonal, or not finite), or a second Gram matrix
+// further than kCqOrth from I, marks the problem failed: its later
+// launches return at once and the host factors it again from the copy
+// taken first, with the Householder panels.
+constexpr int kCqT = 256;            // threads of the Gram and form-Y kernels
+constexpr int kCqTile = 64;          // rows per LDS tile
+constexpr int kCqLD = kNB + 2;       // LDS row stride of a tile (16-byte aligned rows)
+constexpr double kCqPivot = 1e-15;   // relative Cholesky pivot floor
+constexpr double kCqOrth = 0.1;      // max |G2 - I| for which two passes give an orthonormal Q
+
+// Rows per Gram chunk: a multiple of the tile, at least two tiles, at most
+// 64 chunks.  Chunks never exceed ceil(rows / 128).
+__host__ __device__ inline int64_t cq_chunk_rows(int64_t rows) {
+    int64_t c = (rows + 63) / 64;
+    c = (c + kCqTile - 1) / kCqTile * kCqTile;
+    return c < 2 * kCqTile ? 2 * kCqTile : c;
+}
+
+// Row splits of the CholeskyQR2 path's W = Y^T A: about 512 rows each, at
+// most 16.
+__host__ __device__ inline int cq_wsplits(int64_t rows) {
+    const int64_t s = (rows + 511) / 512;
+    return static_cast<int>(s < 1 ? 1 : (s > 16 ? 16 : s));
+}
+
+// The kNB x kNB algebra of the last Gram block runs on one wavefront, lane
+// j (mod 32) owning matrix column j; what a step needs from other lanes
+// goes through a kNB-double LDS buffer (bc) and is read back as broadcasts.
+// LDS accesses of one wavefront complete in order, so a write followed by
+// reads of the same buffer needs no barrier; a compiler memory barrier
+// (cq_order) around each exchange keeps the accesses in program order and
+// stops the compiler from replacing a read by the value this lane wrote
+// (the other lanes' writes are what is read).  All matrices are row-major
+// in LDS.
+__device__ __forceinline__ void cq_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// 1/sqrt(x) and 1/x to full FP64 precision: the hardware estimates
+// refined by Newton steps (two for the square root, two for the
+// reciprocal; each doubles the correct bits of the ~2^-23 estimate).
+__device__ __forceinline__ double cq_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ double cq_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = y * (2.0 - x * y);
+    return y * (2.0 - x * y);
+}
+
+// Upper Cholesky factor R (G = R^T R) of the nb x nb leading block of G,
+// padded with the identity to kNB x kNB, into r.  Step i: the pivot comes
+// from lane i (readlane), every lane scales its own entry of row i and
+// writes it to bc, and every lane updates its column with the row read
+// back.  Returns false (on every lane) if a pivot is not finite or not
+// above kCqPivot times its original diagonal.
+__device__ __forceinline__ bool cq_wave_chol(const double* g, double* r, double* bc, int nb) {
+    const int l = threadIdx.x & 63, j = l & 31;
+    double c[kNB];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) c[i] = (i < nb && j < nb) ? g[i * kNB + j] : (i == j ? 1.0 : 0.0);
+    int ok = 1;
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) {
+        const double piv = read_lane(c[i], i);
+        const double gii = i < nb ? g[i * kNB + i] : 1.0;  // broadcast read
+        ok &= static_cast<int>(piv > kCqPivot * gii) & static_cast<int>(piv < __builtin_huge_val());
+        const double rs = cq_rsqrt(piv);
+        const double rij = j >= i ? c[i] * rs : 0.0;
+        c[i] = rij;
+        cq_order();
+        bc[j] = rij;
+        cq_order();
+#pragma unroll
+        for (int p = i + 1; p < kNB; ++p) c[p] -= bc[p] * rij;
+    }
+    if (l < kNB) {
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) r[i * kNB + j] = c[i];
+    }
+    return ok != 0;
+}
+
+// x = r^-1 for r upper triangular (kNB x kNB, nonzero diagonal): lane j
+// solves r x = e_j by back substitution, x_i = (e_ij - r(i, >i) x(>i)) /
+// r_ii, the dot product over row i (contiguous, broadcast reads) in four
+// running sums.
+__device__ __forceinline__ void cq_wave_inv_upper(const double* r, double* x_out) {
+    const int l = threadIdx.x & 63, j = l & 31;
+    double x[kNB];
+#pragma unroll
+    for (int i = kNB - 1; i >= 0; --i) {
+        double s0 = i == j ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < kNB; ++k) {
+            const double t = r[i * kNB + k] * x[k];
+            if ((k - i - 1) % 4 == 0) s0 -= t;
+            else if ((k - i - 1) % 4 == 1) s1 -= t;
+            else if ((k - i - 1) % 4 == 2) s2 -= t;
+            else s3 -= t;
+        }
+        x[i] = ((s0 + s1) + (s2 + s3)) * cq_rcp(r[i * kNB + i]);
+    }
+    if (l < kNB) {
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) x_out[i * kNB + j] = x[i];
+    }
+}
+
+// x = L^-1 for L unit lower triangular (diagonal not read), forward
+// substitution over row i the same way.
+__device__ __forceinline__ void cq_wave_inv_unit_lower(const double* lo, double* x_out) {
+    const int l = threadIdx.x & 63, j = l & 31;
+    double x[kNB];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) {
+        double s0 = i == j ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            const double t = lo[i * kNB + k] * x[k];
+            if (k % 4 == 0) s0 -= t;
+            else if (k % 4 == 1) s1 -= t;
+            else if (k % 4 == 2) s2 -= t;
+            else s3 -= t;
+        }
+        x[i] = (s0 + s1) + (s2 + s3);
+    }
+    if (l < kNB) {
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) x_out[i * kNB + j] = x[i];
+    }
+}
+
+// LU without pivoting of v (kNB x kNB): lo gets the unit lower factor, up
+// the upper one.  Step i: lane i publishes its column below the pivot (the
+// multipliers before scaling) in bc and writes the scaled multipliers to
+// lo; every lane j > i then subtracts multiplier x U(i, j) from its column.
+__device__ __forceinline__ void cq_wave_lu(const double* v, double* lo, double* up, double* bc) {
+    const int l = threadIdx.x & 63, j = l & 31;
+    double c[kNB];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) c[i] = v[i * kNB + j];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) {
+        const double inv = cq_rcp(read_lane(c[i], i));
+        cq_order();
+        if (j == i) {
+#pragma unroll
+            for (int p = i + 1; p < kNB; ++p) bc[p] = c[p];
+        }
+        cq_order();
+        const double uij = j > i ? c[i] : 0.0;
+#pragma unroll
+        for (int p = i + 1; p < kNB; ++p) c[p] -= (bc[p] * inv) * uij;
+        if (l == i) {
+#pragma unroll
+            for (int p = i + 1; p < kNB; ++p) lo[p * kNB + i] = bc[p] * inv;
+        }
+    }
+    if (l < kNB) {
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) {
+            up[i * kNB + j] = i <= j ? c[i] : 0.0;
+            if (i <= j) lo[i * kNB + j] = i == j ? 1.0 : 0.0;
+        }
+    }
+}
+
+// c = a b for kNB x kNB row-major LDS matrices, all kCqT threads (a 2 x 2
+// block of c each); the caller separates producers and consumers with
+// barriers.
+__device__ __forceinline__ void cq_block_mm(const double* a, const double* b, double* c) {
+    const int i0 = 2 * (threadIdx.x / 16), j0 = 2 * (threadIdx.x % 16);
+    double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < kNB; ++k) {
+        const double a0 = a[i0 * kNB + k], a1 = a[(i0 + 1) * kNB + k];
+        const double b0 = b[k * kNB + j0], b1 = b[k * kNB + j0 + 1];
+        c00 += a0 * b0;
+        c01 += a0 * b1;
+        c10 += a1 * b0;
+        c11 += a1 * b1;
+    }
+    c[i0 * kNB + j0] = c00;
+    c[i0 * kNB + j0 + 1] = c01;
+    c[(i0 + 1) * kNB + j0] = c10;
+    c[(i0 + 1) * kNB + j0 + 1] = c11;
+}
+
+// Reset tickets and failure flags (and the panel kernel's sync words).
+__global__ void qr_cq_reset_kernel(const QrDesc* __restrict__ D, int nprob) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p < nprob) {
+        int* ci = cq_ints(D[p]);
+        ci[0] = 0;
+        ci[1] = 0;
+        ci[2] = 0;
+    }
+}
+
+// A (k x n, lda) <-> the copy bk (ld k): TO_COPY = true saves, false restores.
+template <bool TO_COPY>
+__global__ __launch_bounds__(kT) void qr_cq_copy_kernel(const QrDesc* __restrict__ D) {
+    const QrDesc& d = D[blockIdx.z];
+    const int64_t total = d.k * d.n;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; t < total;
+         t += static_cast<int64_t>(gridDim.x) * kT) {
+        const int64_t j = t / d.k, i = t - j * d.k;
+        if (TO_COPY) d.bk[t] = d.a[i + j * d.lda];
+        else d.a[i + j * d.lda] = d.bk[t];
+    }
+}
+
+// Partial Gram matrix of one row chunk of the panel (grid (chunks, 1,
+// problems)).  PASS 1: P^T P; PASS 2: (P R1^-1)^T (P R1^-1).  Tiles of 64
+// rows go through LDS; the next tile is loaded into registers while the
+// current one is multiplied.  Each lane keeps a 4 x 4 block of the Gram
+// matrix; the four waves' blocks are added in LDS and the chunk's partial
+// stored to d.gp (the small-algebra kernel adds the chunks in order).
+template <int PASS>
+__global__ __launch_bounds__(kCqT) void qr_cq_gram_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n || cq_failed(d)) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t rows = d.k - c0;
+    const int64_t cr = cq_chunk_rows(rows);
+    const int nch = static_cast<int>((rows + cr - 1) / cr);
+    if (static_cast<int>(blockIdx.x) >= nch) return;
+    const int64_t ra = c0 + static_cast<int64_t>(blockIdx.x) * cr, rb = min(d.k, ra + cr);
+    __shared__ __align__(16) double sred[4 * kCqMat];
+    __shared__ __align__(16) double sp[kCqTile * kCqLD];
+    __shared__ __align__(16) double sx[kCqMat];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    if (PASS == 2)
+        for (int e = t; e < kCqMat; e += kCqT) sx[e] = d.cq[e];  // R1^-1, row-major
+    constexpr int kPer = kCqTile * kNB / kCqT;  // tile elements per thread
+    double nxt[kPer];
+    auto load_tile = [&](int64_t r) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kCqT, rr = e & (kCqTile - 1), q = e / kCqTile;
+            const int64_t i = r + rr;
+            nxt[u] = (i < rb && q < nbp) ? d.a[i + (c0 + q) * d.lda] : 0.0;
+        }
+    };
+    load_tile(ra);
+    const int p0 = 4 * (l & 7), q0 = 4 * (l >> 3);  // this lane's 4 x 4 block of G
+    double acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+    for (int64_t r = ra; r < rb; r += kCqTile) {
+        __syncthreads();  // the previous tile has been read
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kCqT;
+            sp[(e & (kCqTile - 1)) * kCqLD + e / kCqTile] = nxt[u];
+        }
+        __syncthreads();
+        if (r + kCqTile < rb) load_tile(r + kCqTile);
+        if (PASS == 2) {  // rows of P R1^-1 (R1^-1 upper: column j uses rows i <= j)
+            const int rr = t >> 2, j0 = (t & 3) * 8;
+            double o[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) o[u] = 0.0;
+            for (int i = 0; i < j0 + 8; ++i) {
+                const double pv = sp[rr * kCqLD + i];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) o[u] += pv * sx[i * kNB + j0 + u];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sp[rr * kCqLD + j0 + u] = o[u];
+            __syncthreads();
+        }
+        for (int rr = 16 * w; rr < 16 * w + 16; ++rr) {
+            double a4[4], b4[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                a4[x] = sp[rr * kCqLD + p0 + x];
+                b4[x] = sp[rr * kCqLD + q0 + x];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] += a4[x] * b4[y];
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) sred[w * kCqMat + (p0 + x) * kNB + q0 + y] = acc[x][y];
+    __syncthreads();
+    double* gpart = d.gp + static_cast<int64_t>(blockIdx.x) * kCqMat;
+    for (int e = t; e < kCqMat; e += kCqT)
+        gpart[e] = (sred[e] + sred[kCqMat + e]) + (sred[2 * kCqMat + e] + sred[3 * kCqMat + e]);
+}
+
+// The Gram matrix from the chunk partials (added in chunk order, all loads
+// of a batch of 16 chunks in flight), into g (row-major, LDS).
+__device__ __forceinline__ void cq_sum_partials(const QrDesc& d, int nch, double* g) {
+    const int t = threadIdx.x;
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int cb = 0; cb < nch; cb += 16) {
+        double v[16][4];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                v[u][q] = cb + u < nch ? d.gp[static_cast<int64_t>(cb + u) * kCqMat + t + q * kCqT] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sum[q] += v[u][q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) g[t + q * kCqT] = sum[q];
+}
+
+// The kNB x kNB algebra of a panel, one block per problem (grid (1, 1,
+// problems)).  PASS 1: R1 = chol(G1) and R1^-1.  PASS 2: R2 = chol(G2),
+// Rc = R2 R1, the reconstruction (S, L1, U, T, N), the top rows of Y and
+// the panel's rows of R.
+template <int PASS>
+__global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n || cq_failed(d)) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t rows = d.k - c0;
+    const int nch = static_cast<int>((rows + cq_chunk_rows(rows) - 1) / cq_chunk_rows(rows));
+    __shared__ __align__(16) double sm[7 * kCqMat];
+    __shared__ __align__(16) double bc[kNB];
+    __shared__ __align__(16) double sg[kNB];
+    __shared__ int s_ok;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    double* B0 = sm;
+    double* B1 = sm + kCqMat;
+    double* B2 = sm + 2 * kCqMat;
+    double* B3 = sm + 3 * kCqMat;
+    double* B4 = sm + 4 * kCqMat;
+    double* B5 = sm + 5 * kCqMat;
+    double* B6 = sm + 6 * kCqMat;
+    int* ci = cq_ints(d);
+    cq_sum_partials(d, nch, B0);
+    if (PASS == 2) {
+        for (int e = t; e < kCqMat; e += kCqT) {
+            B3[e] = d.cq[e];           // R1^-1
+            B5[e] = d.cq[kCqMat + e];  // R1
+            const int i = e / kNB, j = e % kNB;
+            B6[e] = (i < nbp && j < nbp) ? d.a[(c0 + i) + (c0 + j) * d.lda] : 0.0;  // top rows of P
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        bool far = false;
+        if (PASS == 2) {
+            const int j = l & 31;
+            for (int i = 0; i < nbp; ++i)
+                if (j < nbp) far = far || !(fabs(B0[i * kNB + j] - (i == j ? 1.0 : 0.0)) < kCqOrth);
+        }
+        const bool ok = cq_wave_chol(B0, B1, bc, nbp) && !__any(far);  // R1 or R2
+        cq_wave_inv_upper(B1, B2);                                     // its inverse
+        if (l == 0) s_ok = ok;
+    }
+    __syncthreads();
+    if (!s_ok) {
+        if (t == 0) ci[2] = 1;
+        return;
+    }
+    if (PASS == 1) {
+        for (int e = t; e < kCqMat; e += kCqT) {
+            d.cq[e] = B2[e];
+            d.cq[kCqMat + e] = B1[e];
+        }
+        return;
+    }
+    cq_block_mm(B3, B2, B4);  // M = R1^-1 R2^-1 = Rc^-1
+    cq_block_mm(B1, B5, B0);  // Rc = R2 R1
+    __syncthreads();
+    cq_block_mm(B6, B4, B2);  // Q top rows = P_top M
+    __syncthreads();
+    // S (sg) and V1 = Q_top S - I (identity-padded beyond nbp)
+    if (t < kNB) sg[t] = B2[t * kNB + t] >= 0.0 ? -1.0 : 1.0;
+    for (int e = t; e < kCqMat; e += kCqT) {
+        const int i = e / kNB, j = e % kNB;
+        const double sj = B2[j * kNB + j] >= 0.0 ? -1.0 : 1.0;
+        B1[e] = (i < nbp && j < nbp) ? B2[e] * sj - (i == j ? 1.0 : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (w == 0) cq_wave_lu(B1, B5, B3, bc);  // L1 -> B5, U -> B3
+    __syncthreads();
+    if (w == 0) cq_wave_inv_upper(B3, B1);       // U^-1
+    if (w == 1) cq_wave_inv_unit_lower(B5, B2);  // L1^-1
+    __syncthreads();
+    // T = -U L1^-T into B6, S U^-1 into B3, then N = M (S U^-1) into B2
+    // (B5 keeps L1 for the outputs)
+    {
+        const int i0 = 2 * (t / 16), j0 = 2 * (t % 16);
+        double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < kNB; ++k) {
+            const double a0 = B3[i0 * kNB + k], a1 = B3[(i0 + 1) * kNB + k];
+            const double b0 = B2[j0 * kNB + k], b1 = B2[(j0 + 1) * kNB + k];
+            c00 -= a0 * b0;
+            c01 -= a0 * b1;
+            c10 -= a1 * b0;
+            c11 -= a1 * b1;
+        }
+        B6[i0 * kNB + j0] = c00;
+        B6[i0 * kNB + j0 + 1] = c01;
+        B6[(i0 + 1) * kNB + j0] = c10;
+        B6[(i0 + 1) * kNB + j0 + 1] = c11;
+    }
+    __syncthreads();
+    for (int e = t; e < kCqMat; e += kCqT) B3[e] = sg[e / kNB] * B1[e];  // S U^-1 (U no longer needed)
+    __syncthreads();
+    cq_block_mm(B4, B3, B2);  // N = M S U^-1 (L1^-1 no longer needed)
+    __syncthreads();
+    // outputs: T (column-major, zero beyond nbp), N (row-major), Y's top rows, the panel's R rows
+    for (int e = t; e < kCqMat; e += kCqT) {
+        const int i = e / kNB, j = e % kNB;
+        const bool in = i < nbp && j < nbp;
+        d.t[i + j * kNB] = in ? B6[e] : 0.0;
+        d.cq[2 * kCqMat + e] = in ? B2[e] : 0.0;
+        if (i < nbp) d.v[(c0 + i) + static_cast<int64_t>(j) * d.k] = j < nbp ? (i > j ? B5[e] : (i == j ? 1.0 : 0.0)) : 0.0;
+        if (i < nbp && j < nbp && i <= j) {
+            const double rv = sg[i] * B0[e];  // S Rc
+            if (i == j) d.diag[c0 + i] = rv;
+            else d.a[(c0 + i) + (c0 + j) * d.lda] = rv;
+        }
+    }
+}
+
+// Y's rows below the panel's top block: row i of P times N.  Grid
+// (row blocks of kCqYT, 1, problems); one row per thread.
+constexpr int kCqYT = 128;
+__global__ __launch_bounds__(kCqYT) void qr_cq_form_y_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n || cq_failed(d)) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    if (c0 + nbp >= d.n) return;  // no trailing columns: Y is not used
+    __shared__ __align__(16) double sn[kCqMat];
+    const int64_t i = c0 + nbp + static_cast<int64_t>(blockIdx.x) * kCqYT + threadIdx.x;
+    double p[kNB];  // the row is loaded first so that its loads and N's overlap
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) p[q] = (i < d.k && q < nbp) ? d.a[i + (c0 + q) * d.lda] : 0.0;
+    for (int e = threadIdx.x; e < kCqMat; e += kCqYT) sn[e] = d.cq[2 * kCqMat + e];
+    __syncthreads();
+    if (i >= d.k) return;
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q <= j; ++q) acc += p[q] * sn[q * kNB + j];  // N upper triangular
+        d.v[i + static_cast<int64_t>(j) * d.k] = acc;
+    }
+}
+
+// W = Y^T A(c0:k, trailing) and W2 = T^T W for the CholeskyQR2 panels.
+// Grid (32-column tiles, 16, problems): block (tile, split) forms the
+// split's partial Y^T A over ~512 rows (64-row tiles of Y and A staged in
+// LDS, the next tile loaded into registers while the current one is
+// multiplied, a 4 x 4 block of W per lane, the four waves' sums added in
+// LDS) and stores it; the last split block of a column tile (ticket in
+// d.sync) adds the partials in split order and writes T^T W to d.w2.
+__global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n || cq_failed(d)) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t j0 = c0 + nbp, nt = d.n - j0;
+    const int64_t jt0 = static_cast<int64_t>(blockIdx.x) * kNB;
+    if (nt <= 0 || jt0 >= nt) return;
+    const int64_t rows = d.k - c0;
+    const int ns = cq_wsplits(rows);
+    if (static_cast<int>(blockIdx.y) >= ns) return;
+    const int64_t rps = (rows + ns - 1) / ns;
+    const int64_t ra = c0 + static_cast<int64_t>(blockIdx.y) * rps, rb = min(d.k, ra + rps);
+    const int ncols = static_cast<int>(min<int64_t>(kNB, nt - jt0));
+    __shared__ __align__(16) double sy[kCqTile * kCqLD];
+    __shared__ __align__(16) double sa[kCqTile * kCqLD];
+    __shared__ __align__(16) double sred[4 * kCqMat];
+    __shared__ int s_last;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int p0 = 4 * (l & 7), q0 = 4 * (l >> 3);  // W rows (Y columns) p0.., W columns q0..
+    double acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+    constexpr int kPer = kCqTile * kNB / kCqT;  // tile elements per thread
+    double ny[kPer], na[kPer];
+    auto load_tile = [&](int64_t r) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kCqT, rr = e & (kCqTile - 1), q = e / kCqTile;
+            const int64_t i = r + rr;
+            const bool in = i < rb;
+            ny[u] = (in && q < nbp) ? d.v[i + static_cast<int64_t>(q) * d.k] : 0.0;
+            na[u] = (in && q < ncols) ? d.a[i + (j0 + jt0 + q) * d.lda] : 0.0;
+        }
+    };
+    load_tile(ra);
+    for (int64_t r = ra; r < rb; r += kCqTile) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kCqT, o = (e & (kCqTile - 1)) * kCqLD + e / kCqTile;
+            sy[o] = ny[u];
+            sa[o] = na[u];
+        }
+        __syncthreads();
+        if (r + kCqTile < rb) load_tile(r + kCqTile);
+        for (int rr = 16 * w; rr < 16 * w + 16; ++rr) {
+            double a4[4], b4[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                a4[x] = sy[rr * kCqLD + p0 + x];
+                b4[x] = sa[rr * kCqLD + q0 + x];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] += a4[x] * b4[y];
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) sred[w * kCqMat + (p0 + x) * kNB + q0 + y] = acc[x][y];
+    __syncthreads();
+    const int64_t ntiles = (nt + kNB - 1) / kNB;
+    double* part = d.wp + (static_cast<int64_t>(blockIdx.y) * ntiles + blockIdx.x) * kCqMat;
+    for (int e = t; e < kCqMat; e += kCqT)
+        part[e] = (sred[e] + sred[kCqMat + e]) + (sred[2 * kCqMat + e] + sred[3 * kCqMat + e]);
+    __threadfence();
+    __syncthreads();
+    int* ticket = d.sync + kSyncHead + blockIdx.x;
+    if (t == 0) s_last = atomicAdd(ticket, 1) == ns - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (t == 0) *ticket = 0;  // for the next panel
+    // W (row-major 32 x 32 in sred) = the split partials in split order,
+    // every load of this thread (4 entries x up to 16 splits) in flight at
+    // once; the acquire fence above makes the other blocks' stores visible
+    {
+        double v[16][4];
+#pragma unroll
+        for (int sp = 0; sp < 16; ++sp)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                v[sp][q] = sp < ns ? d.wp[(static_cast<int64_t>(sp) * ntiles + blockIdx.x) * kCqMat + t + q * kCqT]
+                                   : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double sum = 0.0;
+#pragma unroll
+            for (int sp = 0; sp < 16; ++sp) sum += v[sp][q];
+            sred[t + q * kCqT] = sum;
+        }
+    }
+    for (int e = t; e < kCqMat; e += kCqT) sred[kCqMat + e] = d.t[e];  // T, column-major
+    __syncthreads();
+    // W2(p, q) = sum_{r <= p} T(r, p) W(r, q), stored at w2[(jt0 + q) * kNB + p];
+    // a 2 x 2 block of W2 per thread (T is zero below its diagonal)
+    {
+        const int p0 = 2 * (t % 16), q0 = 2 * (t / 16);
+        double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+#pragma unroll 8
+        for (int r = 0; r < kNB; ++r) {
+            const double t0 = sred[kCqMat + r + p0 * kNB], t1 = sred[kCqMat + r + (p0 + 1) * kNB];
+            const double w0 = sred[r * kNB + q0], w1 = sred[r * kNB + q0 + 1];
+            a00 += t0 * w0;
+            a01 += t0 * w1;
+            a10 += t1 * w0;
+            a11 += t1 * w1;
+        }
+        const bool p0in = p0 < nbp, p1in = p0 + 1 < nbp;
+        if (q0 < ncols) {
+            d.w2[(jt0 + q0) * kNB + p0] = p0in ? a00 : 0.0;
+            d.w2[(jt0 + q0) * kNB + p0 + 1] = p1in ? a10 : 0.0;
+        }
+        if (q0 + 1 < ncols) {
+            d.w2[(jt0 + q0 + 1) * kNB + p0] = p0in ? a01 : 0.0;
+            d.w2[(jt0 + q0 + 1) * kNB + p0 + 1] = p1in ? a11 : 0.0;
         }
     }
 }
@@ -838,7 +1472,7 @@ size_t a256(size_t v) { return (v + 255) / 256 * 256; }
 // per-problem scratch: diag, tau, inv_v0 | V | Gram partials | T |
 // W partials (also the inverse's n x n copy of R) | bad flag
 struct Sizes {
-    size_t vec3, v, gp, t, wp, w2, sync, tsqr, total;
+    size_t vec3, v, gp, t, wp, w2, sync, tsqr, cq, bk, total;
 };
 
 Sizes sizes(int64_t k, int64_t n) {
@@ -849,11 +1483,14 @@ Sizes sizes(int64_t k, int64_t n) {
     z.v = a256(static_cast<size_t>(kk * kNB) * 8);
     z.gp = a256(static_cast<size_t>(nch * kNB * kNB) * 8);
     z.t = a256(static_cast<size_t>(kNB * kNB) * 8);
-    z.wp = a256(static_cast<size_t>(std::max(qr_nsplit(kk) * nn * kNB, nn * nn)) * 8);
+    z.wp = a256(static_cast<size_t>(std::max({qr_nsplit(kk) * nn * kNB, nn * nn,
+                                              int64_t(16) * ((nn + kNB - 1) / kNB) * kCqMat})) * 8);
     z.w2 = a256(static_cast<size_t>(nn * kNB) * 8);
     z.sync = a256(static_cast<size_t>(kSyncHead + nn) * 4);
     z.tsqr = a256(static_cast<size_t>(2 * kTsqrSet) * 8);
-    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + z.sync + z.tsqr + 256;
+    z.cq = a256(static_cast<size_t>(kCqDoubles) * 8 + 4 * sizeof(int));
+    z.bk = a256(static_cast<size_t>(kk * nn) * 8);
+    z.total = z.vec3 + z.v + z.gp + z.t + z.wp + z.w2 + z.sync + z.tsqr + z.cq + z.bk + 256;
     return z;
 }
 
@@ -887,12 +1524,16 @@ QrDesc carve(char*& p, const QrProblem& q) {
     d.vc = d.tl + kTsqrMaxLeaves * kNB * kNB;
     d.tc = d.vc + kTsqrComb * kNB;
     p += z.tsqr;
+    d.cq = reinterpret_cast<double*>(p);
+    p += z.cq;
+    d.bk = reinterpret_cast<double*>(p);
+    p += z.bk;
     d.bad = reinterpret_cast<int*>(p);
     p += 256;
     return d;
 }
 
-size_t status_bytes(int nprob) { return a256(static_cast<size_t>(std::max(nprob, 1) + 1) * 4); }
+size_t status_bytes(int nprob) { return a256(static_cast<size_t>(2 * std::max(nprob, 1) + 1) * 4); }
 
 size_t batch_bytes(int nprob, const QrProblem* probs) {
     size_t b = a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)) + status_bytes(nprob);
@@ -901,8 +1542,9 @@ size_t batch_bytes(int nprob, const QrProblem* probs) {
 }
 
 // Descriptor table at the start of scratch, then the status words
-// ([0] panel wait failure, [1 + i] problem i's first zero diagonal), then
-// the problems' regions.
+// ([0] panel wait failure, [1 + i] problem i's first zero diagonal,
+// [1 + nprob + i] problem i's CholeskyQR2 failure flag), then the
+// problems' regions.
 struct Batch {
     QrDesc* dev = nullptr;
     int* status = nullptr;
@@ -928,8 +1570,51 @@ Batch setup(int nprob, const QrProblem* probs, void* scratch, size_t scratch_byt
     }
     TOMOS_LAUNCH_CHECK("qr_set_desc_kernel");
     qr_status_reset_kernel<<<1, 1, 0, s>>>(B.status);
+    qr_cq_reset_kernel<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(B.dev, nprob);
     TOMOS_LAUNCH_CHECK("qr_status_reset_kernel");
     return B;
+}
+
+// Which panels factor(): 0 = CholeskyQR2 panels, the Householder panels for
+// problems where they fail; 1 = the Householder panels only.
+std::atomic<int> g_qr_panels{0};
+
+// A (k x n, lda) <-> its saved copy, every problem of the batch.
+template <bool TO_COPY>
+void cq_copy(const Batch& B, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(ceil_div(B.kmax * B.nmax, kT), 2048);
+    qr_cq_copy_kernel<TO_COPY><<<dim3(static_cast<unsigned>(blocks), 1, static_cast<unsigned>(B.nprob)), kT, 0, s>>>(
+        B.dev);
+    TOMOS_LAUNCH_CHECK("qr_cq_copy_kernel");
+}
+
+// CholeskyQR2 panels (see qr_cq_gram_kernel) with the blocked Householder
+// trailing update: per panel two Gram passes, Y, W = Y^T A, T^T W and
+// A -= Y T^T W, all problems in the same launches.  A is saved first.
+void factor_cq(const Batch& B, hipStream_t s) {
+    const unsigned P = static_cast<unsigned>(B.nprob);
+    cq_copy<true>(B, s);
+    qr_sync_reset_kernel<<<dim3(static_cast<unsigned>(ceil_div(kSyncHead + B.nmax, kT)), 1, P), kT, 0, s>>>(
+        B.dev, B.status);  // the W tickets
+    TOMOS_LAUNCH_CHECK("qr_sync_reset_kernel");
+    for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
+        const int64_t rows = B.kmax - c0;
+        const unsigned gx = static_cast<unsigned>(std::min<int64_t>(64, ceil_div(rows, 2 * kCqTile)));
+        qr_cq_gram_kernel<1><<<dim3(gx, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_cq_small_kernel<1><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_cq_gram_kernel<2><<<dim3(gx, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_cq_small_kernel<2><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        TOMOS_LAUNCH_CHECK("qr_cq_gram_kernel");
+        if (c0 + kNB >= B.nmax) break;  // no problem has columns right of this panel
+        const int64_t nt = B.nmax - c0 - kNB;
+        qr_cq_form_y_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, kCqYT)), 1, P), kCqYT, 0, s>>>(B.dev, c0);
+        qr_cq_w_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, kNB)), 16, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
+                           kT, 0, s>>>(B.dev, c0);
+        TOMOS_LAUNCH_CHECK("qr_cq trailing update");
+    }
+    extract_r_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax * B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
+    TOMOS_LAUNCH_CHECK("extract_r_kernel");
 }
 
 void factor(const Batch& B, hipStream_t s) {
@@ -1010,10 +1695,14 @@ void factor(const Batch& B, hipStream_t s) {
 }
 
 // status[1 + i] = problem i's first zero diagonal (when with_bad), else
-// INT_MAX; status[0] is the panel kernel's wait-failure flag.
+// INT_MAX; status[1 + nprob + i] its CholeskyQR2 failure flag; status[0]
+// is the panel kernel's wait-failure flag.
 __global__ void gather_status_kernel(const QrDesc* __restrict__ D, int nprob, bool with_bad, int* __restrict__ status) {
     const int i = blockIdx.x * 64 + threadIdx.x;
-    if (i < nprob) status[1 + i] = (with_bad && D[i].n > 0) ? *D[i].bad : 0x7FFFFFFF;
+    if (i < nprob) {
+        status[1 + i] = (with_bad && D[i].n > 0) ? *D[i].bad : 0x7FFFFFFF;
+        status[1 + nprob + i] = cq_ints(D[i])[2];
+    }
 }
 
 // The status words on the host (one copy, one synchronisation); throws if
@@ -1022,21 +1711,48 @@ std::vector<int> read_status(const Batch& B, bool with_bad, hipStream_t s) {
     gather_status_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob, with_bad,
                                                                                     B.status);
     TOMOS_LAUNCH_CHECK("gather_status_kernel");
-    std::vector<int> h(static_cast<size_t>(B.nprob) + 1, 0);
+    std::vector<int> h(2 * static_cast<size_t>(B.nprob) + 1, 0);
     TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), B.status, h.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     TOMOS_HIP_CHECK(hipStreamSynchronize(s));
     if (h[0] != 0) throw Error(kErrInternal, "geqrf: panel kernel wait limit reached");
     return h;
 }
 
-// R -> R^-1 in place for every problem; throws on a zero diagonal.
-void invert(const Batch& B, hipStream_t s) {
+// The status words after marking each R's first zero diagonal (with_bad).
+std::vector<int> check_status(const Batch& B, bool with_bad, hipStream_t s) {
+    if (with_bad) {
+        reset_bad_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob);
+        diag_zero_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax, kT)), 1, static_cast<unsigned>(B.nprob)), kT,
+                           0, s>>>(B.dev);
+        TOMOS_LAUNCH_CHECK("diag_zero_kernel");
+    }
+    return read_status(B, with_bad, s);
+}
+
+// Factor every problem of the batch and return the status words (one
+// synchronisation when the CholeskyQR2 panels succeed).  If any problem's
+// CholeskyQR2 panel failed, every A is restored from its copy and the
+// batch is factored again with the Householder panels.
+std::vector<int> factor_checked(const Batch& B, bool with_bad, hipStream_t s) {
+    if (g_qr_panels.load(std::memory_order_relaxed) == 0) {
+        factor_cq(B, s);
+        std::vector<int> h = check_status(B, with_bad, s);
+        bool failed = false;
+        for (int i = 0; i < B.nprob; ++i) failed = failed || h[1 + B.nprob + i] != 0;
+        if (!failed) return h;
+        cq_copy<false>(B, s);
+        qr_cq_reset_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob);
+        TOMOS_LAUNCH_CHECK("qr_cq_reset_kernel");
+    }
+    factor(B, s);
+    return check_status(B, with_bad, s);
+}
+
+// R -> R^-1 in place for every problem (st from check_status with
+// with_bad); throws on a zero diagonal.
+void invert(const Batch& B, const std::vector<int>& st, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
     require(B.nmax * 8 <= 128 * 1024, "trtri: n too large (max 16384)");
-    reset_bad_kernel<<<static_cast<unsigned>(ceil_div(B.nprob, 64)), 64, 0, s>>>(B.dev, B.nprob);
-    diag_zero_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
-    TOMOS_LAUNCH_CHECK("diag_zero_kernel");
-    const std::vector<int> st = read_status(B, true, s);
     for (int i = 0; i < B.nprob; ++i) {
         const int bad = st[1 + i];
         if (bad != 0x7FFFFFFF)
@@ -1077,9 +1793,8 @@ void geqrf_r_batched(int nprob, const QrProblem* probs, bool invert_r, void* scr
     }
     if (nprob == 0 || nmax == 0) return;
     const Batch B = setup(nprob, probs, scratch, scratch_bytes, s);
-    factor(B, s);
-    if (invert_r) invert(B, s);
-    else read_status(B, false, s);
+    const std::vector<int> st = factor_checked(B, invert_r, s);
+    if (invert_r) invert(B, st, s);
     TOMOS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1096,7 +1811,7 @@ void trtri_upper(int64_t n, double* r, int64_t ldr, hipStream_t s) {
     require(n * 8 <= 128 * 1024, "trtri: n too large (max 16384)");
     const QrProblem q{n, n, n, ldr, nullptr, r};
     const Batch B = setup(1, &q, nullptr, 0, s);
-    invert(B, s);
+    invert(B, check_status(B, true, s), s);
     TOMOS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1145,6 +1860,13 @@ int tomos_geqrf_r_f64(int64_t k, int64_t n, double* a, int64_t lda, double* r, i
 
 int tomos_trtri_upper_f64(int64_t n, double* r, int64_t ldr, tomos_stream_t s) {
     return guarded([&] { trtri_upper(n, r, ldr, s); });
+}
+
+int tomos_qr_panels_set(int mode) {
+    return guarded([&] {
+        require(mode == 0 || mode == 1, "tomos_qr_panels_set: mode must be 0 or 1");
+        g_qr_panels.store(mode, std::memory_order_relaxed);
+    });
 }
 
 int tomos_sketch_qr_precond_f64(int64_t k, int64_t n, double* a, int64_t lda, tomos_stream_t s) {
