@@ -49,9 +49,9 @@ struct QrDesc {
     double* diag;    // n
     double* tau;     // n
     double* inv_v0;  // n
-    double* v;       // k x kNB (ld k)
+    double* v;       // k x kNB (ld k); the CholeskyQR2 path keeps a second set at v + k kNB
     double* gp;      // Gram partials: ceil(k/kGR) x kNB^2
-    double* t;       // kNB^2
+    double* t;       // kNB^2; the CholeskyQR2 path keeps a second set at t + kNB^2
     double* wp;      // W partials: nsplit x n x kNB; the inverse's copy of R (n x n)
     double* w2;      // T^T W: n x kNB (column jt at w2 + jt*kNB)
     int* bad;        // trtri: first zero diagonal (INT_MAX if none)
@@ -699,8 +699,9 @@ __global__ __launch_bounds__(kT) void qr_w2_kernel(const QrDesc* __restrict__ D,
     }
 }
 
-// A(c0:k, c_end+jt) -= V W2 on 64 x 64 tiles.
-__global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+// A(c0:k, c_end+jt) -= V W2 on 64 x 64 tiles; V is set `par` of d.v and the
+// column tiles start at tile y0.
+__global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict__ D, int64_t c0, int par, int y0) {
     const QrDesc& d = D[blockIdx.z];
     if (cq_failed(d)) return;  // a CholeskyQR2 panel failed: the host redoes this problem
     const int64_t nbp64 = min<int64_t>(kNB, d.n - c0);
@@ -708,8 +709,9 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
     const int nbp = static_cast<int>(nbp64);
     const int64_t j0 = c0 + nbp, nt = d.n - j0;
     const int64_t r0 = c0 + static_cast<int64_t>(blockIdx.x) * 64;
-    const int64_t jt0 = static_cast<int64_t>(blockIdx.y) * 64;
+    const int64_t jt0 = static_cast<int64_t>(y0 + static_cast<int>(blockIdx.y)) * 64;
     if (nt <= 0 || r0 >= d.k || jt0 >= nt) return;
+    const double* vv = d.v + par * d.k * kNB;
     // 256 threads: 16 row groups x 16 column groups, 4 x 4 outputs each; this
     // thread's A entries are loaded first, overlapping the staging of V and W2
     const int rg = threadIdx.x % 16, cgp = threadIdx.x / 16;
@@ -733,7 +735,7 @@ __global__ __launch_bounds__(kT) void qr_update_kernel(const QrDesc* __restrict_
     for (int e = threadIdx.x; e < 64 * kNB; e += kT) {
         const int rr = e % 64, q = e / 64;
         const int64_t i = r0 + rr;
-        sv[rr][q] = (i < d.k && q < nbp) ? d.v[i + q * d.k] : 0.0;
+        sv[rr][q] = (i < d.k && q < nbp) ? vv[i + q * d.k] : 0.0;
     }
     __syncthreads();
     double acc[4][4];
@@ -801,10 +803,10 @@ __host__ __device__ inline int64_t cq_chunk_rows(int64_t rows) {
     return c < 2 * kCqTile ? 2 * kCqTile : c;
 }
 
-// Row splits of the CholeskyQR2 path's W = Y^T A: about 512 rows each, at
+// Row splits of the CholeskyQR2 path's W = Y^T A: about 256 rows each, at
 // most 16.
 __host__ __device__ inline int cq_wsplits(int64_t rows) {
-    const int64_t s = (rows + 511) / 512;
+    const int64_t s = (rows + 255) / 256;
     return static_cast<int>(s < 1 ? 1 : (s > 16 ? 16 : s));
 }
 
@@ -1108,7 +1110,7 @@ __device__ __forceinline__ void cq_sum_partials(const QrDesc& d, int nch, double
 // Rc = R2 R1, the reconstruction (S, L1, U, T, N), the top rows of Y and
 // the panel's rows of R.
 template <int PASS>
-__global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+__global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restrict__ D, int64_t c0, int par) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n || cq_failed(d)) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
@@ -1206,9 +1208,11 @@ __global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restr
     for (int e = t; e < kCqMat; e += kCqT) {
         const int i = e / kNB, j = e % kNB;
         const bool in = i < nbp && j < nbp;
-        d.t[i + j * kNB] = in ? B6[e] : 0.0;
+        d.t[par * kCqMat + i + j * kNB] = in ? B6[e] : 0.0;
         d.cq[2 * kCqMat + e] = in ? B2[e] : 0.0;
-        if (i < nbp) d.v[(c0 + i) + static_cast<int64_t>(j) * d.k] = j < nbp ? (i > j ? B5[e] : (i == j ? 1.0 : 0.0)) : 0.0;
+        if (i < nbp)
+            d.v[par * d.k * kNB + (c0 + i) + static_cast<int64_t>(j) * d.k] =
+                j < nbp ? (i > j ? B5[e] : (i == j ? 1.0 : 0.0)) : 0.0;
         if (i < nbp && j < nbp && i <= j) {
             const double rv = sg[i] * B0[e];  // S Rc
             if (i == j) d.diag[c0 + i] = rv;
@@ -1220,7 +1224,7 @@ __global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restr
 // Y's rows below the panel's top block: row i of P times N.  Grid
 // (row blocks of kCqYT, 1, problems); one row per thread.
 constexpr int kCqYT = 128;
-__global__ __launch_bounds__(kCqYT) void qr_cq_form_y_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+__global__ __launch_bounds__(kCqYT) void qr_cq_form_y_kernel(const QrDesc* __restrict__ D, int64_t c0, int par) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n || cq_failed(d)) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
@@ -1238,23 +1242,24 @@ __global__ __launch_bounds__(kCqYT) void qr_cq_form_y_kernel(const QrDesc* __res
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q <= j; ++q) acc += p[q] * sn[q * kNB + j];  // N upper triangular
-        d.v[i + static_cast<int64_t>(j) * d.k] = acc;
+        d.v[par * d.k * kNB + i + static_cast<int64_t>(j) * d.k] = acc;
     }
 }
 
-// W = Y^T A(c0:k, trailing) and W2 = T^T W for the CholeskyQR2 panels.
-// Grid (32-column tiles, 16, problems): block (tile, split) forms the
-// split's partial Y^T A over ~512 rows (64-row tiles of Y and A staged in
-// LDS, the next tile loaded into registers while the current one is
-// multiplied, a 4 x 4 block of W per lane, the four waves' sums added in
-// LDS) and stores it; the last split block of a column tile (ticket in
-// d.sync) adds the partials in split order and writes T^T W to d.w2.
-__global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict__ D, int64_t c0) {
+// Split partials of W = Y^T A(c0:k, trailing) for the CholeskyQR2 panels.
+// Grid (32-column tiles from tile0, 16, problems): block (tile, split)
+// forms the split's partial Y^T A over ~256 rows (64-row tiles of Y and A
+// staged in LDS with two tiles in flight, a 4 x 4 block of W per lane, the
+// four waves' sums added in LDS) and stores it to d.wp; qr_cq_w2_kernel
+// adds them.
+__global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict__ D, int64_t c0, int par, int tile0) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n || cq_failed(d)) return;
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
     const int64_t j0 = c0 + nbp, nt = d.n - j0;
-    const int64_t jt0 = static_cast<int64_t>(blockIdx.x) * kNB;
+    const int tile = tile0 + static_cast<int>(blockIdx.x);
+    const int64_t jt0 = static_cast<int64_t>(tile) * kNB;
+    const double* yv = d.v + par * d.k * kNB;
     if (nt <= 0 || jt0 >= nt) return;
     const int64_t rows = d.k - c0;
     const int ns = cq_wsplits(rows);
@@ -1262,10 +1267,11 @@ __global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict_
     const int64_t rps = (rows + ns - 1) / ns;
     const int64_t ra = c0 + static_cast<int64_t>(blockIdx.y) * rps, rb = min(d.k, ra + rps);
     const int ncols = static_cast<int>(min<int64_t>(kNB, nt - jt0));
-    __shared__ __align__(16) double sy[kCqTile * kCqLD];
-    __shared__ __align__(16) double sa[kCqTile * kCqLD];
-    __shared__ __align__(16) double sred[4 * kCqMat];
-    __shared__ int s_last;
+    __shared__ __align__(16) double sya[2 * kCqTile * kCqLD];  // Y tile, A tile; then the per-wave W blocks
+    static_assert(2 * kCqTile * kCqLD >= 4 * kCqMat, "the W blocks reuse the tile buffers");
+    double* sy = sya;
+    double* sa = sya + kCqTile * kCqLD;
+    double* sred = sya;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int p0 = 4 * (l & 7), q0 = 4 * (l >> 3);  // W rows (Y columns) p0.., W columns q0..
     double acc[4][4];
@@ -1274,28 +1280,30 @@ __global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict_
 #pragma unroll
         for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
     constexpr int kPer = kCqTile * kNB / kCqT;  // tile elements per thread
-    double ny[kPer], na[kPer];
-    auto load_tile = [&](int64_t r) {
+    // two tiles in flight: while tile r is multiplied, tiles r+1 (buffer b ^ 1)
+    // and r+2 are being loaded
+    double ny[2][kPer], na[2][kPer];
+    auto load_tile = [&](int64_t r, double (&by)[kPer], double (&ba)[kPer]) {
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int e = t + u * kCqT, rr = e & (kCqTile - 1), q = e / kCqTile;
             const int64_t i = r + rr;
             const bool in = i < rb;
-            ny[u] = (in && q < nbp) ? d.v[i + static_cast<int64_t>(q) * d.k] : 0.0;
-            na[u] = (in && q < ncols) ? d.a[i + (j0 + jt0 + q) * d.lda] : 0.0;
+            by[u] = (in && q < nbp) ? yv[i + static_cast<int64_t>(q) * d.k] : 0.0;
+            ba[u] = (in && q < ncols) ? d.a[i + (j0 + jt0 + q) * d.lda] : 0.0;
         }
     };
-    load_tile(ra);
-    for (int64_t r = ra; r < rb; r += kCqTile) {
-        __syncthreads();
+    auto mult_tile = [&](const double (&by)[kPer], const double (&ba)[kPer]) {
+        __syncthreads();  // the previous tile has been read
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int e = t + u * kCqT, o = (e & (kCqTile - 1)) * kCqLD + e / kCqTile;
-            sy[o] = ny[u];
-            sa[o] = na[u];
+            sy[o] = by[u];
+            sa[o] = ba[u];
         }
         __syncthreads();
-        if (r + kCqTile < rb) load_tile(r + kCqTile);
+    };
+    auto fma_tile = [&]() {
         for (int rr = 16 * w; rr < 16 * w + 16; ++rr) {
             double a4[4], b4[4];
 #pragma unroll
@@ -1308,68 +1316,87 @@ __global__ __launch_bounds__(kCqT) void qr_cq_w_kernel(const QrDesc* __restrict_
 #pragma unroll
                 for (int y = 0; y < 4; ++y) acc[x][y] += a4[x] * b4[y];
         }
+    };
+    load_tile(ra, ny[0], na[0]);
+    if (ra + kCqTile < rb) load_tile(ra + kCqTile, ny[1], na[1]);
+    for (int64_t r = ra; r < rb; r += 2 * kCqTile) {
+        mult_tile(ny[0], na[0]);
+        if (r + 2 * kCqTile < rb) load_tile(r + 2 * kCqTile, ny[0], na[0]);
+        fma_tile();
+        if (r + kCqTile >= rb) break;
+        mult_tile(ny[1], na[1]);
+        if (r + 3 * kCqTile < rb) load_tile(r + 3 * kCqTile, ny[1], na[1]);
+        fma_tile();
     }
+    __syncthreads();  // the last tile has been read: the buffers take the W blocks
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) sred[w * kCqMat + (p0 + x) * kNB + q0 + y] = acc[x][y];
     __syncthreads();
     const int64_t ntiles = (nt + kNB - 1) / kNB;
-    double* part = d.wp + (static_cast<int64_t>(blockIdx.y) * ntiles + blockIdx.x) * kCqMat;
+    double* part = d.wp + (static_cast<int64_t>(blockIdx.y) * ntiles + tile) * kCqMat;
     for (int e = t; e < kCqMat; e += kCqT)
         part[e] = (sred[e] + sred[kCqMat + e]) + (sred[2 * kCqMat + e] + sred[3 * kCqMat + e]);
-    __threadfence();
-    __syncthreads();
-    int* ticket = d.sync + kSyncHead + blockIdx.x;
-    if (t == 0) s_last = atomicAdd(ticket, 1) == ns - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (t == 0) *ticket = 0;  // for the next panel
-    // W (row-major 32 x 32 in sred) = the split partials in split order,
-    // every load of this thread (4 entries x up to 16 splits) in flight at
-    // once; the acquire fence above makes the other blocks' stores visible
+}
+
+// W2 = T^T W for 32-column tiles tile0.. (grid (tiles, 1, problems)): W is
+// the split partials of qr_cq_w_kernel added in split order (every load of
+// a thread in flight at once), stored at w2[(jt0 + q) * kNB + p].  A launch
+// of its own: a last-block ticket would need a device-scope release fence
+// in every W block, which writes back each XCD's L2.
+__global__ __launch_bounds__(kCqT) void qr_cq_w2_kernel(const QrDesc* __restrict__ D, int64_t c0, int par, int tile0) {
+    const QrDesc& d = D[blockIdx.z];
+    if (c0 >= d.n || cq_failed(d)) return;
+    const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
+    const int64_t j0 = c0 + nbp, nt = d.n - j0;
+    const int tile = tile0 + static_cast<int>(blockIdx.x);
+    const int64_t jt0 = static_cast<int64_t>(tile) * kNB;
+    if (nt <= 0 || jt0 >= nt) return;
+    const int ns = cq_wsplits(d.k - c0);
+    const int ncols = static_cast<int>(min<int64_t>(kNB, nt - jt0));
+    const int64_t ntiles = (nt + kNB - 1) / kNB;
+    __shared__ __align__(16) double sw[kCqMat];
+    __shared__ __align__(16) double st[kCqMat];
+    const int t = threadIdx.x;
     {
         double v[16][4];
 #pragma unroll
         for (int sp = 0; sp < 16; ++sp)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                v[sp][q] = sp < ns ? d.wp[(static_cast<int64_t>(sp) * ntiles + blockIdx.x) * kCqMat + t + q * kCqT]
-                                   : 0.0;
+                v[sp][q] = sp < ns ? d.wp[(static_cast<int64_t>(sp) * ntiles + tile) * kCqMat + t + q * kCqT] : 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             double sum = 0.0;
 #pragma unroll
             for (int sp = 0; sp < 16; ++sp) sum += v[sp][q];
-            sred[t + q * kCqT] = sum;
+            sw[t + q * kCqT] = sum;
+            st[t + q * kCqT] = d.t[par * kCqMat + t + q * kCqT];  // T, column-major
         }
     }
-    for (int e = t; e < kCqMat; e += kCqT) sred[kCqMat + e] = d.t[e];  // T, column-major
     __syncthreads();
-    // W2(p, q) = sum_{r <= p} T(r, p) W(r, q), stored at w2[(jt0 + q) * kNB + p];
-    // a 2 x 2 block of W2 per thread (T is zero below its diagonal)
-    {
-        const int p0 = 2 * (t % 16), q0 = 2 * (t / 16);
-        double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+    // W2(p, q) = sum_{r <= p} T(r, p) W(r, q); a 2 x 2 block per thread (T is
+    // zero below its diagonal)
+    const int p0 = 2 * (t % 16), q0 = 2 * (t / 16);
+    double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
 #pragma unroll 8
-        for (int r = 0; r < kNB; ++r) {
-            const double t0 = sred[kCqMat + r + p0 * kNB], t1 = sred[kCqMat + r + (p0 + 1) * kNB];
-            const double w0 = sred[r * kNB + q0], w1 = sred[r * kNB + q0 + 1];
-            a00 += t0 * w0;
-            a01 += t0 * w1;
-            a10 += t1 * w0;
-            a11 += t1 * w1;
-        }
-        const bool p0in = p0 < nbp, p1in = p0 + 1 < nbp;
-        if (q0 < ncols) {
-            d.w2[(jt0 + q0) * kNB + p0] = p0in ? a00 : 0.0;
-            d.w2[(jt0 + q0) * kNB + p0 + 1] = p1in ? a10 : 0.0;
-        }
-        if (q0 + 1 < ncols) {
-            d.w2[(jt0 + q0 + 1) * kNB + p0] = p0in ? a01 : 0.0;
-            d.w2[(jt0 + q0 + 1) * kNB + p0 + 1] = p1in ? a11 : 0.0;
-        }
+    for (int r = 0; r < kNB; ++r) {
+        const double t0 = st[r + p0 * kNB], t1 = st[r + (p0 + 1) * kNB];
+        const double w0 = sw[r * kNB + q0], w1 = sw[r * kNB + q0 + 1];
+        a00 += t0 * w0;
+        a01 += t0 * w1;
+        a10 += t1 * w0;
+        a11 += t1 * w1;
+    }
+    const bool p0in = p0 < nbp, p1in = p0 + 1 < nbp;
+    if (q0 < ncols) {
+        d.w2[(jt0 + q0) * kNB + p0] = p0in ? a00 : 0.0;
+        d.w2[(jt0 + q0) * kNB + p0 + 1] = p1in ? a10 : 0.0;
+    }
+    if (q0 + 1 < ncols) {
+        d.w2[(jt0 + q0 + 1) * kNB + p0] = p0in ? a01 : 0.0;
+        d.w2[(jt0 + q0 + 1) * kNB + p0 + 1] = p1in ? a11 : 0.0;
     }
 }
 
@@ -1480,9 +1507,9 @@ Sizes sizes(int64_t k, int64_t n) {
     const int64_t nch = (kk + kGR - 1) / kGR;
     Sizes z;
     z.vec3 = a256(static_cast<size_t>(3 * nn) * 8);
-    z.v = a256(static_cast<size_t>(kk * kNB) * 8);
+    z.v = a256(static_cast<size_t>(2 * kk * kNB) * 8);  // two sets (panel parity) for the look-ahead
     z.gp = a256(static_cast<size_t>(nch * kNB * kNB) * 8);
-    z.t = a256(static_cast<size_t>(kNB * kNB) * 8);
+    z.t = a256(static_cast<size_t>(2 * kNB * kNB) * 8);
     z.wp = a256(static_cast<size_t>(std::max({qr_nsplit(kk) * nn * kNB, nn * nn,
                                               int64_t(16) * ((nn + kNB - 1) / kNB) * kCqMat})) * 8);
     z.w2 = a256(static_cast<size_t>(nn * kNB) * 8);
@@ -1589,33 +1616,65 @@ void cq_copy(const Batch& B, hipStream_t s) {
 }
 
 // CholeskyQR2 panels (see qr_cq_gram_kernel) with the blocked Householder
-// trailing update: per panel two Gram passes, Y, W = Y^T A, T^T W and
-// A -= Y T^T W, all problems in the same launches.  A is saved first.
+// trailing update, all problems in the same launches; A is saved first.
+// Per panel p: two Gram passes and their kNB x kNB algebra, Y, then W =
+// Y^T A, T^T W and A -= Y T^T W.  Look-ahead: the update of the 64 columns
+// after the panel (the next panel and the one after) runs on `s`, the rest
+// on the side stream, so the far update of panel p overlaps panel p+1's
+// factorization (few blocks: it is latency-bound).  Panel p+1's near update
+// waits for panel p's far update (the columns it reads); Y and T
+// alternate between two sets by panel parity so that a far update can
+// still read them while the next panel forms its own.
 void factor_cq(const Batch& B, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
     cq_copy<true>(B, s);
     qr_sync_reset_kernel<<<dim3(static_cast<unsigned>(ceil_div(kSyncHead + B.nmax, kT)), 1, P), kT, 0, s>>>(
         B.dev, B.status);  // the W tickets
     TOMOS_LAUNCH_CHECK("qr_sync_reset_kernel");
+    const SideStream& side = side_stream();
+    bool side_used = false;
     for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
+        const int p = static_cast<int>(c0 / kNB), par = p & 1;
         const int64_t rows = B.kmax - c0;
         const unsigned gx = static_cast<unsigned>(std::min<int64_t>(64, ceil_div(rows, 2 * kCqTile)));
         qr_cq_gram_kernel<1><<<dim3(gx, 1, P), kCqT, 0, s>>>(B.dev, c0);
-        qr_cq_small_kernel<1><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_cq_small_kernel<1><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0, par);
         qr_cq_gram_kernel<2><<<dim3(gx, 1, P), kCqT, 0, s>>>(B.dev, c0);
-        qr_cq_small_kernel<2><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0);
+        qr_cq_small_kernel<2><<<dim3(1, 1, P), kCqT, 0, s>>>(B.dev, c0, par);
         TOMOS_LAUNCH_CHECK("qr_cq_gram_kernel");
         if (c0 + kNB >= B.nmax) break;  // no problem has columns right of this panel
         const int64_t nt = B.nmax - c0 - kNB;
-        qr_cq_form_y_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, kCqYT)), 1, P), kCqYT, 0, s>>>(B.dev, c0);
-        qr_cq_w_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, kNB)), 16, P), kCqT, 0, s>>>(B.dev, c0);
-        qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
-                           kT, 0, s>>>(B.dev, c0);
-        TOMOS_LAUNCH_CHECK("qr_cq trailing update");
+        qr_cq_form_y_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, kCqYT)), 1, P), kCqYT, 0, s>>>(B.dev, c0, par);
+        const int wtiles = static_cast<int>(ceil_div(nt, kNB)), utiles = static_cast<int>(ceil_div(nt, 64));
+        const int wnear = std::min(wtiles, 2), unear = std::min(utiles, 1);  // the first 64 columns
+        const unsigned urows = static_cast<unsigned>(ceil_div(rows, 64));
+        if (utiles > unear) {
+            TOMOS_HIP_CHECK(hipEventRecord(side.fork, s));
+            TOMOS_HIP_CHECK(hipStreamWaitEvent(side.stream, side.fork, 0));
+            qr_cq_w_kernel<<<dim3(static_cast<unsigned>(wtiles - wnear), 16, P), kCqT, 0, side.stream>>>(B.dev, c0, par,
+                                                                                                        wnear);
+            qr_cq_w2_kernel<<<dim3(static_cast<unsigned>(wtiles - wnear), 1, P), kCqT, 0, side.stream>>>(B.dev, c0, par,
+                                                                                                        wnear);
+            qr_update_kernel<<<dim3(urows, static_cast<unsigned>(utiles - unear), P), kT, 0, side.stream>>>(B.dev, c0,
+                                                                                                       par, unear);
+            TOMOS_LAUNCH_CHECK("qr_cq far update");
+            TOMOS_HIP_CHECK(hipEventRecord(side.join[par], side.stream));
+            side_used = true;
+        }
+        if (p >= 1 && side_used) TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[par ^ 1], 0));
+        qr_cq_w_kernel<<<dim3(static_cast<unsigned>(wnear), 16, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
+        qr_cq_w2_kernel<<<dim3(static_cast<unsigned>(wnear), 1, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
+        qr_update_kernel<<<dim3(urows, static_cast<unsigned>(unear), P), kT, 0, s>>>(B.dev, c0, par, 0);
+        TOMOS_LAUNCH_CHECK("qr_cq near update");
+    }
+    if (side_used) {  // every far update is done before R is read
+        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[0], 0));
+        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[1], 0));
     }
     extract_r_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax * B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
     TOMOS_LAUNCH_CHECK("extract_r_kernel");
 }
+
 
 void factor(const Batch& B, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
@@ -1683,7 +1742,7 @@ void factor(const Batch& B, hipStream_t s) {
                       kT, 0, s>>>(B.dev, c0);
         qr_w2_kernel<<<dim3(static_cast<unsigned>(ceil_div(nt, 64)), 1, P), kT, 0, s>>>(B.dev, c0);
         qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
-                           kT, 0, s>>>(B.dev, c0);
+                           kT, 0, s>>>(B.dev, c0, 0, 0);
         TOMOS_LAUNCH_CHECK("qr trailing update");
     }
     if (lookahead) {  // every side-stream update is done before R is read
