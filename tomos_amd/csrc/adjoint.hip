@@ -51,6 +51,7 @@ struct AdjState {
     double gnorm;     // |g|
     double zero;      // 0.0: u coefficient of the single-pass kernel (q = A d)
     double resnorm;   // |res| after the last update
+    double res0;      // |res| at the start of the current restart (exact residual)
     int done;         // final: converged or restarts exhausted
     int inner_done;   // ends the inner iterations of the current restart
     int stop;         // done || inner_done (checked by every inner kernel)
@@ -60,7 +61,7 @@ struct AdjState {
 
 __global__ void k_adj_init(AdjState* S) {
     if (threadIdx.x == 0) {
-        S->rho = S->gnorm = S->zero = S->resnorm = 0.0;
+        S->rho = S->gnorm = S->zero = S->resnorm = S->res0 = 0.0;
         S->done = S->inner_done = S->stop = 0;
         S->completed = S->hist_len = 0;
     }
@@ -102,6 +103,7 @@ __global__ __launch_bounds__(kBlk) void k_adj_residual(AdjState* S, const double
     if (threadIdx.x != 0) return;
     if (!rfull) S->gnorm = sqrt(gg);
     S->resnorm = sqrt(rr);
+    S->res0 = S->resnorm;
     S->inner_done = 0;
     S->stop = 0;
     if (S->gnorm == 0.0) { S->done = 1; S->stop = 1; return; }
@@ -132,7 +134,8 @@ __global__ __launch_bounds__(kBlk) void k_adj_step(AdjState* S, const double* __
                                                    int nq, const double* __restrict__ tsum,
                                                    int splits, const double* __restrict__ d,
                                                    double* __restrict__ w, double* __restrict__ res,
-                                                   int64_t n, double* hist, int cap, double tol) {
+                                                   int64_t n, double* hist, int cap, double tol,
+                                                   double drop) {
     if (S->stop) return;
     __shared__ double sh[kBlk / 64];
     double q2 = 0.0;
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(kBlk) void k_adj_step(AdjState* S, const double* __
     S->resnorm = sqrt(rr);
     const double rel = S->resnorm / S->gnorm;
     push_hist(S, hist, cap, rel);
-    if (rel < tol || q2 <= 0.0) { S->inner_done = 1; S->stop = 1; }
+    if (rel < tol || q2 <= 0.0 || S->resnorm < drop * S->res0) { S->inner_done = 1; S->stop = 1; }
 }
 
 // rho' = res.z, d = z + (rho'/rho) d
@@ -361,7 +364,7 @@ struct Adj {
                  TOMOS_F32, sc + L.q32, TOMOS_F64, buf(L.t), gs(), L.gemv_bytes, s);
         }
         k_adj_step<<<1, kBlk, 0, s>>>(st, buf(L.qsq), nq, tsrc, splits, buf(L.d), w, buf(L.res), n,
-                                      buf(L.hist), L.hist_cap, o.tol);
+                                      buf(L.hist), L.hist_cap, o.tol, adjoint_inner_drop(o));
         const double* z = precond(&st->stop);
         k_adj_dir<<<1, kBlk, 0, s>>>(st, buf(L.res), z, buf(L.d), n);
     }

@@ -49,6 +49,7 @@ enum : int { kPhaseRestart = 0, kPhaseIterate = 1, kPhaseDone = 2 };
 
 struct GState {
     double rho, gnorm, resnorm;
+    double res0;        // |res| at the start of the current restart
     double zero;        // 0.0: u coefficient of the single-pass kernel (u = A d)
     int phase;          // kPhase*
     int restarts_done;  // exact residuals formed so far
@@ -75,7 +76,7 @@ __global__ void k_gadj_init(const GProb* __restrict__ P, int nprob) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= nprob) return;
     GState* s = P[i].st;
-    s->rho = s->gnorm = s->resnorm = s->zero = 0.0;
+    s->rho = s->gnorm = s->resnorm = s->zero = s->res0 = 0.0;
     s->phase = P[i].n > 0 ? kPhaseRestart : kPhaseDone;
     s->restarts_done = s->iters = s->completed = 0;
     s->skip_restart = 1;  // the first restart has w = 0: res = g, no products
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict_
     if (threadIdx.x == 0) {
         if (first) S->gnorm = sqrt(gg);
         S->resnorm = sqrt(rr);
+        S->res0 = S->resnorm;
         S->restarts_done = k + 1;
         S->skip_restart = 1;
         const double rel = S->gnorm > 0.0 ? S->resnorm / S->gnorm : 0.0;
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict_
 // grouped single-pass kernel left |q|^2 and A^T q as per-item partials.
 // The restart's inner loop ends when |res|/|g| < tol, |q| = 0 or after
 // max_iters steps; the problem then waits for its next restart.
-__global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P, double tol, int max_iters) {
+__global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P, double tol, int max_iters,
+                                                    double drop) {
     const GProb& p = P[blockIdx.x];
     GState* S = p.st;
     if (S->phase != kPhaseIterate) return;
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P
         S->iters = iters + 1;
         S->resnorm = sqrt(rr);
         const double rel = S->resnorm / S->gnorm;
-        const bool end = rel < tol || q2 <= 0.0 || iters + 1 >= max_iters;
+        const bool end = rel < tol || q2 <= 0.0 || iters + 1 >= max_iters || S->resnorm < drop * S->res0;
         go = end ? 0 : 1;
         if (end) {
             S->phase = kPhaseRestart;
@@ -439,7 +442,8 @@ struct Grouped {
     void iter_seq() {
         for (int it = 0; it < kSegIters; ++it) {
             if (L.nitems) grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s);
-            k_gadj_step<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.max_iters);
+            k_gadj_step<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.max_iters,
+                                                                          adjoint_inner_drop(o));
         }
         TOMOS_LAUNCH_CHECK("k_gadj_step");
     }
