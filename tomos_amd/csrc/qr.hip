@@ -6,22 +6,28 @@
 // blockIdx.z, so a batch of uneven problems costs the launches of the
 // largest one.
 //
-// QR, blocked by panels of NB = 32 columns (LAPACK dgeqrf's structure):
-//  * panel: one launch per panel, one block per panel column.  The block
-//    owning column j applies the reflectors c < j as each column c becomes
-//    final (a per-column flag, published with release/acquire at device
-//    scope), then forms column j's reflector (LAPACK dlarfg convention:
-//    beta = -sign(x0)|x|, v = [1, x(1:)/(x0-beta)], tau = (beta-x0)/beta; a
-//    zero column gives no reflection) and publishes it: a wavefront over
-//    the panel's columns instead of one launch per column.  Column c keeps
-//    x (the reflector is implicit); R's diagonal, tau and the reflector
-//    scale go to separate arrays.
-//  * V and T: the panel's reflectors are written as an explicit unit lower
-//    trapezoidal V, and T (upper triangular, H_1...H_nb = I - V T V^T) is
-//    formed from the Gram matrix V^T V (dlarft's recurrence).
-//  * trailing update A <- (I - V T^T V^T) A over the columns right of the
-//    panel, as two tiled FP64 products: W = V^T A (split over rows, partials
-//    summed in a fixed order, then W2 = T^T W once) and A -= V W2.
+// QR, blocked by panels of NB = 32 columns (LAPACK dgeqrf's structure).
+// Three panel factorizations, all giving R up to the signs of its rows:
+//  * CholeskyQR2 panels (the default, tomos_qr_panels_set(0)): two Gram
+//    passes over row chunks, the 32 x 32 Cholesky factors, and the
+//    panel's Householder vectors reconstructed from Q by an LU without
+//    pivoting, then the blocked update A -= Y T^T (Y^T A) with a look-ahead
+//    (see qr_cq_gram_kernel).  A sketch where a panel is too
+//    ill-conditioned for two Gram passes (or rank deficient) is factored
+//    again from a saved copy with the column-by-column panels below.
+//  * TSQR panels (k <= 8192): leaves of 256 or 512 rows factored column by
+//    column in registers, their stacked R factors likewise, and the trailing
+//    columns updated with each level's block reflector (qr_tsqr_*).
+//  * wavefront panels (k > 8192): one launch per panel, one block per
+//    panel column.  The block owning column j applies the reflectors c < j
+//    as each column c becomes final (a per-column flag, published with
+//    release/acquire at device scope), then forms column j's reflector
+//    (LAPACK dlarfg convention: beta = -sign(x0)|x|, v = [1,
+//    x(1:)/(x0-beta)], tau = (beta-x0)/beta; a zero column gives no
+//    reflection) and publishes it; V, T (dlarft's recurrence from V^T V)
+//    and the trailing update A <- (I - V T^T V^T) A as two tiled FP64
+//    products: W = V^T A (split over rows, partials summed in a fixed
+//    order, then W2 = T^T W once) and A -= V W2.
 // All reductions run in a fixed order (deterministic).
 //
 // Triangular inverse: one block per (column j, problem) solves R x = e_j by
