@@ -64,6 +64,33 @@ def test_apply_vs_oracle(capi, orc, a_type, tier, m, lda, n, s):
     assert err < 2e-6, err
 
 
+@pytest.mark.parametrize("a_type,tier", [(C.F32, 0), (C.FP32_BF16, 2), (C.FP32_16, 2)])
+@pytest.mark.parametrize("spike", [False, True])
+def test_apply_sampled_exponents(capi, orc, a_type, tier, spike):
+    """A tall enough A takes the fixed-point exponents from a row sample
+    (plus a margin); with `spike` one value far above the sampled range
+    sits in a row no sample covers, so the bound check sends the call back
+    to exponents over every row.  Both match the oracle."""
+    m, n, s = 300000, 9, 2000
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((m, n)).astype(np.float32, order="F")
+    if spike:
+        A[5000, 3] = 3.0e7  # rows 1024..9367 are between the first two sampled segments
+    if a_type == C.F32:
+        Aref = A.astype(np.float64)
+        Y = H.countsketch_apply(a_type, tier, m, n, H.dev(A), None, m, s)
+    else:
+        variant = C.PACK_FP32_BF16 if a_type == C.FP32_BF16 else C.PACK_FP32_16
+        wh, wr = H.pack(variant, A)
+        eh, er = orc.pack(0 if a_type == C.FP32_16 else 3, A)
+        Aref = orc.unpack(4 if a_type == C.FP32_BF16 else 0, eh, er).astype(np.float64)
+        Y = H.countsketch_apply(a_type, tier, m, n, wh, wr, m, s)
+    ref = orc.countsketch_apply(Aref, s)
+    for j in range(n):  # column by column: the spike dominates its column only
+        err = np.abs(Y[:, j] - ref[:, j]).max() / np.abs(ref[:, j]).max()
+        assert err < 2e-6, (j, err)
+
+
 def test_norm_preservation(capi):
     """E |Sx|^2 / |x|^2 ~ 1 (test_countsketch_cpu_kernels.cpp:93-179, 200 trials)."""
     m, s, trials = 5000, 500, 200

@@ -258,17 +258,18 @@ __global__ __launch_bounds__(256) void cs_finish_kernel(int64_t s, int64_t n, in
 // colmax_bits[j] = max over rows of |value| as float bits (monotone for
 // non-negative floats) of the values the sketch sums, or (BOUND_HI) of the
 // hi words, which bound the exact FP32_BF16 values within a factor 2.
-// grid (n, splits); 16-byte row vectors.
+// grid (n, segments): segment y covers rows [y stride, y stride + len)
+// (stride == len: every row; stride > len: a sample); 16-byte row vectors.
 template <int FMT, bool EXACT, bool NTL>
 __global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* __restrict__ a,
                                                         const uint16_t* __restrict__ wr, int64_t lda,
-                                                        int64_t rows_per_split,
+                                                        int64_t stride, int64_t len,
                                                         unsigned* __restrict__ colmax_bits) {
     constexpr int V = vec_rows<FMT>();
     constexpr int EB = 16 / V;
     const int64_t j = blockIdx.x;
-    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rows_per_split;
-    const int64_t r1 = min(m, r0 + rows_per_split);
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * stride;
+    const int64_t r1 = min(m, r0 + len);
     const char* col = static_cast<const char*>(a) + j * lda * EB;
     float mx = 0.0f;
     for (int64_t i0 = r0 + static_cast<int64_t>(threadIdx.x) * V; i0 < r1; i0 += 256 * V) {
@@ -294,9 +295,11 @@ __global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* _
     if ((threadIdx.x & 63) == 0) atomicMax(colmax_bits + j, __float_as_uint(mx));
 }
 
-// e_j from the max bits; flag = 1 if any column is not finite
+// e_j from the max bits, plus `margin` bits when the max is a sample's;
+// flag = 1 if any column is not finite
 __global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigned* __restrict__ colmax_bits,
-                                                        int* __restrict__ colexp, int* __restrict__ flag) {
+                                                        int margin, int* __restrict__ colexp,
+                                                        int* __restrict__ flag) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (j >= n) return;
     const unsigned bits = colmax_bits[j];
@@ -310,14 +313,14 @@ __global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigne
         frexpf(__uint_as_float(bits), &e);
         e += 1;  // margin: an exact FP32_BF16 value is below twice its hi word
     }
-    colexp[j] = e;
+    colexp[j] = e + margin;
 }
 
 template <int FMT, bool EXACT, int CG, bool NTL>
 __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
     int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
     int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
-    const int* __restrict__ colexp, int shift, double* __restrict__ partial) {
+    const int* __restrict__ colexp, int shift, double* __restrict__ partial, int* __restrict__ over) {
     constexpr int V = vec_rows<FMT>();
     constexpr int EB = 16 / V;
     extern __shared__ unsigned long long yfx[];  // [CG][s]
@@ -327,8 +330,13 @@ __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
     const int ncols = static_cast<int>(min<int64_t>(CG, n - c0));
     for (int64_t t = threadIdx.x; t < s * CG; t += kVecThreads) yfx[t] = 0ull;
     int sc[CG];
+    float bnd[CG];  // |value| < 2^e_c keeps every term below 2^shift (no overflow)
 #pragma unroll
-    for (int c = 0; c < CG; ++c) sc[c] = c < ncols ? shift - colexp[c0 + c] : 0;
+    for (int c = 0; c < CG; ++c) {
+        sc[c] = c < ncols ? shift - colexp[c0 + c] : 0;
+        bnd[c] = c < ncols ? ldexpf(1.0f, colexp[c0 + c]) : 0.0f;
+    }
+    bool beyond = false;  // a value at or above its column's bound (or not finite)
     __syncthreads();
 
     const int64_t r0 = static_cast<int64_t>(split) * rows_per_split;
@@ -370,6 +378,7 @@ __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
 #pragma unroll
                     for (int r = 0; r < V; ++r) {
                         const int b = static_cast<int>(w[r] & 0x7FFFFFFFu);
+                        beyond = beyond || !(fabsf(v[r]) < bnd[c]);
                         const long long q = __float2ll_rn(ldexpf(v[r], sc[c]));
                         atomicAdd(&yfx[c * s + b],
                                   static_cast<unsigned long long>((w[r] & 0x80000000u) ? -q : q));
@@ -382,13 +391,14 @@ __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
                 const int b = static_cast<int>(w & 0x7FFFFFFFu);
                 for (int c = 0; c < ncols; ++c) {
                     const float v = load_elem<FMT, EXACT>(a, wr, i + (c0 + c) * lda);
+                    beyond = beyond || !(fabsf(v) < ldexpf(1.0f, colexp[c0 + c]));
                     const long long q = __float2ll_rn(ldexpf(v, sc[c]));
                     atomicAdd(&yfx[c * s + b], static_cast<unsigned long long>((w & 0x80000000u) ? -q : q));
                 }
             }
         }
     }
-    __syncthreads();
+    if (__syncthreads_or(beyond) && threadIdx.x == 0) atomicOr(over, 1);
     double* out = partial + static_cast<int64_t>(split) * s * n;
     for (int64_t t = threadIdx.x; t < s * ncols; t += kVecThreads) {
         const int64_t c = t / s;
@@ -525,8 +535,17 @@ void launch_apply_cg(const CsPlan& p, int64_t m, int64_t n, const void* a, const
     }
 }
 
-// Fixed-point apply on the vector path; returns false (nothing launched
-// after the exponent pass) when a column's max is not finite.
+// Fixed-point apply on the vector path.  The column exponents come from a
+// sample of the rows (kSampleSegs segments of kSampleLen rows spread over
+// A) plus kSampleMargin bits when A is tall enough for the sample to save
+// a pass, else from every row.  The apply kernel checks every value
+// against its column's bound; a value beyond it (or not finite) makes the
+// call redo the exponents from every row and apply again.  Returns false
+// (the float path runs) when a column's max over every row is not finite.
+constexpr int64_t kSampleLen = 1024;
+constexpr int64_t kSampleSegs = 32;
+constexpr int kSampleMargin = 12;  // 2^-31 of the sampled max per term at 2e6 rows, 4 splits
+
 template <int FMT, bool EXACT>
 bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
                   const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy,
@@ -535,29 +554,37 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
     const CsPlan p = cs_plan_fx(m, n, s);
     unsigned* colmax = reinterpret_cast<unsigned*>(scratch + L.colmax);
     int* colexp = reinterpret_cast<int*>(scratch + L.colexp);
-    int* flag = reinterpret_cast<int*>(scratch + L.flag);
+    int* flag = reinterpret_cast<int*>(scratch + L.flag);  // [0] not finite, [1] beyond a bound
     double* partial = reinterpret_cast<double*>(scratch + L.partial);
-    TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
-    TOMOS_HIP_CHECK(hipMemsetAsync(flag, 0, 4, st));
-    const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
-    const int64_t crows = ceil_div(ceil_div(m, csplits), 8) * 8;
     require(n < (int64_t(1) << 31), "countsketch_apply: n too large");
     // FP32_BF16's exact values are bounded by 2x their hi words: the hi plane
     // suffices; FP32_16's exact values need their own decode
     constexpr bool kExactMax = EXACT && FMT == FMT_P16;
-    // the apply pass rereads A right after this one: a matrix that fits the
-    // MALL is read from HBM once
     constexpr int64_t kElemBytes = (FMT == FMT_F64 ? 8 : FMT == FMT_F32 ? 4 : 2) * (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF) ? 2 : 1);
     const bool ntl = stream_loads(m * n * kElemBytes);
-    const dim3 cgrid(static_cast<unsigned>(n), static_cast<unsigned>(ceil_div(m, crows)));
-    if (ntl) cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, crows, colmax);
-    else cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, crows, colmax);
-    cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(n, colmax, colexp, flag);
-    TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
-    int hflag = 0;
-    TOMOS_HIP_CHECK(hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, st));
-    TOMOS_HIP_CHECK(hipStreamSynchronize(st));
-    if (hflag) return false;
+    auto exponents = [&](bool sample) {
+        TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
+        TOMOS_HIP_CHECK(hipMemsetAsync(flag, 0, 8, st));
+        int64_t stride, len;
+        unsigned segs;
+        if (sample) {
+            stride = (m / kSampleSegs) / 8 * 8;
+            len = kSampleLen;
+            segs = static_cast<unsigned>(kSampleSegs);
+        } else {
+            const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
+            stride = len = ceil_div(ceil_div(m, csplits), 8) * 8;
+            segs = static_cast<unsigned>(ceil_div(m, len));
+        }
+        const dim3 cgrid(static_cast<unsigned>(n), segs);
+        if (ntl && !sample)
+            cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
+        else
+            cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
+        cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+            n, colmax, sample ? kSampleMargin : 0, colexp, flag);
+        TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
+    };
     const int shift = fx_shift(p.rows_per_split);
     const size_t lds = static_cast<size_t>(s) * p.cg * 8;
     const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
@@ -567,27 +594,44 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         k<<<dim3(static_cast<unsigned>(blocks)), kVecThreads, lds, st>>>(m, n, a, wr, lda, s, packed,
                                                                         p.rows_per_split, p.ncg, colexp, shift,
-                                                                        partial);
+                                                                        partial, flag + 1);
     };
-    if (ntl) {
-        switch (p.cg) {
-            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, true>); break;
-            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, true>); break;
-            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, true>); break;
-            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, true>); break;
+    auto apply = [&]() {
+        if (ntl) {
+            switch (p.cg) {
+                case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, true>); break;
+                case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, true>); break;
+                case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, true>); break;
+                default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, true>); break;
+            }
+        } else {
+            switch (p.cg) {
+                case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, false>); break;
+                case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, false>); break;
+                case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, false>); break;
+                default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, false>); break;
+            }
         }
-    } else {
-        switch (p.cg) {
-            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, false>); break;
-            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, false>); break;
-            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, false>); break;
-            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, false>); break;
-        }
+        TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
+        cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits,
+                                                                                              partial, y, ldy);
+        TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
+    };
+    int hflag[2] = {0, 0};
+    auto read_flags = [&]() {
+        TOMOS_HIP_CHECK(hipMemcpyAsync(hflag, flag, 8, hipMemcpyDeviceToHost, st));
+        TOMOS_HIP_CHECK(hipStreamSynchronize(st));
+    };
+    if (m >= 8 * kSampleSegs * kSampleLen) {  // sampled exponents: one pass over A when they hold
+        exponents(true);
+        apply();
+        read_flags();
+        if (!hflag[0] && !hflag[1]) return true;
     }
-    TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
-    cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits, partial,
-                                                                                          y, ldy);
-    TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
+    exponents(false);
+    read_flags();
+    if (hflag[0]) return false;
+    apply();
     return true;
 }
 

@@ -78,8 +78,9 @@ constexpr int64_t kTsqrSet = kTsqrMaxLeaves * kNB * kNB + kTsqrComb * kNB + kNB 
 constexpr int64_t kTsqrMaxK = 16 * 512;      // k beyond this: the wavefront panel path
 
 // CholeskyQR2 panel state per problem: three kNB x kNB matrices, then ints
-// [0], [1] the ticket counters of the two Gram passes and [2] the failure
-// flag (nonzero: a panel could not be factored this way).
+// [0], [1] (unused), [2] the failure flag (nonzero: a panel could not be
+// factored this way) and [3] "one Gram pass suffices" for the current
+// panel.
 constexpr int kCqMat = kNB * kNB;
 constexpr int kCqDoubles = 3 * kCqMat;
 static_assert(kCqMat == 4 * 256, "the CholeskyQR2 kernels give each of 256 threads four kNB x kNB entries");
@@ -800,6 +801,13 @@ constexpr int kCqTile = 64;          // rows per LDS tile
 constexpr int kCqLD = kNB + 2;       // LDS row stride of a tile (16-byte aligned rows)
 constexpr double kCqPivot = 1e-15;   // relative Cholesky pivot floor
 constexpr double kCqOrth = 0.1;      // max |G2 - I| for which two passes give an orthonormal Q
+// One Gram pass suffices when the column-normalized Gram matrix Gn = D^-1 G1
+// D^-1 (D = sqrt(diag G1)) has every off-diagonal absolute row sum below
+// kCqOnePass: by Gershgorin its eigenvalues then lie in [1 - 0.8, 1 + 0.8],
+// cond(P D^-1) < 3, and Q = P R1^-1 is orthonormal to a few ulps (Cholesky
+// is invariant under column scaling, so the graded columns of a sketch do
+// not count against it).
+constexpr double kCqOnePass = 0.8;
 
 // Rows per Gram chunk: a multiple of the tile, at least two tiles, at most
 // 64 chunks.  Chunks never exceed ceil(rows / 128).
@@ -988,6 +996,7 @@ __global__ void qr_cq_reset_kernel(const QrDesc* __restrict__ D, int nprob) {
         ci[0] = 0;
         ci[1] = 0;
         ci[2] = 0;
+        ci[3] = 0;
     }
 }
 
@@ -1014,6 +1023,7 @@ template <int PASS>
 __global__ __launch_bounds__(kCqT) void qr_cq_gram_kernel(const QrDesc* __restrict__ D, int64_t c0) {
     const QrDesc& d = D[blockIdx.z];
     if (c0 >= d.n || cq_failed(d)) return;
+    if (PASS == 2 && cq_ints(d)[3]) return;  // one pass suffices for this panel
     const int nbp = static_cast<int>(min<int64_t>(kNB, d.n - c0));
     const int64_t rows = d.k - c0;
     const int64_t cr = cq_chunk_rows(rows);
@@ -1135,7 +1145,8 @@ __global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restr
     double* B5 = sm + 5 * kCqMat;
     double* B6 = sm + 6 * kCqMat;
     int* ci = cq_ints(d);
-    cq_sum_partials(d, nch, B0);
+    const bool one_pass = PASS == 2 && ci[3] != 0;  // read before thread 0 can rewrite it
+    if (!one_pass) cq_sum_partials(d, nch, B0);
     if (PASS == 2) {
         for (int e = t; e < kCqMat; e += kCqT) {
             B3[e] = d.cq[e];           // R1^-1
@@ -1145,7 +1156,10 @@ __global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restr
         }
     }
     __syncthreads();
-    if (w == 0) {
+    if (one_pass) {  // R2 = R2^-1 = I
+        for (int e = t; e < kCqMat; e += kCqT) B1[e] = B2[e] = (e / kNB == e % kNB) ? 1.0 : 0.0;
+        if (t == 0) s_ok = 1;
+    } else if (w == 0) {
         bool far = false;
         if (PASS == 2) {
             const int j = l & 31;
@@ -1154,7 +1168,23 @@ __global__ __launch_bounds__(kCqT) void qr_cq_small_kernel(const QrDesc* __restr
         }
         const bool ok = cq_wave_chol(B0, B1, bc, nbp) && !__any(far);  // R1 or R2
         cq_wave_inv_upper(B1, B2);                                     // its inverse
-        if (l == 0) s_ok = ok;
+        int one = 0;
+        if (PASS == 1) {  // Gershgorin row sums of the column-normalized G1 (lane j: column j)
+            const int j = l & 31;
+            const double dj = j < nbp ? sqrt(B0[j * kNB + j]) : 1.0;
+            cq_order();
+            bc[j] = dj;
+            cq_order();
+            double rs = 0.0;
+            for (int k = 0; k < nbp; ++k)
+                if (k != j && j < nbp) rs += fabs(B0[k * kNB + j]) / (bc[k] * dj);
+            for (int o = 16; o >= 1; o >>= 1) rs = fmax(rs, __shfl_xor(rs, o));
+            one = rs < kCqOnePass ? 1 : 0;  // NaN (a failed Cholesky) gives 0
+        }
+        if (l == 0) {
+            s_ok = ok;
+            if (PASS == 1) ci[3] = one;
+        }
     }
     __syncthreads();
     if (!s_ok) {
@@ -1623,24 +1653,18 @@ void cq_copy(const Batch& B, hipStream_t s) {
 
 // CholeskyQR2 panels (see qr_cq_gram_kernel) with the blocked Householder
 // trailing update, all problems in the same launches; A is saved first.
-// Per panel p: two Gram passes and their kNB x kNB algebra, Y, then W =
-// Y^T A, T^T W and A -= Y T^T W.  Look-ahead: the update of the 64 columns
-// after the panel (the next panel and the one after) runs on `s`, the rest
-// on the side stream, so the far update of panel p overlaps panel p+1's
-// factorization (few blocks: it is latency-bound).  Panel p+1's near update
-// waits for panel p's far update (the columns it reads); Y and T
-// alternate between two sets by panel parity so that a far update can
-// still read them while the next panel forms its own.
+// Per panel p: two Gram passes (the second skipped when one suffices) and
+// their kNB x kNB algebra, Y, then W = Y^T A, T^T W and A -= Y T^T W, in
+// order on one stream.  (An overlap of each panel's far trailing update
+// with the next panel's factorization on a second stream was measured
+// slower at 4000 x 1000: the cross-stream events cost ~20 us per panel and
+// the update's blocks slowed the latency-bound panel kernels.)  Y and T
+// alternate between two sets by panel parity; the kernels take the set.
 void factor_cq(const Batch& B, hipStream_t s) {
     const unsigned P = static_cast<unsigned>(B.nprob);
     cq_copy<true>(B, s);
-    qr_sync_reset_kernel<<<dim3(static_cast<unsigned>(ceil_div(kSyncHead + B.nmax, kT)), 1, P), kT, 0, s>>>(
-        B.dev, B.status);  // the W tickets
-    TOMOS_LAUNCH_CHECK("qr_sync_reset_kernel");
-    const SideStream& side = side_stream();
-    bool side_used = false;
     for (int64_t c0 = 0; c0 < B.nmax; c0 += kNB) {
-        const int p = static_cast<int>(c0 / kNB), par = p & 1;
+        const int par = static_cast<int>(c0 / kNB) & 1;
         const int64_t rows = B.kmax - c0;
         const unsigned gx = static_cast<unsigned>(std::min<int64_t>(64, ceil_div(rows, 2 * kCqTile)));
         qr_cq_gram_kernel<1><<<dim3(gx, 1, P), kCqT, 0, s>>>(B.dev, c0);
@@ -1650,32 +1674,13 @@ void factor_cq(const Batch& B, hipStream_t s) {
         TOMOS_LAUNCH_CHECK("qr_cq_gram_kernel");
         if (c0 + kNB >= B.nmax) break;  // no problem has columns right of this panel
         const int64_t nt = B.nmax - c0 - kNB;
+        const unsigned wtiles = static_cast<unsigned>(ceil_div(nt, kNB));
         qr_cq_form_y_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, kCqYT)), 1, P), kCqYT, 0, s>>>(B.dev, c0, par);
-        const int wtiles = static_cast<int>(ceil_div(nt, kNB)), utiles = static_cast<int>(ceil_div(nt, 64));
-        const int wnear = std::min(wtiles, 2), unear = std::min(utiles, 1);  // the first 64 columns
-        const unsigned urows = static_cast<unsigned>(ceil_div(rows, 64));
-        if (utiles > unear) {
-            TOMOS_HIP_CHECK(hipEventRecord(side.fork, s));
-            TOMOS_HIP_CHECK(hipStreamWaitEvent(side.stream, side.fork, 0));
-            qr_cq_w_kernel<<<dim3(static_cast<unsigned>(wtiles - wnear), 16, P), kCqT, 0, side.stream>>>(B.dev, c0, par,
-                                                                                                        wnear);
-            qr_cq_w2_kernel<<<dim3(static_cast<unsigned>(wtiles - wnear), 1, P), kCqT, 0, side.stream>>>(B.dev, c0, par,
-                                                                                                        wnear);
-            qr_update_kernel<<<dim3(urows, static_cast<unsigned>(utiles - unear), P), kT, 0, side.stream>>>(B.dev, c0,
-                                                                                                       par, unear);
-            TOMOS_LAUNCH_CHECK("qr_cq far update");
-            TOMOS_HIP_CHECK(hipEventRecord(side.join[par], side.stream));
-            side_used = true;
-        }
-        if (p >= 1 && side_used) TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[par ^ 1], 0));
-        qr_cq_w_kernel<<<dim3(static_cast<unsigned>(wnear), 16, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
-        qr_cq_w2_kernel<<<dim3(static_cast<unsigned>(wnear), 1, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
-        qr_update_kernel<<<dim3(urows, static_cast<unsigned>(unear), P), kT, 0, s>>>(B.dev, c0, par, 0);
-        TOMOS_LAUNCH_CHECK("qr_cq near update");
-    }
-    if (side_used) {  // every far update is done before R is read
-        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[0], 0));
-        TOMOS_HIP_CHECK(hipStreamWaitEvent(s, side.join[1], 0));
+        qr_cq_w_kernel<<<dim3(wtiles, 16, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
+        qr_cq_w2_kernel<<<dim3(wtiles, 1, P), kCqT, 0, s>>>(B.dev, c0, par, 0);
+        qr_update_kernel<<<dim3(static_cast<unsigned>(ceil_div(rows, 64)), static_cast<unsigned>(ceil_div(nt, 64)), P),
+                           kT, 0, s>>>(B.dev, c0, par, 0);
+        TOMOS_LAUNCH_CHECK("qr_cq trailing update");
     }
     extract_r_kernel<<<dim3(static_cast<unsigned>(ceil_div(B.nmax * B.nmax, kT)), 1, P), kT, 0, s>>>(B.dev);
     TOMOS_LAUNCH_CHECK("extract_r_kernel");
