@@ -872,8 +872,12 @@ void timer_destroy() {
 
 FusedShape lsqr_fused_default_shape(int64_t n) {
     // 128 contiguous bytes per column per tile (G = 8): the fetch granule,
-    // so no over-fetch; up to n = 1024 the whole tile fits one 256-thread
-    // block's registers.  Beyond, 64-byte runs (G = 4, twice the fetch).
+    // so no over-fetch.  Above 512 columns, 512 threads with the tiles staged
+    // through LDS by DMA (the shape the autotuner picks at 2e6 x 1000:
+    // 0.64 ms per launch against 0.80 ms for 256 threads); up to 1024
+    // columns the whole tile fits the block.  Beyond, 64-byte runs (G = 4,
+    // twice the fetch).
+    if (n > 512 && n <= 1024) return FusedShape{8, 512};
     if (n <= 1024) return FusedShape{8, 256};
     return FusedShape{4, 512};
 }
