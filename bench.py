@@ -203,7 +203,7 @@ def sparse_leg(args, C, DeviceBuffer, ev0, ev1):
 def batched_leg(args, C, DeviceBuffer, ev0, ev1):
     """64 uneven problems: m ~ U{20000..200000}, n ~ U{64..512}, kappa
     log-uniform in [1e2, 1e5] (seed 7), A = G diag(d) stored FP32_BF16,
-    b = A x_true.  Forward: CountSketch (s = 4n) + batched sketch-QR +
+    b = A x_true.  Forward: batched CountSketch (s = 4n) + batched sketch-QR +
     batched LSQR to 1e-6 (one hipGraph per restart).  Backward: for
     g ~ N(0,1), w = (A^T A)^-1 g per problem (adjoint CG) and grad_b, grad_A
     (rank-2 kernel, m x n FP32 each).  Also the same forward one problem at
@@ -231,7 +231,9 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
         s = 4 * n
         probs.append(dict(m=m, n=n, lda=lda, wh=wh, wr=wr, b=bd, xt=xt, s=s, sk=DeviceBuffer(s * n * 8),
                           x=DeviceBuffer(n * 8)))
-    cs_bytes = max(C.lib().tomos_countsketch_scratch_size(p["m"], p["n"], p["s"]) for p in probs)
+    cp = (C.CsProblem * P)(*[C.CsProblem(p["m"], p["n"], p["lda"], p["s"], p["s"], p["wh"].ptr, p["wr"].ptr,
+                                         p["sk"].ptr) for p in probs])
+    cs_bytes = C.lib().tomos_countsketch_apply_batched_scratch_size(P, cp)
     cs = DeviceBuffer(cs_bytes)
     qp = (C.QrProblem * P)(*[C.QrProblem(p["s"], p["n"], p["s"], p["sk"].ptr) for p in probs])
     qsb = C.lib().tomos_sketch_qr_precond_batched_scratch_size(P, qp)
@@ -251,9 +253,7 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
 
     def forward():
         t0 = time.perf_counter()
-        for p in probs:
-            C.call("tomos_countsketch_apply", C.FP32_BF16, C.TIER_EXACT, p["m"], p["n"], p["wh"].ptr, p["wr"].ptr,
-                   p["lda"], p["s"], p["sk"].ptr, p["s"], cs.ptr, cs_bytes, None)
+        C.call("tomos_countsketch_apply_batched", C.FP32_BF16, C.TIER_EXACT, P, cp, cs.ptr, cs_bytes, None)
         C.call("tomos_sketch_qr_precond_batched_f64", P, qp, qs.ptr, qsb, None)
         t1 = time.perf_counter()
         C.call("tomos_lsqr_batched_solve", ctypes.byref(o), P, lp, 0, 0, 0, 1, scr.ptr, sb, its, rr, None)

@@ -379,6 +379,27 @@ int tomos_countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const vo
                             int64_t ldy, void* scratch, size_t scratch_bytes,
                             tomos_stream_t stream);
 
+/* One CountSketch of a batch: Y (s x n, ldy) = S A, A m x n (lda; wr the
+ * lo plane of a packed A at the EXACT tier, else unused). */
+typedef struct {
+    int64_t m, n, lda, s, ldy;
+    const void* a;
+    const uint16_t* wr;
+    double* y;
+} tomos_cs_problem;
+
+/* tomos_countsketch_apply over nprob problems of one a_type and tier, with
+ * one host synchronisation for the whole batch instead of one per problem
+ * (the fixed-point path's exponent checks are read together at the end;
+ * the problems they reject are redone one by one).  Results equal the
+ * one-problem call's bit for bit.  No counterpart in the reference, whose
+ * batched preconditioner calls countsketch_apply_matrix_hashed once per
+ * problem; it is for batched-solve callers.  scratch:
+ * tomos_countsketch_apply_batched_scratch_size() bytes or NULL. */
+size_t tomos_countsketch_apply_batched_scratch_size(int nprob, const tomos_cs_problem* probs);
+int tomos_countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_problem* probs,
+                                    void* scratch, size_t scratch_bytes, tomos_stream_t stream);
+
 /* ---- Householder QR / triangular inverse (FP64) ---------------------- */
 /* R (n x n upper, ldr) of the Householder QR of the k x n FP64 matrix A
  * (lda; A is overwritten with intermediate data).  k >= n required.

@@ -91,6 +91,58 @@ def test_apply_sampled_exponents(capi, orc, a_type, tier, spike):
         assert err < 2e-6, (j, err)
 
 
+@pytest.mark.parametrize("a_type,tier", [(C.F32, 0), (C.FP32_BF16, 2), (C.FP32_16, 2)])
+@pytest.mark.parametrize("own_scratch", [False, True])
+def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
+    """tomos_countsketch_apply_batched over uneven problems equals the
+    one-problem call bit for bit on the fixed-point path: sampled exponents
+    that hold, a spike that sends its problem back to full-row exponents, a
+    problem below the sampling threshold, m = 0 (zeros).  An lda that is not
+    a multiple of the 16-byte row vector (FP32_BF16/FP32_16: 8 rows) takes
+    the float path, whose LDS float atomics add in no fixed order: that one
+    agrees to FP32 rounding.  n = 0 writes nothing."""
+    rng = np.random.default_rng(33)
+    specs = [(300000, 9, 2000, 0, True), (300000, 9, 2000, 0, False), (50000, 40, 160, 0, False),
+             (5000, 20, 80, 0, False), (1001, 7, 28, 3, False), (0, 5, 20, 0, False), (100, 0, 10, 0, False)]
+    probs, keep, singles = [], [], []
+    for m, n, s_rows, pad, spike in specs:
+        lda = max(m + pad, 1)
+        A = np.zeros((lda, n), dtype=np.float32, order="F")
+        A[:m] = rng.standard_normal((m, n))
+        if spike:
+            A[5000, 3] = 3.0e7  # between the first two sampled segments
+        if a_type == C.F32:
+            a, wr = H.dev(A), None
+        else:
+            variant = C.PACK_FP32_BF16 if a_type == C.FP32_BF16 else C.PACK_FP32_16
+            a, wr = H.pack(variant, A)
+        y = H.dev(np.full((s_rows, n), np.nan))
+        keep += [a, wr, y]
+        probs.append(C.CsProblem(m, n, lda, s_rows, s_rows, a.ptr, wr.ptr if wr else None, y.ptr))
+        singles.append(H.countsketch_apply(a_type, tier, m, n, a, wr, lda, s_rows) if n else None)
+    table = (C.CsProblem * len(probs))(*probs)
+    if own_scratch:
+        nb = C.lib().tomos_countsketch_apply_batched_scratch_size(len(probs), table)
+        assert nb > 0
+        scr = DeviceBuffer(nb)
+        C.call("tomos_countsketch_apply_batched", a_type, tier, len(probs), table, scr.ptr, nb, None)
+    else:
+        C.call("tomos_countsketch_apply_batched", a_type, tier, len(probs), table, None, 0, None)
+    for k, ((m, n, s_rows, pad, _), ref) in enumerate(zip(specs, singles)):
+        got = keep[3 * k + 2].to_numpy(np.float64, (s_rows, n))
+        if n == 0:
+            continue
+        if pad and a_type != C.F32:  # float path
+            assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max(), (k, m, n)
+        else:
+            assert np.array_equal(got, ref), (k, m, n)
+    rc = C.lib().tomos_countsketch_apply_batched(a_type, tier, len(probs), table, None, 1, None)
+    assert rc == 0  # NULL scratch ignores the size
+    if own_scratch:
+        rc = C.lib().tomos_countsketch_apply_batched(a_type, tier, len(probs), table, scr.ptr, nb - 1, None)
+        assert rc != 0  # scratch too small
+
+
 def test_norm_preservation(capi):
     """E |Sx|^2 / |x|^2 ~ 1 (test_countsketch_cpu_kernels.cpp:93-179, 200 trials)."""
     m, s, trials = 5000, 500, 200

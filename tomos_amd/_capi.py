@@ -56,6 +56,11 @@ class LsqProblem(ctypes.Structure):
                 ("rinv", vp), ("ldr", i64)]
 
 
+class CsProblem(ctypes.Structure):
+    _fields_ = [("m", i64), ("n", i64), ("lda", i64), ("s", i64), ("ldy", i64), ("a", vp), ("wr", vp),
+                ("y", vp)]
+
+
 class LsqrOptions(ctypes.Structure):
     _fields_ = [("a_type", cint), ("mv_tier", cint), ("restart_tier", cint),
                 ("value_type", cint), ("refine_type", cint), ("precond_type", cint),
@@ -137,6 +142,8 @@ _SIGS = {
     "tomos_countsketch_scratch_size": (sz, [i64, i64, i64]),
     "tomos_countsketch_apply": (cint, [cint, cint, i64, i64, vp, vp, i64, i64, vp, i64, vp, sz,
                                        vp]),
+    "tomos_countsketch_apply_batched_scratch_size": (sz, [cint, vp]),
+    "tomos_countsketch_apply_batched": (cint, [cint, cint, cint, vp, vp, sz, vp]),
     "tomos_geqrf_scratch_size": (sz, [i64, i64]),
     "tomos_geqrf_r_f64": (cint, [i64, i64, vp, i64, vp, i64, vp, sz, vp]),
     "tomos_trtri_upper_f64": (cint, [i64, vp, i64, vp]),

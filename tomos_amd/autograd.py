@@ -67,7 +67,7 @@ class SolveOptions:
 class PackedProblem:
     """A packed as FP32_BF16 planes plus its sketch-QR preconditioner."""
 
-    def __init__(self, A, opts):
+    def __init__(self, A, opts, precondition=True):
         if not A.is_cuda or A.dtype != torch.float32 or A.dim() != 2:
             raise ValueError("A must be a 2-D float32 CUDA tensor")
         m, n = A.shape
@@ -83,8 +83,15 @@ class PackedProblem:
         C.call("tomos_pack", C.PACK_FP32_BF16, m, n, _p(acm), m, _p(self.wh), _p(self.wr), self.ld, s)
         self.s_rows = max(opts.sketch_factor * n, n)
         self.sketch = torch.empty((n, self.s_rows), dtype=torch.float64, device=dev)
+        if precondition:
+            self.precondition()
+
+    def precondition(self):
+        """sketch = S A (hashed CountSketch), then R^-1 of its QR over the
+        leading n x n block."""
+        m, n, s = self.m, self.n, _stream()
         cs = C.lib().tomos_countsketch_scratch_size(m, n, self.s_rows)
-        scr = _bytes(cs, dev)
+        scr = _bytes(cs, self.wh.device)
         C.call("tomos_countsketch_apply", C.FP32_BF16, C.TIER_EXACT, m, n, _p(self.wh), _p(self.wr),
                self.ld, self.s_rows, _p(self.sketch), self.s_rows, _p(scr), cs, s)
         # R^-1 overwrites the leading n x n block (ld = s_rows)
@@ -177,6 +184,29 @@ def _problem_table(probs, b64s, xs):
                      p.sketch.data_ptr(), p.s_rows) for p, b, x in zip(probs, b64s, xs)])
 
 
+def _precondition_batched(probs):
+    """Every problem's sketch and R^-1 in two batched calls: the CountSketches
+    with one host synchronisation (tomos_countsketch_apply_batched), then the
+    sketch QRs and inverses in grouped launches
+    (tomos_sketch_qr_precond_batched_f64).  Equal to each problem's
+    precondition() bit for bit."""
+    P, dev, s = len(probs), probs[0].wh.device, _stream()
+    cp = (C.CsProblem * P)(*[C.CsProblem(p.m, p.n, p.ld, p.s_rows, p.s_rows, _p(p.wh), _p(p.wr), _p(p.sketch))
+                             for p in probs])
+    cb = C.lib().tomos_countsketch_apply_batched_scratch_size(P, cp)
+    if cb == 0:
+        raise RuntimeError(C.lib().tomos_last_error().decode())
+    scr = _bytes(cb, dev)
+    C.call("tomos_countsketch_apply_batched", C.FP32_BF16, C.TIER_EXACT, P, cp, _p(scr), cb, s)
+    del scr
+    qp = (C.QrProblem * P)(*[C.QrProblem(p.s_rows, p.n, p.s_rows, _p(p.sketch)) for p in probs])
+    qb = C.lib().tomos_sketch_qr_precond_batched_scratch_size(P, qp)
+    if qb == 0:
+        raise RuntimeError(C.lib().tomos_last_error().decode())
+    qs = _bytes(qb, dev)
+    C.call("tomos_sketch_qr_precond_batched_f64", P, qp, _p(qs), qb, s)
+
+
 def _ptrs(ts):
     return (ctypes.c_void_p * len(ts))(*[t.data_ptr() if t is not None else None for t in ts])
 
@@ -208,7 +238,8 @@ class _LstsqBatched(torch.autograd.Function):
     @staticmethod
     def forward(ctx, opts, nprob, *tensors):
         As, bs = tensors[:nprob], tensors[nprob:]
-        probs = [PackedProblem(A.detach(), opts) for A in As]
+        probs = [PackedProblem(A.detach(), opts, precondition=False) for A in As]
+        _precondition_batched(probs)
         dev = As[0].device
         b64s = [b.detach().to(torch.float64).contiguous() for b in bs]
         xs = [torch.zeros(p.n, dtype=torch.float64, device=dev) for p in probs]

@@ -19,6 +19,8 @@
 // order), FP64 across splits.  A, lda or the split start not 16-byte
 // aligned takes the element-wise kernel.
 #include <algorithm>
+#include <type_traits>
+#include <vector>
 
 #include "codec.hpp"
 #include "common.hpp"
@@ -536,25 +538,26 @@ void launch_apply_cg(const CsPlan& p, int64_t m, int64_t n, const void* a, const
 }
 
 // Fixed-point apply on the vector path.  The column exponents come from a
-// sample of the rows (kSampleSegs segments of kSampleLen rows spread over
-// A) plus kSampleMargin bits when A is tall enough for the sample to save
-// a pass, else from every row.  The apply kernel checks every value
-// against its column's bound; a value beyond it (or not finite) makes the
-// call redo the exponents from every row and apply again.  Returns false
-// (the float path runs) when a column's max over every row is not finite.
-constexpr int64_t kSampleLen = 1024;
+// sample of the rows (kSampleSegs segments spread over A, each 1/256 of
+// the rows long, 64..1024 rows) plus kSampleMargin bits when A has at least
+// kSampleMinRows rows, else from every row.  The apply kernel checks every
+// value against its column's bound; a value beyond it (or not finite)
+// makes the call redo the exponents from every row and apply again.
 constexpr int64_t kSampleSegs = 32;
-constexpr int kSampleMargin = 12;  // 2^-31 of the sampled max per term at 2e6 rows, 4 splits
+constexpr int64_t kSampleMinRows = 16384;  // the sample is at most 1/8 of the rows
+constexpr int kSampleMargin = 12;           // 2^-31 of the sampled max per term at 2e6 rows, 4 splits
 
+// Enqueue the fixed-point path with the exponents from a sample (sample)
+// or from every row, reading nothing back: flags[0] is set if a column's
+// max is not finite, flags[1] if a value is beyond its column's bound.
+// The caller reads the flags once its work is enqueued.
 template <int FMT, bool EXACT>
-bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
-                  const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy,
-                  hipStream_t st) {
-    if (!vec_ok<FMT>(a, wr, lda, EXACT)) return false;
+void fx_enqueue(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
+                const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy, bool sample,
+                int* flags, hipStream_t st) {
     const CsPlan p = cs_plan_fx(m, n, s);
     unsigned* colmax = reinterpret_cast<unsigned*>(scratch + L.colmax);
     int* colexp = reinterpret_cast<int*>(scratch + L.colexp);
-    int* flag = reinterpret_cast<int*>(scratch + L.flag);  // [0] not finite, [1] beyond a bound
     double* partial = reinterpret_cast<double*>(scratch + L.partial);
     require(n < (int64_t(1) << 31), "countsketch_apply: n too large");
     // FP32_BF16's exact values are bounded by 2x their hi words: the hi plane
@@ -562,29 +565,27 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
     constexpr bool kExactMax = EXACT && FMT == FMT_P16;
     constexpr int64_t kElemBytes = (FMT == FMT_F64 ? 8 : FMT == FMT_F32 ? 4 : 2) * (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF) ? 2 : 1);
     const bool ntl = stream_loads(m * n * kElemBytes);
-    auto exponents = [&](bool sample) {
-        TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
-        TOMOS_HIP_CHECK(hipMemsetAsync(flag, 0, 8, st));
-        int64_t stride, len;
-        unsigned segs;
-        if (sample) {
-            stride = (m / kSampleSegs) / 8 * 8;
-            len = kSampleLen;
-            segs = static_cast<unsigned>(kSampleSegs);
-        } else {
-            const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
-            stride = len = ceil_div(ceil_div(m, csplits), 8) * 8;
-            segs = static_cast<unsigned>(ceil_div(m, len));
-        }
-        const dim3 cgrid(static_cast<unsigned>(n), segs);
-        if (ntl && !sample)
-            cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
-        else
-            cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
-        cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
-            n, colmax, sample ? kSampleMargin : 0, colexp, flag);
-        TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
-    };
+    TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
+    TOMOS_HIP_CHECK(hipMemsetAsync(flags, 0, 8, st));
+    int64_t stride, len;
+    unsigned segs;
+    if (sample) {
+        stride = (m / kSampleSegs) / 8 * 8;
+        len = std::max<int64_t>(64, std::min<int64_t>(1024, (m / 256) / 8 * 8));
+        segs = static_cast<unsigned>(kSampleSegs);
+    } else {
+        const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
+        stride = len = ceil_div(ceil_div(m, csplits), 8) * 8;
+        segs = static_cast<unsigned>(ceil_div(m, len));
+    }
+    const dim3 cgrid(static_cast<unsigned>(n), segs);
+    if (ntl && !sample)
+        cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
+    else
+        cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
+    cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
+        n, colmax, sample ? kSampleMargin : 0, colexp, flags);
+    TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
     const int shift = fx_shift(p.rows_per_split);
     const size_t lds = static_cast<size_t>(s) * p.cg * 8;
     const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
@@ -594,45 +595,51 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         k<<<dim3(static_cast<unsigned>(blocks)), kVecThreads, lds, st>>>(m, n, a, wr, lda, s, packed,
                                                                         p.rows_per_split, p.ncg, colexp, shift,
-                                                                        partial, flag + 1);
+                                                                        partial, flags + 1);
     };
-    auto apply = [&]() {
-        if (ntl) {
-            switch (p.cg) {
-                case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, true>); break;
-                case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, true>); break;
-                case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, true>); break;
-                default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, true>); break;
-            }
-        } else {
-            switch (p.cg) {
-                case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, false>); break;
-                case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, false>); break;
-                case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, false>); break;
-                default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, false>); break;
-            }
+    if (ntl) {
+        switch (p.cg) {
+            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, true>); break;
+            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, true>); break;
+            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, true>); break;
+            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, true>); break;
         }
-        TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
-        cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits,
-                                                                                              partial, y, ldy);
-        TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
-    };
-    int hflag[2] = {0, 0};
+    } else {
+        switch (p.cg) {
+            case 8: launch(cs_apply_fx_kernel<FMT, EXACT, 8, false>); break;
+            case 4: launch(cs_apply_fx_kernel<FMT, EXACT, 4, false>); break;
+            case 2: launch(cs_apply_fx_kernel<FMT, EXACT, 2, false>); break;
+            default: launch(cs_apply_fx_kernel<FMT, EXACT, 1, false>); break;
+        }
+    }
+    TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
+    cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits, partial,
+                                                                                          y, ldy);
+    TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
+}
+
+// The fixed-point path for one call; returns false (the float path runs)
+// when it does not apply (unaligned A) or a column's max over every row is
+// not finite.
+template <int FMT, bool EXACT>
+bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
+                  const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy,
+                  hipStream_t st) {
+    if (!vec_ok<FMT>(a, wr, lda, EXACT)) return false;
+    int* flags = reinterpret_cast<int*>(scratch + L.flag);
+    int h[2] = {0, 0};
     auto read_flags = [&]() {
-        TOMOS_HIP_CHECK(hipMemcpyAsync(hflag, flag, 8, hipMemcpyDeviceToHost, st));
+        TOMOS_HIP_CHECK(hipMemcpyAsync(h, flags, 8, hipMemcpyDeviceToHost, st));
         TOMOS_HIP_CHECK(hipStreamSynchronize(st));
     };
-    if (m >= 8 * kSampleSegs * kSampleLen) {  // sampled exponents: one pass over A when they hold
-        exponents(true);
-        apply();
+    if (m >= kSampleMinRows) {  // sampled exponents: one pass over A when they hold
+        fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, true, flags, st);
         read_flags();
-        if (!hflag[0] && !hflag[1]) return true;
+        if (!h[0] && !h[1]) return true;
     }
-    exponents(false);
+    fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, false, flags, st);
     read_flags();
-    if (hflag[0]) return false;
-    apply();
-    return true;
+    return !h[0] && !h[1];  // the float path redoes a call whose full-row exponents are not finite
 }
 
 }  // namespace
@@ -703,11 +710,98 @@ void countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a
     TOMOS_LAUNCH_CHECK("cs_finish_kernel");
 }
 
+// Many sketches in one call, no host synchronisation between them: every
+// problem's fixed-point path is enqueued in turn on `st` (sharing one
+// scratch region, which the stream order makes safe), each with its own two
+// flag words after that region; one read of all flags at the end sends the
+// problems whose sampled exponents did not hold (or that the fixed-point
+// path does not take) through the one-problem call.
+size_t countsketch_batched_scratch_bytes(int nprob, const tomos_cs_problem* pr) {
+    size_t b = 256;
+    for (int i = 0; i < nprob; ++i) b = std::max(b, cs_scratch(pr[i].m, pr[i].n, pr[i].s));
+    return a256(b) + a256(static_cast<size_t>(std::max(nprob, 1)) * 8);
+}
+
+void countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_problem* pr, void* scratch,
+                               size_t scratch_bytes, hipStream_t st) {
+    require(nprob >= 0 && (nprob == 0 || pr), "countsketch_apply_batched: bad problem list");
+    require(tier == TOMOS_TIER_DEFAULT || tier == TOMOS_TIER_FAST || tier == TOMOS_TIER_EXACT,
+            "countsketch_apply_batched: bad tier");
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_cs_problem& q = pr[i];
+        require(q.m >= 0 && q.n >= 0 && q.s >= 1 && q.s <= kLdsBytes / 4 && q.m < (int64_t(1) << 31),
+                "countsketch_apply_batched: bad sizes");
+        require(q.lda >= std::max<int64_t>(q.m, 1) && q.ldy >= q.s, "countsketch_apply_batched: bad leading dimension");
+        require(q.n == 0 || q.y, "countsketch_apply_batched: null y");
+        require(q.n == 0 || q.m == 0 || q.a, "countsketch_apply_batched: null A");
+    }
+    if (nprob == 0) return;
+    const size_t need = countsketch_batched_scratch_bytes(nprob, pr);
+    if (scratch == nullptr) scratch = fallback_scratch(need);
+    else require(scratch_bytes >= need, "countsketch_apply_batched: scratch too small");
+    char* sc = static_cast<char*>(scratch);
+    int* flags = reinterpret_cast<int*>(sc + (need - a256(static_cast<size_t>(nprob) * 8)));
+    const bool exact = tier == TOMOS_TIER_EXACT;
+    std::vector<char> queued(static_cast<size_t>(nprob), 0);
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_cs_problem& q = pr[i];
+        if (q.n == 0 || q.m == 0) continue;
+        if (exact && (a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16) && !q.wr) continue;
+        const CsLayout L = cs_layout(q.m, q.n, q.s);
+        uint32_t* packed = reinterpret_cast<uint32_t*>(sc + L.idx);
+        const bool sample = q.m >= kSampleMinRows;
+        auto go = [&](auto fmt_tag, auto exact_tag) {
+            constexpr int F = decltype(fmt_tag)::value;
+            constexpr bool E = decltype(exact_tag)::value;
+            if (!vec_ok<F>(q.a, q.wr, q.lda, E)) return;
+            cs_index_kernel<<<dim3(static_cast<unsigned>(ceil_div(q.m, 256))), 256, 0, st>>>(q.m, q.s, nullptr,
+                                                                                             nullptr, packed);
+            TOMOS_LAUNCH_CHECK("cs_index_kernel");
+            fx_enqueue<F, E>(q.m, q.n, q.a, q.wr, q.lda, q.s, packed, sc, L, q.y, q.ldy, sample, flags + 2 * i, st);
+            queued[i] = 1;
+        };
+        using T = std::true_type;
+        using Fl = std::false_type;
+        switch (a_type) {
+            case TOMOS_F32: go(std::integral_constant<int, FMT_F32>{}, Fl{}); break;
+            case TOMOS_F64: go(std::integral_constant<int, FMT_F64>{}, Fl{}); break;
+            case TOMOS_FP32_16:
+                if (exact) go(std::integral_constant<int, FMT_P16>{}, T{});
+                else go(std::integral_constant<int, FMT_P16>{}, Fl{});
+                break;
+            case TOMOS_FP32_BF16:
+                if (exact) go(std::integral_constant<int, FMT_PBF>{}, T{});
+                else go(std::integral_constant<int, FMT_PBF>{}, Fl{});
+                break;
+            default: break;
+        }
+    }
+    std::vector<int> h(2 * static_cast<size_t>(nprob), 0);
+    TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), flags, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    TOMOS_HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_cs_problem& q = pr[i];
+        if (q.n == 0) continue;
+        if (queued[i] && !h[2 * i] && !h[2 * i + 1]) continue;
+        countsketch_apply(a_type, tier, q.m, q.n, q.a, q.wr, q.lda, q.s, q.y, q.ldy, sc, need, st);
+    }
+}
+
 }  // namespace tomos_hip
 
 using namespace tomos_hip;
 
 extern "C" {
+
+size_t tomos_countsketch_apply_batched_scratch_size(int nprob, const tomos_cs_problem* probs) {
+    if (nprob < 0 || (nprob > 0 && !probs)) return 0;
+    return countsketch_batched_scratch_bytes(nprob, probs);
+}
+
+int tomos_countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_problem* probs, void* scratch,
+                                    size_t scratch_bytes, tomos_stream_t s) {
+    return guarded([&] { countsketch_apply_batched(a_type, tier, nprob, probs, scratch, scratch_bytes, s); });
+}
 
 int tomos_countsketch_indices(int64_t m, int64_t s, int32_t* bucket, int8_t* sign,
                               tomos_stream_t st) {
