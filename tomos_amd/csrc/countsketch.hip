@@ -298,16 +298,12 @@ __global__ __launch_bounds__(256) void cs_colmax_kernel(int64_t m, const void* _
 }
 
 // e_j from the max bits, plus `margin` bits when the max is a sample's;
-// flag = 1 if any column is not finite
-__global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigned* __restrict__ colmax_bits,
-                                                        int margin, int* __restrict__ colexp,
-                                                        int* __restrict__ flag) {
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (j >= n) return;
-    const unsigned bits = colmax_bits[j];
+// flag = 1 if the column is not finite
+__device__ __forceinline__ void cs_set_exponent(unsigned bits, int margin, int* __restrict__ colexp,
+                                                int* __restrict__ flag) {
     if (bits >= 0x7F800000u) {
         atomicOr(flag, 1);
-        colexp[j] = 0;
+        *colexp = 0;
         return;
     }
     int e = -149;
@@ -315,19 +311,139 @@ __global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigne
         frexpf(__uint_as_float(bits), &e);
         e += 1;  // margin: an exact FP32_BF16 value is below twice its hi word
     }
-    colexp[j] = e + margin;
+    *colexp = e + margin;
+}
+
+__global__ __launch_bounds__(256) void cs_colexp_kernel(int64_t n, const unsigned* __restrict__ colmax_bits,
+                                                        int margin, int* __restrict__ colexp,
+                                                        int* __restrict__ flag) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (j >= n) return;
+    cs_set_exponent(colmax_bits[j], margin, colexp + j, flag);
+}
+
+// Sampled exponents in one pass, one block per column: the max over `segs`
+// segments of `len` rows (starts `stride` apart; len a multiple of 8) reduced
+// in the block, then e_j + margin written directly.  The whole sample of a
+// column is one block's work, so no column max crosses blocks (no atomics
+// to one address from many blocks, no cleared buffer).  segs = 1, stride =
+// 0, len = m rounded up to 8 and margin 0 give the exact exponents.
+template <int FMT, bool EXACT>
+__device__ __forceinline__ void cs_sample_exp_body(int64_t j, int64_t m, const void* __restrict__ a,
+                                                   const uint16_t* __restrict__ wr, int64_t lda, int64_t stride,
+                                                   int len, int segs, int margin, int* __restrict__ colexp,
+                                                   int* __restrict__ flag) {
+    constexpr int V = vec_rows<FMT>();
+    constexpr int EB = 16 / V;
+    __shared__ float wmax[4];
+    const char* col = static_cast<const char*>(a) + j * lda * EB;
+    const unsigned vps = static_cast<unsigned>(len / V);
+    const unsigned total = vps * static_cast<unsigned>(segs);
+    float mx = 0.0f;
+#pragma unroll 4
+    for (unsigned t = threadIdx.x; t < total; t += 256) {
+        const unsigned sg = t / vps;
+        const int64_t i0 = static_cast<int64_t>(sg) * stride + static_cast<int64_t>(t - sg * vps) * V;
+        float v[V];
+        if (i0 + V <= m) {
+            const uint4 h = load_a16<false>(col + i0 * EB);
+            uint4 l = make_uint4(0, 0, 0, 0);
+            if constexpr (EXACT) l = load_a16<false>(wr + i0 + j * lda);
+            decode_rows<FMT, EXACT>(h, l, v);
+        } else {
+            for (int r = 0; r < V; ++r) v[r] = (i0 + r < m) ? load_elem<FMT, EXACT>(a, wr, i0 + r + j * lda) : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+            const float av = fabsf(v[r]);
+            mx = (av > mx || av != av) ? av : mx;  // NaN wins
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float other = __shfl_xor(mx, o);
+        mx = (other > mx || other != other) ? other : mx;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) mx = (wmax[w] > mx || wmax[w] != wmax[w]) ? wmax[w] : mx;
+        cs_set_exponent(__float_as_uint(mx), margin, colexp + j, flag);
+    }
+}
+
+template <int FMT, bool EXACT>
+__global__ __launch_bounds__(256) void cs_sample_exp_kernel(int64_t m, const void* __restrict__ a,
+                                                            const uint16_t* __restrict__ wr, int64_t lda,
+                                                            int64_t stride, int len, int segs, int margin,
+                                                            int* __restrict__ colexp, int* __restrict__ flag) {
+    cs_sample_exp_body<FMT, EXACT>(blockIdx.x, m, a, wr, lda, stride, len, segs, margin, colexp, flag);
+}
+
+// One problem of a grouped CountSketch: its operands, its fixed-point plan
+// (splits of rows_per_split rows x column groups of the launch's CG), the
+// sample that sets its exponents and where its scratch and result lie.
+struct CsGroupDesc {
+    const void* a;
+    const uint16_t* wr;
+    uint32_t* packed;
+    int* colexp;
+    unsigned long long* partial;
+    int* flags;  // [0] a column max not finite, [1] a value beyond its bound
+    double* y;
+    int64_t m, n, lda, s, ldy, rows_per_split, stride;
+    int ncg, splits, shift, len, segs, margin;
+    int block0;  // first block within its apply launch
+    int local;   // index within its apply launch's descriptors
+    int owner0;  // block0 plus the launch's first entry in the owner table
+    int pad;
+};
+
+// Descriptors travel as kernel arguments, kCsChunk a launch (under 4 KB)
+constexpr int kCsChunk = 24;
+struct CsDescChunk {
+    CsGroupDesc d[kCsChunk];
+};
+static_assert(sizeof(CsDescChunk) + 16 <= 4096, "descriptor chunk exceeds the kernel-argument limit");
+
+__global__ __launch_bounds__(64) void cs_set_desc_kernel(CsGroupDesc* __restrict__ dst, CsDescChunk c, int count) {
+    if (static_cast<int>(threadIdx.x) < count) dst[threadIdx.x] = c.d[threadIdx.x];
+}
+
+// owner[b] = the launch-local descriptor of apply block b; grid (problems)
+__global__ __launch_bounds__(256) void cs_owner_kernel(const CsGroupDesc* __restrict__ d, int* __restrict__ owner) {
+    const CsGroupDesc& q = d[blockIdx.x];
+    const int nb = q.splits * q.ncg;
+    for (int b = threadIdx.x; b < nb; b += 256) owner[q.owner0 + b] = q.local;
+}
+
+// grid (ceil(max m / 256), problems)
+__global__ __launch_bounds__(256) void cs_group_index_kernel(const CsGroupDesc* __restrict__ d) {
+    const CsGroupDesc& q = d[blockIdx.y];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= q.m) return;
+    const uint64_t hb = splitmix64(static_cast<uint64_t>(i) ^ kBucketSeed);
+    const uint64_t hs = splitmix64(static_cast<uint64_t>(i) ^ kSignSeed);
+    const uint32_t b = static_cast<uint32_t>(hb % static_cast<uint64_t>(q.s));
+    q.packed[i] = b | ((hs & 1ull) ? 0u : 0x80000000u);
+}
+
+// grid (max n, problems)
+template <int FMT, bool EXACT>
+__global__ __launch_bounds__(256) void cs_group_exp_kernel(const CsGroupDesc* __restrict__ d) {
+    const CsGroupDesc& q = d[blockIdx.y];
+    if (static_cast<int64_t>(blockIdx.x) >= q.n) return;
+    cs_sample_exp_body<FMT, EXACT>(blockIdx.x, q.m, q.a, q.wr, q.lda, q.stride, q.len, q.segs, q.margin, q.colexp,
+                                   q.flags);
 }
 
 template <int FMT, bool EXACT, int CG, bool NTL>
-__global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
-    int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
-    int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
-    const int* __restrict__ colexp, int shift, double* __restrict__ partial, int* __restrict__ over) {
+__device__ __forceinline__ void cs_apply_fx_body(
+    int split, int cg, int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr,
+    int64_t lda, int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split,
+    const int* __restrict__ colexp, int shift, unsigned long long* __restrict__ partial, int* __restrict__ over) {
     constexpr int V = vec_rows<FMT>();
     constexpr int EB = 16 / V;
     extern __shared__ unsigned long long yfx[];  // [CG][s]
-    const int split = blockIdx.x / ncg;
-    const int cg = blockIdx.x - split * ncg;
     const int64_t c0 = static_cast<int64_t>(cg) * CG;
     const int ncols = static_cast<int>(min<int64_t>(CG, n - c0));
     for (int64_t t = threadIdx.x; t < s * CG; t += kVecThreads) yfx[t] = 0ull;
@@ -401,24 +517,59 @@ __global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
         }
     }
     if (__syncthreads_or(beyond) && threadIdx.x == 0) atomicOr(over, 1);
-    double* out = partial + static_cast<int64_t>(split) * s * n;
-    for (int64_t t = threadIdx.x; t < s * ncols; t += kVecThreads) {
-        const int64_t c = t / s;
-        const long long q = static_cast<long long>(yfx[t]);
-        out[(c0 + c) * s + (t - c * s)] = ldexp(static_cast<double>(q), colexp[c0 + c] - shift);
-    }
+    unsigned long long* out = partial + static_cast<int64_t>(split) * s * n + c0 * s;
+    for (int64_t t = threadIdx.x; t < s * ncols; t += kVecThreads) out[t] = yfx[t];
 }
 
+template <int FMT, bool EXACT, int CG, bool NTL>
+__global__ __launch_bounds__(kVecThreads) void cs_apply_fx_kernel(
+    int64_t m, int64_t n, const void* __restrict__ a, const uint16_t* __restrict__ wr, int64_t lda,
+    int64_t s, const uint32_t* __restrict__ packed, int64_t rows_per_split, int ncg,
+    const int* __restrict__ colexp, int shift, unsigned long long* __restrict__ partial,
+    int* __restrict__ over) {
+    const int split = blockIdx.x / ncg;
+    cs_apply_fx_body<FMT, EXACT, CG, NTL>(split, blockIdx.x - split * ncg, m, n, a, wr, lda, s, packed,
+                                          rows_per_split, colexp, shift, partial, over);
+}
+
+// The blocks of every problem in one launch: block b belongs to problem
+// owner[b] and is its block b - block0 (split-major over its column groups)
+template <int FMT, bool EXACT, int CG, bool NTL>
+__global__ __launch_bounds__(kVecThreads) void cs_group_apply_kernel(const CsGroupDesc* __restrict__ d,
+                                                                     const int* __restrict__ owner) {
+    const CsGroupDesc& q = d[owner[blockIdx.x]];
+    const int lb = static_cast<int>(blockIdx.x) - q.block0;
+    const int split = lb / q.ncg;
+    cs_apply_fx_body<FMT, EXACT, CG, NTL>(split, lb - split * q.ncg, q.m, q.n, q.a, q.wr, q.lda, q.s, q.packed,
+                                          q.rows_per_split, q.colexp, q.shift, q.partial, q.flags + 1);
+}
+
+// y = the splits' integer sums added as integers (exact: |sum| < m 2^shift
+// < 2^61), then rounded once to FP64 and scaled by 2^(e_c - shift).  The
+// result depends on the rows and the exponents only, not on how the rows
+// were split.
 __global__ __launch_bounds__(256) void cs_finish_fx_kernel(int64_t s, int64_t n, int splits,
-                                                           const double* __restrict__ partial,
+                                                           const unsigned long long* __restrict__ partial,
+                                                           const int* __restrict__ colexp, int shift,
                                                            double* __restrict__ y, int64_t ldy) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (t >= s * n) return;
     const int64_t c = t / s;
     const int64_t b = t - c * s;
-    double acc = 0.0;
+    unsigned long long acc = 0ull;
     for (int r = 0; r < splits; ++r) acc += partial[static_cast<int64_t>(r) * s * n + t];
-    y[b + c * ldy] = acc;
+    y[b + c * ldy] = ldexp(static_cast<double>(static_cast<long long>(acc)), colexp[c] - shift);
+}
+
+// grid (ceil(max s n / 256), problems)
+__global__ __launch_bounds__(256) void cs_group_finish_kernel(const CsGroupDesc* __restrict__ d) {
+    const CsGroupDesc& q = d[blockIdx.y];
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= q.s * q.n) return;
+    const int64_t c = t / q.s;
+    unsigned long long acc = 0ull;
+    for (int r = 0; r < q.splits; ++r) acc += q.partial[static_cast<int64_t>(r) * q.s * q.n + t];
+    q.y[(t - c * q.s) + c * q.ldy] = ldexp(static_cast<double>(static_cast<long long>(acc)), q.colexp[c] - q.shift);
 }
 
 constexpr int64_t kLdsBytes = 128 * 1024;
@@ -462,11 +613,11 @@ CsPlan cs_plan_fx(int64_t m, int64_t n, int64_t s) {
     return p;
 }
 
-// 62 - H with 2^H > rows per split (plus one bit): a bucket sum of values
-// below 2^shift each stays below 2^62
-int fx_shift(int64_t rows_per_split) {
+// 62 - H with 2^H > m (plus one bit): a bucket sum over all m rows of
+// values below 2^shift each stays below 2^61, so split sums add exactly
+int fx_shift(int64_t m) {
     int h = 1;
-    while ((int64_t(1) << h) <= rows_per_split) ++h;
+    while ((int64_t(1) << h) <= m) ++h;
     return 62 - (h + 1);
 }
 
@@ -554,39 +705,42 @@ constexpr int kSampleMargin = 12;           // 2^-31 of the sampled max per term
 template <int FMT, bool EXACT>
 void fx_enqueue(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t lda, int64_t s,
                 const uint32_t* packed, char* scratch, const CsLayout& L, double* y, int64_t ldy, bool sample,
-                int* flags, hipStream_t st) {
+                int* flags, bool clear_flags, hipStream_t st) {
     const CsPlan p = cs_plan_fx(m, n, s);
     unsigned* colmax = reinterpret_cast<unsigned*>(scratch + L.colmax);
     int* colexp = reinterpret_cast<int*>(scratch + L.colexp);
-    double* partial = reinterpret_cast<double*>(scratch + L.partial);
+    unsigned long long* partial = reinterpret_cast<unsigned long long*>(scratch + L.partial);
     require(n < (int64_t(1) << 31), "countsketch_apply: n too large");
     // FP32_BF16's exact values are bounded by 2x their hi words: the hi plane
     // suffices; FP32_16's exact values need their own decode
     constexpr bool kExactMax = EXACT && FMT == FMT_P16;
     constexpr int64_t kElemBytes = (FMT == FMT_F64 ? 8 : FMT == FMT_F32 ? 4 : 2) * (EXACT && (FMT == FMT_P16 || FMT == FMT_PBF) ? 2 : 1);
     const bool ntl = stream_loads(m * n * kElemBytes);
-    TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
-    TOMOS_HIP_CHECK(hipMemsetAsync(flags, 0, 8, st));
-    int64_t stride, len;
-    unsigned segs;
+    if (clear_flags) TOMOS_HIP_CHECK(hipMemsetAsync(flags, 0, 8, st));
     if (sample) {
-        stride = (m / kSampleSegs) / 8 * 8;
-        len = std::max<int64_t>(64, std::min<int64_t>(1024, (m / 256) / 8 * 8));
-        segs = static_cast<unsigned>(kSampleSegs);
+        // segments start (m / 32) rounded down to 8 rows apart and are
+        // m / 256 rows (64..1024, a multiple of 8) long: inside A for m >= 16384
+        const int64_t stride = (m / kSampleSegs) / 8 * 8;
+        const int64_t len = std::max<int64_t>(64, std::min<int64_t>(1024, (m / 256) / 8 * 8));
+        require(m >= kSampleMinRows && (kSampleSegs - 1) * stride + len <= m, "countsketch_apply: bad sample");
+        cs_sample_exp_kernel<FMT, kExactMax><<<dim3(static_cast<unsigned>(n)), 256, 0, st>>>(
+            m, a, wr, lda, stride, static_cast<int>(len), static_cast<int>(kSampleSegs), kSampleMargin, colexp,
+            flags);
+        TOMOS_LAUNCH_CHECK("cs_sample_exp_kernel");
     } else {
+        TOMOS_HIP_CHECK(hipMemsetAsync(colmax, 0, static_cast<size_t>(n) * 4, st));
         const int64_t csplits = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(m, 256 * 8 * 64)));
-        stride = len = ceil_div(ceil_div(m, csplits), 8) * 8;
-        segs = static_cast<unsigned>(ceil_div(m, len));
+        const int64_t stride = ceil_div(ceil_div(m, csplits), 8) * 8;
+        const dim3 cgrid(static_cast<unsigned>(n), static_cast<unsigned>(ceil_div(m, stride)));
+        if (ntl)
+            cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, stride, colmax);
+        else
+            cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, stride, colmax);
+        cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(n, colmax, 0, colexp,
+                                                                                        flags);
+        TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
     }
-    const dim3 cgrid(static_cast<unsigned>(n), segs);
-    if (ntl && !sample)
-        cs_colmax_kernel<FMT, kExactMax, true><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
-    else
-        cs_colmax_kernel<FMT, kExactMax, false><<<cgrid, 256, 0, st>>>(m, a, wr, lda, stride, len, colmax);
-    cs_colexp_kernel<<<dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, st>>>(
-        n, colmax, sample ? kSampleMargin : 0, colexp, flags);
-    TOMOS_LAUNCH_CHECK("cs_colmax_kernel");
-    const int shift = fx_shift(p.rows_per_split);
+    const int shift = fx_shift(m);
     const size_t lds = static_cast<size_t>(s) * p.cg * 8;
     const int64_t blocks = static_cast<int64_t>(p.splits) * p.ncg;
     require(blocks < (int64_t(1) << 31), "countsketch_apply: grid too large");
@@ -614,7 +768,7 @@ void fx_enqueue(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64_t
     }
     TOMOS_LAUNCH_CHECK("cs_apply_fx_kernel");
     cs_finish_fx_kernel<<<dim3(static_cast<unsigned>(ceil_div(s * n, 256))), 256, 0, st>>>(s, n, p.splits, partial,
-                                                                                          y, ldy);
+                                                                                          colexp, shift, y, ldy);
     TOMOS_LAUNCH_CHECK("cs_finish_fx_kernel");
 }
 
@@ -633,11 +787,11 @@ bool try_apply_fx(int64_t m, int64_t n, const void* a, const uint16_t* wr, int64
         TOMOS_HIP_CHECK(hipStreamSynchronize(st));
     };
     if (m >= kSampleMinRows) {  // sampled exponents: one pass over A when they hold
-        fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, true, flags, st);
+        fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, true, flags, true, st);
         read_flags();
         if (!h[0] && !h[1]) return true;
     }
-    fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, false, flags, st);
+    fx_enqueue<FMT, EXACT>(m, n, a, wr, lda, s, packed, scratch, L, y, ldy, false, flags, true, st);
     read_flags();
     return !h[0] && !h[1];  // the float path redoes a call whose full-row exponents are not finite
 }
@@ -710,26 +864,147 @@ void countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a
     TOMOS_LAUNCH_CHECK("cs_finish_kernel");
 }
 
-// Many sketches in one call, no host synchronisation between them: every
-// problem's fixed-point path is enqueued in turn on `st` (sharing one
-// scratch region, which the stream order makes safe), each with its own two
-// flag words after that region; one read of all flags at the end sends the
-// problems whose sampled exponents did not hold (or that the fixed-point
-// path does not take) through the one-problem call.
+// Many sketches in one call.  Every problem that the fixed-point path takes
+// (16-byte aligned rows, s <= 8192) runs in grouped launches: one index
+// launch, one exponent launch, one apply launch per column-group width
+// (8, 4, 2, 1 columns, whichever keep a block's int64 buckets within
+// 64 KB of LDS) whose blocks cover every such problem's splits of up to
+// 65536 rows, and one finish launch.  The flags of all problems are read
+// once at the end; the problems they reject (sampled exponents that do not
+// hold, a column max that is not finite) and the ones not grouped go
+// through the one-problem call.  Results equal the one-problem call's bit
+// for bit: a fixed-point sketch depends on the rows and the exponents only.
+namespace {
+
+constexpr int64_t kGroupRowsPerSplit = 65536;
+constexpr int64_t kGroupLdsBytes = 64 * 1024;
+
+int group_cg(int64_t s) {
+    int cg = 8;
+    while (cg > 1 && s * cg * 8 > kGroupLdsBytes) cg >>= 1;
+    return cg;
+}
+
+bool group_candidate(const tomos_cs_problem& q) {
+    return q.n > 0 && q.m > 0 && q.s * 8 <= kGroupLdsBytes;
+}
+
+bool group_takes(int a_type, bool exact, const tomos_cs_problem& q) {
+    if (!group_candidate(q)) return false;
+    const bool packed = a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16;
+    const int v = a_type == TOMOS_F64 ? 2 : a_type == TOMOS_F32 ? 4 : 8;
+    if (!packed && a_type != TOMOS_F32 && a_type != TOMOS_F64) return false;
+    if (exact && packed && (!q.wr || reinterpret_cast<uintptr_t>(q.wr) % 16 != 0)) return false;
+    return reinterpret_cast<uintptr_t>(q.a) % 16 == 0 && q.lda % v == 0;
+}
+
+struct GroupLayout {
+    size_t flags, desc, owner, total;
+    std::vector<size_t> base;  // per problem: packed, then colexp, then partial
+    int64_t blocks;
+};
+
+// Offsets for every candidate problem (an upper bound of what a call uses)
+GroupLayout group_layout(int nprob, const tomos_cs_problem* pr) {
+    GroupLayout L;
+    L.blocks = 0;
+    int nc = 0;
+    L.base.assign(static_cast<size_t>(nprob), 0);
+    size_t off = 0;
+    L.flags = off;
+    off += a256(static_cast<size_t>(std::max(nprob, 1)) * 8);
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_cs_problem& q = pr[i];
+        if (!group_candidate(q)) continue;
+        ++nc;
+        const int64_t splits = ceil_div(q.m, kGroupRowsPerSplit);
+        L.blocks += splits * ceil_div(q.n, group_cg(q.s));
+    }
+    L.desc = off;
+    off += a256(static_cast<size_t>(std::max(nc, 1)) * sizeof(CsGroupDesc));
+    L.owner = off;
+    off += a256(static_cast<size_t>(std::max<int64_t>(L.blocks, 1)) * 4);
+    for (int i = 0; i < nprob; ++i) {
+        const tomos_cs_problem& q = pr[i];
+        if (!group_candidate(q)) continue;
+        L.base[i] = off;
+        const int64_t splits = ceil_div(q.m, kGroupRowsPerSplit);
+        off += a256(static_cast<size_t>(q.m) * 4) + a256(static_cast<size_t>(q.n) * 4) +
+               a256(static_cast<size_t>(splits * q.s * q.n) * 8);
+    }
+    L.total = off;
+    return L;
+}
+
+template <int FMT, bool EXACT>
+void group_launch(const std::vector<CsGroupDesc>& hd, const int* cg_first,
+                  const int* cg_count, const int64_t* blk_first, const int64_t* blk_count, CsGroupDesc* dd,
+                  int* downer, int64_t max_m, int64_t max_n, int64_t max_sn, bool ntl, hipStream_t st) {
+    const int nd = static_cast<int>(hd.size());
+    for (int i0 = 0; i0 < nd; i0 += kCsChunk) {
+        CsDescChunk c{};
+        const int cnt = std::min(kCsChunk, nd - i0);
+        for (int k = 0; k < cnt; ++k) c.d[k] = hd[static_cast<size_t>(i0 + k)];
+        cs_set_desc_kernel<<<1, 64, 0, st>>>(dd + i0, c, cnt);
+    }
+    TOMOS_LAUNCH_CHECK("cs_set_desc_kernel");
+    cs_owner_kernel<<<dim3(static_cast<unsigned>(nd)), 256, 0, st>>>(dd, downer);
+    TOMOS_LAUNCH_CHECK("cs_owner_kernel");
+    cs_group_index_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_m, 256)), static_cast<unsigned>(nd)), 256, 0,
+                            st>>>(dd);
+    TOMOS_LAUNCH_CHECK("cs_group_index_kernel");
+    constexpr bool kExactMax = EXACT && FMT == FMT_P16;
+    cs_group_exp_kernel<FMT, kExactMax><<<dim3(static_cast<unsigned>(max_n), static_cast<unsigned>(nd)), 256, 0, st>>>(
+        dd);
+    TOMOS_LAUNCH_CHECK("cs_group_exp_kernel");
+    const int widths[4] = {8, 4, 2, 1};
+    for (int w = 0; w < 4; ++w) {
+        if (cg_count[w] == 0) continue;
+        int64_t smax = 0;
+        for (int k = cg_first[w]; k < cg_first[w] + cg_count[w]; ++k) smax = std::max(smax, hd[k].s);
+        const size_t lds = static_cast<size_t>(smax) * widths[w] * 8;
+        const unsigned blocks = static_cast<unsigned>(blk_count[w]);
+        const CsGroupDesc* d0 = dd + cg_first[w];
+        const int* o0 = downer + blk_first[w];
+        auto launch = [&](auto* k) {
+            TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            k<<<dim3(blocks), kVecThreads, lds, st>>>(d0, o0);
+        };
+        switch (widths[w] * 2 + (ntl ? 1 : 0)) {
+            case 17: launch(cs_group_apply_kernel<FMT, EXACT, 8, true>); break;
+            case 16: launch(cs_group_apply_kernel<FMT, EXACT, 8, false>); break;
+            case 9: launch(cs_group_apply_kernel<FMT, EXACT, 4, true>); break;
+            case 8: launch(cs_group_apply_kernel<FMT, EXACT, 4, false>); break;
+            case 5: launch(cs_group_apply_kernel<FMT, EXACT, 2, true>); break;
+            case 4: launch(cs_group_apply_kernel<FMT, EXACT, 2, false>); break;
+            case 3: launch(cs_group_apply_kernel<FMT, EXACT, 1, true>); break;
+            default: launch(cs_group_apply_kernel<FMT, EXACT, 1, false>); break;
+        }
+        TOMOS_LAUNCH_CHECK("cs_group_apply_kernel");
+    }
+    cs_group_finish_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_sn, 256)), static_cast<unsigned>(nd)), 256, 0,
+                             st>>>(dd);
+    TOMOS_LAUNCH_CHECK("cs_group_finish_kernel");
+}
+
+}  // namespace
+
 size_t countsketch_batched_scratch_bytes(int nprob, const tomos_cs_problem* pr) {
-    size_t b = 256;
+    size_t b = group_layout(nprob, pr).total;
     for (int i = 0; i < nprob; ++i) b = std::max(b, cs_scratch(pr[i].m, pr[i].n, pr[i].s));
-    return a256(b) + a256(static_cast<size_t>(std::max(nprob, 1)) * 8);
+    return b;
 }
 
 void countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_problem* pr, void* scratch,
                                size_t scratch_bytes, hipStream_t st) {
-    require(nprob >= 0 && (nprob == 0 || pr), "countsketch_apply_batched: bad problem list");
+    require(nprob >= 0 && nprob <= 65535 && (nprob == 0 || pr), "countsketch_apply_batched: bad problem list");
     require(tier == TOMOS_TIER_DEFAULT || tier == TOMOS_TIER_FAST || tier == TOMOS_TIER_EXACT,
             "countsketch_apply_batched: bad tier");
     for (int i = 0; i < nprob; ++i) {
         const tomos_cs_problem& q = pr[i];
-        require(q.m >= 0 && q.n >= 0 && q.s >= 1 && q.s <= kLdsBytes / 4 && q.m < (int64_t(1) << 31),
+        require(q.m >= 0 && q.n >= 0 && q.s >= 1 && q.s <= kLdsBytes / 4 && q.m < (int64_t(1) << 31) &&
+                    q.n < (int64_t(1) << 31),
                 "countsketch_apply_batched: bad sizes");
         require(q.lda >= std::max<int64_t>(q.m, 1) && q.ldy >= q.s, "countsketch_apply_batched: bad leading dimension");
         require(q.n == 0 || q.y, "countsketch_apply_batched: null y");
@@ -740,41 +1015,104 @@ void countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_p
     if (scratch == nullptr) scratch = fallback_scratch(need);
     else require(scratch_bytes >= need, "countsketch_apply_batched: scratch too small");
     char* sc = static_cast<char*>(scratch);
-    int* flags = reinterpret_cast<int*>(sc + (need - a256(static_cast<size_t>(nprob) * 8)));
+    const GroupLayout L = group_layout(nprob, pr);
     const bool exact = tier == TOMOS_TIER_EXACT;
-    std::vector<char> queued(static_cast<size_t>(nprob), 0);
-    for (int i = 0; i < nprob; ++i) {
-        const tomos_cs_problem& q = pr[i];
-        if (q.n == 0 || q.m == 0) continue;
-        if (exact && (a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16) && !q.wr) continue;
-        const CsLayout L = cs_layout(q.m, q.n, q.s);
-        uint32_t* packed = reinterpret_cast<uint32_t*>(sc + L.idx);
-        const bool sample = q.m >= kSampleMinRows;
-        auto go = [&](auto fmt_tag, auto exact_tag) {
-            constexpr int F = decltype(fmt_tag)::value;
-            constexpr bool E = decltype(exact_tag)::value;
-            if (!vec_ok<F>(q.a, q.wr, q.lda, E)) return;
-            cs_index_kernel<<<dim3(static_cast<unsigned>(ceil_div(q.m, 256))), 256, 0, st>>>(q.m, q.s, nullptr,
-                                                                                             nullptr, packed);
-            TOMOS_LAUNCH_CHECK("cs_index_kernel");
-            fx_enqueue<F, E>(q.m, q.n, q.a, q.wr, q.lda, q.s, packed, sc, L, q.y, q.ldy, sample, flags + 2 * i, st);
-            queued[i] = 1;
-        };
-        using T = std::true_type;
-        using Fl = std::false_type;
-        switch (a_type) {
-            case TOMOS_F32: go(std::integral_constant<int, FMT_F32>{}, Fl{}); break;
-            case TOMOS_F64: go(std::integral_constant<int, FMT_F64>{}, Fl{}); break;
-            case TOMOS_FP32_16:
-                if (exact) go(std::integral_constant<int, FMT_P16>{}, T{});
-                else go(std::integral_constant<int, FMT_P16>{}, Fl{});
-                break;
-            case TOMOS_FP32_BF16:
-                if (exact) go(std::integral_constant<int, FMT_PBF>{}, T{});
-                else go(std::integral_constant<int, FMT_PBF>{}, Fl{});
-                break;
-            default: break;
+    int* flags = reinterpret_cast<int*>(sc + L.flags);
+    // grouped problems ordered by column-group width (one apply launch each)
+    std::vector<CsGroupDesc> hd;
+    std::vector<int> which;
+    int64_t nblocks = 0;
+    int cg_first[4] = {0, 0, 0, 0}, cg_count[4] = {0, 0, 0, 0};
+    int64_t blk_first[4] = {0, 0, 0, 0}, blk_count[4] = {0, 0, 0, 0};
+    int64_t max_m = 0, max_n = 0, max_sn = 0, bytes = 0;
+    const int64_t eb = (a_type == TOMOS_F64 ? 8 : a_type == TOMOS_F32 ? 4 : 2) *
+                       (exact && (a_type == TOMOS_FP32_16 || a_type == TOMOS_FP32_BF16) ? 2 : 1);
+    const int widths[4] = {8, 4, 2, 1};
+    for (int w = 0; w < 4; ++w) {
+        cg_first[w] = static_cast<int>(hd.size());
+        blk_first[w] = nblocks;
+        for (int i = 0; i < nprob; ++i) {
+            const tomos_cs_problem& q = pr[i];
+            if (!group_takes(a_type, exact, q) || group_cg(q.s) != widths[w]) continue;
+            CsGroupDesc d{};
+            d.a = q.a;
+            d.wr = q.wr;
+            char* pb = sc + L.base[i];
+            d.packed = reinterpret_cast<uint32_t*>(pb);
+            d.colexp = reinterpret_cast<int*>(pb + a256(static_cast<size_t>(q.m) * 4));
+            d.partial = reinterpret_cast<unsigned long long*>(pb + a256(static_cast<size_t>(q.m) * 4) +
+                                                              a256(static_cast<size_t>(q.n) * 4));
+            d.flags = flags + 2 * i;
+            d.y = q.y;
+            d.m = q.m;
+            d.n = q.n;
+            d.lda = q.lda;
+            d.s = q.s;
+            d.ldy = q.ldy;
+            d.splits = static_cast<int>(ceil_div(q.m, kGroupRowsPerSplit));
+            d.rows_per_split = ceil_div(ceil_div(q.m, d.splits), 8) * 8;
+            d.splits = static_cast<int>(ceil_div(q.m, d.rows_per_split));
+            d.ncg = static_cast<int>(ceil_div(q.n, widths[w]));
+            d.shift = fx_shift(q.m);
+            if (q.m >= kSampleMinRows) {
+                d.stride = (q.m / kSampleSegs) / 8 * 8;
+                d.len = static_cast<int>(std::max<int64_t>(64, std::min<int64_t>(1024, (q.m / 256) / 8 * 8)));
+                d.segs = static_cast<int>(kSampleSegs);
+                d.margin = kSampleMargin;
+            } else {  // every row: one segment
+                d.stride = 0;
+                d.len = static_cast<int>(ceil_div(q.m, 8) * 8);
+                d.segs = 1;
+                d.margin = 0;
+            }
+            d.block0 = static_cast<int>(nblocks - blk_first[w]);
+            d.local = static_cast<int>(hd.size()) - cg_first[w];
+            d.owner0 = static_cast<int>(nblocks);
+            nblocks += static_cast<int64_t>(d.splits) * d.ncg;
+            hd.push_back(d);
+            which.push_back(i);
+            max_m = std::max(max_m, q.m);
+            max_n = std::max(max_n, q.n);
+            max_sn = std::max(max_sn, q.s * q.n);
+            bytes += q.m * q.n * eb;
         }
+        cg_count[w] = static_cast<int>(hd.size()) - cg_first[w];
+        blk_count[w] = nblocks - blk_first[w];
+    }
+    require(nblocks <= L.blocks && nblocks < (int64_t(1) << 31), "countsketch_apply_batched: grid too large");
+    TOMOS_HIP_CHECK(hipMemsetAsync(flags, 0, static_cast<size_t>(nprob) * 8, st));
+    std::vector<char> grouped(static_cast<size_t>(nprob), 0);
+    if (!hd.empty()) {
+        CsGroupDesc* dd = reinterpret_cast<CsGroupDesc*>(sc + L.desc);
+        int* downer = reinterpret_cast<int*>(sc + L.owner);
+        const bool ntl = stream_loads(bytes);
+        switch (a_type) {
+            case TOMOS_F32:
+                group_launch<FMT_F32, false>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer, max_m,
+                                             max_n, max_sn, ntl, st);
+                break;
+            case TOMOS_F64:
+                group_launch<FMT_F64, false>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer, max_m,
+                                             max_n, max_sn, ntl, st);
+                break;
+            case TOMOS_FP32_16:
+                if (exact)
+                    group_launch<FMT_P16, true>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer,
+                                                max_m, max_n, max_sn, ntl, st);
+                else
+                    group_launch<FMT_P16, false>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer,
+                                                 max_m, max_n, max_sn, ntl, st);
+                break;
+            default:
+                if (exact)
+                    group_launch<FMT_PBF, true>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer,
+                                                max_m, max_n, max_sn, ntl, st);
+                else
+                    group_launch<FMT_PBF, false>(hd, cg_first, cg_count, blk_first, blk_count, dd, downer,
+                                                 max_m, max_n, max_sn, ntl, st);
+                break;
+        }
+        for (int i : which) grouped[static_cast<size_t>(i)] = 1;
     }
     std::vector<int> h(2 * static_cast<size_t>(nprob), 0);
     TOMOS_HIP_CHECK(hipMemcpyAsync(h.data(), flags, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -782,7 +1120,7 @@ void countsketch_apply_batched(int a_type, int tier, int nprob, const tomos_cs_p
     for (int i = 0; i < nprob; ++i) {
         const tomos_cs_problem& q = pr[i];
         if (q.n == 0) continue;
-        if (queued[i] && !h[2 * i] && !h[2 * i + 1]) continue;
+        if (grouped[static_cast<size_t>(i)] && !h[2 * i] && !h[2 * i + 1]) continue;
         countsketch_apply(a_type, tier, q.m, q.n, q.a, q.wr, q.lda, q.s, q.y, q.ldy, sc, need, st);
     }
 }

@@ -109,7 +109,16 @@ __device__ __forceinline__ double strided_sum(const double* __restrict__ p, int 
     return (s0 + s1) + (s2 + s3);
 }
 
-__global__ void qr_set_desc_kernel(QrDesc* d, QrDesc v) { *d = v; }
+// Descriptors travel as kernel arguments, kQrDescChunk a launch (under 4 KB)
+constexpr int kQrDescChunk = 24;
+struct QrDescChunk {
+    QrDesc d[kQrDescChunk];
+};
+static_assert(sizeof(QrDescChunk) + 16 <= 4096, "descriptor chunk exceeds the kernel-argument limit");
+
+__global__ __launch_bounds__(64) void qr_set_desc_kernel(QrDesc* __restrict__ dst, QrDescChunk c, int count) {
+    if (static_cast<int>(threadIdx.x) < count) dst[threadIdx.x] = c.d[threadIdx.x];
+}
 __global__ void qr_status_reset_kernel(int* status) { *status = 0; }
 
 constexpr int kPT = 1024;  // panel-step block
@@ -1465,6 +1474,15 @@ __global__ __launch_bounds__(kT) void copy_r_kernel(const QrDesc* __restrict__ D
     d.wp[i + j * d.n] = d.r[i + j * d.ldr];
 }
 
+// R^-1 (d.r, ld ldr) over the leading n x n block of the problem's A (ld lda)
+__global__ __launch_bounds__(kT) void copy_back_kernel(const QrDesc* __restrict__ D) {
+    const QrDesc& d = D[blockIdx.z];
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+    if (t >= d.n * d.n) return;
+    const int64_t j = t / d.n, i = t - j * d.n;
+    d.a[i + j * d.lda] = d.r[i + j * d.ldr];
+}
+
 // Column j of R^-1 (problem blockIdx.z) from the copy in wp into d.r:
 // R x = e_j by back substitution in 32-row blocks.  For block I (bottom
 // up) the block's right-hand side e_j - R(I, >I) x(>I) is a GEMV over the
@@ -1625,11 +1643,16 @@ Batch setup(int nprob, const QrProblem* probs, void* scratch, size_t scratch_byt
     B.dev = reinterpret_cast<QrDesc*>(base);
     B.status = reinterpret_cast<int*>(base + a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)));
     char* p = base + a256(static_cast<size_t>(std::max(nprob, 1)) * sizeof(QrDesc)) + status_bytes(nprob);
-    for (int i = 0; i < nprob; ++i) {
-        const QrDesc d = carve(p, probs[i]);
-        qr_set_desc_kernel<<<1, 1, 0, s>>>(B.dev + i, d);  // by value: no host buffer to keep alive
-        B.kmax = std::max(B.kmax, probs[i].k);
-        B.nmax = std::max(B.nmax, probs[i].n);
+    for (int i0 = 0; i0 < nprob; i0 += kQrDescChunk) {
+        QrDescChunk c{};  // by value: no host buffer to keep alive
+        const int cnt = std::min(kQrDescChunk, nprob - i0);
+        for (int k = 0; k < cnt; ++k) {
+            const QrProblem& q = probs[i0 + k];
+            c.d[k] = carve(p, q);
+            B.kmax = std::max(B.kmax, q.k);
+            B.nmax = std::max(B.nmax, q.n);
+        }
+        qr_set_desc_kernel<<<1, 64, 0, s>>>(B.dev + i0, c, cnt);
     }
     TOMOS_LAUNCH_CHECK("qr_set_desc_kernel");
     qr_status_reset_kernel<<<1, 1, 0, s>>>(B.status);
@@ -1854,7 +1877,7 @@ size_t geqrf_scratch_bytes(int64_t k, int64_t n) {
 size_t geqrf_batched_scratch_bytes(int nprob, const QrProblem* probs) { return batch_bytes(nprob, probs); }
 
 void geqrf_r_batched(int nprob, const QrProblem* probs, bool invert_r, void* scratch,
-                     size_t scratch_bytes, hipStream_t s) {
+                     size_t scratch_bytes, hipStream_t s, bool copy_back) {
     require(nprob >= 0 && (nprob == 0 || probs), "geqrf: bad problem list");
     int64_t nmax = 0;
     for (int i = 0; i < nprob; ++i) {
@@ -1865,6 +1888,11 @@ void geqrf_r_batched(int nprob, const QrProblem* probs, bool invert_r, void* scr
     const Batch B = setup(nprob, probs, scratch, scratch_bytes, s);
     const std::vector<int> st = factor_checked(B, invert_r, s);
     if (invert_r) invert(B, st, s);
+    if (copy_back) {
+        copy_back_kernel<<<dim3(static_cast<unsigned>(ceil_div(nmax * nmax, kT)), 1, static_cast<unsigned>(nprob)),
+                           kT, 0, s>>>(B.dev);
+        TOMOS_LAUNCH_CHECK("copy_back_kernel");
+    }
     TOMOS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1904,12 +1932,7 @@ void sketch_precond_batched(int nprob, QrProblem* probs, void* scratch, size_t s
         p += a256(static_cast<size_t>(probs[i].ldr) * probs[i].n * 8);
     }
     const size_t rest = scratch_bytes - static_cast<size_t>(p - static_cast<char*>(scratch));
-    geqrf_r_batched(nprob, probs, true, p, rest, s);
-    for (int i = 0; i < nprob; ++i)
-        if (probs[i].n > 0)
-            TOMOS_HIP_CHECK(hipMemcpy2DAsync(probs[i].a, probs[i].lda * 8, probs[i].r, probs[i].ldr * 8,
-                                             probs[i].n * 8, probs[i].n, hipMemcpyDeviceToDevice, s));
-    TOMOS_HIP_CHECK(hipStreamSynchronize(s));
+    geqrf_r_batched(nprob, probs, true, p, rest, s, true);
 }
 
 }  // namespace tomos_hip
