@@ -63,6 +63,9 @@ def parse():
                    help="entries per row of the CSR LSQR leg (0 = skip)")
     p.add_argument("--batched", type=int, default=64,
                    help="batched differentiable least squares: number of uneven problems (0 = skip)")
+    p.add_argument("--batched-sketch-factor", type=int, default=8,
+                   help="sketch rows of the batched problems: s = factor * n (8: fewer forward and adjoint "
+                        "iterations than 4 for a slightly dearer sketch QR)")
     p.add_argument("--batched-restarts", type=int, default=12,
                    help="restart cap of the batched solve (its graph holds that many restarts)")
     p.add_argument("--adjoint-workers", type=int, default=8,
@@ -203,7 +206,7 @@ def sparse_leg(args, C, DeviceBuffer, ev0, ev1):
 def batched_leg(args, C, DeviceBuffer, ev0, ev1):
     """64 uneven problems: m ~ U{20000..200000}, n ~ U{64..512}, kappa
     log-uniform in [1e2, 1e5] (seed 7), A = G diag(d) stored FP32_BF16,
-    b = A x_true.  Forward: batched CountSketch (s = 4n) + batched sketch-QR +
+    b = A x_true.  Forward: batched CountSketch (s = 8n by default) + batched sketch-QR +
     batched LSQR to 1e-6 (one hipGraph per restart).  Backward: for
     g ~ N(0,1), w = (A^T A)^-1 g per problem (adjoint CG) and grad_b, grad_A
     (rank-2 kernel, m x n FP32 each).  Also the same forward one problem at
@@ -228,7 +231,7 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
         bd = DeviceBuffer(m * 8)
         C.call("tomos_gemv", C.FP32_BF16, C.TIER_EXACT, 0, 1.0, m, n, wh.ptr, wr.ptr, lda, C.F64, xt.ptr, 0.0,
                C.F64, bd.ptr, None, 0, None)
-        s = 4 * n
+        s = args.batched_sketch_factor * n
         probs.append(dict(m=m, n=n, lda=lda, wh=wh, wr=wr, b=bd, xt=xt, s=s, sk=DeviceBuffer(s * n * 8),
                           x=DeviceBuffer(n * 8)))
     cp = (C.CsProblem * P)(*[C.CsProblem(p["m"], p["n"], p["lda"], p["s"], p["s"], p["wh"].ptr, p["wr"].ptr,
@@ -366,6 +369,7 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
     del wds, gbds, gads
     its_l = list(its)
     return {"problems": P, "elements": int((ms * ns).sum()), "storage": "FP32_BF16 planar",
+            "sketch_rows": "%d n" % args.batched_sketch_factor,
             "precond_ms": round(pre_ms, 2), "solve_ms": round(solve_ms, 2), "forward_ms": round(pre_ms + solve_ms, 2),
             "solve_one_at_a_time_ms": round(seq_ms, 2), "batching_speedup": round(seq_ms / solve_ms, 2),
             "iterations_min": min(its_l), "iterations_max": max(its_l), "worst_relres": worst,
@@ -373,6 +377,10 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
             "backward_one_at_a_time_ms": round(bwd_seq_ms, 2), "adjoint_tol": adj_tol,
             "adjoint_iterations_max": max(grp_its), "adjoint_iterations_max_one_problem": max(adj_its),
             "gradient_errors_vs_dense_fp64_qr": grad_err,
+            "forward": "forward_ms = tomos_countsketch_apply_batched + tomos_sketch_qr_precond_batched_f64 + "
+                       "tomos_lsqr_batched_solve to relres 1e-6 (one graph launch); tomos_amd.autograd.lstsq_batched "
+                       "runs the same and, with its default normal_correction, one more batched adjoint solve "
+                       "(about backward_ms without the gradients)",
             "backward": "tomos_lsq_adjoint_batched, workers = 0: every problem's adjoint CG (tol %g) in grouped "
                         "device launches (replayed graph segments), then grad_b and grad_A (m x n FP32) per problem; "
                         "backward_threads_ms: the same over %d host threads / HIP streams" % (adj_tol, aw)}

@@ -55,7 +55,7 @@ class SolveOptions:
     tol: float = 1e-6            # relative residual target (forward and adjoint)
     inner: int = 30              # Krylov iterations per restart
     restarts: int = 40
-    sketch_factor: int = 4       # sketch rows s = sketch_factor * n
+    sketch_factor: int = 8       # sketch rows s = sketch_factor * n (8: fewer adjoint iterations than 4)
     adjoint_tol: float = 1e-10   # |g - A^T A w| / |g| for the adjoint solve
     check_every: int = 8
     normal_correction: bool = True  # forward: x += (A^T A)^-1 A^T (b - A x), exact products

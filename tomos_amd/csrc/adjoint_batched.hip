@@ -11,8 +11,10 @@
 //              u = A d with a zero u coefficient, A^T u per tile range, |u|^2
 //              per tile range) for all iterating problems, then one
 //              1024-thread block per problem for the n-sized work: alpha =
-//              rho/|q|^2, w += alpha d, res -= alpha A^T q, the stop test,
-//              z = R^-1 R^-T res, rho' = res.z, d = z + (rho'/rho) d;
+//              rho/|q|^2, w += alpha d, res -= alpha A^T q, the stop test;
+//              then, over many blocks per problem, y = R^-T res (a wave per
+//              column) and z = R^-1 y (64 rows a block), and one block per
+//              problem for rho' = res.z, d = z + (rho'/rho) d;
 //   restart:   per problem, the exact-tier GEMV pair (a no-op launch for a
 //              problem that is not restarting), then one block per problem:
 //              res, |res|/|g|, the stop test, z, rho and d.
@@ -57,7 +59,8 @@ struct GState {
     int completed;      // inner iterations in total
     int skip_restart;   // restart GEMVs are a no-op (not restarting, or w == 0)
     int skip_iter;      // grouped single-pass kernel is a no-op (not iterating)
-    int pad[2];
+    int pc;             // a step continues: z = R^-1 R^-T res and d pending
+    int pad;
 };
 
 struct GProb {
@@ -66,6 +69,8 @@ struct GProb {
     double* w;
     const double* rinv;
     double *res, *z, *d, *rfull;  // n each
+    double* y;                    // R^-T res (n)
+    double* rzp;                  // res.z partial per 64-row chunk of z
     GState* st;
     int nitems;
     double* sumsq;  // [nitems]
@@ -81,6 +86,7 @@ __global__ void k_gadj_init(const GProb* __restrict__ P, int nprob) {
     s->restarts_done = s->iters = s->completed = 0;
     s->skip_restart = 1;  // the first restart has w = 0: res = g, no products
     s->skip_iter = 1;
+    s->pc = 0;
 }
 
 __global__ __launch_bounds__(kTail) void k_gadj_zero_w(const GProb* __restrict__ P) {
@@ -174,9 +180,7 @@ __global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P
     const GProb& p = P[blockIdx.x];
     GState* S = p.st;
     if (S->phase != kPhaseIterate) return;
-    extern __shared__ double y[];
     __shared__ double sh[kTail / 64];
-    __shared__ int go;
     const double rho = S->rho;
     const int iters = S->iters;
     const int64_t n = p.n;
@@ -200,23 +204,90 @@ __global__ __launch_bounds__(kTail) void k_gadj_step(const GProb* __restrict__ P
         S->resnorm = sqrt(rr);
         const double rel = S->resnorm / S->gnorm;
         const bool end = rel < tol || q2 <= 0.0 || iters + 1 >= max_iters || S->resnorm < drop * S->res0;
-        go = end ? 0 : 1;
         if (end) {
             S->phase = kPhaseRestart;
             S->skip_iter = 1;
             S->skip_restart = 0;
         }
+        S->pc = end ? 0 : 1;  // k_gadj_pre1 / _pre2 / _fin finish a continuing step
     }
+}
+
+// The step's z = R^-1 (R^-T res), rho' = res.z and d = z + (rho'/rho) d,
+// spread over many blocks per problem (one block per problem reads all of
+// R^-1 twice through one CU).  pre1: y = R^-T res, one wave per column j
+// (column j of R^-1, rows 0..j, dotted with res).
+constexpr int kPreRows = 64;                 // rows of z per k_gadj_pre2 block
+constexpr int kPreSlices = kTail / kPreRows;  // column slices per row in k_gadj_pre2
+__global__ __launch_bounds__(kTail) void k_gadj_pre1(const GProb* __restrict__ P) {
+    const GProb& p = P[blockIdx.y];
+    if (!p.st->pc) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * (kTail / 64) + (threadIdx.x >> 6);
+    if (j >= p.n) return;
+    const double* col = p.rinv + j * p.ldr;
+    double a = 0.0;
+    for (int64_t i = lane; i <= j; i += 64) a += col[i] * p.res[i];
+    for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
+    if (lane == 0) p.y[j] = a;
+}
+
+// pre2: z = R^-1 y for kPreRows rows per block, kPreSlices column slices
+// (slice k takes columns i0 + k, i0 + k + kPreSlices, ...; a wave reads one
+// column's kPreRows entries per step), the slices added in a fixed order;
+// res.z of the block's rows into rzp[blockIdx.x]
+__global__ __launch_bounds__(kTail) void k_gadj_pre2(const GProb* __restrict__ P) {
+    const GProb& p = P[blockIdx.y];
+    if (!p.st->pc) return;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kPreRows;
+    if (i0 >= p.n) return;
+    __shared__ double part[kPreSlices][kPreRows];
+    const int r = threadIdx.x & (kPreRows - 1), sl = threadIdx.x / kPreRows;
+    const int64_t i = i0 + r;
+    double a = 0.0;
+    if (i < p.n) {
+#pragma unroll 4
+        for (int64_t j = i0 + sl; j < p.n; j += kPreSlices)
+            if (j >= i) a += p.rinv[i + j * p.ldr] * p.y[j];
+    }
+    part[sl][r] = a;
     __syncthreads();
-    if (!go) return;
-    const double rz = gadj_precond(p, y, sh);
+    if (threadIdx.x < kPreRows) {
+        double z = 0.0;
+#pragma unroll
+        for (int k = 0; k < kPreSlices; ++k) z += part[k][r];
+        double rz = 0.0;
+        if (i < p.n) {
+            p.z[i] = z;
+            rz = p.res[i] * z;
+        }
+        for (int o = 32; o >= 1; o >>= 1) rz += __shfl_xor(rz, o);
+        if (r == 0) p.rzp[blockIdx.x] = rz;
+    }
+}
+
+// fin: rho' = sum of the chunks' res.z (fixed order), d = z + (rho'/rho) d
+__global__ __launch_bounds__(256) void k_gadj_fin(const GProb* __restrict__ P) {
+    const GProb& p = P[blockIdx.x];
+    GState* S = p.st;
+    if (!S->pc) return;
+    const int64_t nch = (p.n + kPreRows - 1) / kPreRows;
+    double rz = 0.0;
+    for (int64_t c = 0; c < nch; ++c) rz += p.rzp[c];
+    const double rho = S->rho;
     const double beta = rho != 0.0 ? rz / rho : 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += kTail) p.d[j] = p.z[j] + beta * p.d[j];
-    if (threadIdx.x == 0) S->rho = rz;
+    for (int64_t j = threadIdx.x; j < p.n; j += 256) p.d[j] = p.z[j] + beta * p.d[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        S->rho = rz;
+        S->pc = 0;
+    }
 }
 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
-constexpr int kNVecs = 5;  // n-vectors per problem in the scratch
+constexpr int kNVecs = 6;  // n-vectors per problem in the scratch
+
+size_t rzp_bytes(int64_t n) { return a256(static_cast<size_t>(ceil_div(std::max<int64_t>(n, 1), kPreRows)) * 8); }
 
 struct GPlan {
     int g = 0, threads = 256, rounds = 0;
@@ -290,8 +361,10 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
     L.states = take(sizeof(GState) * static_cast<size_t>(std::max(nprob, 1)));
     for (int i = 0; i < nprob; ++i) {
         const size_t nb = a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8);
-        // res, z, d, rfull, w (FP64, n), q (FP32, m), q64 (FP64, m)
-        L.vec_at.push_back(take(kNVecs * nb + a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 4) +
+        // res, z, d, rfull, w, y (FP64, n), rzp (FP64, one per 64 rows of n),
+        // q (FP32, m), q64 (FP64, m)
+        L.vec_at.push_back(take(kNVecs * nb + rzp_bytes(pr[i].n) +
+                                a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 4) +
                                 a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 8)));
         L.sumsq_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[static_cast<size_t>(i)], 1)) * 8));
         L.tpart_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[static_cast<size_t>(i)], 1)) *
@@ -387,9 +460,12 @@ struct Grouped {
             q.d = reinterpret_cast<double*>(vb + 2 * nb);
             q.rfull = reinterpret_cast<double*>(vb + 3 * nb);
             q.w = reinterpret_cast<double*>(vb + 4 * nb);
-            float* qf = reinterpret_cast<float*>(vb + kNVecs * nb);
-            q64[static_cast<size_t>(i)] = reinterpret_cast<double*>(
-                vb + kNVecs * nb + a256(static_cast<size_t>(std::max<int64_t>(p.m, 1)) * 4));
+            q.y = reinterpret_cast<double*>(vb + 5 * nb);
+            q.rzp = reinterpret_cast<double*>(vb + kNVecs * nb);
+            char* qb = vb + kNVecs * nb + rzp_bytes(p.n);
+            float* qf = reinterpret_cast<float*>(qb);
+            q64[static_cast<size_t>(i)] =
+                reinterpret_cast<double*>(qb + a256(static_cast<size_t>(std::max<int64_t>(p.m, 1)) * 4));
             wbuf[static_cast<size_t>(i)] = q.w;
             rfull[static_cast<size_t>(i)] = q.rfull;
             q.st = states() + i;
@@ -442,8 +518,15 @@ struct Grouped {
     void iter_seq() {
         for (int it = 0; it < kSegIters; ++it) {
             if (L.nitems) grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s);
-            k_gadj_step<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.max_iters,
-                                                                          adjoint_inner_drop(o));
+            k_gadj_step<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs(), o.tol, o.max_iters,
+                                                                      adjoint_inner_drop(o));
+            k_gadj_pre1<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTail / 64))),
+                               static_cast<unsigned>(nprob)),
+                          kTail, 0, s>>>(dprobs());
+            k_gadj_pre2<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kPreRows))),
+                               static_cast<unsigned>(nprob)),
+                          kTail, 0, s>>>(dprobs());
+            k_gadj_fin<<<static_cast<unsigned>(nprob), 256, 0, s>>>(dprobs());
         }
         TOMOS_LAUNCH_CHECK("k_gadj_step");
     }
@@ -497,8 +580,6 @@ struct Grouped {
     void run(const double* const* g, double* const* w, int* iterations) {
         upload(g);
         TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gadj_restart),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds())));
-        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gadj_step),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds())));
         k_gadj_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
         k_gadj_zero_w<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
