@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "adjoint.hpp"
+#include "codec.hpp"
 #include "common.hpp"
 #include "gemv.hpp"
 #include "lsqr_fused.hpp"
@@ -64,13 +65,17 @@ struct GState {
 };
 
 struct GProb {
-    int64_t m, n, ldr;
+    int64_t m, n, ldr, lda;
+    const void* a;        // A (hi plane when packed)
+    const uint16_t* wr;   // lo plane (packed)
     const double* g;
     double* w;
     const double* rinv;
     double *res, *z, *d, *rfull;  // n each
     double* y;                    // R^-T res (n)
     double* rzp;                  // res.z partial per 64-row chunk of z
+    double* rpart;                // exact restart: A^T A w per row item [nrx][n]
+    int nrx;                      // row items of the exact restart (0: rfull from the GEMV pair)
     GState* st;
     int nitems;
     double* sumsq;  // [nitems]
@@ -129,6 +134,152 @@ __device__ double gadj_precond(const GProb& p, double* y, double* sh) {
     return gsum(rz, sh);
 }
 
+// Exact restart products of every restarting problem in one launch.  An
+// item is kRxRows rows of one problem: each lane takes one 16-byte row
+// vector (8 packed / 4 FP32 rows) of both planes, forms its rows of
+// q = A w with the exact values and FP64 sums over all n columns (phase 1,
+// q stays in registers), then reads the same vectors again (from the
+// caches) for the item's share of A^T q (phase 2): per batch of 8 columns
+// a transposing reduction over the wave's lanes leaves each column's wave
+// sum in one lane group, the 4 waves' sums are added in a fixed order and
+// written to rpart[item] (n FP64).  The restart kernel adds the items of a
+// problem in order.
+constexpr int kRxThreads = 256;
+constexpr int64_t kRxMaxN = 1024;  // LDS: 4 waves x n FP64 sums
+struct RxItem {
+    int prob, slot;
+    int64_t row0, row1;
+};
+
+template <int FMT>  // 0: F32, 1: FP32_BF16, 2: FP32_16 (wh + wr)
+__device__ __forceinline__ void rx_decode(const uint4& h, const uint4& l, float (&v)[FMT == 0 ? 4 : 8]) {
+    if constexpr (FMT == 0) {
+        v[0] = __uint_as_float(h.x);
+        v[1] = __uint_as_float(h.y);
+        v[2] = __uint_as_float(h.z);
+        v[3] = __uint_as_float(h.w);
+    } else {
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (FMT == 1) {
+                v[2 * k] = __uint_as_float((hw[k] << 16) | (lw[k] & 0xFFFFu));
+                v[2 * k + 1] = __uint_as_float((hw[k] & 0xFFFF0000u) | (lw[k] >> 16));
+            } else {
+                v[2 * k] = codec::fp32_16_decode_exact(hw[k] & 0xFFFFu, lw[k] & 0xFFFFu);
+                v[2 * k + 1] = codec::fp32_16_decode_exact(hw[k] >> 16, lw[k] >> 16);
+            }
+        }
+    }
+}
+
+template <int FMT>
+__device__ __forceinline__ void rx_load(const GProb& p, int64_t r, int nv, int64_t j, float (&v)[FMT == 0 ? 4 : 8]) {
+    constexpr int RPL = FMT == 0 ? 4 : 8;
+    constexpr int ESZ = FMT == 0 ? 4 : 2;
+    const int64_t e = r + j * p.lda;
+    if (nv == RPL) {
+        const uint4 h = *reinterpret_cast<const uint4*>(static_cast<const char*>(p.a) + e * ESZ);
+        uint4 l = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (FMT != 0) l = *reinterpret_cast<const uint4*>(p.wr + e);
+        rx_decode<FMT>(h, l, v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            if (k < nv) {
+                if constexpr (FMT == 0) {
+                    v[k] = static_cast<const float*>(p.a)[e + k];
+                } else {
+                    const uint32_t h = static_cast<const uint16_t*>(p.a)[e + k], l = p.wr[e + k];
+                    v[k] = FMT == 1 ? __uint_as_float((h << 16) | l) : codec::fp32_16_decode_exact(h, l);
+                }
+            } else {
+                v[k] = 0.0f;
+            }
+        }
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kRxThreads) void k_gadj_rx(const GProb* __restrict__ P, const RxItem* __restrict__ items) {
+    constexpr int RPL = FMT == 0 ? 4 : 8;
+    constexpr int CB = 8;  // columns per transposing reduction
+    const RxItem it = items[blockIdx.x];
+    const GProb& p = P[it.prob];
+    if (p.st->skip_restart) return;
+    __shared__ double wsum[kRxThreads / 64][kRxMaxN];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t n = p.n;
+    const int64_t r = it.row0 + static_cast<int64_t>(threadIdx.x) * RPL;
+    const int nv = r < it.row1 ? static_cast<int>(min<int64_t>(RPL, it.row1 - r)) : 0;
+    // phase 1: q = A w for this lane's rows
+    double q[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) q[k] = 0.0;
+    if (nv > 0) {
+        int64_t j = 0;
+        for (; j + 4 <= n; j += 4) {
+            float v[4][RPL];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) rx_load<FMT>(p, r, nv, j + c, v[c]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double wj = p.w[j + c];
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) q[k] = fma(static_cast<double>(v[c][k]), wj, q[k]);
+            }
+        }
+        for (; j < n; ++j) {
+            float v[RPL];
+            rx_load<FMT>(p, r, nv, j, v);
+            const double wj = p.w[j];
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) q[k] = fma(static_cast<double>(v[k]), wj, q[k]);
+        }
+    }
+    // phase 2: A^T q over this item's rows, CB columns at a time
+    for (int64_t j0 = 0; j0 < n; j0 += CB) {
+        double cs[CB];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            double t = 0.0;
+            if (nv > 0 && j0 + c < n) {
+                float v[RPL];
+                rx_load<FMT>(p, r, nv, j0 + c, v);
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) t = fma(static_cast<double>(v[k]), q[k], t);
+            }
+            cs[c] = t;
+        }
+        // halving stages on lane bits 5, 4, 3 (8 -> 1 values), then bits 2..0
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            const int off = 32 >> st, half = CB >> (st + 1);
+            const bool upper = (lane & off) != 0;
+#pragma unroll
+            for (int t2 = 0; t2 < 4; ++t2) {
+                if (t2 < half) {
+                    const double send = upper ? cs[t2] : cs[t2 + half];
+                    const double keep = upper ? cs[t2 + half] : cs[t2];
+                    cs[t2] = keep + __shfl_xor(send, off);
+                }
+            }
+        }
+        for (int off = 4; off >= 1; off >>= 1) cs[0] += __shfl_xor(cs[0], off);
+        // the column this lane group holds: bit 5 -> +4, bit 4 -> +2, bit 3 -> +1
+        const int sub = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+        if ((lane & 7) == 0 && j0 + sub < n) wsum[wave][j0 + sub] = cs[0];
+    }
+    __syncthreads();
+    double* out = p.rpart + static_cast<int64_t>(it.slot) * n;
+    for (int64_t j = threadIdx.x; j < n; j += kRxThreads) {
+        double t = wsum[0][j];
+#pragma unroll
+        for (int w2 = 1; w2 < kRxThreads / 64; ++w2) t += wsum[w2][j];
+        out[j] = t;
+    }
+}
+
 // Restart of every problem whose phase is kPhaseRestart: res = g - rfull
 // (rfull = A^T A w, or nothing on the first restart), |g| on the first, the
 // exact-residual stop test (the restart after `max_restarts` earlier ones is
@@ -144,7 +295,15 @@ __global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict_
     const bool first = k == 0;
     double gg = 0.0, rr = 0.0;
     for (int64_t j = threadIdx.x; j < p.n; j += kTail) {
-        const double rj = p.g[j] - (first ? 0.0 : p.rfull[j]);
+        double rf = 0.0;
+        if (!first) {
+            if (p.nrx > 0) {
+                for (int k2 = 0; k2 < p.nrx; ++k2) rf += p.rpart[static_cast<int64_t>(k2) * p.n + j];
+            } else {
+                rf = p.rfull[j];
+            }
+        }
+        const double rj = p.g[j] - rf;
         p.res[j] = rj;
         gg += p.g[j] * p.g[j];
         rr += rj * rj;
@@ -298,6 +457,15 @@ struct GPlan {
     size_t probs = 0, gprobs = 0, items_off = 0, states = 0, gemv_sc = 0, gemv_bytes = 0, grad_sc = 0,
            grad_bytes = 0, total = 0;
     std::vector<size_t> vec_at, sumsq_at, tpart_at;
+    // exact restart products in one grouped launch (k_gadj_rx) when the
+    // restart tier decodes exactly, n <= kRxMaxN and the planes are aligned;
+    // otherwise the per-problem GEMV pair
+    bool use_rx = false;
+    int nrx = 0;
+    std::vector<RxItem> rx;
+    std::vector<int> prob_rx;
+    std::vector<size_t> rpart_at;
+    size_t rx_off = 0;
 };
 
 void grouped_validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr) {
@@ -349,6 +517,21 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
         L.prob_items[static_cast<size_t>(i)] = k;
     }
     L.nitems = static_cast<int>(L.items.size());
+    L.use_rx = (o.a_type == TOMOS_F32 || o.restart_tier == TOMOS_TIER_EXACT) && L.nmax <= kRxMaxN;
+    for (int i = 0; i < nprob && L.use_rx; ++i)
+        if (pr[i].n > 0 && o.a_type != TOMOS_F32 && reinterpret_cast<uintptr_t>(pr[i].wr) % 16 != 0) L.use_rx = false;
+    L.prob_rx.assign(static_cast<size_t>(nprob), 0);
+    if (L.use_rx) {
+        const int64_t rows = static_cast<int64_t>(kRxThreads) * (o.a_type == TOMOS_F32 ? 4 : 8);
+        for (int i = 0; i < nprob; ++i) {
+            if (pr[i].n == 0) continue;
+            int k = 0;
+            for (int64_t r0 = 0; r0 < pr[i].m; r0 += rows, ++k)
+                L.rx.push_back(RxItem{i, k, r0, std::min(pr[i].m, r0 + rows)});
+            L.prob_rx[static_cast<size_t>(i)] = k;
+        }
+        L.nrx = static_cast<int>(L.rx.size());
+    }
     size_t off = 0;
     auto take = [&](size_t b) {
         const size_t at = off;
@@ -359,7 +542,10 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
     L.gprobs = take(sizeof(GroupProb) * static_cast<size_t>(std::max(nprob, 1)));
     L.items_off = take(sizeof(GroupItem) * static_cast<size_t>(std::max(L.nitems, 1)));
     L.states = take(sizeof(GState) * static_cast<size_t>(std::max(nprob, 1)));
+    L.rx_off = take(sizeof(RxItem) * static_cast<size_t>(std::max(L.nrx, 1)));
     for (int i = 0; i < nprob; ++i) {
+        L.rpart_at.push_back(take(static_cast<size_t>(L.prob_rx[static_cast<size_t>(i)]) *
+                                  static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8 + 8));
         const size_t nb = a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8);
         // res, z, d, rfull, w, y (FP64, n), rzp (FP64, one per 64 rows of n),
         // q (FP32, m), q64 (FP64, m)
@@ -380,16 +566,20 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
 }
 
 // Everything a captured segment depends on (grids, LDS, scalars, scratch).
+// The per-problem GEMV pair of the restart (no grouped exact pass) takes
+// each problem's matrix, shape and vectors as launch arguments, so its
+// graph also depends on those (`prob`).
 struct SegKey {
-    int device, kind, nprob, nitems, g, threads, rounds, a_type, restart_tier, max_iters, restarts;
+    int device, kind, nprob, nitems, nrx, g, threads, rounds, a_type, restart_tier, max_iters, restarts;
     size_t lds;
     double tol;
     const void* scratch;
+    std::vector<int64_t> prob;
     bool operator==(const SegKey& k) const {
-        return device == k.device && kind == k.kind && nprob == k.nprob && nitems == k.nitems && g == k.g &&
-               threads == k.threads && rounds == k.rounds && a_type == k.a_type && restart_tier == k.restart_tier &&
-               max_iters == k.max_iters && restarts == k.restarts && lds == k.lds && tol == k.tol &&
-               scratch == k.scratch;
+        return device == k.device && kind == k.kind && nprob == k.nprob && nitems == k.nitems && nrx == k.nrx &&
+               g == k.g && threads == k.threads && rounds == k.rounds && a_type == k.a_type &&
+               restart_tier == k.restart_tier && max_iters == k.max_iters && restarts == k.restarts && lds == k.lds &&
+               tol == k.tol && scratch == k.scratch && prob == k.prob;
     }
 };
 
@@ -453,6 +643,9 @@ struct Grouped {
             q.m = p.m;
             q.n = p.n;
             q.ldr = p.ldr;
+            q.lda = p.lda;
+            q.a = p.a;
+            q.wr = p.wr;
             q.g = g[i];
             q.rinv = p.rinv;
             q.res = reinterpret_cast<double*>(vb);
@@ -469,6 +662,8 @@ struct Grouped {
             wbuf[static_cast<size_t>(i)] = q.w;
             rfull[static_cast<size_t>(i)] = q.rfull;
             q.st = states() + i;
+            q.rpart = reinterpret_cast<double*>(sc + L.rpart_at[static_cast<size_t>(i)]);
+            q.nrx = L.use_rx ? L.prob_rx[static_cast<size_t>(i)] : 0;
             q.nitems = L.prob_items[static_cast<size_t>(i)];
             q.sumsq = reinterpret_cast<double*>(sc + L.sumsq_at[static_cast<size_t>(i)]);
             q.tpart = reinterpret_cast<float*>(sc + L.tpart_at[static_cast<size_t>(i)]);
@@ -490,6 +685,9 @@ struct Grouped {
         if (L.nitems)
             TOMOS_HIP_CHECK(hipMemcpyAsync(ditems(), L.items.data(), sizeof(GroupItem) * L.items.size(),
                                            hipMemcpyHostToDevice, s));
+        if (L.nrx)
+            TOMOS_HIP_CHECK(hipMemcpyAsync(sc + L.rx_off, L.rx.data(), sizeof(RxItem) * L.rx.size(),
+                                           hipMemcpyHostToDevice, s));
         TOMOS_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors must outlive the copies
     }
 
@@ -497,6 +695,19 @@ struct Grouped {
     // rfull = A^T q64 (no-ops unless that problem restarts with w != 0),
     // then the restart block of every problem.
     void restart_seq() {
+        if (L.use_rx) {
+            if (L.nrx > 0) {
+                const RxItem* d = reinterpret_cast<const RxItem*>(sc + L.rx_off);
+                const unsigned nb = static_cast<unsigned>(L.nrx);
+                if (o.a_type == TOMOS_F32) k_gadj_rx<0><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
+                else if (o.a_type == TOMOS_FP32_BF16) k_gadj_rx<1><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
+                else k_gadj_rx<2><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
+                TOMOS_LAUNCH_CHECK("k_gadj_rx");
+            }
+            k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.restarts);
+            TOMOS_LAUNCH_CHECK("k_gadj_restart");
+            return;
+        }
         void* gsc = sc + L.gemv_sc;
         for (int i = 0; i < nprob; ++i) {
             const tomos_lsq_problem& p = pr[i];
@@ -547,6 +758,15 @@ struct Grouped {
         k.lds = lds();
         k.tol = o.tol;
         k.scratch = sc;
+        k.nrx = L.nrx;
+        if (kind == 0 && !L.use_rx) {
+            for (int i = 0; i < nprob; ++i) {
+                const tomos_lsq_problem& p = pr[i];
+                for (int64_t v : {p.m, p.n, p.lda, static_cast<int64_t>(reinterpret_cast<uintptr_t>(p.a)),
+                                  static_cast<int64_t>(reinterpret_cast<uintptr_t>(p.wr))})
+                    k.prob.push_back(v);
+            }
+        }
         return k;
     }
     hipGraphExec_t graph(int kind) {
