@@ -15,9 +15,10 @@
 //              then, over many blocks per problem, y = R^-T res (a wave per
 //              column) and z = R^-1 y (64 rows a block), and one block per
 //              problem for rho' = res.z, d = z + (rho'/rho) d;
-//   restart:   per problem, the exact-tier GEMV pair (a no-op launch for a
-//              problem that is not restarting), then one block per problem:
-//              res, |res|/|g|, the stop test, z, rho and d.
+//   restart:   A^T (A w) at the exact tier for every restarting problem
+//              (one grouped launch over row items; a per-problem GEMV pair
+//              when some n > 1024), then one block per problem: res,
+//              |res|/|g|, the stop test; z, rho and d = z as in a step.
 // Each problem carries a small state machine on the device (restart
 // pending -> iterating -> restart pending ... -> done), so the host never
 // looks at a problem: it replays a captured segment of `kSegIters`
@@ -108,30 +109,6 @@ __device__ __forceinline__ double gsum(double v, double* sh) {
     double t = 0.0;
     for (int i = 0; i < kTail / 64; ++i) t += sh[i];  // same order in every thread
     return t;
-}
-
-// z = R^-1 (R^-T res) with y = R^-T res in LDS (n doubles): column j of R^-1
-// (rows 0..j) dotted with res by one wave, then row i of R^-1 (columns
-// i..n-1) dotted with y by one thread.  Returns rho' = res . z.
-__device__ double gadj_precond(const GProb& p, double* y, double* sh) {
-    const int64_t n = p.n;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int64_t j = wave; j < n; j += kTail / 64) {
-        const double* col = p.rinv + j * p.ldr;
-        double a = 0.0;
-        for (int64_t i = lane; i <= j; i += 64) a += col[i] * p.res[i];
-        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
-        if (lane == 0) y[j] = a;
-    }
-    __syncthreads();
-    double rz = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += kTail) {
-        double a = 0.0;
-        for (int64_t j = i; j < n; ++j) a += p.rinv[i + j * p.ldr] * y[j];
-        p.z[i] = a;
-        rz += p.res[i] * a;
-    }
-    return gsum(rz, sh);
 }
 
 // Exact restart products of every restarting problem in one launch.  An
@@ -288,9 +265,7 @@ __global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict_
     const GProb& p = P[blockIdx.x];
     GState* S = p.st;
     if (S->phase != kPhaseRestart) return;
-    extern __shared__ double y[];
     __shared__ double sh[kTail / 64];
-    __shared__ int go;
     const int k = S->restarts_done;  // read before the barriers: thread 0 rewrites it
     const bool first = k == 0;
     double gg = 0.0, rr = 0.0;
@@ -318,16 +293,12 @@ __global__ __launch_bounds__(kTail) void k_gadj_restart(const GProb* __restrict_
         S->skip_restart = 1;
         const double rel = S->gnorm > 0.0 ? S->resnorm / S->gnorm : 0.0;
         const bool stop = S->gnorm == 0.0 || rel < tol || k >= max_restarts;
-        go = stop ? 0 : 1;
         S->phase = stop ? kPhaseDone : kPhaseIterate;
         S->skip_iter = stop ? 1 : 0;
         S->iters = 0;
+        // z = R^-1 R^-T res, rho = res.z and d = z follow in k_gadj_pre1 / _pre2 / _fin
+        S->pc = stop ? 0 : 2;
     }
-    __syncthreads();
-    if (!go) return;
-    const double rz = gadj_precond(p, y, sh);
-    for (int64_t j = threadIdx.x; j < p.n; j += kTail) p.d[j] = p.z[j];
-    if (threadIdx.x == 0) S->rho = rz;
 }
 
 // The n-sized part of one CG step of every iterating problem, after the
@@ -434,8 +405,9 @@ __global__ __launch_bounds__(256) void k_gadj_fin(const GProb* __restrict__ P) {
     double rz = 0.0;
     for (int64_t c = 0; c < nch; ++c) rz += p.rzp[c];
     const double rho = S->rho;
+    const bool restart = S->pc == 2;  // a restart's first direction: d = z
     const double beta = rho != 0.0 ? rz / rho : 0.0;
-    for (int64_t j = threadIdx.x; j < p.n; j += 256) p.d[j] = p.z[j] + beta * p.d[j];
+    for (int64_t j = threadIdx.x; j < p.n; j += 256) p.d[j] = restart ? p.z[j] : p.z[j] + beta * p.d[j];
     __syncthreads();
     if (threadIdx.x == 0) {
         S->rho = rz;
@@ -704,8 +676,9 @@ struct Grouped {
                 else k_gadj_rx<2><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
                 TOMOS_LAUNCH_CHECK("k_gadj_rx");
             }
-            k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.restarts);
+            k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs(), o.tol, o.restarts);
             TOMOS_LAUNCH_CHECK("k_gadj_restart");
+            precond_seq();
             return;
         }
         void* gsc = sc + L.gemv_sc;
@@ -723,21 +696,27 @@ struct Grouped {
             gemv(o.a_type, o.restart_tier, TOMOS_TRANSPOSE, c, p.m, p.n, p.a, p.wr, p.lda, TOMOS_F64, q64[k],
                  TOMOS_F64, rfull[k], gsc, L.gemv_bytes, s);
         }
-        k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, lds(), s>>>(dprobs(), o.tol, o.restarts);
+        k_gadj_restart<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs(), o.tol, o.restarts);
         TOMOS_LAUNCH_CHECK("k_gadj_restart");
+        precond_seq();
+    }
+    // z = R^-1 R^-T res, rho and d for the problems whose pc flag is set
+    void precond_seq() {
+        k_gadj_pre1<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTail / 64))),
+                           static_cast<unsigned>(nprob)),
+                      kTail, 0, s>>>(dprobs());
+        k_gadj_pre2<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kPreRows))),
+                           static_cast<unsigned>(nprob)),
+                      kTail, 0, s>>>(dprobs());
+        k_gadj_fin<<<static_cast<unsigned>(nprob), 256, 0, s>>>(dprobs());
+        TOMOS_LAUNCH_CHECK("k_gadj_pre");
     }
     void iter_seq() {
         for (int it = 0; it < kSegIters; ++it) {
             if (L.nitems) grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s);
             k_gadj_step<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs(), o.tol, o.max_iters,
                                                                       adjoint_inner_drop(o));
-            k_gadj_pre1<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTail / 64))),
-                               static_cast<unsigned>(nprob)),
-                          kTail, 0, s>>>(dprobs());
-            k_gadj_pre2<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kPreRows))),
-                               static_cast<unsigned>(nprob)),
-                          kTail, 0, s>>>(dprobs());
-            k_gadj_fin<<<static_cast<unsigned>(nprob), 256, 0, s>>>(dprobs());
+            precond_seq();
         }
         TOMOS_LAUNCH_CHECK("k_gadj_step");
     }
@@ -799,8 +778,6 @@ struct Grouped {
 
     void run(const double* const* g, double* const* w, int* iterations) {
         upload(g);
-        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_gadj_restart),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds())));
         k_gadj_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
         k_gadj_zero_w<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
         TOMOS_LAUNCH_CHECK("adjoint (grouped) set-up");

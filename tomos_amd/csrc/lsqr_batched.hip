@@ -48,7 +48,8 @@ struct BState {
     double u_coef;    // -alpha / beta: next u_raw = A p + u_coef u_raw
     double one;       // 1.0: u coefficient of the restart's single-pass call
     int done, completed, hist_len, z_fresh;
-    int pad[4];
+    int tail_on;  // the current step's tail runs (set by k_batch_tail_a)
+    int pad[3];
 };
 
 // Per-problem device descriptor (the tail and residual kernels).
@@ -63,6 +64,8 @@ struct BProb {
     double* v;
     double* z;
     double* pv;
+    double* temp;  // A^T u / beta (n)
+    double* src;   // R^-T temp (n)
     BState* st;
     double* hist;
     int hist_cap;
@@ -85,6 +88,7 @@ __global__ void k_bstate_init(BProb* P, int nprob) {
     s->one = 1.0;
     s->done = s->completed = s->hist_len = 0;
     s->z_fresh = 1;
+    s->tail_on = 0;
 }
 
 // |b_i| (FP64, fixed order), x_i = 0
@@ -205,27 +209,29 @@ __device__ __forceinline__ double tail_sum(double v, double* sh) {
     return t;
 }
 
-// The n-sized part of one LSQR step of problem blockIdx.x (lsqr.cpp:557-590
-// after A^T u, and :632-679), with P = R^-1:
-//   beta = |u_raw|, temp = A^T u_raw / beta, src = R^-T temp,
-//   q = src - beta v (iteration) or src (initialize), alpha = |q|, v = q/alpha;
-//   iteration: Givens update, z = P v_old + w_coef_prev z (z = P w), x += (phi/rho) z;
-//   initialize: rho_bar = alpha, phi_bar = beta, true-residual stop test;
-//   u_coef = -alpha/beta, stop test + history, next p = R^-1 v.
-__global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ P, int init, double tol,
-                                                      double maxrr, int confirm) {
+// The n-sized part of one LSQR step (lsqr.cpp:557-590 after A^T u, and
+// :632-679), with P = R^-1, in four launches so that the two products with
+// R^-1 (n^2 / 2 entries each) spread over many blocks per problem instead
+// of one block streaming R^-1 twice:
+//   a (one block per problem): beta = |u_raw|, temp = A^T u_raw / beta;
+//   b (a wave per column):     src = R^-T temp;
+//   c (one block per problem): q = src - beta v (iteration) or src
+//        (initialize), alpha = |q|, v = q/alpha; iteration: Givens update,
+//        z = P v_old + w_coef_prev z (z = P w), x += (phi/rho) z;
+//        initialize: rho_bar = alpha, phi_bar = beta, true-residual stop
+//        test; u_coef = -alpha/beta, stop test + history;
+//   d (64 rows a block):       next p = R^-1 v.
+// tail_on (written by a) gates b, c and d: a problem that has stopped skips
+// the whole tail.
+__global__ __launch_bounds__(kTail) void k_batch_tail_a(const BProb* __restrict__ P) {
     const BProb& p = P[blockIdx.x];
     BState* S = p.st;
-    if (S->done) return;
-    extern __shared__ double lds[];
-    double* temp = lds;          // n
-    double* src = lds + p.n;     // n
     __shared__ double sh[kTail / 64];
-    __shared__ double bc[4];
+    if (S->done) {
+        if (threadIdx.x == 0) S->tail_on = 0;
+        return;
+    }
     const int64_t n = p.n;
-    // state read before any barrier (thread 0 rewrites it at the end)
-    const double rho_bar = S->rho_bar, phi_bar = S->phi_bar, wc_prev = S->w_coef;
-    const bool fresh = S->z_fresh != 0;
     double ss = 0.0;
     for (int k = threadIdx.x; k < p.nitems; k += kTail) ss += p.sumsq[k];
     const double beta = sqrt(tail_sum(ss, sh));
@@ -233,23 +239,43 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
     for (int64_t j = threadIdx.x; j < n; j += kTail) {
         double a = 0.0;
         for (int k = 0; k < p.nitems; ++k) a += static_cast<double>(p.tpart[static_cast<int64_t>(k) * n + j]);
-        temp[j] = a * inv_beta;
+        p.temp[j] = a * inv_beta;
     }
-    __syncthreads();
-    // src = R^-T temp: column j of R^-1 (upper: rows 0..j) dotted with temp,
-    // one wave per column, lanes along the column
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int64_t j = wave; j < n; j += kTail / 64) {
-        const double* col = p.rinv + j * p.ldr;
-        double a = 0.0;
-        for (int64_t i = lane; i <= j; i += 64) a += col[i] * temp[i];
-        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
-        if (lane == 0) src[j] = a;
+    if (threadIdx.x == 0) {
+        S->beta = beta;
+        S->tail_on = 1;
     }
-    __syncthreads();
+}
+
+// src = R^-T temp: column j of R^-1 (upper: rows 0..j) dotted with temp,
+// one wave per column, lanes along the column; grid (ceil(nmax / 16), problems)
+__global__ __launch_bounds__(kTail) void k_batch_tail_b(const BProb* __restrict__ P) {
+    const BProb& p = P[blockIdx.y];
+    if (!p.st->tail_on) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * (kTail / 64) + (threadIdx.x >> 6);
+    if (j >= p.n) return;
+    const double* col = p.rinv + j * p.ldr;
+    double a = 0.0;
+    for (int64_t i = lane; i <= j; i += 64) a += col[i] * p.temp[i];
+    for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o);
+    if (lane == 0) p.src[j] = a;
+}
+
+__global__ __launch_bounds__(kTail) void k_batch_tail_c(const BProb* __restrict__ P, int init, double tol,
+                                                        double maxrr, int confirm) {
+    const BProb& p = P[blockIdx.x];
+    BState* S = p.st;
+    if (!S->tail_on) return;
+    __shared__ double sh[kTail / 64];
+    __shared__ double bc[2];
+    const int64_t n = p.n;
+    // state read before any barrier (thread 0 rewrites it at the end)
+    const double rho_bar = S->rho_bar, phi_bar = S->phi_bar, wc_prev = S->w_coef, beta = S->beta;
+    const bool fresh = S->z_fresh != 0;
     double qq = 0.0;
     for (int64_t j = threadIdx.x; j < n; j += kTail) {
-        const double q = init ? src[j] : src[j] - beta * p.v[j];
+        const double q = init ? p.src[j] : p.src[j] - beta * p.v[j];
         qq += q * q;
     }
     const double alpha = sqrt(tail_sum(qq, sh));
@@ -272,7 +298,7 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
     const double xc = bc[0], wc = bc[1];
     const double inv_alpha = 1.0 / alpha;
     for (int64_t j = threadIdx.x; j < n; j += kTail) {
-        const double q = init ? src[j] : src[j] - beta * p.v[j];
+        const double q = init ? p.src[j] : p.src[j] - beta * p.v[j];
         if (!init) {
             const double pj = p.pv[j];
             const double zj = fresh ? pj : pj + wc_prev * p.z[j];
@@ -281,16 +307,8 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
         }
         p.v[j] = q * inv_alpha;
     }
-    __syncthreads();
-    // next p = R^-1 v (upper: row i uses columns i..n-1)
-    for (int64_t i = threadIdx.x; i < n; i += kTail) {
-        double a = 0.0;
-        for (int64_t j = i; j < n; ++j) a += p.rinv[i + j * p.ldr] * p.v[j];
-        p.pv[i] = a;
-    }
     if (threadIdx.x != 0) return;
     S->alpha = alpha;
-    S->beta = beta;
     S->u_coef = -alpha / beta;
     if (init) {
         S->rho_bar = alpha;
@@ -308,6 +326,36 @@ __global__ __launch_bounds__(kTail) void k_batch_tail(const BProb* __restrict__ 
     if (S->hist_len < p.hist_cap) p.hist[S->hist_len] = rr;
     S->hist_len += 1;
     if (rr < tol || rr < maxrr) S->done = confirm ? 2 : 1;
+}
+
+// next p = R^-1 v (upper: row i uses columns i..n-1), kTailRows rows a
+// block, kTailSlices column slices (slice k takes columns i0 + k,
+// i0 + k + kTailSlices, ...; a wave reads one column's kTailRows entries
+// per step) added in a fixed order; grid (ceil(nmax / 64), problems)
+constexpr int kTailRows = 64;
+constexpr int kTailSlices = kTail / kTailRows;
+__global__ __launch_bounds__(kTail) void k_batch_tail_d(const BProb* __restrict__ P) {
+    const BProb& p = P[blockIdx.y];
+    if (!p.st->tail_on) return;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTailRows;
+    if (i0 >= p.n) return;
+    __shared__ double part[kTailSlices][kTailRows];
+    const int r = threadIdx.x & (kTailRows - 1), sl = threadIdx.x / kTailRows;
+    const int64_t i = i0 + r;
+    double a = 0.0;
+    if (i < p.n) {
+#pragma unroll 4
+        for (int64_t j = i0 + sl; j < p.n; j += kTailSlices)
+            if (j >= i) a += p.rinv[i + j * p.ldr] * p.v[j];
+    }
+    part[sl][r] = a;
+    __syncthreads();
+    if (threadIdx.x < kTailRows && i < p.n) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < kTailSlices; ++k) t += part[k][r];
+        p.pv[i] = t;
+    }
 }
 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
@@ -388,9 +436,9 @@ Plan make_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* 
         L.sumsq_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[i], 1)) * 8));
         L.tpart_at.push_back(take(static_cast<size_t>(std::max(L.prob_items[i], 1)) *
                                   std::max<int64_t>(pr[i].n, 1) * 4));
-        // u (FP32, m) then v, z, pv (FP64, n)
+        // u (FP32, m) then v, z, pv, temp, src (FP64, n)
         L.vec_at.push_back(take(a256(static_cast<size_t>(std::max<int64_t>(pr[i].m, 1)) * 4) +
-                                3 * a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8)));
+                                5 * a256(static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8)));
     }
     L.total = off;
     return L;
@@ -456,6 +504,8 @@ struct Batched {
             q.v = reinterpret_cast<double*>(vb + ub);
             q.z = reinterpret_cast<double*>(vb + ub + nb);
             q.pv = reinterpret_cast<double*>(vb + ub + 2 * nb);
+            q.temp = reinterpret_cast<double*>(vb + ub + 3 * nb);
+            q.src = reinterpret_cast<double*>(vb + ub + 4 * nb);
             q.st = states() + i;
             q.hist = reinterpret_cast<double*>(sc + L.hist_at[i]);
             q.hist_cap = L.hist_cap;
@@ -493,11 +543,16 @@ struct Batched {
         TOMOS_HIP_CHECK(hipStreamSynchronize(s));
     }
 
-    size_t tail_lds() const { return static_cast<size_t>(std::max<int64_t>(L.nmax, 1)) * 16; }
+    int64_t tail_nmax() const { return L.nmax; }
 
     void tail(bool init) {
-        k_batch_tail<<<static_cast<unsigned>(nprob), kTail, tail_lds(), s>>>(dprobs(), init ? 1 : 0, o.tol,
-                                                                             o.max_relres, o.confirm);
+        const unsigned np = static_cast<unsigned>(nprob);
+        k_batch_tail_a<<<np, kTail, 0, s>>>(dprobs());
+        k_batch_tail_b<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTail / 64))), np), kTail, 0,
+                         s>>>(dprobs());
+        k_batch_tail_c<<<np, kTail, 0, s>>>(dprobs(), init ? 1 : 0, o.tol, o.max_relres, o.confirm);
+        k_batch_tail_d<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTailRows))), np), kTail, 0,
+                         s>>>(dprobs());
     }
     void residual() {
         if (!L.nrow_items) return;
@@ -562,7 +617,7 @@ struct Batched {
         k.max_iters = o.max_iters;
         k.restarts = o.restarts;
         k.confirm = o.confirm;
-        k.lds = tail_lds();
+        k.lds = static_cast<size_t>(tail_nmax());
         k.tol = o.tol;
         k.maxrr = o.max_relres;
         k.scratch = sc;
@@ -580,10 +635,6 @@ struct Batched {
 
     void run(bool use_graph, int* iterations, double* final_relres) {
         upload();
-        const size_t lds = tail_lds();
-        TOMOS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_tail),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            static_cast<int>(lds)));
         std::vector<BState> h(nprob);
         if (use_graph && o.restarts > 0) {
             // one graph launch for the whole solve, instantiated once per key
