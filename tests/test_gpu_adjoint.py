@@ -267,15 +267,16 @@ def test_torch_lstsq_batched_vs_dense(capi, workers):
         assert np.linalg.norm(gA - gA_ref) / np.linalg.norm(gA_ref) < 1e-5, k
 
 
-@pytest.mark.parametrize("wide", [False, True])
-def test_adjoint_grouped_restart_paths_and_new_matrices(capi, wide):
+@pytest.mark.parametrize("n2", [100, 700, 1030])
+def test_adjoint_grouped_restart_paths_and_new_matrices(capi, n2):
     """The grouped adjoint's exact restart products come from one grouped
-    launch over row items (n <= 1024) or from a per-problem GEMV pair (a
-    wider problem in the batch).  Both are checked against the dense FP64
-    QR adjoint, on two calls with the same scratch and shapes but different
-    matrices (the second call must not replay launches bound to the first
-    call's matrices)."""
-    shapes = [(6000, 40), (5000, 1030 if wide else 100)]
+    launch over row items: register tiles for n <= 512 (n2 = 100), one row
+    vector a lane for n <= 1024 (n2 = 700), or from a per-problem GEMV pair
+    (a wider problem in the batch, n2 = 1030).  Each is checked against the
+    dense FP64 QR adjoint, on two calls with the same scratch and shapes but
+    different matrices (the second call must not replay launches bound to
+    the first call's matrices)."""
+    shapes = [(6000, 40), (5000, n2)]
     opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-10, 1e-10, 12, 20, 4, 1)
     P = len(shapes)
     scr = None

@@ -257,6 +257,148 @@ __global__ __launch_bounds__(kRxThreads) void k_gadj_rx(const GProb* __restrict_
     }
 }
 
+// The same products with the tile held in registers (packed formats,
+// n <= kRx2MaxN): a tile is 64 rows (8 row groups x 8 rows) by 512 columns
+// (64 column slots x 8 rounds), both planes, 16 vectors a lane, the next
+// tile's vectors brought into LDS by DMA during this tile's arithmetic.
+// Phase 1: a lane's 8 row sums over its 8 columns (FP64), a halving
+// reduction over the wave's 8 column slots (lane bits 5..3) leaves each
+// lane one row, the 8 waves add through LDS in a fixed order -> q for the
+// tile's rows.  Phase 2 reuses the registers: a lane's 8 column sums over
+// its 8 rows with q, a halving reduction over the 8 row groups (lane bits
+// 2..0) leaves lane (g, c) column g * 64 + c, kept in a register across the
+// item's tiles and written once to rpart[item].  Items are row ranges of
+// whole tiles.
+constexpr int kRx2Threads = 512;
+constexpr int64_t kRx2MaxN = 512;
+constexpr int64_t kRx2Rows = 64;  // rows per tile
+
+template <int FMT>  // 1: FP32_BF16, 2: FP32_16
+__global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restrict__ P,
+                                                          const RxItem* __restrict__ items) {
+    const RxItem it = items[blockIdx.x];
+    const GProb& p = P[it.prob];
+    if (p.st->skip_restart) return;
+    __shared__ double red[kRx2Threads / 64][kRx2Rows];
+    __shared__ double qsh[kRx2Rows];
+    __shared__ uint4 stage[kRx2Threads / 64][16][64];  // a wave's next tile: 8 hi, 8 lo vectors a lane
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = tid & 7, c = tid >> 3;  // row group, column slot
+    const int64_t n = p.n, m = p.m;
+    const char* hb = static_cast<const char*>(p.a);
+    double wv[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t j = static_cast<int64_t>(r) * 64 + c;
+        wv[r] = j < n ? p.w[j] : 0.0;
+    }
+    // LDS-DMA of the tile at t0 into this wave's stage slots (each lane's
+    // 16 bytes at slot base + 16 * lane); lanes outside the item or past
+    // column n load the first vector of the planes instead (in bounds) and
+    // are cleared when read back
+    auto issue = [&](int64_t t0) {
+        const int64_t row = t0 + g * 8;
+        const bool ok = row < it.row1;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t j = static_cast<int64_t>(r) * 64 + c;
+            const int64_t e = (ok && j < n) ? row + j * p.lda : 0;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &stage[wave][r][0], 16, 0,
+                                             0);
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &stage[wave][8 + r][0], 16, 0,
+                                             0);
+        }
+    };
+    issue(it.row0);
+    double tacc = 0.0;
+    for (int64_t t0 = it.row0; t0 < it.row1; t0 += kRx2Rows) {
+        uint4 h[8], l[8];
+        const int64_t row = t0 + g * 8;
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA has landed
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            h[r] = stage[wave][r][lane];
+            l[r] = stage[wave][8 + r][lane];
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): slots read before they are refilled
+        if (t0 + kRx2Rows < it.row1) issue(t0 + kRx2Rows);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t j = static_cast<int64_t>(r) * 64 + c;
+            if (!(row < it.row1 && j < n)) {
+                h[r] = make_uint4(0u, 0u, 0u, 0u);
+                l[r] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        const int nv = row < m ? static_cast<int>(min<int64_t>(8, m - row)) : 0;
+        // phase 1: row sums
+        double acc[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            float v[8];
+            rx_decode<FMT>(h[r], l[r], v);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] = fma(static_cast<double>(v[k]), wv[r], acc[k]);
+        }
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            const int off = 32 >> st, half = 4 >> st;
+            const bool upper = (lane & off) != 0;
+#pragma unroll
+            for (int t2 = 0; t2 < 4; ++t2) {
+                if (t2 < half) {
+                    const double send = upper ? acc[t2] : acc[t2 + half];
+                    const double keep = upper ? acc[t2 + half] : acc[t2];
+                    acc[t2] = keep + __shfl_xor(send, off);
+                }
+            }
+        }
+        const int kk = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+        red[wave][g * 8 + kk] = acc[0];
+        __syncthreads();
+        if (tid < kRx2Rows) {
+            double qv = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) qv += red[w2][tid];
+            qsh[tid] = t0 + tid < m ? qv : 0.0;
+        }
+        __syncthreads();
+        // phase 2: column sums with q over this lane's rows
+        double qk[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) qk[k] = qsh[g * 8 + k];
+        double cs[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            float v[8];
+            rx_decode<FMT>(h[r], l[r], v);
+            double sacc = 0.0;  // rows past m (padding of any value) are left out
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sacc = k < nv ? fma(static_cast<double>(v[k]), qk[k], sacc) : sacc;
+            cs[r] = sacc;
+        }
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            const int off = 4 >> st, half = 4 >> st;
+            const bool upper = (lane & off) != 0;
+#pragma unroll
+            for (int t2 = 0; t2 < 4; ++t2) {
+                if (t2 < half) {
+                    const double send = upper ? cs[t2] : cs[t2 + half];
+                    const double keep = upper ? cs[t2 + half] : cs[t2];
+                    cs[t2] = keep + __shfl_xor(send, off);
+                }
+            }
+        }
+        tacc += cs[0];
+        __syncthreads();  // red / qsh reuse by the next tile
+    }
+    const int64_t j = static_cast<int64_t>(g) * 64 + c;  // this lane's column after the row-group reduction
+    if (j < n) p.rpart[static_cast<int64_t>(it.slot) * n + j] = tacc;
+}
+
 // Restart of every problem whose phase is kPhaseRestart: res = g - rfull
 // (rfull = A^T A w, or nothing on the first restart), |g| on the first, the
 // exact-residual stop test (the restart after `max_restarts` earlier ones is
@@ -433,6 +575,7 @@ struct GPlan {
     // restart tier decodes exactly, n <= kRxMaxN and the planes are aligned;
     // otherwise the per-problem GEMV pair
     bool use_rx = false;
+    bool rx2 = false;  // the register-tile form (packed, n <= kRx2MaxN)
     int nrx = 0;
     std::vector<RxItem> rx;
     std::vector<int> prob_rx;
@@ -493,8 +636,10 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
     for (int i = 0; i < nprob && L.use_rx; ++i)
         if (pr[i].n > 0 && o.a_type != TOMOS_F32 && reinterpret_cast<uintptr_t>(pr[i].wr) % 16 != 0) L.use_rx = false;
     L.prob_rx.assign(static_cast<size_t>(nprob), 0);
+    L.rx2 = L.use_rx && o.a_type != TOMOS_F32 && L.nmax <= kRx2MaxN;
     if (L.use_rx) {
-        const int64_t rows = static_cast<int64_t>(kRxThreads) * (o.a_type == TOMOS_F32 ? 4 : 8);
+        // rx2: 4096-row items (64 tiles); rx: one 16-byte row vector a lane
+        const int64_t rows = L.rx2 ? 64 * kRx2Rows : static_cast<int64_t>(kRxThreads) * (o.a_type == TOMOS_F32 ? 4 : 8);
         for (int i = 0; i < nprob; ++i) {
             if (pr[i].n == 0) continue;
             int k = 0;
@@ -671,7 +816,9 @@ struct Grouped {
             if (L.nrx > 0) {
                 const RxItem* d = reinterpret_cast<const RxItem*>(sc + L.rx_off);
                 const unsigned nb = static_cast<unsigned>(L.nrx);
-                if (o.a_type == TOMOS_F32) k_gadj_rx<0><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
+                if (L.rx2 && o.a_type == TOMOS_FP32_BF16) k_gadj_rx2<1><<<nb, kRx2Threads, 0, s>>>(dprobs(), d);
+                else if (L.rx2) k_gadj_rx2<2><<<nb, kRx2Threads, 0, s>>>(dprobs(), d);
+                else if (o.a_type == TOMOS_F32) k_gadj_rx<0><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
                 else if (o.a_type == TOMOS_FP32_BF16) k_gadj_rx<1><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
                 else k_gadj_rx<2><<<nb, kRxThreads, 0, s>>>(dprobs(), d);
                 TOMOS_LAUNCH_CHECK("k_gadj_rx");
