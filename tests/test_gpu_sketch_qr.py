@@ -96,7 +96,7 @@ def test_apply_sampled_exponents(capi, orc, a_type, tier, spike):
 def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
     """tomos_countsketch_apply_batched over uneven problems equals the
     one-problem call bit for bit on the fixed-point path, grouped (4- and
-    8-column groups, several splits, one split) or not (s > 8192): sampled
+    8-column groups, several splits, one split) or not (s > 16384): sampled
     exponents that hold, a spike that sends its problem back to full-row
     exponents, a problem below the sampling threshold, m = 0 (zeros).  An
     lda that is not a multiple of the 16-byte row vector (FP32_BF16/FP32_16:
@@ -105,7 +105,7 @@ def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
     rng = np.random.default_rng(33)
     specs = [(300000, 9, 2000, 0, True), (300000, 9, 2000, 0, False), (50000, 40, 160, 0, False),
              (5000, 20, 80, 0, False), (1001, 7, 28, 3, False), (0, 5, 20, 0, False), (100, 0, 10, 0, False),
-             (20000, 3, 9000, 0, False), (70001, 13, 700, 7, False)]
+             (20000, 3, 17000, 0, False), (70001, 13, 700, 7, False)]
     probs, keep, singles = [], [], []
     for m, n, s_rows, pad, spike in specs:
         lda = max(m + pad, 1)

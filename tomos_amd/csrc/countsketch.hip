@@ -865,10 +865,10 @@ void countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a
 }
 
 // Many sketches in one call.  Every problem that the fixed-point path takes
-// (16-byte aligned rows, s <= 8192) runs in grouped launches: one index
+// (16-byte aligned rows, s <= 16384) runs in grouped launches: one index
 // launch, one exponent launch, one apply launch per column-group width
 // (8, 4, 2, 1 columns, whichever keep a block's int64 buckets within
-// 64 KB of LDS) whose blocks cover every such problem's splits of up to
+// 128 KB of LDS) whose blocks cover every such problem's splits of up to
 // 65536 rows, and one finish launch.  The flags of all problems are read
 // once at the end; the problems they reject (sampled exponents that do not
 // hold, a column max that is not finite) and the ones not grouped go
@@ -877,7 +877,7 @@ void countsketch_apply(int a_type, int tier, int64_t m, int64_t n, const void* a
 namespace {
 
 constexpr int64_t kGroupRowsPerSplit = 65536;
-constexpr int64_t kGroupLdsBytes = 64 * 1024;
+constexpr int64_t kGroupLdsBytes = 128 * 1024;
 
 int group_cg(int64_t s) {
     int cg = 8;
