@@ -153,17 +153,31 @@ __global__ __launch_bounds__(kT) void k_beta_finish(LsqrState* s, const double* 
     }
 }
 
-// temp[j] = inv_beta * sum_s tsum[s][j] (fixed order): A^T u from the
-// single-pass kernel's level-1 sums
-__global__ __launch_bounds__(kT) void k_temp_finish(const LsqrState* __restrict__ S, int64_t n,
-                                                    int splits, const double* __restrict__ tsum,
-                                                    void* temp, bool t64) {
-    if (S->done) return;
+// beta = |u_raw| (as k_beta_finish) and temp[j] = sum_s tsum[s][j] / beta
+// (A^T u from the single-pass kernel's level-1 sums, fixed order) in one
+// launch: every block forms beta from the same partials in the same order,
+// block 0 stores it, and each block scales its columns by its own copy
+__global__ __launch_bounds__(kT) void k_beta_temp_finish(LsqrState* s, const double* __restrict__ red, int nb,
+                                                         int64_t n, int splits, const double* __restrict__ tsum,
+                                                         void* temp, bool t64) {
+    if (s->done) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kT) acc += red[i];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    const double b = sqrt((w[0] + w[1]) + (w[2] + w[3]));
+    const double inv_b = 1.0 / b;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->beta = b;
+        s->inv_beta = inv_b;
+    }
     const int64_t j = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
     if (j >= n) return;
     double a = 0.0;
     for (int sp = 0; sp < splits; ++sp) a += tsum[static_cast<int64_t>(sp) * n + j];
-    st(temp, t64, j, a * S->inv_beta);
+    st(temp, t64, j, a * inv_b);
 }
 
 constexpr int kTail = 1024;
@@ -509,8 +523,7 @@ struct Solver {
             double* tsum = reinterpret_cast<double*>(sc + L.tsum);
             lsqr_restart_launch(o.a_type, m, n, a, wr, lda, static_cast<const double*>(x), x_zero, U(), v64,
                                 red(), tpart, tsum, rplan, &S->done, s);
-            k_beta_finish<<<1, kT, 0, s>>>(S, red(), rplan.nblocks);
-            k_temp_finish<<<grid_for(n), kT, 0, s>>>(S, n, rplan.splits, tsum, Tm(), v64);
+            k_beta_temp_finish<<<grid_for(n), kT, 0, s>>>(S, red(), rplan.nblocks, n, rplan.splits, tsum, Tm(), v64);
             tail(true, true);
             return;
         }
@@ -558,9 +571,8 @@ struct Solver {
             lsqr_fused_launch(o.a_type, m, n, a, lda, xin, x64, U(), v64, &S->u_coef, red(),
                               reinterpret_cast<float*>(sc + L.tpart),
                               reinterpret_cast<double*>(sc + L.tsum), L.fplan, &S->done, s);
-            k_beta_finish<<<1, kT, 0, s>>>(S, red(), L.fplan.nblocks);
-            k_temp_finish<<<grid_for(n), kT, 0, s>>>(S, n, L.fplan.splits,
-                                                    reinterpret_cast<const double*>(sc + L.tsum), Tm(), v64);
+            k_beta_temp_finish<<<grid_for(n), kT, 0, s>>>(S, red(), L.fplan.nblocks, n, L.fplan.splits,
+                                                         reinterpret_cast<const double*>(sc + L.tsum), Tm(), v64);
             tail(false, true);
             return;
         }
