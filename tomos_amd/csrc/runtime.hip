@@ -15,23 +15,24 @@ namespace tomos_hip {
 namespace {
 thread_local std::string g_last_error;
 
+// Per-device state, created on first use.  A fixed table (no reallocation
+// while another host thread holds a reference) and one mutex around every
+// lazy initialisation, so library calls from several host threads are safe.
+constexpr int kMaxDevices = 64;
 struct DeviceCache {
-    int cu = 0;
+    std::atomic<int> cu{0};
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     SideStream side;
     bool side_ready = false;
 };
-std::vector<DeviceCache>& caches() {
-    static std::vector<DeviceCache> c;
-    return c;
-}
+DeviceCache g_caches[kMaxDevices];
+std::mutex g_cache_mu;
 DeviceCache& cache_for_current() {
     int dev = 0;
     TOMOS_HIP_CHECK(hipGetDevice(&dev));
-    auto& c = caches();
-    if (static_cast<int>(c.size()) <= dev) c.resize(dev + 1);
-    return c[dev];
+    require(dev >= 0 && dev < kMaxDevices, "device index beyond the library's table (64 devices)");
+    return g_caches[dev];
 }
 }  // namespace
 
@@ -40,17 +41,20 @@ const char* last_error() { return g_last_error.c_str(); }
 
 int cu_count() {
     DeviceCache& c = cache_for_current();
-    if (c.cu == 0) {
+    int cu = c.cu.load(std::memory_order_relaxed);
+    if (cu == 0) {
         int dev = 0;
         TOMOS_HIP_CHECK(hipGetDevice(&dev));
-        TOMOS_HIP_CHECK(hipDeviceGetAttribute(&c.cu, hipDeviceAttributeMultiprocessorCount, dev));
-        if (c.cu <= 0) c.cu = 256;
+        TOMOS_HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cu <= 0) cu = 256;
+        c.cu.store(cu, std::memory_order_relaxed);  // every thread stores the same value
     }
-    return c.cu;
+    return cu;
 }
 
 const SideStream& side_stream() {
     DeviceCache& c = cache_for_current();
+    std::lock_guard<std::mutex> lk(g_cache_mu);
     if (!c.side_ready) {
         // non-blocking: no implicit ordering with the legacy default stream;
         // default priority
@@ -62,8 +66,12 @@ const SideStream& side_stream() {
     return c.side;
 }
 
+// Calls made with NULL scratch share this one buffer per device (grown as
+// needed, stream-ordered use): host threads that run such calls at the same
+// time race on it and should pass scratch of their own.
 void* fallback_scratch(size_t bytes) {
     DeviceCache& c = cache_for_current();
+    std::lock_guard<std::mutex> lk(g_cache_mu);
     if (bytes == 0) bytes = 256;
     if (c.scratch_bytes < bytes) {
         if (c.scratch) {
