@@ -545,11 +545,13 @@ typedef struct {
 /* Restarted, R-preconditioned LSQR on every problem of the batch, each with
  * the single-problem solver's control flow, scalars and stop test
  * (fused_lsqr_impl, src/core/solver/lsqr.cpp:685-747, per problem), but one
- * step of all problems is three kernels whatever their number and shapes:
- * a grouped exact-residual kernel (restarts), the grouped single-pass
- * A p / A^T u kernel and a one-block-per-problem tail.  With use_graph the
- * launches of one restart (1 + opt->max_iters steps) are captured once as a
- * hipGraph and replayed; a NULL stream then runs on a private stream.
+ * step of all problems is the same few launches whatever their number and
+ * shapes: a grouped exact-residual kernel (restarts), the grouped
+ * single-pass A p / A^T u kernel and a tail (per-problem scalars, R^-1
+ * products spread over many blocks).  With use_graph the whole solve
+ * (opt->restarts restarts of 1 + opt->max_iters steps; problems that have
+ * stopped exit at once) is captured once per launch configuration and
+ * submitted as one graph launch.
  * Options: a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, value_type F32,
  * refine_type F64, precond_type F64 (R^-1 required), n <= 2048, 16-byte
  * aligned planes with lda a multiple of 4 (F32) / 8 (packed).
