@@ -551,7 +551,8 @@ typedef struct {
  * products spread over many blocks).  With use_graph the whole solve
  * (opt->restarts restarts of 1 + opt->max_iters steps; problems that have
  * stopped exit at once) is captured once per launch configuration and
- * submitted as one graph launch.
+ * submitted as one graph launch; a NULL stream then runs on a private
+ * stream ordered after earlier NULL-stream work.
  * Options: a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, value_type F32,
  * refine_type F64, precond_type F64 (R^-1 required), n <= 2048, 16-byte
  * aligned planes with lda a multiple of 4 (F32) / 8 (packed).
