@@ -221,7 +221,7 @@ def batched_leg(args, C, DeviceBuffer, ev0, ev1):
     probs, keep = [], []
     for i in range(P):
         m, n = int(ms[i]), int(ns[i])
-        lda = (m + 7) // 8 * 8
+        lda = (m + 63) // 64 * 64  # 128-byte columns: the tiles' column runs start on cache lines
         d = DeviceBuffer.from_numpy(np.logspace(0, -np.log10(kap[i]), n).astype(np.float32))
         A = DeviceBuffer(lda * n * 4)
         C.call("tomos_fill_normal_f32", A.ptr, m, n, lda, 100 + i, d.ptr, 0, 0, None)
