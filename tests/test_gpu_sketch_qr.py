@@ -91,7 +91,7 @@ def test_apply_sampled_exponents(capi, orc, a_type, tier, spike):
         assert err < 2e-6, (j, err)
 
 
-@pytest.mark.parametrize("a_type,tier", [(C.F32, 0), (C.FP32_BF16, 2), (C.FP32_16, 2)])
+@pytest.mark.parametrize("a_type,tier", [(C.F32, 0), (C.F64, 0), (C.FP32_BF16, 2), (C.FP32_16, 2)])
 @pytest.mark.parametrize("own_scratch", [False, True])
 def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
     """tomos_countsketch_apply_batched over uneven problems equals the
@@ -115,6 +115,8 @@ def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
             A[5000, 3] = 3.0e7  # between the first two sampled segments
         if a_type == C.F32:
             a, wr = H.dev(A), None
+        elif a_type == C.F64:
+            a, wr = H.dev(A.astype(np.float64)), None
         else:
             variant = C.PACK_FP32_BF16 if a_type == C.FP32_BF16 else C.PACK_FP32_16
             a, wr = H.pack(variant, A)
@@ -134,7 +136,7 @@ def test_apply_batched_matches_single(capi, a_type, tier, own_scratch):
         got = keep[3 * k + 2].to_numpy(np.float64, (s_rows, n))
         if n == 0:
             continue
-        if pad and a_type != C.F32:  # float path
+        if pad and a_type not in (C.F32, C.F64):  # float path
             assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max(), (k, m, n)
         else:
             assert np.array_equal(got, ref), (k, m, n)
