@@ -574,6 +574,7 @@ struct GPlan {
     // exact restart products in one grouped launch (k_gadj_rx) when the
     // restart tier decodes exactly, n <= kRxMaxN and the planes are aligned;
     // otherwise the per-problem GEMV pair
+    bool stream = false;  // non-temporal loads in the grouped pass (stream_loads of its bytes)
     bool use_rx = false;
     bool rx2 = false;  // the register-tile form (packed, n <= kRx2MaxN)
     int nrx = 0;
@@ -632,6 +633,9 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
         L.prob_items[static_cast<size_t>(i)] = k;
     }
     L.nitems = static_cast<int>(L.items.size());
+    int64_t pass_bytes = 0;
+    for (int i = 0; i < nprob; ++i) pass_bytes += pr[i].m * pr[i].n * (o.a_type == TOMOS_F32 ? 4 : 2);
+    L.stream = stream_loads(pass_bytes);
     L.use_rx = (o.a_type == TOMOS_F32 || o.restart_tier == TOMOS_TIER_EXACT) && L.nmax <= kRxMaxN;
     for (int i = 0; i < nprob && L.use_rx; ++i)
         if (pr[i].n > 0 && o.a_type != TOMOS_F32 && reinterpret_cast<uintptr_t>(pr[i].wr) % 16 != 0) L.use_rx = false;
@@ -687,13 +691,14 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
 // each problem's matrix, shape and vectors as launch arguments, so its
 // graph also depends on those (`prob`).
 struct SegKey {
-    int device, kind, nprob, nitems, nrx, g, threads, rounds, a_type, restart_tier, max_iters, restarts;
+    int device, kind, nprob, nitems, nrx, stream, g, threads, rounds, a_type, restart_tier, max_iters, restarts;
     size_t lds;
     double tol;
     const void* scratch;
     std::vector<int64_t> prob;
     bool operator==(const SegKey& k) const {
         return device == k.device && kind == k.kind && nprob == k.nprob && nitems == k.nitems && nrx == k.nrx &&
+               stream == k.stream &&
                g == k.g && threads == k.threads && rounds == k.rounds && a_type == k.a_type &&
                restart_tier == k.restart_tier && max_iters == k.max_iters && restarts == k.restarts && lds == k.lds &&
                tol == k.tol && scratch == k.scratch && prob == k.prob;
@@ -860,7 +865,8 @@ struct Grouped {
     }
     void iter_seq() {
         for (int it = 0; it < kSegIters; ++it) {
-            if (L.nitems) grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s);
+            if (L.nitems)
+                grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, dgprobs(), ditems(), L.nitems, s, L.stream);
             k_gadj_step<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs(), o.tol, o.max_iters,
                                                                       adjoint_inner_drop(o));
             precond_seq();
@@ -885,6 +891,7 @@ struct Grouped {
         k.tol = o.tol;
         k.scratch = sc;
         k.nrx = L.nrx;
+        k.stream = L.stream ? 1 : 0;
         if (kind == 0 && !L.use_rx) {
             for (int i = 0; i < nprob; ++i) {
                 const tomos_lsq_problem& p = pr[i];

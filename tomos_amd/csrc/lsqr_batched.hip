@@ -369,6 +369,7 @@ struct Plan {
     std::vector<GroupItem> items;
     std::vector<RowItem> row_items;
     std::vector<int> prob_items;  // items per problem
+    bool stream = false;          // non-temporal loads in the grouped pass (stream_loads of its bytes)
     // scratch layout
     size_t probs, gprobs_iter, gprobs_init, items_off, row_items_off, zeros, states, hist, sumsq,
         tpart, vecs, total;
@@ -417,6 +418,9 @@ Plan make_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* 
     }
     L.nitems = static_cast<int>(L.items.size());
     L.nrow_items = static_cast<int>(L.row_items.size());
+    int64_t pass_bytes = 0;
+    for (int i = 0; i < nprob; ++i) pass_bytes += pr[i].m * pr[i].n * (o.a_type == TOMOS_F32 ? 4 : 2);
+    L.stream = stream_loads(pass_bytes);
     const int64_t cap = static_cast<int64_t>(std::max(o.restarts, 0)) * (std::max(o.max_iters, 0) + 1) + 1;
     L.hist_cap = static_cast<int>(std::min<int64_t>(cap, 1 << 20));
     size_t off = 0;
@@ -565,7 +569,8 @@ struct Batched {
             k_batch_residual<2><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
     }
     void fused(bool init) {
-        grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s);
+        grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s,
+                             L.stream);
     }
     // One restart: check, exact residual, A^T u, tail(init), then max_iters steps.
     void restart_sequence() {
@@ -598,11 +603,12 @@ struct Batched {
         size_t lds;
         double tol, maxrr;
         const void* scratch;
+        bool stream;
         bool operator==(const GraphKey& k) const {
             return device == k.device && nprob == k.nprob && nitems == k.nitems && nrow_items == k.nrow_items &&
                    g == k.g && threads == k.threads && rounds == k.rounds && a_type == k.a_type &&
                    max_iters == k.max_iters && restarts == k.restarts && confirm == k.confirm && lds == k.lds &&
-                   tol == k.tol && maxrr == k.maxrr && scratch == k.scratch;
+                   tol == k.tol && maxrr == k.maxrr && scratch == k.scratch && stream == k.stream;
         }
     };
     GraphKey key() const {
@@ -622,6 +628,7 @@ struct Batched {
         k.tol = o.tol;
         k.maxrr = o.max_relres;
         k.scratch = sc;
+        k.stream = L.stream;
         return k;
     }
 

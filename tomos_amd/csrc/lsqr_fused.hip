@@ -572,7 +572,7 @@ __global__ __launch_bounds__(NT) void lsqr_fused_kernel(
 // stages its tiles through LDS by DMA when G == 8 (as the one-problem
 // kernel does); only it does, because each form's stage buffer is LDS of
 // its own.
-template <int FMT, int G, int R, int RMAX, int NT>
+template <int FMT, int G, int R, int RMAX, int NT, bool NTL>
 __device__ __forceinline__ void grouped_item(const GroupProb& P, const GroupItem& it) {
     if constexpr (R < RMAX) {
         if (P.rounds <= R) {
@@ -581,25 +581,27 @@ __device__ __forceinline__ void grouped_item(const GroupProb& P, const GroupItem
                                              P.tpart + static_cast<int64_t>(it.slot) * P.n);
             return;
         }
-        grouped_item<FMT, G, (2 * R < RMAX ? 2 * R : RMAX), RMAX, NT>(P, it);
+        grouped_item<FMT, G, (2 * R < RMAX ? 2 * R : RMAX), RMAX, NT, NTL>(P, it);
     } else {
-        fused_body<FMT, G, RMAX, NT, false, G == 8, true>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef,
-                                                    it.tile0, it.tile1, P.sumsq + it.slot,
-                                                    P.tpart + static_cast<int64_t>(it.slot) * P.n);
+        fused_body<FMT, G, RMAX, NT, NTL, G == 8, true>(P.m, P.n, P.a, P.lda, P.p, true, P.u, false, *P.u_coef,
+                                                  it.tile0, it.tile1, P.sumsq + it.slot,
+                                                  P.tpart + static_cast<int64_t>(it.slot) * P.n);
     }
 }
 
 // Many problems in one launch: block b runs work item items[b] (a tile
 // range of one problem).  A problem whose skip flag is set is a no-op.
-// Loads are cacheable: a batch of problems small enough to batch is
-// reread from the MALL at the next iteration.
-template <int FMT, int G, int ROUNDS, int NT>
+// NTL: non-temporal loads in the widest (LDS-staged) instance, for a batch
+// far larger than the MALL (C5 with 128-byte columns: uniform wide batches
+// 812 -> 748 us a pass, the C5 solve 30.2 -> 30.0 ms); a batch that fits is
+// left cacheable and is reread from the MALL at the next iteration.
+template <int FMT, int G, int ROUNDS, int NT, bool NTL>
 __global__ __launch_bounds__(NT) void grouped_fused_kernel(const GroupProb* __restrict__ probs,
                                                                  const GroupItem* __restrict__ items) {
     const GroupItem it = items[blockIdx.x];
     const GroupProb& P = probs[it.prob];
     if (P.skip && *P.skip) return;
-    grouped_item<FMT, G, G, ROUNDS, NT>(P, it);
+    grouped_item<FMT, G, G, ROUNDS, NT, NTL>(P, it);
 }
 
 // Exact values of the RPL rows of one 16-byte vector: F32 as stored, packed
@@ -822,20 +824,26 @@ const void* fused_kernel_ptr(int g, int nt, int rounds) {
     return nullptr;
 }
 
-template <int FMT>
+template <int FMT, bool NTL>
 const void* grouped_kernel_ptr(int g, int nt, int rounds) {
     if (g == 32 && nt == 1024) return rounds > 0 ? reinterpret_cast<const void*>(grouped_fused2_kernel<FMT>) : nullptr;
-#define FUSED_CASE(G, NT, R) \
-    if (g == G && nt == NT && rounds == R) return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R, NT>);
+#define FUSED_CASE(G, NT, R)                           \
+    if (g == G && nt == NT && rounds == R)             \
+        return reinterpret_cast<const void*>(grouped_fused_kernel<FMT, G, R, NT, NTL>);
     TOMOS_FUSED_SHAPES(FUSED_CASE)
 #undef FUSED_CASE
     return nullptr;
 }
 
-const void* grouped_kernel_for(int a_type, int g, int nt, int rounds) {
-    if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF>(g, nt, rounds);
-    if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16>(g, nt, rounds);
-    return grouped_kernel_ptr<FMT_F32>(g, nt, rounds);
+const void* grouped_kernel_for(int a_type, int g, int nt, int rounds, bool ntl) {
+    if (ntl) {
+        if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF, true>(g, nt, rounds);
+        if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16, true>(g, nt, rounds);
+        return grouped_kernel_ptr<FMT_F32, true>(g, nt, rounds);
+    }
+    if (a_type == TOMOS_FP32_BF16) return grouped_kernel_ptr<FMT_PBF, false>(g, nt, rounds);
+    if (a_type == TOMOS_FP32_16) return grouped_kernel_ptr<FMT_P16, false>(g, nt, rounds);
+    return grouped_kernel_ptr<FMT_F32, false>(g, nt, rounds);
 }
 
 // Native BF16 storage runs the FP32_BF16 kernels: the array is a hi plane.
@@ -887,10 +895,10 @@ int64_t fused_tile_rows(int a_type, int g) { return static_cast<int64_t>(g) * (a
 int fused_rounds(int g, int threads, int64_t n) { return rounds_for(g, threads, n); }
 
 void grouped_fused_launch(int a_type, int g, int threads, int rounds, const GroupProb* probs,
-                          const GroupItem* items, int nitems, hipStream_t s) {
+                          const GroupItem* items, int nitems, hipStream_t s, bool ntl) {
     require(a_type == TOMOS_F32 || a_type == TOMOS_FP32_BF16 || a_type == TOMOS_FP32_16,
             "grouped_fused: unsupported a_type");
-    const void* k = grouped_kernel_for(a_type, g, threads, rounds);
+    const void* k = grouped_kernel_for(a_type, g, threads, rounds, ntl);
     if (!k) throw Error(kErrInternal, "grouped_fused: no kernel for this shape");
     if (nitems <= 0) return;
     void* args[] = {const_cast<GroupProb**>(&probs), const_cast<GroupItem**>(&items)};
@@ -898,7 +906,7 @@ void grouped_fused_launch(int a_type, int g, int threads, int rounds, const Grou
 }
 
 int grouped_fused_occupancy(int a_type, int g, int threads, int rounds) {
-    const void* k = grouped_kernel_for(a_type, g, threads, rounds);
+    const void* k = grouped_kernel_for(a_type, g, threads, rounds, false);
     int occ = 0;
     if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0) != hipSuccess) occ = 0;
     return std::max(occ, 1);

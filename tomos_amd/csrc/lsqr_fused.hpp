@@ -93,8 +93,10 @@ int64_t fused_tile_rows(int a_type, int g);
 // Column rounds of the kernel for (g, threads, n); 0 when no kernel covers n.
 int fused_rounds(int g, int threads, int64_t n);
 // grid = nitems blocks; rounds must cover the largest n of the problems.
+// ntl: non-temporal loads (a batch far larger than the MALL: stream_loads()
+// of the batch's bytes per pass)
 void grouped_fused_launch(int a_type, int g, int threads, int rounds, const GroupProb* probs,
-                          const GroupItem* items, int nitems, hipStream_t s);
+                          const GroupItem* items, int nitems, hipStream_t s, bool ntl = false);
 // Blocks of the grouped kernel resident per CU (occupancy), >= 1.
 int grouped_fused_occupancy(int a_type, int g, int threads, int rounds);
 
