@@ -152,7 +152,9 @@ __device__ __forceinline__ void residual_decode(const uint4& h, const uint4& l, 
 // columns' loads in flight per lane.  Non-temporal loads: a batch worth
 // grouping is far larger than the MALL, and the residual reads both planes
 // (measured at C5: the forward solve 34.3 -> 33.3 ms against cacheable loads).
-template <int FMT>
+// XZERO (the first restart of a solve, x = 0): A x is the zero vector for
+// finite A, so u = b and A is not read.
+template <int FMT, bool XZERO>
 __global__ __launch_bounds__(kT) void k_batch_residual(const BProb* __restrict__ P,
                                                        const RowItem* __restrict__ items) {
     constexpr int RPL = FMT == 0 ? 4 : 8;  // FP32 / packed rows per 16-byte vector
@@ -164,6 +166,10 @@ __global__ __launch_bounds__(kT) void k_batch_residual(const BProb* __restrict__
     const int64_t r = it.row0 + static_cast<int64_t>(threadIdx.x) * RPL;
     if (r >= it.row1) return;
     const int nv = static_cast<int>(min<int64_t>(RPL, p.m - r));
+    if constexpr (XZERO) {
+        for (int k = 0; k < nv; ++k) p.u[r + k] = static_cast<float>(p.b[r + k]);
+        return;
+    }
     const char* hb = static_cast<const char*>(p.a) + r * ESZ;
     const uint16_t* lb = FMT != 0 ? p.wr + r : nullptr;
     double acc[RPL];
@@ -559,24 +565,29 @@ struct Batched {
         k_batch_tail_d<<<dim3(static_cast<unsigned>(ceil_div(L.nmax, static_cast<int64_t>(kTailRows))), np), kTail, 0,
                          s>>>(dprobs());
     }
-    void residual() {
+    void residual(bool xzero) {
         if (!L.nrow_items) return;
-        if (o.a_type == TOMOS_F32)
-            k_batch_residual<0><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
-        else if (o.a_type == TOMOS_FP32_BF16)
-            k_batch_residual<1><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
-        else
-            k_batch_residual<2><<<static_cast<unsigned>(L.nrow_items), kT, 0, s>>>(dprobs(), drows());
+        const unsigned nb = static_cast<unsigned>(L.nrow_items);
+        if (xzero) {
+            if (o.a_type == TOMOS_F32) k_batch_residual<0, true><<<nb, kT, 0, s>>>(dprobs(), drows());
+            else if (o.a_type == TOMOS_FP32_BF16) k_batch_residual<1, true><<<nb, kT, 0, s>>>(dprobs(), drows());
+            else k_batch_residual<2, true><<<nb, kT, 0, s>>>(dprobs(), drows());
+            return;
+        }
+        if (o.a_type == TOMOS_F32) k_batch_residual<0, false><<<nb, kT, 0, s>>>(dprobs(), drows());
+        else if (o.a_type == TOMOS_FP32_BF16) k_batch_residual<1, false><<<nb, kT, 0, s>>>(dprobs(), drows());
+        else k_batch_residual<2, false><<<nb, kT, 0, s>>>(dprobs(), drows());
     }
     void fused(bool init) {
         grouped_fused_launch(o.a_type, L.g, L.threads, L.rounds, init ? ginit() : giter(), ditems(), L.nitems, s,
                              L.stream);
     }
-    // One restart: check, exact residual, A^T u, tail(init), then max_iters steps.
-    void restart_sequence() {
+    // One restart: check, exact residual, A^T u, tail(init), then max_iters
+    // steps.  The first restart of a solve starts from x = 0.
+    void restart_sequence(bool first) {
         k_batch_restart_check<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob, o.tol,
                                                                                       o.max_relres, o.confirm);
-        residual();
+        residual(first);
         fused(true);
         tail(true);
         for (int it = 0; it < o.max_iters; ++it) {
@@ -638,7 +649,7 @@ struct Batched {
     void solve_sequence() {
         k_bstate_init<<<static_cast<unsigned>(ceil_div(nprob, 64)), 64, 0, s>>>(dprobs(), nprob);
         k_batch_setup<<<static_cast<unsigned>(nprob), kTail, 0, s>>>(dprobs());
-        for (int r = 0; r < o.restarts; ++r) restart_sequence();
+        for (int r = 0; r < o.restarts; ++r) restart_sequence(r == 0);
     }
 
     void run(bool use_graph, int* iterations, double* final_relres) {
@@ -674,7 +685,7 @@ struct Batched {
             TOMOS_LAUNCH_CHECK("lsqr_batched setup");
             for (int r = 0; r < o.restarts; ++r) {
                 if (all_done(h)) break;
-                restart_sequence();
+                restart_sequence(r == 0);
                 TOMOS_LAUNCH_CHECK("lsqr_batched restart");
             }
             all_done(h);
