@@ -267,10 +267,12 @@ def test_torch_lstsq_batched_vs_dense(capi, workers):
         assert np.linalg.norm(gA - gA_ref) / np.linalg.norm(gA_ref) < 1e-5, k
 
 
-@pytest.mark.parametrize("n2", [100, 700, 1030])
+@pytest.mark.parametrize("n2", [100, 200, 400, 700, 1030])
 def test_adjoint_grouped_restart_paths_and_new_matrices(capi, n2):
     """The grouped adjoint's exact restart products come from one grouped
-    launch over row items: register tiles for n <= 512 (n2 = 100), one row
+    launch over row items: register tiles for n <= 512 (n2 = 100, 200, 400:
+    one, two and four 128-column rounds; the other problem, n = 40, takes
+    one round in the same launch), one row
     vector a lane for n <= 1024 (n2 = 700), or from a per-problem GEMV pair
     (a wider problem in the batch, n2 = 1030).  Each is checked against the
     dense FP64 QR adjoint, on two calls with the same scratch and shapes but

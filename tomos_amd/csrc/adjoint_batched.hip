@@ -34,6 +34,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "adjoint.hpp"
@@ -259,7 +260,8 @@ __global__ __launch_bounds__(kRxThreads) void k_gadj_rx(const GProb* __restrict_
 
 // The same products with the tile held in registers (packed formats,
 // n <= kRx2MaxN): a tile is 32 rows (4 row groups x 8 rows) by 512 columns
-// (128 column slots x 4 rounds), both planes, 8 vectors a lane.  Tiles
+// (128 column slots x 4 rounds), both planes, 8 vectors a lane; a problem
+// with n <= 128 (n <= 256) uses only the first round (two rounds).  Tiles
 // come through LDS by DMA, double-buffered: while a tile is multiplied the
 // next one has landed or is landing and the one after is requested, so
 // two tiles (128 KB) are in flight per CU.  Phase 1: a lane's 8 row sums
@@ -335,117 +337,127 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
         const int64_t j = static_cast<int64_t>(r) * 128 + c;
         wv[r] = j < n ? p.w[j] : 0.0;
     }
-    // LDS-DMA of the tile at t0 into this wave's slots of st (each lane's
-    // 16 bytes at slot base + 16 * lane); lanes outside the item or past
-    // column n load the first vector of the planes instead (in bounds) and
-    // are cleared when read back
-    auto issue = [&](int64_t t0, uint4 (*st)[8][64]) {
-        const int64_t row = t0 + g * 8;
-        const bool ok = row < it.row1;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t j = static_cast<int64_t>(r) * 128 + c;
-            const int64_t e = (ok && j < n) ? row + j * p.lda : 0;
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &st[wave][r][0], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &st[wave][4 + r][0], 16, 0,
-                                             0);
-        }
-    };
     double tacc = 0.0;
-    // one tile from st: wait for its DMA (the next tile's 8 loads a lane may
-    // still be in flight: vmcnt(8); none after the last tile: vmcnt(0)),
-    // read it (the reads have completed when st is refilled), refill st
-    // with the tile after next, multiply
-    auto tile = [&](int64_t t0, uint4 (*st)[8][64]) {
-        uint4 h[4], l[4];
-        const int64_t row = t0 + g * 8;
-        // LDS offset of this lane's first slot: the low 32 bits of its flat address
-        const uint32_t sa = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&st[wave][0][lane]));
-        if (t0 + kRx2Rows < it.row1) rx2_read_stage<8>(sa, h, l);
-        else rx2_read_stage<0>(sa, h, l);
-        if (t0 + 2 * kRx2Rows < it.row1) issue(t0 + 2 * kRx2Rows, st);
+    // the item's tiles with R column rounds, the fewest of 1, 2, 4 that
+    // cover n: a narrow problem neither loads nor multiplies the columns
+    // past its last round (slots of rounds >= R are left unread)
+    auto run = [&](auto rounds) {
+        constexpr int R = decltype(rounds)::value;
+        // LDS-DMA of the tile at t0 into this wave's slots of st (each lane's
+        // 16 bytes at slot base + 16 * lane); lanes outside the item or past
+        // column n load the first vector of the planes instead (in bounds) and
+        // are cleared when read back
+        auto issue = [&](int64_t t0, uint4 (*st)[8][64]) {
+            const int64_t row = t0 + g * 8;
+            const bool ok = row < it.row1;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t j = static_cast<int64_t>(r) * 128 + c;
-            if (!(row < it.row1 && j < n)) {
-                h[r] = make_uint4(0u, 0u, 0u, 0u);
-                l[r] = make_uint4(0u, 0u, 0u, 0u);
+            for (int r = 0; r < R; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * 128 + c;
+                const int64_t e = (ok && j < n) ? row + j * p.lda : 0;
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &st[wave][r][0], 16, 0,
+                                                 0);
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &st[wave][4 + r][0], 16, 0,
+                                                 0);
             }
-        }
-        const int nv = row < m ? static_cast<int>(min<int64_t>(8, m - row)) : 0;
-        // phase 1: row sums
-        double acc[8];
+        };
+        // one tile from st: wait for its DMA (the next tile's 2R loads a lane
+        // may still be in flight: vmcnt(2R); none after the last tile: vmcnt(0)),
+        // read it (the reads have completed when st is refilled), refill st
+        // with the tile after next, multiply
+        auto tile = [&](int64_t t0, uint4 (*st)[8][64]) {
+            uint4 h[4], l[4];
+            const int64_t row = t0 + g * 8;
+            // LDS offset of this lane's first slot: the low 32 bits of its flat address
+            const uint32_t sa = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&st[wave][0][lane]));
+            if (t0 + kRx2Rows < it.row1) rx2_read_stage<2 * R>(sa, h, l);
+            else rx2_read_stage<0>(sa, h, l);
+            if (t0 + 2 * kRx2Rows < it.row1) issue(t0 + 2 * kRx2Rows, st);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] = 0.0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            float v[8];
-            rx_decode<FMT>(h[r], l[r], v);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) acc[k] = fma(static_cast<double>(v[k]), wv[r], acc[k]);
-        }
-#pragma unroll
-        for (int st2 = 0; st2 < 3; ++st2) {
-            const int off = 32 >> st2, half = 4 >> st2;
-            const bool upper = (lane & off) != 0;
-#pragma unroll
-            for (int t2 = 0; t2 < 4; ++t2) {
-                if (t2 < half) {
-                    const double send = upper ? acc[t2] : acc[t2 + half];
-                    const double keep = upper ? acc[t2 + half] : acc[t2];
-                    acc[t2] = keep + __shfl_xor(send, off);
+            for (int r = 0; r < R; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * 128 + c;
+                if (!(row < it.row1 && j < n)) {
+                    h[r] = make_uint4(0u, 0u, 0u, 0u);
+                    l[r] = make_uint4(0u, 0u, 0u, 0u);
                 }
             }
-        }
-        acc[0] += __shfl_xor(acc[0], 4);
-        const int kk = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-        if ((lane & 4) == 0) red[wave][g * 8 + kk] = acc[0];
-        lds_exchange_barrier();
-        if (tid < kRx2Rows) {
-            double qv = 0.0;
+            const int nv = row < m ? static_cast<int>(min<int64_t>(8, m - row)) : 0;
+            // phase 1: row sums
+            double acc[8];
 #pragma unroll
-            for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) qv += red[w2][tid];
-            qsh[tid] = t0 + tid < m ? qv : 0.0;
-        }
-        lds_exchange_barrier();
-        // phase 2: column sums with q over this lane's rows
-        double qk[8];
+            for (int k = 0; k < 8; ++k) acc[k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) qk[k] = qsh[g * 8 + k];
-        double cs[4];
+            for (int r = 0; r < R; ++r) {
+                float v[8];
+                rx_decode<FMT>(h[r], l[r], v);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            float v[8];
-            rx_decode<FMT>(h[r], l[r], v);
-            double sacc = 0.0;  // rows past m (padding of any value) are left out
+                for (int k = 0; k < 8; ++k) acc[k] = fma(static_cast<double>(v[k]), wv[r], acc[k]);
+            }
 #pragma unroll
-            for (int k = 0; k < 8; ++k) sacc = k < nv ? fma(static_cast<double>(v[k]), qk[k], sacc) : sacc;
-            cs[r] = sacc;
-        }
+            for (int st2 = 0; st2 < 3; ++st2) {
+                const int off = 32 >> st2, half = 4 >> st2;
+                const bool upper = (lane & off) != 0;
 #pragma unroll
-        for (int st2 = 0; st2 < 2; ++st2) {
-            const int off = 2 >> st2, half = 2 >> st2;
-            const bool upper = (lane & off) != 0;
-#pragma unroll
-            for (int t2 = 0; t2 < 2; ++t2) {
-                if (t2 < half) {
-                    const double send = upper ? cs[t2] : cs[t2 + half];
-                    const double keep = upper ? cs[t2 + half] : cs[t2];
-                    cs[t2] = keep + __shfl_xor(send, off);
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    if (t2 < half) {
+                        const double send = upper ? acc[t2] : acc[t2 + half];
+                        const double keep = upper ? acc[t2 + half] : acc[t2];
+                        acc[t2] = keep + __shfl_xor(send, off);
+                    }
                 }
             }
+            acc[0] += __shfl_xor(acc[0], 4);
+            const int kk = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+            if ((lane & 4) == 0) red[wave][g * 8 + kk] = acc[0];
+            lds_exchange_barrier();
+            if (tid < kRx2Rows) {
+                double qv = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) qv += red[w2][tid];
+                qsh[tid] = t0 + tid < m ? qv : 0.0;
+            }
+            lds_exchange_barrier();
+            // phase 2: column sums with q over this lane's rows
+            double qk[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) qk[k] = qsh[g * 8 + k];
+            double cs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float v[8];
+                rx_decode<FMT>(h[r], l[r], v);
+                double sacc = 0.0;  // rows past m (padding of any value) are left out
+#pragma unroll
+                for (int k = 0; k < 8; ++k) sacc = k < nv ? fma(static_cast<double>(v[k]), qk[k], sacc) : sacc;
+                cs[r] = sacc;
+            }
+#pragma unroll
+            for (int st2 = 0; st2 < 2; ++st2) {
+                const int off = 2 >> st2, half = 2 >> st2;
+                const bool upper = (lane & off) != 0;
+#pragma unroll
+                for (int t2 = 0; t2 < 2; ++t2) {
+                    if (t2 < half) {
+                        const double send = upper ? cs[t2] : cs[t2 + half];
+                        const double keep = upper ? cs[t2 + half] : cs[t2];
+                        cs[t2] = keep + __shfl_xor(send, off);
+                    }
+                }
+            }
+            tacc += cs[0];
+            lds_exchange_barrier();  // red / qsh reuse by the next tile
+        };
+        // tiles in pairs, the second of an odd count wholly outside the item
+        // (its lanes load in bounds, are cleared and add zero)
+        issue(it.row0, stage0);
+        issue(it.row0 + kRx2Rows, stage1);
+        for (int64_t t0 = it.row0; t0 < it.row1; t0 += 2 * kRx2Rows) {
+            tile(t0, stage0);
+            tile(t0 + kRx2Rows, stage1);
         }
-        tacc += cs[0];
-        lds_exchange_barrier();  // red / qsh reuse by the next tile
     };
-    // tiles in pairs, the second of an odd count wholly outside the item
-    // (its lanes load in bounds, are cleared and add zero)
-    issue(it.row0, stage0);
-    issue(it.row0 + kRx2Rows, stage1);
-    for (int64_t t0 = it.row0; t0 < it.row1; t0 += 2 * kRx2Rows) {
-        tile(t0, stage0);
-        tile(t0 + kRx2Rows, stage1);
-    }
+    if (n <= 128) run(std::integral_constant<int, 1>{});
+    else if (n <= 256) run(std::integral_constant<int, 2>{});
+    else run(std::integral_constant<int, 4>{});
     const int64_t j = static_cast<int64_t>(g) * 128 + c;  // this lane's column after the row-group reduction
     if (j < n) p.rpart[static_cast<int64_t>(it.slot) * n + j] = tacc;
 }
