@@ -259,23 +259,22 @@ __global__ __launch_bounds__(kRxThreads) void k_gadj_rx(const GProb* __restrict_
 }
 
 // The same products with the tile held in registers (packed formats,
-// n <= kRx2MaxN): a tile is 32 rows (4 row groups x 8 rows) by 512 columns
-// (128 column slots x 4 rounds), both planes, 8 vectors a lane; a problem
-// with n <= 128 (n <= 256) uses only the first round (two rounds).  Tiles
-// come through LDS by DMA, double-buffered: while a tile is multiplied the
-// next one has landed or is landing and the one after is requested, so
-// two tiles (128 KB) are in flight per CU.  Phase 1: a lane's 8 row sums
-// over its 4 columns (FP64); a halving reduction over lane bits 5..3 and a
-// plain one over bit 2 (the wave's 16 column slots) leave each lane one
-// row; the 8 waves add through LDS in a fixed order -> q for the tile's
-// rows.  Phase 2 reuses the registers: a lane's 4 column sums over its 8
-// rows with q, a halving reduction over the 4 row groups (lane bits 1..0)
-// leaves lane (g, c) column g * 128 + c, kept in a register across the
-// item's tiles and written once to rpart[item].  Items are row ranges of
-// whole tiles.
+// n <= kRx2MaxN): a tile is 64 rows (8 row groups x 8 rows) by 512 columns
+// (64 column slots x 8 rounds), both planes, 16 vectors a lane; a problem
+// uses only the first R rounds, the fewest of 1, 2, 4, 8 that cover its n.
+// The tile comes through LDS by DMA, and the next tile's DMA is issued as
+// soon as this one is in registers, so it stays in flight through this
+// tile's arithmetic and LDS-only barriers.  Phase 1: a lane's 8 row sums
+// over its columns (FP64), a halving reduction over the wave's 8 column
+// slots (lane bits 5..3) leaves each lane one row, the 8 waves add through
+// LDS in a fixed order -> q for the tile's rows.  Phase 2 reuses the
+// registers: a lane's column sums over its 8 rows with q, a halving
+// reduction over the 8 row groups (lane bits 2..0) leaves lane (g, c)
+// column g * 64 + c, kept in a register across the item's tiles and
+// written once to rpart[item].  Items are row ranges of whole tiles.
 constexpr int kRx2Threads = 512;
 constexpr int64_t kRx2MaxN = 512;
-constexpr int64_t kRx2Rows = 32;  // rows per tile
+constexpr int64_t kRx2Rows = 64;  // rows per tile
 
 // Workgroup barrier for LDS exchanges only: waits for this wave's LDS
 // reads and writes (lgkmcnt) but not for its outstanding LDS-DMA loads,
@@ -283,98 +282,63 @@ constexpr int64_t kRx2Rows = 32;  // rows per tile
 // "memory" clobber keeps the compiler from moving memory accesses across.
 __device__ __forceinline__ void lds_exchange_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Reads a lane's 8 staged vectors (4 hi-plane rounds, then 4 lo-plane
-// rounds, 1024 bytes apart from LDS address addr) once at most VM vector
-// memory operations of the wave are outstanding; the reads complete inside
-// the block.  The wait is explicit because the compiler, left to order
-// these reads after the LDS-DMA loads itself, drains every load in flight,
-// including the next tile's.
-template <int VM>
-__device__ __forceinline__ void rx2_read_stage(uint32_t addr, uint4 (&h)[4], uint4 (&l)[4]) {
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    v4u h0, h1, h2, h3, l0, l1, l2, l3;
-    asm volatile(
-        "s_waitcnt vmcnt(%8)\n\t"
-        "ds_read_b128 %0, %9\n\t"
-        "ds_read_b128 %1, %9 offset:1024\n\t"
-        "ds_read_b128 %2, %9 offset:2048\n\t"
-        "ds_read_b128 %3, %9 offset:3072\n\t"
-        "ds_read_b128 %4, %9 offset:4096\n\t"
-        "ds_read_b128 %5, %9 offset:5120\n\t"
-        "ds_read_b128 %6, %9 offset:6144\n\t"
-        "ds_read_b128 %7, %9 offset:7168\n\t"
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
-        : "n"(VM), "v"(addr)
-        : "memory");
-    const v4u hv[4] = {h0, h1, h2, h3}, lv[4] = {l0, l1, l2, l3};
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        h[r] = make_uint4(hv[r].x, hv[r].y, hv[r].z, hv[r].w);
-        l[r] = make_uint4(lv[r].x, lv[r].y, lv[r].z, lv[r].w);
-    }
-}
-
 template <int FMT>  // 1: FP32_BF16, 2: FP32_16
 __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restrict__ P,
                                                           const RxItem* __restrict__ items) {
     const RxItem it = items[blockIdx.x];
     const GProb& p = P[it.prob];
     if (p.st->skip_restart) return;
-    // two stage buffers as separate arrays, each accessed at fixed
-    // addresses, so that reading one waits only for the DMA into it
-    __shared__ double red[kRx2Threads / 64][kRx2Rows];
-    __shared__ double qsh[kRx2Rows];
-    __shared__ uint4 stage0[kRx2Threads / 64][8][64];  // [wave][4 hi, 4 lo rounds][lane]
-    __shared__ uint4 stage1[kRx2Threads / 64][8][64];
+    // row partials and q in two copies, by tile parity: a wave may write
+    // the next tile's partials while slower waves still read this tile's q
+    __shared__ double red[2][kRx2Threads / 64][kRx2Rows];
+    __shared__ double qsh[2][kRx2Rows];
+    __shared__ uint4 stage[kRx2Threads / 64][16][64];  // a wave's tile: 8 hi, 8 lo vectors a lane
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int g = tid & 3, c = tid >> 2;  // row group, column slot
+    const int g = tid & 7, c = tid >> 3;  // row group, column slot
     const int64_t n = p.n, m = p.m;
     const char* hb = static_cast<const char*>(p.a);
-    double wv[4];
+    double wv[8];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int64_t j = static_cast<int64_t>(r) * 128 + c;
+    for (int r = 0; r < 8; ++r) {
+        const int64_t j = static_cast<int64_t>(r) * 64 + c;
         wv[r] = j < n ? p.w[j] : 0.0;
     }
     double tacc = 0.0;
-    // the item's tiles with R column rounds, the fewest of 1, 2, 4 that
-    // cover n: a narrow problem neither loads nor multiplies the columns
-    // past its last round (slots of rounds >= R are left unread)
     auto run = [&](auto rounds) {
         constexpr int R = decltype(rounds)::value;
-        // LDS-DMA of the tile at t0 into this wave's slots of st (each lane's
-        // 16 bytes at slot base + 16 * lane); lanes outside the item or past
-        // column n load the first vector of the planes instead (in bounds) and
-        // are cleared when read back
-        auto issue = [&](int64_t t0, uint4 (*st)[8][64]) {
+        // LDS-DMA of the tile at t0 into this wave's stage slots (each
+        // lane's 16 bytes at slot base + 16 * lane); lanes outside the item
+        // or past column n load the first vector of the planes instead (in
+        // bounds) and are cleared when read back
+        auto issue = [&](int64_t t0) {
             const int64_t row = t0 + g * 8;
             const bool ok = row < it.row1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int64_t j = static_cast<int64_t>(r) * 128 + c;
+                const int64_t j = static_cast<int64_t>(r) * 64 + c;
                 const int64_t e = (ok && j < n) ? row + j * p.lda : 0;
-                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &st[wave][r][0], 16, 0,
-                                                 0);
-                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &st[wave][4 + r][0], 16, 0,
-                                                 0);
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &stage[wave][r][0], 16,
+                                                 0, 0);
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &stage[wave][8 + r][0],
+                                                 16, 0, 0);
             }
         };
-        // one tile from st: wait for its DMA (the next tile's 2R loads a lane
-        // may still be in flight: vmcnt(2R); none after the last tile: vmcnt(0)),
-        // read it (the reads have completed when st is refilled), refill st
-        // with the tile after next, multiply
-        auto tile = [&](int64_t t0, uint4 (*st)[8][64]) {
-            uint4 h[4], l[4];
+        issue(it.row0);
+        int par = 0;
+        for (int64_t t0 = it.row0; t0 < it.row1; t0 += kRx2Rows, par ^= 1) {
+            uint4 h[R], l[R];
             const int64_t row = t0 + g * 8;
-            // LDS offset of this lane's first slot: the low 32 bits of its flat address
-            const uint32_t sa = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&st[wave][0][lane]));
-            if (t0 + kRx2Rows < it.row1) rx2_read_stage<2 * R>(sa, h, l);
-            else rx2_read_stage<0>(sa, h, l);
-            if (t0 + 2 * kRx2Rows < it.row1) issue(t0 + 2 * kRx2Rows, st);
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA has landed
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int64_t j = static_cast<int64_t>(r) * 128 + c;
+                h[r] = stage[wave][r][lane];
+                l[r] = stage[wave][8 + r][lane];
+            }
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): slots read before they are refilled
+            if (t0 + kRx2Rows < it.row1) issue(t0 + kRx2Rows);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * 64 + c;
                 if (!(row < it.row1 && j < n)) {
                     h[r] = make_uint4(0u, 0u, 0u, 0u);
                     l[r] = make_uint4(0u, 0u, 0u, 0u);
@@ -393,8 +357,8 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
                 for (int k = 0; k < 8; ++k) acc[k] = fma(static_cast<double>(v[k]), wv[r], acc[k]);
             }
 #pragma unroll
-            for (int st2 = 0; st2 < 3; ++st2) {
-                const int off = 32 >> st2, half = 4 >> st2;
+            for (int st = 0; st < 3; ++st) {
+                const int off = 32 >> st, half = 4 >> st;
                 const bool upper = (lane & off) != 0;
 #pragma unroll
                 for (int t2 = 0; t2 < 4; ++t2) {
@@ -405,22 +369,21 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
                     }
                 }
             }
-            acc[0] += __shfl_xor(acc[0], 4);
             const int kk = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-            if ((lane & 4) == 0) red[wave][g * 8 + kk] = acc[0];
+            red[par][wave][g * 8 + kk] = acc[0];
             lds_exchange_barrier();
             if (tid < kRx2Rows) {
                 double qv = 0.0;
 #pragma unroll
-                for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) qv += red[w2][tid];
-                qsh[tid] = t0 + tid < m ? qv : 0.0;
+                for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) qv += red[par][w2][tid];
+                qsh[par][tid] = t0 + tid < m ? qv : 0.0;
             }
             lds_exchange_barrier();
             // phase 2: column sums with q over this lane's rows
             double qk[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) qk[k] = qsh[g * 8 + k];
-            double cs[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 8; ++k) qk[k] = qsh[par][g * 8 + k];
+            double cs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 float v[8];
@@ -431,11 +394,11 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
                 cs[r] = sacc;
             }
 #pragma unroll
-            for (int st2 = 0; st2 < 2; ++st2) {
-                const int off = 2 >> st2, half = 2 >> st2;
+            for (int st = 0; st < 3; ++st) {
+                const int off = 4 >> st, half = 4 >> st;
                 const bool upper = (lane & off) != 0;
 #pragma unroll
-                for (int t2 = 0; t2 < 2; ++t2) {
+                for (int t2 = 0; t2 < 4; ++t2) {
                     if (t2 < half) {
                         const double send = upper ? cs[t2] : cs[t2 + half];
                         const double keep = upper ? cs[t2 + half] : cs[t2];
@@ -444,21 +407,16 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
                 }
             }
             tacc += cs[0];
-            lds_exchange_barrier();  // red / qsh reuse by the next tile
-        };
-        // tiles in pairs, the second of an odd count wholly outside the item
-        // (its lanes load in bounds, are cleared and add zero)
-        issue(it.row0, stage0);
-        issue(it.row0 + kRx2Rows, stage1);
-        for (int64_t t0 = it.row0; t0 < it.row1; t0 += 2 * kRx2Rows) {
-            tile(t0, stage0);
-            tile(t0 + kRx2Rows, stage1);
+            // no barrier here: the next tile writes the other copy, and this
+            // copy is rewritten only after every wave has passed the next
+            // tile's first barrier, that is, finished reading it
         }
     };
-    if (n <= 128) run(std::integral_constant<int, 1>{});
-    else if (n <= 256) run(std::integral_constant<int, 2>{});
-    else run(std::integral_constant<int, 4>{});
-    const int64_t j = static_cast<int64_t>(g) * 128 + c;  // this lane's column after the row-group reduction
+    if (n <= 64) run(std::integral_constant<int, 1>{});
+    else if (n <= 128) run(std::integral_constant<int, 2>{});
+    else if (n <= 256) run(std::integral_constant<int, 4>{});
+    else run(std::integral_constant<int, 8>{});
+    const int64_t j = static_cast<int64_t>(g) * 64 + c;  // this lane's column after the row-group reduction
     if (j < n) p.rpart[static_cast<int64_t>(it.slot) * n + j] = tacc;
 }
 
@@ -705,8 +663,8 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
     L.prob_rx.assign(static_cast<size_t>(nprob), 0);
     L.rx2 = L.use_rx && o.a_type != TOMOS_F32 && L.nmax <= kRx2MaxN;
     if (L.use_rx) {
-        // rx2: 1024-row items (32 tiles); rx: one 16-byte row vector a lane
-        const int64_t rows = L.rx2 ? 32 * kRx2Rows : static_cast<int64_t>(kRxThreads) * (o.a_type == TOMOS_F32 ? 4 : 8);
+        // rx2: 1024-row items (16 tiles); rx: one 16-byte row vector a lane
+        const int64_t rows = L.rx2 ? 16 * kRx2Rows : static_cast<int64_t>(kRxThreads) * (o.a_type == TOMOS_F32 ? 4 : 8);
         for (int i = 0; i < nprob; ++i) {
             if (pr[i].n == 0) continue;
             int k = 0;
