@@ -223,8 +223,11 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) tacc[i] = 0.0f;
 
-    __shared__ float red[NW][TR];
-    __shared__ float ush[TR];
+    // row partials and u rows in two copies, by tile parity: a wave may
+    // write the next tile's partials while slower waves still read this
+    // tile's u rows
+    __shared__ float red[2][NW][TR];
+    __shared__ float ush[2][TR];
     __shared__ uint4 stage[STAGE ? NW : 1][STAGE ? ROUNDS : 1][64];
     double sq = 0.0;
     // LDS-DMA of tile `tile` into this wave's slots; lanes outside A load
@@ -243,7 +246,8 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
     if constexpr (STAGE) issue(tile0);
     double upre = 0.0;  // STAGE: this tile's u_raw row (threads < TR)
 
-    for (int64_t tile = tile0; tile < tile1; ++tile) {
+    int par = 0;
+    for (int64_t tile = tile0; tile < tile1; ++tile, par ^= 1) {
         const int64_t row0 = tile * TR;
         const int64_t my_row = row0 + g * RPL;
         const bool in = my_row < m;
@@ -303,16 +307,16 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         {
             const int k = (lane >> (6 - LOG_RPL)) & (RPL - 1);
             constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
-            if ((lane & kDupMask) == 0) red[wave][g * RPL + k] = acc[0];
+            if ((lane & kDupMask) == 0) red[par][wave][g * RPL + k] = acc[0];
         }
         tile_barrier<STAGE>();
         if (tid < TR) {
             const int64_t row = row0 + tid;
             float un = 0.0f;
             if (row < m) {
-                float s = red[0][tid];
+                float s = red[par][0][tid];
 #pragma unroll
-                for (int w = 1; w < NW; ++w) s += red[w][tid];
+                for (int w = 1; w < NW; ++w) s += red[par][w][tid];
                 double r = static_cast<double>(s);
                 if (cu != 0.0) {  // u is not read for a zero coefficient
                     if constexpr (STAGE) r += cu * upre;
@@ -329,13 +333,13 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
                     sq += static_cast<double>(un) * un;
                 }
             }
-            ush[tid] = un;
+            ush[par][tid] = un;
         }
         tile_barrier<STAGE>();
         // phase 2: column sums over this lane's rows with u_raw
         float ur[RPL];
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
+        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[par][g * RPL + k] : 0.0f;
         float cs[ROUNDS];
         if (tile_full) {
 #pragma unroll
@@ -362,7 +366,9 @@ __device__ __forceinline__ void fused_body(int64_t m, int64_t n, const void* __r
         LaneTRf<ROUNDS, G / 2, 1, true>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
-        tile_barrier<STAGE>();  // red/ush reuse in the next tile
+        // no barrier here: the next tile writes the other copy, and this
+        // copy is rewritten only after every wave has passed the next
+        // tile's first barrier, that is, finished reading it
     }
     // column ids of tacc: r = base + i, the halving stage on lane bit b kept
     // the upper half for lanes with bit b set
