@@ -667,10 +667,13 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
     float tacc[TOUT];
 #pragma unroll
     for (int i = 0; i < TOUT; ++i) tacc[i] = 0.0f;
-    __shared__ double red[NW][TR];
-    __shared__ float ush[TR];
+    // row partials and u rows in two copies by tile parity (the next
+    // tile writes the other copy)
+    __shared__ double red[2][NW][TR];
+    __shared__ float ush[2][TR];
     double sq = 0.0;
-    for (int64_t tile = tile0; tile < tile1; ++tile) {
+    int par = 0;
+    for (int64_t tile = tile0; tile < tile1; ++tile, par ^= 1) {
         const int64_t row0 = tile * TR;
         const int64_t my_row = row0 + g * RPL;
         const bool in = my_row < m;
@@ -704,16 +707,16 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
         {
             const int k = (lane >> (6 - LOG_RPL)) & (RPL - 1);
             constexpr int kDupMask = ((1 << (6 - LOG_RPL)) - 1) & ~(G - 1);
-            if ((lane & kDupMask) == 0) red[wave][g * RPL + k] = acc[0];
+            if ((lane & kDupMask) == 0) red[par][wave][g * RPL + k] = acc[0];
         }
         __syncthreads();
         if (tid < TR) {
             const int64_t row = row0 + tid;
             float un = 0.0f;
             if (row < m) {
-                double sx = red[0][tid];
+                double sx = red[par][0][tid];
 #pragma unroll
-                for (int w = 1; w < NW; ++w) sx += red[w][tid];
+                for (int w = 1; w < NW; ++w) sx += red[par][w][tid];
                 if (u64) {
                     const double r = static_cast<double*>(u)[row] - sx;
                     static_cast<double*>(u)[row] = r;
@@ -725,13 +728,13 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
                     sq += static_cast<double>(un) * un;
                 }
             }
-            ush[tid] = un;
+            ush[par][tid] = un;
         }
         __syncthreads();
         // A_hi^T u_raw over this lane's rows
         float ur[RPL];
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[g * RPL + k] : 0.0f;
+        for (int k = 0; k < RPL; ++k) ur[k] = k < nvalid ? ush[par][g * RPL + k] : 0.0f;
         float cs[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -745,7 +748,8 @@ __global__ __launch_bounds__(kRsNT) void lsqr_restart_kernel(
         LaneTRf<R, G / 2, 1, true>::run(cs, lane);
 #pragma unroll
         for (int i = 0; i < TOUT; ++i) tacc[i] += cs[i];
-        __syncthreads();
+        // no barrier here: this copy is rewritten only after every wave has
+        // passed the next tile's first barrier, that is, finished reading it
     }
     int base = 0;
 #pragma unroll
