@@ -277,7 +277,9 @@ def test_adjoint_grouped_restart_paths_and_new_matrices(capi, n2):
     (a wider problem in the batch, n2 = 1030).  Each is checked against the
     dense FP64 QR adjoint, on two calls with the same scratch and shapes but
     different matrices (the second call must not replay launches bound to
-    the first call's matrices)."""
+    the first call's matrices).  The gradients (one fused launch over the
+    same row items when n <= 512) are checked against tomos_lsq_grad with
+    the same w."""
     shapes = [(6000, 40), (5000, n2)]
     opt = C.LsqrOptions(C.FP32_BF16, C.TIER_DEFAULT, C.TIER_EXACT, C.F32, C.F64, C.F64, 1e-10, 1e-10, 12, 20, 4, 1)
     P = len(shapes)
@@ -289,7 +291,7 @@ def test_adjoint_grouped_restart_paths_and_new_matrices(capi, n2):
             lda = (m + 7) // 8 * 8
             wh, wr = H.pack(C.PACK_FP32_BF16, A, ld=lda)
             _, sk = H.sketch_rinv(C.FP32_BF16, C.TIER_EXACT, m, n, wh, wr, lda, 4 * n)
-            bd, xd = H.dev(b), H.dev(np.zeros(n))
+            bd, xd = H.dev(b), H.dev(np.random.default_rng(90 + k).standard_normal(n))
             gd, wd, gbd, gad = H.dev(g), DeviceBuffer(n * 8), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
             keep += [wh, wr, sk, bd, xd]
             probs.append(C.LsqProblem(m, n, lda, wh.ptr, wr.ptr, bd.ptr, xd.ptr, sk.ptr, 4 * n))
@@ -312,3 +314,15 @@ def test_adjoint_grouped_restart_paths_and_new_matrices(capi, n2):
             w0 = ws[k].to_numpy(np.float64, (n,))
             rel = np.linalg.norm(w0 - dense[k]) / np.linalg.norm(dense[k])
             assert rel < 1e-6, (call, k, rel)
+            # the batch's gradients (one fused launch for n <= 512) against
+            # the one-problem gradient kernel with the same w: the FP64 row
+            # sums differ only in order, grad_A by FP32 rounding of those
+            q = probs[k]
+            gsb = C.lib().tomos_lsq_grad_scratch_size(C.FP32_BF16, C.TIER_EXACT, m, n)
+            gscr, gb1, ga1 = DeviceBuffer(gsb), DeviceBuffer(m * 8), DeviceBuffer(m * n * 4)
+            C.call("tomos_lsq_grad", C.FP32_BF16, C.TIER_EXACT, m, n, q.a, q.wr, q.lda, q.b, q.x, ws[k].ptr,
+                   gb1.ptr, ga1.ptr, m, gscr.ptr, gsb, None)
+            gb0, gbr = gbs[k].to_numpy(np.float64, (m,)), gb1.to_numpy(np.float64, (m,))
+            ga0, gar = gas[k].to_numpy(np.float32, (m, n)), ga1.to_numpy(np.float32, (m, n))
+            assert np.linalg.norm(gb0 - gbr) <= 1e-12 * np.linalg.norm(gbr), (call, k)
+            assert np.max(np.abs(ga0.astype(np.float64) - gar)) <= 2.5e-7 * np.max(np.abs(gar)), (call, k)

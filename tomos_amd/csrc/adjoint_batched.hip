@@ -420,6 +420,166 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_rx2(const GProb* __restric
     if (j < n) p.rpart[static_cast<int64_t>(it.slot) * n + j] = tacc;
 }
 
+// The gradients of every problem in one launch over the restart's row
+// items (packed formats, n <= kRx2MaxN, lsq_grad's products in one pass
+// over A): per 64-row tile, the exact row sums A w and A x in FP64 (the
+// first phase of k_gadj_rx2 with two vectors), then grad_b = A w,
+// r = b - A x and the tile's rows of grad_A = r w^T - (A w) x^T (FP32,
+// column-major, ldg), so A is read once and grad_A written once.
+struct GGrad {
+    int64_t m, n, lda, ldg;
+    const void* a;
+    const uint16_t* wr;
+    const double *b, *x, *w;
+    double* grad_b;
+    float* grad_a;  // null: grad_b only
+};
+
+template <int FMT>  // 1: FP32_BF16, 2: FP32_16
+__global__ __launch_bounds__(kRx2Threads) void k_gadj_grad(const GGrad* __restrict__ T,
+                                                           const RxItem* __restrict__ items) {
+    const RxItem it = items[blockIdx.x];
+    const GGrad& p = T[it.prob];
+    // row sums and (A w, r) rows in two copies by tile parity, as in k_gadj_rx2
+    __shared__ double redw[2][kRx2Threads / 64][kRx2Rows];
+    __shared__ double redx[2][kRx2Threads / 64][kRx2Rows];
+    __shared__ double gsh[2][kRx2Rows];
+    __shared__ double rsh[2][kRx2Rows];
+    __shared__ uint4 stage[kRx2Threads / 64][16][64];  // a wave's tile: 8 hi, 8 lo vectors a lane
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = tid & 7, c = tid >> 3;  // row group, column slot
+    const int64_t n = p.n, m = p.m;
+    const char* hb = static_cast<const char*>(p.a);
+    double wv[8], xv[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t j = static_cast<int64_t>(r) * 64 + c;
+        wv[r] = j < n ? p.w[j] : 0.0;
+        xv[r] = j < n ? p.x[j] : 0.0;
+    }
+    auto run = [&](auto rounds) {
+        constexpr int R = decltype(rounds)::value;
+        auto issue = [&](int64_t t0) {
+            const int64_t row = t0 + g * 8;
+            const bool ok = row < it.row1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * 64 + c;
+                const int64_t e = (ok && j < n) ? row + j * p.lda : 0;
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(hb + e * 2), &stage[wave][r][0], 16,
+                                                 0, 0);
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4*>(p.wr + e), &stage[wave][8 + r][0],
+                                                 16, 0, 0);
+            }
+        };
+        issue(it.row0);
+        int par = 0;
+        for (int64_t t0 = it.row0; t0 < it.row1; t0 += kRx2Rows, par ^= 1) {
+            uint4 h[R], l[R];
+            const int64_t row = t0 + g * 8;
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's DMA has landed
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                h[r] = stage[wave][r][lane];
+                l[r] = stage[wave][8 + r][lane];
+            }
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): slots read before they are refilled
+            if (t0 + kRx2Rows < it.row1) issue(t0 + kRx2Rows);
+            double aw[8], ax[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) aw[k] = ax[k] = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t j = static_cast<int64_t>(r) * 64 + c;
+                if (!(row < it.row1 && j < n)) {
+                    h[r] = make_uint4(0u, 0u, 0u, 0u);
+                    l[r] = make_uint4(0u, 0u, 0u, 0u);
+                }
+                float v[8];
+                rx_decode<FMT>(h[r], l[r], v);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    aw[k] = fma(static_cast<double>(v[k]), wv[r], aw[k]);
+                    ax[k] = fma(static_cast<double>(v[k]), xv[r], ax[k]);
+                }
+            }
+#pragma unroll
+            for (int st = 0; st < 3; ++st) {
+                const int off = 32 >> st, half = 4 >> st;
+                const bool upper = (lane & off) != 0;
+#pragma unroll
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    if (t2 < half) {
+                        const double sw = upper ? aw[t2] : aw[t2 + half];
+                        const double kw = upper ? aw[t2 + half] : aw[t2];
+                        aw[t2] = kw + __shfl_xor(sw, off);
+                        const double sx = upper ? ax[t2] : ax[t2 + half];
+                        const double kx = upper ? ax[t2 + half] : ax[t2];
+                        ax[t2] = kx + __shfl_xor(sx, off);
+                    }
+                }
+            }
+            const int kk = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+            redw[par][wave][g * 8 + kk] = aw[0];
+            redx[par][wave][g * 8 + kk] = ax[0];
+            lds_exchange_barrier();
+            if (tid < kRx2Rows) {
+                double sw = 0.0, sx = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < kRx2Threads / 64; ++w2) {
+                    sw += redw[par][w2][tid];
+                    sx += redx[par][w2][tid];
+                }
+                const int64_t i = t0 + tid;
+                double ri = 0.0;
+                if (i < it.row1) {
+                    ri = p.b[i] - sx;
+                    p.grad_b[i] = sw;
+                }
+                gsh[par][tid] = sw;
+                rsh[par][tid] = ri;
+            }
+            lds_exchange_barrier();
+            // grad_A rows of this tile: lane (g, c) writes its 8 rows of
+            // columns r * 64 + c (32 contiguous bytes of each column)
+            if (p.grad_a) {
+                const int nv = row < it.row1 ? static_cast<int>(min<int64_t>(8, it.row1 - row)) : 0;
+                // 16-byte stores when grad_A's columns keep 16-byte alignment
+                // (row is a multiple of 8)
+                const bool vec = nv == 8 && p.ldg % 4 == 0 && reinterpret_cast<uintptr_t>(p.grad_a) % 16 == 0;
+                double rk[8], gk[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    rk[k] = rsh[par][g * 8 + k];
+                    gk[k] = gsh[par][g * 8 + k];
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int64_t j = static_cast<int64_t>(r) * 64 + c;
+                    if (j >= n || nv == 0) continue;
+                    float* out = p.grad_a + row + j * p.ldg;
+                    float o[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) o[k] = static_cast<float>(rk[k] * wv[r] - gk[k] * xv[r]);
+                    if (vec) {
+                        reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
+                        reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if (k < nv) out[k] = o[k];
+                    }
+                }
+            }
+            // no third barrier: the next tile writes the other copies
+        }
+    };
+    if (n <= 64) run(std::integral_constant<int, 1>{});
+    else if (n <= 128) run(std::integral_constant<int, 2>{});
+    else if (n <= 256) run(std::integral_constant<int, 4>{});
+    else run(std::integral_constant<int, 8>{});
+}
+
 // Restart of every problem whose phase is kPhaseRestart: res = g - rfull
 // (rfull = A^T A w, or nothing on the first restart), |g| on the first, the
 // exact-residual stop test (the restart after `max_restarts` earlier ones is
@@ -603,6 +763,7 @@ struct GPlan {
     std::vector<int> prob_rx;
     std::vector<size_t> rpart_at;
     size_t rx_off = 0;
+    size_t grad_tab = 0;  // rx2: the gradients' table (GGrad per problem)
 };
 
 void grouped_validate(const tomos_lsqr_options& o, int nprob, const tomos_lsq_problem* pr) {
@@ -685,6 +846,7 @@ GPlan grouped_plan(const tomos_lsqr_options& o, int nprob, const tomos_lsq_probl
     L.items_off = take(sizeof(GroupItem) * static_cast<size_t>(std::max(L.nitems, 1)));
     L.states = take(sizeof(GState) * static_cast<size_t>(std::max(nprob, 1)));
     L.rx_off = take(sizeof(RxItem) * static_cast<size_t>(std::max(L.nrx, 1)));
+    L.grad_tab = take(sizeof(GGrad) * static_cast<size_t>(std::max(nprob, 1)));
     for (int i = 0; i < nprob; ++i) {
         L.rpart_at.push_back(take(static_cast<size_t>(L.prob_rx[static_cast<size_t>(i)]) *
                                   static_cast<size_t>(std::max<int64_t>(pr[i].n, 1)) * 8 + 8));
@@ -1012,10 +1174,39 @@ void adjoint_grouped(const tomos_lsqr_options& o, int nprob, const tomos_lsq_pro
     G.s = own ? own : stream;
     try {
         G.run(g, w, iterations);
-        for (int i = 0; i < nprob; ++i) {
-            const tomos_lsq_problem& q = probs[i];
-            lsq_grad(o.a_type, o.restart_tier, q.m, q.n, q.a, q.wr, q.lda, q.b, q.x, w[i], grad_b[i],
-                     grad_a ? grad_a[i] : nullptr, ldg ? ldg[i] : q.m, G.sc + G.L.grad_sc, G.L.grad_bytes, G.s);
+        std::vector<GGrad> tab;  // outlives its copy: the stream is synchronized below
+        if (G.L.rx2) {
+            // one launch for every problem with columns (rows of n = 0
+            // problems: lsq_grad's zero fill)
+            tab.resize(static_cast<size_t>(nprob));
+            for (int i = 0; i < nprob; ++i) {
+                const tomos_lsq_problem& q = probs[i];
+                const int64_t lg = ldg ? ldg[i] : q.m;
+                float* ga = grad_a ? grad_a[i] : nullptr;
+                if (q.m > 0 && q.n > 0) {
+                    require(q.b && q.x && w[i] && grad_b[i], "tomos_lsq_grad: null pointer");
+                    require(!ga || lg >= q.m, "tomos_lsq_grad: ldg < m");
+                }
+                tab[static_cast<size_t>(i)] = GGrad{q.m, q.n, q.lda, lg, q.a, q.wr, q.b, q.x, w[i], grad_b[i], ga};
+                if (q.n == 0)
+                    lsq_grad(o.a_type, o.restart_tier, q.m, q.n, q.a, q.wr, q.lda, q.b, q.x, w[i], grad_b[i], ga, lg,
+                             G.sc + G.L.grad_sc, G.L.grad_bytes, G.s);
+            }
+            if (G.L.nrx > 0) {
+                GGrad* dt = reinterpret_cast<GGrad*>(G.sc + G.L.grad_tab);
+                TOMOS_HIP_CHECK(hipMemcpyAsync(dt, tab.data(), sizeof(GGrad) * tab.size(), hipMemcpyHostToDevice, G.s));
+                const RxItem* d = reinterpret_cast<const RxItem*>(G.sc + G.L.rx_off);
+                const unsigned nb = static_cast<unsigned>(G.L.nrx);
+                if (o.a_type == TOMOS_FP32_BF16) k_gadj_grad<1><<<nb, kRx2Threads, 0, G.s>>>(dt, d);
+                else k_gadj_grad<2><<<nb, kRx2Threads, 0, G.s>>>(dt, d);
+                TOMOS_LAUNCH_CHECK("k_gadj_grad");
+            }
+        } else {
+            for (int i = 0; i < nprob; ++i) {
+                const tomos_lsq_problem& q = probs[i];
+                lsq_grad(o.a_type, o.restart_tier, q.m, q.n, q.a, q.wr, q.lda, q.b, q.x, w[i], grad_b[i],
+                         grad_a ? grad_a[i] : nullptr, ldg ? ldg[i] : q.m, G.sc + G.L.grad_sc, G.L.grad_bytes, G.s);
+            }
         }
         TOMOS_HIP_CHECK(hipStreamSynchronize(G.s));
     } catch (...) {
