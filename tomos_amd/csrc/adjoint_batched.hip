@@ -448,7 +448,7 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_grad(const GGrad* __restri
     __shared__ uint4 stage[kRx2Threads / 64][16][64];  // a wave's tile: 8 hi, 8 lo vectors a lane
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = tid & 7, c = tid >> 3;  // row group, column slot
-    const int64_t n = p.n, m = p.m;
+    const int64_t n = p.n;
     const char* hb = static_cast<const char*>(p.a);
     double wv[8], xv[8];
 #pragma unroll
