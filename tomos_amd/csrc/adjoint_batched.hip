@@ -540,34 +540,39 @@ __global__ __launch_bounds__(kRx2Threads) void k_gadj_grad(const GGrad* __restri
                 rsh[par][tid] = ri;
             }
             lds_exchange_barrier();
-            // grad_A rows of this tile: lane (g, c) writes its 8 rows of
-            // columns r * 64 + c (32 contiguous bytes of each column)
+            // grad_A rows of this tile from the (A w, r) rows in LDS: the 8
+            // lanes of column slot c write 128 contiguous bytes of column
+            // r * 64 + c per store (lane g: rows 32 h + 4 g .. + 3, h = 0, 1)
             if (p.grad_a) {
-                const int nv = row < it.row1 ? static_cast<int>(min<int64_t>(8, it.row1 - row)) : 0;
-                // 16-byte stores when grad_A's columns keep 16-byte alignment
-                // (row is a multiple of 8)
-                const bool vec = nv == 8 && p.ldg % 4 == 0 && reinterpret_cast<uintptr_t>(p.grad_a) % 16 == 0;
-                double rk[8], gk[8];
+                const bool al = p.ldg % 4 == 0 && reinterpret_cast<uintptr_t>(p.grad_a) % 16 == 0;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    rk[k] = rsh[par][g * 8 + k];
-                    gk[k] = gsh[par][g * 8 + k];
-                }
+                for (int hh = 0; hh < 2; ++hh) {
+                    const int lr = hh * 32 + g * 4;  // first of this lane's 4 rows in the tile
+                    const int64_t r0 = t0 + lr;
+                    const int nv = r0 < it.row1 ? static_cast<int>(min<int64_t>(4, it.row1 - r0)) : 0;
+                    if (nv == 0) continue;
+                    double rk[4], gk[4];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int64_t j = static_cast<int64_t>(r) * 64 + c;
-                    if (j >= n || nv == 0) continue;
-                    float* out = p.grad_a + row + j * p.ldg;
-                    float o[8];
+                    for (int k = 0; k < 4; ++k) {
+                        rk[k] = rsh[par][lr + k];
+                        gk[k] = gsh[par][lr + k];
+                    }
+                    const bool vec = al && nv == 4;  // r0 is a multiple of 4: 16-byte aligned
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) o[k] = static_cast<float>(rk[k] * wv[r] - gk[k] * xv[r]);
-                    if (vec) {
-                        reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
-                        reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
-                    } else {
+                    for (int r = 0; r < R; ++r) {
+                        const int64_t j = static_cast<int64_t>(r) * 64 + c;
+                        if (j >= n) continue;
+                        float* out = p.grad_a + r0 + j * p.ldg;
+                        float o[4];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k)
-                            if (k < nv) out[k] = o[k];
+                        for (int k = 0; k < 4; ++k) o[k] = static_cast<float>(rk[k] * wv[r] - gk[k] * xv[r]);
+                        if (vec) {
+                            *reinterpret_cast<float4*>(out) = make_float4(o[0], o[1], o[2], o[3]);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                if (k < nv) out[k] = o[k];
+                        }
                     }
                 }
             }
