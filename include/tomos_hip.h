@@ -614,9 +614,14 @@ int tomos_lsq_grad(int a_type, int tier, int64_t m, int64_t n, const void* a, co
  *     captured hipGraph of 16 CG iterations of every problem (the batched
  *     LSQR solver's grouped single-pass kernel for A d and A^T A d, one
  *     block per problem for the n-sized work) replayed until every problem
- *     has stopped, with a captured restart sequence (per problem, the
- *     restart-tier GEMV pair) between replays when some problem restarts.
- *     Same control flow and stop tests per problem as
+ *     has stopped, with a captured restart sequence between replays when
+ *     some problem restarts (the exact A^T A w of every restarting problem
+ *     in one grouped launch over row items when the restart tier decodes
+ *     exactly and every n <= 1024, otherwise per problem the restart-tier
+ *     GEMV pair).  The gradients follow: for packed problems with every
+ *     n <= 512, one launch over the same row items (A w and A x from one
+ *     read of A, then grad_b and grad_A); otherwise tomos_lsq_grad per
+ *     problem.  Same control flow and stop tests per problem as
  *     tomos_lsq_adjoint_solve; results agree with it to rounding.  Needs
  *     a_type F32/FP32_16/FP32_BF16, mv_tier DEFAULT, precond_type F64 (R^-1),
  *     n <= 2048, 16-byte aligned A with lda a multiple of 4 (F32) / 8.
